@@ -1,0 +1,203 @@
+"""Benchmark: Landau implicit steps/s at N = 65536 on one MI355X.
+
+Workload (bimodal_n65536): N = 65536 markers from two equal-weight Maxwellians
+at (+-2, 0) with covariance 0.5 I, weights 1/N, P_DEG = 2 B-splines on 32 x 32
+cells over [-6, 6]^2, dt = 0.02, Anderson m = 5, tol 1e-12 (solver knobs of the
+reference's Landau presets).  A "step" is one implicit time step: Euler
+predictor, Anderson solve of the Gonzalez discrete-gradient Picard map to the
+residual tolerance, then the projection / entropy / energy / momentum of the
+new state.  The markers are resident in HBM before the timed region starts.
+
+Two paths run on identical markers:
+  * plain FP64: every Picard map in FP64 (the reference's GPU path, restated),
+  * chosen:     FP32-compensated pair sum for the early maps, FP64 maps to the
+                same tolerance (the exit map is always FP64).
+`value` is the chosen path's steps/s; the plain FP64 figure and the ratio are
+reported beside it.  Per step, |E_{n+1} - E_n| / E_0 and |P_{n+1} - P_n| are
+checked against 1e-10 and the entropy against non-decrease.
+
+The roofline object covers the dominant kernel of the timed region (the pair
+sum): algorithmic work 23 flops per ordered in-domain pair, i.e. 23 N_in
+(N_in - 1) per launch, over the mean launch duration measured with HIP events
+on the library's stream.  The CPU baseline is the oracle (a C++ FP64
+restatement of the reference CPU path, OpenMP over targets like
+functions.jl:210) timing whole Picard maps at the same N on this host.
+
+Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu]
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, os.path.join(ROOT, "collisionoperators.jl_amd"))
+
+FP64_VECTOR_PEAK_TFLOPS = 78.6     # MI355X published FP64 vector peak
+FP32_VECTOR_PEAK_TFLOPS = 157.3    # MI355X published FP32 vector peak (packed)
+FLOPS_PER_PAIR = 23
+
+
+def log(msg: str):
+    print(msg, file=sys.stderr, flush=True)
+
+
+def run_path(ws, p, v, w, precision, warmup, steps):
+    import collop
+    sim = collop.Simulation(ws, v, w)
+    prm = p.step_params(precision=precision)
+    E0 = 0.5 * float(np.sum(w * (v[:, 0] ** 2 + v[:, 1] ** 2)))
+    for i in range(warmup):
+        st = sim.run(prm, 1)[0]
+        log(f"  warmup step {i}: iters {st['iters']} (f32 {st['iters_f32']}) {st['seconds'] * 1e3:.1f} ms")
+    # state before the timed region: energy / momentum / entropy of the current markers
+    vc = sim.markers()
+    E_prev = 0.5 * float(np.sum(w * (vc[:, 0] ** 2 + vc[:, 1] ** 2)))
+    P_prev = (w[:, None] * vc).sum(axis=0)
+    S_prev = collop.compute_r_and_entropy_gpu(ws, sim.coeffs())[1]
+    collop.profile(ws, True)
+    t0 = time.perf_counter()
+    stats = []
+    for i in range(steps):
+        st = sim.run(prm, 1)[0]
+        stats.append(st)
+        log(f"  step {i}: iters {st['iters']} (f32 {st['iters_f32']}) exit {st['exit_reason']} "
+            f"res {st['residual']:.2e} {st['seconds'] * 1e3:.1f} ms")
+    elapsed = time.perf_counter() - t0
+    ms, launches = collop.profile(ws, False)
+    dE = dP = 0.0
+    dS_min = np.inf
+    for st in stats:
+        dE = max(dE, abs(st["energy"] - E_prev) / E0)
+        dP = max(dP, float(np.max(np.abs(np.array(st["momentum"]) - P_prev))))
+        dS_min = min(dS_min, st["entropy"] - S_prev)
+        E_prev, P_prev, S_prev = st["energy"], np.array(st["momentum"]), st["entropy"]
+    return dict(steps_per_s=steps / elapsed, elapsed=elapsed, stats=stats, pair_ms=ms.tolist(),
+                pair_launches=launches, max_dE_rel=dE, max_dP=dP, min_dS=float(dS_min),
+                mean_iters=float(np.mean([s["iters"] for s in stats])),
+                mean_iters_f32=float(np.mean([s["iters_f32"] for s in stats])),
+                all_converged=all(s["exit_reason"] == 0 for s in stats), markers=sim.markers())
+
+
+def cpu_baseline(p, v, w, mean_maps_per_step, budget_s=20.0):
+    """Oracle Picard maps (FP64, OpenMP over targets) on this host, extrapolated to steps/s."""
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import oracle as O
+    threads = int(os.environ.get("OMP_NUM_THREADS", "0")) or min(16, os.cpu_count() or 1)
+    m = O.Mesh(p.bp1, p.bp2)
+    fc = m.l2_project(v, w)
+    S0 = m.entropy(fc)
+    P = O.default_params(p.DT, m=p.m_anderson, nthreads=threads)
+    v1, _ = m.predictor(P, v, w, fc)
+    times = []
+    t_all = time.perf_counter()
+    while True:
+        t0 = time.perf_counter()
+        v1 = m.picard_map(P, v1, v, w, S0)
+        times.append(time.perf_counter() - t0)
+        log(f"  cpu map {len(times)}: {times[-1]:.2f} s")
+        if time.perf_counter() - t_all + times[-1] > budget_s or len(times) >= 5:
+            break
+    t_map = float(np.mean(times))
+    N = v.shape[0]
+    return dict(value=1.0 / (t_map * mean_maps_per_step), unit="steps/s", cores=threads, kind="port",
+                sample=f"{len(times)} full FP64 Picard maps of the oracle at N={N} (pair sum + projection chain), "
+                       f"{t_map:.2f} s each; steps/s = 1 / (map time x {mean_maps_per_step:.1f} maps per step, the "
+                       f"FP64 GPU run's mean incl. the predictor)",
+                pairs_per_s=N * (N - 1) / t_map)
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=10)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--config", default="bimodal_n65536")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py measures one GPU: this build has no multi-GPU path (asked for --gpus {args.gpus}).",
+              file=sys.stderr)
+        sys.exit(2)
+
+    import collop
+    p = collop.config(args.config)
+    v, w = collop.initial_markers(p)
+    ws = collop.build_workspace(p, device=0)
+    N = v.shape[0]
+    box = ws.box
+    inside = (v[:, 0] > box[0]) & (v[:, 0] < box[1]) & (v[:, 1] > box[2]) & (v[:, 1] < box[3])
+    n_in = int(inside.sum())
+
+    log(f"[bench] {args.config}: N={N} (in domain {n_in}), mesh {len(p.bp1) - 1}x{len(p.bp2) - 1}, dt={p.DT}")
+    log("[bench] plain FP64 path")
+    r64 = run_path(ws, p, v, w, collop.PREC_F64, args.warmup, args.steps)
+    log(f"[bench] plain FP64: {r64['steps_per_s']:.3f} steps/s, mean iters {r64['mean_iters']:.1f}")
+    log("[bench] chosen path (FP32-compensated maps + FP64 finish)")
+    rch = run_path(ws, p, v, w, collop.PREC_F32C, args.warmup, args.steps)
+    log(f"[bench] chosen: {rch['steps_per_s']:.3f} steps/s, mean iters {rch['mean_iters']:.1f} "
+        f"(f32 {rch['mean_iters_f32']:.1f})")
+
+    # roofline of the dominant kernel in the chosen path's timed region
+    ms64, ms32 = rch["pair_ms"]
+    n64, n32 = rch["pair_launches"]
+    flops = FLOPS_PER_PAIR * n_in * (n_in - 1)
+    if ms64 >= ms32:
+        avg_ms = ms64 / max(n64, 1)
+        peak, kname = FP64_VECTOR_PEAK_TFLOPS, "k_pair_f64"
+    else:
+        avg_ms = ms32 / max(n32, 1)
+        peak, kname = FP32_VECTOR_PEAK_TFLOPS, "k_pair_f32c"
+    achieved = flops / (avg_ms * 1e-3) / 1e12
+    avg64 = r64["pair_ms"][0] / max(r64["pair_launches"][0], 1)
+    roofline = {"bound": "valu", "kernel": kname, "achieved": round(achieved, 3), "peak": peak, "unit": "TFLOP/s",
+                "frac": round(achieved / peak, 4), "traffic": None, "avg_launch_ms": round(avg_ms, 4),
+                "flops_per_launch": flops,
+                "fp64_pair_avg_ms": round(avg64, 4),
+                "fp64_pair_tflops": round(flops / (avg64 * 1e-3) / 1e12, 3),
+                "fp32c_pair_avg_ms": round(ms32 / max(n32, 1), 4) if n32 else None}
+
+    cpu = None
+    if not args.no_cpu:
+        log("[bench] CPU baseline (oracle)")
+        cpu = cpu_baseline(p, v, w, r64["mean_iters"] + 1.0)
+
+    ratio = rch["steps_per_s"] / r64["steps_per_s"]
+    dv = float(np.max(np.abs(rch["markers"] - r64["markers"])))
+    out = {
+        "metric": "Landau implicit steps/s at N=65536, chosen vs plain FP64 GPU, |ΔE|/E≤1e-10",
+        "value": round(rch["steps_per_s"], 4),
+        "unit": "steps/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(1e3 / rch["steps_per_s"], 3),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32c+f64",
+        "data": "synthetic (seeded bimodal Maxwellian markers, PCG64 seed 42)",
+        "config": {"workload": args.config, "N": N, "mesh": f"{len(p.bp1) - 1}x{len(p.bp2) - 1}", "dt": p.DT,
+                   "m_anderson": p.m_anderson, "tol": p.tol, "abs_floor": p.abs_floor},
+        "fp64_steps_per_s": round(r64["steps_per_s"], 4),
+        "ratio_vs_fp64": round(ratio, 4),
+        "mean_iters": {"fp64": r64["mean_iters"], "chosen": rch["mean_iters"], "chosen_f32": rch["mean_iters_f32"]},
+        "conservation": {"fp64": {"max_dE_rel": r64["max_dE_rel"], "max_dP": r64["max_dP"], "min_dS": r64["min_dS"]},
+                         "chosen": {"max_dE_rel": rch["max_dE_rel"], "max_dP": rch["max_dP"],
+                                    "min_dS": rch["min_dS"]}},
+        "all_converged": r64["all_converged"] and rch["all_converged"],
+        "chosen_vs_fp64_max_abs_dv": dv,
+        "pair_interactions_per_s": round(n_in * (n_in - 1) / (avg64 * 1e-3), 1),
+        "roofline": roofline,
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

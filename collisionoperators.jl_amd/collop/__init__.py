@@ -1,0 +1,17 @@
+"""collop -- MI355X (gfx950) Landau collision hot path behind the reference's hooks.
+
+The compute lives in ``libcollop.so`` (hand-written HIP for gfx950, C-ABI in
+include/collop.h).  This package is its host-side mirror of the reference
+interface: the Workspace tables (MantisWrappers.jl), the GPU hook functions
+(collision_gpu.jl, projection_gpu.jl) and the device-resident implicit step
+(solver.jl, main.jl).
+"""
+from ._lib import (CollopError, Context, StepParams, StepStats, load, PREC_F64, PREC_F32C, WARM_EULER,  # noqa: F401
+                   WARM_MLP)
+from .hooks import (compute_collision_gpu, compute_collision_gpu32, compute_collision_metric_gpu,  # noqa: F401
+                    compute_G_gpu, compute_r_and_entropy_gpu, enable_gpu, eval_loggrad_gpu, l2_project_gpu,
+                    l2_rhs_gpu, mass_solve_gpu)
+from .ic import initial_markers, markers  # noqa: F401
+from .mesh import Workspace, build_workspace, extraction_p2c1  # noqa: F401
+from .params import CONFIGS, SimParameters, config, parse_overrides  # noqa: F401
+from .sim import Simulation, profile, set_pair_tile  # noqa: F401

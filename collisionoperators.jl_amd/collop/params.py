@@ -1,0 +1,120 @@
+"""SimParameters: the fields of the reference's configuration struct
+(Parameters.jl:18-140) that the Landau hot path reads, plus the named
+workloads this library is measured on.
+"""
+from __future__ import annotations
+
+import dataclasses
+import math
+from dataclasses import dataclass, field
+
+import numpy as np
+
+from . import _lib
+
+
+def _uniform(lo: float, hi: float, n: int) -> np.ndarray:
+    # lo + (hi - lo) i / n: exact for the power-of-two cell counts used here
+    return np.array([lo + (hi - lo) * i / n for i in range(n + 1)], dtype=np.float64)
+
+
+@dataclass
+class SimParameters:
+    """Subset of Parameters.jl:18-140 (same names and defaults unless noted)."""
+
+    bp1: np.ndarray = field(default_factory=lambda: _uniform(-6.0, 6.0, 16))
+    bp2: np.ndarray = field(default_factory=lambda: _uniform(-6.0, 6.0, 16))
+    P_DEG: int = 2
+    K_REG: int = 1
+    N_QUAD: int = 6
+    N_PARTICLES: int = 10_000
+    σ1: float = 4 / 3
+    σ2: float = 0.5
+    v1_peak: float = 0.0
+    DT: float = 0.001
+    N_STEPS: int = 400
+    use_gonzalez: bool = True
+    use_logsq: bool = False
+    use_anderson: bool = True
+    damping: float = 0.7
+    m_anderson: int = 8
+    tol: float = 1e-12
+    max_iter: int = 2000
+    abs_floor: float = 1e-7
+    stag_window: int = 50
+    stag_rel_tol: float = 0.01
+    damp_decay_start: int = 200
+    damp_decay_factor: float = 0.5
+    exit_picard_step: bool = True
+    warmstart: str = "euler"        # :euler | :nn  (Parameters.jl:111)
+    use_gpu: bool = True
+    gpu_fp32: bool = False           # FP32-compensated pair sum for the early maps
+    seed: int = 42
+    ic: str = "gaussian"             # gaussian | bimodal | bimaxwellian
+    weight_jitter: float = 0.0       # w ∝ 1 + jitter u, u ~ U(-1, 1)
+
+    def step_params(self, precision: int | None = None, warmstart: int | None = None) -> _lib.StepParams:
+        prec = precision if precision is not None else (_lib.PREC_F32C if self.gpu_fp32 else _lib.PREC_F64)
+        warm = warmstart if warmstart is not None else (_lib.WARM_MLP if self.warmstart == "nn" else _lib.WARM_EULER)
+        return _lib.StepParams(
+            dt=self.DT, m_anderson=self.m_anderson, max_iter=self.max_iter, tol=self.tol,
+            abs_floor=self.abs_floor, stag_window=self.stag_window, stag_rel_tol=self.stag_rel_tol,
+            damp_decay_start=self.damp_decay_start, damp_decay_factor=self.damp_decay_factor,
+            restart_factor=math.inf, damping=self.damping, reg_factor=1e-10,
+            use_anderson=int(self.use_anderson), use_gonzalez=int(self.use_gonzalez),
+            exit_picard_step=int(self.exit_picard_step), use_logsq=int(self.use_logsq),
+            precision=prec, warmstart=warm, f32_floor_factor=16.0)
+
+
+def parse_overrides(p: SimParameters, args: list[str]) -> SimParameters:
+    """``--key=value`` scalar overrides (Parameters.jl:157-192): unknown keys and
+    vector fields raise, untouched fields survive."""
+    kw = {}
+    fields = {f.name: f for f in dataclasses.fields(p)}
+    for tok in args:
+        if not tok.startswith("--") or "=" not in tok:
+            raise ValueError(f"override must look like --key=value: {tok!r}")
+        k, v = tok[2:].split("=", 1)
+        if k not in fields:
+            raise ValueError(f"no such field: {k}")
+        cur = getattr(p, k)
+        if isinstance(cur, np.ndarray):
+            raise ValueError(f"vector field {k} cannot be overridden on the command line")
+        if isinstance(cur, bool):
+            if v.lower() not in ("true", "false"):
+                raise ValueError(f"{k} expects true/false")
+            kw[k] = v.lower() == "true"
+        elif isinstance(cur, int):
+            kw[k] = int(v)
+        elif isinstance(cur, float):
+            kw[k] = float(v)
+        else:
+            kw[k] = v.lstrip(":")
+    return dataclasses.replace(p, **kw)
+
+
+# Solver knobs pinned for every path (the Landau presets, parameters_sq_d04.jl:16-20):
+_KNOBS = dict(damping=0.7, abs_floor=1e-10, stag_window=30, stag_rel_tol=0.1, damp_decay_start=200,
+              damp_decay_factor=0.5, max_iter=2000, N_QUAD=6, K_REG=1, use_logsq=False, use_gonzalez=True,
+              exit_picard_step=True, m_anderson=5, tol=1e-12)
+
+# The named workloads (N markers, IC, mesh, dt, steps).
+CONFIGS = {
+    # isotropic N(0, I), 16x16 cells on [-6,6]^2, dt 0.05, 10 steps
+    "gaussian_n4096": dict(N_PARTICLES=4096, ic="gaussian", σ1=1.0, σ2=1.0, bp1=_uniform(-6, 6, 16),
+                           bp2=_uniform(-6, 6, 16), DT=0.05, N_STEPS=10, seed=42),
+    # two Maxwellians at (+-2, 0) with covariance 0.5 I, 32x32 cells on [-6,6]^2, dt 0.02, 50 steps
+    "bimodal_n65536": dict(N_PARTICLES=65536, ic="bimodal", v1_peak=2.0, σ1=math.sqrt(0.5), σ2=math.sqrt(0.5),
+                           bp1=_uniform(-6, 6, 32), bp2=_uniform(-6, 6, 32), DT=0.02, N_STEPS=50, seed=42),
+    # bi-Maxwellian T_x = 2, T_y = 0.5, w ∝ 1 + 0.2 u; 64x64 cells on [-8,8]^2, dt 0.01, 20 steps
+    "bimaxwellian_n262144": dict(N_PARTICLES=262144, ic="bimaxwellian", σ1=math.sqrt(2.0), σ2=math.sqrt(0.5),
+                                 weight_jitter=0.2, bp1=_uniform(-8, 8, 64), bp2=_uniform(-8, 8, 64), DT=0.01,
+                                 N_STEPS=20, seed=42),
+}
+
+
+def config(name: str, **over) -> SimParameters:
+    kw = dict(_KNOBS)
+    kw.update(CONFIGS[name])
+    kw.update(over)
+    return SimParameters(**kw)

@@ -1,0 +1,1137 @@
+// capi.hip -- the C-ABI of include/collop.h: context, mesh set-up, hook-level
+// entry points and the device-resident implicit step.
+//
+// Host-side responsibilities only: argument checking, device buffers cached in
+// the context (grow-only), the mesh tables derived once per mesh (1D mass
+// matrices and their banded Cholesky factors, basis values at the Gauss points),
+// and the Anderson state machine of solver.jl:300-423, which reads a handful of
+// reduced scalars per iteration while every O(N) and O(N^2) pass runs on the GPU.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <limits>
+#include <string>
+#include <utility>
+#include <vector>
+
+#include "../../include/collop.h"
+#include "collop_launch.hpp"
+
+using namespace collop;
+
+namespace {
+
+struct DBuf {
+  void* p = nullptr;
+  size_t bytes = 0;
+};
+
+}  // namespace
+
+struct collop_ctx {
+  int device = 0;
+  hipStream_t stream = nullptr;
+  std::string err;
+  // ---- mesh
+  bool has_mesh = false;
+  MeshView mv{};
+  std::vector<double> h_bp1, h_bp2;
+  std::vector<DBuf*> mesh_bufs;
+  // ---- scratch buffers by name (grow-only)
+  DBuf b_v, b_w, b_G, b_out, b_vmid, b_dv, b_dS, b_s64, b_s32, b_part, b_spart_sc, b_mom, b_dpart, b_iota, b_spart_q,
+      b_vec, b_scal, b_statp, b_stat;
+  // ---- device-resident state
+  bool has_state = false;
+  int64_t N = 0;
+  double w_abs_sum = 0.0;  // sum |w| of the state (fixed-point scale of the scatter)
+  DBuf s_v, s_w, s_v1, s_Gv, s_r, s_rp, s_Gp, s_best, s_dF, s_dG, s_fc, s_dotv;
+  double* h_stat = nullptr;  // pinned, kStatWidth doubles
+  // ---- pair tile and profiling
+  PairTile tile64{1, 8, 2};
+  PairTile tile32{1, 8, 0};
+  bool tile_auto = true;
+  bool profile = false;
+  std::vector<std::pair<hipEvent_t, hipEvent_t>> ev_pool;
+  size_t ev_used = 0;
+  std::vector<int> ev_kind;
+  double prof_ms[2] = {0.0, 0.0};
+  long prof_n[2] = {0, 0};
+};
+
+namespace {
+
+int fail(collop_ctx* c, int code, const std::string& msg) {
+  if (c) c->err = msg;
+  return code;
+}
+
+#define HIPC(expr)                                                                                     \
+  do {                                                                                                 \
+    hipError_t _e = (expr);                                                                            \
+    if (_e != hipSuccess)                                                                              \
+      return fail(ctx, COLLOP_ERR_HIP, std::string(#expr) + ": " + hipGetErrorString(_e));             \
+  } while (0)
+
+#define CHECK_LAUNCH()                                                                                 \
+  do {                                                                                                 \
+    hipError_t _e = hipGetLastError();                                                                 \
+    if (_e != hipSuccess) return fail(ctx, COLLOP_ERR_HIP, std::string("kernel launch: ") + hipGetErrorString(_e)); \
+  } while (0)
+
+int ensure(collop_ctx* ctx, DBuf& b, size_t bytes) {
+  if (bytes == 0) bytes = 8;
+  if (b.bytes >= bytes) return COLLOP_OK;
+  if (b.p) {
+    hipError_t e = hipFree(b.p);
+    b.p = nullptr;
+    b.bytes = 0;
+    if (e != hipSuccess) return fail(ctx, COLLOP_ERR_HIP, std::string("hipFree: ") + hipGetErrorString(e));
+  }
+  hipError_t e = hipMalloc(&b.p, bytes);
+  if (e != hipSuccess) {
+    b.p = nullptr;
+    return fail(ctx, COLLOP_ERR_HIP, "hipMalloc(" + std::to_string(bytes) + "): " + hipGetErrorString(e));
+  }
+  b.bytes = bytes;
+  return COLLOP_OK;
+}
+
+void release(DBuf& b) {
+  if (b.p) (void)hipFree(b.p);
+  b.p = nullptr;
+  b.bytes = 0;
+}
+
+template <class T>
+T* P(DBuf& b) {
+  return static_cast<T*>(b.p);
+}
+
+#define ENSURE(buf, bytes)                             \
+  do {                                                 \
+    int _r = ensure(ctx, (buf), (bytes));              \
+    if (_r != COLLOP_OK) return _r;                    \
+  } while (0)
+
+// Gauss-Legendre rule on [0,1]: Newton's method on the three-term recurrence
+// of P_n (any textbook; e.g. Press et al., Numerical Recipes, 4.6).
+void gauss_legendre(int n, std::vector<double>& x, std::vector<double>& w) {
+  x.assign(n, 0.0);
+  w.assign(n, 0.0);
+  for (int i = 0; i < n; ++i) {
+    double t = std::cos(M_PI * (i + 0.75) / (n + 0.5)), dp = 1.0;
+    for (int it = 0; it < 100; ++it) {
+      double p0 = 1.0, p1 = t;
+      for (int k = 2; k <= n; ++k) {
+        const double p2 = ((2.0 * k - 1.0) * t * p1 - (k - 1.0) * p0) / k;
+        p0 = p1;
+        p1 = p2;
+      }
+      if (n == 1) p0 = 1.0;
+      dp = n * (t * p1 - p0) / (t * t - 1.0);
+      const double d = p1 / dp;
+      t -= d;
+      if (std::fabs(d) < 1e-16) break;
+    }
+    double p0 = 1.0, p1 = t;
+    for (int k = 2; k <= n; ++k) {
+      const double p2 = ((2.0 * k - 1.0) * t * p1 - (k - 1.0) * p0) / k;
+      p0 = p1;
+      p1 = p2;
+    }
+    if (n == 1) p0 = 1.0;
+    dp = n * (t * p1 - p0) / (t * t - 1.0);
+    x[n - 1 - i] = 0.5 * (1.0 + t);
+    w[n - 1 - i] = 1.0 / ((1.0 - t * t) * dp * dp);
+  }
+}
+
+// phi_j(xi) on one cell from the reference's extraction layout C[i + 3j + 9 cell]
+void cell_basis(const double* C, int cell, double xi, double phi[3]) {
+  const double om = 1.0 - xi;
+  const double B[3] = {om * om, 2.0 * xi * om, xi * xi};
+  for (int j = 0; j < 3; ++j)
+    phi[j] = C[9 * cell + 3 * j] * B[0] + C[9 * cell + 3 * j + 1] * B[1] + C[9 * cell + 3 * j + 2] * B[2];
+}
+
+// 1D mass matrix of one direction, exact with a 3-point Gauss rule per cell,
+// and its banded Cholesky factor stored as {1/L_ii, L_i,i-1, L_i,i-2}.
+bool mass_factor_1d(const std::vector<double>& bp, const double* C, const int* bs0, int nd, std::vector<double>& out) {
+  const int ne = (int)bp.size() - 1;
+  std::vector<double> gx, gw;
+  gauss_legendre(3, gx, gw);
+  std::vector<double> M((size_t)nd * 3, 0.0);  // M[i][k] = M(i, i-k), k = 0..2
+  for (int e = 0; e < ne; ++e) {
+    const double h = bp[e + 1] - bp[e];
+    for (int q = 0; q < 3; ++q) {
+      double phi[3];
+      cell_basis(C, e, gx[q], phi);
+      for (int a = 0; a < 3; ++a)
+        for (int b = 0; b <= a; ++b) {
+          const int i = bs0[e] + a, j = bs0[e] + b;  // i >= j
+          M[(size_t)i * 3 + (i - j)] += phi[a] * phi[b] * gw[q] * h;
+        }
+    }
+  }
+  out.assign((size_t)nd * 3, 0.0);
+  std::vector<double> L((size_t)nd * 3, 0.0);  // L[i][k] = L(i, i-k)
+  for (int j = 0; j < nd; ++j) {
+    double s = M[(size_t)j * 3];
+    for (int k = 1; k <= 2 && j - k >= 0; ++k) s -= L[(size_t)j * 3 + k] * L[(size_t)j * 3 + k];
+    if (!(s > 0.0)) return false;
+    const double ljj = std::sqrt(s);
+    L[(size_t)j * 3] = ljj;
+    for (int i = j + 1; i <= std::min(nd - 1, j + 2); ++i) {
+      double t = M[(size_t)i * 3 + (i - j)];
+      // sum_k L(i,k) L(j,k) over k < j within the band
+      for (int k = std::max(0, i - 2); k < j; ++k) t -= L[(size_t)i * 3 + (i - k)] * L[(size_t)j * 3 + (j - k)];
+      L[(size_t)i * 3 + (i - j)] = t / ljj;
+    }
+  }
+  for (int i = 0; i < nd; ++i) {
+    out[(size_t)i * 3] = 1.0 / L[(size_t)i * 3];
+    out[(size_t)i * 3 + 1] = i >= 1 ? L[(size_t)i * 3 + 1] : 0.0;
+    out[(size_t)i * 3 + 2] = i >= 2 ? L[(size_t)i * 3 + 2] : 0.0;
+  }
+  return true;
+}
+
+int upload(collop_ctx* ctx, const void* host, size_t bytes, const void** dev_out) {
+  auto* b = new DBuf();
+  ctx->mesh_bufs.push_back(b);
+  int r = ensure(ctx, *b, bytes);
+  if (r != COLLOP_OK) return r;
+  HIPC(hipMemcpy(b->p, host, bytes, hipMemcpyHostToDevice));
+  *dev_out = b->p;
+  return COLLOP_OK;
+}
+
+void free_mesh(collop_ctx* ctx) {
+  for (DBuf* b : ctx->mesh_bufs) {
+    release(*b);
+    delete b;
+  }
+  ctx->mesh_bufs.clear();
+  ctx->has_mesh = false;
+}
+
+int need_mesh(collop_ctx* ctx) {
+  if (!ctx) return COLLOP_ERR_INVALID;
+  if (!ctx->has_mesh) return fail(ctx, COLLOP_ERR_NO_MESH, "collop_set_mesh has not been called on this context");
+  return COLLOP_OK;
+}
+
+int check_N(collop_ctx* ctx, int64_t N) {
+  if (N < 0 || N > (int64_t)1 << 27) return fail(ctx, COLLOP_ERR_INVALID, "N must be in [0, 2^27]");
+  return COLLOP_OK;
+}
+
+void choose_tile(collop_ctx* ctx, int N) {
+  if (!ctx->tile_auto) return;
+  // Enough workgroups (>= 2048, i.e. ~8 waves per SIMD on 256 CUs) by cutting
+  // the source range into slices; multiples of 8 keep one slice set per XCD.
+  for (PairTile* t : {&ctx->tile64, &ctx->tile32}) {
+    const int per = 256 * t->gamma_per_lane;
+    const int gb = std::max(1, (N + per - 1) / per);
+    int s = (2048 + gb - 1) / gb;
+    if (s >= 8) s = (s + 7) / 8 * 8;
+    s = std::max(1, std::min(s, 64));
+    while (s > 1 && (N + s - 1) / s < 16) s /= 2;  // keep slices non-trivial
+    t->slices = s;
+  }
+}
+
+// ---- profiling of the pair kernel (HIP events on the launch stream) -------
+void prof_begin(collop_ctx* ctx, int kind) {
+  if (!ctx->profile) return;
+  if (ctx->ev_used == ctx->ev_pool.size()) {
+    hipEvent_t a, b;
+    (void)hipEventCreate(&a);
+    (void)hipEventCreate(&b);
+    ctx->ev_pool.push_back({a, b});
+    ctx->ev_kind.push_back(0);
+  }
+  ctx->ev_kind[ctx->ev_used] = kind;
+  (void)hipEventRecord(ctx->ev_pool[ctx->ev_used].first, ctx->stream);
+}
+
+void prof_end(collop_ctx* ctx) {
+  if (!ctx->profile) return;
+  (void)hipEventRecord(ctx->ev_pool[ctx->ev_used].second, ctx->stream);
+  ctx->ev_used++;
+}
+
+// Called after a stream synchronisation.
+void prof_collect(collop_ctx* ctx) {
+  if (!ctx->profile) return;
+  for (size_t i = 0; i < ctx->ev_used; ++i) {
+    float ms = 0.0f;
+    if (hipEventElapsedTime(&ms, ctx->ev_pool[i].first, ctx->ev_pool[i].second) == hipSuccess) {
+      ctx->prof_ms[ctx->ev_kind[i]] += ms;
+      ctx->prof_n[ctx->ev_kind[i]]++;
+    }
+  }
+  ctx->ev_used = 0;
+}
+
+// ---- building blocks on device buffers -------------------------------------
+// part <- pair sum of the packed sources (FP64: kind 0; FP32c: kind 1)
+int run_pair(collop_ctx* ctx, int kind, int N, double* part) {
+  if (kind == 0) {
+    prof_begin(ctx, 0);
+    launch_pair_f64(P<Src64>(ctx->b_s64), N, ctx->tile64, part, ctx->stream);
+  } else {
+    prof_begin(ctx, 1);
+    launch_pair_f32c(P<Src32>(ctx->b_s32), N, ctx->tile32, part, ctx->stream);
+  }
+  CHECK_LAUNCH();
+  prof_end(ctx);
+  return COLLOP_OK;
+}
+
+int ensure_scratch(collop_ctx* ctx, int64_t N) {
+  const size_t n2 = (size_t)2 * std::max<int64_t>(N, 1);
+  const MeshView& m = ctx->mv;
+  ENSURE(ctx->b_v, n2 * 8);
+  ENSURE(ctx->b_w, n2 / 2 * 8);
+  ENSURE(ctx->b_G, n2 * 8);
+  ENSURE(ctx->b_out, (n2 / 2) * 5 * 8);
+  ENSURE(ctx->b_vmid, n2 * 8);
+  ENSURE(ctx->b_dv, n2 * 8);
+  ENSURE(ctx->b_dS, n2 * 8);
+  ENSURE(ctx->b_s64, (n2 / 2) * sizeof(Src64));
+  ENSURE(ctx->b_s32, (n2 / 2) * sizeof(Src32));
+  const int Smax = 64;
+  ENSURE(ctx->b_part, (size_t)Smax * (n2 / 2) * 5 * 8);
+  const int nsb = scatter_blocks((int)N);
+  ENSURE(ctx->b_spart_sc, (size_t)2 * nsb * m.n_dofs * 8);
+  ENSURE(ctx->b_mom, (size_t)nsb * 3 * 8);
+  ENSURE(ctx->b_dpart, (size_t)gather_blocks((int)N) * 2 * 8);
+  ENSURE(ctx->b_iota, (size_t)m.ne1 * m.ne2 * m.nq * m.nq * 8);
+  ENSURE(ctx->b_spart_q, (size_t)2 * quad_blocks(m) * 8);
+  ENSURE(ctx->b_vec, (size_t)4 * m.n_dofs * 8);  // [f_mid, f_in, r, L]
+  ENSURE(ctx->b_scal, 16 * 8);                    // S0, S1, lambda, E, P1, P2, ...
+  ENSURE(ctx->b_statp, (size_t)epi_blocks((int)N) * kStatWidth * 8);
+  ENSURE(ctx->b_stat, kStatWidth * 8);
+  return COLLOP_OK;
+}
+
+}  // namespace
+
+// ===========================================================================
+extern "C" {
+
+int collop_version(void) { return 100; }
+
+int collop_create(int device, collop_ctx** out) {
+  if (!out) return COLLOP_ERR_INVALID;
+  *out = nullptr;
+  int n = 0;
+  if (hipGetDeviceCount(&n) != hipSuccess || n <= 0) return COLLOP_ERR_HIP;
+  if (device < 0 || device >= n) return COLLOP_ERR_INVALID;
+  if (hipSetDevice(device) != hipSuccess) return COLLOP_ERR_HIP;
+  auto* c = new collop_ctx();
+  c->device = device;
+  if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess) {
+    delete c;
+    return COLLOP_ERR_HIP;
+  }
+  if (hipHostMalloc((void**)&c->h_stat, kStatWidth * sizeof(double), hipHostMallocDefault) != hipSuccess) {
+    (void)hipStreamDestroy(c->stream);
+    delete c;
+    return COLLOP_ERR_HIP;
+  }
+  *out = c;
+  return COLLOP_OK;
+}
+
+int collop_destroy(collop_ctx* ctx) {
+  if (!ctx) return COLLOP_OK;
+  (void)hipSetDevice(ctx->device);
+  (void)hipStreamSynchronize(ctx->stream);
+  free_mesh(ctx);
+  for (DBuf* b : {&ctx->b_v, &ctx->b_w, &ctx->b_G, &ctx->b_out, &ctx->b_vmid, &ctx->b_dv, &ctx->b_dS, &ctx->b_s64,
+                  &ctx->b_s32, &ctx->b_part, &ctx->b_spart_sc, &ctx->b_mom, &ctx->b_dpart, &ctx->b_iota,
+                  &ctx->b_spart_q, &ctx->b_vec, &ctx->b_scal, &ctx->b_statp, &ctx->b_stat, &ctx->s_v, &ctx->s_w,
+                  &ctx->s_v1, &ctx->s_Gv, &ctx->s_r, &ctx->s_rp, &ctx->s_Gp, &ctx->s_best, &ctx->s_dF, &ctx->s_dG,
+                  &ctx->s_fc, &ctx->s_dotv})
+    release(*b);
+  for (auto& e : ctx->ev_pool) {
+    (void)hipEventDestroy(e.first);
+    (void)hipEventDestroy(e.second);
+  }
+  if (ctx->h_stat) (void)hipHostFree(ctx->h_stat);
+  (void)hipStreamDestroy(ctx->stream);
+  delete ctx;
+  return COLLOP_OK;
+}
+
+const char* collop_last_error(const collop_ctx* ctx) { return ctx ? ctx->err.c_str() : "null context"; }
+
+int collop_set_mesh(collop_ctx* ctx, const double* bp1, int nb1, const double* bp2, int nb2, const double* C1,
+                    const double* C2, const int32_t* bs1, const int32_t* bs2, int nd1, int nd2, int p_deg,
+                    int n_quad) {
+  if (!ctx) return COLLOP_ERR_INVALID;
+  if (p_deg != 2)
+    return fail(ctx, COLLOP_ERR_UNSUPPORTED, "the device kernels are specialised to P_DEG=2 (got " +
+                                                 std::to_string(p_deg) + ")");
+  if (!bp1 || !bp2 || !C1 || !C2 || !bs1 || !bs2) return fail(ctx, COLLOP_ERR_INVALID, "null mesh table");
+  if (nb1 < 2 || nb2 < 2 || nb1 > 512 || nb2 > 512) return fail(ctx, COLLOP_ERR_INVALID, "breakpoint count out of range [2, 512]");
+  if (n_quad < 1 || n_quad > 16) return fail(ctx, COLLOP_ERR_INVALID, "n_quad out of range [1, 16]");
+  for (int i = 0; i + 1 < nb1; ++i)
+    if (!(bp1[i] < bp1[i + 1])) return fail(ctx, COLLOP_ERR_INVALID, "bp1 must be strictly increasing");
+  for (int i = 0; i + 1 < nb2; ++i)
+    if (!(bp2[i] < bp2[i + 1])) return fail(ctx, COLLOP_ERR_INVALID, "bp2 must be strictly increasing");
+  const int ne1 = nb1 - 1, ne2 = nb2 - 1;
+  std::vector<int> b1(ne1), b2(ne2);
+  for (int e = 0; e < ne1; ++e) b1[e] = bs1[e] - 1;
+  for (int e = 0; e < ne2; ++e) b2[e] = bs2[e] - 1;
+  for (int e = 0; e < ne1; ++e)
+    if (b1[e] < 0 || b1[e] + 2 >= nd1) return fail(ctx, COLLOP_ERR_INVALID, "bs1 inconsistent with nd1");
+  for (int e = 0; e < ne2; ++e)
+    if (b2[e] < 0 || b2[e] + 2 >= nd2) return fail(ctx, COLLOP_ERR_INVALID, "bs2 inconsistent with nd2");
+  const long n_dofs = (long)nd1 * nd2;
+  if (n_dofs * 8 + 2 * 512 * 8 > 64 * 1024)
+    return fail(ctx, COLLOP_ERR_UNSUPPORTED, "n_dofs too large for the LDS-privatised scatter (max 6144)");
+  (void)hipSetDevice(ctx->device);
+  (void)hipStreamSynchronize(ctx->stream);
+  free_mesh(ctx);
+  ctx->has_state = false;
+
+  std::vector<double> v1(bp1, bp1 + nb1), v2(bp2, bp2 + nb2);
+  std::vector<double> ch1, ch2;
+  if (!mass_factor_1d(v1, C1, b1.data(), nd1, ch1) || !mass_factor_1d(v2, C2, b2.data(), nd2, ch2))
+    return fail(ctx, COLLOP_ERR_INVALID, "1D mass matrix is not positive definite (check the extraction tables)");
+  std::vector<double> qx, qw;
+  gauss_legendre(n_quad, qx, qw);
+  std::vector<double> ph1((size_t)ne1 * 3 * n_quad), ph2((size_t)ne2 * 3 * n_quad);
+  for (int e = 0; e < ne1; ++e)
+    for (int a = 0; a < n_quad; ++a) {
+      double phi[3];
+      cell_basis(C1, e, qx[a], phi);
+      for (int j = 0; j < 3; ++j) ph1[(size_t)(3 * e + j) * n_quad + a] = phi[j];
+    }
+  for (int e = 0; e < ne2; ++e)
+    for (int a = 0; a < n_quad; ++a) {
+      double phi[3];
+      cell_basis(C2, e, qx[a], phi);
+      for (int j = 0; j < 3; ++j) ph2[(size_t)(3 * e + j) * n_quad + a] = phi[j];
+    }
+  auto erange = [](const std::vector<int>& bs, int nd, std::vector<int>& out) {
+    out.assign((size_t)2 * nd, 0);
+    for (int i = 0; i < nd; ++i) {
+      out[2 * i] = std::numeric_limits<int>::max();
+      out[2 * i + 1] = -1;
+    }
+    for (int e = 0; e < (int)bs.size(); ++e)
+      for (int j = 0; j < 3; ++j) {
+        const int i = bs[e] + j;
+        out[2 * i] = std::min(out[2 * i], e);
+        out[2 * i + 1] = std::max(out[2 * i + 1], e);
+      }
+    for (int i = 0; i < nd; ++i)
+      if (out[2 * i + 1] < 0) out[2 * i] = 0;  // empty support (cannot happen for valid tables)
+  };
+  std::vector<int> er1, er2;
+  erange(b1, nd1, er1);
+  erange(b2, nd2, er2);
+
+  MeshView m{};
+  int r;
+  const void* d;
+#define UP(field, vec, T)                                                      \
+  r = upload(ctx, (vec).data(), (vec).size() * sizeof(T), &d);                 \
+  if (r != COLLOP_OK) return r;                                                \
+  m.field = static_cast<const T*>(d);
+  UP(bp1, v1, double)
+  UP(bp2, v2, double)
+  std::vector<double> c1(C1, C1 + (size_t)9 * ne1), c2(C2, C2 + (size_t)9 * ne2);
+  UP(C1, c1, double)
+  UP(C2, c2, double)
+  UP(bs1, b1, int)
+  UP(bs2, b2, int)
+  UP(chol1, ch1, double)
+  UP(chol2, ch2, double)
+  UP(phiq1, ph1, double)
+  UP(phiq2, ph2, double)
+  UP(qw, qw, double)
+  UP(erange1, er1, int)
+  UP(erange2, er2, int)
+#undef UP
+  m.nb1 = nb1;
+  m.nb2 = nb2;
+  m.ne1 = ne1;
+  m.ne2 = ne2;
+  m.nd1 = nd1;
+  m.nd2 = nd2;
+  m.n_dofs = (int)n_dofs;
+  m.nq = n_quad;
+  m.lo1 = bp1[0];
+  m.hi1 = bp1[nb1 - 1];
+  m.lo2 = bp2[0];
+  m.hi2 = bp2[nb2 - 1];
+  m.inv_h1 = ne1 / (m.hi1 - m.lo1);
+  m.inv_h2 = ne2 / (m.hi2 - m.lo2);
+  auto is_uniform = [](const std::vector<double>& bp) {
+    const int ne = (int)bp.size() - 1;
+    const double h = (bp.back() - bp.front()) / ne;
+    for (int i = 0; i < ne; ++i)
+      if (std::fabs((bp[i + 1] - bp[i]) - h) > 1e-6 * h) return 0;
+    return 1;
+  };
+  m.uniform1 = is_uniform(v1);
+  m.uniform2 = is_uniform(v2);
+  ctx->mv = m;
+  ctx->h_bp1 = v1;
+  ctx->h_bp2 = v2;
+  ctx->has_mesh = true;
+  return COLLOP_OK;
+}
+
+// ---- hook-level ------------------------------------------------------------
+
+static int collision_common(collop_ctx* ctx, int kind, const double* v, const double* w, const double* G,
+                            double* dot_v, double* A, int64_t N) {
+  int r = need_mesh(ctx);
+  if (r) return r;
+  if ((r = check_N(ctx, N))) return r;
+  if (N == 0) return COLLOP_OK;
+  if (!v || !w || !G || !dot_v || (kind == 2 && !A)) return fail(ctx, COLLOP_ERR_INVALID, "null array");
+  HIPC(hipSetDevice(ctx->device));
+  if ((r = ensure_scratch(ctx, N))) return r;
+  const int n = (int)N;
+  choose_tile(ctx, n);
+  HIPC(hipMemcpyAsync(ctx->b_v.p, v, 2 * N * 8, hipMemcpyHostToDevice, ctx->stream));
+  HIPC(hipMemcpyAsync(ctx->b_w.p, w, N * 8, hipMemcpyHostToDevice, ctx->stream));
+  HIPC(hipMemcpyAsync(ctx->b_G.p, G, 2 * N * 8, hipMemcpyHostToDevice, ctx->stream));
+  const MeshView& m = ctx->mv;
+  launch_pack_src(P<double>(ctx->b_v), P<double>(ctx->b_G), P<double>(ctx->b_w), n, m.lo1, m.hi1, m.lo2, m.hi2,
+                  kind == 1 ? nullptr : P<Src64>(ctx->b_s64), kind == 1 ? P<Src32>(ctx->b_s32) : nullptr,
+                  ctx->stream);
+  CHECK_LAUNCH();
+  int S;
+  if (kind == 2) {
+    launch_pair_metric_f64(P<Src64>(ctx->b_s64), n, ctx->tile64, P<double>(ctx->b_part), ctx->stream);
+    CHECK_LAUNCH();
+    S = ctx->tile64.slices;
+  } else {
+    if ((r = run_pair(ctx, kind, n, P<double>(ctx->b_part)))) return r;
+    S = kind == 0 ? ctx->tile64.slices : ctx->tile32.slices;
+  }
+  const int64_t len = (kind == 2 ? 5 : 2) * N;
+  launch_slice_sum(P<double>(ctx->b_part), S, len, P<double>(ctx->b_out), ctx->stream);
+  CHECK_LAUNCH();
+  HIPC(hipMemcpyAsync(dot_v, ctx->b_out.p, 2 * N * 8, hipMemcpyDeviceToHost, ctx->stream));
+  if (kind == 2) HIPC(hipMemcpyAsync(A, P<double>(ctx->b_out) + 2 * N, 3 * N * 8, hipMemcpyDeviceToHost, ctx->stream));
+  HIPC(hipStreamSynchronize(ctx->stream));
+  prof_collect(ctx);
+  return COLLOP_OK;
+}
+
+int collop_collision_f64(collop_ctx* ctx, const double* v, const double* w, const double* G, double* dot_v,
+                         int64_t N) {
+  return collision_common(ctx, 0, v, w, G, dot_v, nullptr, N);
+}
+
+int collop_collision_f32c(collop_ctx* ctx, const double* v, const double* w, const double* G, double* dot_v,
+                          int64_t N) {
+  return collision_common(ctx, 1, v, w, G, dot_v, nullptr, N);
+}
+
+int collop_collision_metric_f64(collop_ctx* ctx, const double* v, const double* w, const double* G, double* dot_v,
+                                double* A, int64_t N) {
+  return collision_common(ctx, 2, v, w, G, dot_v, A, N);
+}
+
+static int rhs_common(collop_ctx* ctx, const double* v, const double* w, int64_t N, double* out, bool solve) {
+  int r = need_mesh(ctx);
+  if (r) return r;
+  if ((r = check_N(ctx, N))) return r;
+  if (!out || (N > 0 && (!v || !w))) return fail(ctx, COLLOP_ERR_INVALID, "null array");
+  HIPC(hipSetDevice(ctx->device));
+  if ((r = ensure_scratch(ctx, N))) return r;
+  const int n = (int)N;
+  const MeshView& m = ctx->mv;
+  if (N > 0) {
+    HIPC(hipMemcpyAsync(ctx->b_v.p, v, 2 * N * 8, hipMemcpyHostToDevice, ctx->stream));
+    HIPC(hipMemcpyAsync(ctx->b_w.p, w, N * 8, hipMemcpyHostToDevice, ctx->stream));
+  }
+  const int nsb = scatter_blocks(n);
+  double wsum = 0.0;
+  for (int64_t i = 0; i < N; ++i) wsum += std::fabs(w[i]);
+  launch_scatter(m, nullptr, P<double>(ctx->b_v), P<double>(ctx->b_w), n, 1, 1, wsum, nullptr, nullptr,
+                 P<long long>(ctx->b_spart_sc), nullptr, ctx->stream);
+  CHECK_LAUNCH();
+  double* vec = P<double>(ctx->b_vec);
+  if (solve) {
+    launch_reduce_solve(m, P<long long>(ctx->b_spart_sc), nsb, wsum, nullptr, 1, vec, ctx->stream);
+  } else {
+    launch_rhs(m, P<long long>(ctx->b_spart_sc), nsb, wsum, vec, ctx->stream);
+  }
+  CHECK_LAUNCH();
+  HIPC(hipMemcpyAsync(out, vec, (size_t)m.n_dofs * 8, hipMemcpyDeviceToHost, ctx->stream));
+  HIPC(hipStreamSynchronize(ctx->stream));
+  return COLLOP_OK;
+}
+
+int collop_l2_rhs(collop_ctx* ctx, const double* v, const double* w, int64_t N, double* rhs) {
+  return rhs_common(ctx, v, w, N, rhs, false);
+}
+
+int collop_l2_project(collop_ctx* ctx, const double* v, const double* w, int64_t N, double* f_coeffs) {
+  return rhs_common(ctx, v, w, N, f_coeffs, true);
+}
+
+static int gather_common(collop_ctx* ctx, const double* v, int64_t N, const double* coeff, int mode, double fs_floor,
+                         double g_max, double* out) {
+  int r = need_mesh(ctx);
+  if (r) return r;
+  if ((r = check_N(ctx, N))) return r;
+  if (N == 0) return COLLOP_OK;
+  if (!v || !coeff || !out) return fail(ctx, COLLOP_ERR_INVALID, "null array");
+  HIPC(hipSetDevice(ctx->device));
+  if ((r = ensure_scratch(ctx, N))) return r;
+  const MeshView& m = ctx->mv;
+  double* vec = P<double>(ctx->b_vec);
+  HIPC(hipMemcpyAsync(ctx->b_v.p, v, 2 * N * 8, hipMemcpyHostToDevice, ctx->stream));
+  HIPC(hipMemcpyAsync(vec, coeff, (size_t)m.n_dofs * 8, hipMemcpyHostToDevice, ctx->stream));
+  launch_gather(m, P<double>(ctx->b_v), vec, nullptr, nullptr, (int)N, mode, fs_floor, g_max, P<double>(ctx->b_out),
+                nullptr, ctx->stream);
+  CHECK_LAUNCH();
+  HIPC(hipMemcpyAsync(out, ctx->b_out.p, 2 * N * 8, hipMemcpyDeviceToHost, ctx->stream));
+  HIPC(hipStreamSynchronize(ctx->stream));
+  return COLLOP_OK;
+}
+
+int collop_grad_gather(collop_ctx* ctx, const double* v, int64_t N, const double* L, double* G) {
+  return gather_common(ctx, v, N, L, 0, 0.0, 0.0, G);
+}
+
+int collop_loggrad_gather(collop_ctx* ctx, const double* v, int64_t N, const double* f_coeffs, double fs_floor,
+                          double g_max, double* g) {
+  if (ctx && !(fs_floor > 0.0 && g_max > 0.0))
+    return fail(ctx, COLLOP_ERR_INVALID, "fs_floor and g_max must be positive");
+  return gather_common(ctx, v, N, f_coeffs, 2, fs_floor, g_max, g);
+}
+
+int collop_quadrature(collop_ctx* ctx, const double* f_coeffs, int use_logsq, double* r_out, double* entropy) {
+  int r = need_mesh(ctx);
+  if (r) return r;
+  if (!f_coeffs) return fail(ctx, COLLOP_ERR_INVALID, "null array");
+  HIPC(hipSetDevice(ctx->device));
+  if ((r = ensure_scratch(ctx, 1))) return r;
+  const MeshView& m = ctx->mv;
+  double* vec = P<double>(ctx->b_vec);
+  double* sc = P<double>(ctx->b_scal);
+  HIPC(hipMemcpyAsync(vec, f_coeffs, (size_t)m.n_dofs * 8, hipMemcpyHostToDevice, ctx->stream));
+  launch_quad1(m, vec, vec, 1, use_logsq, P<double>(ctx->b_iota), P<double>(ctx->b_spart_q), ctx->stream);
+  CHECK_LAUNCH();
+  launch_quad2(m, P<double>(ctx->b_iota), vec + 2 * m.n_dofs, ctx->stream);
+  CHECK_LAUNCH();
+  launch_sum_rows(P<double>(ctx->b_spart_q), quad_blocks(m), 1, 0, 1, sc, ctx->stream);
+  CHECK_LAUNCH();
+  if (r_out) HIPC(hipMemcpyAsync(r_out, vec + 2 * m.n_dofs, (size_t)m.n_dofs * 8, hipMemcpyDeviceToHost, ctx->stream));
+  if (entropy) HIPC(hipMemcpyAsync(entropy, sc, 8, hipMemcpyDeviceToHost, ctx->stream));
+  HIPC(hipStreamSynchronize(ctx->stream));
+  return COLLOP_OK;
+}
+
+int collop_mass_solve(collop_ctx* ctx, const double* b, double* x) {
+  int r = need_mesh(ctx);
+  if (r) return r;
+  if (!b || !x) return fail(ctx, COLLOP_ERR_INVALID, "null array");
+  HIPC(hipSetDevice(ctx->device));
+  if ((r = ensure_scratch(ctx, 1))) return r;
+  const MeshView& m = ctx->mv;
+  double* vec = P<double>(ctx->b_vec);
+  HIPC(hipMemcpyAsync(vec, b, (size_t)m.n_dofs * 8, hipMemcpyHostToDevice, ctx->stream));
+  launch_reduce_solve(m, nullptr, 0, 1.0, vec, 1, vec + m.n_dofs, ctx->stream);
+  CHECK_LAUNCH();
+  HIPC(hipMemcpyAsync(x, vec + m.n_dofs, (size_t)m.n_dofs * 8, hipMemcpyDeviceToHost, ctx->stream));
+  HIPC(hipStreamSynchronize(ctx->stream));
+  return COLLOP_OK;
+}
+
+}  // extern "C"
+
+// ===========================================================================
+// Device-resident step
+namespace {
+
+// Projection of the state markers, its entropy S0 and the momentum / energy
+// (main.jl:298-303).  Leaves fc in s_fc and S0, E, P1, P2 in scal[0], [3..5].
+int project_state(collop_ctx* ctx, int use_logsq) {
+  const MeshView& m = ctx->mv;
+  const int n = (int)ctx->N;
+  const int nsb = scatter_blocks(n);
+  double* sc = P<double>(ctx->b_scal);
+  launch_scatter(m, nullptr, P<double>(ctx->s_v), P<double>(ctx->s_w), n, 1, 1, ctx->w_abs_sum, nullptr, nullptr,
+                 P<long long>(ctx->b_spart_sc), P<double>(ctx->b_mom), ctx->stream);
+  CHECK_LAUNCH();
+  launch_reduce_solve(m, P<long long>(ctx->b_spart_sc), nsb, ctx->w_abs_sum, nullptr, 1, P<double>(ctx->s_fc),
+                      ctx->stream);
+  CHECK_LAUNCH();
+  launch_quad1(m, P<double>(ctx->s_fc), P<double>(ctx->s_fc), 1, use_logsq, nullptr, P<double>(ctx->b_spart_q),
+               ctx->stream);
+  CHECK_LAUNCH();
+  launch_sum_rows(P<double>(ctx->b_spart_q), quad_blocks(m), 1, 0, 1, sc + 0, ctx->stream);
+  CHECK_LAUNCH();
+  // momentum / energy: columns of mom[nsb][3]
+  launch_sum_rows(P<double>(ctx->b_mom), nsb, 3, 1, 3, sc + 3, ctx->stream);
+  CHECK_LAUNCH();
+  return COLLOP_OK;
+}
+
+// G = grad(M^{-1} r(f_s)) at x, with f_s = fc (main.jl:240-242); result in b_G.
+int entropy_seed_gradient(collop_ctx* ctx, const double* x, const double* fc, int use_logsq) {
+  const MeshView& m = ctx->mv;
+  double* vec = P<double>(ctx->b_vec);
+  launch_quad1(m, fc, fc, 1, use_logsq, P<double>(ctx->b_iota), nullptr, ctx->stream);
+  CHECK_LAUNCH();
+  launch_quad2(m, P<double>(ctx->b_iota), vec + 2 * m.n_dofs, ctx->stream);
+  CHECK_LAUNCH();
+  launch_reduce_solve(m, nullptr, 0, 1.0, vec + 2 * m.n_dofs, 1, vec + 3 * m.n_dofs, ctx->stream);
+  CHECK_LAUNCH();
+  launch_gather(m, x, vec + 3 * m.n_dofs, nullptr, nullptr, (int)ctx->N, 0, 0.0, 0.0, P<double>(ctx->b_G), nullptr,
+                ctx->stream);
+  CHECK_LAUNCH();
+  return COLLOP_OK;
+}
+
+// One Picard map (solver.jl:176-227, Landau branch): Gv = v0 + dt vdot(v_mid, G_eff),
+// followed by the epilogue of the Anderson iteration.
+int picard_map_dev(collop_ctx* ctx, const collop_step_params& p, int kind, const double* vin, int col, int hist) {
+  const MeshView& m = ctx->mv;
+  const int n = (int)ctx->N;
+  const int nsb = scatter_blocks(n);
+  double* vec = P<double>(ctx->b_vec);
+  double* sc = P<double>(ctx->b_scal);
+  // v_mid, dv and the rhs of v_mid (and of v_in for the Gonzalez entropy S(v_in))
+  launch_scatter(m, P<double>(ctx->s_v), vin, P<double>(ctx->s_w), n, 2, p.use_gonzalez ? 2 : 1, ctx->w_abs_sum,
+                 P<double>(ctx->b_vmid), P<double>(ctx->b_dv), P<long long>(ctx->b_spart_sc), nullptr, ctx->stream);
+  CHECK_LAUNCH();
+  launch_reduce_solve(m, P<long long>(ctx->b_spart_sc), nsb, ctx->w_abs_sum, nullptr, p.use_gonzalez ? 2 : 1, vec,
+                      ctx->stream);
+  CHECK_LAUNCH();
+  // r(f_mid) and S(f_in)
+  launch_quad1(m, vec, vec + m.n_dofs, p.use_gonzalez ? 2 : 1, p.use_logsq, P<double>(ctx->b_iota),
+               P<double>(ctx->b_spart_q), ctx->stream);
+  CHECK_LAUNCH();
+  launch_quad2(m, P<double>(ctx->b_iota), vec + 2 * m.n_dofs, ctx->stream);
+  CHECK_LAUNCH();
+  launch_reduce_solve(m, nullptr, 0, 1.0, vec + 2 * m.n_dofs, 1, vec + 3 * m.n_dofs, ctx->stream);
+  CHECK_LAUNCH();
+  const int nqb = quad_blocks(m);
+  if (p.use_gonzalez) {
+    launch_sum_rows(P<double>(ctx->b_spart_q) + nqb, nqb, 1, 0, 1, sc + 1, ctx->stream);
+    CHECK_LAUNCH();
+  }
+  // dS = -w grad L(v_mid) and the partial sums of dv.dS, |dv|^2
+  launch_gather(m, P<double>(ctx->b_vmid), vec + 3 * m.n_dofs, P<double>(ctx->s_w), P<double>(ctx->b_dv), n, 1, 0.0,
+                0.0, P<double>(ctx->b_dS), P<double>(ctx->b_dpart), ctx->stream);
+  CHECK_LAUNCH();
+  launch_geff_pack(P<double>(ctx->b_vmid), P<double>(ctx->b_dS), P<double>(ctx->b_dv), P<double>(ctx->s_w), n,
+                   P<double>(ctx->b_dpart), gather_blocks(n), sc + 1, sc + 0, p.use_gonzalez, m.lo1, m.hi1, m.lo2,
+                   m.hi2, kind == 0 ? P<Src64>(ctx->b_s64) : nullptr, kind == 1 ? P<Src32>(ctx->b_s32) : nullptr,
+                   sc + 2, ctx->stream);
+  CHECK_LAUNCH();
+  int r = run_pair(ctx, kind, n, P<double>(ctx->b_part));
+  if (r) return r;
+  EpiArgs a{};
+  a.part = P<double>(ctx->b_part);
+  a.S = kind == 0 ? ctx->tile64.slices : ctx->tile32.slices;
+  a.v0 = P<double>(ctx->s_v);
+  a.v1 = vin;
+  a.dt = p.dt;
+  a.Gv = P<double>(ctx->s_Gv);
+  a.r = P<double>(ctx->s_r);
+  a.rp = P<double>(ctx->s_rp);
+  a.Gp = P<double>(ctx->s_Gp);
+  a.dF = P<double>(ctx->s_dF);
+  a.dG = P<double>(ctx->s_dG);
+  a.col = col;
+  a.hist = hist;
+  a.stat_part = P<double>(ctx->b_statp);
+  launch_epilogue(a, n, ctx->stream);
+  CHECK_LAUNCH();
+  launch_sum_rows(P<double>(ctx->b_statp), epi_blocks(n), kStatWidth, 1, kStatWidth, P<double>(ctx->b_stat),
+                  ctx->stream);
+  CHECK_LAUNCH();
+  HIPC(hipMemcpyAsync(ctx->h_stat, ctx->b_stat.p, kStatWidth * 8, hipMemcpyDeviceToHost, ctx->stream));
+  HIPC(hipStreamSynchronize(ctx->stream));
+  prof_collect(ctx);
+  return COLLOP_OK;
+}
+
+// Small SPD solve (Cholesky; Julia's factorize picks it for the exactly
+// symmetric ATA), with partial-pivot LU as the fallback.
+void small_solve(int n, const double* A0, const double* b0, double* x) {
+  double A[kMaxWindow * kMaxWindow], L[kMaxWindow * kMaxWindow], b[kMaxWindow];
+  std::memcpy(A, A0, sizeof(double) * n * n);
+  std::memcpy(b, b0, sizeof(double) * n);
+  std::memcpy(L, A0, sizeof(double) * n * n);
+  bool ok = true;
+  for (int j = 0; j < n && ok; ++j) {
+    double d = L[j * n + j];
+    for (int k = 0; k < j; ++k) d -= L[j * n + k] * L[j * n + k];
+    if (!(d > 0.0)) {
+      ok = false;
+      break;
+    }
+    d = std::sqrt(d);
+    L[j * n + j] = d;
+    for (int i = j + 1; i < n; ++i) {
+      double t = L[i * n + j];
+      for (int k = 0; k < j; ++k) t -= L[i * n + k] * L[j * n + k];
+      L[i * n + j] = t / d;
+    }
+  }
+  if (ok) {
+    double y[kMaxWindow];
+    for (int i = 0; i < n; ++i) {
+      double t = b[i];
+      for (int k = 0; k < i; ++k) t -= L[i * n + k] * y[k];
+      y[i] = t / L[i * n + i];
+    }
+    for (int i = n - 1; i >= 0; --i) {
+      double t = y[i];
+      for (int k = i + 1; k < n; ++k) t -= L[k * n + i] * x[k];
+      x[i] = t / L[i * n + i];
+    }
+    return;
+  }
+  for (int k = 0; k < n; ++k) {
+    int pr = k;
+    for (int i = k + 1; i < n; ++i)
+      if (std::fabs(A[i * n + k]) > std::fabs(A[pr * n + k])) pr = i;
+    if (pr != k) {
+      for (int j = 0; j < n; ++j) std::swap(A[k * n + j], A[pr * n + j]);
+      std::swap(b[k], b[pr]);
+    }
+    for (int i = k + 1; i < n; ++i) {
+      const double f = A[i * n + k] / A[k * n + k];
+      for (int j = k; j < n; ++j) A[i * n + j] -= f * A[k * n + j];
+      b[i] -= f * b[k];
+    }
+  }
+  for (int i = n - 1; i >= 0; --i) {
+    double t = b[i];
+    for (int j = i + 1; j < n; ++j) t -= A[i * n + j] * x[j];
+    x[i] = t / A[i * n + i];
+  }
+}
+
+int copy_d2d(collop_ctx* ctx, DBuf& dst, const DBuf& src, size_t bytes) {
+  HIPC(hipMemcpyAsync(dst.p, src.p, bytes, hipMemcpyDeviceToDevice, ctx->stream));
+  return COLLOP_OK;
+}
+
+int validate_params(collop_ctx* ctx, const collop_step_params* p) {
+  if (!p) return fail(ctx, COLLOP_ERR_INVALID, "null params");
+  if (!(p->dt > 0.0) || !std::isfinite(p->dt)) return fail(ctx, COLLOP_ERR_INVALID, "dt must be positive");
+  if (p->m_anderson < 1 || p->m_anderson > kMaxWindow)
+    return fail(ctx, COLLOP_ERR_INVALID, "m_anderson must be in [1, 8]");
+  if (p->max_iter < 1) return fail(ctx, COLLOP_ERR_INVALID, "max_iter must be >= 1");
+  if (p->stag_window < 1) return fail(ctx, COLLOP_ERR_INVALID, "stag_window must be >= 1");
+  if (!(p->damping > 0.0 && p->damping <= 1.0)) return fail(ctx, COLLOP_ERR_INVALID, "damping must be in (0, 1]");
+  if (p->precision != COLLOP_PREC_F64 && p->precision != COLLOP_PREC_F32C)
+    return fail(ctx, COLLOP_ERR_INVALID, "unknown precision");
+  if (p->warmstart != COLLOP_WARM_EULER && p->warmstart != COLLOP_WARM_MLP)
+    return fail(ctx, COLLOP_ERR_INVALID, "unknown warmstart");
+  if (p->warmstart == COLLOP_WARM_MLP)
+    return fail(ctx, COLLOP_ERR_NO_MODEL, "no warm-start model loaded on this context");
+  return COLLOP_OK;
+}
+
+// One implicit step on the device-resident state (main.jl:226-307 with
+// step_anderson!, solver.jl:300-423).
+int step_once(collop_ctx* ctx, const collop_step_params& p, collop_step_stats* out) {
+  const auto t0 = std::chrono::steady_clock::now();
+  const int n = (int)ctx->N;
+  const size_t n2b = (size_t)2 * n * 8;
+  int r;
+  // ---- predictor: v1 = v + dt vdot(v, G), G = grad(M^{-1} r(f_s(v)))
+  if ((r = entropy_seed_gradient(ctx, P<double>(ctx->s_v), P<double>(ctx->s_fc), p.use_logsq))) return r;
+  const MeshView& m = ctx->mv;
+  launch_pack_src(P<double>(ctx->s_v), P<double>(ctx->b_G), P<double>(ctx->s_w), n, m.lo1, m.hi1, m.lo2, m.hi2,
+                  P<Src64>(ctx->b_s64), nullptr, ctx->stream);
+  CHECK_LAUNCH();
+  if ((r = run_pair(ctx, 0, n, P<double>(ctx->b_part)))) return r;
+  launch_euler(P<double>(ctx->b_part), ctx->tile64.slices, P<double>(ctx->s_v), p.dt, n, P<double>(ctx->s_v1),
+               P<double>(ctx->s_dotv), ctx->stream);
+  CHECK_LAUNCH();
+
+  // ---- Anderson (solver.jl:300-423)
+  const int mw = p.m_anderson;
+  int history = 0, slot = 0, n_restart = 0;
+  double nrm_r0 = 0.0, nrm_r = 0.0, nrm_best = std::numeric_limits<double>::infinity();
+  double nrm_best_window = std::numeric_limits<double>::infinity();
+  double ATA[kMaxWindow * kMaxWindow] = {0};
+  bool use32 = p.precision == COLLOP_PREC_F32C;
+  int iters32 = 0;
+  const double floor_factor = p.f32_floor_factor > 0.0 ? p.f32_floor_factor : 16.0;
+  int exit_reason = 2, k_done = p.max_iter;
+  double res_out = 0.0;
+  // The best iterate starts as the initial guess (solver.jl:331).
+  if ((r = copy_d2d(ctx, ctx->s_best, ctx->s_v1, n2b))) return r;
+  int since_reset = 0;  // maps since the window was last (re)started
+  double best32 = std::numeric_limits<double>::infinity();
+  int since_best32 = 0;
+  const bool trace = std::getenv("COLLOP_TRACE") != nullptr;
+  for (int k = 1; k <= p.max_iter; ++k) {
+    const bool anderson_step = p.use_anderson && since_reset > 0;
+    const int col = anderson_step ? slot % mw : -1;
+    const int hist_new = anderson_step ? std::min(history + 1, mw) : 0;
+    const int kind = use32 ? 1 : 0;
+    if ((r = picard_map_dev(ctx, p, kind, P<double>(ctx->s_v1), col, hist_new))) return r;
+    since_reset++;
+    const double* st = ctx->h_stat;
+    nrm_r = std::sqrt(st[0]);
+    const double nrm_v1 = std::sqrt(st[1]);
+    if (!std::isfinite(nrm_r) || !std::isfinite(nrm_v1))
+      return fail(ctx, COLLOP_ERR_NONFINITE, "non-finite residual in Picard map " + std::to_string(k));
+    if (k == 1) nrm_r0 = nrm_r;
+    const double eff_tol = std::max(p.tol * (nrm_v1 + 1e-30), p.abs_floor);
+    if (use32) {
+      iters32++;
+      // FP32 maps run until their own rounding floor (a few FP32 ulps of the
+      // step increment |Gv - v0|), or until six maps bring no new best
+      // residual; the solve then restarts its window on FP64 maps, so every
+      // accepted iterate and the exit map are FP64.  Switching well inside the
+      // basin keeps the FP64 maps on the root the plain FP64 solve finds (the
+      // implicit system can have several roots at large dt).
+      const double floor32 = floor_factor * 5.96e-8 * std::sqrt(st[2]);
+      if (nrm_r < best32) {
+        best32 = nrm_r;
+        since_best32 = 0;
+      } else {
+        since_best32++;
+      }
+      const bool switch_now = nrm_r < std::max(floor32, eff_tol) || since_best32 >= 6 || k >= p.max_iter / 2;
+      if (trace)
+        std::fprintf(stderr, "  [f32] k=%d |r|=%.3e floor=%.3e best=%.3e\n", k, nrm_r, floor32, best32);
+      if (switch_now) {
+        use32 = false;
+        history = 0;
+        slot = 0;
+        since_reset = 0;
+        launch_update(P<double>(ctx->s_v1), P<double>(ctx->s_Gv), P<double>(ctx->s_dG), nullptr, 0, p.damping, n,
+                      ctx->stream);
+        CHECK_LAUNCH();
+        std::swap(ctx->s_rp, ctx->s_r);
+        std::swap(ctx->s_Gp, ctx->s_Gv);
+        continue;
+      }
+    } else {
+      if (trace) std::fprintf(stderr, "  [f64] k=%d |r|=%.3e tol=%.3e\n", k, nrm_r, eff_tol);
+      if (nrm_r < nrm_best) {
+        nrm_best = nrm_r;
+        if ((r = copy_d2d(ctx, ctx->s_best, p.exit_picard_step ? ctx->s_Gv : ctx->s_v1, n2b))) return r;
+      }
+      if (nrm_r < eff_tol) {
+        if (p.exit_picard_step) std::swap(ctx->s_v1, ctx->s_Gv);
+        exit_reason = 0;
+        k_done = k;
+        res_out = nrm_r;
+        break;
+      }
+      if (k > p.stag_window && k % p.stag_window == 0) {
+        const double rel = (nrm_best_window - nrm_best) / (nrm_best_window + 1e-30);
+        if (rel < p.stag_rel_tol) {
+          std::swap(ctx->s_v1, ctx->s_best);
+          exit_reason = 1;
+          k_done = k;
+          res_out = nrm_best;
+          break;
+        }
+        nrm_best_window = nrm_best;
+      }
+    }
+    bool just_restarted = false;
+    if (!use32 && k > 1 && nrm_r > p.restart_factor * nrm_best) {
+      history = 0;
+      slot = 0;
+      n_restart++;
+      just_restarted = true;
+    }
+    const double beta = k > p.damp_decay_start ? p.damping * p.damp_decay_factor : p.damping;
+    if (just_restarted || col < 0) {
+      launch_update(P<double>(ctx->s_v1), P<double>(ctx->s_Gv), P<double>(ctx->s_dG), nullptr, 0, beta, n,
+                    ctx->stream);
+      CHECK_LAUNCH();
+    } else {
+      slot = slot % mw + 1;
+      history = hist_new;
+      const int h = history;
+      for (int j = 0; j < h; ++j) {
+        ATA[col * kMaxWindow + j] = st[3 + j];
+        ATA[j * kMaxWindow + col] = st[3 + j];
+      }
+      double A[kMaxWindow * kMaxWindow], b[kMaxWindow], g[kMaxWindow];
+      double dm = 0.0;
+      for (int a = 0; a < h; ++a) {
+        for (int c = 0; c < h; ++c) A[a * h + c] = ATA[a * kMaxWindow + c];
+        b[a] = st[3 + kMaxWindow + a];
+        dm += A[a * h + a];
+      }
+      dm /= h;
+      const double l2 = p.reg_factor * dm + 1e-30;
+      for (int a = 0; a < h; ++a) A[a * h + a] += l2;
+      small_solve(h, A, b, g);
+      launch_update(P<double>(ctx->s_v1), P<double>(ctx->s_Gv), P<double>(ctx->s_dG), g, h, beta, n, ctx->stream);
+      CHECK_LAUNCH();
+    }
+    // rp <- r, Gp <- Gv (pointer rotation instead of copies)
+    std::swap(ctx->s_rp, ctx->s_r);
+    std::swap(ctx->s_Gp, ctx->s_Gv);
+  }
+  if (exit_reason == 2) {
+    std::swap(ctx->s_v1, ctx->s_best);
+    res_out = nrm_best;
+  }
+  // v <- v1 and the new projection / entropy / moments (main.jl:296-303)
+  std::swap(ctx->s_v, ctx->s_v1);
+  if ((r = project_state(ctx, p.use_logsq))) return r;
+  double sc[6];
+  HIPC(hipMemcpyAsync(sc, ctx->b_scal.p, 6 * 8, hipMemcpyDeviceToHost, ctx->stream));
+  HIPC(hipStreamSynchronize(ctx->stream));
+  prof_collect(ctx);
+  if (out) {
+    out->iters = k_done;
+    out->iters_f32 = iters32;
+    out->n_restart = n_restart;
+    out->exit_reason = exit_reason;
+    out->residual = res_out;
+    out->r0 = nrm_r0;
+    out->entropy = sc[0];
+    out->energy = 0.5 * sc[5];
+    out->momentum[0] = sc[3];
+    out->momentum[1] = sc[4];
+    out->seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+  }
+  return COLLOP_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int collop_state_set(collop_ctx* ctx, const double* v, const double* w, int64_t N) {
+  int r = need_mesh(ctx);
+  if (r) return r;
+  if ((r = check_N(ctx, N))) return r;
+  if (N < 2) return fail(ctx, COLLOP_ERR_INVALID, "the implicit step needs N >= 2 markers");
+  if (!v || !w) return fail(ctx, COLLOP_ERR_INVALID, "null array");
+  double wsum = 0.0;
+  for (int64_t i = 0; i < N; ++i) {
+    if (!(w[i] > 0.0) || !std::isfinite(w[i]))
+      return fail(ctx, COLLOP_ERR_INVALID, "weights must be positive and finite (G_eff divides by w)");
+    wsum += w[i];
+  }
+  HIPC(hipSetDevice(ctx->device));
+  if ((r = ensure_scratch(ctx, N))) return r;
+  const size_t n2b = (size_t)2 * N * 8;
+  for (DBuf* b : {&ctx->s_v, &ctx->s_v1, &ctx->s_Gv, &ctx->s_r, &ctx->s_rp, &ctx->s_Gp, &ctx->s_best, &ctx->s_dotv})
+    ENSURE(*b, n2b);
+  ENSURE(ctx->s_w, (size_t)N * 8);
+  ENSURE(ctx->s_dF, n2b * kMaxWindow);
+  ENSURE(ctx->s_dG, n2b * kMaxWindow);
+  ENSURE(ctx->s_fc, (size_t)ctx->mv.n_dofs * 8);
+  ctx->N = N;
+  ctx->w_abs_sum = wsum;
+  choose_tile(ctx, (int)N);
+  HIPC(hipMemcpyAsync(ctx->s_v.p, v, n2b, hipMemcpyHostToDevice, ctx->stream));
+  HIPC(hipMemcpyAsync(ctx->s_w.p, w, (size_t)N * 8, hipMemcpyHostToDevice, ctx->stream));
+  HIPC(hipMemsetAsync(ctx->s_rp.p, 0, n2b, ctx->stream));
+  HIPC(hipMemsetAsync(ctx->s_Gp.p, 0, n2b, ctx->stream));
+  if ((r = project_state(ctx, 0))) return r;
+  HIPC(hipStreamSynchronize(ctx->stream));
+  ctx->has_state = true;
+  return COLLOP_OK;
+}
+
+int collop_state_get(collop_ctx* ctx, double* v) {
+  if (!ctx) return COLLOP_ERR_INVALID;
+  if (!ctx->has_state) return fail(ctx, COLLOP_ERR_NO_STATE, "no device-resident state (collop_state_set)");
+  if (!v) return fail(ctx, COLLOP_ERR_INVALID, "null array");
+  HIPC(hipSetDevice(ctx->device));
+  HIPC(hipMemcpyAsync(v, ctx->s_v.p, (size_t)2 * ctx->N * 8, hipMemcpyDeviceToHost, ctx->stream));
+  HIPC(hipStreamSynchronize(ctx->stream));
+  return COLLOP_OK;
+}
+
+int collop_state_coeffs(collop_ctx* ctx, double* f_coeffs) {
+  if (!ctx) return COLLOP_ERR_INVALID;
+  if (!ctx->has_state) return fail(ctx, COLLOP_ERR_NO_STATE, "no device-resident state (collop_state_set)");
+  if (!f_coeffs) return fail(ctx, COLLOP_ERR_INVALID, "null array");
+  HIPC(hipSetDevice(ctx->device));
+  HIPC(hipMemcpyAsync(f_coeffs, ctx->s_fc.p, (size_t)ctx->mv.n_dofs * 8, hipMemcpyDeviceToHost, ctx->stream));
+  HIPC(hipStreamSynchronize(ctx->stream));
+  return COLLOP_OK;
+}
+
+int collop_run_steps(collop_ctx* ctx, const collop_step_params* params, int n_steps, collop_step_stats* stats) {
+  if (!ctx) return COLLOP_ERR_INVALID;
+  if (!ctx->has_state) return fail(ctx, COLLOP_ERR_NO_STATE, "no device-resident state (collop_state_set)");
+  int r = validate_params(ctx, params);
+  if (r) return r;
+  if (n_steps < 0) return fail(ctx, COLLOP_ERR_INVALID, "n_steps must be >= 0");
+  HIPC(hipSetDevice(ctx->device));
+  choose_tile(ctx, (int)ctx->N);
+  // the entropy of the current state uses the same integrand variant as the steps
+  if ((r = project_state(ctx, params->use_logsq))) return r;
+  for (int s = 0; s < n_steps; ++s) {
+    r = step_once(ctx, *params, stats ? &stats[s] : nullptr);
+    if (r) return r;
+  }
+  return COLLOP_OK;
+}
+
+int collop_step(collop_ctx* ctx, const collop_step_params* params, double* v_inout, const double* w, int64_t N,
+                collop_step_stats* stats) {
+  int r = collop_state_set(ctx, v_inout, w, N);
+  if (r) return r;
+  if ((r = collop_run_steps(ctx, params, 1, stats))) return r;
+  return collop_state_get(ctx, v_inout);
+}
+
+// ---- extras (not part of the reference's hook set) ----------------------
+// Pair-sum tile: gamma_per_lane in {1,2}, slices in [1, 64], newton in {1,2};
+// slices = 0 restores the automatic choice.
+int collop_set_pair_tile(collop_ctx* ctx, int gamma_per_lane, int slices, int newton) {
+  if (!ctx) return COLLOP_ERR_INVALID;
+  if (gamma_per_lane != 1 && gamma_per_lane != 2) return fail(ctx, COLLOP_ERR_INVALID, "gamma_per_lane must be 1 or 2");
+  if (newton != 1 && newton != 2) return fail(ctx, COLLOP_ERR_INVALID, "newton must be 1 or 2");
+  if (slices < 0 || slices > 64) return fail(ctx, COLLOP_ERR_INVALID, "slices must be in [0, 64]");
+  ctx->tile64.gamma_per_lane = ctx->tile32.gamma_per_lane = gamma_per_lane;
+  ctx->tile64.newton = newton;
+  ctx->tile_auto = slices == 0;
+  if (slices > 0) ctx->tile64.slices = ctx->tile32.slices = slices;
+  if (ctx->has_state) choose_tile(ctx, (int)ctx->N);
+  return COLLOP_OK;
+}
+
+// HIP-event timing of every pair-kernel launch (kind 0 = FP64, 1 = FP32c).
+int collop_profile(collop_ctx* ctx, int enable, double* ms_total, long* launches) {
+  if (!ctx) return COLLOP_ERR_INVALID;
+  if (ms_total) {
+    ms_total[0] = ctx->prof_ms[0];
+    ms_total[1] = ctx->prof_ms[1];
+  }
+  if (launches) {
+    launches[0] = ctx->prof_n[0];
+    launches[1] = ctx->prof_n[1];
+  }
+  ctx->profile = enable != 0;
+  ctx->prof_ms[0] = ctx->prof_ms[1] = 0.0;
+  ctx->prof_n[0] = ctx->prof_n[1] = 0;
+  ctx->ev_used = 0;
+  return COLLOP_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,88 @@
+// collop_launch.hpp -- host-side launchers of the gfx950 kernels.  Each .hip
+// translation unit owns its kernels; the C-ABI layer calls them through these.
+#pragma once
+#include "collop_dev.hpp"
+
+namespace collop {
+
+// ---- pair sum (pair.hip) ----------------------------------------------------
+struct PairTile {
+  int gamma_per_lane;  // 1 or 2 targets per lane
+  int slices;          // alpha slices (partials per target)
+  int newton;          // FP64 rsqrt refinement steps (1 or 2)
+};
+
+// Pack (v, G, w) into Src64/Src32 records.  Markers outside the open box get
+// w_masked = 0 and in_domain = 0 (functions.jl:212-213, :219-220).
+void launch_pack_src(const double* v, const double* G, const double* w, int N, double lo1, double hi1, double lo2,
+                     double hi2, Src64* s64, Src32* s32, hipStream_t st);
+// part[S][2N] <- slice sums of the FP64 pair sum
+void launch_pair_f64(const Src64* src, int N, const PairTile& t, double* part, hipStream_t st);
+// part[S][2N] <- slice sums of the FP32 pair sum (compensated, written in FP64)
+void launch_pair_f32c(const Src32* src, int N, const PairTile& t, double* part, hipStream_t st);
+// part[S][5N] <- slice sums of F (2 comps) and A (3 comps)
+void launch_pair_metric_f64(const Src64* src, int N, const PairTile& t, double* part, hipStream_t st);
+// out[ncomp*N] <- sum over S slices in fixed order
+void launch_slice_sum(const double* part, int S, int64_t len, double* out, hipStream_t st);
+
+// ---- FEM side (fem.hip) ------------------------------------------------------
+// Scatter: fields = 1: rhs partials of x (= vin); fields = 2: v_mid = (v0+vin)/2
+// and dv = vin - v0 are written, and rhs partials of v_mid (field 0) and vin
+// (field 1); grid_fields (<= fields) says how many of them are scattered.
+// part layout [field][blocks][n_dofs], 64-bit fixed point with scale
+// 2^62 / w_abs_sum (w_abs_sum = sum |w|).  When mom != nullptr the blocks of
+// field 0 also write partial sums (w v1, w v2, w |x|^2) of the scattered
+// positions to mom[blocks][3].
+int scatter_blocks(int N);
+void launch_scatter(const MeshView& m, const double* v0, const double* vin, const double* w, int N, int fields,
+                    int grid_fields, double w_abs_sum, double* vmid, double* dv, long long* part, double* mom,
+                    hipStream_t st);
+// c[field] = M^{-1} rhs[field]; rhs is the exact sum of the fixed-point partial
+// rows, or read from `direct[field * n_dofs]` when part == nullptr.  One
+// workgroup per field.
+void launch_reduce_solve(const MeshView& m, const long long* part, int nblocks, double w_abs_sum,
+                         const double* direct, int fields, double* out, hipStream_t st);
+// rhs (no solve) from the fixed-point partial rows of field 0
+void launch_rhs(const MeshView& m, const long long* part, int nblocks, double w_abs_sum, double* out,
+                hipStream_t st);
+// Quadrature pass 1: per element and Gauss point.  For field y: iota[y] (if
+// non-null) gets (1 + log f) w_q |e| (0 where f is below the floor) and spart[y]
+// (if non-null) partial sums of -f log f w_q |e|.  coeff[y] points at n_dofs.
+int quad_blocks(const MeshView& m);
+void launch_quad1(const MeshView& m, const double* c0, const double* c1, int fields, int use_logsq, double* iota,
+                  double* spart, hipStream_t st);
+// r_k = sum over cells / points of phi_k iota  (compute_r!, functions.jl:158-179)
+void launch_quad2(const MeshView& m, const double* iota, double* r, hipStream_t st);
+// out[i] = sum_{j<n} part[i * istride + j * jstride] in fixed order, i < count
+void launch_sum_rows(const double* part, int n, int count, int istride, int jstride, double* out, hipStream_t st);
+// Gather: G (mode 0), or dS = -w G plus partial (dv.dS, |dv|^2) (mode 1),
+// or clamped log-gradient (mode 2).
+int gather_blocks(int N);
+void launch_gather(const MeshView& m, const double* x, const double* coeff, const double* w, const double* dv,
+                   int N, int mode, double fs_floor, double g_max, double* out, double* dpart, hipStream_t st);
+
+// ---- solver side (step.hip) -------------------------------------------------
+// Gonzalez lambda from reduced partials, then G_eff and the pair-source pack.
+void launch_geff_pack(const double* vmid, const double* dS, const double* dv, const double* w, int N,
+                      const double* dpart, int nparts, const double* S1, const double* S0, int use_gonzalez,
+                      double lo1, double hi1, double lo2, double hi2, Src64* s64, Src32* s32, double* lambda_out,
+                      hipStream_t st);
+
+struct EpiArgs {
+  const double* part; int S;
+  const double* v0; const double* v1; double dt;
+  double* Gv; double* r; const double* rp; const double* Gp;
+  double* dF; double* dG; int col; int hist;
+  double* stat_part;  // [blocks][kStatWidth]: |r|^2, |v1|^2, |Gv-v0|^2, ATA row, ATr
+};
+int epi_blocks(int N);
+constexpr int kStatWidth = 3 + 2 * kMaxWindow;
+void launch_epilogue(const EpiArgs& a, int N, hipStream_t st);
+// v1 = beta (Gv - sum_j dG_j gamma_j) + (1 - beta) v1
+void launch_update(double* v1, const double* Gv, const double* dG, const double* gam, int hist, double beta, int N,
+                   hipStream_t st);
+// out = v + dt * (sum over slices of part)  (the Euler predictor, main.jl:245)
+void launch_euler(const double* part, int S, const double* v, double dt, int N, double* out, double* dotv,
+                  hipStream_t st);
+
+}  // namespace collop

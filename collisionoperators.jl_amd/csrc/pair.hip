@@ -1,0 +1,293 @@
+// pair.hip -- the O(N^2) Landau pair sum on gfx950 (kernel (a)).
+//
+//   dot_v_g = sum_{a != g, a in box, |d|^2 >= 1e-24} w_a U(d) (G_a - G_g),
+//   d = v_g - v_a,  U(d) = (I - d d^T / |d|^2) / |d|        (functions.jl:205-237)
+//
+// In 2D, U(d) g = c (d2, -d1) / |d|^3 with c = g1 d2 - g2 d1 (the 2D cross
+// product), so one pair costs two subtractions for d, a dot for |d|^2, one
+// rsqrt, two subtractions for g, one cross product and two multiply-adds.
+// The pair term is odd under (g <-> a): U(-d) = U(d) and g changes sign, so
+// the FP32 variant keeps each pair contribution exactly antisymmetric.
+//
+// Mapping to CDNA4: one target gamma (or two) per lane, held in VGPRs; the
+// source alpha is wave-uniform, so its 64-byte record is fetched with scalar
+// loads into SGPRs and broadcast to the 64 lanes for free -- no LDS traffic and
+// no barriers.  The alpha range is cut into S slices; workgroup id -> (slice =
+// id % S, target block = id / S), so with S a multiple of 8 every XCD (which
+// receives workgroups round-robin) streams only its own slices of the source
+// array through its private L2.  Slice partials are summed in a fixed order by
+// a second pass, so results are deterministic run to run.  No MFMA: this is
+// not a dense contraction.  FP64 1/sqrt is v_rsq_f64 refined by Newton steps
+// y <- y (3/2 - q y^2 / 2) (textbook Newton-Raphson on 1/y^2 = q).
+#include "collop_launch.hpp"
+
+namespace collop {
+namespace {
+
+constexpr int kThreads = 256;
+
+template <int NEWTON>
+__device__ __forceinline__ double rsqrt_f64(double q) {
+  double y = __builtin_amdgcn_rsq(q);
+  const double h = 0.5 * q;
+#pragma unroll
+  for (int i = 0; i < NEWTON; ++i) {
+    const double e = __builtin_fma(-h, y * y, 0.5);
+    y = __builtin_fma(y, e, y);
+  }
+  return y;
+}
+
+// One ordered pair (gamma in registers, alpha uniform).  Returns the weight
+// factor t = w_a / |d|^3 (0 for masked pairs) through `t_out` for the metric.
+template <int NEWTON>
+__device__ __forceinline__ void pair_term_f64(double vg1, double vg2, double Gg1, double Gg2, double av1, double av2,
+                                              double ag1, double ag2, double aw, double& acc1, double& acc2,
+                                              double& d1o, double& d2o, double& t_out) {
+  const double d1 = vg1 - av1;
+  const double d2 = vg2 - av2;
+  const double q = __builtin_fma(d2, d2, d1 * d1);
+  const double y = rsqrt_f64<NEWTON>(__builtin_fmax(q, 1e-24));
+  const double g1 = ag1 - Gg1;
+  const double g2 = ag2 - Gg2;
+  const double c = __builtin_fma(g1, d2, -(g2 * d1));
+  double t = (aw * y) * (y * y);
+  t = q < 1e-24 ? 0.0 : t;  // coincident-pair guard (functions.jl:224)
+  const double k = t * c;
+  acc1 = __builtin_fma(k, d2, acc1);
+  acc2 = __builtin_fma(-k, d1, acc2);
+  d1o = d1;
+  d2o = d2;
+  t_out = t;
+}
+
+template <int K, int NEWTON, bool METRIC>
+__global__ __launch_bounds__(kThreads) void k_pair_f64(const Src64* __restrict__ src, int N, int S, int chunk,
+                                                        double* __restrict__ part) {
+  const int bid = blockIdx.x;
+  const int slice = bid % S;
+  const int gblk = bid / S;
+  const int a0 = slice * chunk;
+  const int a1 = min(N, a0 + chunk);
+  double vg1[K], vg2[K], Gg1[K], Gg2[K], acc1[K], acc2[K], m11[K], m12[K], m22[K];
+  bool act[K];
+#pragma unroll
+  for (int k = 0; k < K; ++k) {
+    const int g = gblk * (kThreads * K) + k * kThreads + (int)threadIdx.x;
+    act[k] = false;
+    vg1[k] = vg2[k] = Gg1[k] = Gg2[k] = 0.0;
+    if (g < N) {
+      const Src64 s = src[g];
+      vg1[k] = s.v1;
+      vg2[k] = s.v2;
+      Gg1[k] = s.g1;
+      Gg2[k] = s.g2;
+      act[k] = s.in_domain != 0.0;
+    }
+    acc1[k] = acc2[k] = m11[k] = m12[k] = m22[k] = 0.0;
+  }
+#pragma unroll 2
+  for (int a = a0; a < a1; ++a) {
+    const double av1 = src[a].v1, av2 = src[a].v2, ag1 = src[a].g1, ag2 = src[a].g2, aw = src[a].w;
+#pragma unroll
+    for (int k = 0; k < K; ++k) {
+      double d1, d2, t;
+      pair_term_f64<NEWTON>(vg1[k], vg2[k], Gg1[k], Gg2[k], av1, av2, ag1, ag2, aw, acc1[k], acc2[k], d1, d2, t);
+      if (METRIC) {
+        // A += w U(d) = w (I |d|^2 - d d^T) / |d|^3  (functions.jl:278-281)
+        const double td2 = t * d2;
+        m11[k] = __builtin_fma(td2, d2, m11[k]);
+        m12[k] = __builtin_fma(-td2, d1, m12[k]);
+        const double td1 = t * d1;
+        m22[k] = __builtin_fma(td1, d1, m22[k]);
+      }
+    }
+  }
+  const size_t stride = (size_t)(METRIC ? 5 : 2) * (size_t)N;
+#pragma unroll
+  for (int k = 0; k < K; ++k) {
+    const int g = gblk * (kThreads * K) + k * kThreads + (int)threadIdx.x;
+    if (g < N) {
+      double* p = part + (size_t)slice * stride;
+      p[g] = act[k] ? acc1[k] : 0.0;
+      p[N + g] = act[k] ? acc2[k] : 0.0;
+      if (METRIC) {
+        p[2 * (size_t)N + g] = act[k] ? m11[k] : 0.0;
+        p[3 * (size_t)N + g] = act[k] ? m12[k] : 0.0;
+        p[4 * (size_t)N + g] = act[k] ? m22[k] : 0.0;
+      }
+    }
+  }
+}
+
+// FP32 pair math with blocked two-sum compensation: BLK terms are summed in
+// FP32, then folded into a (hi, lo) pair with Knuth's TwoSum, so the error of
+// the per-marker sum does not grow with N.
+__device__ __forceinline__ void two_sum_fold(float& hi, float& lo, float x) {
+  const float s = hi + x;
+  const float bp = s - hi;
+  const float e = (hi - (s - bp)) + (x - bp);
+  hi = s;
+  lo += e;
+}
+
+template <int K, int BLK>
+__global__ __launch_bounds__(kThreads) void k_pair_f32c(const Src32* __restrict__ src, int N, int S, int chunk,
+                                                         double* __restrict__ part) {
+  const int bid = blockIdx.x;
+  const int slice = bid % S;
+  const int gblk = bid / S;
+  const int a0 = slice * chunk;
+  const int a1 = min(N, a0 + chunk);
+  float vg1[K], vg2[K], Gg1[K], Gg2[K], b1[K], b2[K], h1[K], h2[K], l1[K], l2[K];
+  bool act[K];
+#pragma unroll
+  for (int k = 0; k < K; ++k) {
+    const int g = gblk * (kThreads * K) + k * kThreads + (int)threadIdx.x;
+    act[k] = false;
+    vg1[k] = vg2[k] = Gg1[k] = Gg2[k] = 0.0f;
+    if (g < N) {
+      const Src32 s = src[g];
+      vg1[k] = s.v1;
+      vg2[k] = s.v2;
+      Gg1[k] = s.g1;
+      Gg2[k] = s.g2;
+      act[k] = s.in_domain != 0.0f;
+    }
+    b1[k] = b2[k] = h1[k] = h2[k] = l1[k] = l2[k] = 0.0f;
+  }
+  for (int ab = a0; ab < a1; ab += BLK) {
+    const int ae = min(a1, ab + BLK);
+#pragma unroll 4
+    for (int a = ab; a < ae; ++a) {
+      const float av1 = src[a].v1, av2 = src[a].v2, ag1 = src[a].g1, ag2 = src[a].g2, aw = src[a].w;
+#pragma unroll
+      for (int k = 0; k < K; ++k) {
+        const float d1 = vg1[k] - av1;
+        const float d2 = vg2[k] - av2;
+        const float q = __builtin_fmaf(d2, d2, d1 * d1);
+        const float y = __builtin_amdgcn_rsqf(__builtin_fmaxf(q, 1e-12f));
+        const float g1 = ag1 - Gg1[k];
+        const float g2 = ag2 - Gg2[k];
+        const float c = __builtin_fmaf(g1, d2, -(g2 * d1));
+        float t = (aw * y) * (y * y);
+        t = q < 1e-12f ? 0.0f : t;  // FP32 guard of collision_gpu.jl:187
+        const float kk = t * c;
+        b1[k] = __builtin_fmaf(kk, d2, b1[k]);
+        b2[k] = __builtin_fmaf(-kk, d1, b2[k]);
+      }
+    }
+#pragma unroll
+    for (int k = 0; k < K; ++k) {
+      two_sum_fold(h1[k], l1[k], b1[k]);
+      two_sum_fold(h2[k], l2[k], b2[k]);
+      b1[k] = b2[k] = 0.0f;
+    }
+  }
+#pragma unroll
+  for (int k = 0; k < K; ++k) {
+    const int g = gblk * (kThreads * K) + k * kThreads + (int)threadIdx.x;
+    if (g < N) {
+      double* p = part + (size_t)slice * 2 * (size_t)N;
+      p[g] = act[k] ? (double)h1[k] + (double)l1[k] : 0.0;
+      p[N + g] = act[k] ? (double)h2[k] + (double)l2[k] : 0.0;
+    }
+  }
+}
+
+__global__ void k_pack(const double* __restrict__ v, const double* __restrict__ G, const double* __restrict__ w,
+                       int N, double lo1, double hi1, double lo2, double hi2, Src64* __restrict__ s64,
+                       Src32* __restrict__ s32) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= N) return;
+  const double a = v[i], b = v[N + i];
+  const bool in = in_box(a, b, lo1, hi1, lo2, hi2);
+  const double wm = in ? w[i] : 0.0;
+  if (s64) {
+    Src64 s;
+    s.v1 = a;
+    s.v2 = b;
+    s.g1 = G[i];
+    s.g2 = G[N + i];
+    s.w = wm;
+    s.in_domain = in ? 1.0 : 0.0;
+    s.pad0 = s.pad1 = 0.0;
+    s64[i] = s;
+  }
+  if (s32) {
+    Src32 s;
+    s.v1 = (float)a;
+    s.v2 = (float)b;
+    s.g1 = (float)G[i];
+    s.g2 = (float)G[N + i];
+    s.w = (float)wm;
+    s.in_domain = in ? 1.0f : 0.0f;
+    s.pad0 = s.pad1 = 0.0f;
+    s32[i] = s;
+  }
+}
+
+__global__ void k_slice_sum(const double* __restrict__ part, int S, int64_t len, double* __restrict__ out) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= len) return;
+  double s = 0.0;
+  for (int k = 0; k < S; ++k) s += part[(size_t)k * len + i];
+  out[i] = s;
+}
+
+int grid_for(int N, const PairTile& t) {
+  const int per_block = kThreads * t.gamma_per_lane;
+  return ((N + per_block - 1) / per_block) * t.slices;
+}
+
+}  // namespace
+
+void launch_pack_src(const double* v, const double* G, const double* w, int N, double lo1, double hi1, double lo2,
+                     double hi2, Src64* s64, Src32* s32, hipStream_t st) {
+  if (N <= 0) return;
+  hipLaunchKernelGGL(k_pack, dim3((N + 255) / 256), dim3(256), 0, st, v, G, w, N, lo1, hi1, lo2, hi2, s64, s32);
+}
+
+void launch_pair_f64(const Src64* src, int N, const PairTile& t, double* part, hipStream_t st) {
+  if (N <= 0) return;
+  const int chunk = (N + t.slices - 1) / t.slices;
+  const dim3 grid(grid_for(N, t)), block(kThreads);
+  if (t.gamma_per_lane == 2) {
+    if (t.newton == 1)
+      hipLaunchKernelGGL((k_pair_f64<2, 1, false>), grid, block, 0, st, src, N, t.slices, chunk, part);
+    else
+      hipLaunchKernelGGL((k_pair_f64<2, 2, false>), grid, block, 0, st, src, N, t.slices, chunk, part);
+  } else {
+    if (t.newton == 1)
+      hipLaunchKernelGGL((k_pair_f64<1, 1, false>), grid, block, 0, st, src, N, t.slices, chunk, part);
+    else
+      hipLaunchKernelGGL((k_pair_f64<1, 2, false>), grid, block, 0, st, src, N, t.slices, chunk, part);
+  }
+}
+
+void launch_pair_metric_f64(const Src64* src, int N, const PairTile& t, double* part, hipStream_t st) {
+  if (N <= 0) return;
+  const int chunk = (N + t.slices - 1) / t.slices;
+  const dim3 grid(grid_for(N, t)), block(kThreads);
+  if (t.gamma_per_lane == 2)
+    hipLaunchKernelGGL((k_pair_f64<2, 2, true>), grid, block, 0, st, src, N, t.slices, chunk, part);
+  else
+    hipLaunchKernelGGL((k_pair_f64<1, 2, true>), grid, block, 0, st, src, N, t.slices, chunk, part);
+}
+
+void launch_pair_f32c(const Src32* src, int N, const PairTile& t, double* part, hipStream_t st) {
+  if (N <= 0) return;
+  const int chunk = (N + t.slices - 1) / t.slices;
+  const dim3 grid(grid_for(N, t)), block(kThreads);
+  if (t.gamma_per_lane == 2)
+    hipLaunchKernelGGL((k_pair_f32c<2, 32>), grid, block, 0, st, src, N, t.slices, chunk, part);
+  else
+    hipLaunchKernelGGL((k_pair_f32c<1, 32>), grid, block, 0, st, src, N, t.slices, chunk, part);
+}
+
+void launch_slice_sum(const double* part, int S, int64_t len, double* out, hipStream_t st) {
+  if (len <= 0) return;
+  hipLaunchKernelGGL(k_slice_sum, dim3((unsigned)((len + 255) / 256)), dim3(256), 0, st, part, S, len, out);
+}
+
+}  // namespace collop

@@ -1,0 +1,191 @@
+/* collop.h -- C-ABI of the MI355X (gfx950) Landau collision library.
+ *
+ * The library replaces the CUDA.jl hot loops of junyixu/CollisionOperators.jl
+ * behind the reference's own swappable hooks (solver.jl:30-34).  Every entry
+ * point takes plain pointers and sizes; no Julia, torch or HIP types cross
+ * this boundary.
+ *
+ * Conventions
+ *   - Marker arrays are Julia's column-major N x 2 Float64 matrices:
+ *     v[0..N) = v1, v[N..2N) = v2 (the same for G, dot_v; A is N x 3).
+ *   - Host pointers belong to the caller and are not retained after a call.
+ *     The library owns the device buffers cached in the context.
+ *   - Calls are synchronous.  One context per caller thread; a context is not
+ *     thread-safe.  Several contexts may share one device.
+ *   - Every function returns a collop_status; on failure the message is in
+ *     collop_last_error(ctx).  The reference raises a Julia error in the same
+ *     places (e.g. P_DEG != 2 at projection_gpu.jl:189).
+ */
+#ifndef COLLOP_H
+#define COLLOP_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct collop_ctx collop_ctx;
+
+typedef enum {
+  COLLOP_OK = 0,
+  COLLOP_ERR_INVALID = 1,      /* bad argument or size                        */
+  COLLOP_ERR_UNSUPPORTED = 2,  /* e.g. p_deg != 2 (projection_gpu.jl:189)      */
+  COLLOP_ERR_HIP = 3,          /* HIP runtime error (message has the detail)  */
+  COLLOP_ERR_NO_MESH = 4,      /* collop_set_mesh not called                  */
+  COLLOP_ERR_NONFINITE = 5,    /* NaN/Inf reached the residual of a map       */
+  COLLOP_ERR_NO_STATE = 6,     /* device-resident call before collop_state_set */
+  COLLOP_ERR_NO_MODEL = 7      /* warm start requested without loaded weights  */
+} collop_status;
+
+/* ---- context ------------------------------------------------------------ */
+int collop_create(int device, collop_ctx** out);
+int collop_destroy(collop_ctx* ctx);
+const char* collop_last_error(const collop_ctx* ctx);
+/* Library version as MAJOR*10000 + MINOR*100 + PATCH. */
+int collop_version(void);
+
+/* ---- mesh (the Workspace fields the reference's GPU code reads) ----------
+ * Replaces the cached mesh tables of projection_gpu.jl:186-211 (bp1, bp2,
+ * ext1d_1/2, basis_start_1d_1/2, n_dofs_1d_1) and the mass-matrix LU of
+ * MantisWrappers.jl:107-120, which the library rebuilds as the Kronecker
+ * product M1 (x) M2 of banded 1D factors.
+ *   bp1[nb1], bp2[nb2]        breakpoints (strictly increasing)
+ *   C1[9*(nb1-1)], C2[...]    Bezier extraction, C[i + 3*j + 9*cell] with
+ *                             phi_j = sum_i C[i,j] B_i  (MantisWrappers.jl:237)
+ *   bs1[nb1-1], bs2[nb2-1]    1-based first basis index per cell
+ *   nd1, nd2                  1D basis counts (n_dofs = nd1*nd2)
+ *   p_deg                     must be 2 (the kernels are P_DEG=2 specialised)
+ *   n_quad                    Gauss-Legendre points per direction for the
+ *                             entropy / r quadrature (N_QUAD, Parameters.jl:27) */
+int collop_set_mesh(collop_ctx* ctx, const double* bp1, int nb1, const double* bp2, int nb2, const double* C1,
+                    const double* C2, const int32_t* bs1, const int32_t* bs2, int nd1, int nd2, int p_deg,
+                    int n_quad);
+
+/* ---- hook-level entry points (host arrays in / out) --------------------- */
+
+/* COLLISION_FN (solver.jl:30): the O(N^2) Landau pair sum of functions.jl:205-237
+ * on the GPU.  Replaces compute_collision_gpu! (collision_gpu.jl:258-280). */
+int collop_collision_f64(collop_ctx* ctx, const double* v, const double* w, const double* G, double* dot_v,
+                         int64_t N);
+
+/* COLLISION_FN with gpu_fp32=true: pair math in FP32 from FP32 copies of
+ * (v, G, w), per-marker sums with two-sum compensated accumulation, result
+ * returned in FP64.  Replaces compute_collision_gpu32! (collision_gpu.jl:233-256). */
+int collop_collision_f32c(collop_ctx* ctx, const double* v, const double* w, const double* G, double* dot_v,
+                          int64_t N);
+
+/* COLLMETRIC_FN (solver.jl:34): pair sum plus the metric A = sum w U(d)
+ * (A11, A12, A22), functions.jl:249-290.  Replaces
+ * compute_collision_metric_gpu! (collision_gpu.jl:377-402). */
+int collop_collision_metric_f64(collop_ctx* ctx, const double* v, const double* w, const double* G, double* dot_v,
+                                double* A, int64_t N);
+
+/* L2PROJ_FN, scatter only: rhs_k = sum_alpha w_alpha phi_k(v_alpha) over
+ * in-domain markers (functions.jl:121-131; _l2_scatter_kernel!,
+ * projection_gpu.jl:47-71).  A caller wanting bitwise host-solve parity
+ * applies its own M_lu \ rhs. */
+int collop_l2_rhs(collop_ctx* ctx, const double* v, const double* w, int64_t N, double* rhs);
+
+/* L2PROJ_FN: scatter plus the mass solve on the device, f = M^{-1} rhs.
+ * Replaces l2_project_gpu! (projection_gpu.jl:213-226). */
+int collop_l2_project(collop_ctx* ctx, const double* v, const double* w, int64_t N, double* f_coeffs);
+
+/* COMPG_FN (solver.jl:32): G_alpha = grad(sum_k L_k phi_k)(v_alpha), zero out of
+ * domain (functions.jl:182-202).  Replaces compute_G_gpu! (projection_gpu.jl:228-246). */
+int collop_grad_gather(collop_ctx* ctx, const double* v, int64_t N, const double* L, double* G);
+
+/* LOGGRAD_FN (solver.jl:33): g = clamp(grad f / max(|f|, fs_floor), +-g_max),
+ * zero out of domain (functions.jl:37-64).  Replaces eval_loggrad_gpu!
+ * (projection_gpu.jl:251-269); the reference passes fs_floor=1e-30, g_max=100. */
+int collop_loggrad_gather(collop_ctx* ctx, const double* v, int64_t N, const double* f_coeffs, double fs_floor,
+                          double g_max, double* g);
+
+/* compute_r! and compute_entropy (functions.jl:137-179) on the device: the
+ * N_QUAD^2-point element quadrature the reference runs on the CPU for every
+ * map.  Either output pointer may be NULL. */
+int collop_quadrature(collop_ctx* ctx, const double* f_coeffs, int use_logsq, double* r, double* entropy);
+
+/* x = M^{-1} b (the reference's ws.M_lu \ b, functions.jl:132, solver.jl:67). */
+int collop_mass_solve(collop_ctx* ctx, const double* b, double* x);
+
+/* ---- device-resident implicit step ---------------------------------------
+ * The step of main.jl:226-307 (Landau branch, solver = :anderson) with the
+ * Picard map of solver.jl:176-227 and the Anderson solve of solver.jl:300-423
+ * running on the device; the host only reads a handful of reduced scalars per
+ * Anderson iteration. */
+
+typedef enum {
+  COLLOP_PREC_F64 = 0,      /* every map in FP64 (the plain FP64 GPU path)     */
+  COLLOP_PREC_F32C = 1      /* FP32-compensated pair sum for the early maps,
+                               FP64 maps to finish; last map always FP64      */
+} collop_precision;
+
+typedef enum {
+  COLLOP_WARM_EULER = 0,    /* explicit Euler predictor (main.jl:239-245)       */
+  COLLOP_WARM_MLP = 1       /* Euler + per-marker MLP correction (main.jl:252-254) */
+} collop_warmstart;
+
+typedef struct {
+  double dt;                /* DT                                               */
+  int32_t m_anderson;       /* window m (<= 8)                                  */
+  int32_t max_iter;
+  double tol;               /* relative tol on ||r|| / ||v||                     */
+  double abs_floor;
+  int32_t stag_window;
+  double stag_rel_tol;
+  int32_t damp_decay_start;
+  double damp_decay_factor;
+  double restart_factor;    /* +inf = never (the reference default)             */
+  double damping;           /* beta                                             */
+  double reg_factor;        /* 1e-10 in the reference                           */
+  int32_t use_anderson;
+  int32_t use_gonzalez;
+  int32_t exit_picard_step;
+  int32_t use_logsq;
+  int32_t precision;        /* collop_precision                                 */
+  int32_t warmstart;        /* collop_warmstart                                 */
+  double f32_floor_factor;  /* F32C: FP32 maps run until ||r|| < factor * 2^-24
+                               * ||Gv - v0|| (0 = 16), then FP64 maps finish    */
+} collop_step_params;
+
+typedef struct {
+  int32_t iters;            /* Picard-map evaluations (the reference's `iter`)   */
+  int32_t iters_f32;        /* of which FP32-compensated maps                    */
+  int32_t n_restart;
+  int32_t exit_reason;      /* 0 converged, 1 stagnation, 2 max_iter            */
+  double residual;          /* ||r|| at exit (the reference's `res_final`)       */
+  double r0;                /* ||r|| of the first map (`r0_init`)               */
+  double entropy;           /* S_h of the new state                            */
+  double energy;            /* E = 1/2 sum w |v|^2                              */
+  double momentum[2];       /* P = sum w v                                     */
+  double seconds;           /* wall time of the step                           */
+} collop_step_stats;
+
+/* Upload markers (v: N x 2 column-major, w: N) and project them; keeps S0. */
+int collop_state_set(collop_ctx* ctx, const double* v, const double* w, int64_t N);
+/* Copy the current markers back (N x 2). */
+int collop_state_get(collop_ctx* ctx, double* v);
+/* Current projection coefficients f_s (n_dofs). */
+int collop_state_coeffs(collop_ctx* ctx, double* f_coeffs);
+/* Advance n_steps implicit steps on the device; stats[n_steps] optional. */
+int collop_run_steps(collop_ctx* ctx, const collop_step_params* params, int n_steps, collop_step_stats* stats);
+/* One step on host arrays: state_set + run_steps(1) + state_get. */
+int collop_step(collop_ctx* ctx, const collop_step_params* params, double* v_inout, const double* w, int64_t N,
+                collop_step_stats* stats);
+
+/* ---- tuning and measurement (no counterpart in the reference) -----------
+ * Pair-sum tile: gamma_per_lane in {1, 2} targets per lane, slices in [1, 64]
+ * source slices (0 = automatic), newton in {1, 2} FP64 rsqrt refinements. */
+int collop_set_pair_tile(collop_ctx* ctx, int gamma_per_lane, int slices, int newton);
+/* HIP-event timing of every pair-kernel launch on the context's stream.
+ * Returns the totals accumulated since the previous call (ms_total[2],
+ * launches[2]; index 0 = FP64, 1 = FP32c), then resets them and sets the
+ * enable flag. */
+int collop_profile(collop_ctx* ctx, int enable, double* ms_total, long* launches);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* COLLOP_H */

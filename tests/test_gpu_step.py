@@ -1,0 +1,141 @@
+"""The device-resident implicit step against the oracle's step (main.jl:226-307
+with step_anderson!, solver.jl:300-423), and the invariants the scheme
+guarantees at full size: momentum and energy conserved to roundoff, entropy
+non-decreasing (Gonzalez discrete gradient).
+"""
+import numpy as np
+import pytest
+
+import collop
+import oracle as O
+from conftest import relerr, uniform_bp
+
+pytestmark = pytest.mark.gpu
+
+
+def _params(dt, precision=collop.PREC_F64, **kw):
+    p = collop.config("gaussian_n4096", DT=dt, **kw)
+    return p.step_params(precision=precision)
+
+
+def _oracle_residual(m, P, v_new, v, w, S0):
+    """||map(v_new) - v_new|| for the oracle's Picard map: the returned state is
+    G(v*) for an iterate v* with ||G(v*) - v*|| < tol ||v*||, so it is itself a
+    root of the oracle's implicit system to that level."""
+    return float(np.linalg.norm(m.picard_map(P, v_new, v, w, S0) - v_new))
+
+
+# (ic, N, cells, dt, strict): with `strict` the step is well conditioned (one
+# root near the predictor) and the iterates must agree; otherwise the system can
+# have several roots a few 1e-3 apart and roundoff may pick another one, so only
+# "is a root of the oracle's system" is required.
+@pytest.mark.parametrize("ic,N,ne,dt,strict", [("gaussian", 1024, 16, 0.005, True),
+                                               ("bimodal", 2048, 32, 0.02, True),
+                                               ("gaussian", 1024, 16, 0.05, False)])
+def test_one_step_matches_oracle(gpu_ctx, ic, N, ne, dt, strict):
+    sig = (1.0, 1.0) if ic == "gaussian" else (np.sqrt(0.5), np.sqrt(0.5))
+    v, w = collop.markers(ic, N, 42, sigma1=sig[0], sigma2=sig[1], peak=2.0)
+    bp = uniform_bp(-6, 6, ne)
+    m = O.Mesh(bp, bp)
+    fc = m.l2_project(v, w)
+    S0 = m.entropy(fc)
+    P = O.default_params(dt, m=5, nthreads=0)
+    v_ref, fc_ref, st_ref = m.time_step(P, v, w, fc, S0)
+
+    ws = collop.Workspace(bp, bp, ctx=gpu_ctx)
+    sim = collop.Simulation(ws, v, w)
+    st = sim.run(_params(dt), 1)[0]
+    v_gpu = sim.markers()
+    res = _oracle_residual(m, P, v_gpu, v, w, S0)
+    tol = max(1e-12 * np.linalg.norm(v_gpu), 1e-10)
+    print(f"{ic} N={N} dt={dt}: oracle iters {st_ref.iters} exit {st_ref.exit_reason}; gpu iters {st['iters']} "
+          f"exit {st['exit_reason']}; max|dv| {np.max(np.abs(v_gpu - v_ref)):.2e}; oracle-map residual of the "
+          f"gpu state {res:.2e} (tol {tol:.2e})")
+    assert st["exit_reason"] == 0 and st_ref.exit_reason == 0
+    assert res <= 20 * tol
+    (p1, p2), E0 = O.momentum_energy(v, w)
+    assert abs(st["energy"] - E0) <= 1e-12 * E0
+    assert abs(st["momentum"][0] - p1) <= 1e-13 and abs(st["momentum"][1] - p2) <= 1e-13
+    assert st["entropy"] >= S0
+    if strict:
+        assert abs(st["iters"] - st_ref.iters) <= 2
+        assert np.max(np.abs(v_gpu - v_ref)) <= 1e-9
+        assert abs(st["entropy"] - st_ref.entropy) <= 1e-10
+        assert relerr(sim.coeffs(), fc_ref) <= 1e-9
+
+
+@pytest.mark.parametrize("precision", [collop.PREC_F64, collop.PREC_F32C])
+def test_conservation_bimodal_n65536(gpu_ctx, precision):
+    p = collop.config("bimodal_n65536")
+    v, w = collop.initial_markers(p)
+    ws = collop.build_workspace(p, ctx=gpu_ctx)
+    sim = collop.Simulation(ws, v, w)
+    E0 = 0.5 * np.sum(w * (v[:, 0] ** 2 + v[:, 1] ** 2))
+    P0 = (w[:, None] * v).sum(axis=0)
+    fc = sim.coeffs()
+    S_prev, _ = collop.compute_r_and_entropy_gpu(ws, fc)[1], None
+    E_prev, P_prev = E0, P0
+    stats = sim.run(p.step_params(precision=precision), 3)
+    for s in stats:
+        print({k: s[k] for k in ("iters", "iters_f32", "exit_reason", "residual", "entropy", "energy", "seconds")})
+        assert s["exit_reason"] == 0
+        assert abs(s["energy"] - E_prev) / E0 <= 1e-10
+        assert np.all(np.abs(np.array(s["momentum"]) - P_prev) <= 1e-10)
+        assert s["entropy"] >= S_prev - 1e-12
+        S_prev, E_prev, P_prev = s["entropy"], s["energy"], np.array(s["momentum"])
+
+
+@pytest.mark.parametrize("dt,strict", [(0.005, True), (0.02, False)])
+def test_f32c_reaches_fp64_solution(gpu_ctx, dt, strict):
+    """The FP32-compensated path solves the same FP64 system: where the step is
+    well conditioned it lands on the FP64 path's state; at dt = 0.02, where the
+    system has several roots, it lands on a root of the oracle's system."""
+    p = collop.config("bimodal_n65536", N_PARTICLES=8192, DT=dt)
+    v, w = collop.initial_markers(p)
+    ws = collop.build_workspace(p, ctx=gpu_ctx)
+    a = collop.Simulation(ws, v, w)
+    sa = a.run(p.step_params(precision=collop.PREC_F64), 1)
+    va = a.markers()
+    b = collop.Simulation(ws, v, w)
+    sb = b.run(p.step_params(precision=collop.PREC_F32C), 1)
+    vb = b.markers()
+    print(f"dt={dt}: fp64 iters {sa[0]['iters']}, f32c iters {sb[0]['iters']} (f32 {sb[0]['iters_f32']}), "
+          f"max|dv| {np.max(np.abs(va - vb)):.2e}")
+    assert sa[0]["exit_reason"] == 0 and sb[0]["exit_reason"] == 0
+    assert sb[0]["iters_f32"] >= 1
+    if strict:
+        assert np.max(np.abs(va - vb)) <= 2e-9
+    m = O.Mesh(p.bp1, p.bp2)
+    P = O.default_params(dt, m=5, nthreads=0)
+    S0 = m.entropy(m.l2_project(v, w))
+    assert _oracle_residual(m, P, vb, v, w, S0) <= 20 * max(1e-12 * np.linalg.norm(vb), 1e-10)
+
+
+def test_step_deterministic(gpu_ctx):
+    p = collop.config("gaussian_n4096", N_PARTICLES=2048)
+    v, w = collop.initial_markers(p)
+    ws = collop.build_workspace(p, ctx=gpu_ctx)
+    out = []
+    for _ in range(2):
+        sim = collop.Simulation(ws, v, w)
+        sim.run(p.step_params(), 1)
+        out.append(sim.markers())
+    np.testing.assert_array_equal(out[0], out[1])
+
+
+def test_state_errors(gpu_ctx):
+    bp = uniform_bp(-6, 6, 16)
+    ws = collop.Workspace(bp, bp, ctx=gpu_ctx)
+    v, w = collop.markers("gaussian", 64, 1)
+    with pytest.raises(collop.CollopError):
+        collop.Simulation(ws, v, -w)
+    sim = collop.Simulation(ws, v, w)
+    bad = collop.config("gaussian_n4096").step_params()
+    bad.m_anderson = 9
+    with pytest.raises(collop.CollopError):
+        sim.run(bad, 1)
+    bad = collop.config("gaussian_n4096").step_params()
+    bad.warmstart = collop.WARM_MLP
+    with pytest.raises(collop.CollopError) as ei:
+        sim.run(bad, 1)
+    assert ei.value.status == collop._lib.ERR_NO_MODEL

@@ -1,0 +1,123 @@
+"""Pins the oracle's implicit step (solver.jl:62-423, main.jl:226-307) by the
+properties the scheme guarantees: the Gonzalez discrete chain rule holds to
+roundoff on every map, a map conserves momentum and energy at the midpoint,
+Anderson and plain damped Picard reach the same fixed point, the step
+conserves momentum/energy and does not decrease entropy, and the solver's
+exits (convergence, stagnation, iteration cap) behave as written.
+"""
+import numpy as np
+import pytest
+
+import collop
+import oracle as O
+from conftest import uniform_bp
+
+
+@pytest.fixture(scope="module")
+def small():
+    p = collop.config("gaussian_n4096", N_PARTICLES=600)
+    v, w = collop.initial_markers(p)
+    m = O.Mesh(p.bp1, p.bp2)
+    fc = m.l2_project(v, w)
+    S0 = m.entropy(fc)
+    P = O.default_params(p.DT, m=5, nthreads=4)
+    v1, _ = m.predictor(P, v, w, fc)
+    return dict(p=p, v=v, w=w, m=m, fc=fc, S0=S0, P=P, v1=v1)
+
+
+def test_gonzalez_discrete_chain_rule(small):
+    m, v, w, S0 = small["m"], small["v"], small["w"], small["S0"]
+    vin = small["v1"]
+    Geff, vmid, lam = m.landau_geff(vin, v, w, S0)
+    S1 = m.entropy(m.l2_project(vin, w))
+    dv = vin - v
+    # dbar S . dv = S(v_in) - S0 with dbar S = -w G_eff  (solver.jl:137-149)
+    lhs = np.sum(-w[:, None] * Geff * dv)
+    assert abs(lhs - (S1 - S0)) < 1e-13 * max(1.0, abs(S1 - S0)) + 1e-15
+    assert lam != 0.0
+    Gm, _, lam0 = m.landau_geff(vin, v, w, S0, use_gonzalez=False)
+    assert lam0 == 0.0
+    np.testing.assert_allclose(Gm - lam * dv / w[:, None], Geff, rtol=0, atol=1e-12 * np.abs(Geff).max())
+    np.testing.assert_allclose(vmid, 0.5 * (v + vin), rtol=0, atol=0)
+
+
+def test_picard_map_conserves_at_midpoint(small):
+    m, v, w, S0, P = small["m"], small["v"], small["w"], small["S0"], small["P"]
+    vin = small["v1"]
+    vout = m.picard_map(P, vin, v, w, S0)
+    inc = vout - v
+    mid = 0.5 * (v + vin)
+    s = np.abs(w[:, None] * inc).sum()
+    assert np.all(np.abs((w[:, None] * inc).sum(axis=0)) < 1e-14 * s)
+    assert abs(np.sum(w[:, None] * mid * inc)) < 1e-14 * s * np.abs(mid).max()
+
+
+def test_anderson_matches_picard_fixed_point():
+    # At dt = 0.005 the implicit system has one root near the predictor; at
+    # dt >= 0.01 damped Picard and Anderson can settle on different roots a
+    # few 1e-3 apart (both with residual < tol), so the comparison uses 0.005.
+    p = collop.config("gaussian_n4096", N_PARTICLES=600, DT=0.005)
+    v, w = collop.initial_markers(p)
+    m = O.Mesh(p.bp1, p.bp2)
+    fc = m.l2_project(v, w)
+    S0 = m.entropy(fc)
+    P = O.default_params(p.DT, m=5, nthreads=4)
+    v1, _ = m.predictor(P, v, w, fc)
+    va, sa, ra = m.step_anderson(P, v1, v, w, S0)
+    Pp = O.default_params(p.DT, m=5, nthreads=4, use_anderson=0, damping=0.7)
+    vp, sp, rp = m.step_anderson(Pp, v1, v, w, S0)
+    assert sa.exit_reason == 0 and sp.exit_reason == 0
+    assert np.max(np.abs(va - vp)) < 1e-9
+    # the residual trace ends below the effective tolerance
+    assert ra[-1] < max(1e-12 * np.linalg.norm(va), 1e-10) * 1.0001
+
+
+def test_time_step_invariants(small):
+    m, v, w, fc, S0 = small["m"], small["v"], small["w"], small["fc"], small["S0"]
+    P = small["P"]
+    (p1, p2), E0 = O.momentum_energy(v, w)
+    vv, ff, S = v, fc, S0
+    for _ in range(3):
+        vv, ff, st = m.time_step(P, vv, w, ff, S)
+        assert st.exit_reason == 0
+        assert abs(st.energy - E0) < 1e-12 * E0
+        assert abs(st.p1 - p1) < 1e-14 and abs(st.p2 - p2) < 1e-14
+        assert st.entropy >= S - 1e-14
+        S = st.entropy
+
+
+def test_exits(small):
+    m, v, w, S0 = small["m"], small["v"], small["w"], small["S0"]
+    # iteration cap: returns the best iterate, reason 2
+    P = O.default_params(small["p"].DT, m=5, nthreads=4, max_iter=3)
+    out, st, res = m.step_anderson(P, small["v1"], v, w, S0)
+    assert st.exit_reason == 2 and st.iters == 3
+    assert st.residual == pytest.approx(min(res))
+    # stagnation: an unreachable tolerance with a short window
+    P = O.default_params(small["p"].DT, m=5, nthreads=4, tol=0.0, abs_floor=0.0, stag_window=5, stag_rel_tol=0.5)
+    out, st, res = m.step_anderson(P, small["v1"], v, w, S0)
+    assert st.exit_reason == 1 and st.iters % 5 == 0
+
+
+def test_ring_window_permutation_invariance():
+    """The Anderson least-squares step is invariant under a common column
+    permutation of dF and dG (the ring cursor of solver.jl:386-394)."""
+    rng = np.random.default_rng(9)
+    n, h = 50, 4
+    dF = rng.standard_normal((n, h))
+    dG = rng.standard_normal((n, h))
+    r = rng.standard_normal(n)
+
+    def step(F, Gm):
+        A = F.T @ F
+        A += (1e-10 * np.mean(np.diag(A)) + 1e-30) * np.eye(h)
+        return Gm @ np.linalg.solve(A, F.T @ r)
+
+    perm = [2, 0, 3, 1]
+    np.testing.assert_allclose(step(dF, dG), step(dF[:, perm], dG[:, perm]), rtol=1e-12)
+
+
+def test_uniform_breakpoints_exact():
+    for n in (16, 32, 64):
+        bp = collop.config("bimodal_n65536", bp1=uniform_bp(-6, 6, n)).bp1
+        assert np.all(np.diff(bp) == bp[1] - bp[0])
