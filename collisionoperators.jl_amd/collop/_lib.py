@@ -61,6 +61,8 @@ class StepParams(C.Structure):
         ("precision", C.c_int32),
         ("warmstart", C.c_int32),
         ("f32_floor_factor", C.c_double),
+        ("f32_keep_window", C.c_int32),
+        ("record_training", C.c_int32),
     ]
 
 
@@ -90,6 +92,7 @@ class StepStats(C.Structure):
 
 
 _dp = C.POINTER(C.c_double)
+_fp = C.POINTER(C.c_float)
 _i32p = C.POINTER(C.c_int32)
 _vp = C.c_void_p
 
@@ -115,6 +118,14 @@ SIGNATURES = [
     ("collop_state_coeffs", C.c_int, [_vp, _dp]),
     ("collop_run_steps", C.c_int, [_vp, C.POINTER(StepParams), C.c_int, C.POINTER(StepStats)]),
     ("collop_step", C.c_int, [_vp, C.POINTER(StepParams), _dp, _dp, C.c_int64, C.POINTER(StepStats)]),
+    ("collop_mlp_set_params", C.c_int, [_vp, _fp, C.c_int]),
+    ("collop_mlp_get_params", C.c_int, [_vp, _fp, C.c_int]),
+    ("collop_mlp_set_norm", C.c_int, [_vp, _fp, _fp, _fp]),
+    ("collop_mlp_dataset_alloc", C.c_int, [_vp, C.c_int64]),
+    ("collop_mlp_dataset_size", C.c_int, [_vp, C.POINTER(C.c_int64)]),
+    ("collop_mlp_dataset_read", C.c_int, [_vp, C.c_int64, C.c_int64, _fp]),
+    ("collop_mlp_train", C.c_int, [_vp, C.c_int, C.c_int, C.c_double, C.c_uint64, _dp]),
+    ("collop_mlp_eval", C.c_int, [_vp, _fp, C.c_int64, C.c_int, _fp, _dp, _fp]),
     ("collop_set_pair_tile", C.c_int, [_vp, C.c_int, C.c_int, C.c_int]),
     ("collop_profile", C.c_int, [_vp, C.c_int, _dp, C.POINTER(C.c_long)]),
 ]
@@ -143,6 +154,11 @@ def load(path: str | None = None):
 def dptr(a: np.ndarray):
     assert a.dtype == np.float64
     return a.ctypes.data_as(_dp)
+
+
+def fptr(a: np.ndarray):
+    assert a.dtype == np.float32
+    return a.ctypes.data_as(_fp)
 
 
 def iptr(a: np.ndarray):

@@ -63,7 +63,7 @@ class SimParameters:
             restart_factor=math.inf, damping=self.damping, reg_factor=1e-10,
             use_anderson=int(self.use_anderson), use_gonzalez=int(self.use_gonzalez),
             exit_picard_step=int(self.exit_picard_step), use_logsq=int(self.use_logsq),
-            precision=prec, warmstart=warm, f32_floor_factor=16.0)
+            precision=prec, warmstart=warm, f32_floor_factor=16.0, f32_keep_window=1, record_training=0)
 
 
 def parse_overrides(p: SimParameters, args: list[str]) -> SimParameters:

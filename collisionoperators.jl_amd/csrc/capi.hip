@@ -51,6 +51,12 @@ struct collop_ctx {
   double w_abs_sum = 0.0;  // sum |w| of the state (fixed-point scale of the scatter)
   DBuf s_v, s_w, s_v1, s_Gv, s_r, s_rp, s_Gp, s_best, s_dF, s_dG, s_fc, s_dotv;
   double* h_stat = nullptr;  // pinned, kStatWidth doubles
+  // ---- warm-start MLP
+  DBuf m_params, m_adam_m, m_adam_v, m_data, m_rec, m_y, m_gpart, m_lpart, m_tiles, m_eval, m_grad;
+  bool m_has_params = false, m_has_norm = false;
+  float m_mu[6] = {0}, m_isd[6] = {0}, m_oscale[2] = {1, 1};
+  int64_t m_cap = 0, m_size = 0;
+  long m_adam_step = 0;
   // ---- pair tile and profiling
   PairTile tile64{1, 8, 2};
   PairTile tile32{1, 8, 0};
@@ -359,7 +365,8 @@ int collop_destroy(collop_ctx* ctx) {
                   &ctx->b_s32, &ctx->b_part, &ctx->b_spart_sc, &ctx->b_mom, &ctx->b_dpart, &ctx->b_iota,
                   &ctx->b_spart_q, &ctx->b_vec, &ctx->b_scal, &ctx->b_statp, &ctx->b_stat, &ctx->s_v, &ctx->s_w,
                   &ctx->s_v1, &ctx->s_Gv, &ctx->s_r, &ctx->s_rp, &ctx->s_Gp, &ctx->s_best, &ctx->s_dF, &ctx->s_dG,
-                  &ctx->s_fc, &ctx->s_dotv})
+                  &ctx->s_fc, &ctx->s_dotv, &ctx->m_params, &ctx->m_adam_m, &ctx->m_adam_v, &ctx->m_data, &ctx->m_rec,
+                  &ctx->m_y, &ctx->m_gpart, &ctx->m_lpart, &ctx->m_tiles, &ctx->m_eval, &ctx->m_grad})
     release(*b);
   for (auto& e : ctx->ev_pool) {
     (void)hipEventDestroy(e.first);
@@ -843,8 +850,11 @@ int validate_params(collop_ctx* ctx, const collop_step_params* p) {
     return fail(ctx, COLLOP_ERR_INVALID, "unknown precision");
   if (p->warmstart != COLLOP_WARM_EULER && p->warmstart != COLLOP_WARM_MLP)
     return fail(ctx, COLLOP_ERR_INVALID, "unknown warmstart");
-  if (p->warmstart == COLLOP_WARM_MLP)
-    return fail(ctx, COLLOP_ERR_NO_MODEL, "no warm-start model loaded on this context");
+  if (p->warmstart == COLLOP_WARM_MLP && !(ctx->m_has_params && ctx->m_has_norm))
+    return fail(ctx, COLLOP_ERR_NO_MODEL, "no warm-start model loaded on this context (collop_mlp_set_params / "
+                                          "collop_mlp_set_norm)");
+  if (p->record_training && ctx->m_size + ctx->N > ctx->m_cap)
+    return fail(ctx, COLLOP_ERR_INVALID, "training dataset full (collop_mlp_dataset_alloc)");
   return COLLOP_OK;
 }
 
@@ -865,6 +875,45 @@ int step_once(collop_ctx* ctx, const collop_step_params& p, collop_step_stats* o
   launch_euler(P<double>(ctx->b_part), ctx->tile64.slices, P<double>(ctx->s_v), p.dt, n, P<double>(ctx->s_v1),
                P<double>(ctx->s_dotv), ctx->stream);
   CHECK_LAUNCH();
+  // ---- warm-start hook (main.jl:249-254): features of v, optional record,
+  // optional MLP correction of the predictor
+  float* rec = nullptr;
+  if (p.record_training) {
+    if (ctx->m_size + n > ctx->m_cap) return fail(ctx, COLLOP_ERR_INVALID, "training dataset full");
+    rec = P<float>(ctx->m_data) + (size_t)ctx->m_size * 8;
+  } else if (p.warmstart == COLLOP_WARM_MLP) {
+    ENSURE(ctx->m_rec, (size_t)n * 8 * sizeof(float));
+    rec = P<float>(ctx->m_rec);
+  }
+  if (rec) {
+    launch_gather(m, P<double>(ctx->s_v), P<double>(ctx->s_fc), nullptr, nullptr, n, 3, 0.0, 0.0,
+                  P<double>(ctx->b_out), nullptr, ctx->stream);
+    CHECK_LAUNCH();
+    launch_features(P<double>(ctx->s_v), P<double>(ctx->b_G), P<double>(ctx->b_out), n, rec, ctx->stream);
+    CHECK_LAUNCH();
+  }
+  if (p.warmstart == COLLOP_WARM_MLP) {
+    ENSURE(ctx->m_y, (size_t)n * 2 * sizeof(float));
+    MlpArgs a{};
+    a.params = P<float>(ctx->m_params);
+    a.rec = rec;
+    a.nsamples = n;
+    a.tiles = nullptr;
+    a.ntiles = (n + 15) / 16;
+    for (int j = 0; j < 6; ++j) {
+      a.mu[j] = ctx->m_mu[j];
+      a.isd[j] = ctx->m_isd[j];
+    }
+    a.oscale[0] = ctx->m_oscale[0];
+    a.oscale[1] = ctx->m_oscale[1];
+    a.mode = 0;
+    a.y = P<float>(ctx->m_y);
+    launch_mlp(a, std::min(a.ntiles, 2048), ctx->stream);
+    CHECK_LAUNCH();
+    launch_warm_apply(P<double>(ctx->s_v), P<double>(ctx->s_dotv), P<float>(ctx->m_y), ctx->m_oscale[0],
+                      ctx->m_oscale[1], p.dt, n, P<double>(ctx->s_v1), ctx->stream);
+    CHECK_LAUNCH();
+  }
 
   // ---- Anderson (solver.jl:300-423)
   const int mw = p.m_anderson;
@@ -883,10 +932,11 @@ int step_once(collop_ctx* ctx, const collop_step_params& p, collop_step_stats* o
   double best32 = std::numeric_limits<double>::infinity();
   int since_best32 = 0;
   const bool trace = std::getenv("COLLOP_TRACE") != nullptr;
+  bool rebase = false;  // this map re-evaluates the last FP32 iterate in FP64
   for (int k = 1; k <= p.max_iter; ++k) {
-    const bool anderson_step = p.use_anderson && since_reset > 0;
+    const bool anderson_step = p.use_anderson && since_reset > 0 && !rebase;
     const int col = anderson_step ? slot % mw : -1;
-    const int hist_new = anderson_step ? std::min(history + 1, mw) : 0;
+    const int hist_new = anderson_step ? std::min(history + 1, mw) : (rebase ? history : 0);
     const int kind = use32 ? 1 : 0;
     if ((r = picard_map_dev(ctx, p, kind, P<double>(ctx->s_v1), col, hist_new))) return r;
     since_reset++;
@@ -900,11 +950,11 @@ int step_once(collop_ctx* ctx, const collop_step_params& p, collop_step_stats* o
     if (use32) {
       iters32++;
       // FP32 maps run until their own rounding floor (a few FP32 ulps of the
-      // step increment |Gv - v0|), or until six maps bring no new best
-      // residual; the solve then restarts its window on FP64 maps, so every
-      // accepted iterate and the exit map are FP64.  Switching well inside the
-      // basin keeps the FP64 maps on the root the plain FP64 solve finds (the
-      // implicit system can have several roots at large dt).
+      // step increment |Gv - v0|), or until four maps bring no new best
+      // residual; FP64 maps then finish, so every accepted iterate and the
+      // exit map are FP64.  Switching well inside the basin keeps the FP64
+      // maps on the root the plain FP64 solve finds (the implicit system can
+      // have several roots at large dt).
       const double floor32 = floor_factor * 5.96e-8 * std::sqrt(st[2]);
       if (nrm_r < best32) {
         best32 = nrm_r;
@@ -912,9 +962,17 @@ int step_once(collop_ctx* ctx, const collop_step_params& p, collop_step_stats* o
       } else {
         since_best32++;
       }
-      const bool switch_now = nrm_r < std::max(floor32, eff_tol) || since_best32 >= 6 || k >= p.max_iter / 2;
+      const bool switch_now = nrm_r < std::max(floor32, eff_tol) || since_best32 >= 4 || k >= p.max_iter / 2;
       if (trace)
         std::fprintf(stderr, "  [f32] k=%d |r|=%.3e floor=%.3e best=%.3e\n", k, nrm_r, floor32, best32);
+      if (switch_now && p.f32_keep_window && p.use_anderson && history > 0) {
+        // Keep the secant window built from FP32 maps (its columns are
+        // differences far above FP32 rounding) and re-evaluate this iterate
+        // with an FP64 map before the next update: no column mixes the two.
+        use32 = false;
+        rebase = true;
+        continue;
+      }
       if (switch_now) {
         use32 = false;
         history = 0;
@@ -960,18 +1018,22 @@ int step_once(collop_ctx* ctx, const collop_step_params& p, collop_step_stats* o
       just_restarted = true;
     }
     const double beta = k > p.damp_decay_start ? p.damping * p.damp_decay_factor : p.damping;
-    if (just_restarted || col < 0) {
+    const bool rebased = rebase;
+    rebase = false;
+    if (just_restarted || (col < 0 && !(rebased && history > 0))) {
       launch_update(P<double>(ctx->s_v1), P<double>(ctx->s_Gv), P<double>(ctx->s_dG), nullptr, 0, beta, n,
                     ctx->stream);
       CHECK_LAUNCH();
     } else {
-      slot = slot % mw + 1;
-      history = hist_new;
-      const int h = history;
-      for (int j = 0; j < h; ++j) {
-        ATA[col * kMaxWindow + j] = st[3 + j];
-        ATA[j * kMaxWindow + col] = st[3 + j];
+      if (!rebased) {
+        slot = slot % mw + 1;
+        history = hist_new;
+        for (int j = 0; j < history; ++j) {
+          ATA[col * kMaxWindow + j] = st[3 + j];
+          ATA[j * kMaxWindow + col] = st[3 + j];
+        }
       }
+      const int h = history;
       double A[kMaxWindow * kMaxWindow], b[kMaxWindow], g[kMaxWindow];
       double dm = 0.0;
       for (int a = 0; a < h; ++a) {
@@ -996,6 +1058,11 @@ int step_once(collop_ctx* ctx, const collop_step_params& p, collop_step_stats* o
   }
   // v <- v1 and the new projection / entropy / moments (main.jl:296-303)
   std::swap(ctx->s_v, ctx->s_v1);
+  if (p.record_training) {
+    launch_targets(P<double>(ctx->s_v1), P<double>(ctx->s_v), P<double>(ctx->s_dotv), p.dt, n, rec, ctx->stream);
+    CHECK_LAUNCH();
+    ctx->m_size += n;
+  }
   if ((r = project_state(ctx, p.use_logsq))) return r;
   double sc[6];
   HIPC(hipMemcpyAsync(sc, ctx->b_scal.p, 6 * 8, hipMemcpyDeviceToHost, ctx->stream));
@@ -1098,6 +1165,198 @@ int collop_step(collop_ctx* ctx, const collop_step_params* params, double* v_ino
   if (r) return r;
   if ((r = collop_run_steps(ctx, params, 1, stats))) return r;
   return collop_state_get(ctx, v_inout);
+}
+
+// ---- warm-start MLP ----------------------------------------------------------
+int collop_mlp_set_params(collop_ctx* ctx, const float* params, int n) {
+  if (!ctx) return COLLOP_ERR_INVALID;
+  if (!params || n != kMlpParams) return fail(ctx, COLLOP_ERR_INVALID, "expected 4738 MLP parameters");
+  for (int i = 0; i < n; ++i)
+    if (!std::isfinite(params[i])) return fail(ctx, COLLOP_ERR_INVALID, "non-finite MLP parameter");
+  HIPC(hipSetDevice(ctx->device));
+  ENSURE(ctx->m_params, kMlpParams * sizeof(float));
+  ENSURE(ctx->m_adam_m, kMlpParams * sizeof(float));
+  ENSURE(ctx->m_adam_v, kMlpParams * sizeof(float));
+  HIPC(hipMemcpyAsync(ctx->m_params.p, params, kMlpParams * sizeof(float), hipMemcpyHostToDevice, ctx->stream));
+  HIPC(hipMemsetAsync(ctx->m_adam_m.p, 0, kMlpParams * sizeof(float), ctx->stream));
+  HIPC(hipMemsetAsync(ctx->m_adam_v.p, 0, kMlpParams * sizeof(float), ctx->stream));
+  HIPC(hipStreamSynchronize(ctx->stream));
+  ctx->m_adam_step = 0;
+  ctx->m_has_params = true;
+  return COLLOP_OK;
+}
+
+int collop_mlp_get_params(collop_ctx* ctx, float* params, int n) {
+  if (!ctx) return COLLOP_ERR_INVALID;
+  if (!ctx->m_has_params) return fail(ctx, COLLOP_ERR_NO_MODEL, "no MLP parameters on this context");
+  if (!params || n != kMlpParams) return fail(ctx, COLLOP_ERR_INVALID, "expected 4738 MLP parameters");
+  HIPC(hipSetDevice(ctx->device));
+  HIPC(hipMemcpyAsync(params, ctx->m_params.p, kMlpParams * sizeof(float), hipMemcpyDeviceToHost, ctx->stream));
+  HIPC(hipStreamSynchronize(ctx->stream));
+  return COLLOP_OK;
+}
+
+int collop_mlp_set_norm(collop_ctx* ctx, const float* mu, const float* sd, const float* out_scale) {
+  if (!ctx) return COLLOP_ERR_INVALID;
+  if (!mu || !sd || !out_scale) return fail(ctx, COLLOP_ERR_INVALID, "null normalisation array");
+  for (int j = 0; j < 6; ++j) {
+    if (!(sd[j] > 0.0f) || !std::isfinite(mu[j])) return fail(ctx, COLLOP_ERR_INVALID, "bad feature normalisation");
+    ctx->m_mu[j] = mu[j];
+    ctx->m_isd[j] = 1.0f / sd[j];
+  }
+  for (int j = 0; j < 2; ++j) {
+    if (!(out_scale[j] > 0.0f)) return fail(ctx, COLLOP_ERR_INVALID, "bad output scale");
+    ctx->m_oscale[j] = out_scale[j];
+  }
+  ctx->m_has_norm = true;
+  return COLLOP_OK;
+}
+
+int collop_mlp_dataset_alloc(collop_ctx* ctx, int64_t capacity) {
+  if (!ctx) return COLLOP_ERR_INVALID;
+  if (capacity < 0 || capacity > ((int64_t)1 << 31)) return fail(ctx, COLLOP_ERR_INVALID, "capacity out of range");
+  HIPC(hipSetDevice(ctx->device));
+  ENSURE(ctx->m_data, (size_t)std::max<int64_t>(capacity, 1) * 8 * sizeof(float));
+  ctx->m_cap = capacity;
+  ctx->m_size = 0;
+  return COLLOP_OK;
+}
+
+int collop_mlp_dataset_size(collop_ctx* ctx, int64_t* n) {
+  if (!ctx || !n) return COLLOP_ERR_INVALID;
+  *n = ctx->m_size;
+  return COLLOP_OK;
+}
+
+int collop_mlp_dataset_read(collop_ctx* ctx, int64_t first, int64_t count, float* out) {
+  if (!ctx) return COLLOP_ERR_INVALID;
+  if (!out || first < 0 || count < 0 || first + count > ctx->m_size)
+    return fail(ctx, COLLOP_ERR_INVALID, "dataset range out of bounds");
+  HIPC(hipSetDevice(ctx->device));
+  HIPC(hipMemcpyAsync(out, P<float>(ctx->m_data) + (size_t)first * 8, (size_t)count * 8 * sizeof(float),
+                      hipMemcpyDeviceToHost, ctx->stream));
+  HIPC(hipStreamSynchronize(ctx->stream));
+  return COLLOP_OK;
+}
+
+static void mlp_args(collop_ctx* ctx, MlpArgs& a) {
+  a.params = P<float>(ctx->m_params);
+  for (int j = 0; j < 6; ++j) {
+    a.mu[j] = ctx->m_mu[j];
+    a.isd[j] = ctx->m_isd[j];
+  }
+  a.oscale[0] = ctx->m_oscale[0];
+  a.oscale[1] = ctx->m_oscale[1];
+}
+
+// Sum of the per-workgroup loss partials (fixed order, host side).
+static int mlp_pass(collop_ctx* ctx, MlpArgs& a, int grid, double* loss_sum) {
+  ENSURE(ctx->m_lpart, (size_t)grid * sizeof(double));
+  a.loss_part = P<double>(ctx->m_lpart);
+  launch_mlp(a, grid, ctx->stream);
+  CHECK_LAUNCH();
+  if (loss_sum) {
+    std::vector<double> lp(grid);
+    HIPC(hipMemcpyAsync(lp.data(), a.loss_part, grid * sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
+    HIPC(hipStreamSynchronize(ctx->stream));
+    double t = 0.0;
+    for (double x : lp) t += x;
+    *loss_sum = t;
+  }
+  return COLLOP_OK;
+}
+
+int collop_mlp_train(collop_ctx* ctx, int epochs, int batch, double lr, uint64_t seed, double* loss_per_epoch) {
+  if (!ctx) return COLLOP_ERR_INVALID;
+  if (!(ctx->m_has_params && ctx->m_has_norm)) return fail(ctx, COLLOP_ERR_NO_MODEL, "set MLP parameters and norm first");
+  if (epochs < 0 || batch < 16 || batch % 16 || !(lr > 0.0)) return fail(ctx, COLLOP_ERR_INVALID, "bad training knobs");
+  const int64_t ntiles = ctx->m_size / 16;
+  if (ntiles < 1) return fail(ctx, COLLOP_ERR_INVALID, "the training dataset is empty");
+  HIPC(hipSetDevice(ctx->device));
+  const int tpb = batch / 16;
+  const int grid = std::min(tpb, 512);
+  ENSURE(ctx->m_tiles, (size_t)ntiles * sizeof(int));
+  ENSURE(ctx->m_gpart, (size_t)grid * kMlpParams * sizeof(float));
+  std::vector<int> perm((size_t)ntiles);
+  uint64_t st = seed * 0x9E3779B97F4A7C15ull + 1;  // splitmix64 stream for the shuffles
+  auto next = [&]() {
+    uint64_t z = (st += 0x9E3779B97F4A7C15ull);
+    z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+    z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+    return z ^ (z >> 31);
+  };
+  for (int e = 0; e < epochs; ++e) {
+    for (int64_t i = 0; i < ntiles; ++i) perm[i] = (int)i;
+    for (int64_t i = ntiles - 1; i > 0; --i) std::swap(perm[i], perm[next() % (uint64_t)(i + 1)]);
+    HIPC(hipMemcpyAsync(ctx->m_tiles.p, perm.data(), (size_t)ntiles * sizeof(int), hipMemcpyHostToDevice, ctx->stream));
+    for (int64_t b0 = 0; b0 < ntiles; b0 += tpb) {
+      MlpArgs a{};
+      mlp_args(ctx, a);
+      a.rec = P<float>(ctx->m_data);
+      a.nsamples = ctx->m_size;
+      a.tiles = P<int>(ctx->m_tiles) + b0;
+      a.ntiles = (int)std::min<int64_t>(tpb, ntiles - b0);
+      a.inv_batch = 2.0f / (16.0f * a.ntiles);
+      a.mode = 2;
+      a.grad_part = P<float>(ctx->m_gpart);
+      const int g = std::min(a.ntiles, grid);
+      int r = mlp_pass(ctx, a, g, nullptr);
+      if (r) return r;
+      launch_adam(P<float>(ctx->m_gpart), g, P<float>(ctx->m_params), P<float>(ctx->m_adam_m),
+                  P<float>(ctx->m_adam_v), nullptr, (float)lr, (int)++ctx->m_adam_step, 1, ctx->stream);
+      CHECK_LAUNCH();
+    }
+    if (loss_per_epoch) {
+      MlpArgs a{};
+      mlp_args(ctx, a);
+      a.rec = P<float>(ctx->m_data);
+      a.nsamples = ctx->m_size;
+      a.ntiles = (int)ntiles;
+      a.mode = 1;
+      double tot = 0.0;
+      int r = mlp_pass(ctx, a, 1024, &tot);
+      if (r) return r;
+      loss_per_epoch[e] = tot / (double)(ntiles * 16);
+    }
+  }
+  HIPC(hipStreamSynchronize(ctx->stream));
+  return COLLOP_OK;
+}
+
+int collop_mlp_eval(collop_ctx* ctx, const float* rec, int64_t n, int mode, float* y, double* loss, float* grad) {
+  if (!ctx) return COLLOP_ERR_INVALID;
+  if (!(ctx->m_has_params && ctx->m_has_norm)) return fail(ctx, COLLOP_ERR_NO_MODEL, "set MLP parameters and norm first");
+  if (!rec || n < 1 || n > ((int64_t)1 << 28) || mode < 0 || mode > 2 || (mode == 0 && !y) ||
+      (mode >= 1 && !loss) || (mode == 2 && !grad))
+    return fail(ctx, COLLOP_ERR_INVALID, "bad arguments");
+  HIPC(hipSetDevice(ctx->device));
+  ENSURE(ctx->m_eval, (size_t)n * 8 * sizeof(float));
+  ENSURE(ctx->m_y, (size_t)n * 2 * sizeof(float));
+  HIPC(hipMemcpyAsync(ctx->m_eval.p, rec, (size_t)n * 8 * sizeof(float), hipMemcpyHostToDevice, ctx->stream));
+  MlpArgs a{};
+  mlp_args(ctx, a);
+  a.rec = P<float>(ctx->m_eval);
+  a.nsamples = n;
+  a.ntiles = (int)((n + 15) / 16);
+  a.inv_batch = 2.0f / (float)n;
+  a.mode = mode;
+  a.y = P<float>(ctx->m_y);
+  const int grid = std::min(a.ntiles, 256);
+  ENSURE(ctx->m_gpart, (size_t)grid * kMlpParams * sizeof(float));
+  a.grad_part = P<float>(ctx->m_gpart);
+  double tot = 0.0;
+  int r = mlp_pass(ctx, a, grid, mode >= 1 ? &tot : nullptr);
+  if (r) return r;
+  if (mode == 0) HIPC(hipMemcpyAsync(y, ctx->m_y.p, (size_t)n * 2 * sizeof(float), hipMemcpyDeviceToHost, ctx->stream));
+  if (mode >= 1) *loss = tot / (double)n;
+  if (mode == 2) {
+    ENSURE(ctx->m_grad, kMlpParams * sizeof(float));
+    launch_adam(a.grad_part, grid, nullptr, nullptr, nullptr, P<float>(ctx->m_grad), 0.0f, 1, 0, ctx->stream);
+    CHECK_LAUNCH();
+    HIPC(hipMemcpyAsync(grad, ctx->m_grad.p, kMlpParams * sizeof(float), hipMemcpyDeviceToHost, ctx->stream));
+  }
+  HIPC(hipStreamSynchronize(ctx->stream));
+  return COLLOP_OK;
 }
 
 // ---- extras (not part of the reference's hook set) ----------------------

@@ -56,7 +56,7 @@ void launch_quad2(const MeshView& m, const double* iota, double* r, hipStream_t 
 // out[i] = sum_{j<n} part[i * istride + j * jstride] in fixed order, i < count
 void launch_sum_rows(const double* part, int n, int count, int istride, int jstride, double* out, hipStream_t st);
 // Gather: G (mode 0), or dS = -w G plus partial (dv.dS, |dv|^2) (mode 1),
-// or clamped log-gradient (mode 2).
+// or clamped log-gradient (mode 2), or (f, |grad f|) (mode 3).
 int gather_blocks(int N);
 void launch_gather(const MeshView& m, const double* x, const double* coeff, const double* w, const double* dv,
                    int N, int mode, double fs_floor, double g_max, double* out, double* dpart, hipStream_t st);
@@ -84,5 +84,30 @@ void launch_update(double* v1, const double* Gv, const double* dG, const double*
 // out = v + dt * (sum over slices of part)  (the Euler predictor, main.jl:245)
 void launch_euler(const double* part, int S, const double* v, double dt, int N, double* out, double* dotv,
                   hipStream_t st);
+
+// ---- warm-start MLP (mlp.hip) --------------------------------------------------
+constexpr int kMlpParams = 4738;
+struct MlpArgs {
+  const float* params;  // kMlpParams
+  const float* rec;     // [samples][8]: x0..x5 raw features, t0, t1 raw targets
+  long long nsamples;
+  const int* tiles;     // tile ids (16 samples each), or nullptr = 0..ntiles-1
+  int ntiles;
+  float mu[6], isd[6], oscale[2];
+  float inv_batch;      // 2 / B: gradient of the batch-mean squared error
+  int mode;             // 0: y only, 1: + loss, 2: + gradients
+  float* y;             // [samples][2] (mode 0)
+  double* loss_part;    // [grid]
+  float* grad_part;     // [grid][kMlpParams]
+};
+void launch_mlp(const MlpArgs& a, int grid, hipStream_t st);
+// grad = row sum of part; apply != 0: Adam step (lr, step >= 1) on params
+void launch_adam(const float* part, int rows, float* params, float* m, float* v, float* grad_out, float lr, int step,
+                 int apply, hipStream_t st);
+void launch_features(const double* v, const double* G, const double* fg, int N, float* rec, hipStream_t st);
+void launch_targets(const double* v_old, const double* v_new, const double* dotv, double dt, int N, float* rec,
+                    hipStream_t st);
+void launch_warm_apply(const double* v, const double* dotv, const float* y, float os0, float os1, double dt, int N,
+                       double* v1, hipStream_t st);
 
 }  // namespace collop

@@ -267,19 +267,24 @@ __global__ void k_rhs(const long long* __restrict__ part, int nblocks, int nd, d
   if (k < nd) out[k] = rhs_of(part, nblocks, nd, k, inv_scale);
 }
 
-// out[i] = sum_j part[i * istride + j * jstride], j < n, in fixed order.
-__global__ void k_sum_rows(const double* __restrict__ part, int n, int count, int istride, int jstride,
-                           double* __restrict__ out) {
-  const int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= count) return;
+// out[i] = sum_j part[i * istride + j * jstride], j < n: one workgroup per
+// output, fixed strided partition over threads, then the fixed block tree.
+constexpr int kSumThreads = 256;
+
+__global__ __launch_bounds__(kSumThreads) void k_sum_rows(const double* __restrict__ part, int n, int istride,
+                                                         int jstride, double* __restrict__ out) {
+  __shared__ double red[kSumThreads / 64];
+  const int i = blockIdx.x;
   double s = 0.0;
-  for (int j = 0; j < n; ++j) s += part[(size_t)i * istride + (size_t)j * jstride];
-  out[i] = s;
+  for (int j = threadIdx.x; j < n; j += kSumThreads) s += part[(size_t)i * istride + (size_t)j * jstride];
+  s = block_sum<kSumThreads>(s, red);
+  if (threadIdx.x == 0) out[i] = s;
 }
 
 // ---------------------------------------------------------------------------
 // Gathers.  mode 0: out = G (N x 2).  mode 1: out = dS = -w G, dpart[block] =
 // (dv . dS, |dv|^2).  mode 2: out = clamp(grad f / max(|f|, fs_floor), +-g_max).
+// mode 3: out = (f, |grad f|) at the marker (warm-start features).
 constexpr int kGatherThreads = 256;
 
 __global__ __launch_bounds__(kGatherThreads) void k_gather(MeshView m, const double* __restrict__ x,
@@ -312,9 +317,12 @@ __global__ __launch_bounds__(kGatherThreads) void k_gather(MeshView m, const dou
           const double c = coeff[(s1 + j1) + (s2 + j2) * m.nd1];
           acc1 += c * (d1[j1] * f2[j2]) * ih1;
           acc2 += c * (f1[j1] * d2[j2]) * ih2;
-          if (mode == 2) fv += c * (f1[j1] * f2[j2]);
+          if (mode >= 2) fv += c * (f1[j1] * f2[j2]);
         }
-      if (mode == 2) {
+      if (mode == 3) {
+        o1 = fv;
+        o2 = sqrt(acc1 * acc1 + acc2 * acc2);
+      } else if (mode == 2) {
         const double invf = 1.0 / fmax(fabs(fv), fs_floor);
         o1 = fmin(fmax(acc1 * invf, -g_max), g_max);
         o2 = fmin(fmax(acc2 * invf, -g_max), g_max);
@@ -393,7 +401,8 @@ void launch_quad2(const MeshView& m, const double* iota, double* r, hipStream_t 
 }
 
 void launch_sum_rows(const double* part, int n, int count, int istride, int jstride, double* out, hipStream_t st) {
-  hipLaunchKernelGGL(k_sum_rows, dim3((count + 255) / 256), dim3(256), 0, st, part, n, count, istride, jstride, out);
+  if (count <= 0) return;
+  hipLaunchKernelGGL(k_sum_rows, dim3(count), dim3(kSumThreads), 0, st, part, n, istride, jstride, out);
 }
 
 int gather_blocks(int N) {

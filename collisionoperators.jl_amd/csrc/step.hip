@@ -94,24 +94,27 @@ __global__ __launch_bounds__(kEwThreads) void k_epilogue(EpiArgs A, int N) {
     st[1] = x1 * x1;
     const double inc = gv - A.v0[i];
     st[2] = inc * inc;
+    // col >= 0: write the new column and its Gram row; hist > 0: dF_j . r
+    // (col < 0 with hist > 0 re-bases the window on a new map at the same iterate)
+    double f = 0.0;
     if (A.col >= 0) {
-      const double f = ri - A.rp[i];
+      f = ri - A.rp[i];
       A.dF[(size_t)A.col * n2 + i] = f;
       A.dG[(size_t)A.col * n2 + i] = gv - A.Gp[i];
+    }
 #pragma unroll
-      for (int j = 0; j < kMaxWindow; ++j) {
-        if (j < A.hist) {
-          const double xj = (j == A.col) ? f : A.dF[(size_t)j * n2 + i];
-          st[3 + j] = xj * f;
-          st[3 + kMaxWindow + j] = xj * ri;
-        }
+    for (int j = 0; j < kMaxWindow; ++j) {
+      if (j < A.hist) {
+        const double xj = (j == A.col) ? f : A.dF[(size_t)j * n2 + i];
+        if (A.col >= 0) st[3 + j] = xj * f;
+        st[3 + kMaxWindow + j] = xj * ri;
       }
     }
   }
 #pragma unroll
   for (int j = 0; j < kStatWidth; ++j) {
-    const bool used = j < 3 || (A.col >= 0 && ((j >= 3 && j < 3 + A.hist) ||
-                                               (j >= 3 + kMaxWindow && j < 3 + kMaxWindow + A.hist)));
+    const bool used = j < 3 || (A.col >= 0 && j >= 3 && j < 3 + A.hist) ||
+                      (j >= 3 + kMaxWindow && j < 3 + kMaxWindow + A.hist);
     if (!used) continue;  // uniform across the block
     const double t = block_sum<kEwThreads>(st[j], red);
     if (threadIdx.x == 0) A.stat_part[(size_t)blockIdx.x * kStatWidth + j] = t;

@@ -147,6 +147,11 @@ typedef struct {
   int32_t warmstart;        /* collop_warmstart                                 */
   double f32_floor_factor;  /* F32C: FP32 maps run until ||r|| < factor * 2^-24
                                * ||Gv - v0|| (0 = 16), then FP64 maps finish    */
+  int32_t f32_keep_window;  /* F32C: 1 = keep the Anderson window across the
+                               switch (the last FP32 iterate is re-evaluated in
+                               FP64 first), 0 = restart it                      */
+  int32_t record_training;  /* 1 = append (features, target) of every marker of
+                               every step to the warm-start training dataset   */
 } collop_step_params;
 
 typedef struct {
@@ -173,6 +178,32 @@ int collop_run_steps(collop_ctx* ctx, const collop_step_params* params, int n_st
 /* One step on host arrays: state_set + run_steps(1) + state_get. */
 int collop_step(collop_ctx* ctx, const collop_step_params* params, double* v_inout, const double* w, int64_t N,
                 collop_step_stats* stats);
+
+/* ---- learned warm start ----------------------------------------------------
+ * The reference calls nn_warmstart_correct!(v1, model, ...) after its Euler
+ * predictor (main.jl:252-254, hook site only: warmstart_nn.jl is absent).  Here
+ * the model is a per-marker MLP 6 -> 64 -> 64 -> 2 (GELU) run on the matrix
+ * cores (BF16 operands, FP32 accumulation).  Features of marker a at the start
+ * of a step: v_a (2), G_a = -(dS/dv_a)/w_a (2), f_s(v_a), |grad f_s(v_a)|.  The
+ * output y corrects the predictor: v1 = v + dt (vdot_euler + y * out_scale),
+ * i.e. it predicts the step's midpoint velocity.  Training targets are
+ * (v_{n+1} - v_n)/dt - vdot_euler from the solver's own converged steps.
+ * Parameters: float32[4738] = W1[64][6] b1[64] W2[64][64] b2[64] W3[2][64] b3[2]. */
+int collop_mlp_set_params(collop_ctx* ctx, const float* params, int n);
+int collop_mlp_get_params(collop_ctx* ctx, float* params, int n);
+/* features are normalised as (x - mu) / sd; targets scaled by 1 / out_scale */
+int collop_mlp_set_norm(collop_ctx* ctx, const float* mu, const float* sd, const float* out_scale);
+/* (Re)allocate and clear the device-resident training dataset (samples). */
+int collop_mlp_dataset_alloc(collop_ctx* ctx, int64_t capacity);
+int collop_mlp_dataset_size(collop_ctx* ctx, int64_t* n);
+/* Copy samples [first, first + count) as float32[count][8] = (x0..x5, t0, t1). */
+int collop_mlp_dataset_read(collop_ctx* ctx, int64_t first, int64_t count, float* out);
+/* Adam (beta 0.9 / 0.999, eps 1e-8) on the batch-mean squared error over
+ * shuffled 16-sample tiles; loss_per_epoch[epochs] optional. */
+int collop_mlp_train(collop_ctx* ctx, int epochs, int batch, double lr, uint64_t seed, double* loss_per_epoch);
+/* Run the network on host records float32[n][8]: mode 0 -> y[n][2]; mode 1 ->
+ * mean loss; mode 2 -> mean loss and its gradient grad[4738]. */
+int collop_mlp_eval(collop_ctx* ctx, const float* rec, int64_t n, int mode, float* y, double* loss, float* grad);
 
 /* ---- tuning and measurement (no counterpart in the reference) -----------
  * Pair-sum tile: gamma_per_lane in {1, 2} targets per lane, slices in [1, 64]

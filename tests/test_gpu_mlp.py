@@ -1,0 +1,85 @@
+"""The warm-start MLP kernels (BF16 MFMA, FP32 accumulate) against the numpy
+reference oracle/mlp_ref.py, which rounds the same operands to bfloat16:
+forward outputs, loss, and the full parameter gradient; plus a few Adam steps
+that must lower the loss, and the recording / warm-start plumbing of a step."""
+import numpy as np
+import pytest
+
+import collop
+import mlp_ref as M
+from conftest import uniform_bp
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module")
+def wsm(gpu_ctx):
+    bp = uniform_bp(-6, 6, 16)
+    ws = collop.Workspace(bp, bp, ctx=gpu_ctx)
+    return ws, collop.WarmStart(ws)
+
+
+def records(n, seed=0):
+    rng = np.random.default_rng(seed)
+    rec = np.zeros((n, 8), np.float32)
+    rec[:, :6] = rng.normal([0.5, -0.2, 1.0, 2.0, 0.1, 0.3], [1.5, 1.0, 3.0, 2.0, 0.05, 0.2], (n, 6))
+    rec[:, 6:] = rng.normal(0, 0.05, (n, 2))
+    return rec
+
+
+def normalised(rec, mu, sd, os_):
+    x = ((rec[:, :6] - mu) * (np.float32(1.0) / sd)).astype(np.float32)
+    t = (rec[:, 6:] / os_).astype(np.float32)
+    return x, t
+
+
+def test_forward_loss_grad_match_numpy(wsm):
+    ws, w = wsm
+    p = M.init(3)
+    # larger weights so activations leave GELU's linear range
+    p[:384] *= 3.0
+    rec = records(1000)
+    mu = rec[:, :6].mean(0).astype(np.float32)
+    sd = rec[:, :6].std(0).astype(np.float32)
+    os_ = np.array([0.05, 0.04], np.float32)
+    w.set_params(p)
+    w.set_norm(mu, sd, os_)
+    x, t = normalised(rec, mu, sd, os_)
+    y, _, _ = w.eval(rec, 0)
+    y_ref, _ = M.forward(p, x)
+    err = np.max(np.abs(y - y_ref)) / np.max(np.abs(y_ref))
+    print(f"forward rel err {err:.2e}")
+    assert err < 2e-3
+    _, loss, g = w.eval(rec, 2)
+    loss_ref, g_ref = M.loss_and_grad(p, x, t)
+    print(f"loss {loss:.6e} ref {loss_ref:.6e}; grad rel err {np.max(np.abs(g - g_ref)) / np.max(np.abs(g_ref)):.2e}")
+    assert abs(loss - loss_ref) < 2e-3 * loss_ref
+    for name, sl in [("W1", slice(0, 384)), ("b1", slice(384, 448)), ("W2", slice(448, 4544)),
+                     ("b2", slice(4544, 4608)), ("W3", slice(4608, 4736)), ("b3", slice(4736, 4738))]:
+        e = np.max(np.abs(g[sl] - g_ref[sl])) / np.max(np.abs(g_ref[sl]))
+        assert e < 2e-2, (name, e)
+    # against exact (unrounded) finite differences the bf16 gradient is close too
+    _, g64 = M.loss_and_grad(p, x, t, emulate_bf16=False)
+    assert np.dot(g, g64) / (np.linalg.norm(g) * np.linalg.norm(g64)) > 0.995
+
+
+def test_adam_lowers_loss(wsm):
+    ws, w = wsm
+    sim = collop.Simulation(ws, *collop.markers("gaussian", 4096, 5))
+    w.dataset_alloc(4 * 4096)
+    prm = collop.config("gaussian_n4096", DT=0.01).step_params()
+    prm.record_training = 1
+    sim.run(prm, 4)
+    assert w.dataset_size() == 4 * 4096
+    d = w.dataset_read(0, 4096)
+    assert np.all(np.isfinite(d)) and np.abs(d[:, 6:]).max() > 0
+    w.set_params(M.init(0))
+    w.fit_norm()
+    losses = w.train(epochs=3, batch=1024, lr=1e-3, seed=1)
+    print("losses", losses)
+    assert losses[-1] < losses[0]
+    # warm start runs and the step still converges to the solver tolerance
+    prm.record_training = 0
+    prm.warmstart = collop.WARM_MLP
+    st = sim.run(prm, 1)[0]
+    assert st["exit_reason"] == 0
