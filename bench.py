@@ -24,6 +24,7 @@ restatement of the reference CPU path, OpenMP over targets like
 functions.jl:210) timing whole Picard maps at the same N on this host.
 
 Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu]
+       python bench.py --report precision --config bimaxwellian_n262144 --steps 20
 """
 from __future__ import annotations
 
@@ -112,6 +113,107 @@ def cpu_baseline(p, v, w, mean_maps_per_step, budget_s=20.0):
                 pairs_per_s=N * (N - 1) / t_map)
 
 
+def precision_report(args):
+    """FP32-compensated vs FP64 on the same markers: per-field errors and the
+    per-step conservation drift of both paths, plus the FP64 pair sum against
+    the CPU oracle on a subsample of targets."""
+    import collop
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import oracle as O
+    p = collop.config(args.config)
+    v, w = collop.initial_markers(p)
+    ws = collop.build_workspace(p, device=0)
+    N = v.shape[0]
+    steps = args.steps
+    out = {"workload": args.config, "N": N, "steps": steps, "dt": p.DT}
+    runs = {}
+    for name, prec in (("fp64", collop.PREC_F64), ("f32c", collop.PREC_F32C)):
+        sim = collop.Simulation(ws, v, w)
+        E0 = 0.5 * float(np.sum(w * (v[:, 0] ** 2 + v[:, 1] ** 2)))
+        P0 = (w[:, None] * v).sum(axis=0)
+        S_prev = collop.compute_r_and_entropy_gpu(ws, sim.coeffs())[1]
+        E_prev, P_prev = E0, P0
+        drift = []
+        t0 = time.perf_counter()
+        for i in range(steps):
+            st = sim.run(p.step_params(precision=prec), 1)[0]
+            drift.append({"dE_rel": abs(st["energy"] - E_prev) / E0,
+                          "dP": float(np.max(np.abs(np.array(st["momentum"]) - P_prev))),
+                          "dS": st["entropy"] - S_prev, "iters": st["iters"], "iters_f32": st["iters_f32"],
+                          "exit": st["exit_reason"]})
+            E_prev, P_prev, S_prev = st["energy"], np.array(st["momentum"]), st["entropy"]
+            log(f"  {name} step {i}: iters {st['iters']} (f32 {st['iters_f32']}) exit {st['exit_reason']} "
+                f"dE/E {drift[-1]['dE_rel']:.1e} dS {drift[-1]['dS']:.2e}")
+        runs[name] = dict(v=sim.markers(), fc=sim.coeffs(), S=S_prev, E=E_prev, P=P_prev, drift=drift,
+                          seconds=time.perf_counter() - t0)
+    a, b = runs["fp64"], runs["f32c"]
+    rel = lambda x, y: float(np.max(np.abs(x - y)) / np.max(np.abs(y)))  # noqa: E731
+    out["f32c_vs_fp64_after_steps"] = {"v_max_abs": float(np.max(np.abs(b["v"] - a["v"]))), "v_rel": rel(b["v"], a["v"]),
+                                       "f_coeffs_rel": rel(b["fc"], a["fc"]), "entropy_abs": abs(b["S"] - a["S"]),
+                                       "energy_rel": abs(b["E"] - a["E"]) / abs(a["E"]),
+                                       "momentum_abs": float(np.max(np.abs(b["P"] - a["P"])))}
+    for name in runs:
+        d = runs[name]["drift"]
+        out[name] = {"max_dE_rel": max(x["dE_rel"] for x in d), "max_dP": max(x["dP"] for x in d),
+                     "min_dS": min(x["dS"] for x in d), "all_converged": all(x["exit"] == 0 for x in d),
+                     "mean_iters": float(np.mean([x["iters"] for x in d])),
+                     "mean_iters_f32": float(np.mean([x["iters_f32"] for x in d])),
+                     "steps_per_s": steps / runs[name]["seconds"], "per_step": d}
+    # per-map field errors on the initial state: pair sum FP32c and FP64 vs the oracle (targets subsample)
+    m = O.Mesh(p.bp1, p.bp2)
+    fc = m.l2_project(v, w)
+    L = m.mass_solve(m.compute_r(fc))
+    G = m.compute_G(v, L)
+    sub = min(N, 2048)
+    ref = O.collision_rows(m.box, v, w, G, 0, sub)[:sub]
+    d64 = np.zeros((N, 2), order="F")
+    d32 = np.zeros((N, 2), order="F")
+    collop.compute_collision_gpu(ws, d64, v, w, G)
+    collop.compute_collision_gpu32(ws, d32, v, w, G)
+    fcg = np.zeros(ws.n_dofs)
+    collop.l2_project_gpu(ws, fcg, v, w)
+    Gg = np.zeros((N, 2), order="F")
+    collop.compute_G_gpu(ws, Gg, v, L)
+    out["per_map_vs_oracle"] = {"pair_sum_fp64_rel": rel(d64[:sub], ref), "pair_sum_f32c_rel": rel(d32[:sub], ref),
+                                "f_coeffs_rel": rel(fcg, fc), "G_rel": rel(Gg, G), "targets_checked": sub}
+    print(json.dumps(out), flush=True)
+
+
+def kernel_probe(args):
+    """Three launches of each pair kernel at the workload, then one 256 MiB
+    device-to-device copy (the known-size calibration of the rocprofv3
+    FETCH_SIZE / WRITE_SIZE counters)."""
+    import ctypes as C
+
+    import collop
+    p = collop.config(args.config)
+    v, w = collop.initial_markers(p)
+    ws = collop.build_workspace(p, device=0)
+    N = v.shape[0]
+    fc = np.zeros(ws.n_dofs)
+    collop.l2_project_gpu(ws, fc, v, w)
+    r, _ = collop.compute_r_and_entropy_gpu(ws, fc)
+    L = collop.mass_solve_gpu(ws, r)
+    G = np.zeros((N, 2), order="F")
+    collop.compute_G_gpu(ws, G, v, L)
+    out = np.zeros((N, 2), order="F")
+    for _ in range(3):
+        collop.compute_collision_gpu(ws, out, v, w, G)
+    for _ in range(3):
+        collop.compute_collision_gpu32(ws, out, v, w, G)
+    hip = C.CDLL("libamdhip64.so")
+    nbytes = 256 << 20
+    a, b = C.c_void_p(), C.c_void_p()
+    if hip.hipMalloc(C.byref(a), C.c_size_t(nbytes)) or hip.hipMalloc(C.byref(b), C.c_size_t(nbytes)):
+        raise RuntimeError("hipMalloc failed")
+    hip.hipMemset(a, 1, C.c_size_t(nbytes))
+    hip.hipMemcpy(b, a, C.c_size_t(nbytes), 3)  # device to device
+    hip.hipDeviceSynchronize()
+    hip.hipFree(a)
+    hip.hipFree(b)
+    log(f"[probe] pair kernels x3 (FP64, FP32c) at N={N}; copy {nbytes} bytes")
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -119,11 +221,22 @@ def main():
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--config", default="bimodal_n65536")
+    ap.add_argument("--kernel-probe", action="store_true",
+                    help="run only the pair kernels and a 256 MiB copy (rocprofv3 counter runs)")
+    ap.add_argument("--report", choices=["precision"], default=None,
+                    help="precision: FP32c vs FP64 field errors and drift (use --config bimaxwellian_n262144)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU: this build has no multi-GPU path (asked for --gpus {args.gpus}).",
               file=sys.stderr)
         sys.exit(2)
+
+    if args.report == "precision":
+        precision_report(args)
+        return
+    if args.kernel_probe:
+        kernel_probe(args)
+        return
 
     import collop
     p = collop.config(args.config)

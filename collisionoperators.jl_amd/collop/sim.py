@@ -58,6 +58,6 @@ def profile(ws: Workspace, enable: bool):
     return ms, [n[0], n[1]]
 
 
-def set_pair_tile(ws: Workspace, gamma_per_lane: int = 1, slices: int = 0, newton: int = 2):
+def set_pair_tile(ws: Workspace, gamma_per_lane: int = 1, slices: int = 0, newton: int = 1):
     """Pin the pair-sum tile (slices = 0: automatic); see collop_set_pair_tile."""
     ws.ctx.check(ws.ctx.lib.collop_set_pair_tile(ws.ctx.h, gamma_per_lane, slices, newton))

@@ -20,109 +20,11 @@
 #include <vector>
 
 #include "../../include/collop.h"
-#include "collop_launch.hpp"
+#include "collop_ctx.hpp"
 
 using namespace collop;
 
 namespace {
-
-struct DBuf {
-  void* p = nullptr;
-  size_t bytes = 0;
-};
-
-}  // namespace
-
-struct collop_ctx {
-  int device = 0;
-  hipStream_t stream = nullptr;
-  std::string err;
-  // ---- mesh
-  bool has_mesh = false;
-  MeshView mv{};
-  std::vector<double> h_bp1, h_bp2;
-  std::vector<DBuf*> mesh_bufs;
-  // ---- scratch buffers by name (grow-only)
-  DBuf b_v, b_w, b_G, b_out, b_vmid, b_dv, b_dS, b_s64, b_s32, b_part, b_spart_sc, b_mom, b_dpart, b_iota, b_spart_q,
-      b_vec, b_scal, b_statp, b_stat;
-  // ---- device-resident state
-  bool has_state = false;
-  int64_t N = 0;
-  double w_abs_sum = 0.0;  // sum |w| of the state (fixed-point scale of the scatter)
-  DBuf s_v, s_w, s_v1, s_Gv, s_r, s_rp, s_Gp, s_best, s_dF, s_dG, s_fc, s_dotv;
-  double* h_stat = nullptr;  // pinned, kStatWidth doubles
-  // ---- warm-start MLP
-  DBuf m_params, m_adam_m, m_adam_v, m_data, m_rec, m_y, m_gpart, m_lpart, m_tiles, m_eval, m_grad;
-  bool m_has_params = false, m_has_norm = false;
-  float m_mu[6] = {0}, m_isd[6] = {0}, m_oscale[2] = {1, 1};
-  int64_t m_cap = 0, m_size = 0;
-  long m_adam_step = 0;
-  // ---- pair tile and profiling
-  PairTile tile64{1, 8, 2};
-  PairTile tile32{1, 8, 0};
-  bool tile_auto = true;
-  bool profile = false;
-  std::vector<std::pair<hipEvent_t, hipEvent_t>> ev_pool;
-  size_t ev_used = 0;
-  std::vector<int> ev_kind;
-  double prof_ms[2] = {0.0, 0.0};
-  long prof_n[2] = {0, 0};
-};
-
-namespace {
-
-int fail(collop_ctx* c, int code, const std::string& msg) {
-  if (c) c->err = msg;
-  return code;
-}
-
-#define HIPC(expr)                                                                                     \
-  do {                                                                                                 \
-    hipError_t _e = (expr);                                                                            \
-    if (_e != hipSuccess)                                                                              \
-      return fail(ctx, COLLOP_ERR_HIP, std::string(#expr) + ": " + hipGetErrorString(_e));             \
-  } while (0)
-
-#define CHECK_LAUNCH()                                                                                 \
-  do {                                                                                                 \
-    hipError_t _e = hipGetLastError();                                                                 \
-    if (_e != hipSuccess) return fail(ctx, COLLOP_ERR_HIP, std::string("kernel launch: ") + hipGetErrorString(_e)); \
-  } while (0)
-
-int ensure(collop_ctx* ctx, DBuf& b, size_t bytes) {
-  if (bytes == 0) bytes = 8;
-  if (b.bytes >= bytes) return COLLOP_OK;
-  if (b.p) {
-    hipError_t e = hipFree(b.p);
-    b.p = nullptr;
-    b.bytes = 0;
-    if (e != hipSuccess) return fail(ctx, COLLOP_ERR_HIP, std::string("hipFree: ") + hipGetErrorString(e));
-  }
-  hipError_t e = hipMalloc(&b.p, bytes);
-  if (e != hipSuccess) {
-    b.p = nullptr;
-    return fail(ctx, COLLOP_ERR_HIP, "hipMalloc(" + std::to_string(bytes) + "): " + hipGetErrorString(e));
-  }
-  b.bytes = bytes;
-  return COLLOP_OK;
-}
-
-void release(DBuf& b) {
-  if (b.p) (void)hipFree(b.p);
-  b.p = nullptr;
-  b.bytes = 0;
-}
-
-template <class T>
-T* P(DBuf& b) {
-  return static_cast<T*>(b.p);
-}
-
-#define ENSURE(buf, bytes)                             \
-  do {                                                 \
-    int _r = ensure(ctx, (buf), (bytes));              \
-    if (_r != COLLOP_OK) return _r;                    \
-  } while (0)
 
 // Gauss-Legendre rule on [0,1]: Newton's method on the three-term recurrence
 // of P_n (any textbook; e.g. Press et al., Numerical Recipes, 4.6).
@@ -239,12 +141,13 @@ int check_N(collop_ctx* ctx, int64_t N) {
 
 void choose_tile(collop_ctx* ctx, int N) {
   if (!ctx->tile_auto) return;
-  // Enough workgroups (>= 2048, i.e. ~8 waves per SIMD on 256 CUs) by cutting
-  // the source range into slices; multiples of 8 keep one slice set per XCD.
+  // Enough workgroups (>= 8192, i.e. ~32 waves per SIMD on 256 CUs, measured
+  // best at N = 65536) by cutting the source range into slices; multiples of 8
+  // keep one slice set per XCD.
   for (PairTile* t : {&ctx->tile64, &ctx->tile32}) {
     const int per = 256 * t->gamma_per_lane;
     const int gb = std::max(1, (N + per - 1) / per);
-    int s = (2048 + gb - 1) / gb;
+    int s = (8192 + gb - 1) / gb;
     if (s >= 8) s = (s + 7) / 8 * 8;
     s = std::max(1, std::min(s, 64));
     while (s > 1 && (N + s - 1) / s < 16) s /= 2;  // keep slices non-trivial
@@ -360,6 +263,8 @@ int collop_destroy(collop_ctx* ctx) {
   if (!ctx) return COLLOP_OK;
   (void)hipSetDevice(ctx->device);
   (void)hipStreamSynchronize(ctx->stream);
+  ensemble_free(ctx->ens);
+  ctx->ens = nullptr;
   free_mesh(ctx);
   for (DBuf* b : {&ctx->b_v, &ctx->b_w, &ctx->b_G, &ctx->b_out, &ctx->b_vmid, &ctx->b_dv, &ctx->b_dS, &ctx->b_s64,
                   &ctx->b_s32, &ctx->b_part, &ctx->b_spart_sc, &ctx->b_mom, &ctx->b_dpart, &ctx->b_iota,

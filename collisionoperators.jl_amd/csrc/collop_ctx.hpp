@@ -1,0 +1,121 @@
+// collop_ctx.hpp -- the context behind the opaque collop_ctx handle and the
+// host-side helpers shared by the C-ABI translation units (capi.hip,
+// ensemble.hip).  Internal: not part of the public header.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <string>
+#include <utility>
+#include <vector>
+
+#include "../../include/collop.h"
+#include "collop_launch.hpp"
+
+namespace collop {
+
+struct DBuf {
+  void* p = nullptr;
+  size_t bytes = 0;
+};
+
+struct EnsembleState;  // ensemble.hip
+void ensemble_free(EnsembleState* e);
+
+}  // namespace collop
+
+struct collop_ctx {
+  using DBuf = collop::DBuf;
+  int device = 0;
+  hipStream_t stream = nullptr;
+  std::string err;
+  // ---- mesh
+  bool has_mesh = false;
+  collop::MeshView mv{};
+  std::vector<double> h_bp1, h_bp2;
+  std::vector<DBuf*> mesh_bufs;
+  // ---- scratch buffers by name (grow-only)
+  DBuf b_v, b_w, b_G, b_out, b_vmid, b_dv, b_dS, b_s64, b_s32, b_part, b_spart_sc, b_mom, b_dpart, b_iota, b_spart_q,
+      b_vec, b_scal, b_statp, b_stat;
+  // ---- device-resident state
+  bool has_state = false;
+  int64_t N = 0;
+  double w_abs_sum = 0.0;  // sum |w| of the state (fixed-point scale of the scatter)
+  DBuf s_v, s_w, s_v1, s_Gv, s_r, s_rp, s_Gp, s_best, s_dF, s_dG, s_fc, s_dotv;
+  double* h_stat = nullptr;  // pinned, kStatWidth doubles
+  // ---- warm-start MLP
+  DBuf m_params, m_adam_m, m_adam_v, m_data, m_rec, m_y, m_gpart, m_lpart, m_tiles, m_eval, m_grad;
+  bool m_has_params = false, m_has_norm = false;
+  float m_mu[6] = {0}, m_isd[6] = {0}, m_oscale[2] = {1, 1};
+  int64_t m_cap = 0, m_size = 0;
+  long m_adam_step = 0;
+  // ---- pair tile and profiling
+  collop::PairTile tile64{1, 32, 1};
+  collop::PairTile tile32{1, 32, 0};
+  bool tile_auto = true;
+  bool profile = false;
+  std::vector<std::pair<hipEvent_t, hipEvent_t>> ev_pool;
+  size_t ev_used = 0;
+  std::vector<int> ev_kind;
+  double prof_ms[2] = {0.0, 0.0};
+  long prof_n[2] = {0, 0};
+  // ---- batched ensemble (ensemble.hip)
+  collop::EnsembleState* ens = nullptr;
+};
+
+namespace collop {
+
+inline int fail(collop_ctx* c, int code, const std::string& msg) {
+  if (c) c->err = msg;
+  return code;
+}
+
+#define HIPC(expr)                                                                                     \
+  do {                                                                                                 \
+    hipError_t _e = (expr);                                                                            \
+    if (_e != hipSuccess)                                                                              \
+      return fail(ctx, COLLOP_ERR_HIP, std::string(#expr) + ": " + hipGetErrorString(_e));             \
+  } while (0)
+
+#define CHECK_LAUNCH()                                                                                 \
+  do {                                                                                                 \
+    hipError_t _e = hipGetLastError();                                                                 \
+    if (_e != hipSuccess) return fail(ctx, COLLOP_ERR_HIP, std::string("kernel launch: ") + hipGetErrorString(_e)); \
+  } while (0)
+
+inline int ensure(collop_ctx* ctx, DBuf& b, size_t bytes) {
+  if (bytes == 0) bytes = 8;
+  if (b.bytes >= bytes) return COLLOP_OK;
+  if (b.p) {
+    hipError_t e = hipFree(b.p);
+    b.p = nullptr;
+    b.bytes = 0;
+    if (e != hipSuccess) return fail(ctx, COLLOP_ERR_HIP, std::string("hipFree: ") + hipGetErrorString(e));
+  }
+  hipError_t e = hipMalloc(&b.p, bytes);
+  if (e != hipSuccess) {
+    b.p = nullptr;
+    return fail(ctx, COLLOP_ERR_HIP, "hipMalloc(" + std::to_string(bytes) + "): " + hipGetErrorString(e));
+  }
+  b.bytes = bytes;
+  return COLLOP_OK;
+}
+
+inline void release(DBuf& b) {
+  if (b.p) (void)hipFree(b.p);
+  b.p = nullptr;
+  b.bytes = 0;
+}
+
+template <class T>
+T* P(DBuf& b) {
+  return static_cast<T*>(b.p);
+}
+
+#define ENSURE(buf, bytes)                             \
+  do {                                                 \
+    int _r = ensure(ctx, (buf), (bytes));              \
+    if (_r != COLLOP_OK) return _r;                    \
+  } while (0)
+
+}  // namespace collop

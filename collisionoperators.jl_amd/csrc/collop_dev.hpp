@@ -20,6 +20,7 @@ namespace collop {
 
 constexpr int kMaxWindow = 8;     // largest Anderson window m
 constexpr int kScatterThreads = 256;
+constexpr int kMaxBp = 512;       // breakpoints per direction staged in LDS
 
 struct MeshView {
   const double* bp1;
@@ -110,6 +111,122 @@ __device__ __forceinline__ double block_sum(double x, double* sh) {
     for (int i = 0; i < NT / 64; ++i) t += sh[i];
   }
   return t;
+}
+
+// ---- marker location against LDS breakpoints (fem.hip, ensemble.hip) ----
+__device__ __forceinline__ void stage_bp(const MeshView& m, double* sb1, double* sb2) {
+  for (int i = threadIdx.x; i < m.nb1; i += blockDim.x) sb1[i] = m.bp1[i];
+  for (int i = threadIdx.x; i < m.nb2; i += blockDim.x) sb2[i] = m.bp2[i];
+}
+
+struct Located {
+  int i, j;
+  double xi1, xi2, h1, h2;
+};
+
+// locate_particle (MantisWrappers.jl:168-179) against LDS breakpoints
+__device__ __forceinline__ bool locate(const MeshView& m, const double* sb1, const double* sb2, double x, double y,
+                                       Located& L) {
+  if (!in_box(x, y, m.lo1, m.hi1, m.lo2, m.hi2)) return false;
+  L.i = cell_of(sb1, m.nb1, m.uniform1, m.lo1, m.inv_h1, x);
+  L.j = cell_of(sb2, m.nb2, m.uniform2, m.lo2, m.inv_h2, y);
+  L.h1 = sb1[L.i + 1] - sb1[L.i];
+  L.h2 = sb2[L.j + 1] - sb2[L.j];
+  L.xi1 = (x - sb1[L.i]) / L.h1;
+  L.xi2 = (y - sb2[L.j]) / L.h2;
+  return true;
+}
+
+// ---- banded Cholesky solve along one direction of the coefficient array held
+// in LDS: chol = {1/L_ii, L_i,i-1, L_i,i-2}; element i of the line is X[i * istride].
+__device__ __forceinline__ void band_solve_line(double* X, int n, int istride, const double* __restrict__ chol) {
+  double y2 = 0.0, y1 = 0.0;  // y_{i-2}, y_{i-1}
+  for (int i = 0; i < n; ++i) {
+    const double l1 = chol[3 * i + 1], l2 = chol[3 * i + 2];
+    double s = X[i * istride];
+    s -= l1 * y1;
+    s -= l2 * y2;
+    const double yi = s * chol[3 * i];
+    X[i * istride] = yi;
+    y2 = y1;
+    y1 = yi;
+  }
+  double x1 = 0.0, x2 = 0.0;  // x_{i+1}, x_{i+2}
+  for (int i = n - 1; i >= 0; --i) {
+    const double u1 = i + 1 < n ? chol[3 * (i + 1) + 1] : 0.0;
+    const double u2 = i + 2 < n ? chol[3 * (i + 2) + 2] : 0.0;
+    double s = X[i * istride];
+    s -= u1 * x1;
+    s -= u2 * x2;
+    const double xi = s * chol[3 * i];
+    X[i * istride] = xi;
+    x2 = x1;
+    x1 = xi;
+  }
+}
+
+// ---- pair terms (pair.hip, ensemble.hip) ----------------------------------
+template <int NEWTON>
+__device__ __forceinline__ double rsqrt_f64(double q) {
+  double y = __builtin_amdgcn_rsq(q);
+  const double h = 0.5 * q;
+#pragma unroll
+  for (int i = 0; i < NEWTON; ++i) {
+    const double e = __builtin_fma(-h, y * y, 0.5);
+    y = __builtin_fma(y, e, y);
+  }
+  return y;
+}
+
+// One ordered pair (gamma in registers, alpha uniform).  Returns the weight
+// factor t = w_a / |d|^3 (0 for masked pairs) through `t_out` for the metric.
+template <int NEWTON>
+__device__ __forceinline__ void pair_term_f64(double vg1, double vg2, double Gg1, double Gg2, double av1, double av2,
+                                              double ag1, double ag2, double aw, double& acc1, double& acc2,
+                                              double& d1o, double& d2o, double& t_out) {
+  const double d1 = vg1 - av1;
+  const double d2 = vg2 - av2;
+  const double q = __builtin_fma(d2, d2, d1 * d1);
+  const double y = rsqrt_f64<NEWTON>(__builtin_fmax(q, 1e-24));
+  const double g1 = ag1 - Gg1;
+  const double g2 = ag2 - Gg2;
+  const double c = __builtin_fma(g1, d2, -(g2 * d1));
+  double t = (aw * y) * (y * y);
+  t = q < 1e-24 ? 0.0 : t;  // coincident-pair guard (functions.jl:224)
+  const double k = t * c;
+  acc1 = __builtin_fma(k, d2, acc1);
+  acc2 = __builtin_fma(-k, d1, acc2);
+  d1o = d1;
+  d2o = d2;
+  t_out = t;
+}
+
+// FP32 pair term with the FP32 coincidence guard of collision_gpu.jl:187.
+__device__ __forceinline__ void pair_term_f32(float vg1, float vg2, float Gg1, float Gg2, float av1, float av2,
+                                              float ag1, float ag2, float aw, float& b1, float& b2) {
+  const float d1 = vg1 - av1;
+  const float d2 = vg2 - av2;
+  const float q = __builtin_fmaf(d2, d2, d1 * d1);
+  const float y = __builtin_amdgcn_rsqf(__builtin_fmaxf(q, 1e-12f));
+  const float g1 = ag1 - Gg1;
+  const float g2 = ag2 - Gg2;
+  const float c = __builtin_fmaf(g1, d2, -(g2 * d1));
+  float t = (aw * y) * (y * y);
+  t = q < 1e-12f ? 0.0f : t;
+  const float kk = t * c;
+  b1 = __builtin_fmaf(kk, d2, b1);
+  b2 = __builtin_fmaf(-kk, d1, b2);
+}
+
+// FP32 pair sums are compensated blockwise: BLK terms are summed in FP32, then
+// folded into a (hi, lo) pair with Knuth's TwoSum, so the error of the
+// per-marker sum does not grow with N.
+__device__ __forceinline__ void two_sum_fold(float& hi, float& lo, float x) {
+  const float s = hi + x;
+  const float bp = s - hi;
+  const float e = (hi - (s - bp)) + (x - bp);
+  hi = s;
+  lo += e;
 }
 
 }  // namespace collop

@@ -26,30 +26,6 @@ namespace collop {
 namespace {
 
 constexpr int kSolveThreads = 1024;
-constexpr int kMaxBp = 512;  // breakpoints per direction staged in LDS
-
-__device__ __forceinline__ void stage_bp(const MeshView& m, double* sb1, double* sb2) {
-  for (int i = threadIdx.x; i < m.nb1; i += blockDim.x) sb1[i] = m.bp1[i];
-  for (int i = threadIdx.x; i < m.nb2; i += blockDim.x) sb2[i] = m.bp2[i];
-}
-
-struct Located {
-  int i, j;
-  double xi1, xi2, h1, h2;
-};
-
-// locate_particle (MantisWrappers.jl:168-179) against LDS breakpoints
-__device__ __forceinline__ bool locate(const MeshView& m, const double* sb1, const double* sb2, double x, double y,
-                                       Located& L) {
-  if (!in_box(x, y, m.lo1, m.hi1, m.lo2, m.hi2)) return false;
-  L.i = cell_of(sb1, m.nb1, m.uniform1, m.lo1, m.inv_h1, x);
-  L.j = cell_of(sb2, m.nb2, m.uniform2, m.lo2, m.inv_h2, y);
-  L.h1 = sb1[L.i + 1] - sb1[L.i];
-  L.h2 = sb2[L.j + 1] - sb2[L.j];
-  L.xi1 = (x - sb1[L.i]) / L.h1;
-  L.xi2 = (y - sb2[L.j]) / L.h2;
-  return true;
-}
 
 // ---------------------------------------------------------------------------
 // Scatter.  grid = (blocks, fields).  Dynamic LDS: n_dofs doubles.
@@ -123,36 +99,6 @@ __global__ __launch_bounds__(kScatterThreads) void k_scatter(MeshView m, const d
       mom[3 * blockIdx.x + 1] = s2;
       mom[3 * blockIdx.x + 2] = s3;
     }
-  }
-}
-
-// ---------------------------------------------------------------------------
-// Banded Cholesky solve along one direction of the coefficient array held in
-// LDS.  chol = {1/L_ii, L_i,i-1, L_i,i-2}; element (i) of line `line` sits at
-// X[line * lstride + i * istride].
-__device__ __forceinline__ void band_solve_line(double* X, int n, int istride, const double* __restrict__ chol) {
-  double y2 = 0.0, y1 = 0.0;  // y_{i-2}, y_{i-1}
-  for (int i = 0; i < n; ++i) {
-    const double l1 = chol[3 * i + 1], l2 = chol[3 * i + 2];
-    double s = X[i * istride];
-    s -= l1 * y1;
-    s -= l2 * y2;
-    const double yi = s * chol[3 * i];
-    X[i * istride] = yi;
-    y2 = y1;
-    y1 = yi;
-  }
-  double x1 = 0.0, x2 = 0.0;  // x_{i+1}, x_{i+2}
-  for (int i = n - 1; i >= 0; --i) {
-    const double u1 = i + 1 < n ? chol[3 * (i + 1) + 1] : 0.0;
-    const double u2 = i + 2 < n ? chol[3 * (i + 2) + 2] : 0.0;
-    double s = X[i * istride];
-    s -= u1 * x1;
-    s -= u2 * x2;
-    const double xi = s * chol[3 * i];
-    X[i * istride] = xi;
-    x2 = x1;
-    x1 = xi;
   }
 }
 

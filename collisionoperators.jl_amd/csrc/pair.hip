@@ -26,41 +26,6 @@ namespace {
 
 constexpr int kThreads = 256;
 
-template <int NEWTON>
-__device__ __forceinline__ double rsqrt_f64(double q) {
-  double y = __builtin_amdgcn_rsq(q);
-  const double h = 0.5 * q;
-#pragma unroll
-  for (int i = 0; i < NEWTON; ++i) {
-    const double e = __builtin_fma(-h, y * y, 0.5);
-    y = __builtin_fma(y, e, y);
-  }
-  return y;
-}
-
-// One ordered pair (gamma in registers, alpha uniform).  Returns the weight
-// factor t = w_a / |d|^3 (0 for masked pairs) through `t_out` for the metric.
-template <int NEWTON>
-__device__ __forceinline__ void pair_term_f64(double vg1, double vg2, double Gg1, double Gg2, double av1, double av2,
-                                              double ag1, double ag2, double aw, double& acc1, double& acc2,
-                                              double& d1o, double& d2o, double& t_out) {
-  const double d1 = vg1 - av1;
-  const double d2 = vg2 - av2;
-  const double q = __builtin_fma(d2, d2, d1 * d1);
-  const double y = rsqrt_f64<NEWTON>(__builtin_fmax(q, 1e-24));
-  const double g1 = ag1 - Gg1;
-  const double g2 = ag2 - Gg2;
-  const double c = __builtin_fma(g1, d2, -(g2 * d1));
-  double t = (aw * y) * (y * y);
-  t = q < 1e-24 ? 0.0 : t;  // coincident-pair guard (functions.jl:224)
-  const double k = t * c;
-  acc1 = __builtin_fma(k, d2, acc1);
-  acc2 = __builtin_fma(-k, d1, acc2);
-  d1o = d1;
-  d2o = d2;
-  t_out = t;
-}
-
 template <int K, int NEWTON, bool METRIC>
 __global__ __launch_bounds__(kThreads) void k_pair_f64(const Src64* __restrict__ src, int N, int S, int chunk,
                                                         double* __restrict__ part) {
@@ -120,17 +85,6 @@ __global__ __launch_bounds__(kThreads) void k_pair_f64(const Src64* __restrict__
   }
 }
 
-// FP32 pair math with blocked two-sum compensation: BLK terms are summed in
-// FP32, then folded into a (hi, lo) pair with Knuth's TwoSum, so the error of
-// the per-marker sum does not grow with N.
-__device__ __forceinline__ void two_sum_fold(float& hi, float& lo, float x) {
-  const float s = hi + x;
-  const float bp = s - hi;
-  const float e = (hi - (s - bp)) + (x - bp);
-  hi = s;
-  lo += e;
-}
-
 template <int K, int BLK>
 __global__ __launch_bounds__(kThreads) void k_pair_f32c(const Src32* __restrict__ src, int N, int S, int chunk,
                                                          double* __restrict__ part) {
@@ -162,20 +116,7 @@ __global__ __launch_bounds__(kThreads) void k_pair_f32c(const Src32* __restrict_
     for (int a = ab; a < ae; ++a) {
       const float av1 = src[a].v1, av2 = src[a].v2, ag1 = src[a].g1, ag2 = src[a].g2, aw = src[a].w;
 #pragma unroll
-      for (int k = 0; k < K; ++k) {
-        const float d1 = vg1[k] - av1;
-        const float d2 = vg2[k] - av2;
-        const float q = __builtin_fmaf(d2, d2, d1 * d1);
-        const float y = __builtin_amdgcn_rsqf(__builtin_fmaxf(q, 1e-12f));
-        const float g1 = ag1 - Gg1[k];
-        const float g2 = ag2 - Gg2[k];
-        const float c = __builtin_fmaf(g1, d2, -(g2 * d1));
-        float t = (aw * y) * (y * y);
-        t = q < 1e-12f ? 0.0f : t;  // FP32 guard of collision_gpu.jl:187
-        const float kk = t * c;
-        b1[k] = __builtin_fmaf(kk, d2, b1[k]);
-        b2[k] = __builtin_fmaf(-kk, d1, b2[k]);
-      }
+      for (int k = 0; k < K; ++k) pair_term_f32(vg1[k], vg2[k], Gg1[k], Gg2[k], av1, av2, ag1, ag2, aw, b1[k], b2[k]);
     }
 #pragma unroll
     for (int k = 0; k < K; ++k) {

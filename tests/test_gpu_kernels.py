@@ -58,7 +58,7 @@ def test_collision_f64_tiles(c1, tile):
         out = np.zeros_like(ref)
         collop.compute_collision_gpu(c1["ws"], out, c1["v"], c1["w"], c1["G"])
     finally:
-        collop.set_pair_tile(c1["ws"], 1, 0, 2)
+        collop.set_pair_tile(c1["ws"], 1, 0, 1)
     assert relerr(out, ref) <= 1e-12
 
 

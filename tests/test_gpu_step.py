@@ -123,9 +123,9 @@ def test_step_deterministic(gpu_ctx):
     np.testing.assert_array_equal(out[0], out[1])
 
 
-def test_state_errors(gpu_ctx):
+def test_state_errors():
     bp = uniform_bp(-6, 6, 16)
-    ws = collop.Workspace(bp, bp, ctx=gpu_ctx)
+    ws = collop.Workspace(bp, bp)  # a fresh context: no warm-start model loaded
     v, w = collop.markers("gaussian", 64, 1)
     with pytest.raises(collop.CollopError):
         collop.Simulation(ws, v, -w)
