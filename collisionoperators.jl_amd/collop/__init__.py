@@ -14,5 +14,5 @@ from .hooks import (compute_collision_gpu, compute_collision_gpu32, compute_coll
 from .ic import initial_markers, markers  # noqa: F401
 from .mesh import Workspace, build_workspace, extraction_p2c1  # noqa: F401
 from .params import CONFIGS, SimParameters, config, parse_overrides  # noqa: F401
-from .sim import Simulation, profile, set_pair_tile  # noqa: F401
+from .sim import Ensemble, Simulation, profile, set_pair_tile  # noqa: F401
 from .warmstart import WarmStart, glorot_init  # noqa: F401

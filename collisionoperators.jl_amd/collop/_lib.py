@@ -126,6 +126,13 @@ SIGNATURES = [
     ("collop_mlp_dataset_read", C.c_int, [_vp, C.c_int64, C.c_int64, _fp]),
     ("collop_mlp_train", C.c_int, [_vp, C.c_int, C.c_int, C.c_double, C.c_uint64, _dp]),
     ("collop_mlp_eval", C.c_int, [_vp, _fp, C.c_int64, C.c_int, _fp, _dp, _fp]),
+    ("collop_ensemble_set", C.c_int, [_vp, C.c_int, C.POINTER(C.c_int64), _dp, _dp]),
+    ("collop_ensemble_get", C.c_int, [_vp, _dp]),
+    ("collop_ensemble_coeffs", C.c_int, [_vp, _dp]),
+    ("collop_ensemble_step", C.c_int, [_vp, C.POINTER(StepParams), C.c_int, C.POINTER(StepStats)]),
+    ("collop_ensemble_slices", C.c_int, [C.c_int64]),
+    ("collop_ensemble_set_tile", C.c_int, [_vp, C.c_int, C.c_int, C.c_int]),
+    ("collop_ensemble_tune", C.c_int, [_vp, C.POINTER(C.c_int), C.POINTER(C.c_int), _dp]),
     ("collop_set_pair_tile", C.c_int, [_vp, C.c_int, C.c_int, C.c_int]),
     ("collop_profile", C.c_int, [_vp, C.c_int, _dp, C.POINTER(C.c_long)]),
 ]

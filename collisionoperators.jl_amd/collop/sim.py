@@ -61,3 +61,81 @@ def profile(ws: Workspace, enable: bool):
 def set_pair_tile(ws: Workspace, gamma_per_lane: int = 1, slices: int = 0, newton: int = 1):
     """Pin the pair-sum tile (slices = 0: automatic); see collop_set_pair_tile."""
     ws.ctx.check(ws.ctx.lib.collop_set_pair_tile(ws.ctx.h, gamma_per_lane, slices, newton))
+
+
+class Ensemble:
+    """Independent runs with uneven N on one workspace, advanced together
+    (collop_ensemble_*).  Each run solves exactly as a :class:`Simulation` of its
+    own markers would with ``set_pair_tile(ws, 1, Ensemble.slices(N), 1)``."""
+
+    def __init__(self, ws: Workspace, runs):
+        self.ws = ws
+        vs, ws_ = [], []
+        for v, w in runs:
+            v = np.asfortranarray(np.asarray(v, dtype=np.float64))
+            w = np.ascontiguousarray(np.asarray(w, dtype=np.float64))
+            if v.ndim != 2 or v.shape[1] != 2 or w.shape != (v.shape[0],):
+                raise ValueError("each run needs v (N, 2) and w (N,)")
+            vs.append(v)
+            ws_.append(w)
+        self.n = np.array([v.shape[0] for v in vs], dtype=np.int64)
+        vcat = np.concatenate([v.ravel(order="F") for v in vs]) if vs else np.zeros(0)
+        wcat = np.concatenate(ws_) if ws_ else np.zeros(0)
+        ws.activate()
+        ctx = ws.ctx
+        ctx.check(ctx.lib.collop_ensemble_set(ctx.h, len(vs), self.n.ctypes.data_as(C.POINTER(C.c_int64)),
+                                              _lib.dptr(vcat), _lib.dptr(wcat)))
+        ctx.ens_owner = self
+        self._mesh = ws._token
+
+    @property
+    def n_runs(self) -> int:
+        return len(self.n)
+
+    @staticmethod
+    def slices(n: int) -> int:
+        return int(_lib.load().collop_ensemble_slices(int(n)))
+
+    def _own(self):
+        ctx = self.ws.ctx
+        if getattr(ctx, "ens_owner", None) is not self or getattr(ctx, "mesh_owner", None) is not self._mesh:
+            raise _lib.CollopError(_lib.ERR_NO_STATE, "this ensemble's device state was replaced (another ensemble "
+                                   "or mesh on the same context)")
+
+    def step(self, params: _lib.StepParams, n_steps: int = 1) -> list[list[dict]]:
+        """n_steps steps of every run; returns stats[step][run]."""
+        self._own()
+        R = self.n_runs
+        stats = (_lib.StepStats * max(n_steps * R, 1))()
+        self.ws.ctx.check(self.ws.ctx.lib.collop_ensemble_step(self.ws.ctx.h, C.byref(params), n_steps, stats))
+        return [[stats[s * R + r].as_dict() for r in range(R)] for s in range(n_steps)]
+
+    def markers(self) -> list[np.ndarray]:
+        self._own()
+        flat = np.empty(2 * int(self.n.sum()))
+        self.ws.ctx.check(self.ws.ctx.lib.collop_ensemble_get(self.ws.ctx.h, _lib.dptr(flat)))
+        out, o = [], 0
+        for n in self.n:
+            out.append(flat[o:o + 2 * n].reshape((n, 2), order="F").copy(order="F"))
+            o += 2 * n
+        return out
+
+    def coeffs(self) -> np.ndarray:
+        self._own()
+        fc = np.empty((self.n_runs, self.ws.n_dofs))
+        self.ws.ctx.check(self.ws.ctx.lib.collop_ensemble_coeffs(self.ws.ctx.h, _lib.dptr(fc)))
+        return fc
+
+    def set_tile(self, gamma_tile: int = 256, waves_per_cu: int = 8, use_graph: bool = True):
+        self._own()
+        self.ws.ctx.check(self.ws.ctx.lib.collop_ensemble_set_tile(self.ws.ctx.h, gamma_tile, waves_per_cu,
+                                                                    int(use_graph)))
+
+    def tune(self):
+        """Autotune the persistent pair sum; returns (gamma_tile, waves_per_cu, ms[4][4])."""
+        self._own()
+        bt, wv = C.c_int(), C.c_int()
+        ms = np.zeros(16)
+        self.ws.ctx.check(self.ws.ctx.lib.collop_ensemble_tune(self.ws.ctx.h, C.byref(bt), C.byref(wv),
+                                                                _lib.dptr(ms)))
+        return bt.value, wv.value, ms.reshape(4, 4)

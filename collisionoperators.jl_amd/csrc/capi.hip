@@ -20,6 +20,7 @@
 #include <vector>
 
 #include "../../include/collop.h"
+#include "anderson.hpp"
 #include "collop_ctx.hpp"
 
 using namespace collop;
@@ -314,6 +315,8 @@ int collop_set_mesh(collop_ctx* ctx, const double* bp1, int nb1, const double* b
   (void)hipStreamSynchronize(ctx->stream);
   free_mesh(ctx);
   ctx->has_state = false;
+  ensemble_free(ctx->ens);  // its tables and captured graph refer to the old mesh
+  ctx->ens = nullptr;
 
   std::vector<double> v1(bp1, bp1 + nb1), v2(bp2, bp2 + nb2);
   std::vector<double> ch1, ch2;
@@ -680,6 +683,10 @@ int picard_map_dev(collop_ctx* ctx, const collop_step_params& p, int kind, const
   return COLLOP_OK;
 }
 
+}  // namespace
+
+namespace collop {
+
 // Small SPD solve (Cholesky; Julia's factorize picks it for the exactly
 // symmetric ATA), with partial-pivot LU as the fallback.
 void small_solve(int n, const double* A0, const double* b0, double* x) {
@@ -737,6 +744,10 @@ void small_solve(int n, const double* A0, const double* b0, double* x) {
     x[i] = t / A[i * n + i];
   }
 }
+
+}  // namespace collop
+
+namespace {
 
 int copy_d2d(collop_ctx* ctx, DBuf& dst, const DBuf& src, size_t bytes) {
   HIPC(hipMemcpyAsync(dst.p, src.p, bytes, hipMemcpyDeviceToDevice, ctx->stream));
@@ -821,145 +832,35 @@ int step_once(collop_ctx* ctx, const collop_step_params& p, collop_step_stats* o
   }
 
   // ---- Anderson (solver.jl:300-423)
-  const int mw = p.m_anderson;
-  int history = 0, slot = 0, n_restart = 0;
-  double nrm_r0 = 0.0, nrm_r = 0.0, nrm_best = std::numeric_limits<double>::infinity();
-  double nrm_best_window = std::numeric_limits<double>::infinity();
-  double ATA[kMaxWindow * kMaxWindow] = {0};
-  bool use32 = p.precision == COLLOP_PREC_F32C;
-  int iters32 = 0;
-  const double floor_factor = p.f32_floor_factor > 0.0 ? p.f32_floor_factor : 16.0;
-  int exit_reason = 2, k_done = p.max_iter;
-  double res_out = 0.0;
+  AndersonRun an;
+  an.start(p);
   // The best iterate starts as the initial guess (solver.jl:331).
   if ((r = copy_d2d(ctx, ctx->s_best, ctx->s_v1, n2b))) return r;
-  int since_reset = 0;  // maps since the window was last (re)started
-  double best32 = std::numeric_limits<double>::infinity();
-  int since_best32 = 0;
-  const bool trace = std::getenv("COLLOP_TRACE") != nullptr;
-  bool rebase = false;  // this map re-evaluates the last FP32 iterate in FP64
-  for (int k = 1; k <= p.max_iter; ++k) {
-    const bool anderson_step = p.use_anderson && since_reset > 0 && !rebase;
-    const int col = anderson_step ? slot % mw : -1;
-    const int hist_new = anderson_step ? std::min(history + 1, mw) : (rebase ? history : 0);
-    const int kind = use32 ? 1 : 0;
-    if ((r = picard_map_dev(ctx, p, kind, P<double>(ctx->s_v1), col, hist_new))) return r;
-    since_reset++;
-    const double* st = ctx->h_stat;
-    nrm_r = std::sqrt(st[0]);
-    const double nrm_v1 = std::sqrt(st[1]);
-    if (!std::isfinite(nrm_r) || !std::isfinite(nrm_v1))
-      return fail(ctx, COLLOP_ERR_NONFINITE, "non-finite residual in Picard map " + std::to_string(k));
-    if (k == 1) nrm_r0 = nrm_r;
-    const double eff_tol = std::max(p.tol * (nrm_v1 + 1e-30), p.abs_floor);
-    if (use32) {
-      iters32++;
-      // FP32 maps run until their own rounding floor (a few FP32 ulps of the
-      // step increment |Gv - v0|), or until four maps bring no new best
-      // residual; FP64 maps then finish, so every accepted iterate and the
-      // exit map are FP64.  Switching well inside the basin keeps the FP64
-      // maps on the root the plain FP64 solve finds (the implicit system can
-      // have several roots at large dt).
-      const double floor32 = floor_factor * 5.96e-8 * std::sqrt(st[2]);
-      if (nrm_r < best32) {
-        best32 = nrm_r;
-        since_best32 = 0;
-      } else {
-        since_best32++;
-      }
-      const bool switch_now = nrm_r < std::max(floor32, eff_tol) || since_best32 >= 4 || k >= p.max_iter / 2;
-      if (trace)
-        std::fprintf(stderr, "  [f32] k=%d |r|=%.3e floor=%.3e best=%.3e\n", k, nrm_r, floor32, best32);
-      if (switch_now && p.f32_keep_window && p.use_anderson && history > 0) {
-        // Keep the secant window built from FP32 maps (its columns are
-        // differences far above FP32 rounding) and re-evaluate this iterate
-        // with an FP64 map before the next update: no column mixes the two.
-        use32 = false;
-        rebase = true;
-        continue;
-      }
-      if (switch_now) {
-        use32 = false;
-        history = 0;
-        slot = 0;
-        since_reset = 0;
-        launch_update(P<double>(ctx->s_v1), P<double>(ctx->s_Gv), P<double>(ctx->s_dG), nullptr, 0, p.damping, n,
-                      ctx->stream);
-        CHECK_LAUNCH();
-        std::swap(ctx->s_rp, ctx->s_r);
-        std::swap(ctx->s_Gp, ctx->s_Gv);
-        continue;
-      }
-    } else {
-      if (trace) std::fprintf(stderr, "  [f64] k=%d |r|=%.3e tol=%.3e\n", k, nrm_r, eff_tol);
-      if (nrm_r < nrm_best) {
-        nrm_best = nrm_r;
-        if ((r = copy_d2d(ctx, ctx->s_best, p.exit_picard_step ? ctx->s_Gv : ctx->s_v1, n2b))) return r;
-      }
-      if (nrm_r < eff_tol) {
-        if (p.exit_picard_step) std::swap(ctx->s_v1, ctx->s_Gv);
-        exit_reason = 0;
-        k_done = k;
-        res_out = nrm_r;
-        break;
-      }
-      if (k > p.stag_window && k % p.stag_window == 0) {
-        const double rel = (nrm_best_window - nrm_best) / (nrm_best_window + 1e-30);
-        if (rel < p.stag_rel_tol) {
-          std::swap(ctx->s_v1, ctx->s_best);
-          exit_reason = 1;
-          k_done = k;
-          res_out = nrm_best;
-          break;
-        }
-        nrm_best_window = nrm_best;
-      }
+  const char* trace = std::getenv("COLLOP_TRACE") ? "run" : nullptr;
+  while (!an.done) {
+    const AndersonPlan pl = an.plan();
+    if ((r = picard_map_dev(ctx, p, pl.kind, P<double>(ctx->s_v1), pl.col, pl.hist))) return r;
+    AndersonAct act;
+    if (an.after_map(ctx->h_stat, act, trace) != COLLOP_OK)
+      return fail(ctx, COLLOP_ERR_NONFINITE, "non-finite residual in Picard map " + std::to_string(an.maps()));
+    if (act.copy_best && (r = copy_d2d(ctx, ctx->s_best, p.exit_picard_step ? ctx->s_Gv : ctx->s_v1, n2b))) return r;
+    if (act.exit == kConverged) {
+      if (p.exit_picard_step) std::swap(ctx->s_v1, ctx->s_Gv);
+      break;
     }
-    bool just_restarted = false;
-    if (!use32 && k > 1 && nrm_r > p.restart_factor * nrm_best) {
-      history = 0;
-      slot = 0;
-      n_restart++;
-      just_restarted = true;
+    if (act.exit != kContinue) {
+      std::swap(ctx->s_v1, ctx->s_best);
+      break;
     }
-    const double beta = k > p.damp_decay_start ? p.damping * p.damp_decay_factor : p.damping;
-    const bool rebased = rebase;
-    rebase = false;
-    if (just_restarted || (col < 0 && !(rebased && history > 0))) {
-      launch_update(P<double>(ctx->s_v1), P<double>(ctx->s_Gv), P<double>(ctx->s_dG), nullptr, 0, beta, n,
+    if (act.update) {
+      launch_update(P<double>(ctx->s_v1), P<double>(ctx->s_Gv), P<double>(ctx->s_dG), act.gamma, act.h, act.beta, n,
                     ctx->stream);
       CHECK_LAUNCH();
-    } else {
-      if (!rebased) {
-        slot = slot % mw + 1;
-        history = hist_new;
-        for (int j = 0; j < history; ++j) {
-          ATA[col * kMaxWindow + j] = st[3 + j];
-          ATA[j * kMaxWindow + col] = st[3 + j];
-        }
-      }
-      const int h = history;
-      double A[kMaxWindow * kMaxWindow], b[kMaxWindow], g[kMaxWindow];
-      double dm = 0.0;
-      for (int a = 0; a < h; ++a) {
-        for (int c = 0; c < h; ++c) A[a * h + c] = ATA[a * kMaxWindow + c];
-        b[a] = st[3 + kMaxWindow + a];
-        dm += A[a * h + a];
-      }
-      dm /= h;
-      const double l2 = p.reg_factor * dm + 1e-30;
-      for (int a = 0; a < h; ++a) A[a * h + a] += l2;
-      small_solve(h, A, b, g);
-      launch_update(P<double>(ctx->s_v1), P<double>(ctx->s_Gv), P<double>(ctx->s_dG), g, h, beta, n, ctx->stream);
-      CHECK_LAUNCH();
     }
-    // rp <- r, Gp <- Gv (pointer rotation instead of copies)
-    std::swap(ctx->s_rp, ctx->s_r);
-    std::swap(ctx->s_Gp, ctx->s_Gv);
-  }
-  if (exit_reason == 2) {
-    std::swap(ctx->s_v1, ctx->s_best);
-    res_out = nrm_best;
+    if (act.rotate) {  // rp <- r, Gp <- Gv (pointer rotation instead of copies)
+      std::swap(ctx->s_rp, ctx->s_r);
+      std::swap(ctx->s_Gp, ctx->s_Gv);
+    }
   }
   // v <- v1 and the new projection / entropy / moments (main.jl:296-303)
   std::swap(ctx->s_v, ctx->s_v1);
@@ -974,12 +875,12 @@ int step_once(collop_ctx* ctx, const collop_step_params& p, collop_step_stats* o
   HIPC(hipStreamSynchronize(ctx->stream));
   prof_collect(ctx);
   if (out) {
-    out->iters = k_done;
-    out->iters_f32 = iters32;
-    out->n_restart = n_restart;
-    out->exit_reason = exit_reason;
-    out->residual = res_out;
-    out->r0 = nrm_r0;
+    out->iters = an.k_done;
+    out->iters_f32 = an.iters32;
+    out->n_restart = an.n_restart;
+    out->exit_reason = an.exit_reason;
+    out->residual = an.res_out;
+    out->r0 = an.nrm_r0;
     out->entropy = sc[0];
     out->energy = 0.5 * sc[5];
     out->momentum[0] = sc[3];

@@ -5,6 +5,60 @@
 
 namespace collop {
 
+// Block counts of the grid-stride scatter and gather launches for N markers.
+__host__ __device__ inline int scatter_blocks(int N) {
+  const int b = (N + kScatterThreads * 4 - 1) / (kScatterThreads * 4);
+  return b < 1 ? 1 : (b > 128 ? 128 : b);
+}
+__host__ __device__ inline int gather_blocks(int N) {
+  const int b = (N + 256 * 2 - 1) / (256 * 2);
+  return b < 1 ? 1 : (b > 512 ? 512 : b);
+}
+
+// ---- batched ensemble: run tables ---------------------------------------------
+// Runs are concatenated: run r holds markers [off, off + n) of arrays whose
+// component stride is NT (off is a multiple of 256).  Each batched kernel gives
+// run r the same number of blocks, walking the markers in the same order, as
+// the single-run kernel at N = n, so per-run results match it bit for bit.
+struct EnsRun {
+  int n, off;
+  int S;             // pair slices
+  int sblk0, nsb;    // scatter blocks: first row, count
+  int gblk0, ngb;    // gather blocks
+  int pblk0, npb;    // blocks of 256 markers
+  int eblk0, neb;    // blocks of 256 over the flattened 2n components
+  long long poff;    // pair partials [S][2n] at part + poff
+  double scale, inv_scale;  // scatter fixed-point scale 2^62 / sum|w| and its inverse
+};
+
+// Per-run control of one Anderson iteration (uploaded before each replay).
+struct RunCtl {
+  int active;       // the run maps in this iteration
+  int kind;         // 0 FP64 pair sum, 1 FP32c
+  int col;          // Anderson column written by the map (-1: none)
+  int hist;         // window columns dotted with the new column / residual
+  int cur;          // ping-pong buffer the map writes (r, Gv); 1 - cur holds (rp, Gp)
+  int upd;          // before the map: 0 none, 1 v1 <- beta (Gv - dG gamma) + (1 - beta) v1,
+                    // 2 v1 <- Gv, 3 v1 <- best   (Gv of the previous map: buffer 1 - cur)
+  int upd_hist;     // columns of the upd = 1 step
+  int copy_best;    // before the update: best <- Gv (exit_picard) or v1
+  int exit_picard;
+  int pad;
+  double beta;
+  double gamma[kMaxWindow];
+};
+
+struct EnsView {
+  int R, NT;
+  int n_sblk, n_gblk, n_pblk, n_eblk;
+  const EnsRun* runs;
+  const RunCtl* ctl;  // nullptr: every run takes part (predictor, projection)
+  const int2* sblk;   // {run, local block} per block of each family
+  const int2* gblk;
+  const int2* pblk;
+  const int2* eblk;
+};
+
 // ---- pair sum (pair.hip) ----------------------------------------------------
 struct PairTile {
   int gamma_per_lane;  // 1 or 2 targets per lane
@@ -33,7 +87,7 @@ void launch_slice_sum(const double* part, int S, int64_t len, double* out, hipSt
 // 2^62 / w_abs_sum (w_abs_sum = sum |w|).  When mom != nullptr the blocks of
 // field 0 also write partial sums (w v1, w v2, w |x|^2) of the scattered
 // positions to mom[blocks][3].
-int scatter_blocks(int N);
+double scatter_scale(double w_abs_sum);  // 2^62 / W (fixed-point scale of the scatter)
 void launch_scatter(const MeshView& m, const double* v0, const double* vin, const double* w, int N, int fields,
                     int grid_fields, double w_abs_sum, double* vmid, double* dv, long long* part, double* mom,
                     hipStream_t st);
@@ -57,9 +111,29 @@ void launch_quad2(const MeshView& m, const double* iota, double* r, hipStream_t 
 void launch_sum_rows(const double* part, int n, int count, int istride, int jstride, double* out, hipStream_t st);
 // Gather: G (mode 0), or dS = -w G plus partial (dv.dS, |dv|^2) (mode 1),
 // or clamped log-gradient (mode 2), or (f, |grad f|) (mode 3).
-int gather_blocks(int N);
 void launch_gather(const MeshView& m, const double* x, const double* coeff, const double* w, const double* dv,
                    int N, int mode, double fs_floor, double g_max, double* out, double* dpart, hipStream_t st);
+
+// Batched forms (one launch for every run; see EnsView):
+//   scatter:  part [field][n_sblk][n_dofs], mom [n_sblk][3]
+//   solve:    grid (R, fields), from scatter rows or vec slot in_slot, to vec slot
+//             out_slot (or fc[r] when out_slot < 0); vec is [R][4][n_dofs]
+//   quad1:    coefficients c + r * rstride + field * n_dofs; iota [R][points],
+//             spart [R][2][quad blocks];  quad2: vec slot 2 <- r(iota)
+//   sum_rows: out[r * ostride + c] = sum_{j < cnt[r]} part[row0[r] + c istride + j jstride]
+//   gather:   coefficients vec slot `slot`, dpart [n_gblk][2]
+void launch_scatter_b(const MeshView& m, const EnsView& E, const double* v0, const double* vin, const double* w,
+                      int fields, int grid_fields, double* vmid, double* dv, long long* part, double* mom,
+                      hipStream_t st);
+void launch_solve_b(const MeshView& m, const EnsView& E, const long long* part, int fields, int in_slot, int out_slot,
+                    double* vec, double* fc, hipStream_t st);
+void launch_quad1_b(const MeshView& m, const EnsView& E, const double* c, int rstride, int fields, int use_logsq,
+                    double* iota, double* spart, hipStream_t st);
+void launch_quad2_b(const MeshView& m, const EnsView& E, const double* iota, double* vec, hipStream_t st);
+void launch_sum_rows_b(const EnsView& E, const double* part, const int* row0, const int* cnt, int count, int istride,
+                       int jstride, double* out, int ostride, hipStream_t st);
+void launch_gather_b(const MeshView& m, const EnsView& E, const double* x, const double* vec, int slot,
+                     const double* w, const double* dv, int mode, double* out, double* dpart, hipStream_t st);
 
 // ---- solver side (step.hip) -------------------------------------------------
 // Gonzalez lambda from reduced partials, then G_eff and the pair-source pack.
@@ -84,6 +158,44 @@ void launch_update(double* v1, const double* Gv, const double* dG, const double*
 // out = v + dt * (sum over slices of part)  (the Euler predictor, main.jl:245)
 void launch_euler(const double* part, int S, const double* v, double dt, int N, double* out, double* dotv,
                   hipStream_t st);
+
+// Batched forms of the solver-side kernels.  scal is [R][8]: S0, S1, lambda,
+// P1, P2, sum w |v|^2.
+struct EnsBufs {
+  const double* v0;    // state
+  double* v1;          // iterate
+  double* R[2];        // residual ping-pong
+  double* GV[2];       // map value ping-pong
+  double* dF;          // [kMaxWindow][2][NT]
+  double* dG;
+  double* best;
+  const double* part;  // pair partials (run r at poff)
+  double* statp;       // [n_eblk][kStatWidth]
+};
+void launch_geff_pack_b(const EnsView& E, const double* vmid, const double* dS, const double* dv, const double* w,
+                        const double* dpart, double* scal, int use_gonzalez, double lo1, double hi1, double lo2,
+                        double hi2, Src64* s64, Src32* s32, hipStream_t st);
+void launch_epilogue_b(const EnsView& E, const EnsBufs& B, double dt, hipStream_t st);
+void launch_update_b(const EnsView& E, const EnsBufs& B, hipStream_t st);
+void launch_euler_b(const EnsView& E, const double* part, const double* v, double dt, double* out, double* dotv,
+                    hipStream_t st);
+void launch_copy_b(const EnsView& E, const double* src, double* dst, hipStream_t st);
+
+// ---- batched pair sum (pair.hip) ---------------------------------------------------
+// Persistent launch over a tile list: tile t = (run, first target, slice),
+// `bt` targets per workgroup (64, 128, 256 or 512), `grid` workgroups striding
+// over the list.  force_f64: every run, FP64 (predictor); otherwise the runs
+// active in E.ctl, in their own precision.
+struct EnsTiles {
+  const int4* tiles;  // {run, first target, slice, -}
+  int ntiles;
+  int bt;
+  int grid;
+};
+void launch_pack_b(const EnsView& E, const double* v, const double* G, const double* w, double lo1, double hi1,
+                   double lo2, double hi2, Src64* s64, Src32* s32, hipStream_t st);
+void launch_pair_b(const EnsView& E, const EnsTiles& T, const Src64* s64, const Src32* s32, int force_f64,
+                   double* part, hipStream_t st);
 
 // ---- warm-start MLP (mlp.hip) --------------------------------------------------
 constexpr int kMlpParams = 4738;

@@ -1,572 +1,654 @@
-// ensemble.hip -- a batched ensemble of independent Landau runs with uneven N,
-// advanced together: every Anderson iteration is one replay of a captured
-// hipGraph whose pair sum is a single persistent launch over all runs.
+// ensemble.hip -- a batched ensemble of independent Landau runs with uneven N
+// on one mesh, advanced together (collop_ensemble_*).
 //
-// Layout: the markers of all runs are concatenated (run r occupies
-// [off_r, off_r + n_r), off_r a multiple of 256, so no 256-thread workgroup
-// straddles two runs) in one SoA array v[2][NT].  Per-run quantities live in
-// small per-run arrays; a per-run control block (RunCtl, uploaded before each
-// replay) tells every kernel whether the run maps in this iteration, in which
-// precision, which Anderson column it writes, which of the two ping-pong
-// residual buffers is current, and which update (Anderson step, damped step,
-// exit copy) to apply to the run's iterate first.  Kernel arguments therefore
-// never change between replays.
+// Layout in HBM: the runs' markers are concatenated, run r at [off_r, off_r + n_r)
+// with off_r a multiple of 256, in SoA arrays whose component stride is NT
+// (v1 of every run, then v2 of every run).  Per-run tables (EnsRun) give each
+// batched kernel the block count the single-run kernel would use at N = n_r,
+// so each run's arithmetic is the single-run arithmetic.
 //
-// The pair kernel walks a static tile list (run, target block, source slice)
-// sorted longest-first (LPT) with a grid-stride loop; the γ-tile size (threads
-// per workgroup, one target per lane) and the number of resident waves per CU
-// (the grid size) come from the autotuner (collop_ensemble_tune).
+// One Anderson iteration of the whole ensemble is one replay of a captured
+// hipGraph: upload the per-run control blocks (RunCtl) -> per-run update of the
+// iterate -> scatter / solve / quadrature / gather / G_eff of every active run ->
+// ONE persistent pair-sum launch over the tiles of every active run (FP32c or
+// FP64 per run) -> epilogue -> per-run statistics -> download.  The host then
+// advances each run's Anderson state machine (the same one the single-run step
+// uses) and writes the next control blocks.  Kernel arguments never change
+// between replays; everything that varies lives in the control blocks.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
 #include <chrono>
 #include <cmath>
-#include <cstdlib>
 #include <cstring>
-#include <limits>
 #include <numeric>
+#include <string>
 #include <vector>
 
+#include "../../include/collop.h"
+#include "anderson.hpp"
 #include "collop_ctx.hpp"
 
 namespace collop {
 
-struct RunCtl {
-  int active;       // the run maps in this iteration
-  int kind;         // 0 FP64 maps, 1 FP32c maps
-  int col;          // Anderson column written by the epilogue (-1: none)
-  int hist;         // columns the epilogue dots with
-  int cur;          // ping-pong buffer the map writes (r, Gv); the other is (rp, Gp)
-  int upd;          // update before the map: 0 none, 1 Anderson/damped step, 2 v1 <- Gv, 3 v1 <- best
-  int upd_hist;     // columns of the Anderson step (0: damped Picard step)
-  int copy_best;    // before the update: best <- Gv (exit_picard_step) or v1
-  int exit_picard;  // exit_picard_step of the run's parameters
-  int pad;
-  double beta;
-  double gamma[kMaxWindow];
+constexpr int kEnsBlock = 256;
+constexpr int kRowsMomR0 = 0, kRowsMomN = 1, kRowsStatR0 = 2, kRowsStatN = 3, kRowsQ0 = 4, kRowsQ1 = 5, kRowsQN = 6;
+constexpr int kRowsKinds = 7;
+
+struct EnsembleState {
+  int R = 0, NT = 0;
+  std::vector<EnsRun> runs;
+  int n_sblk = 0, n_gblk = 0, n_pblk = 0, n_eblk = 0;
+  long long part_len = 0;
+  DBuf d_runs, d_ctl, d_tab, d_rows;
+  DBuf v, w, v1, rr[2], gv[2], dF, dG, best, vmid, dv, dS, G, s64, s32, part, spart_sc, mom, dpart, iota, spart_q,
+      vec, fc, scal, statp, stat, tiles;
+  RunCtl* h_ctl = nullptr;   // pinned [R]
+  double* h_stat = nullptr;  // pinned [R][kStatWidth]
+  double* h_scal = nullptr;  // pinned [R][8]
+  int ntiles = 0, bt = 256, waves = 8, n_cu = 256;
+  bool use_graph = true;
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t exec = nullptr;
+  double g_dt = 0.0;
+  int g_gonz = -1, g_logsq = -1;
+  std::vector<AndersonRun> an;
 };
 
-struct EnsView {
-  int R, NT;
-  const int* off;      // [R]
-  const int* n;        // [R]
-  const int* blk_run;  // [NT / 256]
-  RunCtl* ctl;         // [R]
-};
-
-constexpr int kEnsBlock = 256;     // markers per element-wise workgroup
-constexpr int kEnsScatter = 1024;  // markers per scatter workgroup (4 per thread)
+void ensemble_free(EnsembleState* e) {
+  if (!e) return;
+  if (e->exec) (void)hipGraphExecDestroy(e->exec);
+  if (e->graph) (void)hipGraphDestroy(e->graph);
+  for (DBuf* b : {&e->d_runs, &e->d_ctl, &e->d_tab, &e->d_rows, &e->v, &e->w, &e->v1, &e->rr[0], &e->rr[1],
+                  &e->gv[0], &e->gv[1], &e->dF, &e->dG, &e->best, &e->vmid, &e->dv, &e->dS, &e->G, &e->s64,
+                  &e->s32, &e->part, &e->spart_sc, &e->mom, &e->dpart, &e->iota, &e->spart_q, &e->vec, &e->fc,
+                  &e->scal, &e->statp, &e->stat, &e->tiles})
+    release(*b);
+  if (e->h_ctl) (void)hipHostFree(e->h_ctl);
+  if (e->h_stat) (void)hipHostFree(e->h_stat);
+  if (e->h_scal) (void)hipHostFree(e->h_scal);
+  delete e;
+}
 
 namespace {
 
-// ---- element-wise helpers ----------------------------------------------------
-struct Where {
-  int run, local;  // local index within the run, or -1 for padding
-};
-
-__device__ __forceinline__ Where where(const EnsView& e, int i) {
-  const int r = e.blk_run[i / kEnsBlock];
-  const int loc = i - e.off[r];
-  return Where{r, loc < e.n[r] ? loc : -1};
+int slices_for(int64_t n) {
+  // about 2048 sources per slice (the single-run choice at N = 65536)
+  const int64_t s = (n + 2047) / 2048;
+  return (int)std::max<int64_t>(1, std::min<int64_t>(64, s));
 }
 
-// ---- scatter (v_mid / v_in or the state), fixed-point LDS rows ---------------
-// grid.x = scatter blocks (sblk_run / sblk_first tables), grid.y = fields.
-__global__ __launch_bounds__(kScatterThreads) void k_ens_scatter(MeshView m, EnsView e, const int* __restrict__ sblk_run,
-                                                                 const int* __restrict__ sblk_first,
-                                                                 const double* __restrict__ v0,
-                                                                 const double* __restrict__ vin,
-                                                                 const double* __restrict__ w,
-                                                                 const double* __restrict__ scale, int map_mode,
-                                                                 double* __restrict__ vmid, double* __restrict__ dv,
-                                                                 long long* __restrict__ part,
-                                                                 double* __restrict__ mom) {
-  const int r = sblk_run[blockIdx.x];
-  if (map_mode && !e.ctl[r].active) return;
-  extern __shared__ unsigned long long lds_u[];
-  __shared__ double sb1[kMaxBp], sb2[kMaxBp];
-  __shared__ double red[kScatterThreads / 64];
-  const int field = blockIdx.y;
-  for (int k = threadIdx.x; k < m.n_dofs; k += blockDim.x) lds_u[k] = 0ull;
-  stage_bp(m, sb1, sb2);
-  __syncthreads();
-  const int NT = e.NT, n = e.n[r], o = e.off[r];
-  const double sc = scale[r];
-  double p1 = 0.0, p2 = 0.0, e2 = 0.0;
-  for (int loc = sblk_first[blockIdx.x] + threadIdx.x; loc < min(n, sblk_first[blockIdx.x] + kEnsScatter);
-       loc += kScatterThreads) {
-    const int i = o + loc;
-    double x, y;
-    if (map_mode) {
-      const double a1 = v0[i], a2 = v0[NT + i], b1 = vin[i], b2 = vin[NT + i];
-      const double m1 = 0.5 * (a1 + b1), m2 = 0.5 * (a2 + b2);
-      if (field == 0) {
-        vmid[i] = m1;
-        vmid[NT + i] = m2;
-        dv[i] = b1 - a1;
-        dv[NT + i] = b2 - a2;
-        x = m1;
-        y = m2;
-      } else {
-        x = b1;
-        y = b2;
-      }
-    } else {
-      x = vin[i];
-      y = vin[NT + i];
-    }
-    const double wa = w[i];
-    p1 += wa * x;
-    p2 += wa * y;
-    e2 += wa * (x * x + y * y);
-    Located L;
-    if (!locate(m, sb1, sb2, x, y, L)) continue;
-    double f1[3], f2[3];
-    basis3(m.C1, L.i, L.xi1, f1);
-    basis3(m.C2, L.j, L.xi2, f2);
-    const int s1 = m.bs1[L.i], s2 = m.bs2[L.j];
-    const double ws = wa * sc;
-#pragma unroll
-    for (int j2 = 0; j2 < 3; ++j2)
-#pragma unroll
-      for (int j1 = 0; j1 < 3; ++j1) {
-        const long long q = __double2ll_rn(ws * f1[j1] * f2[j2]);
-        atomicAdd(&lds_u[(s1 + j1) + (s2 + j2) * m.nd1], (unsigned long long)q);
-      }
-  }
-  __syncthreads();
-  long long* out = part + ((size_t)field * gridDim.x + blockIdx.x) * m.n_dofs;
-  for (int k = threadIdx.x; k < m.n_dofs; k += blockDim.x) out[k] = (long long)lds_u[k];
-  if (mom && field == 0) {
-    const double s1 = block_sum<kScatterThreads>(p1, red);
-    const double s2 = block_sum<kScatterThreads>(p2, red);
-    const double s3 = block_sum<kScatterThreads>(e2, red);
-    if (threadIdx.x == 0) {
-      mom[3 * blockIdx.x + 0] = s1;
-      mom[3 * blockIdx.x + 1] = s2;
-      mom[3 * blockIdx.x + 2] = s3;
-    }
-  }
+EnsView view(EnsembleState* E, bool with_ctl) {
+  EnsView v{};
+  v.R = E->R;
+  v.NT = E->NT;
+  v.n_sblk = E->n_sblk;
+  v.n_gblk = E->n_gblk;
+  v.n_pblk = E->n_pblk;
+  v.n_eblk = E->n_eblk;
+  v.runs = P<EnsRun>(E->d_runs);
+  v.ctl = with_ctl ? P<RunCtl>(E->d_ctl) : nullptr;
+  const int2* tab = P<int2>(E->d_tab);
+  v.sblk = tab;
+  v.gblk = tab + E->n_sblk;
+  v.pblk = tab + E->n_sblk + E->n_gblk;
+  v.eblk = tab + E->n_sblk + E->n_gblk + E->n_pblk;
+  return v;
 }
 
-// ---- per-run mass solve of the exact row sums (or of a direct vector) --------
-// grid = (R, fields); out / direct at vec[(r * 4 + slot) * n_dofs].
-__global__ __launch_bounds__(1024) void k_ens_solve(MeshView m, EnsView e, const int* __restrict__ srow0,
-                                                    const int* __restrict__ srows, const long long* __restrict__ part,
-                                                    int total_rows, const double* __restrict__ scale, int in_slot,
-                                                    int out_slot, int map_mode, double* __restrict__ vec) {
-  const int r = blockIdx.x, field = blockIdx.y;
-  if (map_mode && !e.ctl[r].active) return;
-  extern __shared__ double X[];
-  const int nd = m.n_dofs;
-  if (part) {
-    const long long* P0 = part + ((size_t)field * total_rows + srow0[r]) * nd;
-    const double inv = 1.0 / scale[r];
-    for (int k = threadIdx.x; k < nd; k += blockDim.x) {
-      unsigned long long s = 0ull;
-      for (int b = 0; b < srows[r]; ++b) s += (unsigned long long)P0[(size_t)b * nd + k];
-      X[k] = (double)(long long)s * inv;
-    }
-  } else {
-    const double* src = vec + ((size_t)r * 4 + in_slot) * nd;
-    for (int k = threadIdx.x; k < nd; k += blockDim.x) X[k] = src[k];
-  }
-  __syncthreads();
-  for (int l = threadIdx.x; l < m.nd2; l += blockDim.x) band_solve_line(X + (size_t)l * m.nd1, m.nd1, 1, m.chol1);
-  __syncthreads();
-  for (int l = threadIdx.x; l < m.nd1; l += blockDim.x) band_solve_line(X + l, m.nd2, m.nd1, m.chol2);
-  __syncthreads();
-  double* dst = vec + ((size_t)r * 4 + out_slot + field) * nd;
-  for (int k = threadIdx.x; k < nd; k += blockDim.x) dst[k] = X[k];
+EnsBufs bufs(EnsembleState* E) {
+  EnsBufs b{};
+  b.v0 = P<double>(E->v);
+  b.v1 = P<double>(E->v1);
+  b.R[0] = P<double>(E->rr[0]);
+  b.R[1] = P<double>(E->rr[1]);
+  b.GV[0] = P<double>(E->gv[0]);
+  b.GV[1] = P<double>(E->gv[1]);
+  b.dF = P<double>(E->dF);
+  b.dG = P<double>(E->dG);
+  b.best = P<double>(E->best);
+  b.part = P<double>(E->part);
+  b.statp = P<double>(E->statp);
+  return b;
 }
 
-// ---- quadrature: grid (qblocks, fields, R); field f reads vec slot f ----------
-constexpr int kQT = 256;
-__global__ __launch_bounds__(kQT) void k_ens_quad1(MeshView m, EnsView e, const double* __restrict__ vec, int c_slot,
-                                                   int use_logsq, int map_mode, double* __restrict__ iota,
-                                                   double* __restrict__ spart) {
-  __shared__ double red[kQT / 64];
-  const int r = blockIdx.z, field = blockIdx.y;
-  if (map_mode && !e.ctl[r].active) return;
-  const double* c = vec + ((size_t)r * 4 + c_slot + field) * m.n_dofs;
-  const int npt = m.nq * m.nq, nel = m.ne1 * m.ne2;
-  const int tid = blockIdx.x * blockDim.x + threadIdx.x;
-  double s_loc = 0.0;
-  if (tid < nel * npt) {
-    const int el = tid / npt, pt = tid - el * npt;
-    const int e1 = el % m.ne1, e2 = el / m.ne1, a = pt % m.nq, b = pt / m.nq;
-    const int s1 = m.bs1[e1], s2 = m.bs2[e2];
-    double f = 0.0;
-#pragma unroll
-    for (int j2 = 0; j2 < 3; ++j2) {
-      double row = 0.0;
-#pragma unroll
-      for (int j1 = 0; j1 < 3; ++j1) row += c[(s1 + j1) + (s2 + j2) * m.nd1] * m.phiq1[(3 * e1 + j1) * m.nq + a];
-      f += row * m.phiq2[(3 * e2 + j2) * m.nq + b];
-    }
-    const double wq = m.qw[a] * m.qw[b] * (m.bp1[e1 + 1] - m.bp1[e1]) * (m.bp2[e2 + 1] - m.bp2[e2]);
-    double lg = 0.0;
-    bool ok;
-    if (use_logsq) {
-      ok = f * f > 1e-60;
-      if (ok) lg = 0.5 * log(f * f);
-    } else {
-      ok = f > 1e-30;
-      if (ok) lg = log(f);
-    }
-    if (iota && field == 0) iota[(size_t)r * nel * npt + (size_t)el * npt + pt] = ok ? (1.0 + lg) * wq : 0.0;
-    s_loc = ok ? -f * lg * wq : 0.0;
+const int* rows(EnsembleState* E, int kind) { return P<int>(E->d_rows) + (size_t)kind * E->R; }
+
+// Tile list of the persistent pair sum: runs in decreasing slice length
+// (longest-processing-time order), each run's tiles in (target block, slice)
+// order.  All tiles of a run cost the same: bt targets x one slice of sources.
+int build_tiles(collop_ctx* ctx, EnsembleState* E, int bt) {
+  std::vector<int> order(E->R);
+  std::iota(order.begin(), order.end(), 0);
+  auto chunk = [&](int r) { return (E->runs[r].n + E->runs[r].S - 1) / E->runs[r].S; };
+  std::stable_sort(order.begin(), order.end(), [&](int a, int b) { return chunk(a) > chunk(b); });
+  std::vector<int4> t;
+  for (int r : order) {
+    const int nb = (E->runs[r].n + bt - 1) / bt;
+    for (int g = 0; g < nb; ++g)
+      for (int s = 0; s < E->runs[r].S; ++s) t.push_back(make_int4(r, g * bt, s, 0));
   }
-  if (spart) {
-    const double s = block_sum<kQT>(s_loc, red);
-    if (threadIdx.x == 0) spart[((size_t)r * 2 + field) * gridDim.x + blockIdx.x] = s;
-  }
+  ENSURE(E->tiles, t.size() * sizeof(int4));
+  HIPC(hipMemcpy(E->tiles.p, t.data(), t.size() * sizeof(int4), hipMemcpyHostToDevice));
+  E->ntiles = (int)t.size();
+  E->bt = bt;
+  return COLLOP_OK;
 }
 
-__global__ void k_ens_quad2(MeshView m, EnsView e, const double* __restrict__ iota, int map_mode,
-                            double* __restrict__ vec) {
-  const int r = blockIdx.y;
-  if (map_mode && !e.ctl[r].active) return;
-  const int k = blockIdx.x * blockDim.x + threadIdx.x;
-  if (k >= m.n_dofs) return;
-  const int i1 = k % m.nd1, i2 = k / m.nd1, npt = m.nq * m.nq;
-  const double* io0 = iota + (size_t)r * m.ne1 * m.ne2 * npt;
-  double acc = 0.0;
-  for (int e2 = m.erange2[2 * i2]; e2 <= m.erange2[2 * i2 + 1]; ++e2) {
-    const double* p2 = m.phiq2 + (3 * e2 + i2 - m.bs2[e2]) * m.nq;
-    for (int e1 = m.erange1[2 * i1]; e1 <= m.erange1[2 * i1 + 1]; ++e1) {
-      const double* p1 = m.phiq1 + (3 * e1 + i1 - m.bs1[e1]) * m.nq;
-      const double* io = io0 + (size_t)(e1 + e2 * m.ne1) * npt;
-      for (int b = 0; b < m.nq; ++b) {
-        double row = 0.0;
-        for (int a = 0; a < m.nq; ++a) row += p1[a] * io[a + m.nq * b];
-        acc += row * p2[b];
-      }
-    }
-  }
-  vec[((size_t)r * 4 + 2) * m.n_dofs + k] = acc;
+EnsTiles tiles(EnsembleState* E) {
+  EnsTiles T;
+  T.tiles = P<int4>(E->tiles);
+  T.ntiles = E->ntiles;
+  T.bt = E->bt;
+  T.grid = std::max(1, E->waves * E->n_cu / (E->bt / 64));
+  return T;
 }
 
-// out[r * ostride + c] = sum_{j < cnt[r]} part[(row0[r] + j) * stride + c], c < ncomp
-// (fixed order; one workgroup per (run, component)).
-__global__ __launch_bounds__(256) void k_ens_rowsum(EnsView e, const double* __restrict__ part, const int* __restrict__ row0,
-                                                    const int* __restrict__ cnt, int stride, int fixed_cnt,
-                                                    int fixed_row0_mul, int map_mode, double* __restrict__ out,
-                                                    int ostride) {
-  __shared__ double red[4];
-  const int r = blockIdx.x, c = blockIdx.y;
-  if (map_mode && !e.ctl[r].active) return;
-  const int n = cnt ? cnt[r] : fixed_cnt;
-  const int r0 = row0 ? row0[r] : r * fixed_row0_mul;
-  double s = 0.0;
-  for (int j = threadIdx.x; j < n; j += 256) s += part[(size_t)(r0 + j) * stride + c];
-  s = block_sum<256>(s, red);
-  if (threadIdx.x == 0) out[(size_t)r * ostride + c] = s;
+void drop_graph(EnsembleState* E) {
+  if (E->exec) (void)hipGraphExecDestroy(E->exec);
+  if (E->graph) (void)hipGraphDestroy(E->graph);
+  E->exec = nullptr;
+  E->graph = nullptr;
 }
 
-// ---- gathers: mode 0 G, mode 1 dS = -w G + per-block (dv.dS, |dv|^2) ---------
-__global__ __launch_bounds__(kEnsBlock) void k_ens_gather(MeshView m, EnsView e, const double* __restrict__ x,
-                                                          const double* __restrict__ vec, int L_slot,
-                                                          const double* __restrict__ w,
-                                                          const double* __restrict__ dv, int mode, int map_mode,
-                                                          double* __restrict__ out, double* __restrict__ dpart) {
-  const Where wh = where(e, blockIdx.x * kEnsBlock + threadIdx.x);
-  if (map_mode && !e.ctl[wh.run].active) return;
-  __shared__ double sb1[kMaxBp], sb2[kMaxBp];
-  __shared__ double red[kEnsBlock / 64];
-  stage_bp(m, sb1, sb2);
-  __syncthreads();
-  const int NT = e.NT, i = blockIdx.x * kEnsBlock + threadIdx.x;
-  const double* coeff = vec + ((size_t)wh.run * 4 + L_slot) * m.n_dofs;
-  double dot = 0.0, nrm = 0.0;
-  if (wh.local >= 0) {
-    double o1 = 0.0, o2 = 0.0;
-    Located L;
-    if (locate(m, sb1, sb2, x[i], x[NT + i], L)) {
-      double f1[3], d1[3], f2[3], d2[3];
-      basis3_grad(m.C1, L.i, L.xi1, f1, d1);
-      basis3_grad(m.C2, L.j, L.xi2, f2, d2);
-      const double ih1 = 1.0 / L.h1, ih2 = 1.0 / L.h2;
-      const int s1 = m.bs1[L.i], s2 = m.bs2[L.j];
-#pragma unroll
-      for (int j2 = 0; j2 < 3; ++j2)
-#pragma unroll
-        for (int j1 = 0; j1 < 3; ++j1) {
-          const double c = coeff[(s1 + j1) + (s2 + j2) * m.nd1];
-          o1 += c * (d1[j1] * f2[j2]) * ih1;
-          o2 += c * (f1[j1] * d2[j2]) * ih2;
-        }
-    }
-    if (mode == 1) {
-      const double wa = w[i];
-      o1 = -wa * o1;
-      o2 = -wa * o2;
-      const double e1 = dv[i], e2 = dv[NT + i];
-      dot = e1 * o1 + e2 * o2;
-      nrm = e1 * e1 + e2 * e2;
-    }
-    out[i] = o1;
-    out[NT + i] = o2;
-  }
-  if (mode == 1) {
-    const double s1 = block_sum<kEnsBlock>(dot, red);
-    const double s2 = block_sum<kEnsBlock>(nrm, red);
-    if (threadIdx.x == 0) {
-      dpart[2 * blockIdx.x] = s1;
-      dpart[2 * blockIdx.x + 1] = s2;
-    }
-  }
+// Projection of every run's state markers, its entropy and moments
+// (main.jl:298-303): fc[r], scal[r] = {S0, -, -, P1, P2, sum w|v|^2}.
+int ens_project(collop_ctx* ctx, EnsembleState* E, int use_logsq) {
+  const MeshView& m = ctx->mv;
+  const EnsView V = view(E, false);
+  hipStream_t st = ctx->stream;
+  double* scal = P<double>(E->scal);
+  launch_scatter_b(m, V, nullptr, P<double>(E->v), P<double>(E->w), 1, 1, nullptr, nullptr, P<long long>(E->spart_sc),
+                   P<double>(E->mom), st);
+  CHECK_LAUNCH();
+  launch_solve_b(m, V, P<long long>(E->spart_sc), 1, 0, -1, P<double>(E->vec), P<double>(E->fc), st);
+  CHECK_LAUNCH();
+  launch_quad1_b(m, V, P<double>(E->fc), m.n_dofs, 1, use_logsq, nullptr, P<double>(E->spart_q), st);
+  CHECK_LAUNCH();
+  launch_sum_rows_b(V, P<double>(E->spart_q), rows(E, kRowsQ0), rows(E, kRowsQN), 1, 0, 1, scal + 0, 8, st);
+  CHECK_LAUNCH();
+  launch_sum_rows_b(V, P<double>(E->mom), rows(E, kRowsMomR0), rows(E, kRowsMomN), 3, 1, 3, scal + 3, 8, st);
+  CHECK_LAUNCH();
+  return COLLOP_OK;
 }
 
-// ---- G_eff + pair-source pack (map) or plain pack of (v, G) (predictor) -------
-// scal[r * 8 + {0: S0, 1: S1, 2: lambda}]
-__global__ __launch_bounds__(kEnsBlock) void k_ens_pack(MeshView m, EnsView e, const double* __restrict__ x,
-                                                        const double* __restrict__ G, const double* __restrict__ dv,
-                                                        const double* __restrict__ w, const double* __restrict__ dpart,
-                                                        const int* __restrict__ eblk0, const int* __restrict__ eblks,
-                                                        const double* __restrict__ scal, int use_gonzalez,
-                                                        int map_mode, Src64* __restrict__ s64,
-                                                        Src32* __restrict__ s32) {
-  const Where wh = where(e, blockIdx.x * kEnsBlock + threadIdx.x);
-  const RunCtl& c = e.ctl[wh.run];
-  if (map_mode && !c.active) return;
-  __shared__ double lam_sh;
-  if (threadIdx.x == 0) {
-    double lam = 0.0;
-    if (map_mode && use_gonzalez) {
-      double dot = 0.0, nrm2 = 0.0;
-      const int b0 = eblk0[wh.run];
-      for (int b = 0; b < eblks[wh.run]; ++b) {
-        dot += dpart[2 * (b0 + b)];
-        nrm2 += dpart[2 * (b0 + b) + 1];
-      }
-      lam = nrm2 > 1e-30 ? (scal[wh.run * 8 + 1] - scal[wh.run * 8 + 0] - dot) / nrm2 : 0.0;
-    }
-    lam_sh = lam;
-  }
-  __syncthreads();
-  if (wh.local < 0) return;
-  const int NT = e.NT, i = blockIdx.x * kEnsBlock + threadIdx.x;
-  double g1 = G[i], g2 = G[NT + i];
-  if (map_mode) {  // G holds dS here: G_eff = -(dS + lambda dv) / w  (solver.jl:107-111)
-    const double iw = 1.0 / w[i];
-    g1 = -(g1 + lam_sh * dv[i]) * iw;
-    g2 = -(g2 + lam_sh * dv[NT + i]) * iw;
-  }
-  const double a = x[i], b = x[NT + i];
-  const bool in = in_box(a, b, m.lo1, m.hi1, m.lo2, m.hi2);
-  const double wm = in ? w[i] : 0.0;
-  if (!map_mode || c.kind == 0) {
-    Src64 s;
-    s.v1 = a;
-    s.v2 = b;
-    s.g1 = g1;
-    s.g2 = g2;
-    s.w = wm;
-    s.in_domain = in ? 1.0 : 0.0;
-    s.pad0 = s.pad1 = 0.0;
-    s64[i] = s;
-  } else {
-    Src32 s;
-    s.v1 = (float)a;
-    s.v2 = (float)b;
-    s.g1 = (float)g1;
-    s.g2 = (float)g2;
-    s.w = (float)wm;
-    s.in_domain = in ? 1.0f : 0.0f;
-    s.pad0 = s.pad1 = 0.0f;
-    s32[i] = s;
-  }
+// Euler predictor of every run (main.jl:239-245): v1 = v + dt vdot(v, G) with
+// G = grad(M^{-1} r(f_s)); best <- v1.
+int ens_predict(collop_ctx* ctx, EnsembleState* E, const collop_step_params& p) {
+  const MeshView& m = ctx->mv;
+  const EnsView V = view(E, false);
+  hipStream_t st = ctx->stream;
+  launch_quad1_b(m, V, P<double>(E->fc), m.n_dofs, 1, p.use_logsq, P<double>(E->iota), nullptr, st);
+  CHECK_LAUNCH();
+  launch_quad2_b(m, V, P<double>(E->iota), P<double>(E->vec), st);
+  CHECK_LAUNCH();
+  launch_solve_b(m, V, nullptr, 1, 2, 3, P<double>(E->vec), nullptr, st);
+  CHECK_LAUNCH();
+  launch_gather_b(m, V, P<double>(E->v), P<double>(E->vec), 3, nullptr, nullptr, 0, P<double>(E->G), nullptr, st);
+  CHECK_LAUNCH();
+  launch_pack_b(V, P<double>(E->v), P<double>(E->G), P<double>(E->w), m.lo1, m.hi1, m.lo2, m.hi2, P<Src64>(E->s64),
+                nullptr, st);
+  CHECK_LAUNCH();
+  launch_pair_b(V, tiles(E), P<Src64>(E->s64), nullptr, 1, P<double>(E->part), st);
+  CHECK_LAUNCH();
+  launch_euler_b(V, P<double>(E->part), P<double>(E->v), p.dt, P<double>(E->v1), nullptr, st);
+  CHECK_LAUNCH();
+  launch_copy_b(V, P<double>(E->v1), P<double>(E->best), st);
+  CHECK_LAUNCH();
+  return COLLOP_OK;
 }
 
-// ---- persistent pair sum over the LPT tile list ------------------------------
-// tile t: run tr[t], first target tg[t] (local), slice ts[t] of S_r = slices[r].
-// part layout per run: part[poff[r] + s * 2 n_r + comp * n_r + local].
-template <int BT>
-__global__ __launch_bounds__(BT) void k_ens_pair(EnsView e, const int* __restrict__ tr, const int* __restrict__ tg,
-                                                 const int* __restrict__ ts, int ntiles,
-                                                 const int* __restrict__ slices, const long long* __restrict__ poff,
-                                                 const Src64* __restrict__ s64, const Src32* __restrict__ s32,
-                                                 int force_f64, double* __restrict__ part) {
-  for (int t = blockIdx.x; t < ntiles; t += gridDim.x) {
-    const int r = tr[t];
-    const RunCtl& c = e.ctl[r];
-    if (!force_f64 && !c.active) continue;
-    const int n = e.n[r], o = e.off[r], S = slices[r];
-    const int chunk = (n + S - 1) / S;
-    const int s = ts[t];
-    const int a0 = o + s * chunk, a1 = o + min(n, (s + 1) * chunk);
-    const int loc = tg[t] + (int)threadIdx.x;
-    const int g = o + loc;
-    double* pr = part + poff[r] + (size_t)s * 2 * n;
-    if (force_f64 || c.kind == 0) {
-      double vg1 = 0, vg2 = 0, Gg1 = 0, Gg2 = 0, acc1 = 0, acc2 = 0;
-      bool act = false;
-      if (loc < n) {
-        const Src64 q = s64[g];
-        vg1 = q.v1;
-        vg2 = q.v2;
-        Gg1 = q.g1;
-        Gg2 = q.g2;
-        act = q.in_domain != 0.0;
-      }
-#pragma unroll 2
-      for (int a = a0; a < a1; ++a) {
-        double d1, d2, tt;
-        pair_term_f64<1>(vg1, vg2, Gg1, Gg2, s64[a].v1, s64[a].v2, s64[a].g1, s64[a].g2, s64[a].w, acc1, acc2, d1,
-                         d2, tt);
-      }
-      if (loc < n) {
-        pr[loc] = act ? acc1 : 0.0;
-        pr[n + loc] = act ? acc2 : 0.0;
-      }
-    } else {
-      float vg1 = 0, vg2 = 0, Gg1 = 0, Gg2 = 0, b1 = 0, b2 = 0, h1 = 0, h2 = 0, l1 = 0, l2 = 0;
-      bool act = false;
-      if (loc < n) {
-        const Src32 q = s32[g];
-        vg1 = q.v1;
-        vg2 = q.v2;
-        Gg1 = q.g1;
-        Gg2 = q.g2;
-        act = q.in_domain != 0.0f;
-      }
-      for (int ab = a0; ab < a1; ab += 32) {
-        const int ae = min(a1, ab + 32);
-#pragma unroll 4
-        for (int a = ab; a < ae; ++a)
-          pair_term_f32(vg1, vg2, Gg1, Gg2, s32[a].v1, s32[a].v2, s32[a].g1, s32[a].g2, s32[a].w, b1, b2);
-        two_sum_fold(h1, l1, b1);
-        two_sum_fold(h2, l2, b2);
-        b1 = b2 = 0.0f;
-      }
-      if (loc < n) {
-        pr[loc] = act ? (double)h1 + (double)l1 : 0.0;
-        pr[n + loc] = act ? (double)h2 + (double)l2 : 0.0;
-      }
-    }
+// One ensemble iteration: control upload, per-run update, Picard map of every
+// active run (solver.jl:176-227) with its epilogue, statistics download.
+int ens_iteration(collop_ctx* ctx, EnsembleState* E, const collop_step_params& p) {
+  const MeshView& m = ctx->mv;
+  const EnsView V = view(E, true);
+  const EnsBufs B = bufs(E);
+  hipStream_t st = ctx->stream;
+  const int gf = p.use_gonzalez ? 2 : 1;
+  double* vec = P<double>(E->vec);
+  double* scal = P<double>(E->scal);
+  HIPC(hipMemcpyAsync(E->d_ctl.p, E->h_ctl, (size_t)E->R * sizeof(RunCtl), hipMemcpyHostToDevice, st));
+  launch_update_b(V, B, st);
+  CHECK_LAUNCH();
+  launch_scatter_b(m, V, P<double>(E->v), P<double>(E->v1), P<double>(E->w), 2, gf, P<double>(E->vmid),
+                   P<double>(E->dv), P<long long>(E->spart_sc), nullptr, st);
+  CHECK_LAUNCH();
+  launch_solve_b(m, V, P<long long>(E->spart_sc), gf, 0, 0, vec, nullptr, st);
+  CHECK_LAUNCH();
+  launch_quad1_b(m, V, vec, 4 * m.n_dofs, gf, p.use_logsq, P<double>(E->iota), P<double>(E->spart_q), st);
+  CHECK_LAUNCH();
+  launch_quad2_b(m, V, P<double>(E->iota), vec, st);
+  CHECK_LAUNCH();
+  launch_solve_b(m, V, nullptr, 1, 2, 3, vec, nullptr, st);
+  CHECK_LAUNCH();
+  if (p.use_gonzalez) {
+    launch_sum_rows_b(V, P<double>(E->spart_q), rows(E, kRowsQ1), rows(E, kRowsQN), 1, 0, 1, scal + 1, 8, st);
+    CHECK_LAUNCH();
   }
+  launch_gather_b(m, V, P<double>(E->vmid), vec, 3, P<double>(E->w), P<double>(E->dv), 1, P<double>(E->dS),
+                  P<double>(E->dpart), st);
+  CHECK_LAUNCH();
+  launch_geff_pack_b(V, P<double>(E->vmid), P<double>(E->dS), P<double>(E->dv), P<double>(E->w), P<double>(E->dpart),
+                     scal, p.use_gonzalez, m.lo1, m.hi1, m.lo2, m.hi2, P<Src64>(E->s64), P<Src32>(E->s32), st);
+  CHECK_LAUNCH();
+  launch_pair_b(V, tiles(E), P<Src64>(E->s64), P<Src32>(E->s32), 0, P<double>(E->part), st);
+  CHECK_LAUNCH();
+  launch_epilogue_b(V, B, p.dt, st);
+  CHECK_LAUNCH();
+  launch_sum_rows_b(V, P<double>(E->statp), rows(E, kRowsStatR0), rows(E, kRowsStatN), kStatWidth, 1, kStatWidth,
+                    P<double>(E->stat), kStatWidth, st);
+  CHECK_LAUNCH();
+  HIPC(hipMemcpyAsync(E->h_stat, E->stat.p, (size_t)E->R * kStatWidth * 8, hipMemcpyDeviceToHost, st));
+  return COLLOP_OK;
 }
 
-// ---- map epilogue (solver.jl:225, 339, 391-399) per run ------------------------
-struct EnsBufs {
-  double* v;        // state v0 [2][NT]
-  double* v1;       // iterate
-  double* R[2];     // residual ping-pong
-  double* GV[2];    // map value ping-pong
-  double* dF;       // [kMaxWindow][2][NT]
-  double* dG;
-  double* best;
-  double* statp;    // [eblocks][kStatWidth]
-};
-
-__global__ __launch_bounds__(kEnsBlock) void k_ens_epilogue(EnsView e, EnsBufs B, const double* __restrict__ part,
-                                                            const long long* __restrict__ poff,
-                                                            const int* __restrict__ slices, double dt) {
-  __shared__ double red[kEnsBlock / 64];
-  const Where wh = where(e, blockIdx.x * kEnsBlock + threadIdx.x);
-  const RunCtl& c = e.ctl[wh.run];
-  if (!c.active) return;
-  const int NT = e.NT, i = blockIdx.x * kEnsBlock + threadIdx.x, n = e.n[wh.run];
-  double st[kStatWidth];
-#pragma unroll
-  for (int j = 0; j < kStatWidth; ++j) st[j] = 0.0;
-  if (wh.local >= 0) {
-    const double* pr = part + poff[wh.run];
-    const int S = slices[wh.run];
-    double* R = B.R[c.cur];
-    double* GV = B.GV[c.cur];
-    const double* RP = B.R[1 - c.cur];
-    const double* GP = B.GV[1 - c.cur];
-#pragma unroll
-    for (int comp = 0; comp < 2; ++comp) {
-      double s = 0.0;
-      for (int k = 0; k < S; ++k) s += pr[(size_t)k * 2 * n + comp * n + wh.local];
-      const int idx = comp * NT + i;
-      const double gv = B.v[idx] + dt * s;
-      GV[idx] = gv;
-      const double x1 = B.v1[idx];
-      const double ri = gv - x1;
-      R[idx] = ri;
-      st[0] += ri * ri;
-      st[1] += x1 * x1;
-      st[2] += (gv - B.v[idx]) * (gv - B.v[idx]);
-      double f = 0.0;
-      if (c.col >= 0) {
-        f = ri - RP[idx];
-        B.dF[(size_t)c.col * 2 * NT + idx] = f;
-        B.dG[(size_t)c.col * 2 * NT + idx] = gv - GP[idx];
-      }
-#pragma unroll
-      for (int j = 0; j < kMaxWindow; ++j) {
-        if (j < c.hist) {
-          const double xj = (j == c.col) ? f : B.dF[(size_t)j * 2 * NT + idx];
-          if (c.col >= 0) st[3 + j] += xj * f;
-          st[3 + kMaxWindow + j] += xj * ri;
-        }
-      }
+int ens_replay(collop_ctx* ctx, EnsembleState* E, const collop_step_params& p) {
+  if (!E->use_graph) {
+    int r = ens_iteration(ctx, E, p);
+    if (r) return r;
+    HIPC(hipStreamSynchronize(ctx->stream));
+    return COLLOP_OK;
+  }
+  if (E->exec && (E->g_dt != p.dt || E->g_gonz != p.use_gonzalez || E->g_logsq != p.use_logsq)) drop_graph(E);
+  if (!E->exec) {
+    HIPC(hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeThreadLocal));
+    const int r = ens_iteration(ctx, E, p);
+    hipGraph_t g = nullptr;
+    const hipError_t ec = hipStreamEndCapture(ctx->stream, &g);
+    if (r) {
+      if (g) (void)hipGraphDestroy(g);
+      return r;
     }
+    if (ec != hipSuccess) return fail(ctx, COLLOP_ERR_HIP, std::string("hipStreamEndCapture: ") + hipGetErrorString(ec));
+    E->graph = g;
+    HIPC(hipGraphInstantiate(&E->exec, E->graph, nullptr, nullptr, 0));
+    E->g_dt = p.dt;
+    E->g_gonz = p.use_gonzalez;
+    E->g_logsq = p.use_logsq;
   }
-#pragma unroll
-  for (int j = 0; j < kStatWidth; ++j) {
-    const double t = block_sum<kEnsBlock>(st[j], red);
-    if (threadIdx.x == 0) B.statp[(size_t)blockIdx.x * kStatWidth + j] = t;
-  }
+  HIPC(hipGraphLaunch(E->exec, ctx->stream));
+  HIPC(hipStreamSynchronize(ctx->stream));
+  return COLLOP_OK;
 }
 
-// ---- per-run update of the iterate before the map ----------------------------
-__global__ void k_ens_update(EnsView e, EnsBufs B, int all_to_pending) {
-  const Where wh = where(e, blockIdx.x * kEnsBlock + threadIdx.x);
-  if (wh.local < 0) return;
-  const RunCtl& c = e.ctl[wh.run];
-  if (c.upd == 0 && !c.copy_best) return;
-  const int NT = e.NT, i = blockIdx.x * kEnsBlock + threadIdx.x;
-  const int prev = 1 - c.cur;  // buffers of the previous map
-  const double* GV = B.GV[prev];
-  (void)all_to_pending;
-#pragma unroll
-  for (int comp = 0; comp < 2; ++comp) {
-    const int idx = comp * NT + i;
-    const double gv = GV[idx];
-    const double x = B.v1[idx];
-    if (c.copy_best) B.best[idx] = c.exit_picard ? gv : x;
-    double nx = x;
-    if (c.upd == 1) {
-      double y = gv;
-      for (int j = 0; j < c.upd_hist; ++j) y -= B.dG[(size_t)j * 2 * NT + idx] * c.gamma[j];
-      nx = c.beta * y + (1.0 - c.beta) * x;
-    } else if (c.upd == 2) {
-      nx = gv;
-    } else if (c.upd == 3) {
-      nx = c.copy_best ? (c.exit_picard ? gv : x) : B.best[idx];
-    }
-    B.v1[idx] = nx;
-  }
+void ctl_from_plan(RunCtl& c, const AndersonPlan& pl) {
+  c.active = 1;
+  c.kind = pl.kind;
+  c.col = pl.col;
+  c.hist = pl.hist;
 }
 
-// ---- predictor: v1 = v + dt * (slice sum), keeps dot_v ----------------------
-__global__ void k_ens_euler(EnsView e, const double* __restrict__ part, const long long* __restrict__ poff,
-                            const int* __restrict__ slices, const double* __restrict__ v, double dt,
-                            double* __restrict__ v1, double* __restrict__ dotv) {
-  const Where wh = where(e, blockIdx.x * kEnsBlock + threadIdx.x);
-  if (wh.local < 0) return;
-  const int NT = e.NT, i = blockIdx.x * kEnsBlock + threadIdx.x, n = e.n[wh.run], S = slices[wh.run];
-  const double* pr = part + poff[wh.run];
-#pragma unroll
-  for (int comp = 0; comp < 2; ++comp) {
-    double s = 0.0;
-    for (int k = 0; k < S; ++k) s += pr[(size_t)k * 2 * n + comp * n + wh.local];
-    dotv[comp * NT + i] = s;
-    v1[comp * NT + i] = v[comp * NT + i] + dt * s;
+// One implicit step of every run (main.jl:226-307 per run).
+int ens_step(collop_ctx* ctx, EnsembleState* E, const collop_step_params& p, collop_step_stats* out) {
+  const auto t0 = std::chrono::steady_clock::now();
+  int r;
+  if ((r = ens_predict(ctx, E, p))) return r;
+  E->an.resize(E->R);
+  for (int i = 0; i < E->R; ++i) {
+    E->an[i].start(p);
+    RunCtl& c = E->h_ctl[i];
+    const int cur = c.cur;
+    std::memset(&c, 0, sizeof(RunCtl));
+    c.cur = cur;
+    c.exit_picard = p.exit_picard_step;
+    ctl_from_plan(c, E->an[i].plan());
   }
+  int active = E->R;
+  while (active > 0) {
+    if ((r = ens_replay(ctx, E, p))) return r;
+    for (int i = 0; i < E->R; ++i) {
+      RunCtl& c = E->h_ctl[i];
+      if (!c.active) {  // its exit update ran in this replay
+        c.upd = 0;
+        c.copy_best = 0;
+        continue;
+      }
+      AndersonAct act;
+      if (E->an[i].after_map(E->h_stat + (size_t)i * kStatWidth, act) != COLLOP_OK)
+        return fail(ctx, COLLOP_ERR_NONFINITE,
+                    "non-finite residual in run " + std::to_string(i) + ", Picard map " +
+                        std::to_string(E->an[i].maps()));
+      c.copy_best = act.copy_best ? 1 : 0;
+      if (act.exit != kContinue) {
+        c.active = 0;
+        c.upd = act.exit == kConverged ? (p.exit_picard_step ? 2 : 0) : 3;
+        c.cur = 1 - c.cur;  // the update reads the map just done (buffer 1 - cur)
+        active--;
+        continue;
+      }
+      c.upd = act.update ? 1 : 0;
+      c.upd_hist = act.h;
+      c.beta = act.beta;
+      for (int j = 0; j < kMaxWindow; ++j) c.gamma[j] = act.gamma[j];
+      if (act.rotate) c.cur = 1 - c.cur;
+      ctl_from_plan(c, E->an[i].plan());
+    }
+  }
+  // exit updates of the runs that finished in the last replay, then v <- v1
+  const EnsView V = view(E, true);
+  HIPC(hipMemcpyAsync(E->d_ctl.p, E->h_ctl, (size_t)E->R * sizeof(RunCtl), hipMemcpyHostToDevice, ctx->stream));
+  launch_update_b(V, bufs(E), ctx->stream);
+  CHECK_LAUNCH();
+  launch_copy_b(V, P<double>(E->v1), P<double>(E->v), ctx->stream);
+  CHECK_LAUNCH();
+  if ((r = ens_project(ctx, E, p.use_logsq))) return r;
+  HIPC(hipMemcpyAsync(E->h_scal, E->scal.p, (size_t)E->R * 8 * 8, hipMemcpyDeviceToHost, ctx->stream));
+  HIPC(hipStreamSynchronize(ctx->stream));
+  for (int i = 0; i < E->R; ++i) E->h_ctl[i].upd = E->h_ctl[i].copy_best = 0;
+  const double sec = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+  if (out) {
+    for (int i = 0; i < E->R; ++i) {
+      collop_step_stats& s = out[i];
+      const AndersonRun& a = E->an[i];
+      const double* sc = E->h_scal + 8 * i;
+      s.iters = a.k_done;
+      s.iters_f32 = a.iters32;
+      s.n_restart = a.n_restart;
+      s.exit_reason = a.exit_reason;
+      s.residual = a.res_out;
+      s.r0 = a.nrm_r0;
+      s.entropy = sc[0];
+      s.energy = 0.5 * sc[5];
+      s.momentum[0] = sc[3];
+      s.momentum[1] = sc[4];
+      s.seconds = sec;
+    }
+  }
+  return COLLOP_OK;
+}
+
+int need_ens(collop_ctx* ctx) {
+  if (!ctx) return COLLOP_ERR_INVALID;
+  if (!ctx->ens) return fail(ctx, COLLOP_ERR_NO_STATE, "no ensemble (collop_ensemble_set)");
+  return COLLOP_OK;
+}
+
+// Mean time of one persistent pair launch with every run active, FP32c plus
+// FP64 (ms).
+int time_pair(collop_ctx* ctx, EnsembleState* E, int reps, double* ms_out) {
+  hipEvent_t a, b;
+  HIPC(hipEventCreate(&a));
+  HIPC(hipEventCreate(&b));
+  const EnsView V = view(E, true);
+  double tot = 0.0;
+  for (int kind = 0; kind < 2; ++kind) {
+    for (int i = 0; i < E->R; ++i) {
+      std::memset(&E->h_ctl[i], 0, sizeof(RunCtl));
+      E->h_ctl[i].active = 1;
+      E->h_ctl[i].kind = kind;
+    }
+    HIPC(hipMemcpyAsync(E->d_ctl.p, E->h_ctl, (size_t)E->R * sizeof(RunCtl), hipMemcpyHostToDevice, ctx->stream));
+    launch_pair_b(V, tiles(E), P<Src64>(E->s64), P<Src32>(E->s32), 0, P<double>(E->part), ctx->stream);  // warm-up
+    CHECK_LAUNCH();
+    HIPC(hipEventRecord(a, ctx->stream));
+    for (int k = 0; k < reps; ++k)
+      launch_pair_b(V, tiles(E), P<Src64>(E->s64), P<Src32>(E->s32), 0, P<double>(E->part), ctx->stream);
+    CHECK_LAUNCH();
+    HIPC(hipEventRecord(b, ctx->stream));
+    HIPC(hipEventSynchronize(b));
+    float ms = 0.0f;
+    HIPC(hipEventElapsedTime(&ms, a, b));
+    tot += ms / reps;
+  }
+  (void)hipEventDestroy(a);
+  (void)hipEventDestroy(b);
+  for (int i = 0; i < E->R; ++i) std::memset(&E->h_ctl[i], 0, sizeof(RunCtl));
+  *ms_out = tot;
+  return COLLOP_OK;
+}
+
+int check_params(collop_ctx* ctx, const collop_step_params* p) {
+  if (!p) return fail(ctx, COLLOP_ERR_INVALID, "null params");
+  if (!(p->dt > 0.0) || !std::isfinite(p->dt)) return fail(ctx, COLLOP_ERR_INVALID, "dt must be positive");
+  if (p->m_anderson < 1 || p->m_anderson > kMaxWindow)
+    return fail(ctx, COLLOP_ERR_INVALID, "m_anderson must be in [1, 8]");
+  if (p->max_iter < 1) return fail(ctx, COLLOP_ERR_INVALID, "max_iter must be >= 1");
+  if (p->stag_window < 1) return fail(ctx, COLLOP_ERR_INVALID, "stag_window must be >= 1");
+  if (!(p->damping > 0.0 && p->damping <= 1.0)) return fail(ctx, COLLOP_ERR_INVALID, "damping must be in (0, 1]");
+  if (p->precision != COLLOP_PREC_F64 && p->precision != COLLOP_PREC_F32C)
+    return fail(ctx, COLLOP_ERR_INVALID, "unknown precision");
+  if (p->warmstart != COLLOP_WARM_EULER)
+    return fail(ctx, COLLOP_ERR_UNSUPPORTED, "the ensemble step supports the Euler predictor only");
+  if (p->record_training)
+    return fail(ctx, COLLOP_ERR_UNSUPPORTED, "the ensemble step does not record training data");
+  return COLLOP_OK;
 }
 
 }  // namespace
 }  // namespace collop
 
-namespace collop {
-struct EnsembleState {
-  int R = 0;
-};
-void ensemble_free(EnsembleState* e) { delete e; }
-}  // namespace collop
+using namespace collop;
+
+extern "C" {
+
+int collop_ensemble_slices(int64_t n) { return slices_for(n); }
+
+int collop_ensemble_set(collop_ctx* ctx, int n_runs, const int64_t* n, const double* v, const double* w) {
+  if (!ctx) return COLLOP_ERR_INVALID;
+  if (!ctx->has_mesh) return fail(ctx, COLLOP_ERR_NO_MESH, "no mesh (collop_set_mesh)");
+  if (n_runs < 1 || n_runs > 4096) return fail(ctx, COLLOP_ERR_INVALID, "n_runs must be in [1, 4096]");
+  if (!n || !v || !w) return fail(ctx, COLLOP_ERR_INVALID, "null array");
+  int64_t total = 0, padded = 0;
+  std::vector<double> wabs(n_runs, 0.0);
+  for (int r = 0; r < n_runs; ++r) {
+    if (n[r] < 2 || n[r] > ((int64_t)1 << 24))
+      return fail(ctx, COLLOP_ERR_INVALID, "each run needs 2 <= N <= 2^24 markers");
+    for (int64_t i = 0; i < n[r]; ++i) {
+      const double x = w[total + i];
+      if (!(x > 0.0) || !std::isfinite(x))
+        return fail(ctx, COLLOP_ERR_INVALID, "weights must be positive and finite (G_eff divides by w)");
+      wabs[r] += x;
+    }
+    total += n[r];
+    padded += (n[r] + kEnsBlock - 1) / kEnsBlock * kEnsBlock;
+  }
+  if (padded > ((int64_t)1 << 27)) return fail(ctx, COLLOP_ERR_INVALID, "ensemble too large (> 2^27 markers)");
+  HIPC(hipSetDevice(ctx->device));
+  HIPC(hipStreamSynchronize(ctx->stream));
+  ensemble_free(ctx->ens);
+  ctx->ens = new EnsembleState();
+  EnsembleState* E = ctx->ens;
+  const MeshView& m = ctx->mv;
+  E->R = n_runs;
+  E->NT = (int)padded;
+  E->runs.resize(n_runs);
+  std::vector<int2> sb, gb, pb, eb;
+  std::vector<int> rowtab((size_t)kRowsKinds * n_runs);
+  const int qb = quad_blocks(m);
+  int off = 0;
+  long long poff = 0;
+  for (int r = 0; r < n_runs; ++r) {
+    EnsRun& q = E->runs[r];
+    q.n = (int)n[r];
+    q.off = off;
+    q.S = slices_for(n[r]);
+    q.sblk0 = (int)sb.size();
+    q.nsb = scatter_blocks(q.n);
+    for (int b = 0; b < q.nsb; ++b) sb.push_back(make_int2(r, b));
+    q.gblk0 = (int)gb.size();
+    q.ngb = gather_blocks(q.n);
+    for (int b = 0; b < q.ngb; ++b) gb.push_back(make_int2(r, b));
+    q.pblk0 = (int)pb.size();
+    q.npb = (q.n + kEnsBlock - 1) / kEnsBlock;
+    for (int b = 0; b < q.npb; ++b) pb.push_back(make_int2(r, b));
+    q.eblk0 = (int)eb.size();
+    q.neb = (2 * q.n + kEnsBlock - 1) / kEnsBlock;
+    for (int b = 0; b < q.neb; ++b) eb.push_back(make_int2(r, b));
+    q.poff = poff;
+    poff += (long long)q.S * 2 * q.n;
+    q.scale = scatter_scale(wabs[r]);
+    q.inv_scale = 1.0 / q.scale;
+    rowtab[(size_t)kRowsMomR0 * n_runs + r] = 3 * q.sblk0;
+    rowtab[(size_t)kRowsMomN * n_runs + r] = q.nsb;
+    rowtab[(size_t)kRowsStatR0 * n_runs + r] = kStatWidth * q.eblk0;
+    rowtab[(size_t)kRowsStatN * n_runs + r] = q.neb;
+    rowtab[(size_t)kRowsQ0 * n_runs + r] = 2 * r * qb;
+    rowtab[(size_t)kRowsQ1 * n_runs + r] = (2 * r + 1) * qb;
+    rowtab[(size_t)kRowsQN * n_runs + r] = qb;
+    off += q.npb * kEnsBlock;
+  }
+  E->n_sblk = (int)sb.size();
+  E->n_gblk = (int)gb.size();
+  E->n_pblk = (int)pb.size();
+  E->n_eblk = (int)eb.size();
+  E->part_len = poff;
+  int dev_cu = 0;
+  if (hipDeviceGetAttribute(&dev_cu, hipDeviceAttributeMultiprocessorCount, ctx->device) == hipSuccess && dev_cu > 0)
+    E->n_cu = dev_cu;
+  const size_t NT = (size_t)E->NT, nd = (size_t)m.n_dofs, R = (size_t)n_runs;
+  const size_t n2b = 2 * NT * 8;
+  ENSURE(E->d_runs, R * sizeof(EnsRun));
+  ENSURE(E->d_ctl, R * sizeof(RunCtl));
+  ENSURE(E->d_tab, (sb.size() + gb.size() + pb.size() + eb.size()) * sizeof(int2));
+  ENSURE(E->d_rows, rowtab.size() * sizeof(int));
+  for (DBuf* b : {&E->v, &E->v1, &E->rr[0], &E->rr[1], &E->gv[0], &E->gv[1], &E->best, &E->vmid, &E->dv, &E->dS, &E->G})
+    ENSURE(*b, n2b);
+  ENSURE(E->w, NT * 8);
+  ENSURE(E->dF, n2b * kMaxWindow);
+  ENSURE(E->dG, n2b * kMaxWindow);
+  ENSURE(E->s64, NT * sizeof(Src64));
+  ENSURE(E->s32, NT * sizeof(Src32));
+  ENSURE(E->part, (size_t)poff * 8);
+  ENSURE(E->spart_sc, (size_t)2 * E->n_sblk * nd * 8);
+  ENSURE(E->mom, (size_t)E->n_sblk * 3 * 8);
+  ENSURE(E->dpart, (size_t)E->n_gblk * 2 * 8);
+  ENSURE(E->iota, R * (size_t)m.ne1 * m.ne2 * m.nq * m.nq * 8);
+  ENSURE(E->spart_q, R * 2 * (size_t)qb * 8);
+  ENSURE(E->vec, R * 4 * nd * 8);
+  ENSURE(E->fc, R * nd * 8);
+  ENSURE(E->scal, R * 8 * 8);
+  ENSURE(E->statp, (size_t)E->n_eblk * kStatWidth * 8);
+  ENSURE(E->stat, R * kStatWidth * 8);
+  HIPC(hipHostMalloc((void**)&E->h_ctl, R * sizeof(RunCtl), hipHostMallocDefault));
+  HIPC(hipHostMalloc((void**)&E->h_stat, R * kStatWidth * 8, hipHostMallocDefault));
+  HIPC(hipHostMalloc((void**)&E->h_scal, R * 8 * 8, hipHostMallocDefault));
+  std::memset(E->h_ctl, 0, R * sizeof(RunCtl));
+  // host staging with padding (positions 0, weights 1 outside the runs)
+  std::vector<double> hv(2 * NT, 0.0), hw(NT, 1.0);
+  int64_t src = 0;
+  for (int r = 0; r < n_runs; ++r) {
+    const EnsRun& q = E->runs[r];
+    std::memcpy(&hv[q.off], v + 2 * src, (size_t)q.n * 8);
+    std::memcpy(&hv[NT + q.off], v + 2 * src + q.n, (size_t)q.n * 8);
+    std::memcpy(&hw[q.off], w + src, (size_t)q.n * 8);
+    src += q.n;
+  }
+  std::vector<int2> tab;
+  tab.insert(tab.end(), sb.begin(), sb.end());
+  tab.insert(tab.end(), gb.begin(), gb.end());
+  tab.insert(tab.end(), pb.begin(), pb.end());
+  tab.insert(tab.end(), eb.begin(), eb.end());
+  HIPC(hipMemcpy(E->d_runs.p, E->runs.data(), R * sizeof(EnsRun), hipMemcpyHostToDevice));
+  HIPC(hipMemcpy(E->d_tab.p, tab.data(), tab.size() * sizeof(int2), hipMemcpyHostToDevice));
+  HIPC(hipMemcpy(E->d_rows.p, rowtab.data(), rowtab.size() * sizeof(int), hipMemcpyHostToDevice));
+  HIPC(hipMemcpy(E->v.p, hv.data(), n2b, hipMemcpyHostToDevice));
+  HIPC(hipMemcpy(E->w.p, hw.data(), NT * 8, hipMemcpyHostToDevice));
+  for (DBuf* b : {&E->v1, &E->rr[0], &E->rr[1], &E->gv[0], &E->gv[1], &E->best, &E->vmid, &E->dv, &E->dS, &E->G})
+    HIPC(hipMemset(b->p, 0, n2b));
+  HIPC(hipMemset(E->dF.p, 0, n2b * kMaxWindow));
+  HIPC(hipMemset(E->dG.p, 0, n2b * kMaxWindow));
+  int r;
+  if ((r = build_tiles(ctx, E, E->bt))) return r;
+  if ((r = ens_project(ctx, E, 0))) return r;
+  HIPC(hipStreamSynchronize(ctx->stream));
+  return COLLOP_OK;
+}
+
+int collop_ensemble_get(collop_ctx* ctx, double* v) {
+  int r = need_ens(ctx);
+  if (r) return r;
+  if (!v) return fail(ctx, COLLOP_ERR_INVALID, "null array");
+  EnsembleState* E = ctx->ens;
+  HIPC(hipSetDevice(ctx->device));
+  std::vector<double> hv((size_t)2 * E->NT);
+  HIPC(hipMemcpyAsync(hv.data(), E->v.p, hv.size() * 8, hipMemcpyDeviceToHost, ctx->stream));
+  HIPC(hipStreamSynchronize(ctx->stream));
+  int64_t dst = 0;
+  for (const EnsRun& q : E->runs) {
+    std::memcpy(v + 2 * dst, &hv[q.off], (size_t)q.n * 8);
+    std::memcpy(v + 2 * dst + q.n, &hv[E->NT + q.off], (size_t)q.n * 8);
+    dst += q.n;
+  }
+  return COLLOP_OK;
+}
+
+int collop_ensemble_coeffs(collop_ctx* ctx, double* f_coeffs) {
+  int r = need_ens(ctx);
+  if (r) return r;
+  if (!f_coeffs) return fail(ctx, COLLOP_ERR_INVALID, "null array");
+  HIPC(hipSetDevice(ctx->device));
+  HIPC(hipMemcpyAsync(f_coeffs, ctx->ens->fc.p, (size_t)ctx->ens->R * ctx->mv.n_dofs * 8, hipMemcpyDeviceToHost,
+                      ctx->stream));
+  HIPC(hipStreamSynchronize(ctx->stream));
+  return COLLOP_OK;
+}
+
+int collop_ensemble_step(collop_ctx* ctx, const collop_step_params* params, int n_steps, collop_step_stats* stats) {
+  int r = need_ens(ctx);
+  if (r) return r;
+  if ((r = check_params(ctx, params))) return r;
+  if (n_steps < 0) return fail(ctx, COLLOP_ERR_INVALID, "n_steps must be >= 0");
+  HIPC(hipSetDevice(ctx->device));
+  for (int s = 0; s < n_steps; ++s)
+    if ((r = ens_step(ctx, ctx->ens, *params, stats ? stats + (size_t)s * ctx->ens->R : nullptr))) return r;
+  return COLLOP_OK;
+}
+
+int collop_ensemble_set_tile(collop_ctx* ctx, int gamma_tile, int waves_per_cu, int use_graph) {
+  int r = need_ens(ctx);
+  if (r) return r;
+  if (gamma_tile != 64 && gamma_tile != 128 && gamma_tile != 256 && gamma_tile != 512)
+    return fail(ctx, COLLOP_ERR_INVALID, "gamma_tile must be 64, 128, 256 or 512");
+  if (waves_per_cu < 1 || waves_per_cu > 64) return fail(ctx, COLLOP_ERR_INVALID, "waves_per_cu must be in [1, 64]");
+  EnsembleState* E = ctx->ens;
+  HIPC(hipSetDevice(ctx->device));
+  HIPC(hipStreamSynchronize(ctx->stream));
+  drop_graph(E);
+  E->waves = waves_per_cu;
+  E->use_graph = use_graph != 0;
+  if (gamma_tile != E->bt || !E->ntiles) return build_tiles(ctx, E, gamma_tile);
+  return COLLOP_OK;
+}
+
+int collop_ensemble_tune(collop_ctx* ctx, int* gamma_tile, int* waves_per_cu, double* ms) {
+  int r = need_ens(ctx);
+  if (r) return r;
+  EnsembleState* E = ctx->ens;
+  HIPC(hipSetDevice(ctx->device));
+  HIPC(hipStreamSynchronize(ctx->stream));
+  drop_graph(E);
+  // sources of the current state in both precisions (G = 0: the pair loop has
+  // no data-dependent work besides the coincidence guard)
+  const EnsView V = view(E, false);
+  const MeshView& m = ctx->mv;
+  HIPC(hipMemsetAsync(E->G.p, 0, (size_t)2 * E->NT * 8, ctx->stream));
+  launch_pack_b(V, P<double>(E->v), P<double>(E->G), P<double>(E->w), m.lo1, m.hi1, m.lo2, m.hi2, P<Src64>(E->s64),
+                P<Src32>(E->s32), ctx->stream);
+  CHECK_LAUNCH();
+  const int bts[4] = {64, 128, 256, 512}, ws[4] = {4, 8, 16, 32};
+  double best = 1e300;
+  int best_bt = E->bt, best_w = E->waves, idx = 0;
+  for (int bt : bts) {
+    if ((r = build_tiles(ctx, E, bt))) return r;
+    for (int wv : ws) {
+      E->waves = wv;
+      double t = 0.0;
+      if ((r = time_pair(ctx, E, 3, &t))) return r;
+      if (ms) ms[idx] = t;
+      idx++;
+      if (t < best) {
+        best = t;
+        best_bt = bt;
+        best_w = wv;
+      }
+    }
+  }
+  E->waves = best_w;
+  if ((r = build_tiles(ctx, E, best_bt))) return r;
+  if (gamma_tile) *gamma_tile = best_bt;
+  if (waves_per_cu) *waves_per_cu = best_w;
+  return COLLOP_OK;
+}
+
+}  // extern "C"

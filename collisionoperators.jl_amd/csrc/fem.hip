@@ -28,34 +28,33 @@ namespace {
 constexpr int kSolveThreads = 1024;
 
 // ---------------------------------------------------------------------------
-// Scatter.  grid = (blocks, fields).  Dynamic LDS: n_dofs doubles.
-__global__ __launch_bounds__(kScatterThreads) void k_scatter(MeshView m, const double* __restrict__ v0,
-                                                             const double* __restrict__ vin,
-                                                             const double* __restrict__ w, int N, int fields,
-                                                             double scale, double* __restrict__ vmid,
-                                                             double* __restrict__ dv,
-                                                             long long* __restrict__ part,
-                                                             double* __restrict__ mom) {
-  extern __shared__ unsigned long long lds_u[];
-  __shared__ double sb1[kMaxBp], sb2[kMaxBp];
-  __shared__ double red[kScatterThreads / 64];
-  const int field = blockIdx.y;
-  unsigned long long* rhs = lds_u;
+// Kernel bodies.  Each body serves the single-run kernel and the batched
+// ensemble kernel (which maps its block to a (run, local block) pair first),
+// so a run computes the same arithmetic, in the same order, in both.
+// Marker arrays hold component 0 at [a] and component 1 at [ld + a].
+
+// Scatter of one block: markers lb*NT + tid, stride nbk*NT; rhs in LDS.
+__device__ __forceinline__ void scatter_body(const MeshView& m, unsigned long long* rhs, double* sb1, double* sb2,
+                                             double* red, const double* __restrict__ v0,
+                                             const double* __restrict__ vin, const double* __restrict__ w, int N,
+                                             int ld, int fields, int field, double scale, double* __restrict__ vmid,
+                                             double* __restrict__ dv, long long* __restrict__ out_row,
+                                             double* __restrict__ mom_out, int lb, int nbk) {
   for (int k = threadIdx.x; k < m.n_dofs; k += blockDim.x) rhs[k] = 0ull;
   stage_bp(m, sb1, sb2);
   __syncthreads();
   double p1 = 0.0, p2 = 0.0, e2 = 0.0;
-  const int stride = gridDim.x * blockDim.x;
-  for (int a = blockIdx.x * blockDim.x + threadIdx.x; a < N; a += stride) {
+  const int stride = nbk * blockDim.x;
+  for (int a = lb * blockDim.x + threadIdx.x; a < N; a += stride) {
     double x, y;
     if (fields == 2) {
-      const double a1 = v0[a], a2 = v0[N + a], b1 = vin[a], b2 = vin[N + a];
+      const double a1 = v0[a], a2 = v0[ld + a], b1 = vin[a], b2 = vin[ld + a];
       const double m1 = 0.5 * (a1 + b1), m2 = 0.5 * (a2 + b2);
       if (field == 0) {
         vmid[a] = m1;
-        vmid[N + a] = m2;
+        vmid[ld + a] = m2;
         dv[a] = b1 - a1;
-        dv[N + a] = b2 - a2;
+        dv[ld + a] = b2 - a2;
         x = m1;
         y = m2;
       } else {
@@ -64,10 +63,10 @@ __global__ __launch_bounds__(kScatterThreads) void k_scatter(MeshView m, const d
       }
     } else {
       x = vin[a];
-      y = vin[N + a];
+      y = vin[ld + a];
     }
     const double wa = w[a];
-    if (mom && field == 0) {
+    if (mom_out && field == 0) {
       p1 += wa * x;
       p2 += wa * y;
       e2 += wa * (x * x + y * y);
@@ -88,16 +87,15 @@ __global__ __launch_bounds__(kScatterThreads) void k_scatter(MeshView m, const d
       }
   }
   __syncthreads();
-  long long* out = part + ((size_t)field * gridDim.x + blockIdx.x) * m.n_dofs;
-  for (int k = threadIdx.x; k < m.n_dofs; k += blockDim.x) out[k] = (long long)rhs[k];
-  if (mom && field == 0) {
+  for (int k = threadIdx.x; k < m.n_dofs; k += blockDim.x) out_row[k] = (long long)rhs[k];
+  if (mom_out && field == 0) {
     const double s1 = block_sum<kScatterThreads>(p1, red);
     const double s2 = block_sum<kScatterThreads>(p2, red);
     const double s3 = block_sum<kScatterThreads>(e2, red);
     if (threadIdx.x == 0) {
-      mom[3 * blockIdx.x + 0] = s1;
-      mom[3 * blockIdx.x + 1] = s2;
-      mom[3 * blockIdx.x + 2] = s3;
+      mom_out[0] = s1;
+      mom_out[1] = s2;
+      mom_out[2] = s3;
     }
   }
 }
@@ -110,19 +108,15 @@ __device__ __forceinline__ double rhs_of(const long long* __restrict__ P, int nb
   return (double)(long long)s * inv_scale;
 }
 
-// One workgroup per field.  X lives in dynamic LDS (n_dofs doubles).
-__global__ __launch_bounds__(kSolveThreads) void k_reduce_solve(MeshView m, const long long* __restrict__ part,
-                                                                int nblocks, double inv_scale,
-                                                                const double* __restrict__ direct,
-                                                                double* __restrict__ out) {
-  extern __shared__ double X[];
-  const int field = blockIdx.x;
+// rhs (exact row sum, or `direct`) -> M^{-1} rhs in LDS -> out.  One workgroup.
+__device__ __forceinline__ void solve_body(const MeshView& m, double* X, const long long* __restrict__ P, int nblocks,
+                                           double inv_scale, const double* __restrict__ direct,
+                                           double* __restrict__ out) {
   const int nd = m.n_dofs;
-  if (part) {
-    const long long* P = part + (size_t)field * nblocks * nd;
+  if (P) {
     for (int k = threadIdx.x; k < nd; k += blockDim.x) X[k] = rhs_of(P, nblocks, nd, k, inv_scale);
   } else {
-    for (int k = threadIdx.x; k < nd; k += blockDim.x) X[k] = direct[(size_t)field * nd + k];
+    for (int k = threadIdx.x; k < nd; k += blockDim.x) X[k] = direct[k];
   }
   __syncthreads();
   // direction 1: one line per i2 (contiguous along i1)
@@ -131,22 +125,17 @@ __global__ __launch_bounds__(kSolveThreads) void k_reduce_solve(MeshView m, cons
   // direction 2: one line per i1 (stride nd1)
   for (int l = threadIdx.x; l < m.nd1; l += blockDim.x) band_solve_line(X + l, m.nd2, m.nd1, m.chol2);
   __syncthreads();
-  for (int k = threadIdx.x; k < nd; k += blockDim.x) out[(size_t)field * nd + k] = X[k];
+  for (int k = threadIdx.x; k < nd; k += blockDim.x) out[k] = X[k];
 }
 
-// ---------------------------------------------------------------------------
-// Quadrature pass 1: one thread per (cell, Gauss point).  grid.y = field.
+// Quadrature pass 1 of one block (one thread per cell and Gauss point).
 constexpr int kQuadThreads = 256;
 
-__global__ __launch_bounds__(kQuadThreads) void k_quad1(MeshView m, const double* __restrict__ c0,
-                                                        const double* __restrict__ c1, int use_logsq,
-                                                        double* __restrict__ iota, double* __restrict__ spart) {
-  __shared__ double red[kQuadThreads / 64];
-  const int field = blockIdx.y;
-  const double* c = field == 0 ? c0 : c1;
+__device__ __forceinline__ void quad1_body(const MeshView& m, double* red, const double* __restrict__ c, int use_logsq,
+                                           double* __restrict__ iota, double* __restrict__ spart_out, int lb) {
   const int npt = m.nq * m.nq;
   const int nel = m.ne1 * m.ne2;
-  const int tid = blockIdx.x * blockDim.x + threadIdx.x;
+  const int tid = lb * blockDim.x + threadIdx.x;
   double s_loc = 0.0;
   if (tid < nel * npt) {
     const int e = tid / npt, pt = tid - e * npt;
@@ -173,18 +162,18 @@ __global__ __launch_bounds__(kQuadThreads) void k_quad1(MeshView m, const double
       ok = f > 1e-30;
       if (ok) lg = log(f);
     }
-    if (iota && field == 0) iota[(size_t)e * npt + pt] = ok ? (1.0 + lg) * wq : 0.0;
+    if (iota) iota[(size_t)e * npt + pt] = ok ? (1.0 + lg) * wq : 0.0;
     s_loc = ok ? -f * lg * wq : 0.0;
   }
-  if (spart) {
+  if (spart_out) {
     const double s = block_sum<kQuadThreads>(s_loc, red);
-    if (threadIdx.x == 0) spart[(size_t)field * gridDim.x + blockIdx.x] = s;
+    if (threadIdx.x == 0) spart_out[0] = s;
   }
 }
 
-// Quadrature pass 2: one thread per dof, gathering over the cells of its support.
-__global__ void k_quad2(MeshView m, const double* __restrict__ iota, double* __restrict__ r) {
-  const int k = blockIdx.x * blockDim.x + threadIdx.x;
+// Quadrature pass 2 for dof k: gather over the cells of its support.
+__device__ __forceinline__ void quad2_body(const MeshView& m, const double* __restrict__ iota, double* __restrict__ r,
+                                           int k) {
   if (k >= m.n_dofs) return;
   const int i1 = k % m.nd1, i2 = k / m.nd1;
   const int npt = m.nq * m.nq;
@@ -206,47 +195,32 @@ __global__ void k_quad2(MeshView m, const double* __restrict__ iota, double* __r
   r[k] = acc;
 }
 
-// rhs without the solve (collop_l2_rhs)
-__global__ void k_rhs(const long long* __restrict__ part, int nblocks, int nd, double inv_scale,
-                      double* __restrict__ out) {
-  const int k = blockIdx.x * blockDim.x + threadIdx.x;
-  if (k < nd) out[k] = rhs_of(part, nblocks, nd, k, inv_scale);
-}
-
-// out[i] = sum_j part[i * istride + j * jstride], j < n: one workgroup per
-// output, fixed strided partition over threads, then the fixed block tree.
+// sum_{j < n} base[j * jstride]: fixed strided partition over threads, then the
+// fixed block tree.  Result in thread 0.
 constexpr int kSumThreads = 256;
 
-__global__ __launch_bounds__(kSumThreads) void k_sum_rows(const double* __restrict__ part, int n, int istride,
-                                                         int jstride, double* __restrict__ out) {
-  __shared__ double red[kSumThreads / 64];
-  const int i = blockIdx.x;
+__device__ __forceinline__ double sum_body(const double* __restrict__ base, int n, int jstride, double* red) {
   double s = 0.0;
-  for (int j = threadIdx.x; j < n; j += kSumThreads) s += part[(size_t)i * istride + (size_t)j * jstride];
-  s = block_sum<kSumThreads>(s, red);
-  if (threadIdx.x == 0) out[i] = s;
+  for (int j = threadIdx.x; j < n; j += kSumThreads) s += base[(size_t)j * jstride];
+  return block_sum<kSumThreads>(s, red);
 }
 
-// ---------------------------------------------------------------------------
-// Gathers.  mode 0: out = G (N x 2).  mode 1: out = dS = -w G, dpart[block] =
-// (dv . dS, |dv|^2).  mode 2: out = clamp(grad f / max(|f|, fs_floor), +-g_max).
-// mode 3: out = (f, |grad f|) at the marker (warm-start features).
+// Gathers.  mode 0: out = G.  mode 1: out = dS = -w G, dpart_out = (dv . dS,
+// |dv|^2) of the block.  mode 2: out = clamp(grad f / max(|f|, fs_floor),
+// +-g_max).  mode 3: out = (f, |grad f|) at the marker (warm-start features).
 constexpr int kGatherThreads = 256;
 
-__global__ __launch_bounds__(kGatherThreads) void k_gather(MeshView m, const double* __restrict__ x,
-                                                           const double* __restrict__ coeff,
-                                                           const double* __restrict__ w,
-                                                           const double* __restrict__ dv, int N, int mode,
-                                                           double fs_floor, double g_max, double* __restrict__ out,
-                                                           double* __restrict__ dpart) {
-  __shared__ double sb1[kMaxBp], sb2[kMaxBp];
-  __shared__ double red[kGatherThreads / 64];
+__device__ __forceinline__ void gather_body(const MeshView& m, double* sb1, double* sb2, double* red,
+                                            const double* __restrict__ x, int ld, const double* __restrict__ coeff,
+                                            const double* __restrict__ w, const double* __restrict__ dv, int N,
+                                            int mode, double fs_floor, double g_max, double* __restrict__ out,
+                                            double* __restrict__ dpart_out, int lb, int nbk) {
   stage_bp(m, sb1, sb2);
   __syncthreads();
   double dot = 0.0, nrm = 0.0;
-  const int stride = gridDim.x * blockDim.x;
-  for (int a = blockIdx.x * blockDim.x + threadIdx.x; a < N; a += stride) {
-    const double v1 = x[a], v2 = x[N + a];
+  const int stride = nbk * blockDim.x;
+  for (int a = lb * blockDim.x + threadIdx.x; a < N; a += stride) {
+    const double v1 = x[a], v2 = x[ld + a];
     double o1 = 0.0, o2 = 0.0;
     Located L;
     if (locate(m, sb1, sb2, v1, v2, L)) {
@@ -281,29 +255,188 @@ __global__ __launch_bounds__(kGatherThreads) void k_gather(MeshView m, const dou
       const double wa = w[a];
       o1 = -wa * o1;
       o2 = -wa * o2;
-      const double e1 = dv[a], e2 = dv[N + a];
+      const double e1 = dv[a], e2 = dv[ld + a];
       dot += e1 * o1 + e2 * o2;
       nrm += e1 * e1 + e2 * e2;
     }
     out[a] = o1;
-    out[N + a] = o2;
+    out[ld + a] = o2;
   }
   if (mode == 1) {
     const double s1 = block_sum<kGatherThreads>(dot, red);
     const double s2 = block_sum<kGatherThreads>(nrm, red);
     if (threadIdx.x == 0) {
-      dpart[2 * blockIdx.x] = s1;
-      dpart[2 * blockIdx.x + 1] = s2;
+      dpart_out[0] = s1;
+      dpart_out[1] = s2;
     }
   }
 }
 
+// ---------------------------------------------------------------------------
+// Single-run kernels.
+
+// grid = (blocks, fields).  Dynamic LDS: n_dofs u64.
+__global__ __launch_bounds__(kScatterThreads) void k_scatter(MeshView m, const double* __restrict__ v0,
+                                                             const double* __restrict__ vin,
+                                                             const double* __restrict__ w, int N, int fields,
+                                                             double scale, double* __restrict__ vmid,
+                                                             double* __restrict__ dv,
+                                                             long long* __restrict__ part,
+                                                             double* __restrict__ mom) {
+  extern __shared__ unsigned long long lds_u[];
+  __shared__ double sb1[kMaxBp], sb2[kMaxBp];
+  __shared__ double red[kScatterThreads / 64];
+  const int field = blockIdx.y;
+  scatter_body(m, lds_u, sb1, sb2, red, v0, vin, w, N, N, fields, field, scale, vmid, dv,
+               part + ((size_t)field * gridDim.x + blockIdx.x) * m.n_dofs, mom ? mom + 3 * blockIdx.x : nullptr,
+               blockIdx.x, gridDim.x);
+}
+
+// One workgroup per field.  X lives in dynamic LDS (n_dofs doubles).
+__global__ __launch_bounds__(kSolveThreads) void k_reduce_solve(MeshView m, const long long* __restrict__ part,
+                                                                int nblocks, double inv_scale,
+                                                                const double* __restrict__ direct,
+                                                                double* __restrict__ out) {
+  extern __shared__ double X[];
+  const int field = blockIdx.x;
+  const int nd = m.n_dofs;
+  solve_body(m, X, part ? part + (size_t)field * nblocks * nd : nullptr, nblocks, inv_scale,
+             direct ? direct + (size_t)field * nd : nullptr, out + (size_t)field * nd);
+}
+
+__global__ __launch_bounds__(kQuadThreads) void k_quad1(MeshView m, const double* __restrict__ c0,
+                                                        const double* __restrict__ c1, int use_logsq,
+                                                        double* __restrict__ iota, double* __restrict__ spart) {
+  __shared__ double red[kQuadThreads / 64];
+  const int field = blockIdx.y;
+  quad1_body(m, red, field == 0 ? c0 : c1, use_logsq, field == 0 ? iota : nullptr,
+             spart ? spart + (size_t)field * gridDim.x + blockIdx.x : nullptr, blockIdx.x);
+}
+
+__global__ void k_quad2(MeshView m, const double* __restrict__ iota, double* __restrict__ r) {
+  quad2_body(m, iota, r, blockIdx.x * blockDim.x + threadIdx.x);
+}
+
+// rhs without the solve (collop_l2_rhs)
+__global__ void k_rhs(const long long* __restrict__ part, int nblocks, int nd, double inv_scale,
+                      double* __restrict__ out) {
+  const int k = blockIdx.x * blockDim.x + threadIdx.x;
+  if (k < nd) out[k] = rhs_of(part, nblocks, nd, k, inv_scale);
+}
+
+// out[i] = sum_j part[i * istride + j * jstride], j < n: one workgroup per output.
+__global__ __launch_bounds__(kSumThreads) void k_sum_rows(const double* __restrict__ part, int n, int istride,
+                                                         int jstride, double* __restrict__ out) {
+  __shared__ double red[kSumThreads / 64];
+  const int i = blockIdx.x;
+  const double s = sum_body(part + (size_t)i * istride, n, jstride, red);
+  if (threadIdx.x == 0) out[i] = s;
+}
+
+__global__ __launch_bounds__(kGatherThreads) void k_gather(MeshView m, const double* __restrict__ x,
+                                                           const double* __restrict__ coeff,
+                                                           const double* __restrict__ w,
+                                                           const double* __restrict__ dv, int N, int mode,
+                                                           double fs_floor, double g_max, double* __restrict__ out,
+                                                           double* __restrict__ dpart) {
+  __shared__ double sb1[kMaxBp], sb2[kMaxBp];
+  __shared__ double red[kGatherThreads / 64];
+  gather_body(m, sb1, sb2, red, x, N, coeff, w, dv, N, mode, fs_floor, g_max, out,
+              mode == 1 ? dpart + 2 * blockIdx.x : nullptr, blockIdx.x, gridDim.x);
+}
+
+// ---------------------------------------------------------------------------
+// Batched ensemble kernels.  Block tables map blockIdx.x to (run, local block);
+// runs whose control block says inactive return before any barrier.
+__device__ __forceinline__ bool ens_skip(const EnsView& E, int r) { return E.ctl && !E.ctl[r].active; }
+
+// grid = (sum of scatter blocks, grid_fields).  field 1 reads vin.
+__global__ __launch_bounds__(kScatterThreads) void k_scatter_b(MeshView m, EnsView E, const double* __restrict__ v0,
+                                                               const double* __restrict__ vin,
+                                                               const double* __restrict__ w, int fields,
+                                                               double* __restrict__ vmid, double* __restrict__ dv,
+                                                               long long* __restrict__ part,
+                                                               double* __restrict__ mom) {
+  const int2 t = E.sblk[blockIdx.x];
+  if (ens_skip(E, t.x)) return;
+  extern __shared__ unsigned long long lds_u[];
+  __shared__ double sb1[kMaxBp], sb2[kMaxBp];
+  __shared__ double red[kScatterThreads / 64];
+  const EnsRun& R = E.runs[t.x];
+  const int field = blockIdx.y;
+  const int row = R.sblk0 + t.y;
+  scatter_body(m, lds_u, sb1, sb2, red, v0 ? v0 + R.off : nullptr, vin + R.off, w + R.off, R.n, E.NT, fields, field,
+               R.scale, vmid ? vmid + R.off : nullptr, dv ? dv + R.off : nullptr,
+               part + ((size_t)field * E.n_sblk + row) * m.n_dofs, mom ? mom + 3 * row : nullptr, t.y, R.nsb);
+}
+
+// grid = (R, fields).  From scatter rows (part != nullptr) or from
+// vec[(r * 4 + in_slot + field) * n_dofs]; out to vec[(r * 4 + out_slot + field) * n_dofs]
+// (out_slot < 0: to fc[r * n_dofs]).
+__global__ __launch_bounds__(kSolveThreads) void k_solve_b(MeshView m, EnsView E, const long long* __restrict__ part,
+                                                           int in_slot, int out_slot, double* __restrict__ vec,
+                                                           double* __restrict__ fc) {
+  const int r = blockIdx.x, field = blockIdx.y;
+  if (ens_skip(E, r)) return;
+  extern __shared__ double X[];
+  const EnsRun& R = E.runs[r];
+  const int nd = m.n_dofs;
+  double* out = out_slot >= 0 ? vec + ((size_t)r * 4 + out_slot + field) * nd : fc + (size_t)r * nd;
+  solve_body(m, X, part ? part + ((size_t)field * E.n_sblk + R.sblk0) * nd : nullptr, R.nsb, R.inv_scale,
+             part ? nullptr : vec + ((size_t)r * 4 + in_slot + field) * nd, out);
+}
+
+// grid = (quad blocks, fields, R).  Field y reads c + r * rstride + y * nd.
+__global__ __launch_bounds__(kQuadThreads) void k_quad1_b(MeshView m, EnsView E, const double* __restrict__ c,
+                                                          int rstride, int use_logsq, double* __restrict__ iota,
+                                                          double* __restrict__ spart) {
+  const int r = blockIdx.z, field = blockIdx.y;
+  if (ens_skip(E, r)) return;
+  __shared__ double red[kQuadThreads / 64];
+  const size_t npts = (size_t)m.ne1 * m.ne2 * m.nq * m.nq;
+  quad1_body(m, red, c + (size_t)r * rstride + (size_t)field * m.n_dofs, use_logsq,
+             (iota && field == 0) ? iota + r * npts : nullptr,
+             spart ? spart + ((size_t)r * 2 + field) * gridDim.x + blockIdx.x : nullptr, blockIdx.x);
+}
+
+// grid = (dof blocks, R): r_out = vec[(r * 4 + 2) * n_dofs]
+__global__ void k_quad2_b(MeshView m, EnsView E, const double* __restrict__ iota, double* __restrict__ vec) {
+  const int r = blockIdx.y;
+  if (ens_skip(E, r)) return;
+  const size_t npts = (size_t)m.ne1 * m.ne2 * m.nq * m.nq;
+  quad2_body(m, iota + r * npts, vec + ((size_t)r * 4 + 2) * m.n_dofs, blockIdx.x * blockDim.x + threadIdx.x);
+}
+
+// grid = (R, count): out[r * ostride + c] = sum_{j < cnt[r]} part[row0[r] + c * istride + j * jstride]
+__global__ __launch_bounds__(kSumThreads) void k_sum_rows_b(EnsView E, const double* __restrict__ part,
+                                                           const int* __restrict__ row0, const int* __restrict__ cnt,
+                                                           int istride, int jstride, double* __restrict__ out,
+                                                           int ostride) {
+  const int r = blockIdx.x, c = blockIdx.y;
+  if (ens_skip(E, r)) return;
+  __shared__ double red[kSumThreads / 64];
+  const double s = sum_body(part + (size_t)row0[r] + (size_t)c * istride, cnt[r], jstride, red);
+  if (threadIdx.x == 0) out[(size_t)r * ostride + c] = s;
+}
+
+// grid = sum of gather blocks.  coeff = vec[(r * 4 + slot) * n_dofs].
+__global__ __launch_bounds__(kGatherThreads) void k_gather_b(MeshView m, EnsView E, const double* __restrict__ x,
+                                                             const double* __restrict__ vec, int slot,
+                                                             const double* __restrict__ w,
+                                                             const double* __restrict__ dv, int mode,
+                                                             double* __restrict__ out, double* __restrict__ dpart) {
+  const int2 t = E.gblk[blockIdx.x];
+  if (ens_skip(E, t.x)) return;
+  __shared__ double sb1[kMaxBp], sb2[kMaxBp];
+  __shared__ double red[kGatherThreads / 64];
+  const EnsRun& R = E.runs[t.x];
+  gather_body(m, sb1, sb2, red, x + R.off, E.NT, vec + ((size_t)t.x * 4 + slot) * m.n_dofs, w ? w + R.off : nullptr,
+              dv ? dv + R.off : nullptr, R.n, mode, 0.0, 0.0, out + R.off,
+              mode == 1 ? dpart + 2 * (R.gblk0 + t.y) : nullptr, t.y, R.ngb);
+}
+
 }  // namespace
 
-int scatter_blocks(int N) {
-  int b = (N + kScatterThreads * 4 - 1) / (kScatterThreads * 4);
-  return b < 1 ? 1 : (b > 128 ? 128 : b);
-}
 
 double scatter_scale(double w_abs_sum) {
   return w_abs_sum > 0.0 ? 4611686018427387904.0 / (w_abs_sum * (1.0 + 1e-12)) : 1.0;  // 2^62 / W
@@ -351,15 +484,52 @@ void launch_sum_rows(const double* part, int n, int count, int istride, int jstr
   hipLaunchKernelGGL(k_sum_rows, dim3(count), dim3(kSumThreads), 0, st, part, n, istride, jstride, out);
 }
 
-int gather_blocks(int N) {
-  int b = (N + kGatherThreads * 2 - 1) / (kGatherThreads * 2);
-  return b < 1 ? 1 : (b > 512 ? 512 : b);
-}
 
 void launch_gather(const MeshView& m, const double* x, const double* coeff, const double* w, const double* dv,
                    int N, int mode, double fs_floor, double g_max, double* out, double* dpart, hipStream_t st) {
   hipLaunchKernelGGL(k_gather, dim3(gather_blocks(N)), dim3(kGatherThreads), 0, st, m, x, coeff, w, dv, N, mode,
                      fs_floor, g_max, out, dpart);
+}
+
+}  // namespace collop
+
+namespace collop {
+
+void launch_scatter_b(const MeshView& m, const EnsView& E, const double* v0, const double* vin, const double* w,
+                      int fields, int grid_fields, double* vmid, double* dv, long long* part, double* mom,
+                      hipStream_t st) {
+  const size_t lds = (size_t)m.n_dofs * sizeof(unsigned long long);
+  hipLaunchKernelGGL(k_scatter_b, dim3(E.n_sblk, grid_fields), dim3(kScatterThreads), lds, st, m, E, v0, vin, w,
+                     fields, vmid, dv, part, mom);
+}
+
+void launch_solve_b(const MeshView& m, const EnsView& E, const long long* part, int fields, int in_slot, int out_slot,
+                    double* vec, double* fc, hipStream_t st) {
+  const size_t lds = (size_t)m.n_dofs * sizeof(double);
+  hipLaunchKernelGGL(k_solve_b, dim3(E.R, fields), dim3(kSolveThreads), lds, st, m, E, part, in_slot, out_slot, vec,
+                     fc);
+}
+
+void launch_quad1_b(const MeshView& m, const EnsView& E, const double* c, int rstride, int fields, int use_logsq,
+                    double* iota, double* spart, hipStream_t st) {
+  hipLaunchKernelGGL(k_quad1_b, dim3(quad_blocks(m), fields, E.R), dim3(kQuadThreads), 0, st, m, E, c, rstride,
+                     use_logsq, iota, spart);
+}
+
+void launch_quad2_b(const MeshView& m, const EnsView& E, const double* iota, double* vec, hipStream_t st) {
+  hipLaunchKernelGGL(k_quad2_b, dim3((m.n_dofs + 127) / 128, E.R), dim3(128), 0, st, m, E, iota, vec);
+}
+
+void launch_sum_rows_b(const EnsView& E, const double* part, const int* row0, const int* cnt, int count, int istride,
+                       int jstride, double* out, int ostride, hipStream_t st) {
+  hipLaunchKernelGGL(k_sum_rows_b, dim3(E.R, count), dim3(kSumThreads), 0, st, E, part, row0, cnt, istride, jstride,
+                     out, ostride);
+}
+
+void launch_gather_b(const MeshView& m, const EnsView& E, const double* x, const double* vec, int slot,
+                     const double* w, const double* dv, int mode, double* out, double* dpart, hipStream_t st) {
+  hipLaunchKernelGGL(k_gather_b, dim3(E.n_gblk), dim3(kGatherThreads), 0, st, m, E, x, vec, slot, w, dv, mode, out,
+                     dpart);
 }
 
 }  // namespace collop

@@ -136,12 +136,12 @@ __global__ __launch_bounds__(kThreads) void k_pair_f32c(const Src32* __restrict_
   }
 }
 
-__global__ void k_pack(const double* __restrict__ v, const double* __restrict__ G, const double* __restrict__ w,
-                       int N, double lo1, double hi1, double lo2, double hi2, Src64* __restrict__ s64,
-                       Src32* __restrict__ s32) {
-  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+__device__ __forceinline__ void pack_body(const double* __restrict__ v, const double* __restrict__ G,
+                                          const double* __restrict__ w, int N, int ld, double lo1, double hi1,
+                                          double lo2, double hi2, Src64* __restrict__ s64, Src32* __restrict__ s32,
+                                          int i) {
   if (i >= N) return;
-  const double a = v[i], b = v[N + i];
+  const double a = v[i], b = v[ld + i];
   const bool in = in_box(a, b, lo1, hi1, lo2, hi2);
   const double wm = in ? w[i] : 0.0;
   if (s64) {
@@ -149,7 +149,7 @@ __global__ void k_pack(const double* __restrict__ v, const double* __restrict__ 
     s.v1 = a;
     s.v2 = b;
     s.g1 = G[i];
-    s.g2 = G[N + i];
+    s.g2 = G[ld + i];
     s.w = wm;
     s.in_domain = in ? 1.0 : 0.0;
     s.pad0 = s.pad1 = 0.0;
@@ -160,11 +160,99 @@ __global__ void k_pack(const double* __restrict__ v, const double* __restrict__ 
     s.v1 = (float)a;
     s.v2 = (float)b;
     s.g1 = (float)G[i];
-    s.g2 = (float)G[N + i];
+    s.g2 = (float)G[ld + i];
     s.w = (float)wm;
     s.in_domain = in ? 1.0f : 0.0f;
     s.pad0 = s.pad1 = 0.0f;
     s32[i] = s;
+  }
+}
+
+__global__ void k_pack(const double* __restrict__ v, const double* __restrict__ G, const double* __restrict__ w,
+                       int N, double lo1, double hi1, double lo2, double hi2, Src64* __restrict__ s64,
+                       Src32* __restrict__ s32) {
+  pack_body(v, G, w, N, N, lo1, hi1, lo2, hi2, s64, s32, blockIdx.x * blockDim.x + threadIdx.x);
+}
+
+__global__ void k_pack_b(EnsView E, const double* __restrict__ v, const double* __restrict__ G,
+                         const double* __restrict__ w, double lo1, double hi1, double lo2, double hi2,
+                         Src64* __restrict__ s64, Src32* __restrict__ s32) {
+  const int2 t = E.pblk[blockIdx.x];
+  const EnsRun& R = E.runs[t.x];
+  pack_body(v + R.off, G + R.off, w + R.off, R.n, E.NT, lo1, hi1, lo2, hi2, s64 ? s64 + R.off : nullptr,
+            s32 ? s32 + R.off : nullptr, t.y * blockDim.x + threadIdx.x);
+}
+
+// Batched pair sum: one persistent launch walks the tile list (longest tiles
+// first) with a grid stride.  A tile is BT consecutive targets of one run
+// against one source slice of that run; each lane owns one target and runs the
+// same per-target loop as k_pair_f64<1, 1> / k_pair_f32c<1, 32>, so a run's
+// partials equal the single-run kernel's at the same slice count.
+template <int BT>
+__global__ __launch_bounds__(BT) void k_pair_b(EnsView E, const int4* __restrict__ tiles, int ntiles,
+                                               const Src64* __restrict__ s64, const Src32* __restrict__ s32,
+                                               int force_f64, double* __restrict__ part) {
+  for (int t = blockIdx.x; t < ntiles; t += gridDim.x) {
+    const int4 tl = tiles[t];
+    const int r = tl.x;
+    const bool f64 = force_f64 || E.ctl[r].kind == 0;
+    if (!force_f64 && !E.ctl[r].active) continue;
+    const EnsRun& R = E.runs[r];
+    const int n = R.n, S = R.S;
+    const int chunk = (n + S - 1) / S;
+    const int a0 = R.off + tl.z * chunk;
+    const int a1 = R.off + min(n, tl.z * chunk + chunk);
+    const int loc = tl.y + (int)threadIdx.x;
+    double* pr = part + R.poff + (size_t)tl.z * 2 * n;
+    if (f64) {
+      double vg1 = 0.0, vg2 = 0.0, Gg1 = 0.0, Gg2 = 0.0, acc1 = 0.0, acc2 = 0.0;
+      bool act = false;
+      if (loc < n) {
+        const Src64 q = s64[R.off + loc];
+        vg1 = q.v1;
+        vg2 = q.v2;
+        Gg1 = q.g1;
+        Gg2 = q.g2;
+        act = q.in_domain != 0.0;
+      }
+#pragma unroll 2
+      for (int a = a0; a < a1; ++a) {
+        const double av1 = s64[a].v1, av2 = s64[a].v2, ag1 = s64[a].g1, ag2 = s64[a].g2, aw = s64[a].w;
+        double d1, d2, tt;
+        pair_term_f64<1>(vg1, vg2, Gg1, Gg2, av1, av2, ag1, ag2, aw, acc1, acc2, d1, d2, tt);
+      }
+      if (loc < n) {
+        pr[loc] = act ? acc1 : 0.0;
+        pr[n + loc] = act ? acc2 : 0.0;
+      }
+    } else {
+      float vg1 = 0.0f, vg2 = 0.0f, Gg1 = 0.0f, Gg2 = 0.0f, b1 = 0.0f, b2 = 0.0f, h1 = 0.0f, h2 = 0.0f, l1 = 0.0f,
+            l2 = 0.0f;
+      bool act = false;
+      if (loc < n) {
+        const Src32 q = s32[R.off + loc];
+        vg1 = q.v1;
+        vg2 = q.v2;
+        Gg1 = q.g1;
+        Gg2 = q.g2;
+        act = q.in_domain != 0.0f;
+      }
+      for (int ab = a0; ab < a1; ab += 32) {
+        const int ae = min(a1, ab + 32);
+#pragma unroll 4
+        for (int a = ab; a < ae; ++a) {
+          const float av1 = s32[a].v1, av2 = s32[a].v2, ag1 = s32[a].g1, ag2 = s32[a].g2, aw = s32[a].w;
+          pair_term_f32(vg1, vg2, Gg1, Gg2, av1, av2, ag1, ag2, aw, b1, b2);
+        }
+        two_sum_fold(h1, l1, b1);
+        two_sum_fold(h2, l2, b2);
+        b1 = b2 = 0.0f;
+      }
+      if (loc < n) {
+        pr[loc] = act ? (double)h1 + (double)l1 : 0.0;
+        pr[n + loc] = act ? (double)h2 + (double)l2 : 0.0;
+      }
+    }
   }
 }
 
@@ -229,6 +317,31 @@ void launch_pair_f32c(const Src32* src, int N, const PairTile& t, double* part, 
 void launch_slice_sum(const double* part, int S, int64_t len, double* out, hipStream_t st) {
   if (len <= 0) return;
   hipLaunchKernelGGL(k_slice_sum, dim3((unsigned)((len + 255) / 256)), dim3(256), 0, st, part, S, len, out);
+}
+
+void launch_pack_b(const EnsView& E, const double* v, const double* G, const double* w, double lo1, double hi1,
+                   double lo2, double hi2, Src64* s64, Src32* s32, hipStream_t st) {
+  hipLaunchKernelGGL(k_pack_b, dim3(E.n_pblk), dim3(256), 0, st, E, v, G, w, lo1, hi1, lo2, hi2, s64, s32);
+}
+
+void launch_pair_b(const EnsView& E, const EnsTiles& T, const Src64* s64, const Src32* s32, int force_f64,
+                   double* part, hipStream_t st) {
+  if (T.ntiles <= 0) return;
+  const dim3 grid(T.grid);
+  switch (T.bt) {
+    case 64:
+      hipLaunchKernelGGL(k_pair_b<64>, grid, dim3(64), 0, st, E, T.tiles, T.ntiles, s64, s32, force_f64, part);
+      break;
+    case 128:
+      hipLaunchKernelGGL(k_pair_b<128>, grid, dim3(128), 0, st, E, T.tiles, T.ntiles, s64, s32, force_f64, part);
+      break;
+    case 512:
+      hipLaunchKernelGGL(k_pair_b<512>, grid, dim3(512), 0, st, E, T.tiles, T.ntiles, s64, s32, force_f64, part);
+      break;
+    default:
+      hipLaunchKernelGGL(k_pair_b<256>, grid, dim3(256), 0, st, E, T.tiles, T.ntiles, s64, s32, force_f64, part);
+      break;
+  }
 }
 
 }  // namespace collop

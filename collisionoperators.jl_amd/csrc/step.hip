@@ -10,20 +10,19 @@ namespace {
 
 constexpr int kEwThreads = 256;
 
+// Address of element i of a flattened (2 x n) component pair stored with
+// component stride ld (ld = n for a single run).
+__device__ __forceinline__ size_t el(int i, int n, int ld) { return i < n ? (size_t)i : (size_t)ld + (i - n); }
+
 // Gonzalez lambda (solver.jl:94-105) from the gather partials, recomputed
 // redundantly (same fixed order) by every block, then G_eff = -(dS + lambda dv)/w
-// (solver.jl:107-111) and the pair-source records.
-__global__ __launch_bounds__(kEwThreads) void k_geff_pack(const double* __restrict__ vmid,
-                                                          const double* __restrict__ dS,
-                                                          const double* __restrict__ dv,
-                                                          const double* __restrict__ w, int N,
-                                                          const double* __restrict__ dpart, int nparts,
-                                                          const double* __restrict__ S1,
-                                                          const double* __restrict__ S0, int use_gonzalez,
-                                                          double lo1, double hi1, double lo2, double hi2,
-                                                          Src64* __restrict__ s64, Src32* __restrict__ s32,
-                                                          double* __restrict__ lambda_out) {
-  __shared__ double lam_sh;
+// (solver.jl:107-111) and the pair-source records of markers lb*256 + tid.
+__device__ __forceinline__ void geff_body(const double* __restrict__ vmid, const double* __restrict__ dS,
+                                          const double* __restrict__ dv, const double* __restrict__ w, int N, int ld,
+                                          const double* __restrict__ dpart, int nparts, const double* __restrict__ S1,
+                                          const double* __restrict__ S0, int use_gonzalez, double lo1, double hi1,
+                                          double lo2, double hi2, Src64* __restrict__ s64, Src32* __restrict__ s32,
+                                          double* __restrict__ lambda_out, double* lam_sh, int lb) {
   if (threadIdx.x == 0) {
     double lam = 0.0;
     if (use_gonzalez) {
@@ -34,17 +33,17 @@ __global__ __launch_bounds__(kEwThreads) void k_geff_pack(const double* __restri
       }
       lam = nrm2 > 1e-30 ? (S1[0] - S0[0] - dot) / nrm2 : 0.0;
     }
-    lam_sh = lam;
-    if (blockIdx.x == 0 && lambda_out) lambda_out[0] = lam;
+    *lam_sh = lam;
+    if (lb == 0 && lambda_out) lambda_out[0] = lam;
   }
   __syncthreads();
-  const double lam = lam_sh;
-  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  const double lam = *lam_sh;
+  const int i = lb * blockDim.x + threadIdx.x;
   if (i >= N) return;
   const double iw = 1.0 / w[i];
   const double g1 = -(dS[i] + lam * dv[i]) * iw;
-  const double g2 = -(dS[N + i] + lam * dv[N + i]) * iw;
-  const double a = vmid[i], b = vmid[N + i];
+  const double g2 = -(dS[ld + i] + lam * dv[ld + i]) * iw;
+  const double a = vmid[i], b = vmid[ld + i];
   const bool in = in_box(a, b, lo1, hi1, lo2, hi2);
   const double wm = in ? w[i] : 0.0;
   if (s64) {
@@ -71,41 +70,42 @@ __global__ __launch_bounds__(kEwThreads) void k_geff_pack(const double* __restri
   }
 }
 
-// Map epilogue: slice sum -> v_out = v0 + dt dot_v (solver.jl:225) -> r = Gv - v1
-// (solver.jl:339), the new Anderson column (solver.jl:391-394) and the partial
-// sums of |r|^2, |v1|^2, |Gv - v0|^2, dF_col . dF_j and dF_j . r
-// (solver.jl:398-399).
-__global__ __launch_bounds__(kEwThreads) void k_epilogue(EpiArgs A, int N) {
-  __shared__ double red[kEwThreads / 64];
+// Map epilogue of block lb over the flattened components i < 2n: slice sum ->
+// v_out = v0 + dt dot_v (solver.jl:225) -> r = Gv - v1 (solver.jl:339), the new
+// Anderson column (solver.jl:391-394) and the block's partial sums of |r|^2,
+// |v1|^2, |Gv - v0|^2, dF_col . dF_j and dF_j . r (solver.jl:398-399).  The
+// partials `part` are compact [S][2n]; dF / dG columns are `cs` apart.
+__device__ __forceinline__ void epilogue_body(const EpiArgs& A, int N, int ld, size_t cs, double* red, int lb) {
   const int n2 = 2 * N;
-  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  const int i = lb * blockDim.x + threadIdx.x;
   double st[kStatWidth];
 #pragma unroll
   for (int j = 0; j < kStatWidth; ++j) st[j] = 0.0;
   if (i < n2) {
+    const size_t e = el(i, N, ld);
     double s = 0.0;
     for (int k = 0; k < A.S; ++k) s += A.part[(size_t)k * n2 + i];
-    const double gv = A.v0[i] + A.dt * s;
-    A.Gv[i] = gv;
-    const double x1 = A.v1[i];
+    const double gv = A.v0[e] + A.dt * s;
+    A.Gv[e] = gv;
+    const double x1 = A.v1[e];
     const double ri = gv - x1;
-    A.r[i] = ri;
+    A.r[e] = ri;
     st[0] = ri * ri;
     st[1] = x1 * x1;
-    const double inc = gv - A.v0[i];
+    const double inc = gv - A.v0[e];
     st[2] = inc * inc;
     // col >= 0: write the new column and its Gram row; hist > 0: dF_j . r
     // (col < 0 with hist > 0 re-bases the window on a new map at the same iterate)
     double f = 0.0;
     if (A.col >= 0) {
-      f = ri - A.rp[i];
-      A.dF[(size_t)A.col * n2 + i] = f;
-      A.dG[(size_t)A.col * n2 + i] = gv - A.Gp[i];
+      f = ri - A.rp[e];
+      A.dF[A.col * cs + e] = f;
+      A.dG[A.col * cs + e] = gv - A.Gp[e];
     }
 #pragma unroll
     for (int j = 0; j < kMaxWindow; ++j) {
       if (j < A.hist) {
-        const double xj = (j == A.col) ? f : A.dF[(size_t)j * n2 + i];
+        const double xj = (j == A.col) ? f : A.dF[j * cs + e];
         if (A.col >= 0) st[3 + j] = xj * f;
         st[3 + kMaxWindow + j] = xj * ri;
       }
@@ -117,7 +117,7 @@ __global__ __launch_bounds__(kEwThreads) void k_epilogue(EpiArgs A, int N) {
                       (j >= 3 + kMaxWindow && j < 3 + kMaxWindow + A.hist);
     if (!used) continue;  // uniform across the block
     const double t = block_sum<kEwThreads>(st[j], red);
-    if (threadIdx.x == 0) A.stat_part[(size_t)blockIdx.x * kStatWidth + j] = t;
+    if (threadIdx.x == 0) A.stat_part[j] = t;
   }
 }
 
@@ -126,24 +126,146 @@ struct Gamma {
   double g[kMaxWindow];
 };
 
+__device__ __forceinline__ double update_value(const double* __restrict__ Gv, const double* __restrict__ dG, size_t e,
+                                               size_t cs, const double* g, int hist, double beta, double v1) {
+  double x = Gv[e];
+  for (int j = 0; j < hist; ++j) x -= dG[j * cs + e] * g[j];
+  return beta * x + (1.0 - beta) * v1;
+}
+
+// v_out = v + dt * (slice sum) for flattened element i < 2n  (main.jl:245)
+__device__ __forceinline__ void euler_body(const double* __restrict__ part, int S, const double* __restrict__ v,
+                                           double dt, int N, int ld, double* __restrict__ out,
+                                           double* __restrict__ dotv, int i) {
+  const int n2 = 2 * N;
+  if (i >= n2) return;
+  const size_t e = el(i, N, ld);
+  double s = 0.0;
+  for (int k = 0; k < S; ++k) s += part[(size_t)k * n2 + i];
+  if (dotv) dotv[e] = s;
+  out[e] = v[e] + dt * s;
+}
+
+// ---- single-run kernels ---------------------------------------------------------
+__global__ __launch_bounds__(kEwThreads) void k_geff_pack(const double* __restrict__ vmid,
+                                                          const double* __restrict__ dS,
+                                                          const double* __restrict__ dv,
+                                                          const double* __restrict__ w, int N,
+                                                          const double* __restrict__ dpart, int nparts,
+                                                          const double* __restrict__ S1,
+                                                          const double* __restrict__ S0, int use_gonzalez,
+                                                          double lo1, double hi1, double lo2, double hi2,
+                                                          Src64* __restrict__ s64, Src32* __restrict__ s32,
+                                                          double* __restrict__ lambda_out) {
+  __shared__ double lam_sh;
+  geff_body(vmid, dS, dv, w, N, N, dpart, nparts, S1, S0, use_gonzalez, lo1, hi1, lo2, hi2, s64, s32, lambda_out,
+            &lam_sh, blockIdx.x);
+}
+
+__global__ __launch_bounds__(kEwThreads) void k_epilogue(EpiArgs A, int N) {
+  __shared__ double red[kEwThreads / 64];
+  EpiArgs a = A;
+  a.stat_part = A.stat_part + (size_t)blockIdx.x * kStatWidth;
+  epilogue_body(a, N, N, (size_t)2 * N, red, blockIdx.x);
+}
+
 __global__ void k_update(double* __restrict__ v1, const double* __restrict__ Gv, const double* __restrict__ dG,
                          Gamma gam, int hist, double beta, int n2) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n2) return;
-  double x = Gv[i];
-  for (int j = 0; j < hist; ++j) x -= dG[(size_t)j * n2 + i] * gam.g[j];
-  v1[i] = beta * x + (1.0 - beta) * v1[i];
+  v1[i] = update_value(Gv, dG, i, n2, gam.g, hist, beta, v1[i]);
 }
 
-// v1 = v + dt * dot_v  (main.jl:245); also keeps dot_v.
-__global__ void k_euler(const double* __restrict__ part, int S, const double* __restrict__ v, double dt, int n2,
+__global__ void k_euler(const double* __restrict__ part, int S, const double* __restrict__ v, double dt, int N,
                         double* __restrict__ out, double* __restrict__ dotv) {
-  const int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= n2) return;
-  double s = 0.0;
-  for (int k = 0; k < S; ++k) s += part[(size_t)k * n2 + i];
-  if (dotv) dotv[i] = s;
-  out[i] = v[i] + dt * s;
+  euler_body(part, S, v, dt, N, N, out, dotv, blockIdx.x * blockDim.x + threadIdx.x);
+}
+
+// ---- batched ensemble kernels ---------------------------------------------------
+// scal is [R][8]: S0, S1, lambda, P1, P2, |v|^2 w, -, -
+__global__ __launch_bounds__(kEwThreads) void k_geff_pack_b(EnsView E, const double* __restrict__ vmid,
+                                                            const double* __restrict__ dS,
+                                                            const double* __restrict__ dv,
+                                                            const double* __restrict__ w,
+                                                            const double* __restrict__ dpart, double* __restrict__ scal,
+                                                            int use_gonzalez, double lo1, double hi1, double lo2,
+                                                            double hi2, Src64* __restrict__ s64,
+                                                            Src32* __restrict__ s32) {
+  const int2 t = E.pblk[blockIdx.x];
+  const RunCtl& c = E.ctl[t.x];
+  if (!c.active) return;
+  __shared__ double lam_sh;
+  const EnsRun& R = E.runs[t.x];
+  double* sc = scal + 8 * t.x;
+  geff_body(vmid + R.off, dS + R.off, dv + R.off, w + R.off, R.n, E.NT, dpart + 2 * R.gblk0, R.ngb, sc + 1, sc + 0,
+            use_gonzalez, lo1, hi1, lo2, hi2, c.kind == 0 ? s64 + R.off : nullptr, c.kind == 1 ? s32 + R.off : nullptr,
+            sc + 2, &lam_sh, t.y);
+}
+
+
+__global__ __launch_bounds__(kEwThreads) void k_epilogue_b(EnsView E, EnsBufs B, double dt) {
+  const int2 t = E.eblk[blockIdx.x];
+  const RunCtl& c = E.ctl[t.x];
+  if (!c.active) return;
+  __shared__ double red[kEwThreads / 64];
+  const EnsRun& R = E.runs[t.x];
+  EpiArgs a;
+  a.part = B.part + R.poff;
+  a.S = R.S;
+  a.v0 = B.v0 + R.off;
+  a.v1 = B.v1 + R.off;
+  a.dt = dt;
+  a.Gv = B.GV[c.cur] + R.off;
+  a.r = B.R[c.cur] + R.off;
+  a.rp = B.R[1 - c.cur] + R.off;
+  a.Gp = B.GV[1 - c.cur] + R.off;
+  a.dF = B.dF + R.off;
+  a.dG = B.dG + R.off;
+  a.col = c.col;
+  a.hist = c.hist;
+  a.stat_part = B.statp + (size_t)(R.eblk0 + t.y) * kStatWidth;
+  epilogue_body(a, R.n, E.NT, (size_t)2 * E.NT, red, t.y);
+}
+
+// Per-run update of the iterate before the map (RunCtl::upd, copy_best).
+__global__ __launch_bounds__(kEwThreads) void k_update_b(EnsView E, EnsBufs B) {
+  const int2 t = E.eblk[blockIdx.x];
+  const RunCtl& c = E.ctl[t.x];
+  if (c.upd == 0 && !c.copy_best) return;
+  const EnsRun& R = E.runs[t.x];
+  const int i = t.y * kEwThreads + threadIdx.x;
+  if (i >= 2 * R.n) return;
+  const size_t e = R.off + el(i, R.n, E.NT);
+  const double* GV = B.GV[1 - c.cur];
+  const double x = B.v1[e];
+  if (c.copy_best) B.best[e] = c.exit_picard ? GV[e] : x;
+  double nx = x;
+  if (c.upd == 1) {
+    nx = update_value(GV, B.dG, e, (size_t)2 * E.NT, c.gamma, c.upd_hist, c.beta, x);
+  } else if (c.upd == 2) {
+    nx = GV[e];
+  } else if (c.upd == 3) {
+    nx = c.copy_best ? (c.exit_picard ? GV[e] : x) : B.best[e];
+  }
+  B.v1[e] = nx;
+}
+
+__global__ void k_euler_b(EnsView E, const double* __restrict__ part, const double* __restrict__ v, double dt,
+                          double* __restrict__ out, double* __restrict__ dotv) {
+  const int2 t = E.eblk[blockIdx.x];
+  const EnsRun& R = E.runs[t.x];
+  euler_body(part + R.poff, R.S, v + R.off, dt, R.n, E.NT, out + R.off, dotv ? dotv + R.off : nullptr,
+             t.y * kEwThreads + threadIdx.x);
+}
+
+// dst <- src over every run's markers (both components)
+__global__ void k_copy_b(EnsView E, const double* __restrict__ src, double* __restrict__ dst) {
+  const int2 t = E.eblk[blockIdx.x];
+  const EnsRun& R = E.runs[t.x];
+  const int i = t.y * kEwThreads + threadIdx.x;
+  if (i >= 2 * R.n) return;
+  const size_t e = R.off + el(i, R.n, E.NT);
+  dst[e] = src[e];
 }
 
 }  // namespace
@@ -173,7 +295,35 @@ void launch_update(double* v1, const double* Gv, const double* dG, const double*
 void launch_euler(const double* part, int S, const double* v, double dt, int N, double* out, double* dotv,
                   hipStream_t st) {
   const int n2 = 2 * N;
-  hipLaunchKernelGGL(k_euler, dim3((n2 + 255) / 256), dim3(256), 0, st, part, S, v, dt, n2, out, dotv);
+  hipLaunchKernelGGL(k_euler, dim3((n2 + 255) / 256), dim3(256), 0, st, part, S, v, dt, N, out, dotv);
+}
+
+}  // namespace collop
+
+namespace collop {
+
+void launch_geff_pack_b(const EnsView& E, const double* vmid, const double* dS, const double* dv, const double* w,
+                        const double* dpart, double* scal, int use_gonzalez, double lo1, double hi1, double lo2,
+                        double hi2, Src64* s64, Src32* s32, hipStream_t st) {
+  hipLaunchKernelGGL(k_geff_pack_b, dim3(E.n_pblk), dim3(kEwThreads), 0, st, E, vmid, dS, dv, w, dpart, scal,
+                     use_gonzalez, lo1, hi1, lo2, hi2, s64, s32);
+}
+
+void launch_epilogue_b(const EnsView& E, const EnsBufs& B, double dt, hipStream_t st) {
+  hipLaunchKernelGGL(k_epilogue_b, dim3(E.n_eblk), dim3(kEwThreads), 0, st, E, B, dt);
+}
+
+void launch_update_b(const EnsView& E, const EnsBufs& B, hipStream_t st) {
+  hipLaunchKernelGGL(k_update_b, dim3(E.n_eblk), dim3(kEwThreads), 0, st, E, B);
+}
+
+void launch_euler_b(const EnsView& E, const double* part, const double* v, double dt, double* out, double* dotv,
+                    hipStream_t st) {
+  hipLaunchKernelGGL(k_euler_b, dim3(E.n_eblk), dim3(kEwThreads), 0, st, E, part, v, dt, out, dotv);
+}
+
+void launch_copy_b(const EnsView& E, const double* src, double* dst, hipStream_t st) {
+  hipLaunchKernelGGL(k_copy_b, dim3(E.n_eblk), dim3(kEwThreads), 0, st, E, src, dst);
 }
 
 }  // namespace collop

@@ -205,6 +205,37 @@ int collop_mlp_train(collop_ctx* ctx, int epochs, int batch, double lr, uint64_t
  * mean loss; mode 2 -> mean loss and its gradient grad[4738]. */
 int collop_mlp_eval(collop_ctx* ctx, const float* rec, int64_t n, int mode, float* y, double* loss, float* grad);
 
+/* ---- batched ensemble ----------------------------------------------------
+ * Independent runs with uneven N on the context's mesh, advanced together (the
+ * reference advances one run per process: main.jl:226-307).  Each Anderson
+ * iteration of the whole ensemble is one replay of a captured hipGraph whose
+ * pair sum is ONE persistent launch over the tiles of every active run.  Each
+ * run follows exactly the solve of collop_run_steps (own Anderson window,
+ * precision switch and exit) and its markers equal, bit for bit, those of the
+ * same run stepped alone on a context with
+ * collop_set_pair_tile(ctx, 1, collop_ensemble_slices(N), 1).
+ * v: the runs' N_r x 2 column-major blocks one after another (2 sum N_r
+ * doubles); w: their weights (sum N_r). */
+int collop_ensemble_set(collop_ctx* ctx, int n_runs, const int64_t* n, const double* v, const double* w);
+/* current markers, same layout as collop_ensemble_set's v */
+int collop_ensemble_get(collop_ctx* ctx, double* v);
+/* projection coefficients of every run, [n_runs][n_dofs] */
+int collop_ensemble_coeffs(collop_ctx* ctx, double* f_coeffs);
+/* n_steps implicit steps of every run (Euler predictor); stats[step * n_runs + run]
+ * optional, `seconds` = wall time of the ensemble step. */
+int collop_ensemble_step(collop_ctx* ctx, const collop_step_params* params, int n_steps, collop_step_stats* stats);
+/* pair-sum source slices the ensemble gives a run of n markers */
+int collop_ensemble_slices(int64_t n);
+/* launch shape of the persistent pair sum: gamma_tile targets per workgroup
+ * (64, 128, 256, 512) and waves_per_cu resident waves per CU (sets the grid);
+ * use_graph = 0 launches the iteration's kernels one by one (same results). */
+int collop_ensemble_set_tile(collop_ctx* ctx, int gamma_tile, int waves_per_cu, int use_graph);
+/* autotuner: times the persistent pair sum (one FP32c and one FP64 launch with
+ * every run active) for gamma_tile in {64, 128, 256, 512} x waves_per_cu in
+ * {4, 8, 16, 32} on the current ensemble, keeps the fastest; ms[16] optional
+ * receives each candidate's time in that order. */
+int collop_ensemble_tune(collop_ctx* ctx, int* gamma_tile, int* waves_per_cu, double* ms);
+
 /* ---- tuning and measurement (no counterpart in the reference) -----------
  * Pair-sum tile: gamma_per_lane in {1, 2} targets per lane, slices in [1, 64]
  * source slices (0 = automatic), newton in {1, 2} FP64 rsqrt refinements. */

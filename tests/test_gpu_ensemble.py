@@ -1,0 +1,141 @@
+"""Batched ensemble (collop_ensemble_*): runs with uneven N advanced together.
+
+Each run of the ensemble executes the single-run arithmetic (every batched
+kernel gives a run the block structure of the single-run kernel at its N, and
+the Anderson state machine is shared), so its markers must equal -- bit for bit
+-- those of the same run stepped alone with the same pair slicing.  The launch
+shape of the persistent pair sum (gamma tile, waves per CU) and graph replay vs
+eager launches must not change a single bit either.
+"""
+import numpy as np
+import pytest
+
+import collop
+from conftest import uniform_bp
+
+pytestmark = pytest.mark.gpu
+
+# (IC, N, seed): N below, at and above one slice (2048), not multiples of 256
+RUNS = [("gaussian", 300, 1), ("bimodal", 1000, 2), ("bimaxwellian", 2048, 3), ("gaussian", 2500, 4),
+        ("bimodal", 4100, 5)]
+SIG = {"gaussian": (1.0, 1.0), "bimodal": (np.sqrt(0.5), np.sqrt(0.5)), "bimaxwellian": (np.sqrt(2.0), np.sqrt(0.5))}
+
+
+def make_runs(spec=RUNS):
+    out = []
+    for ic, n, seed in spec:
+        s = SIG[ic]
+        v, w = collop.markers(ic, n, seed, sigma1=s[0], sigma2=s[1], peak=2.0,
+                              weight_jitter=0.2 if ic == "bimaxwellian" else 0.0)
+        out.append((v, w))
+    return out
+
+
+def params(precision, dt=0.005):
+    p = collop.config("gaussian_n4096").step_params(precision=precision)
+    p.dt = dt
+    return p
+
+
+@pytest.fixture(scope="module")
+def ens_ws(gpu_ctx):
+    bp = uniform_bp(-6.0, 6.0, 16)
+    return collop.Workspace(bp, bp, ctx=gpu_ctx)
+
+
+@pytest.fixture(scope="module")
+def single_ws():
+    bp = uniform_bp(-6.0, 6.0, 16)
+    return collop.Workspace(bp, bp)  # its own context
+
+
+def single_reference(ws, runs, prec, steps):
+    out, iters = [], []
+    for v, w in runs:
+        collop.set_pair_tile(ws, 1, collop.Ensemble.slices(v.shape[0]), 1)
+        sim = collop.Simulation(ws, v, w)
+        st = sim.run(params(prec), steps)
+        out.append(sim.markers())
+        iters.append([s["iters"] for s in st])
+    collop.set_pair_tile(ws, 1, 0, 1)
+    return out, iters
+
+
+@pytest.mark.parametrize("prec", [collop.PREC_F64, collop.PREC_F32C])
+def test_ensemble_matches_single_runs_bitwise(ens_ws, single_ws, prec):
+    runs = make_runs()
+    steps = 2
+    ref, ref_iters = single_reference(single_ws, runs, prec, steps)
+    ens = collop.Ensemble(ens_ws, runs)
+    st = ens.step(params(prec), steps)
+    got = ens.markers()
+    for r, (a, b) in enumerate(zip(got, ref)):
+        iters = [st[s][r]["iters"] for s in range(steps)]
+        print(f"run {r} N={runs[r][0].shape[0]}: ensemble iters {iters}, single {ref_iters[r]}, "
+              f"max|dv| {np.max(np.abs(a - b)):.3e}")
+        assert iters == ref_iters[r]
+        np.testing.assert_array_equal(a, b)
+    assert all(s["exit_reason"] == 0 for row in st for s in row)
+
+
+def test_graph_and_tiles_do_not_change_results(ens_ws):
+    runs = make_runs()
+    p = params(collop.PREC_F32C)
+    results = []
+    for tile, waves, graph in [(256, 8, True), (256, 8, False), (64, 4, True), (128, 16, True), (512, 32, True)]:
+        ens = collop.Ensemble(ens_ws, runs)
+        ens.set_tile(tile, waves, graph)
+        ens.step(p, 1)
+        results.append(ens.markers())
+    for res in results[1:]:
+        for a, b in zip(res, results[0]):
+            np.testing.assert_array_equal(a, b)
+
+
+def test_autotune(ens_ws):
+    ens = collop.Ensemble(ens_ws, make_runs())
+    bt, waves, ms = ens.tune()
+    print(f"tuned gamma tile {bt}, waves/CU {waves}; ms (rows 64..512, cols 4..32 waves):\n{ms}")
+    assert bt in (64, 128, 256, 512) and waves in (4, 8, 16, 32)
+    assert np.all(ms > 0) and np.isclose(ms.min(), ms[[64, 128, 256, 512].index(bt), [4, 8, 16, 32].index(waves)])
+    st = ens.step(params(collop.PREC_F32C), 1)[0]
+    assert all(s["exit_reason"] == 0 for s in st)
+
+
+def test_conservation_and_entropy(ens_ws):
+    runs = make_runs()
+    ens = collop.Ensemble(ens_ws, runs)
+    E0 = [0.5 * float(np.sum(w * (v[:, 0] ** 2 + v[:, 1] ** 2))) for v, w in runs]
+    P0 = [(w[:, None] * v).sum(axis=0) for v, w in runs]
+    fc0 = ens.coeffs()
+    S0 = [collop.compute_r_and_entropy_gpu(ens_ws, fc0[r])[1] for r in range(len(runs))]
+    st = ens.step(params(collop.PREC_F32C, dt=0.02), 3)
+    for r in range(len(runs)):
+        E_prev, P_prev, S_prev = E0[r], P0[r], S0[r]
+        for s in range(3):
+            x = st[s][r]
+            assert abs(x["energy"] - E_prev) <= 1e-10 * E0[r]
+            assert np.max(np.abs(np.array(x["momentum"]) - P_prev)) <= 1e-10
+            assert x["entropy"] >= S_prev - 1e-12
+            E_prev, P_prev, S_prev = x["energy"], np.array(x["momentum"]), x["entropy"]
+
+
+def test_errors_and_ownership(ens_ws, gpu_ctx):
+    v, w = make_runs()[0]
+    with pytest.raises(collop.CollopError):
+        collop.Ensemble(ens_ws, [(v[:1], w[:1])])  # N < 2
+    with pytest.raises(collop.CollopError):
+        collop.Ensemble(ens_ws, [(v, -w)])
+    ens = collop.Ensemble(ens_ws, [(v, w)])
+    p = params(collop.PREC_F64)
+    p.warmstart = collop.WARM_MLP
+    with pytest.raises(collop.CollopError) as ei:
+        ens.step(p, 1)
+    assert ei.value.status == collop._lib.ERR_UNSUPPORTED
+    with pytest.raises(collop.CollopError):
+        ens.set_tile(100, 8)
+    # another mesh on the same context drops the ensemble
+    bp = uniform_bp(-5.0, 5.0, 12)
+    collop.Workspace(bp, bp, ctx=gpu_ctx).activate()
+    with pytest.raises(collop.CollopError):
+        ens.markers()
