@@ -42,16 +42,20 @@ sys.path.insert(0, os.path.join(ROOT, "collisionoperators.jl_amd"))
 FP64_VECTOR_PEAK_TFLOPS = 78.6     # MI355X published FP64 vector peak
 FP32_VECTOR_PEAK_TFLOPS = 157.3    # MI355X published FP32 vector peak (packed)
 FLOPS_PER_PAIR = 23
+# HBM bytes per pair-sum launch at bimodal_n65536 from rocprofv3 FETCH_SIZE (x2.0)
+# and WRITE_SIZE (x1.0), both scaled by the counters of a 256 MiB copy; see
+# profiles/pair_sum_bimodal_n65536.md.
+PAIR_TRAFFIC_BYTES = {"k_pair_f64": 68646 * 1024, "k_pair_f32c": 50717 * 1024}
 
 
 def log(msg: str):
     print(msg, file=sys.stderr, flush=True)
 
 
-def run_path(ws, p, v, w, precision, warmup, steps):
+def run_path(ws, p, v, w, precision, warmup, steps, solver=None):
     import collop
     sim = collop.Simulation(ws, v, w)
-    prm = p.step_params(precision=precision)
+    prm = p.step_params(precision=precision, solver=solver)
     E0 = 0.5 * float(np.sum(w * (v[:, 0] ** 2 + v[:, 1] ** 2)))
     for i in range(warmup):
         st = sim.run(prm, 1)[0]
@@ -67,8 +71,8 @@ def run_path(ws, p, v, w, precision, warmup, steps):
     for i in range(steps):
         st = sim.run(prm, 1)[0]
         stats.append(st)
-        log(f"  step {i}: iters {st['iters']} (f32 {st['iters_f32']}) exit {st['exit_reason']} "
-            f"res {st['residual']:.2e} {st['seconds'] * 1e3:.1f} ms")
+        log(f"  step {i}: iters {st['iters']} (f32 {st['iters_f32']}, inner {st['inner_maps']}) "
+            f"exit {st['exit_reason']} res {st['residual']:.2e} {st['seconds'] * 1e3:.1f} ms")
     elapsed = time.perf_counter() - t0
     ms, launches = collop.profile(ws, False)
     dE = dP = 0.0
@@ -214,13 +218,69 @@ def kernel_probe(args):
     log(f"[probe] pair kernels x3 (FP64, FP32c) at N={N}; copy {nbytes} bytes")
 
 
+def ensemble_bench(args):
+    """Batched ensemble: 48 runs, N log-uniform in [2048, 65536], random IC,
+    shared 32x32 mesh, FP32c pair sums (FP64 finish), dt 0.02.  Times the
+    batched step (one persistent pair launch per iteration, hipGraph replay)
+    and, for comparison, the same runs stepped one after another by the
+    single-run path."""
+    import collop
+    runs = collop.ensemble_markers(48, seed=7)
+    ws = collop.Workspace(np.linspace(-6, 6, 33), np.linspace(-6, 6, 33), device=0)
+    p = collop.config("bimodal_n65536").step_params(precision=collop.PREC_F32C)
+    n = np.array([r[3] for r in runs])
+    pairs = float((n.astype(float) * (n - 1)).sum())
+    log(f"[ensemble] {len(runs)} runs, N in [{n.min()}, {n.max()}], sum N {n.sum()}, {pairs:.3e} pairs per iteration")
+    ens = collop.Ensemble(ws, [(v, w) for v, w, _, _ in runs])
+    t0 = time.perf_counter()
+    bt, waves, ms = ens.tune()
+    log(f"[ensemble] autotuner ({time.perf_counter() - t0:.1f} s): gamma tile {bt}, waves/CU {waves}; "
+        f"FP32c+FP64 pair launch ms rows gamma 64..512, cols waves 4..32:\n{np.array2string(ms, precision=3)}")
+    for i in range(args.warmup):
+        st = ens.step(p, 1)[0]
+        log(f"  warmup step {i}: max iters {max(s['iters'] for s in st)} {st[0]['seconds'] * 1e3:.0f} ms")
+    t0 = time.perf_counter()
+    stats = ens.step(p, args.steps)
+    el = time.perf_counter() - t0
+    for i, st in enumerate(stats):
+        log(f"  step {i}: iters mean {np.mean([s['iters'] for s in st]):.1f} max {max(s['iters'] for s in st)} "
+            f"(f32 mean {np.mean([s['iters_f32'] for s in st]):.1f}) {st[0]['seconds'] * 1e3:.0f} ms")
+    conv = all(s["exit_reason"] == 0 for st in stats for s in st)
+    batched = len(runs) * args.steps / el
+    # the same runs stepped one after another (single-run path), a bounded sample of steps
+    seq_steps = max(1, min(args.steps, 2))
+    t0 = time.perf_counter()
+    for v, w, _, nn in runs:
+        collop.set_pair_tile(ws, 1, collop.Ensemble.slices(nn), 1)
+        sim = collop.Simulation(ws, v, w)
+        sim.run(p, seq_steps)
+    seq_el = time.perf_counter() - t0
+    collop.set_pair_tile(ws, 1, 0, 1)
+    sequential = len(runs) * seq_steps / seq_el
+    out = {"metric": "ensemble run-steps/s (48 runs, N log-uniform [2048, 65536], FP32c + FP64 finish, dt 0.02)",
+           "value": round(batched, 4), "unit": "run-steps/s", "n_gpus": 1, "steps": args.steps,
+           "warmup": args.warmup, "ms_per_step": round(1e3 * el / args.steps, 3), "higher_is_better": True,
+           "scaling": "weak", "vs_baseline": None, "dtype": "f32c+f64",
+           "data": "synthetic (seeded: N, IC and markers from PCG64 seed 7)",
+           "config": {"workload": "ensemble48", "runs": len(runs), "sum_N": int(n.sum()), "N_min": int(n.min()),
+                      "N_max": int(n.max()), "mesh": "32x32", "dt": p.dt},
+           "pairs_per_iteration": pairs, "gamma_tile": bt, "waves_per_cu": waves,
+           "all_converged": conv, "sequential_run_steps_per_s": round(sequential, 4),
+           "batched_vs_sequential": round(batched / sequential, 3),
+           "tuner_ms": ms.tolist()}
+    print(json.dumps(out), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=10)
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--config", default="bimodal_n65536")
+    ap.add_argument("--chosen-solver", choices=["anderson", "defect"], default="anderson",
+                    help="solver of the chosen path (the plain FP64 path always runs Anderson)")
+    ap.add_argument("--config", default="bimodal_n65536",
+                    help="bimodal_n65536 (headline), gaussian_n4096, bimaxwellian_n262144 or ensemble48")
     ap.add_argument("--kernel-probe", action="store_true",
                     help="run only the pair kernels and a 256 MiB copy (rocprofv3 counter runs)")
     ap.add_argument("--report", choices=["precision"], default=None,
@@ -237,6 +297,9 @@ def main():
     if args.kernel_probe:
         kernel_probe(args)
         return
+    if args.config == "ensemble48":
+        ensemble_bench(args)
+        return
 
     import collop
     p = collop.config(args.config)
@@ -252,7 +315,8 @@ def main():
     r64 = run_path(ws, p, v, w, collop.PREC_F64, args.warmup, args.steps)
     log(f"[bench] plain FP64: {r64['steps_per_s']:.3f} steps/s, mean iters {r64['mean_iters']:.1f}")
     log("[bench] chosen path (FP32-compensated maps + FP64 finish)")
-    rch = run_path(ws, p, v, w, collop.PREC_F32C, args.warmup, args.steps)
+    rch = run_path(ws, p, v, w, collop.PREC_F32C, args.warmup, args.steps,
+                   solver=collop._lib.SOLVER_DEFECT if args.chosen_solver == "defect" else collop._lib.SOLVER_ANDERSON)
     log(f"[bench] chosen: {rch['steps_per_s']:.3f} steps/s, mean iters {rch['mean_iters']:.1f} "
         f"(f32 {rch['mean_iters_f32']:.1f})")
 
@@ -268,8 +332,9 @@ def main():
         peak, kname = FP32_VECTOR_PEAK_TFLOPS, "k_pair_f32c"
     achieved = flops / (avg_ms * 1e-3) / 1e12
     avg64 = r64["pair_ms"][0] / max(r64["pair_launches"][0], 1)
+    traffic = PAIR_TRAFFIC_BYTES.get(kname) if args.config == "bimodal_n65536" else None
     roofline = {"bound": "valu", "kernel": kname, "achieved": round(achieved, 3), "peak": peak, "unit": "TFLOP/s",
-                "frac": round(achieved / peak, 4), "traffic": None, "avg_launch_ms": round(avg_ms, 4),
+                "frac": round(achieved / peak, 4), "traffic": traffic, "avg_launch_ms": round(avg_ms, 4),
                 "flops_per_launch": flops,
                 "fp64_pair_avg_ms": round(avg64, 4),
                 "fp64_pair_tflops": round(flops / (avg64 * 1e-3) / 1e12, 3),

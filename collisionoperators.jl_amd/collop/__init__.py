@@ -11,7 +11,7 @@ from ._lib import (CollopError, Context, StepParams, StepStats, load, PREC_F64, 
 from .hooks import (compute_collision_gpu, compute_collision_gpu32, compute_collision_metric_gpu,  # noqa: F401
                     compute_G_gpu, compute_r_and_entropy_gpu, enable_gpu, eval_loggrad_gpu, l2_project_gpu,
                     l2_rhs_gpu, mass_solve_gpu)
-from .ic import initial_markers, markers  # noqa: F401
+from .ic import ensemble_markers, initial_markers, markers  # noqa: F401
 from .mesh import Workspace, build_workspace, extraction_p2c1  # noqa: F401
 from .params import CONFIGS, SimParameters, config, parse_overrides  # noqa: F401
 from .sim import Ensemble, Simulation, profile, set_pair_tile  # noqa: F401

@@ -27,6 +27,8 @@ PREC_F64 = 0
 PREC_F32C = 1
 WARM_EULER = 0
 WARM_MLP = 1
+SOLVER_ANDERSON = 0
+SOLVER_DEFECT = 1
 
 
 class CollopError(RuntimeError):
@@ -63,6 +65,10 @@ class StepParams(C.Structure):
         ("f32_floor_factor", C.c_double),
         ("f32_keep_window", C.c_int32),
         ("record_training", C.c_int32),
+        ("solver", C.c_int32),
+        ("dc_eta", C.c_double),
+        ("dc_max_inner", C.c_int32),
+        ("dc_stag_window", C.c_int32),
     ]
 
 
@@ -80,6 +86,8 @@ class StepStats(C.Structure):
         ("energy", C.c_double),
         ("momentum", C.c_double * 2),
         ("seconds", C.c_double),
+        ("inner_maps", C.c_int32),
+        ("pad", C.c_int32),
     ]
 
     def as_dict(self) -> dict:
@@ -88,6 +96,7 @@ class StepStats(C.Structure):
             "exit_reason": self.exit_reason, "residual": self.residual, "r0": self.r0,
             "entropy": self.entropy, "energy": self.energy,
             "momentum": (self.momentum[0], self.momentum[1]), "seconds": self.seconds,
+            "inner_maps": self.inner_maps,
         }
 
 

@@ -37,3 +37,21 @@ def markers(ic: str, N: int, seed: int, sigma1: float = 1.0, sigma2: float = 1.0
     else:
         w = np.full(N, 1.0 / N)
     return v, w
+
+
+# (ic, sigma1, sigma2) of the three initial conditions an ensemble draws from
+ENSEMBLE_ICS = (("gaussian", 1.0, 1.0), ("bimodal", 0.5 ** 0.5, 0.5 ** 0.5), ("bimaxwellian", 2.0 ** 0.5, 0.5 ** 0.5))
+
+
+def ensemble_markers(n_runs: int = 48, seed: int = 7, n_min: int = 2048, n_max: int = 65536):
+    """Seeded ensemble of runs: N log-uniform in [n_min, n_max], the IC drawn
+    from isotropic N(0, I), bimodal (+-2, cov 0.5 I) and bi-Maxwellian
+    (T_x = 2, T_y = 0.5), equal weights.  Returns [(v, w, ic, N)]."""
+    rng = np.random.Generator(np.random.PCG64(seed))
+    out = []
+    for r in range(n_runs):
+        n = int(round(np.exp(rng.uniform(np.log(n_min), np.log(n_max)))))
+        ic, s1, s2 = ENSEMBLE_ICS[int(rng.integers(0, 3))]
+        v, w = markers(ic, n, seed * 1000 + r, sigma1=s1, sigma2=s2, peak=2.0)
+        out.append((v, w, ic, n))
+    return out

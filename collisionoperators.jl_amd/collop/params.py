@@ -52,10 +52,19 @@ class SimParameters:
     seed: int = 42
     ic: str = "gaussian"             # gaussian | bimodal | bimaxwellian
     weight_jitter: float = 0.0       # w ∝ 1 + jitter u, u ~ U(-1, 1)
+    solver: str = "anderson"         # :anderson | :defect  (Parameters.jl:80-88)
+    dc_eta: float = 0.05             # Parameters.jl:89-91
+    dc_max_inner: int = 60
+    dc_stag_window: int = 5
 
-    def step_params(self, precision: int | None = None, warmstart: int | None = None) -> _lib.StepParams:
+    def step_params(self, precision: int | None = None, warmstart: int | None = None,
+                    solver: int | None = None) -> _lib.StepParams:
         prec = precision if precision is not None else (_lib.PREC_F32C if self.gpu_fp32 else _lib.PREC_F64)
         warm = warmstart if warmstart is not None else (_lib.WARM_MLP if self.warmstart == "nn" else _lib.WARM_EULER)
+        if solver is None:
+            if self.solver not in ("anderson", "defect"):
+                raise ValueError(f"unknown solver={self.solver} (anderson | defect)")
+            solver = _lib.SOLVER_DEFECT if self.solver == "defect" else _lib.SOLVER_ANDERSON
         return _lib.StepParams(
             dt=self.DT, m_anderson=self.m_anderson, max_iter=self.max_iter, tol=self.tol,
             abs_floor=self.abs_floor, stag_window=self.stag_window, stag_rel_tol=self.stag_rel_tol,
@@ -63,7 +72,8 @@ class SimParameters:
             restart_factor=math.inf, damping=self.damping, reg_factor=1e-10,
             use_anderson=int(self.use_anderson), use_gonzalez=int(self.use_gonzalez),
             exit_picard_step=int(self.exit_picard_step), use_logsq=int(self.use_logsq),
-            precision=prec, warmstart=warm, f32_floor_factor=16.0, f32_keep_window=1, record_training=0)
+            precision=prec, warmstart=warm, f32_floor_factor=16.0, f32_keep_window=1, record_training=0,
+            solver=solver, dc_eta=self.dc_eta, dc_max_inner=self.dc_max_inner, dc_stag_window=self.dc_stag_window)
 
 
 def parse_overrides(p: SimParameters, args: list[str]) -> SimParameters:

@@ -271,7 +271,7 @@ int collop_destroy(collop_ctx* ctx) {
                   &ctx->b_s32, &ctx->b_part, &ctx->b_spart_sc, &ctx->b_mom, &ctx->b_dpart, &ctx->b_iota,
                   &ctx->b_spart_q, &ctx->b_vec, &ctx->b_scal, &ctx->b_statp, &ctx->b_stat, &ctx->s_v, &ctx->s_w,
                   &ctx->s_v1, &ctx->s_Gv, &ctx->s_r, &ctx->s_rp, &ctx->s_Gp, &ctx->s_best, &ctx->s_dF, &ctx->s_dG,
-                  &ctx->s_fc, &ctx->s_dotv, &ctx->m_params, &ctx->m_adam_m, &ctx->m_adam_v, &ctx->m_data, &ctx->m_rec,
+                  &ctx->s_fc, &ctx->s_dotv, &ctx->s_Gvo, &ctx->s_Gk, &ctx->s_FA, &ctx->s_D, &ctx->m_params, &ctx->m_adam_m, &ctx->m_adam_v, &ctx->m_data, &ctx->m_rec,
                   &ctx->m_y, &ctx->m_gpart, &ctx->m_lpart, &ctx->m_tiles, &ctx->m_eval, &ctx->m_grad})
     release(*b);
   for (auto& e : ctx->ev_pool) {
@@ -618,9 +618,12 @@ int entropy_seed_gradient(collop_ctx* ctx, const double* x, const double* fc, in
   return COLLOP_OK;
 }
 
-// One Picard map (solver.jl:176-227, Landau branch): Gv = v0 + dt vdot(v_mid, G_eff),
-// followed by the epilogue of the Anderson iteration.
-int picard_map_dev(collop_ctx* ctx, const collop_step_params& p, int kind, const double* vin, int col, int hist) {
+// landau_geff! (solver.jl:86-113) at vin: v_mid, dv, the entropy gradient at
+// v_mid and the Gonzalez correction, then G_eff -- packed into pair-source
+// records (s64 / s32, either may be null), copied to geff_out, and/or turned
+// into the frozen-metric product D = A (G_eff - Gk).
+int geff_dev(collop_ctx* ctx, const collop_step_params& p, const double* vin, Src64* s64, Src32* s32,
+             double* geff_out, const double* Gk, const double* A, double* D) {
   const MeshView& m = ctx->mv;
   const int n = (int)ctx->N;
   const int nsb = scatter_blocks(n);
@@ -652,18 +655,23 @@ int picard_map_dev(collop_ctx* ctx, const collop_step_params& p, int kind, const
   CHECK_LAUNCH();
   launch_geff_pack(P<double>(ctx->b_vmid), P<double>(ctx->b_dS), P<double>(ctx->b_dv), P<double>(ctx->s_w), n,
                    P<double>(ctx->b_dpart), gather_blocks(n), sc + 1, sc + 0, p.use_gonzalez, m.lo1, m.hi1, m.lo2,
-                   m.hi2, kind == 0 ? P<Src64>(ctx->b_s64) : nullptr, kind == 1 ? P<Src32>(ctx->b_s32) : nullptr,
-                   sc + 2, ctx->stream);
+                   m.hi2, s64, s32, sc + 2, ctx->stream, geff_out, Gk, A, D);
   CHECK_LAUNCH();
-  int r = run_pair(ctx, kind, n, P<double>(ctx->b_part));
-  if (r) return r;
+  return COLLOP_OK;
+}
+
+// Anderson epilogue of a map whose value is v0e + dte * (slice sum of part)
+// (S slices of 2N), then the statistics reduction and their download.
+int epilogue_dev(collop_ctx* ctx, const double* part, int S, const double* v0e, double dte, const double* vin,
+                 double* Gv, int col, int hist) {
+  const int n = (int)ctx->N;
   EpiArgs a{};
-  a.part = P<double>(ctx->b_part);
-  a.S = kind == 0 ? ctx->tile64.slices : ctx->tile32.slices;
-  a.v0 = P<double>(ctx->s_v);
+  a.part = part;
+  a.S = S;
+  a.v0 = v0e;
   a.v1 = vin;
-  a.dt = p.dt;
-  a.Gv = P<double>(ctx->s_Gv);
+  a.dt = dte;
+  a.Gv = Gv;
   a.r = P<double>(ctx->s_r);
   a.rp = P<double>(ctx->s_rp);
   a.Gp = P<double>(ctx->s_Gp);
@@ -681,6 +689,46 @@ int picard_map_dev(collop_ctx* ctx, const collop_step_params& p, int kind, const
   HIPC(hipStreamSynchronize(ctx->stream));
   prof_collect(ctx);
   return COLLOP_OK;
+}
+
+// One Picard map (solver.jl:176-227, Landau branch): Gv = v0 + dt vdot(v_mid, G_eff),
+// followed by the epilogue of the Anderson iteration.
+int picard_map_dev(collop_ctx* ctx, const collop_step_params& p, int kind, const double* vin, int col, int hist) {
+  int r = geff_dev(ctx, p, vin, kind == 0 ? P<Src64>(ctx->b_s64) : nullptr, kind == 1 ? P<Src32>(ctx->b_s32) : nullptr,
+                   nullptr, nullptr, nullptr, nullptr);
+  if (r) return r;
+  if ((r = run_pair(ctx, kind, (int)ctx->N, P<double>(ctx->b_part)))) return r;
+  return epilogue_dev(ctx, P<double>(ctx->b_part), kind == 0 ? ctx->tile64.slices : ctx->tile32.slices,
+                      P<double>(ctx->s_v), p.dt, vin, P<double>(ctx->s_Gv), col, hist);
+}
+
+// Outer map of defect correction (solver.jl:590-594): G(v1) and the metric A in
+// one FP64 pair pass (compute_collision_metric!, functions.jl:249-290); keeps
+// G_eff(v1) in s_Gk, [F, A] in s_FA and G(v1) in s_Gvo.
+int outer_map_dev(collop_ctx* ctx, const collop_step_params& p) {
+  const int n = (int)ctx->N;
+  int r = geff_dev(ctx, p, P<double>(ctx->s_v1), P<Src64>(ctx->b_s64), nullptr, P<double>(ctx->s_Gk), nullptr,
+                   nullptr, nullptr);
+  if (r) return r;
+  prof_begin(ctx, 0);
+  launch_pair_metric_f64(P<Src64>(ctx->b_s64), n, ctx->tile64, P<double>(ctx->b_part), ctx->stream);
+  CHECK_LAUNCH();
+  prof_end(ctx);
+  launch_slice_sum(P<double>(ctx->b_part), ctx->tile64.slices, (int64_t)5 * n, P<double>(ctx->s_FA), ctx->stream);
+  CHECK_LAUNCH();
+  return epilogue_dev(ctx, P<double>(ctx->s_FA), 1, P<double>(ctx->s_v), p.dt, P<double>(ctx->s_v1),
+                      P<double>(ctx->s_Gvo), -1, 0);
+}
+
+// Inner map of defect correction (solver.jl:598-607): Phi(u) = G(v_k) - dt A_k
+// (G_eff(u) - G_k), no pair sum; Phi(u) lands in s_Gv with the Anderson epilogue.
+int frozen_map_dev(collop_ctx* ctx, const collop_step_params& p, int col, int hist) {
+  const int n = (int)ctx->N;
+  int r = geff_dev(ctx, p, P<double>(ctx->s_v1), nullptr, nullptr, nullptr, P<double>(ctx->s_Gk),
+                   P<double>(ctx->s_FA) + 2 * (size_t)n, P<double>(ctx->s_D));
+  if (r) return r;
+  return epilogue_dev(ctx, P<double>(ctx->s_D), 1, P<double>(ctx->s_Gvo), -p.dt, P<double>(ctx->s_v1),
+                      P<double>(ctx->s_Gv), col, hist);
 }
 
 }  // namespace
@@ -769,8 +817,115 @@ int validate_params(collop_ctx* ctx, const collop_step_params* p) {
   if (p->warmstart == COLLOP_WARM_MLP && !(ctx->m_has_params && ctx->m_has_norm))
     return fail(ctx, COLLOP_ERR_NO_MODEL, "no warm-start model loaded on this context (collop_mlp_set_params / "
                                           "collop_mlp_set_norm)");
+  if (p->solver != COLLOP_SOLVER_ANDERSON && p->solver != COLLOP_SOLVER_DEFECT)
+    return fail(ctx, COLLOP_ERR_INVALID, "unknown solver");
+  if (p->solver == COLLOP_SOLVER_DEFECT &&
+      (!(p->dc_eta > 0.0 && p->dc_eta < 1.0) || p->dc_max_inner < 1 || p->dc_stag_window < 1))
+    return fail(ctx, COLLOP_ERR_INVALID, "defect correction needs 0 < dc_eta < 1, dc_max_inner >= 1, dc_stag_window >= 1");
   if (p->record_training && ctx->m_size + ctx->N > ctx->m_cap)
     return fail(ctx, COLLOP_ERR_INVALID, "training dataset full (collop_mlp_dataset_alloc)");
+  return COLLOP_OK;
+}
+
+// step_defect! (solver.jl:569-634) from the initial guess in s_v1; on return
+// s_v1 holds the solution.  Outer iteration k: the full map G(v_k) with the
+// metric A_k; then Anderson (solver.jl:507-545: window m_anderson, damping,
+// no stagnation logic) on the frozen-metric map Phi_k to
+// max(dc_eta ||G(v_k) - v_k||, abs_floor / 2), at most dc_max_inner maps.
+int defect_solve(collop_ctx* ctx, const collop_step_params& p, collop_step_stats* out) {
+  const int n = (int)ctx->N;
+  const size_t n2b = (size_t)2 * n * 8;
+  const int mw = p.m_anderson;
+  double nrm0 = 0.0, nrm_best = std::numeric_limits<double>::infinity();
+  double nrm_best_window = std::numeric_limits<double>::infinity(), res_out = 0.0;
+  int exit_reason = 2, k_done = p.max_iter, inner = 0, r;
+  const bool trace = std::getenv("COLLOP_TRACE") != nullptr;
+  // the best iterate starts as the initial guess
+  if ((r = copy_d2d(ctx, ctx->s_best, ctx->s_v1, n2b))) return r;
+  for (int k = 1; k <= p.max_iter; ++k) {
+    if ((r = outer_map_dev(ctx, p))) return r;
+    const double nr = std::sqrt(ctx->h_stat[0]), nv = std::sqrt(ctx->h_stat[1]);
+    if (!std::isfinite(nr) || !std::isfinite(nv))
+      return fail(ctx, COLLOP_ERR_NONFINITE, "non-finite residual in outer map " + std::to_string(k));
+    if (trace) std::fprintf(stderr, "  [defect] k=%d |r|=%.3e inner so far %d\n", k, nr, inner);
+    if (k == 1) nrm0 = nr;
+    if (nr < nrm_best) {
+      nrm_best = nr;
+      if ((r = copy_d2d(ctx, ctx->s_best, ctx->s_Gvo, n2b))) return r;
+    }
+    if (nr < std::max(p.tol * (nv + 1e-30), p.abs_floor)) {
+      std::swap(ctx->s_v1, ctx->s_Gvo);
+      exit_reason = 0;
+      k_done = k;
+      res_out = nr;
+      break;
+    }
+    if (k > p.dc_stag_window && k % p.dc_stag_window == 0) {
+      if ((nrm_best_window - nrm_best) / (nrm_best_window + 1e-30) < p.stag_rel_tol) {
+        std::swap(ctx->s_v1, ctx->s_best);
+        exit_reason = 1;
+        k_done = k;
+        res_out = nrm_best;
+        break;
+      }
+      nrm_best_window = nrm_best;
+    }
+    // inner Anderson on u = Phi_k(u), u = s_v1 (solver.jl:507-545)
+    const double tol_in = std::max(p.dc_eta * nr, 0.5 * p.abs_floor);
+    int hist = 0, slot = 0;
+    double ATA[kMaxWindow * kMaxWindow] = {0};
+    for (int j = 1; j <= p.dc_max_inner; ++j) {
+      const int col = j == 1 ? -1 : slot % mw;
+      const int hist_new = j == 1 ? 0 : std::min(hist + 1, mw);
+      if ((r = frozen_map_dev(ctx, p, col, hist_new))) return r;
+      inner++;
+      const double* st = ctx->h_stat;
+      const double ri = std::sqrt(st[0]);
+      if (!std::isfinite(ri)) return fail(ctx, COLLOP_ERR_NONFINITE, "non-finite residual in a frozen-metric map");
+      if (ri < tol_in) {
+        std::swap(ctx->s_v1, ctx->s_Gv);  // u <- Phi(u)
+        break;
+      }
+      if (j == 1) {
+        launch_update(P<double>(ctx->s_v1), P<double>(ctx->s_Gv), P<double>(ctx->s_dG), nullptr, 0, p.damping, n,
+                      ctx->stream);
+        CHECK_LAUNCH();
+      } else {
+        slot = slot % mw + 1;
+        hist = hist_new;
+        for (int i = 0; i < hist; ++i) {
+          ATA[col * kMaxWindow + i] = st[3 + i];
+          ATA[i * kMaxWindow + col] = st[3 + i];
+        }
+        double A[kMaxWindow * kMaxWindow], b[kMaxWindow], g[kMaxWindow];
+        double dm = 0.0;
+        for (int i = 0; i < hist; ++i) {
+          for (int c = 0; c < hist; ++c) A[i * hist + c] = ATA[i * kMaxWindow + c];
+          b[i] = st[3 + kMaxWindow + i];
+          dm += A[i * hist + i];
+        }
+        const double l2 = p.reg_factor * dm / hist + 1e-30;
+        for (int i = 0; i < hist; ++i) A[i * hist + i] += l2;
+        small_solve(hist, A, b, g);
+        launch_update(P<double>(ctx->s_v1), P<double>(ctx->s_Gv), P<double>(ctx->s_dG), g, hist, p.damping, n,
+                      ctx->stream);
+        CHECK_LAUNCH();
+      }
+      std::swap(ctx->s_rp, ctx->s_r);
+      std::swap(ctx->s_Gp, ctx->s_Gv);
+    }
+  }
+  if (exit_reason == 2) {
+    std::swap(ctx->s_v1, ctx->s_best);
+    res_out = nrm_best;
+  }
+  out->iters = k_done;
+  out->iters_f32 = 0;
+  out->n_restart = 0;
+  out->exit_reason = exit_reason;
+  out->residual = res_out;
+  out->r0 = nrm0;
+  out->inner_maps = inner;
   return COLLOP_OK;
 }
 
@@ -831,13 +986,17 @@ int step_once(collop_ctx* ctx, const collop_step_params& p, collop_step_stats* o
     CHECK_LAUNCH();
   }
 
+  collop_step_stats dc{};
+  if (p.solver == COLLOP_SOLVER_DEFECT) {
+    if ((r = defect_solve(ctx, p, &dc))) return r;
+  }
   // ---- Anderson (solver.jl:300-423)
   AndersonRun an;
   an.start(p);
   // The best iterate starts as the initial guess (solver.jl:331).
-  if ((r = copy_d2d(ctx, ctx->s_best, ctx->s_v1, n2b))) return r;
+  if (p.solver == COLLOP_SOLVER_ANDERSON && (r = copy_d2d(ctx, ctx->s_best, ctx->s_v1, n2b))) return r;
   const char* trace = std::getenv("COLLOP_TRACE") ? "run" : nullptr;
-  while (!an.done) {
+  while (p.solver == COLLOP_SOLVER_ANDERSON && !an.done) {
     const AndersonPlan pl = an.plan();
     if ((r = picard_map_dev(ctx, p, pl.kind, P<double>(ctx->s_v1), pl.col, pl.hist))) return r;
     AndersonAct act;
@@ -875,12 +1034,17 @@ int step_once(collop_ctx* ctx, const collop_step_params& p, collop_step_stats* o
   HIPC(hipStreamSynchronize(ctx->stream));
   prof_collect(ctx);
   if (out) {
-    out->iters = an.k_done;
-    out->iters_f32 = an.iters32;
-    out->n_restart = an.n_restart;
-    out->exit_reason = an.exit_reason;
-    out->residual = an.res_out;
-    out->r0 = an.nrm_r0;
+    if (p.solver == COLLOP_SOLVER_DEFECT) {
+      *out = dc;
+    } else {
+      out->iters = an.k_done;
+      out->iters_f32 = an.iters32;
+      out->n_restart = an.n_restart;
+      out->exit_reason = an.exit_reason;
+      out->residual = an.res_out;
+      out->r0 = an.nrm_r0;
+      out->inner_maps = 0;
+    }
     out->entropy = sc[0];
     out->energy = 0.5 * sc[5];
     out->momentum[0] = sc[3];
@@ -909,8 +1073,10 @@ int collop_state_set(collop_ctx* ctx, const double* v, const double* w, int64_t 
   HIPC(hipSetDevice(ctx->device));
   if ((r = ensure_scratch(ctx, N))) return r;
   const size_t n2b = (size_t)2 * N * 8;
-  for (DBuf* b : {&ctx->s_v, &ctx->s_v1, &ctx->s_Gv, &ctx->s_r, &ctx->s_rp, &ctx->s_Gp, &ctx->s_best, &ctx->s_dotv})
+  for (DBuf* b : {&ctx->s_v, &ctx->s_v1, &ctx->s_Gv, &ctx->s_r, &ctx->s_rp, &ctx->s_Gp, &ctx->s_best, &ctx->s_dotv,
+                  &ctx->s_Gvo, &ctx->s_Gk, &ctx->s_D})
     ENSURE(*b, n2b);
+  ENSURE(ctx->s_FA, (size_t)5 * N * 8);
   ENSURE(ctx->s_w, (size_t)N * 8);
   ENSURE(ctx->s_dF, n2b * kMaxWindow);
   ENSURE(ctx->s_dG, n2b * kMaxWindow);

@@ -42,6 +42,7 @@ struct collop_ctx {
   int64_t N = 0;
   double w_abs_sum = 0.0;  // sum |w| of the state (fixed-point scale of the scatter)
   DBuf s_v, s_w, s_v1, s_Gv, s_r, s_rp, s_Gp, s_best, s_dF, s_dG, s_fc, s_dotv;
+  DBuf s_Gvo, s_Gk, s_FA, s_D;  // defect correction: G(v_k), G_eff(v_k), [F, A], A (G_eff - G_k)
   double* h_stat = nullptr;  // pinned, kStatWidth doubles
   // ---- warm-start MLP
   DBuf m_params, m_adam_m, m_adam_v, m_data, m_rec, m_y, m_gpart, m_lpart, m_tiles, m_eval, m_grad;

@@ -136,11 +136,14 @@ void launch_gather_b(const MeshView& m, const EnsView& E, const double* x, const
                      const double* w, const double* dv, int mode, double* out, double* dpart, hipStream_t st);
 
 // ---- solver side (step.hip) -------------------------------------------------
-// Gonzalez lambda from reduced partials, then G_eff and the pair-source pack.
+// Gonzalez lambda from reduced partials, then G_eff and the pair-source pack
+// (s64 / s32 may be null).  Optional: geff_out[2N] <- G_eff, and the frozen-
+// metric product D[2N] <- A (G_eff - Gk) with A = [A11, A12, A22] (3N).
 void launch_geff_pack(const double* vmid, const double* dS, const double* dv, const double* w, int N,
                       const double* dpart, int nparts, const double* S1, const double* S0, int use_gonzalez,
                       double lo1, double hi1, double lo2, double hi2, Src64* s64, Src32* s32, double* lambda_out,
-                      hipStream_t st);
+                      hipStream_t st, double* geff_out = nullptr, const double* Gk = nullptr,
+                      const double* A = nullptr, double* D = nullptr);
 
 struct EpiArgs {
   const double* part; int S;

@@ -412,6 +412,8 @@ int check_params(collop_ctx* ctx, const collop_step_params* p) {
     return fail(ctx, COLLOP_ERR_UNSUPPORTED, "the ensemble step supports the Euler predictor only");
   if (p->record_training)
     return fail(ctx, COLLOP_ERR_UNSUPPORTED, "the ensemble step does not record training data");
+  if (p->solver != COLLOP_SOLVER_ANDERSON)
+    return fail(ctx, COLLOP_ERR_UNSUPPORTED, "the ensemble step runs the Anderson solver only");
   return COLLOP_OK;
 }
 

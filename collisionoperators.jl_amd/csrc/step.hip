@@ -22,7 +22,10 @@ __device__ __forceinline__ void geff_body(const double* __restrict__ vmid, const
                                           const double* __restrict__ dpart, int nparts, const double* __restrict__ S1,
                                           const double* __restrict__ S0, int use_gonzalez, double lo1, double hi1,
                                           double lo2, double hi2, Src64* __restrict__ s64, Src32* __restrict__ s32,
-                                          double* __restrict__ lambda_out, double* lam_sh, int lb) {
+                                          double* __restrict__ lambda_out, double* lam_sh, int lb,
+                                          double* __restrict__ geff_out = nullptr,
+                                          const double* __restrict__ Gk = nullptr,
+                                          const double* __restrict__ A = nullptr, double* __restrict__ D = nullptr) {
   if (threadIdx.x == 0) {
     double lam = 0.0;
     if (use_gonzalez) {
@@ -43,6 +46,16 @@ __device__ __forceinline__ void geff_body(const double* __restrict__ vmid, const
   const double iw = 1.0 / w[i];
   const double g1 = -(dS[i] + lam * dv[i]) * iw;
   const double g2 = -(dS[ld + i] + lam * dv[ld + i]) * iw;
+  if (geff_out) {
+    geff_out[i] = g1;
+    geff_out[ld + i] = g2;
+  }
+  if (D) {  // frozen-metric product A (G_eff - G_k) of the defect-correction inner map (solver.jl:598-607)
+    const double d1 = g1 - Gk[i];
+    const double d2 = g2 - Gk[ld + i];
+    D[i] = A[i] * d1 + A[ld + i] * d2;
+    D[ld + i] = A[ld + i] * d1 + A[2 * (size_t)ld + i] * d2;
+  }
   const double a = vmid[i], b = vmid[ld + i];
   const bool in = in_box(a, b, lo1, hi1, lo2, hi2);
   const double wm = in ? w[i] : 0.0;
@@ -156,10 +169,13 @@ __global__ __launch_bounds__(kEwThreads) void k_geff_pack(const double* __restri
                                                           const double* __restrict__ S0, int use_gonzalez,
                                                           double lo1, double hi1, double lo2, double hi2,
                                                           Src64* __restrict__ s64, Src32* __restrict__ s32,
-                                                          double* __restrict__ lambda_out) {
+                                                          double* __restrict__ lambda_out,
+                                                          double* __restrict__ geff_out,
+                                                          const double* __restrict__ Gk,
+                                                          const double* __restrict__ A, double* __restrict__ D) {
   __shared__ double lam_sh;
   geff_body(vmid, dS, dv, w, N, N, dpart, nparts, S1, S0, use_gonzalez, lo1, hi1, lo2, hi2, s64, s32, lambda_out,
-            &lam_sh, blockIdx.x);
+            &lam_sh, blockIdx.x, geff_out, Gk, A, D);
 }
 
 __global__ __launch_bounds__(kEwThreads) void k_epilogue(EpiArgs A, int N) {
@@ -273,9 +289,10 @@ __global__ void k_copy_b(EnsView E, const double* __restrict__ src, double* __re
 void launch_geff_pack(const double* vmid, const double* dS, const double* dv, const double* w, int N,
                       const double* dpart, int nparts, const double* S1, const double* S0, int use_gonzalez,
                       double lo1, double hi1, double lo2, double hi2, Src64* s64, Src32* s32, double* lambda_out,
-                      hipStream_t st) {
+                      hipStream_t st, double* geff_out, const double* Gk, const double* A, double* D) {
   hipLaunchKernelGGL(k_geff_pack, dim3((N + kEwThreads - 1) / kEwThreads), dim3(kEwThreads), 0, st, vmid, dS, dv, w,
-                     N, dpart, nparts, S1, S0, use_gonzalez, lo1, hi1, lo2, hi2, s64, s32, lambda_out);
+                     N, dpart, nparts, S1, S0, use_gonzalez, lo1, hi1, lo2, hi2, s64, s32, lambda_out, geff_out, Gk,
+                     A, D);
 }
 
 int epi_blocks(int N) { return (2 * N + kEwThreads - 1) / kEwThreads; }

@@ -126,6 +126,14 @@ typedef enum {
   COLLOP_WARM_MLP = 1       /* Euler + per-marker MLP correction (main.jl:252-254) */
 } collop_warmstart;
 
+typedef enum {
+  COLLOP_SOLVER_ANDERSON = 0, /* step_anderson! (solver.jl:300-423)              */
+  COLLOP_SOLVER_DEFECT = 1    /* frozen-metric defect correction, step_defect!
+                                 (solver.jl:569-634): each outer iteration is one
+                                 full map with the metric A; an inner Anderson
+                                 solve on the frozen-metric map needs no pair sum */
+} collop_solver;
+
 typedef struct {
   double dt;                /* DT                                               */
   int32_t m_anderson;       /* window m (<= 8)                                  */
@@ -152,6 +160,12 @@ typedef struct {
                                FP64 first), 0 = restart it                      */
   int32_t record_training;  /* 1 = append (features, target) of every marker of
                                every step to the warm-start training dataset   */
+  int32_t solver;           /* collop_solver                                    */
+  double dc_eta;            /* DEFECT: inner tolerance relative to the outer
+                               residual (Parameters.jl:89, 0.05)                */
+  int32_t dc_max_inner;     /* DEFECT: inner maps per outer iteration (60)      */
+  int32_t dc_stag_window;   /* DEFECT: outer iterations between stagnation
+                               checks (5)                                       */
 } collop_step_params;
 
 typedef struct {
@@ -165,6 +179,8 @@ typedef struct {
   double energy;            /* E = 1/2 sum w |v|^2                              */
   double momentum[2];       /* P = sum w v                                     */
   double seconds;           /* wall time of the step                           */
+  int32_t inner_maps;       /* DEFECT: frozen-metric maps (no pair sum)         */
+  int32_t pad;
 } collop_step_stats;
 
 /* Upload markers (v: N x 2 column-major, w: N) and project them; keeps S0. */

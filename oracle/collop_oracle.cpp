@@ -287,6 +287,10 @@ struct orc_solver_params {
   int exit_picard_step;
   int use_logsq;
   int nthreads;
+  int solver;          // 0 step_anderson!, 1 step_defect!
+  double dc_eta;       // Parameters.jl:89-91
+  int dc_max_inner;
+  int dc_stag_window;
 };
 
 struct orc_step_stats {
@@ -296,6 +300,7 @@ struct orc_step_stats {
   double r0;
   int exit_reason;  // 0 converged, 1 stagnation, 2 max_iter
   double entropy, energy, p1, p2;
+  int inner;        // step_defect!: frozen-metric maps
 };
 
 namespace {
@@ -933,14 +938,14 @@ void orc_step_anderson(const orc_mesh* ms, const orc_solver_params* Pp, double* 
     const double eff_tol = std::max(P.tol * (nrm(v1, n2) + 1e-30), P.abs_floor);
     if (nrm_r < eff_tol) {
       if (P.exit_picard_step) std::memcpy(v1, Gv.data(), sizeof(double) * n2);
-      *st = {k, n_restart, nrm_r, nrm_r0, 0, 0, 0, 0, 0};
+      *st = {k, n_restart, nrm_r, nrm_r0, 0, 0, 0, 0, 0, 0};
       return;
     }
     if (k > P.stag_window && k % P.stag_window == 0) {
       const double rel = (nrm_best_window - nrm_best) / (nrm_best_window + 1e-30);
       if (rel < P.stag_rel_tol) {
         std::memcpy(v1, vbest.data(), sizeof(double) * n2);
-        *st = {k, n_restart, nrm_best, nrm_r0, 1, 0, 0, 0, 0};
+        *st = {k, n_restart, nrm_best, nrm_r0, 1, 0, 0, 0, 0, 0};
         return;
       }
       nrm_best_window = nrm_best;
@@ -995,7 +1000,131 @@ void orc_step_anderson(const orc_mesh* ms, const orc_solver_params* Pp, double* 
     std::memcpy(Gp.data(), Gv.data(), sizeof(double) * n2);
   }
   std::memcpy(v1, vbest.data(), sizeof(double) * n2);
-  *st = {P.max_iter, n_restart, nrm_best, nrm_r0, 2, 0, 0, 0, 0};
+  *st = {P.max_iter, n_restart, nrm_best, nrm_r0, 2, 0, 0, 0, 0, 0};
+}
+
+}  // extern "C"
+
+namespace {
+
+// solver.jl:507-545 (_anderson_inner!): Anderson on the fixed point u = Phi(u)
+// with the outer solve's window, damping and regularisation and no stagnation
+// logic.  Stops at ||Phi(u) - u|| < tol with u <- Phi(u), or after maxit maps
+// with u the last update.  Returns the number of maps.
+template <class MapF>
+int anderson_inner(MapF&& phi, double* u, int64_t n2, double tol, int maxit, int m, double damping, double reg) {
+  std::vector<double> Fu(n2), r(n2), rp(n2), Fp(n2), uold(n2), dF((size_t)m * n2), dG((size_t)m * n2);
+  int hist = 0, slot = 0;
+  for (int k = 1; k <= maxit; ++k) {
+    std::memcpy(uold.data(), u, sizeof(double) * n2);
+    phi(Fu.data(), u);
+    for (int64_t i = 0; i < n2; ++i) r[i] = Fu[i] - u[i];
+    if (nrm(r.data(), n2) < tol) {
+      std::memcpy(u, Fu.data(), sizeof(double) * n2);
+      return k;
+    }
+    if (k == 1) {
+      for (int64_t i = 0; i < n2; ++i) u[i] = damping * Fu[i] + (1 - damping) * uold[i];
+    } else {
+      slot = slot % m + 1;
+      hist = std::min(hist + 1, m);
+      double* F = &dF[(size_t)(slot - 1) * n2];
+      double* Gc = &dG[(size_t)(slot - 1) * n2];
+      for (int64_t i = 0; i < n2; ++i) {
+        F[i] = r[i] - rp[i];
+        Gc[i] = Fu[i] - Fp[i];
+      }
+      std::vector<double> ATA(hist * hist), ATr(hist), gam(hist);
+      for (int a = 0; a < hist; ++a) {
+        const double* Fa = &dF[(size_t)a * n2];
+        for (int b = 0; b <= a; ++b) {
+          const double* Fb = &dF[(size_t)b * n2];
+          double t = 0.0;
+          for (int64_t i = 0; i < n2; ++i) t += Fa[i] * Fb[i];
+          ATA[a * hist + b] = ATA[b * hist + a] = t;
+        }
+        double t = 0.0;
+        for (int64_t i = 0; i < n2; ++i) t += Fa[i] * r[i];
+        ATr[a] = t;
+      }
+      double dm = 0.0;
+      for (int j = 0; j < hist; ++j) dm += ATA[j * hist + j];
+      const double l2 = reg * dm / hist + 1e-30;
+      for (int j = 0; j < hist; ++j) ATA[j * hist + j] += l2;
+      small_solve(hist, ATA, ATr, gam.data());
+      for (int64_t i = 0; i < n2; ++i) {
+        double x = Fu[i];
+        for (int j = 0; j < hist; ++j) x -= dG[(size_t)j * n2 + i] * gam[j];
+        u[i] = damping * x + (1 - damping) * uold[i];
+      }
+    }
+    std::memcpy(rp.data(), r.data(), sizeof(double) * n2);
+    std::memcpy(Fp.data(), Fu.data(), sizeof(double) * n2);
+  }
+  return maxit;
+}
+
+}  // namespace
+
+extern "C" {
+
+// solver.jl:569-634 (step_defect!): defect correction around a frozen metric.
+// Outer iteration k: the full map G(v_k) with the metric A_k in the same pair
+// pass, then Anderson on Phi_k(u) = G(v_k) - dt A_k (G_eff(u) - G_eff(v_k)) to
+// max(eta ||G(v_k) - v_k||, abs_floor / 2).  The exit returns G(v) (or the best
+// G(v) after stagnation / max_iter).
+void orc_step_defect(const orc_mesh* ms, const orc_solver_params* Pp, double* v1, const double* v0, const double* w,
+                     int64_t N, double S0, orc_step_stats* st, double* iter_res) {
+  const orc_solver_params& P = *Pp;
+  const int64_t n2 = 2 * N;
+  Scratch s(N, ms->n_dofs), s_in(N, ms->n_dofs);
+  std::vector<double> Gv(n2), r(n2), A(3 * N), Gk(n2), Ge(n2), vbest(v1, v1 + n2);
+  double box[4];
+  box_of(ms, box);
+  double nrm0 = 0.0, nrm_best = std::numeric_limits<double>::infinity();
+  double nrm_best_window = std::numeric_limits<double>::infinity();
+  int inner = 0;
+  auto frozen = [&](double* out, const double* u) {
+    landau_geff(ms, Ge.data(), u, v0, w, N, S0, s_in, P.use_gonzalez, P.use_logsq);
+    for (int64_t a = 0; a < N; ++a) {
+      const double d1 = Ge[a] - Gk[a], d2 = Ge[N + a] - Gk[N + a];
+      out[a] = Gv[a] - P.dt * (A[a] * d1 + A[N + a] * d2);
+      out[N + a] = Gv[N + a] - P.dt * (A[N + a] * d1 + A[2 * N + a] * d2);
+    }
+  };
+  for (int k = 1; k <= P.max_iter; ++k) {
+    landau_geff(ms, s.Geff.data(), v1, v0, w, N, S0, s, P.use_gonzalez, P.use_logsq);
+    orc_collision_metric(box, s.vmid.data(), w, s.Geff.data(), N, s.dotv.data(), A.data(), P.nthreads);
+    for (int64_t i = 0; i < n2; ++i) {
+      Gv[i] = v0[i] + P.dt * s.dotv[i];
+      r[i] = Gv[i] - v1[i];
+    }
+    const double nr = nrm(r.data(), n2);
+    if (iter_res) iter_res[k - 1] = nr;
+    if (k == 1) nrm0 = nr;
+    if (nr < nrm_best) {
+      nrm_best = nr;
+      std::memcpy(vbest.data(), Gv.data(), sizeof(double) * n2);
+    }
+    if (nr < std::max(P.tol * (nrm(v1, n2) + 1e-30), P.abs_floor)) {
+      std::memcpy(v1, Gv.data(), sizeof(double) * n2);
+      *st = {k, 0, nr, nrm0, 0, 0, 0, 0, 0, inner};
+      return;
+    }
+    if (k > P.dc_stag_window && k % P.dc_stag_window == 0) {
+      if ((nrm_best_window - nrm_best) / (nrm_best_window + 1e-30) < P.stag_rel_tol) {
+        std::memcpy(v1, vbest.data(), sizeof(double) * n2);
+        *st = {k, 0, nrm_best, nrm0, 1, 0, 0, 0, 0, inner};
+        return;
+      }
+      nrm_best_window = nrm_best;
+    }
+    std::memcpy(Gk.data(), s.Geff.data(), sizeof(double) * n2);
+    inner += anderson_inner(frozen, v1, n2, std::max(P.dc_eta * nr, 0.5 * P.abs_floor), P.dc_max_inner,
+                            P.m_anderson, P.damping, P.reg_factor);
+  }
+  std::memcpy(v1, vbest.data(), sizeof(double) * n2);
+  *st = {P.max_iter, 0, nrm_best, nrm0, 2, 0, 0, 0, 0, inner};
 }
 
 // One full time step of main.jl:226-307 (Landau, Anderson): predictor,
@@ -1007,7 +1136,10 @@ void orc_time_step(const orc_mesh* ms, const orc_solver_params* P, double* v, co
   const int64_t n2 = 2 * N;
   std::vector<double> v1(n2);
   orc_predictor(ms, P, v, w, N, fc, v1.data(), nullptr);
-  orc_step_anderson(ms, P, v1.data(), v, w, N, S0, st, nullptr);
+  if (P->solver == 1)
+    orc_step_defect(ms, P, v1.data(), v, w, N, S0, st, nullptr);
+  else
+    orc_step_anderson(ms, P, v1.data(), v, w, N, S0, st, nullptr);
   std::memcpy(v, v1.data(), sizeof(double) * n2);
   orc_l2_project(ms, v, w, N, fc);
   st->entropy = orc_entropy(ms, fc, P->use_logsq);

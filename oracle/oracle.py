@@ -45,6 +45,10 @@ class SolverParams(C.Structure):
         ("exit_picard_step", C.c_int),
         ("use_logsq", C.c_int),
         ("nthreads", C.c_int),
+        ("solver", C.c_int),
+        ("dc_eta", C.c_double),
+        ("dc_max_inner", C.c_int),
+        ("dc_stag_window", C.c_int),
     ]
 
 
@@ -59,6 +63,7 @@ class StepStats(C.Structure):
         ("energy", C.c_double),
         ("p1", C.c_double),
         ("p2", C.c_double),
+        ("inner", C.c_int),
     ]
 
 
@@ -69,7 +74,7 @@ def default_params(dt: float, m: int = 5, nthreads: int = 0, **kw) -> SolverPara
         stag_window=30, stag_rel_tol=0.1, damp_decay_start=200,
         damp_decay_factor=0.5, restart_factor=float("inf"), damping=0.7,
         reg_factor=1e-10, use_anderson=1, use_gonzalez=1, exit_picard_step=1,
-        use_logsq=0, nthreads=nthreads,
+        use_logsq=0, nthreads=nthreads, solver=0, dc_eta=0.05, dc_max_inner=60, dc_stag_window=5,
     )
     for k, v in kw.items():
         setattr(p, k, v)
@@ -91,7 +96,7 @@ def lib():
         for name in ("orc_mesh_info", "orc_mesh_tables", "orc_mass_dense", "orc_mass_solve",
                      "orc_eval_particle", "orc_l2_rhs", "orc_l2_project", "orc_entropy",
                      "orc_compute_r", "orc_compute_G", "orc_loggrad", "orc_landau_geff",
-                     "orc_picard_map", "orc_predictor", "orc_step_anderson", "orc_time_step"):
+                     "orc_picard_map", "orc_predictor", "orc_step_anderson", "orc_step_defect", "orc_time_step"):
             getattr(L, name).argtypes = None
         _LIB = L
     return _LIB
@@ -272,6 +277,17 @@ class Mesh:
         res = np.zeros(params.max_iter)
         lib().orc_step_anderson(self.h, C.byref(params), _p(v1), _p(v0), _p(w), C.c_int64(v0.shape[0]),
                                 C.c_double(S0), C.byref(st), _p(res))
+        return v1, st, res[: st.iters]
+
+    def step_defect(self, params: SolverParams, v1, v0, w, S0):
+        """step_defect! (solver.jl:569-634); returns (v1, stats, outer residuals)."""
+        v1 = colmajor(v1).copy(order="F")
+        v0 = colmajor(v0)
+        w = np.ascontiguousarray(w, dtype=np.float64)
+        st = StepStats()
+        res = np.zeros(params.max_iter)
+        lib().orc_step_defect(self.h, C.byref(params), _p(v1), _p(v0), _p(w), C.c_int64(v0.shape[0]),
+                              C.c_double(S0), C.byref(st), _p(res))
         return v1, st, res[: st.iters]
 
     def time_step(self, params: SolverParams, v, w, fc, S0):

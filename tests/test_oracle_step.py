@@ -121,3 +121,25 @@ def test_uniform_breakpoints_exact():
     for n in (16, 32, 64):
         bp = collop.config("bimodal_n65536", bp1=uniform_bp(-6, 6, n)).bp1
         assert np.all(np.diff(bp) == bp[1] - bp[0])
+
+
+def test_defect_correction_finds_the_anderson_root():
+    """step_defect! (solver.jl:569-634) restated: its fixed point is a fixed point
+    of the full Picard map (Phi_k(v_k) = G(v_k)), so it must return a root of the
+    same implicit system as step_anderson!, with fewer full maps."""
+    v, w = collop.markers("gaussian", 512, 3)
+    bp = uniform_bp(-6, 6, 12)
+    m = O.Mesh(bp, bp)
+    fc = m.l2_project(v, w)
+    S0 = m.entropy(fc)
+    P = O.default_params(0.005, m=5)
+    v1, _ = m.predictor(P, v, w, fc)
+    va, sa, _ = m.step_anderson(P, v1, v, w, S0)
+    P.solver = 1
+    vd, sd, res = m.step_defect(P, v1, v, w, S0)
+    assert sa.exit_reason == 0 and sd.exit_reason == 0
+    assert sd.iters < sa.iters and sd.inner > 0
+    assert np.all(np.diff(res) < 0)                      # the outer residual falls every iteration
+    r = np.linalg.norm(m.picard_map(P, vd, v, w, S0) - vd)
+    assert r <= 20 * max(1e-12 * np.linalg.norm(vd), 1e-10)
+    assert np.max(np.abs(vd - va)) <= 1e-10
