@@ -4,27 +4,35 @@ Workload (bimodal_n65536): N = 65536 markers from two equal-weight Maxwellians
 at (+-2, 0) with covariance 0.5 I, weights 1/N, P_DEG = 2 B-splines on 32 x 32
 cells over [-6, 6]^2, dt = 0.02, Anderson m = 5, tol 1e-12 (solver knobs of the
 reference's Landau presets).  A "step" is one implicit time step: Euler
-predictor, Anderson solve of the Gonzalez discrete-gradient Picard map to the
-residual tolerance, then the projection / entropy / energy / momentum of the
-new state.  The markers are resident in HBM before the timed region starts.
+predictor, the implicit solve of the Gonzalez discrete-gradient Picard map to
+the residual tolerance max(1e-12 ||v||, 1e-10), then the projection / entropy /
+energy / momentum of the new state.  The markers are resident in HBM before the
+timed region starts.
 
-Two paths run on identical markers:
-  * plain FP64: every Picard map in FP64 (the reference's GPU path, restated),
-  * chosen:     FP32-compensated pair sum for the early maps, FP64 maps to the
-                same tolerance (the exit map is always FP64).
+Paths, on identical markers and to the same tolerance:
+  * plain FP64 (the reference's GPU path, restated): every Picard map in FP64,
+    Anderson solver (step_anderson!, solver.jl:300-423);
+  * chosen: frozen-metric defect correction (the reference's step_defect!,
+    solver.jl:569-634) whose outer maps run the FP32-compensated pair sum with
+    the metric A until the FP32 rounding floor, then FP64 outer maps to finish
+    (every accepted state comes from an FP64 map);
+  * comparison: FP32-compensated maps + FP64 finish with the Anderson solver.
 `value` is the chosen path's steps/s; the plain FP64 figure and the ratio are
 reported beside it.  Per step, |E_{n+1} - E_n| / E_0 and |P_{n+1} - P_n| are
 checked against 1e-10 and the entropy against non-decrease.
 
-The roofline object covers the dominant kernel of the timed region (the pair
-sum): algorithmic work 23 flops per ordered in-domain pair, i.e. 23 N_in
-(N_in - 1) per launch, over the mean launch duration measured with HIP events
+The roofline object covers the dominant kernel of the chosen path's timed region
+(the pair sum with the metric): algorithmic work 23 flops per ordered in-domain
+pair for the pair sum (functions.jl:221-231), 38 with the metric
+(functions.jl:276-281), over the mean launch duration measured with HIP events
 on the library's stream.  The CPU baseline is the oracle (a C++ FP64
 restatement of the reference CPU path, OpenMP over targets like
 functions.jl:210) timing whole Picard maps at the same N on this host.
 
 Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu]
        python bench.py --report precision --config bimaxwellian_n262144 --steps 20
+       python bench.py --config ensemble48 --steps 3
+       python bench.py --kernel-probe        (pair kernels + a 256 MiB copy, for rocprofv3 counters)
 """
 from __future__ import annotations
 
@@ -42,10 +50,17 @@ sys.path.insert(0, os.path.join(ROOT, "collisionoperators.jl_amd"))
 FP64_VECTOR_PEAK_TFLOPS = 78.6     # MI355X published FP64 vector peak
 FP32_VECTOR_PEAK_TFLOPS = 157.3    # MI355X published FP32 vector peak (packed)
 FLOPS_PER_PAIR = 23
+# Pair kernels by profiling kind (collop_profile): name, peak TFLOP/s, flops per
+# ordered pair.  The metric pass (functions.jl:276-281) adds 15 flops to the 23
+# of the pair sum (functions.jl:221-231).
+PAIR_KERNELS = [("k_pair_f64<1,1,false>", FP64_VECTOR_PEAK_TFLOPS, 23),
+                ("k_pair_f32c<1,32,false>", FP32_VECTOR_PEAK_TFLOPS, 23),
+                ("k_pair_f64<1,1,true> (pair sum + metric)", FP64_VECTOR_PEAK_TFLOPS, 38),
+                ("k_pair_f32c<1,32,true> (pair sum + metric)", FP32_VECTOR_PEAK_TFLOPS, 38)]
 # HBM bytes per pair-sum launch at bimodal_n65536 from rocprofv3 FETCH_SIZE (x2.0)
 # and WRITE_SIZE (x1.0), both scaled by the counters of a 256 MiB copy; see
 # profiles/pair_sum_bimodal_n65536.md.
-PAIR_TRAFFIC_BYTES = {"k_pair_f64": 68646 * 1024, "k_pair_f32c": 50717 * 1024}
+PAIR_TRAFFIC_BYTES = {"k_pair_f64<1,1,false>": 68646 * 1024, "k_pair_f32c<1,32,false>": 50717 * 1024}
 
 
 def log(msg: str):
@@ -86,6 +101,7 @@ def run_path(ws, p, v, w, precision, warmup, steps, solver=None):
                 pair_launches=launches, max_dE_rel=dE, max_dP=dP, min_dS=float(dS_min),
                 mean_iters=float(np.mean([s["iters"] for s in stats])),
                 mean_iters_f32=float(np.mean([s["iters_f32"] for s in stats])),
+                mean_inner=float(np.mean([s["inner_maps"] for s in stats])),
                 all_converged=all(s["exit_reason"] == 0 for s in stats), markers=sim.markers())
 
 
@@ -184,7 +200,8 @@ def precision_report(args):
 
 
 def kernel_probe(args):
-    """Three launches of each pair kernel at the workload, then one 256 MiB
+    """Three launches of each pair kernel (plain and with the metric, FP64 and
+    FP32c) at the workload, then one 256 MiB
     device-to-device copy (the known-size calibration of the rocprofv3
     FETCH_SIZE / WRITE_SIZE counters)."""
     import ctypes as C
@@ -205,6 +222,11 @@ def kernel_probe(args):
         collop.compute_collision_gpu(ws, out, v, w, G)
     for _ in range(3):
         collop.compute_collision_gpu32(ws, out, v, w, G)
+    A = np.zeros((N, 3), order="F")
+    for _ in range(3):
+        collop.compute_collision_metric_gpu(ws, out, A, v, w, G)
+    for _ in range(3):
+        collop.compute_collision_metric_gpu(ws, out, A, v, w, G, fp32c=True)
     hip = C.CDLL("libamdhip64.so")
     nbytes = 256 << 20
     a, b = C.c_void_p(), C.c_void_p()
@@ -215,7 +237,7 @@ def kernel_probe(args):
     hip.hipDeviceSynchronize()
     hip.hipFree(a)
     hip.hipFree(b)
-    log(f"[probe] pair kernels x3 (FP64, FP32c) at N={N}; copy {nbytes} bytes")
+    log(f"[probe] pair kernels x3 (FP64, FP32c, FP64 + metric, FP32c + metric) at N={N}; copy {nbytes} bytes")
 
 
 def ensemble_bench(args):
@@ -277,8 +299,10 @@ def main():
     ap.add_argument("--steps", type=int, default=10)
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--chosen-solver", choices=["anderson", "defect"], default="anderson",
+    ap.add_argument("--chosen-solver", choices=["anderson", "defect"], default="defect",
                     help="solver of the chosen path (the plain FP64 path always runs Anderson)")
+    ap.add_argument("--no-variants", action="store_true",
+                    help="skip the FP32c + Anderson comparison path")
     ap.add_argument("--config", default="bimodal_n65536",
                     help="bimodal_n65536 (headline), gaussian_n4096, bimaxwellian_n262144 or ensemble48")
     ap.add_argument("--kernel-probe", action="store_true",
@@ -314,31 +338,38 @@ def main():
     log("[bench] plain FP64 path")
     r64 = run_path(ws, p, v, w, collop.PREC_F64, args.warmup, args.steps)
     log(f"[bench] plain FP64: {r64['steps_per_s']:.3f} steps/s, mean iters {r64['mean_iters']:.1f}")
-    log("[bench] chosen path (FP32-compensated maps + FP64 finish)")
+    log(f"[bench] chosen path (FP32-compensated maps + FP64 finish, {args.chosen_solver} solver)")
     rch = run_path(ws, p, v, w, collop.PREC_F32C, args.warmup, args.steps,
                    solver=collop._lib.SOLVER_DEFECT if args.chosen_solver == "defect" else collop._lib.SOLVER_ANDERSON)
     log(f"[bench] chosen: {rch['steps_per_s']:.3f} steps/s, mean iters {rch['mean_iters']:.1f} "
         f"(f32 {rch['mean_iters_f32']:.1f})")
 
+    rva = None
+    if not args.no_variants and args.chosen_solver == "defect":
+        log("[bench] comparison path: FP32c maps + FP64 finish with the Anderson solver")
+        rva = run_path(ws, p, v, w, collop.PREC_F32C, args.warmup, args.steps, solver=collop._lib.SOLVER_ANDERSON)
+        log(f"[bench] FP32c + Anderson: {rva['steps_per_s']:.3f} steps/s, mean iters {rva['mean_iters']:.1f}")
+
     # roofline of the dominant kernel in the chosen path's timed region
-    ms64, ms32 = rch["pair_ms"]
-    n64, n32 = rch["pair_launches"]
-    flops = FLOPS_PER_PAIR * n_in * (n_in - 1)
-    if ms64 >= ms32:
-        avg_ms = ms64 / max(n64, 1)
-        peak, kname = FP64_VECTOR_PEAK_TFLOPS, "k_pair_f64"
-    else:
-        avg_ms = ms32 / max(n32, 1)
-        peak, kname = FP32_VECTOR_PEAK_TFLOPS, "k_pair_f32c"
+    pairs = n_in * (n_in - 1)
+    kinds = []  # every pair kernel either path launched: name, peak, flops per pair, avg ms, launches
+    for path, res in (("chosen", rch), ("plain_fp64", r64)):
+        for k, (kname, peak, fpp) in enumerate(PAIR_KERNELS):
+            if res["pair_launches"][k]:
+                avg = res["pair_ms"][k] / res["pair_launches"][k]
+                kinds.append({"path": path, "kernel": kname, "launches": res["pair_launches"][k],
+                              "total_ms": round(res["pair_ms"][k], 3), "avg_ms": round(avg, 4),
+                              "tflops": round(fpp * pairs / (avg * 1e-3) / 1e12, 3),
+                              "frac": round(fpp * pairs / (avg * 1e-3) / 1e12 / peak, 4)})
+    dom = int(np.argmax(rch["pair_ms"]))
+    kname, peak, fpp = PAIR_KERNELS[dom]
+    avg_ms = rch["pair_ms"][dom] / max(rch["pair_launches"][dom], 1)
+    flops = fpp * pairs
     achieved = flops / (avg_ms * 1e-3) / 1e12
-    avg64 = r64["pair_ms"][0] / max(r64["pair_launches"][0], 1)
     traffic = PAIR_TRAFFIC_BYTES.get(kname) if args.config == "bimodal_n65536" else None
     roofline = {"bound": "valu", "kernel": kname, "achieved": round(achieved, 3), "peak": peak, "unit": "TFLOP/s",
                 "frac": round(achieved / peak, 4), "traffic": traffic, "avg_launch_ms": round(avg_ms, 4),
-                "flops_per_launch": flops,
-                "fp64_pair_avg_ms": round(avg64, 4),
-                "fp64_pair_tflops": round(flops / (avg64 * 1e-3) / 1e12, 3),
-                "fp32c_pair_avg_ms": round(ms32 / max(n32, 1), 4) if n32 else None}
+                "flops_per_pair": fpp, "flops_per_launch": flops, "pair_kernels": kinds}
 
     cpu = None
     if not args.no_cpu:
@@ -364,13 +395,22 @@ def main():
                    "m_anderson": p.m_anderson, "tol": p.tol, "abs_floor": p.abs_floor},
         "fp64_steps_per_s": round(r64["steps_per_s"], 4),
         "ratio_vs_fp64": round(ratio, 4),
-        "mean_iters": {"fp64": r64["mean_iters"], "chosen": rch["mean_iters"], "chosen_f32": rch["mean_iters_f32"]},
+        "chosen_path": ("FP32c outer maps (F and metric A) then FP64 outer maps, frozen-metric defect correction "
+                        "(step_defect!), Euler predictor" if args.chosen_solver == "defect" else
+                        "FP32c maps then FP64 maps, Anderson (step_anderson!), Euler predictor"),
+        "plain_path": "FP64 maps, Anderson (step_anderson!), Euler predictor",
+        "f32c_anderson": (None if rva is None else
+                          {"steps_per_s": round(rva["steps_per_s"], 4),
+                           "ratio_vs_fp64": round(rva["steps_per_s"] / r64["steps_per_s"], 4),
+                           "mean_iters": rva["mean_iters"], "all_converged": rva["all_converged"]}),
+        "mean_iters": {"fp64": r64["mean_iters"], "chosen": rch["mean_iters"], "chosen_f32": rch["mean_iters_f32"],
+                       "chosen_inner": rch["mean_inner"]},
         "conservation": {"fp64": {"max_dE_rel": r64["max_dE_rel"], "max_dP": r64["max_dP"], "min_dS": r64["min_dS"]},
                          "chosen": {"max_dE_rel": rch["max_dE_rel"], "max_dP": rch["max_dP"],
                                     "min_dS": rch["min_dS"]}},
         "all_converged": r64["all_converged"] and rch["all_converged"],
         "chosen_vs_fp64_max_abs_dv": dv,
-        "pair_interactions_per_s": round(n_in * (n_in - 1) / (avg64 * 1e-3), 1),
+        "pair_interactions_per_s": round(pairs / (r64["pair_ms"][0] / max(r64["pair_launches"][0], 1) * 1e-3), 1),
         "roofline": roofline,
         "cpu_baseline": cpu,
     }

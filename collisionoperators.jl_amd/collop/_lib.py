@@ -69,6 +69,8 @@ class StepParams(C.Structure):
         ("dc_eta", C.c_double),
         ("dc_max_inner", C.c_int32),
         ("dc_stag_window", C.c_int32),
+        ("dc_outer_m", C.c_int32),
+        ("dc_near_theta", C.c_double),
     ]
 
 
@@ -116,6 +118,7 @@ SIGNATURES = [
     ("collop_collision_f64", C.c_int, [_vp, _dp, _dp, _dp, _dp, C.c_int64]),
     ("collop_collision_f32c", C.c_int, [_vp, _dp, _dp, _dp, _dp, C.c_int64]),
     ("collop_collision_metric_f64", C.c_int, [_vp, _dp, _dp, _dp, _dp, _dp, C.c_int64]),
+    ("collop_collision_metric_f32c", C.c_int, [_vp, _dp, _dp, _dp, _dp, _dp, C.c_int64]),
     ("collop_l2_rhs", C.c_int, [_vp, _dp, _dp, C.c_int64, _dp]),
     ("collop_l2_project", C.c_int, [_vp, _dp, _dp, C.c_int64, _dp]),
     ("collop_grad_gather", C.c_int, [_vp, _dp, C.c_int64, _dp, _dp]),

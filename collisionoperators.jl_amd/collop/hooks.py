@@ -64,16 +64,17 @@ def compute_collision_gpu32(ws: Workspace, dot_v, v_parts, w_parts, G):
     return _collision(ws.ctx.lib.collop_collision_f32c, ws, dot_v, v_parts, w_parts, G)
 
 
-def compute_collision_metric_gpu(ws: Workspace, dot_v, A, v_parts, w_parts, G):
-    """COLLMETRIC_FN (collision_gpu.jl:377-402; semantics functions.jl:249-290)."""
+def compute_collision_metric_gpu(ws: Workspace, dot_v, A, v_parts, w_parts, G, fp32c: bool = False):
+    """COLLMETRIC_FN (collision_gpu.jl:377-402; semantics functions.jl:249-290).
+    ``fp32c``: FP32 pair terms with compensated sums (collop_collision_metric_f32c)."""
     ws.activate()
     v, N = _markers(v_parts)
     w = _vec(w_parts, N)
     G = _colmajor(G, (N, 2))
     t1 = _out(dot_v, 2, N)
     t2 = _out(A, 3, N)
-    ws.ctx.check(ws.ctx.lib.collop_collision_metric_f64(ws.ctx.h, _lib.dptr(v), _lib.dptr(w), _lib.dptr(G),
-                                                        _lib.dptr(t1), _lib.dptr(t2), N))
+    fn = ws.ctx.lib.collop_collision_metric_f32c if fp32c else ws.ctx.lib.collop_collision_metric_f64
+    ws.ctx.check(fn(ws.ctx.h, _lib.dptr(v), _lib.dptr(w), _lib.dptr(G), _lib.dptr(t1), _lib.dptr(t2), N))
     if t1 is not dot_v:
         dot_v[...] = t1
     if t2 is not A:

@@ -56,6 +56,8 @@ class SimParameters:
     dc_eta: float = 0.05             # Parameters.jl:89-91
     dc_max_inner: int = 60
     dc_stag_window: int = 5
+    dc_outer_m: int = 0              # extension: outer Anderson window (0 = the reference's outer iteration)
+    dc_near_theta: float = 0.0       # extension: exact near pairs in the frozen map (0 = none)
 
     def step_params(self, precision: int | None = None, warmstart: int | None = None,
                     solver: int | None = None) -> _lib.StepParams:
@@ -73,7 +75,8 @@ class SimParameters:
             use_anderson=int(self.use_anderson), use_gonzalez=int(self.use_gonzalez),
             exit_picard_step=int(self.exit_picard_step), use_logsq=int(self.use_logsq),
             precision=prec, warmstart=warm, f32_floor_factor=16.0, f32_keep_window=1, record_training=0,
-            solver=solver, dc_eta=self.dc_eta, dc_max_inner=self.dc_max_inner, dc_stag_window=self.dc_stag_window)
+            solver=solver, dc_eta=self.dc_eta, dc_max_inner=self.dc_max_inner, dc_stag_window=self.dc_stag_window,
+            dc_outer_m=self.dc_outer_m, dc_near_theta=self.dc_near_theta)
 
 
 def parse_overrides(p: SimParameters, args: list[str]) -> SimParameters:

@@ -50,12 +50,13 @@ class Simulation:
 
 
 def profile(ws: Workspace, enable: bool):
-    """Toggle HIP-event timing of the pair kernel; returns (ms_total[2], launches[2])
-    accumulated since the previous call (index 0 FP64, 1 FP32c)."""
-    ms = np.zeros(2)
-    n = (C.c_long * 2)()
+    """Toggle HIP-event timing of the pair kernels; returns (ms_total[4], launches[4])
+    accumulated since the previous call (index 0 FP64, 1 FP32c, 2 FP64 + metric,
+    3 FP32c + metric)."""
+    ms = np.zeros(4)
+    n = (C.c_long * 4)()
     ws.ctx.check(ws.ctx.lib.collop_profile(ws.ctx.h, int(enable), _lib.dptr(ms), n))
-    return ms, [n[0], n[1]]
+    return ms, [n[i] for i in range(4)]
 
 
 def set_pair_tile(ws: Workspace, gamma_per_lane: int = 1, slices: int = 0, newton: int = 1):

@@ -271,7 +271,9 @@ int collop_destroy(collop_ctx* ctx) {
                   &ctx->b_s32, &ctx->b_part, &ctx->b_spart_sc, &ctx->b_mom, &ctx->b_dpart, &ctx->b_iota,
                   &ctx->b_spart_q, &ctx->b_vec, &ctx->b_scal, &ctx->b_statp, &ctx->b_stat, &ctx->s_v, &ctx->s_w,
                   &ctx->s_v1, &ctx->s_Gv, &ctx->s_r, &ctx->s_rp, &ctx->s_Gp, &ctx->s_best, &ctx->s_dF, &ctx->s_dG,
-                  &ctx->s_fc, &ctx->s_dotv, &ctx->s_Gvo, &ctx->s_Gk, &ctx->s_FA, &ctx->s_D, &ctx->m_params, &ctx->m_adam_m, &ctx->m_adam_v, &ctx->m_data, &ctx->m_rec,
+                  &ctx->s_fc, &ctx->s_dotv, &ctx->s_Gvo, &ctx->s_Gk, &ctx->s_FA, &ctx->s_D, &ctx->s_Ge, &ctx->s_vmk, &ctx->s_xk, &ctx->s_oU,
+                  &ctx->s_oUp, &ctx->s_of, &ctx->s_ofp, &ctx->s_odF, &ctx->s_odG, &ctx->b_near_list, &ctx->b_near_count,
+                  &ctx->b_ntg, &ctx->b_noff, &ctx->b_nal, &ctx->m_params, &ctx->m_adam_m, &ctx->m_adam_v, &ctx->m_data, &ctx->m_rec,
                   &ctx->m_y, &ctx->m_gpart, &ctx->m_lpart, &ctx->m_tiles, &ctx->m_eval, &ctx->m_grad})
     release(*b);
   for (auto& e : ctx->ev_pool) {
@@ -416,7 +418,8 @@ static int collision_common(collop_ctx* ctx, int kind, const double* v, const do
   if (r) return r;
   if ((r = check_N(ctx, N))) return r;
   if (N == 0) return COLLOP_OK;
-  if (!v || !w || !G || !dot_v || (kind == 2 && !A)) return fail(ctx, COLLOP_ERR_INVALID, "null array");
+  const bool metric = kind >= 2;  // 2: metric FP64, 3: metric FP32c
+  if (!v || !w || !G || !dot_v || (metric && !A)) return fail(ctx, COLLOP_ERR_INVALID, "null array");
   HIPC(hipSetDevice(ctx->device));
   if ((r = ensure_scratch(ctx, N))) return r;
   const int n = (int)N;
@@ -426,23 +429,31 @@ static int collision_common(collop_ctx* ctx, int kind, const double* v, const do
   HIPC(hipMemcpyAsync(ctx->b_G.p, G, 2 * N * 8, hipMemcpyHostToDevice, ctx->stream));
   const MeshView& m = ctx->mv;
   launch_pack_src(P<double>(ctx->b_v), P<double>(ctx->b_G), P<double>(ctx->b_w), n, m.lo1, m.hi1, m.lo2, m.hi2,
-                  kind == 1 ? nullptr : P<Src64>(ctx->b_s64), kind == 1 ? P<Src32>(ctx->b_s32) : nullptr,
-                  ctx->stream);
+                  (kind == 1 || kind == 3) ? nullptr : P<Src64>(ctx->b_s64),
+                  (kind == 1 || kind == 3) ? P<Src32>(ctx->b_s32) : nullptr, ctx->stream);
   CHECK_LAUNCH();
   int S;
   if (kind == 2) {
+    prof_begin(ctx, 2);
     launch_pair_metric_f64(P<Src64>(ctx->b_s64), n, ctx->tile64, P<double>(ctx->b_part), ctx->stream);
     CHECK_LAUNCH();
+    prof_end(ctx);
     S = ctx->tile64.slices;
+  } else if (kind == 3) {
+    prof_begin(ctx, 3);
+    launch_pair_metric_f32c(P<Src32>(ctx->b_s32), n, ctx->tile32, P<double>(ctx->b_part), ctx->stream);
+    CHECK_LAUNCH();
+    prof_end(ctx);
+    S = ctx->tile32.slices;
   } else {
     if ((r = run_pair(ctx, kind, n, P<double>(ctx->b_part)))) return r;
     S = kind == 0 ? ctx->tile64.slices : ctx->tile32.slices;
   }
-  const int64_t len = (kind == 2 ? 5 : 2) * N;
+  const int64_t len = (metric ? 5 : 2) * N;
   launch_slice_sum(P<double>(ctx->b_part), S, len, P<double>(ctx->b_out), ctx->stream);
   CHECK_LAUNCH();
   HIPC(hipMemcpyAsync(dot_v, ctx->b_out.p, 2 * N * 8, hipMemcpyDeviceToHost, ctx->stream));
-  if (kind == 2) HIPC(hipMemcpyAsync(A, P<double>(ctx->b_out) + 2 * N, 3 * N * 8, hipMemcpyDeviceToHost, ctx->stream));
+  if (metric) HIPC(hipMemcpyAsync(A, P<double>(ctx->b_out) + 2 * N, 3 * N * 8, hipMemcpyDeviceToHost, ctx->stream));
   HIPC(hipStreamSynchronize(ctx->stream));
   prof_collect(ctx);
   return COLLOP_OK;
@@ -461,6 +472,11 @@ int collop_collision_f32c(collop_ctx* ctx, const double* v, const double* w, con
 int collop_collision_metric_f64(collop_ctx* ctx, const double* v, const double* w, const double* G, double* dot_v,
                                 double* A, int64_t N) {
   return collision_common(ctx, 2, v, w, G, dot_v, A, N);
+}
+
+int collop_collision_metric_f32c(collop_ctx* ctx, const double* v, const double* w, const double* G, double* dot_v,
+                                 double* A, int64_t N) {
+  return collision_common(ctx, 3, v, w, G, dot_v, A, N);
 }
 
 static int rhs_common(collop_ctx* ctx, const double* v, const double* w, int64_t N, double* out, bool solve) {
@@ -662,9 +678,21 @@ int geff_dev(collop_ctx* ctx, const collop_step_params& p, const double* vin, Sr
 
 // Anderson epilogue of a map whose value is v0e + dte * (slice sum of part)
 // (S slices of 2N), then the statistics reduction and their download.
+int epilogue_run(collop_ctx* ctx, const EpiArgs& a) {
+  const int n = (int)ctx->N;
+  launch_epilogue(a, n, ctx->stream);
+  CHECK_LAUNCH();
+  launch_sum_rows(P<double>(ctx->b_statp), epi_blocks(n), kStatWidth, 1, kStatWidth, P<double>(ctx->b_stat),
+                  ctx->stream);
+  CHECK_LAUNCH();
+  HIPC(hipMemcpyAsync(ctx->h_stat, ctx->b_stat.p, kStatWidth * 8, hipMemcpyDeviceToHost, ctx->stream));
+  HIPC(hipStreamSynchronize(ctx->stream));
+  prof_collect(ctx);
+  return COLLOP_OK;
+}
+
 int epilogue_dev(collop_ctx* ctx, const double* part, int S, const double* v0e, double dte, const double* vin,
                  double* Gv, int col, int hist) {
-  const int n = (int)ctx->N;
   EpiArgs a{};
   a.part = part;
   a.S = S;
@@ -680,14 +708,32 @@ int epilogue_dev(collop_ctx* ctx, const double* part, int S, const double* v0e, 
   a.col = col;
   a.hist = hist;
   a.stat_part = P<double>(ctx->b_statp);
-  launch_epilogue(a, n, ctx->stream);
-  CHECK_LAUNCH();
-  launch_sum_rows(P<double>(ctx->b_statp), epi_blocks(n), kStatWidth, 1, kStatWidth, P<double>(ctx->b_stat),
-                  ctx->stream);
-  CHECK_LAUNCH();
-  HIPC(hipMemcpyAsync(ctx->h_stat, ctx->b_stat.p, kStatWidth * 8, hipMemcpyDeviceToHost, ctx->stream));
-  HIPC(hipStreamSynchronize(ctx->stream));
-  prof_collect(ctx);
+  return epilogue_run(ctx, a);
+}
+
+// Near pairs found by the step's first outer map -> CSR by target (sorted, so
+// the correction sums in a fixed order).  Too many pairs: no near field.
+int build_near(collop_ctx* ctx) {
+  int cnt = 0;
+  HIPC(hipMemcpy(&cnt, ctx->b_near_count.p, sizeof(int), hipMemcpyDeviceToHost));
+  ctx->near_nt = 0;
+  if (cnt <= 0 || cnt > kNearCap) return COLLOP_OK;
+  std::vector<int2> L(cnt);
+  HIPC(hipMemcpy(L.data(), ctx->b_near_list.p, (size_t)cnt * sizeof(int2), hipMemcpyDeviceToHost));
+  std::sort(L.begin(), L.end(), [](const int2& a, const int2& b) { return a.x != b.x ? a.x < b.x : a.y < b.y; });
+  std::vector<int> tg, off, al(cnt);
+  for (int i = 0; i < cnt; ++i) {
+    if (i == 0 || L[i].x != L[i - 1].x) {
+      tg.push_back(L[i].x);
+      off.push_back(i);
+    }
+    al[i] = L[i].y;
+  }
+  off.push_back(cnt);
+  HIPC(hipMemcpy(ctx->b_ntg.p, tg.data(), tg.size() * 4, hipMemcpyHostToDevice));
+  HIPC(hipMemcpy(ctx->b_noff.p, off.data(), off.size() * 4, hipMemcpyHostToDevice));
+  HIPC(hipMemcpy(ctx->b_nal.p, al.data(), al.size() * 4, hipMemcpyHostToDevice));
+  ctx->near_nt = (int)tg.size();
   return COLLOP_OK;
 }
 
@@ -703,18 +749,26 @@ int picard_map_dev(collop_ctx* ctx, const collop_step_params& p, int kind, const
 }
 
 // Outer map of defect correction (solver.jl:590-594): G(v1) and the metric A in
-// one FP64 pair pass (compute_collision_metric!, functions.jl:249-290); keeps
-// G_eff(v1) in s_Gk, [F, A] in s_FA and G(v1) in s_Gvo.
-int outer_map_dev(collop_ctx* ctx, const collop_step_params& p) {
+// one pair pass (compute_collision_metric!, functions.jl:249-290), FP64 (kind 0)
+// or FP32-compensated (kind 1); keeps G_eff(v1) in s_Gk, [F, A] in s_FA and
+// G(v1) in s_Gvo.
+int outer_map_dev(collop_ctx* ctx, const collop_step_params& p, int kind, bool find_near) {
   const int n = (int)ctx->N;
-  int r = geff_dev(ctx, p, P<double>(ctx->s_v1), P<Src64>(ctx->b_s64), nullptr, P<double>(ctx->s_Gk), nullptr,
-                   nullptr, nullptr);
+  int r = geff_dev(ctx, p, P<double>(ctx->s_v1), kind == 0 ? P<Src64>(ctx->b_s64) : nullptr,
+                   kind == 1 ? P<Src32>(ctx->b_s32) : nullptr, P<double>(ctx->s_Gk), nullptr, nullptr, nullptr);
   if (r) return r;
-  prof_begin(ctx, 0);
-  launch_pair_metric_f64(P<Src64>(ctx->b_s64), n, ctx->tile64, P<double>(ctx->b_part), ctx->stream);
+  const PairTile& t = kind == 0 ? ctx->tile64 : ctx->tile32;
+  NearOut near{p.dc_near_theta * p.dc_near_theta, P<int>(ctx->b_near_count), P<int2>(ctx->b_near_list), kNearCap};
+  if (find_near) HIPC(hipMemsetAsync(ctx->b_near_count.p, 0, sizeof(int), ctx->stream));
+  prof_begin(ctx, 2 + kind);
+  if (kind == 0)
+    launch_pair_metric_f64(P<Src64>(ctx->b_s64), n, t, P<double>(ctx->b_part), ctx->stream, find_near ? &near : nullptr);
+  else
+    launch_pair_metric_f32c(P<Src32>(ctx->b_s32), n, t, P<double>(ctx->b_part), ctx->stream,
+                            find_near ? &near : nullptr);
   CHECK_LAUNCH();
   prof_end(ctx);
-  launch_slice_sum(P<double>(ctx->b_part), ctx->tile64.slices, (int64_t)5 * n, P<double>(ctx->s_FA), ctx->stream);
+  launch_slice_sum(P<double>(ctx->b_part), t.slices, (int64_t)5 * n, P<double>(ctx->s_FA), ctx->stream);
   CHECK_LAUNCH();
   return epilogue_dev(ctx, P<double>(ctx->s_FA), 1, P<double>(ctx->s_v), p.dt, P<double>(ctx->s_v1),
                       P<double>(ctx->s_Gvo), -1, 0);
@@ -724,9 +778,16 @@ int outer_map_dev(collop_ctx* ctx, const collop_step_params& p) {
 // (G_eff(u) - G_k), no pair sum; Phi(u) lands in s_Gv with the Anderson epilogue.
 int frozen_map_dev(collop_ctx* ctx, const collop_step_params& p, int col, int hist) {
   const int n = (int)ctx->N;
-  int r = geff_dev(ctx, p, P<double>(ctx->s_v1), nullptr, nullptr, nullptr, P<double>(ctx->s_Gk),
-                   P<double>(ctx->s_FA) + 2 * (size_t)n, P<double>(ctx->s_D));
+  const bool near = ctx->near_nt > 0;
+  int r = geff_dev(ctx, p, P<double>(ctx->s_v1), nullptr, nullptr, near ? P<double>(ctx->s_Ge) : nullptr,
+                   P<double>(ctx->s_Gk), P<double>(ctx->s_FA) + 2 * (size_t)n, P<double>(ctx->s_D));
   if (r) return r;
+  if (near) {
+    launch_near(ctx->near_nt, P<int>(ctx->b_ntg), P<int>(ctx->b_noff), P<int>(ctx->b_nal), n, P<double>(ctx->s_w),
+                P<double>(ctx->b_vmid), P<double>(ctx->s_Ge), P<double>(ctx->s_vmk), P<double>(ctx->s_Gk),
+                P<double>(ctx->s_D), ctx->stream);
+    CHECK_LAUNCH();
+  }
   return epilogue_dev(ctx, P<double>(ctx->s_D), 1, P<double>(ctx->s_Gvo), -p.dt, P<double>(ctx->s_v1),
                       P<double>(ctx->s_Gv), col, hist);
 }
@@ -822,6 +883,9 @@ int validate_params(collop_ctx* ctx, const collop_step_params* p) {
   if (p->solver == COLLOP_SOLVER_DEFECT &&
       (!(p->dc_eta > 0.0 && p->dc_eta < 1.0) || p->dc_max_inner < 1 || p->dc_stag_window < 1))
     return fail(ctx, COLLOP_ERR_INVALID, "defect correction needs 0 < dc_eta < 1, dc_max_inner >= 1, dc_stag_window >= 1");
+  if (p->solver == COLLOP_SOLVER_DEFECT &&
+      (p->dc_outer_m < 0 || p->dc_outer_m > kMaxWindow || !(p->dc_near_theta >= 0.0) || !std::isfinite(p->dc_near_theta)))
+    return fail(ctx, COLLOP_ERR_INVALID, "defect correction needs 0 <= dc_outer_m <= 8 and dc_near_theta >= 0");
   if (p->record_training && ctx->m_size + ctx->N > ctx->m_cap)
     return fail(ctx, COLLOP_ERR_INVALID, "training dataset full (collop_mlp_dataset_alloc)");
   return COLLOP_OK;
@@ -840,27 +904,62 @@ int defect_solve(collop_ctx* ctx, const collop_step_params& p, collop_step_stats
   double nrm_best_window = std::numeric_limits<double>::infinity(), res_out = 0.0;
   int exit_reason = 2, k_done = p.max_iter, inner = 0, r;
   const bool trace = std::getenv("COLLOP_TRACE") != nullptr;
+  // F32C: FP32-compensated outer maps (F and A) until the outer residual reaches
+  // the FP32 rounding floor of the step increment (or stops improving); then
+  // FP64 outer maps, which alone decide convergence, stagnation and the result.
+  bool use32 = p.precision == COLLOP_PREC_F32C;
+  const double floor_factor = p.f32_floor_factor > 0.0 ? p.f32_floor_factor : 16.0;
+  double best32 = std::numeric_limits<double>::infinity();
+  int since_best32 = 0, iters32 = 0, k64 = 0;
+  // outer Anderson window (extension): Anderson with beta = 1 on v -> T(v), the
+  // inner solution, with the epilogue's column bookkeeping
+  const int mo = p.dc_outer_m;
+  int ohist = 0, oslot = 0;
+  bool oprev = false;
+  double oATA[kMaxWindow * kMaxWindow] = {0};
+  ctx->near_nt = 0;
   // the best iterate starts as the initial guess
   if ((r = copy_d2d(ctx, ctx->s_best, ctx->s_v1, n2b))) return r;
   for (int k = 1; k <= p.max_iter; ++k) {
-    if ((r = outer_map_dev(ctx, p))) return r;
+    const bool find_near = k == 1 && p.dc_near_theta > 0.0;
+    if ((r = outer_map_dev(ctx, p, use32 ? 1 : 0, find_near))) return r;
+    if (find_near && (r = build_near(ctx))) return r;
     const double nr = std::sqrt(ctx->h_stat[0]), nv = std::sqrt(ctx->h_stat[1]);
     if (!std::isfinite(nr) || !std::isfinite(nv))
       return fail(ctx, COLLOP_ERR_NONFINITE, "non-finite residual in outer map " + std::to_string(k));
-    if (trace) std::fprintf(stderr, "  [defect] k=%d |r|=%.3e inner so far %d\n", k, nr, inner);
+    if (trace) std::fprintf(stderr, "  [defect %s] k=%d |r|=%.3e inner so far %d\n", use32 ? "f32" : "f64", k, nr, inner);
     if (k == 1) nrm0 = nr;
-    if (nr < nrm_best) {
-      nrm_best = nr;
-      if ((r = copy_d2d(ctx, ctx->s_best, ctx->s_Gvo, n2b))) return r;
+    if (use32) {
+      iters32++;
+      const double floor32 = floor_factor * 5.96e-8 * std::sqrt(ctx->h_stat[2]);
+      if (nr < best32) {
+        best32 = nr;
+        since_best32 = 0;
+      } else {
+        since_best32++;
+      }
+      if (nr < std::max(floor32, std::max(p.tol * (nv + 1e-30), p.abs_floor)) || since_best32 >= 2 ||
+          k >= p.max_iter / 2) {
+        use32 = false;  // re-evaluate this iterate with an FP64 outer map
+        oprev = false;  // and restart the outer window on FP64 maps
+        ohist = oslot = 0;
+        continue;
+      }
+    } else {
+      k64++;
+      if (nr < nrm_best) {
+        nrm_best = nr;
+        if ((r = copy_d2d(ctx, ctx->s_best, ctx->s_Gvo, n2b))) return r;
+      }
+      if (nr < std::max(p.tol * (nv + 1e-30), p.abs_floor)) {
+        std::swap(ctx->s_v1, ctx->s_Gvo);
+        exit_reason = 0;
+        k_done = k;
+        res_out = nr;
+        break;
+      }
     }
-    if (nr < std::max(p.tol * (nv + 1e-30), p.abs_floor)) {
-      std::swap(ctx->s_v1, ctx->s_Gvo);
-      exit_reason = 0;
-      k_done = k;
-      res_out = nr;
-      break;
-    }
-    if (k > p.dc_stag_window && k % p.dc_stag_window == 0) {
+    if (!use32 && k64 > p.dc_stag_window && k64 % p.dc_stag_window == 0) {
       if ((nrm_best_window - nrm_best) / (nrm_best_window + 1e-30) < p.stag_rel_tol) {
         std::swap(ctx->s_v1, ctx->s_best);
         exit_reason = 1;
@@ -870,6 +969,8 @@ int defect_solve(collop_ctx* ctx, const collop_step_params& p, collop_step_stats
       }
       nrm_best_window = nrm_best;
     }
+    if (ctx->near_nt > 0 && (r = copy_d2d(ctx, ctx->s_vmk, ctx->b_vmid, n2b))) return r;  // v_mid of v_k
+    if (mo > 0 && (r = copy_d2d(ctx, ctx->s_xk, ctx->s_v1, n2b))) return r;
     // inner Anderson on u = Phi_k(u), u = s_v1 (solver.jl:507-545)
     const double tol_in = std::max(p.dc_eta * nr, 0.5 * p.abs_floor);
     int hist = 0, slot = 0;
@@ -914,13 +1015,56 @@ int defect_solve(collop_ctx* ctx, const collop_step_params& p, collop_step_stats
       std::swap(ctx->s_rp, ctx->s_r);
       std::swap(ctx->s_Gp, ctx->s_Gv);
     }
+    if (mo > 0) {  // outer Anderson: f = U - x_k with U the inner solution
+      EpiArgs a{};
+      a.part = nullptr;
+      a.S = 0;
+      a.v0 = P<double>(ctx->s_v1);
+      a.v1 = P<double>(ctx->s_xk);
+      a.dt = 0.0;
+      a.Gv = P<double>(ctx->s_oU);
+      a.r = P<double>(ctx->s_of);
+      a.rp = P<double>(ctx->s_ofp);
+      a.Gp = P<double>(ctx->s_oUp);
+      a.dF = P<double>(ctx->s_odF);
+      a.dG = P<double>(ctx->s_odG);
+      a.col = oprev ? oslot % mo : -1;
+      a.hist = oprev ? std::min(ohist + 1, mo) : 0;
+      a.stat_part = P<double>(ctx->b_statp);
+      if ((r = epilogue_run(ctx, a))) return r;
+      if (oprev) {
+        const double* st = ctx->h_stat;
+        oslot = oslot % mo + 1;
+        ohist = a.hist;
+        for (int i = 0; i < ohist; ++i) {
+          oATA[a.col * kMaxWindow + i] = st[3 + i];
+          oATA[i * kMaxWindow + a.col] = st[3 + i];
+        }
+        double A[kMaxWindow * kMaxWindow], b[kMaxWindow], g[kMaxWindow];
+        double dm = 0.0;
+        for (int i = 0; i < ohist; ++i) {
+          for (int c = 0; c < ohist; ++c) A[i * ohist + c] = oATA[i * kMaxWindow + c];
+          b[i] = st[3 + kMaxWindow + i];
+          dm += A[i * ohist + i];
+        }
+        const double l2 = p.reg_factor * dm / ohist + 1e-30;
+        for (int i = 0; i < ohist; ++i) A[i * ohist + i] += l2;
+        small_solve(ohist, A, b, g);
+        launch_update(P<double>(ctx->s_v1), P<double>(ctx->s_oU), P<double>(ctx->s_odG), g, ohist, 1.0, n,
+                      ctx->stream);
+        CHECK_LAUNCH();
+      }
+      std::swap(ctx->s_of, ctx->s_ofp);
+      std::swap(ctx->s_oU, ctx->s_oUp);
+      oprev = true;
+    }
   }
   if (exit_reason == 2) {
     std::swap(ctx->s_v1, ctx->s_best);
     res_out = nrm_best;
   }
   out->iters = k_done;
-  out->iters_f32 = 0;
+  out->iters_f32 = iters32;
   out->n_restart = 0;
   out->exit_reason = exit_reason;
   out->residual = res_out;
@@ -1074,9 +1218,17 @@ int collop_state_set(collop_ctx* ctx, const double* v, const double* w, int64_t 
   if ((r = ensure_scratch(ctx, N))) return r;
   const size_t n2b = (size_t)2 * N * 8;
   for (DBuf* b : {&ctx->s_v, &ctx->s_v1, &ctx->s_Gv, &ctx->s_r, &ctx->s_rp, &ctx->s_Gp, &ctx->s_best, &ctx->s_dotv,
-                  &ctx->s_Gvo, &ctx->s_Gk, &ctx->s_D})
+                  &ctx->s_Gvo, &ctx->s_Gk, &ctx->s_D, &ctx->s_Ge, &ctx->s_vmk, &ctx->s_xk, &ctx->s_oU, &ctx->s_oUp,
+                  &ctx->s_of, &ctx->s_ofp})
     ENSURE(*b, n2b);
   ENSURE(ctx->s_FA, (size_t)5 * N * 8);
+  ENSURE(ctx->s_odF, n2b * kMaxWindow);
+  ENSURE(ctx->s_odG, n2b * kMaxWindow);
+  ENSURE(ctx->b_near_list, (size_t)kNearCap * sizeof(int2));
+  ENSURE(ctx->b_near_count, 64);
+  ENSURE(ctx->b_ntg, (size_t)kNearCap * 4);
+  ENSURE(ctx->b_noff, (size_t)(kNearCap + 1) * 4);
+  ENSURE(ctx->b_nal, (size_t)kNearCap * 4);
   ENSURE(ctx->s_w, (size_t)N * 8);
   ENSURE(ctx->s_dF, n2b * kMaxWindow);
   ENSURE(ctx->s_dG, n2b * kMaxWindow);
@@ -1347,20 +1499,17 @@ int collop_set_pair_tile(collop_ctx* ctx, int gamma_per_lane, int slices, int ne
   return COLLOP_OK;
 }
 
-// HIP-event timing of every pair-kernel launch (kind 0 = FP64, 1 = FP32c).
+// HIP-event timing of every pair-kernel launch (kind 0 = FP64, 1 = FP32c,
+// 2 = FP64 with the metric, 3 = FP32c with the metric).
 int collop_profile(collop_ctx* ctx, int enable, double* ms_total, long* launches) {
   if (!ctx) return COLLOP_ERR_INVALID;
-  if (ms_total) {
-    ms_total[0] = ctx->prof_ms[0];
-    ms_total[1] = ctx->prof_ms[1];
-  }
-  if (launches) {
-    launches[0] = ctx->prof_n[0];
-    launches[1] = ctx->prof_n[1];
+  for (int k = 0; k < 4; ++k) {
+    if (ms_total) ms_total[k] = ctx->prof_ms[k];
+    if (launches) launches[k] = ctx->prof_n[k];
+    ctx->prof_ms[k] = 0.0;
+    ctx->prof_n[k] = 0;
   }
   ctx->profile = enable != 0;
-  ctx->prof_ms[0] = ctx->prof_ms[1] = 0.0;
-  ctx->prof_n[0] = ctx->prof_n[1] = 0;
   ctx->ev_used = 0;
   return COLLOP_OK;
 }

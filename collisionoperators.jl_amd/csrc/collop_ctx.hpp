@@ -43,6 +43,10 @@ struct collop_ctx {
   double w_abs_sum = 0.0;  // sum |w| of the state (fixed-point scale of the scatter)
   DBuf s_v, s_w, s_v1, s_Gv, s_r, s_rp, s_Gp, s_best, s_dF, s_dG, s_fc, s_dotv;
   DBuf s_Gvo, s_Gk, s_FA, s_D;  // defect correction: G(v_k), G_eff(v_k), [F, A], A (G_eff - G_k)
+  DBuf s_Ge, s_vmk;             // G_eff and v_mid of the inner iterate / of v_k (near field)
+  DBuf s_xk, s_oU, s_oUp, s_of, s_ofp, s_odF, s_odG;  // outer Anderson window
+  DBuf b_near_list, b_near_count, b_ntg, b_noff, b_nal;  // near pairs (list, CSR by target)
+  int near_nt = 0;              // near targets of the current step
   double* h_stat = nullptr;  // pinned, kStatWidth doubles
   // ---- warm-start MLP
   DBuf m_params, m_adam_m, m_adam_v, m_data, m_rec, m_y, m_gpart, m_lpart, m_tiles, m_eval, m_grad;
@@ -58,8 +62,8 @@ struct collop_ctx {
   std::vector<std::pair<hipEvent_t, hipEvent_t>> ev_pool;
   size_t ev_used = 0;
   std::vector<int> ev_kind;
-  double prof_ms[2] = {0.0, 0.0};
-  long prof_n[2] = {0, 0};
+  double prof_ms[4] = {0.0, 0.0, 0.0, 0.0};  // FP64, FP32c, FP64 + metric, FP32c + metric
+  long prof_n[4] = {0, 0, 0, 0};
   // ---- batched ensemble (ensemble.hip)
   collop::EnsembleState* ens = nullptr;
 };

@@ -201,6 +201,28 @@ __device__ __forceinline__ void pair_term_f64(double vg1, double vg2, double Gg1
   t_out = t;
 }
 
+// FP32 pair term that also returns d and the weight factor t = w / |d|^3 (0 for
+// masked pairs), for the metric.
+__device__ __forceinline__ void pair_term_f32_t(float vg1, float vg2, float Gg1, float Gg2, float av1, float av2,
+                                                float ag1, float ag2, float aw, float& b1, float& b2, float& d1o,
+                                                float& d2o, float& t_out) {
+  const float d1 = vg1 - av1;
+  const float d2 = vg2 - av2;
+  const float q = __builtin_fmaf(d2, d2, d1 * d1);
+  const float y = __builtin_amdgcn_rsqf(__builtin_fmaxf(q, 1e-12f));
+  const float g1 = ag1 - Gg1;
+  const float g2 = ag2 - Gg2;
+  const float c = __builtin_fmaf(g1, d2, -(g2 * d1));
+  float t = (aw * y) * (y * y);
+  t = q < 1e-12f ? 0.0f : t;
+  const float kk = t * c;
+  b1 = __builtin_fmaf(kk, d2, b1);
+  b2 = __builtin_fmaf(-kk, d1, b2);
+  d1o = d1;
+  d2o = d2;
+  t_out = t;
+}
+
 // FP32 pair term with the FP32 coincidence guard of collision_gpu.jl:187.
 __device__ __forceinline__ void pair_term_f32(float vg1, float vg2, float Gg1, float Gg2, float av1, float av2,
                                               float ag1, float ag2, float aw, float& b1, float& b2) {

@@ -74,8 +74,24 @@ void launch_pack_src(const double* v, const double* G, const double* w, int N, d
 void launch_pair_f64(const Src64* src, int N, const PairTile& t, double* part, hipStream_t st);
 // part[S][2N] <- slice sums of the FP32 pair sum (compensated, written in FP64)
 void launch_pair_f32c(const Src32* src, int N, const PairTile& t, double* part, hipStream_t st);
-// part[S][5N] <- slice sums of F (2 comps) and A (3 comps)
-void launch_pair_metric_f64(const Src64* src, int N, const PairTile& t, double* part, hipStream_t st);
+// Near-pair detection during a metric pass: every ordered in-domain pair with
+// w_a / |d| > theta (th2 = theta^2) is appended to list[] as (gamma, alpha);
+// count receives the number found (entries beyond cap are dropped, the count
+// still grows).
+struct NearOut {
+  double th2;
+  int* count;
+  int2* list;
+  int cap;
+};
+constexpr int kNearCap = 1 << 16;  // near pairs kept per step (beyond it the near field is skipped)
+// part[S][5N] <- slice sums of F (2 comps) and A (3 comps); near != null also
+// collects the near pairs (one target per lane, one Newton step)
+void launch_pair_metric_f64(const Src64* src, int N, const PairTile& t, double* part, hipStream_t st,
+                            const NearOut* near = nullptr);
+// the same from FP32 pair terms (compensated sums, written in FP64)
+void launch_pair_metric_f32c(const Src32* src, int N, const PairTile& t, double* part, hipStream_t st,
+                             const NearOut* near = nullptr);
 // out[ncomp*N] <- sum over S slices in fixed order
 void launch_slice_sum(const double* part, int S, int64_t len, double* out, hipStream_t st);
 
@@ -158,6 +174,10 @@ void launch_epilogue(const EpiArgs& a, int N, hipStream_t st);
 // v1 = beta (Gv - sum_j dG_j gamma_j) + (1 - beta) v1
 void launch_update(double* v1, const double* Gv, const double* dG, const double* gam, int hist, double beta, int N,
                    hipStream_t st);
+// Near-field correction of the frozen-metric map (pairs in CSR order: targets
+// tg[nt], partners al[off[t] .. off[t+1])): D -= sum (exact - frozen) pair terms.
+void launch_near(int nt, const int* tg, const int* off, const int* al, int N, const double* w, const double* vm,
+                 const double* Ge, const double* vmk, const double* Gk, double* D, hipStream_t st);
 // out = v + dt * (sum over slices of part)  (the Euler predictor, main.jl:245)
 void launch_euler(const double* part, int S, const double* v, double dt, int N, double* out, double* dotv,
                   hipStream_t st);

@@ -26,9 +26,9 @@ namespace {
 
 constexpr int kThreads = 256;
 
-template <int K, int NEWTON, bool METRIC>
+template <int K, int NEWTON, bool METRIC, bool NEAR = false>
 __global__ __launch_bounds__(kThreads) void k_pair_f64(const Src64* __restrict__ src, int N, int S, int chunk,
-                                                        double* __restrict__ part) {
+                                                        double* __restrict__ part, NearOut near = NearOut{}) {
   const int bid = blockIdx.x;
   const int slice = bid % S;
   const int gblk = bid / S;
@@ -66,6 +66,14 @@ __global__ __launch_bounds__(kThreads) void k_pair_f64(const Src64* __restrict__
         const double td1 = t * d1;
         m22[k] = __builtin_fma(td1, d1, m22[k]);
       }
+      if (NEAR) {  // near pair: w_a / |d| > theta
+        const double q = __builtin_fma(d2, d2, d1 * d1);
+        if (act[k] && q >= 1e-24 && near.th2 * q < aw * aw) {
+          const int g = gblk * (kThreads * K) + k * kThreads + (int)threadIdx.x;
+          const int idx = atomicAdd(near.count, 1);
+          if (idx < near.cap) near.list[idx] = make_int2(g, a);
+        }
+      }
     }
   }
   const size_t stride = (size_t)(METRIC ? 5 : 2) * (size_t)N;
@@ -85,15 +93,17 @@ __global__ __launch_bounds__(kThreads) void k_pair_f64(const Src64* __restrict__
   }
 }
 
-template <int K, int BLK>
+template <int K, int BLK, bool METRIC, bool NEAR = false>
 __global__ __launch_bounds__(kThreads) void k_pair_f32c(const Src32* __restrict__ src, int N, int S, int chunk,
-                                                         double* __restrict__ part) {
+                                                         double* __restrict__ part, NearOut near = NearOut{}) {
   const int bid = blockIdx.x;
   const int slice = bid % S;
   const int gblk = bid / S;
   const int a0 = slice * chunk;
   const int a1 = min(N, a0 + chunk);
   float vg1[K], vg2[K], Gg1[K], Gg2[K], b1[K], b2[K], h1[K], h2[K], l1[K], l2[K];
+  // metric A = sum w U(d): block sums folded into FP32 (hi, lo) like the velocity sums
+  float c11[K], c12[K], c22[K], q11[K], q12[K], q22[K], e11[K], e12[K], e22[K];
   bool act[K];
 #pragma unroll
   for (int k = 0; k < K; ++k) {
@@ -109,6 +119,7 @@ __global__ __launch_bounds__(kThreads) void k_pair_f32c(const Src32* __restrict_
       act[k] = s.in_domain != 0.0f;
     }
     b1[k] = b2[k] = h1[k] = h2[k] = l1[k] = l2[k] = 0.0f;
+    c11[k] = c12[k] = c22[k] = q11[k] = q12[k] = q22[k] = e11[k] = e12[k] = e22[k] = 0.0f;
   }
   for (int ab = a0; ab < a1; ab += BLK) {
     const int ae = min(a1, ab + BLK);
@@ -116,22 +127,54 @@ __global__ __launch_bounds__(kThreads) void k_pair_f32c(const Src32* __restrict_
     for (int a = ab; a < ae; ++a) {
       const float av1 = src[a].v1, av2 = src[a].v2, ag1 = src[a].g1, ag2 = src[a].g2, aw = src[a].w;
 #pragma unroll
-      for (int k = 0; k < K; ++k) pair_term_f32(vg1[k], vg2[k], Gg1[k], Gg2[k], av1, av2, ag1, ag2, aw, b1[k], b2[k]);
+      for (int k = 0; k < K; ++k) {
+        if (METRIC) {
+          float d1, d2, t;
+          pair_term_f32_t(vg1[k], vg2[k], Gg1[k], Gg2[k], av1, av2, ag1, ag2, aw, b1[k], b2[k], d1, d2, t);
+          // A += w U(d) = w (I |d|^2 - d d^T) / |d|^3  (functions.jl:278-281)
+          const float td2 = t * d2, td1 = t * d1;
+          c11[k] = __builtin_fmaf(td2, d2, c11[k]);
+          c12[k] = __builtin_fmaf(-td2, d1, c12[k]);
+          c22[k] = __builtin_fmaf(td1, d1, c22[k]);
+          if (NEAR) {  // near pair: w_a / |d| > theta
+            const float q = __builtin_fmaf(d2, d2, d1 * d1);
+            if (act[k] && q >= 1e-12f && (float)near.th2 * q < aw * aw) {
+              const int g = gblk * (kThreads * K) + k * kThreads + (int)threadIdx.x;
+              const int idx = atomicAdd(near.count, 1);
+              if (idx < near.cap) near.list[idx] = make_int2(g, a);
+            }
+          }
+        } else {
+          pair_term_f32(vg1[k], vg2[k], Gg1[k], Gg2[k], av1, av2, ag1, ag2, aw, b1[k], b2[k]);
+        }
+      }
     }
 #pragma unroll
     for (int k = 0; k < K; ++k) {
       two_sum_fold(h1[k], l1[k], b1[k]);
       two_sum_fold(h2[k], l2[k], b2[k]);
       b1[k] = b2[k] = 0.0f;
+      if (METRIC) {
+        two_sum_fold(q11[k], e11[k], c11[k]);
+        two_sum_fold(q12[k], e12[k], c12[k]);
+        two_sum_fold(q22[k], e22[k], c22[k]);
+        c11[k] = c12[k] = c22[k] = 0.0f;
+      }
     }
   }
+  const size_t stride = (size_t)(METRIC ? 5 : 2) * (size_t)N;
 #pragma unroll
   for (int k = 0; k < K; ++k) {
     const int g = gblk * (kThreads * K) + k * kThreads + (int)threadIdx.x;
     if (g < N) {
-      double* p = part + (size_t)slice * 2 * (size_t)N;
+      double* p = part + (size_t)slice * stride;
       p[g] = act[k] ? (double)h1[k] + (double)l1[k] : 0.0;
       p[N + g] = act[k] ? (double)h2[k] + (double)l2[k] : 0.0;
+      if (METRIC) {
+        p[2 * (size_t)N + g] = act[k] ? (double)q11[k] + (double)e11[k] : 0.0;
+        p[3 * (size_t)N + g] = act[k] ? (double)q12[k] + (double)e12[k] : 0.0;
+        p[4 * (size_t)N + g] = act[k] ? (double)q22[k] + (double)e22[k] : 0.0;
+      }
     }
   }
 }
@@ -294,14 +337,27 @@ void launch_pair_f64(const Src64* src, int N, const PairTile& t, double* part, h
   }
 }
 
-void launch_pair_metric_f64(const Src64* src, int N, const PairTile& t, double* part, hipStream_t st) {
+void launch_pair_metric_f64(const Src64* src, int N, const PairTile& t, double* part, hipStream_t st,
+                            const NearOut* near) {
   if (N <= 0) return;
   const int chunk = (N + t.slices - 1) / t.slices;
   const dim3 grid(grid_for(N, t)), block(kThreads);
-  if (t.gamma_per_lane == 2)
-    hipLaunchKernelGGL((k_pair_f64<2, 2, true>), grid, block, 0, st, src, N, t.slices, chunk, part);
-  else
-    hipLaunchKernelGGL((k_pair_f64<1, 2, true>), grid, block, 0, st, src, N, t.slices, chunk, part);
+  if (near) {  // one target per lane, one Newton step
+    hipLaunchKernelGGL((k_pair_f64<1, 1, true, true>), dim3(grid_for(N, PairTile{1, t.slices, 1})), block, 0, st, src,
+                       N, t.slices, chunk, part, *near);
+    return;
+  }
+  if (t.gamma_per_lane == 2) {
+    if (t.newton == 1)
+      hipLaunchKernelGGL((k_pair_f64<2, 1, true>), grid, block, 0, st, src, N, t.slices, chunk, part);
+    else
+      hipLaunchKernelGGL((k_pair_f64<2, 2, true>), grid, block, 0, st, src, N, t.slices, chunk, part);
+  } else {
+    if (t.newton == 1)
+      hipLaunchKernelGGL((k_pair_f64<1, 1, true>), grid, block, 0, st, src, N, t.slices, chunk, part);
+    else
+      hipLaunchKernelGGL((k_pair_f64<1, 2, true>), grid, block, 0, st, src, N, t.slices, chunk, part);
+  }
 }
 
 void launch_pair_f32c(const Src32* src, int N, const PairTile& t, double* part, hipStream_t st) {
@@ -309,9 +365,25 @@ void launch_pair_f32c(const Src32* src, int N, const PairTile& t, double* part, 
   const int chunk = (N + t.slices - 1) / t.slices;
   const dim3 grid(grid_for(N, t)), block(kThreads);
   if (t.gamma_per_lane == 2)
-    hipLaunchKernelGGL((k_pair_f32c<2, 32>), grid, block, 0, st, src, N, t.slices, chunk, part);
+    hipLaunchKernelGGL((k_pair_f32c<2, 32, false>), grid, block, 0, st, src, N, t.slices, chunk, part);
   else
-    hipLaunchKernelGGL((k_pair_f32c<1, 32>), grid, block, 0, st, src, N, t.slices, chunk, part);
+    hipLaunchKernelGGL((k_pair_f32c<1, 32, false>), grid, block, 0, st, src, N, t.slices, chunk, part);
+}
+
+void launch_pair_metric_f32c(const Src32* src, int N, const PairTile& t, double* part, hipStream_t st,
+                             const NearOut* near) {
+  if (N <= 0) return;
+  const int chunk = (N + t.slices - 1) / t.slices;
+  const dim3 grid(grid_for(N, t)), block(kThreads);
+  if (near) {
+    hipLaunchKernelGGL((k_pair_f32c<1, 32, true, true>), dim3(grid_for(N, PairTile{1, t.slices, 0})), block, 0, st,
+                       src, N, t.slices, chunk, part, *near);
+    return;
+  }
+  if (t.gamma_per_lane == 2)
+    hipLaunchKernelGGL((k_pair_f32c<2, 32, true>), grid, block, 0, st, src, N, t.slices, chunk, part);
+  else
+    hipLaunchKernelGGL((k_pair_f32c<1, 32, true>), grid, block, 0, st, src, N, t.slices, chunk, part);
 }
 
 void launch_slice_sum(const double* part, int S, int64_t len, double* out, hipStream_t st) {

@@ -159,6 +159,40 @@ __device__ __forceinline__ void euler_body(const double* __restrict__ part, int 
   out[e] = v[e] + dt * s;
 }
 
+// ---- near field of the frozen-metric map ---------------------------------------
+// w_a U(d) g, U(d) g = (g - d (d.g) / |d|^2) / |d|  (functions.jl:221-231 as written)
+__device__ __forceinline__ void uterm(double wa, double d1, double d2, double g1, double g2, double& o1, double& o2) {
+  const double q = d1 * d1 + d2 * d2;
+  const double dg = (d1 * g1 + d2 * g2) / q;
+  const double inv = 1.0 / sqrt(q);
+  o1 = wa * (g1 - d1 * dg) * inv;
+  o2 = wa * (g2 - d2 * dg) * inv;
+}
+
+// For near target tg[t] with partners al[off[t] .. off[t+1]): replace each
+// pair's frozen-metric model w U(d_k) (G_k,a - G_g(u)) by its exact term
+// w U(d(u)) (G_a(u) - G_g(u)) at the inner iterate: D_g -= sum (exact - frozen),
+// so Phi = G(v_k) - dt D gains + dt (exact - frozen).  Pairs in fixed order.
+__global__ void k_near(int nt, const int* __restrict__ tg, const int* __restrict__ off, const int* __restrict__ al,
+                       int N, const double* __restrict__ w, const double* __restrict__ vm,
+                       const double* __restrict__ Ge, const double* __restrict__ vmk, const double* __restrict__ Gk,
+                       double* __restrict__ D) {
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= nt) return;
+  const int g = tg[t];
+  double c1 = 0.0, c2 = 0.0;
+  for (int p = off[t]; p < off[t + 1]; ++p) {
+    const int a = al[p];
+    double e1, e2, f1, f2;
+    uterm(w[a], vm[g] - vm[a], vm[N + g] - vm[N + a], Ge[a] - Ge[g], Ge[N + a] - Ge[N + g], e1, e2);
+    uterm(w[a], vmk[g] - vmk[a], vmk[N + g] - vmk[N + a], Gk[a] - Ge[g], Gk[N + a] - Ge[N + g], f1, f2);
+    c1 += e1 - f1;
+    c2 += e2 - f2;
+  }
+  D[g] -= c1;
+  D[N + g] -= c2;
+}
+
 // ---- single-run kernels ---------------------------------------------------------
 __global__ __launch_bounds__(kEwThreads) void k_geff_pack(const double* __restrict__ vmid,
                                                           const double* __restrict__ dS,
@@ -307,6 +341,12 @@ void launch_update(double* v1, const double* Gv, const double* dG, const double*
   for (int j = 0; j < hist && j < kMaxWindow; ++j) g.g[j] = gam[j];
   const int n2 = 2 * N;
   hipLaunchKernelGGL(k_update, dim3((n2 + 255) / 256), dim3(256), 0, st, v1, Gv, dG, g, hist, beta, n2);
+}
+
+void launch_near(int nt, const int* tg, const int* off, const int* al, int N, const double* w, const double* vm,
+                 const double* Ge, const double* vmk, const double* Gk, double* D, hipStream_t st) {
+  if (nt <= 0) return;
+  hipLaunchKernelGGL(k_near, dim3((nt + 127) / 128), dim3(128), 0, st, nt, tg, off, al, N, w, vm, Ge, vmk, Gk, D);
 }
 
 void launch_euler(const double* part, int S, const double* v, double dt, int N, double* out, double* dotv,

@@ -80,6 +80,11 @@ int collop_collision_f32c(collop_ctx* ctx, const double* v, const double* w, con
  * compute_collision_metric_gpu! (collision_gpu.jl:377-402). */
 int collop_collision_metric_f64(collop_ctx* ctx, const double* v, const double* w, const double* G, double* dot_v,
                                 double* A, int64_t N);
+/* The same from FP32 pair terms with compensated sums (no counterpart in the
+ * reference, whose metric kernel is FP64 only): the outer maps of the
+ * FP32-compensated defect-correction solve. */
+int collop_collision_metric_f32c(collop_ctx* ctx, const double* v, const double* w, const double* G, double* dot_v,
+                                 double* A, int64_t N);
 
 /* L2PROJ_FN, scatter only: rhs_k = sum_alpha w_alpha phi_k(v_alpha) over
  * in-domain markers (functions.jl:121-131; _l2_scatter_kernel!,
@@ -166,6 +171,13 @@ typedef struct {
   int32_t dc_max_inner;     /* DEFECT: inner maps per outer iteration (60)      */
   int32_t dc_stag_window;   /* DEFECT: outer iterations between stagnation
                                checks (5)                                       */
+  int32_t dc_outer_m;       /* DEFECT extension: Anderson window (<= 8) on the
+                               outer iteration v -> (inner solution); 0 = the
+                               reference's plain outer iteration                */
+  double dc_near_theta;     /* DEFECT extension: pairs with w_a / |d| > theta at
+                               the step's first outer iterate are evaluated
+                               exactly inside the frozen-metric map (0 = none);
+                               the fixed point is unchanged                     */
 } collop_step_params;
 
 typedef struct {
@@ -257,9 +269,9 @@ int collop_ensemble_tune(collop_ctx* ctx, int* gamma_tile, int* waves_per_cu, do
  * source slices (0 = automatic), newton in {1, 2} FP64 rsqrt refinements. */
 int collop_set_pair_tile(collop_ctx* ctx, int gamma_per_lane, int slices, int newton);
 /* HIP-event timing of every pair-kernel launch on the context's stream.
- * Returns the totals accumulated since the previous call (ms_total[2],
- * launches[2]; index 0 = FP64, 1 = FP32c), then resets them and sets the
- * enable flag. */
+ * Returns the totals accumulated since the previous call (ms_total[4],
+ * launches[4]; index 0 = FP64, 1 = FP32c, 2 = FP64 with the metric A,
+ * 3 = FP32c with the metric), then resets them and sets the enable flag. */
 int collop_profile(collop_ctx* ctx, int enable, double* ms_total, long* launches);
 
 #ifdef __cplusplus

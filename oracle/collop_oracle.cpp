@@ -291,6 +291,8 @@ struct orc_solver_params {
   double dc_eta;       // Parameters.jl:89-91
   int dc_max_inner;
   int dc_stag_window;
+  int dc_outer_m;      // extension: Anderson window of the outer iteration (0 = plain outer iteration)
+  double dc_near_theta;  // extension: pairs with w_a / |d| > theta are exact in the frozen map (0 = none)
 };
 
 struct orc_step_stats {
@@ -1084,6 +1086,19 @@ void orc_step_defect(const orc_mesh* ms, const orc_solver_params* Pp, double* v1
   double nrm0 = 0.0, nrm_best = std::numeric_limits<double>::infinity();
   double nrm_best_window = std::numeric_limits<double>::infinity();
   int inner = 0;
+  // near pairs (gamma, alpha) of the current outer iterate, sorted by gamma, and
+  // the midpoints at v_k
+  std::vector<int64_t> nI, nJ;
+  std::vector<double> vmk(n2);
+  const double lo1 = box[0], hi1 = box[1], lo2 = box[2], hi2 = box[3];
+  // w_a U(d) g with U(d) g = (g - d (d.g) / |d|^2) / |d|
+  auto uterm = [](double wa, double d1, double d2, double g1, double g2, double* o) {
+    const double q = d1 * d1 + d2 * d2;
+    const double dg = (d1 * g1 + d2 * g2) / q;
+    const double inv = 1.0 / std::sqrt(q);
+    o[0] = wa * (g1 - d1 * dg) * inv;
+    o[1] = wa * (g2 - d2 * dg) * inv;
+  };
   auto frozen = [&](double* out, const double* u) {
     landau_geff(ms, Ge.data(), u, v0, w, N, S0, s_in, P.use_gonzalez, P.use_logsq);
     for (int64_t a = 0; a < N; ++a) {
@@ -1091,7 +1106,24 @@ void orc_step_defect(const orc_mesh* ms, const orc_solver_params* Pp, double* v1
       out[a] = Gv[a] - P.dt * (A[a] * d1 + A[N + a] * d2);
       out[N + a] = Gv[N + a] - P.dt * (A[N + a] * d1 + A[2 * N + a] * d2);
     }
+    // near field: the pair's exact term at u replaces its frozen-metric model
+    // w U(d_k) (G_k,alpha - G_gamma(u)); both agree at u = v_k, so the fixed
+    // point is unchanged.
+    for (size_t p = 0; p < nI.size(); ++p) {
+      const int64_t g = nI[p], a = nJ[p];
+      double ex[2], fr[2];
+      uterm(w[a], s_in.vmid[g] - s_in.vmid[a], s_in.vmid[N + g] - s_in.vmid[N + a], Ge[a] - Ge[g],
+            Ge[N + a] - Ge[N + g], ex);
+      uterm(w[a], vmk[g] - vmk[a], vmk[N + g] - vmk[N + a], Gk[a] - Ge[g], Gk[N + a] - Ge[N + g], fr);
+      out[g] += P.dt * (ex[0] - fr[0]);
+      out[N + g] += P.dt * (ex[1] - fr[1]);
+    }
   };
+  // outer Anderson window (extension): x_k, f_k = T(x_k) - x_k
+  const int mo = P.dc_outer_m;
+  std::vector<double> xk(n2), U(n2), Up(n2), f(n2), fp(n2), dFo((size_t)std::max(mo, 1) * n2),
+      dGo((size_t)std::max(mo, 1) * n2);
+  int ohist = 0, oslot = 0;
   for (int k = 1; k <= P.max_iter; ++k) {
     landau_geff(ms, s.Geff.data(), v1, v0, w, N, S0, s, P.use_gonzalez, P.use_logsq);
     orc_collision_metric(box, s.vmid.data(), w, s.Geff.data(), N, s.dotv.data(), A.data(), P.nthreads);
@@ -1120,8 +1152,65 @@ void orc_step_defect(const orc_mesh* ms, const orc_solver_params* Pp, double* v1
       nrm_best_window = nrm_best;
     }
     std::memcpy(Gk.data(), s.Geff.data(), sizeof(double) * n2);
+    std::memcpy(vmk.data(), s.vmid.data(), sizeof(double) * n2);
+    if (k == 1 && P.dc_near_theta > 0.0) {  // the step's near pairs: those of its first outer iterate
+      const double th2 = P.dc_near_theta * P.dc_near_theta;
+      for (int64_t g = 0; g < N; ++g) {
+        if (!in_box(vmk[g], vmk[N + g], lo1, hi1, lo2, hi2)) continue;
+        for (int64_t a = 0; a < N; ++a) {
+          if (a == g || !in_box(vmk[a], vmk[N + a], lo1, hi1, lo2, hi2)) continue;
+          const double d1 = vmk[g] - vmk[a], d2 = vmk[N + g] - vmk[N + a];
+          const double q = d1 * d1 + d2 * d2;
+          if (q >= 1e-24 && th2 * q < w[a] * w[a]) {
+            nI.push_back(g);
+            nJ.push_back(a);
+          }
+        }
+      }
+    }
+    std::memcpy(xk.data(), v1, sizeof(double) * n2);
     inner += anderson_inner(frozen, v1, n2, std::max(P.dc_eta * nr, 0.5 * P.abs_floor), P.dc_max_inner,
                             P.m_anderson, P.damping, P.reg_factor);
+    if (mo > 0) {  // Anderson on x -> T(x) (the inner solution), beta = 1
+      std::memcpy(U.data(), v1, sizeof(double) * n2);
+      for (int64_t i = 0; i < n2; ++i) f[i] = U[i] - xk[i];
+      if (k > 1 || ohist > 0) {
+        oslot = oslot % mo + 1;
+        ohist = std::min(ohist + 1, mo);
+        double* F = &dFo[(size_t)(oslot - 1) * n2];
+        double* Gc = &dGo[(size_t)(oslot - 1) * n2];
+        for (int64_t i = 0; i < n2; ++i) {
+          F[i] = f[i] - fp[i];
+          Gc[i] = U[i] - Up[i];
+        }
+        const int h = ohist;
+        std::vector<double> ATA(h * h), ATr(h), gam(h);
+        for (int a2 = 0; a2 < h; ++a2) {
+          const double* Fa = &dFo[(size_t)a2 * n2];
+          for (int b2 = 0; b2 <= a2; ++b2) {
+            const double* Fb = &dFo[(size_t)b2 * n2];
+            double t = 0.0;
+            for (int64_t i = 0; i < n2; ++i) t += Fa[i] * Fb[i];
+            ATA[a2 * h + b2] = ATA[b2 * h + a2] = t;
+          }
+          double t = 0.0;
+          for (int64_t i = 0; i < n2; ++i) t += Fa[i] * f[i];
+          ATr[a2] = t;
+        }
+        double dm = 0.0;
+        for (int j = 0; j < h; ++j) dm += ATA[j * h + j];
+        const double l2 = P.reg_factor * dm / h + 1e-30;
+        for (int j = 0; j < h; ++j) ATA[j * h + j] += l2;
+        small_solve(h, ATA, ATr, gam.data());
+        for (int64_t i = 0; i < n2; ++i) {
+          double x = U[i];
+          for (int j = 0; j < h; ++j) x -= dGo[(size_t)j * n2 + i] * gam[j];
+          v1[i] = x;
+        }
+      }
+      std::memcpy(Up.data(), U.data(), sizeof(double) * n2);
+      std::memcpy(fp.data(), f.data(), sizeof(double) * n2);
+    }
   }
   std::memcpy(v1, vbest.data(), sizeof(double) * n2);
   *st = {P.max_iter, 0, nrm_best, nrm0, 2, 0, 0, 0, 0, inner};

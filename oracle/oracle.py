@@ -49,6 +49,8 @@ class SolverParams(C.Structure):
         ("dc_eta", C.c_double),
         ("dc_max_inner", C.c_int),
         ("dc_stag_window", C.c_int),
+        ("dc_outer_m", C.c_int),
+        ("dc_near_theta", C.c_double),
     ]
 
 
@@ -75,6 +77,7 @@ def default_params(dt: float, m: int = 5, nthreads: int = 0, **kw) -> SolverPara
         damp_decay_factor=0.5, restart_factor=float("inf"), damping=0.7,
         reg_factor=1e-10, use_anderson=1, use_gonzalez=1, exit_picard_step=1,
         use_logsq=0, nthreads=nthreads, solver=0, dc_eta=0.05, dc_max_inner=60, dc_stag_window=5,
+        dc_outer_m=0, dc_near_theta=0.0,
     )
     for k, v in kw.items():
         setattr(p, k, v)

@@ -16,27 +16,29 @@ def _oracle_residual(m, P, v_new, v, w, S0):
     return float(np.linalg.norm(m.picard_map(P, v_new, v, w, S0) - v_new))
 
 
+@pytest.mark.parametrize("ext", [(0, 0.0), (3, 0.04)], ids=["reference", "outer_aa+near"])
 @pytest.mark.parametrize("ic,N,ne,dt,strict", [("gaussian", 1024, 16, 0.005, True),
                                                ("bimodal", 2048, 32, 0.02, True),
                                                ("gaussian", 1024, 16, 0.05, False)])
-def test_defect_step_matches_oracle(gpu_ctx, ic, N, ne, dt, strict):
+def test_defect_step_matches_oracle(gpu_ctx, ic, N, ne, dt, strict, ext):
     sig = (1.0, 1.0) if ic == "gaussian" else (np.sqrt(0.5), np.sqrt(0.5))
     v, w = collop.markers(ic, N, 42, sigma1=sig[0], sigma2=sig[1], peak=2.0)
     bp = uniform_bp(-6, 6, ne)
     m = O.Mesh(bp, bp)
     fc = m.l2_project(v, w)
     S0 = m.entropy(fc)
-    P = O.default_params(dt, m=5, nthreads=0, solver=1)
+    P = O.default_params(dt, m=5, nthreads=0, solver=1, dc_outer_m=ext[0], dc_near_theta=ext[1])
     v_ref, fc_ref, st_ref = m.time_step(P, v, w, fc, S0)
 
     ws = collop.Workspace(bp, bp, ctx=gpu_ctx)
     sim = collop.Simulation(ws, v, w)
-    prm = collop.config("gaussian_n4096", DT=dt, solver="defect").step_params()
+    prm = collop.config("gaussian_n4096", DT=dt, solver="defect", dc_outer_m=ext[0],
+                        dc_near_theta=ext[1]).step_params()
     st = sim.run(prm, 1)[0]
     v_gpu = sim.markers()
     res = _oracle_residual(m, P, v_gpu, v, w, S0)
     tol = max(1e-12 * np.linalg.norm(v_gpu), 1e-10)
-    print(f"{ic} N={N} dt={dt}: oracle outer {st_ref.iters} inner {st_ref.inner} exit {st_ref.exit_reason}; "
+    print(f"{ic} N={N} dt={dt} ext={ext}: oracle outer {st_ref.iters} inner {st_ref.inner} exit {st_ref.exit_reason}; "
           f"gpu outer {st['iters']} inner {st['inner_maps']} exit {st['exit_reason']}; "
           f"max|dv| {np.max(np.abs(v_gpu - v_ref)):.2e}; oracle-map residual {res:.2e} (tol {tol:.2e})")
     assert st["exit_reason"] == 0 and st_ref.exit_reason == 0
@@ -69,8 +71,9 @@ def test_defect_and_anderson_reach_the_same_root(gpu_ctx):
     assert np.max(np.abs(va - vb)) <= 1e-9
 
 
-def test_defect_conservation_bimodal_n65536(gpu_ctx):
-    p = collop.config("bimodal_n65536", solver="defect")
+@pytest.mark.parametrize("ext", [(0, 0.0), (3, 0.04)], ids=["reference", "outer_aa+near"])
+def test_defect_conservation_bimodal_n65536(gpu_ctx, ext):
+    p = collop.config("bimodal_n65536", solver="defect", dc_outer_m=ext[0], dc_near_theta=ext[1])
     v, w = collop.initial_markers(p)
     ws = collop.build_workspace(p, ctx=gpu_ctx)
     sim = collop.Simulation(ws, v, w)
@@ -97,7 +100,30 @@ def test_defect_param_errors(gpu_ctx):
     with pytest.raises(collop.CollopError) as ei:
         sim.run(bad, 1)
     assert ei.value.status == collop._lib.ERR_INVALID
+    bad = collop.config("gaussian_n4096", solver="defect", dc_outer_m=9).step_params()
+    with pytest.raises(collop.CollopError):
+        sim.run(bad, 1)
     bad = collop.config("gaussian_n4096").step_params()
     bad.solver = 7
     with pytest.raises(collop.CollopError):
         sim.run(bad, 1)
+
+
+@pytest.mark.parametrize("dt", [0.005, 0.02])
+def test_defect_f32c_outer_maps_reach_the_fp64_root(gpu_ctx, dt):
+    """FP32-compensated outer maps (F and A) until the FP32 floor, FP64 outer maps
+    to finish: the same root as the all-FP64 defect solve."""
+    p = collop.config("bimodal_n65536", N_PARTICLES=8192, DT=dt, solver="defect", dc_outer_m=3, dc_near_theta=0.04)
+    v, w = collop.initial_markers(p)
+    ws = collop.build_workspace(p, ctx=gpu_ctx)
+    a = collop.Simulation(ws, v, w)
+    sa = a.run(p.step_params(precision=collop.PREC_F64), 1)[0]
+    va = a.markers()
+    b = collop.Simulation(ws, v, w)
+    sb = b.run(p.step_params(precision=collop.PREC_F32C), 1)[0]
+    vb = b.markers()
+    print(f"dt={dt}: fp64 outer {sa['iters']} inner {sa['inner_maps']}; f32c outer {sb['iters']} "
+          f"(f32 {sb['iters_f32']}) inner {sb['inner_maps']}; max|dv| {np.max(np.abs(va - vb)):.2e}")
+    assert sa["exit_reason"] == 0 and sb["exit_reason"] == 0
+    assert sb["iters_f32"] >= 1
+    assert np.max(np.abs(va - vb)) <= 1e-9

@@ -143,3 +143,22 @@ def test_defect_correction_finds_the_anderson_root():
     r = np.linalg.norm(m.picard_map(P, vd, v, w, S0) - vd)
     assert r <= 20 * max(1e-12 * np.linalg.norm(vd), 1e-10)
     assert np.max(np.abs(vd - va)) <= 1e-10
+
+
+def test_defect_extensions_keep_the_root():
+    """Outer Anderson and the exact near field change the iteration, not the
+    fixed point: the extended solve lands on the reference solve's root."""
+    v, w = collop.markers("bimodal", 1024, 5, sigma1=0.5 ** 0.5, sigma2=0.5 ** 0.5, peak=2.0)
+    bp = uniform_bp(-6, 6, 16)
+    m = O.Mesh(bp, bp)
+    fc = m.l2_project(v, w)
+    S0 = m.entropy(fc)
+    P = O.default_params(0.005, m=5, solver=1)
+    v1, _ = m.predictor(P, v, w, fc)
+    va, sa, _ = m.step_defect(P, v1, v, w, S0)
+    P.dc_outer_m, P.dc_near_theta = 3, 0.5   # theta large enough to catch pairs at N = 1024
+    vb, sb, _ = m.step_defect(P, v1, v, w, S0)
+    assert sa.exit_reason == 0 and sb.exit_reason == 0
+    assert np.max(np.abs(va - vb)) <= 1e-10
+    r = np.linalg.norm(m.picard_map(P, vb, v, w, S0) - vb)
+    assert r <= 20 * max(1e-12 * np.linalg.norm(vb), 1e-10)
