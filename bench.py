@@ -71,6 +71,8 @@ def run_path(ws, p, v, w, precision, warmup, steps, solver=None):
     import collop
     sim = collop.Simulation(ws, v, w)
     prm = p.step_params(precision=precision, solver=solver)
+    log(f"  knobs: solver {prm.solver} precision {prm.precision} dc_outer_m {prm.dc_outer_m} "
+        f"dc_near_theta {prm.dc_near_theta} dc_metric_reuse {prm.dc_metric_reuse} dc_eta {prm.dc_eta}")
     E0 = 0.5 * float(np.sum(w * (v[:, 0] ** 2 + v[:, 1] ** 2)))
     for i in range(warmup):
         st = sim.run(prm, 1)[0]
@@ -339,7 +341,8 @@ def main():
     r64 = run_path(ws, p, v, w, collop.PREC_F64, args.warmup, args.steps)
     log(f"[bench] plain FP64: {r64['steps_per_s']:.3f} steps/s, mean iters {r64['mean_iters']:.1f}")
     log(f"[bench] chosen path (FP32-compensated maps + FP64 finish, {args.chosen_solver} solver)")
-    rch = run_path(ws, p, v, w, collop.PREC_F32C, args.warmup, args.steps,
+    pch = collop.chosen_config(args.config) if args.chosen_solver == "defect" else p
+    rch = run_path(ws, pch, v, w, collop.PREC_F32C, args.warmup, args.steps,
                    solver=collop._lib.SOLVER_DEFECT if args.chosen_solver == "defect" else collop._lib.SOLVER_ANDERSON)
     log(f"[bench] chosen: {rch['steps_per_s']:.3f} steps/s, mean iters {rch['mean_iters']:.1f} "
         f"(f32 {rch['mean_iters_f32']:.1f})")
@@ -396,7 +399,8 @@ def main():
         "fp64_steps_per_s": round(r64["steps_per_s"], 4),
         "ratio_vs_fp64": round(ratio, 4),
         "chosen_path": ("FP32c outer maps (F and metric A) then FP64 outer maps, frozen-metric defect correction "
-                        "(step_defect!), Euler predictor" if args.chosen_solver == "defect" else
+                        "(step_defect!) with an outer Anderson window of 3 and exact near pairs (w/|d| > 0.02), "
+                        "FP32c Euler predictor" if args.chosen_solver == "defect" else
                         "FP32c maps then FP64 maps, Anderson (step_anderson!), Euler predictor"),
         "plain_path": "FP64 maps, Anderson (step_anderson!), Euler predictor",
         "f32c_anderson": (None if rva is None else

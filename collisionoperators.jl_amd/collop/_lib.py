@@ -71,6 +71,7 @@ class StepParams(C.Structure):
         ("dc_stag_window", C.c_int32),
         ("dc_outer_m", C.c_int32),
         ("dc_near_theta", C.c_double),
+        ("dc_metric_reuse", C.c_int32),
     ]
 
 

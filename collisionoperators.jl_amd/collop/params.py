@@ -58,6 +58,7 @@ class SimParameters:
     dc_stag_window: int = 5
     dc_outer_m: int = 0              # extension: outer Anderson window (0 = the reference's outer iteration)
     dc_near_theta: float = 0.0       # extension: exact near pairs in the frozen map (0 = none)
+    dc_metric_reuse: int = 0         # extension: 0 A at every outer map, 1 FP64 maps reuse the FP32c A, 2 first only
 
     def step_params(self, precision: int | None = None, warmstart: int | None = None,
                     solver: int | None = None) -> _lib.StepParams:
@@ -76,7 +77,7 @@ class SimParameters:
             exit_picard_step=int(self.exit_picard_step), use_logsq=int(self.use_logsq),
             precision=prec, warmstart=warm, f32_floor_factor=16.0, f32_keep_window=1, record_training=0,
             solver=solver, dc_eta=self.dc_eta, dc_max_inner=self.dc_max_inner, dc_stag_window=self.dc_stag_window,
-            dc_outer_m=self.dc_outer_m, dc_near_theta=self.dc_near_theta)
+            dc_outer_m=self.dc_outer_m, dc_near_theta=self.dc_near_theta, dc_metric_reuse=self.dc_metric_reuse)
 
 
 def parse_overrides(p: SimParameters, args: list[str]) -> SimParameters:
@@ -124,6 +125,20 @@ CONFIGS = {
                                  weight_jitter=0.2, bp1=_uniform(-8, 8, 64), bp2=_uniform(-8, 8, 64), DT=0.01,
                                  N_STEPS=20, seed=42),
 }
+
+
+# The chosen path: FP32-compensated outer maps then FP64 outer maps, frozen-metric
+# defect correction (step_defect!) with the two extensions of this library (an
+# outer Anderson window of 3 and exact near pairs with w / |d| > 0.02), the
+# FP64 outer maps reusing the metric A of the last FP32c outer map.
+CHOSEN = dict(gpu_fp32=True, solver="defect", dc_outer_m=3, dc_near_theta=0.02, dc_metric_reuse=1)
+
+
+def chosen_config(name: str, **over) -> SimParameters:
+    """config(name) with the chosen path's solver knobs (CHOSEN), then `over`."""
+    kw = dict(CHOSEN)
+    kw.update(over)
+    return config(name, **kw)
 
 
 def config(name: str, **over) -> SimParameters:

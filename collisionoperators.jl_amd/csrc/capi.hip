@@ -70,6 +70,37 @@ void cell_basis(const double* C, int cell, double xi, double phi[3]) {
 
 // 1D mass matrix of one direction, exact with a 3-point Gauss rule per cell,
 // and its banded Cholesky factor stored as {1/L_ii, L_i,i-1, L_i,i-2}.
+// Dense inverse of a 1D mass matrix from its banded Cholesky factor
+// (chol = {1/L_ii, L_i,i-1, L_i,i-2}): column j solves M x = e_j; the result is
+// symmetrised.  The 1D mass matrices are well conditioned, so applying the
+// inverse costs no accuracy against the banded solve.
+void mass_inverse_1d(const std::vector<double>& chol, int nd, std::vector<double>& inv) {
+  inv.assign((size_t)nd * nd, 0.0);
+  std::vector<double> x(nd);
+  for (int j = 0; j < nd; ++j) {
+    std::fill(x.begin(), x.end(), 0.0);
+    x[j] = 1.0;
+    for (int i = 0; i < nd; ++i) {  // L y = e_j
+      double t = x[i];
+      if (i >= 1) t -= chol[3 * i + 1] * x[i - 1];
+      if (i >= 2) t -= chol[3 * i + 2] * x[i - 2];
+      x[i] = t * chol[3 * i];
+    }
+    for (int i = nd - 1; i >= 0; --i) {  // L^T x = y
+      double t = x[i];
+      if (i + 1 < nd) t -= chol[3 * (i + 1) + 1] * x[i + 1];
+      if (i + 2 < nd) t -= chol[3 * (i + 2) + 2] * x[i + 2];
+      x[i] = t * chol[3 * i];
+    }
+    for (int i = 0; i < nd; ++i) inv[(size_t)i * nd + j] = x[i];
+  }
+  for (int i = 0; i < nd; ++i)
+    for (int j = 0; j < i; ++j) {
+      const double a = 0.5 * (inv[(size_t)i * nd + j] + inv[(size_t)j * nd + i]);
+      inv[(size_t)i * nd + j] = inv[(size_t)j * nd + i] = a;
+    }
+}
+
 bool mass_factor_1d(const std::vector<double>& bp, const double* C, const int* bs0, int nd, std::vector<double>& out) {
   const int ne = (int)bp.size() - 1;
   std::vector<double> gx, gw;
@@ -324,6 +355,9 @@ int collop_set_mesh(collop_ctx* ctx, const double* bp1, int nb1, const double* b
   std::vector<double> ch1, ch2;
   if (!mass_factor_1d(v1, C1, b1.data(), nd1, ch1) || !mass_factor_1d(v2, C2, b2.data(), nd2, ch2))
     return fail(ctx, COLLOP_ERR_INVALID, "1D mass matrix is not positive definite (check the extraction tables)");
+  std::vector<double> mi1, mi2;
+  mass_inverse_1d(ch1, nd1, mi1);
+  mass_inverse_1d(ch2, nd2, mi2);
   std::vector<double> qx, qw;
   gauss_legendre(n_quad, qx, qw);
   std::vector<double> ph1((size_t)ne1 * 3 * n_quad), ph2((size_t)ne2 * 3 * n_quad);
@@ -374,6 +408,8 @@ int collop_set_mesh(collop_ctx* ctx, const double* bp1, int nb1, const double* b
   UP(bs2, b2, int)
   UP(chol1, ch1, double)
   UP(chol2, ch2, double)
+  UP(minv1, mi1, double)
+  UP(minv2, mi2, double)
   UP(phiq1, ph1, double)
   UP(phiq2, ph2, double)
   UP(qw, qw, double)
@@ -752,12 +788,19 @@ int picard_map_dev(collop_ctx* ctx, const collop_step_params& p, int kind, const
 // one pair pass (compute_collision_metric!, functions.jl:249-290), FP64 (kind 0)
 // or FP32-compensated (kind 1); keeps G_eff(v1) in s_Gk, [F, A] in s_FA and
 // G(v1) in s_Gvo.
-int outer_map_dev(collop_ctx* ctx, const collop_step_params& p, int kind, bool find_near) {
+int outer_map_dev(collop_ctx* ctx, const collop_step_params& p, int kind, bool find_near, bool metric) {
   const int n = (int)ctx->N;
   int r = geff_dev(ctx, p, P<double>(ctx->s_v1), kind == 0 ? P<Src64>(ctx->b_s64) : nullptr,
                    kind == 1 ? P<Src32>(ctx->b_s32) : nullptr, P<double>(ctx->s_Gk), nullptr, nullptr, nullptr);
   if (r) return r;
   const PairTile& t = kind == 0 ? ctx->tile64 : ctx->tile32;
+  if (!metric) {  // the plain pair sum; A of an earlier outer map stays in s_FA + 2N
+    if ((r = run_pair(ctx, kind, n, P<double>(ctx->b_part)))) return r;
+    launch_slice_sum(P<double>(ctx->b_part), t.slices, (int64_t)2 * n, P<double>(ctx->s_FA), ctx->stream);
+    CHECK_LAUNCH();
+    return epilogue_dev(ctx, P<double>(ctx->s_FA), 1, P<double>(ctx->s_v), p.dt, P<double>(ctx->s_v1),
+                        P<double>(ctx->s_Gvo), -1, 0);
+  }
   NearOut near{p.dc_near_theta * p.dc_near_theta, P<int>(ctx->b_near_count), P<int2>(ctx->b_near_list), kNearCap};
   if (find_near) HIPC(hipMemsetAsync(ctx->b_near_count.p, 0, sizeof(int), ctx->stream));
   prof_begin(ctx, 2 + kind);
@@ -884,8 +927,10 @@ int validate_params(collop_ctx* ctx, const collop_step_params* p) {
       (!(p->dc_eta > 0.0 && p->dc_eta < 1.0) || p->dc_max_inner < 1 || p->dc_stag_window < 1))
     return fail(ctx, COLLOP_ERR_INVALID, "defect correction needs 0 < dc_eta < 1, dc_max_inner >= 1, dc_stag_window >= 1");
   if (p->solver == COLLOP_SOLVER_DEFECT &&
-      (p->dc_outer_m < 0 || p->dc_outer_m > kMaxWindow || !(p->dc_near_theta >= 0.0) || !std::isfinite(p->dc_near_theta)))
-    return fail(ctx, COLLOP_ERR_INVALID, "defect correction needs 0 <= dc_outer_m <= 8 and dc_near_theta >= 0");
+      (p->dc_outer_m < 0 || p->dc_outer_m > kMaxWindow || !(p->dc_near_theta >= 0.0) ||
+       !std::isfinite(p->dc_near_theta) || p->dc_metric_reuse < 0 || p->dc_metric_reuse > 2))
+    return fail(ctx, COLLOP_ERR_INVALID,
+                "defect correction needs 0 <= dc_outer_m <= 8, dc_near_theta >= 0 and dc_metric_reuse in {0, 1, 2}");
   if (p->record_training && ctx->m_size + ctx->N > ctx->m_cap)
     return fail(ctx, COLLOP_ERR_INVALID, "training dataset full (collop_mlp_dataset_alloc)");
   return COLLOP_OK;
@@ -922,7 +967,10 @@ int defect_solve(collop_ctx* ctx, const collop_step_params& p, collop_step_stats
   if ((r = copy_d2d(ctx, ctx->s_best, ctx->s_v1, n2b))) return r;
   for (int k = 1; k <= p.max_iter; ++k) {
     const bool find_near = k == 1 && p.dc_near_theta > 0.0;
-    if ((r = outer_map_dev(ctx, p, use32 ? 1 : 0, find_near))) return r;
+    // the metric A: every outer map (mode 0); FP64 maps after an FP32c phase
+    // reuse the last FP32c A (mode 1); the step's first outer map only (mode 2)
+    const bool metric = k == 1 || p.dc_metric_reuse == 0 || (p.dc_metric_reuse == 1 && (use32 || iters32 == 0));
+    if ((r = outer_map_dev(ctx, p, use32 ? 1 : 0, find_near, metric))) return r;
     if (find_near && (r = build_near(ctx))) return r;
     const double nr = std::sqrt(ctx->h_stat[0]), nv = std::sqrt(ctx->h_stat[1]);
     if (!std::isfinite(nr) || !std::isfinite(nv))
@@ -1082,13 +1130,15 @@ int step_once(collop_ctx* ctx, const collop_step_params& p, collop_step_stats* o
   int r;
   // ---- predictor: v1 = v + dt vdot(v, G), G = grad(M^{-1} r(f_s(v)))
   if ((r = entropy_seed_gradient(ctx, P<double>(ctx->s_v), P<double>(ctx->s_fc), p.use_logsq))) return r;
+  // (the predictor is only the initial guess: F32C runs take its pair sum in FP32c)
   const MeshView& m = ctx->mv;
+  const int pk = p.precision == COLLOP_PREC_F32C ? 1 : 0;
   launch_pack_src(P<double>(ctx->s_v), P<double>(ctx->b_G), P<double>(ctx->s_w), n, m.lo1, m.hi1, m.lo2, m.hi2,
-                  P<Src64>(ctx->b_s64), nullptr, ctx->stream);
+                  pk == 0 ? P<Src64>(ctx->b_s64) : nullptr, pk == 1 ? P<Src32>(ctx->b_s32) : nullptr, ctx->stream);
   CHECK_LAUNCH();
-  if ((r = run_pair(ctx, 0, n, P<double>(ctx->b_part)))) return r;
-  launch_euler(P<double>(ctx->b_part), ctx->tile64.slices, P<double>(ctx->s_v), p.dt, n, P<double>(ctx->s_v1),
-               P<double>(ctx->s_dotv), ctx->stream);
+  if ((r = run_pair(ctx, pk, n, P<double>(ctx->b_part)))) return r;
+  launch_euler(P<double>(ctx->b_part), pk == 0 ? ctx->tile64.slices : ctx->tile32.slices, P<double>(ctx->s_v), p.dt,
+               n, P<double>(ctx->s_v1), P<double>(ctx->s_dotv), ctx->stream);
   CHECK_LAUNCH();
   // ---- warm-start hook (main.jl:249-254): features of v, optional record,
   // optional MLP correction of the predictor

@@ -31,6 +31,8 @@ struct MeshView {
   const int* bs2;
   const double* chol1;   // 3 * nd1: {1 / L_ii, L_i,i-1, L_i,i-2}
   const double* chol2;
+  const double* minv1;   // nd1 x nd1 dense inverse of the 1D mass matrix (symmetric)
+  const double* minv2;
   const double* phiq1;   // ne1 * 3 * nq: phi_j(x_a) on cell e at [(3 e + j) nq + a]
   const double* phiq2;
   const double* qw;      // nq Gauss weights on [0,1]
@@ -111,6 +113,33 @@ __device__ __forceinline__ double block_sum(double x, double* sh) {
     for (int i = 0; i < NT / 64; ++i) t += sh[i];
   }
   return t;
+}
+
+// Deterministic block sums of NV values at once: one shuffle tree per value,
+// one pair of barriers for all of them; the sums are valid in thread 0 and are
+// bitwise those of NV separate block_sum calls.  sh holds (NT / 64) * NV doubles.
+template <int NT, int NV>
+__device__ __forceinline__ void block_sum_n(double (&x)[NV], double* sh) {
+#pragma unroll
+  for (int v = 0; v < NV; ++v)
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) x[v] += __shfl_xor(x[v], off, 64);
+  const int wid = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  __syncthreads();
+  if (lane == 0) {
+#pragma unroll
+    for (int v = 0; v < NV; ++v) sh[wid * NV + v] = x[v];
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+#pragma unroll
+    for (int v = 0; v < NV; ++v) {
+      double t = 0.0;
+#pragma unroll
+      for (int i = 0; i < NT / 64; ++i) t += sh[i * NV + v];
+      x[v] = t;
+    }
+  }
 }
 
 // ---- marker location against LDS breakpoints (fem.hip, ensemble.hip) ----

@@ -186,11 +186,25 @@ int ens_predict(collop_ctx* ctx, EnsembleState* E, const collop_step_params& p) 
   CHECK_LAUNCH();
   launch_gather_b(m, V, P<double>(E->v), P<double>(E->vec), 3, nullptr, nullptr, 0, P<double>(E->G), nullptr, st);
   CHECK_LAUNCH();
-  launch_pack_b(V, P<double>(E->v), P<double>(E->G), P<double>(E->w), m.lo1, m.hi1, m.lo2, m.hi2, P<Src64>(E->s64),
-                nullptr, st);
-  CHECK_LAUNCH();
-  launch_pair_b(V, tiles(E), P<Src64>(E->s64), nullptr, 1, P<double>(E->part), st);
-  CHECK_LAUNCH();
+  if (p.precision == COLLOP_PREC_F32C) {  // as the single-run step: an FP32c predictor
+    for (int i = 0; i < E->R; ++i) {
+      std::memset(&E->h_ctl[i], 0, sizeof(RunCtl));
+      E->h_ctl[i].active = 1;
+      E->h_ctl[i].kind = 1;
+    }
+    HIPC(hipMemcpyAsync(E->d_ctl.p, E->h_ctl, (size_t)E->R * sizeof(RunCtl), hipMemcpyHostToDevice, st));
+    launch_pack_b(V, P<double>(E->v), P<double>(E->G), P<double>(E->w), m.lo1, m.hi1, m.lo2, m.hi2, nullptr,
+                  P<Src32>(E->s32), st);
+    CHECK_LAUNCH();
+    launch_pair_b(view(E, true), tiles(E), nullptr, P<Src32>(E->s32), 0, P<double>(E->part), st);
+    CHECK_LAUNCH();
+  } else {
+    launch_pack_b(V, P<double>(E->v), P<double>(E->G), P<double>(E->w), m.lo1, m.hi1, m.lo2, m.hi2, P<Src64>(E->s64),
+                  nullptr, st);
+    CHECK_LAUNCH();
+    launch_pair_b(V, tiles(E), P<Src64>(E->s64), nullptr, 1, P<double>(E->part), st);
+    CHECK_LAUNCH();
+  }
   launch_euler_b(V, P<double>(E->part), P<double>(E->v), p.dt, P<double>(E->v1), nullptr, st);
   CHECK_LAUNCH();
   launch_copy_b(V, P<double>(E->v1), P<double>(E->best), st);

@@ -25,7 +25,6 @@
 namespace collop {
 namespace {
 
-constexpr int kSolveThreads = 1024;
 
 // ---------------------------------------------------------------------------
 // Kernel bodies.  Each body serves the single-run kernel and the batched
@@ -103,29 +102,57 @@ __device__ __forceinline__ void scatter_body(const MeshView& m, unsigned long lo
 // Exact sum of the fixed-point partial rows of one field, back to FP64.
 __device__ __forceinline__ double rhs_of(const long long* __restrict__ P, int nblocks, int nd, int k,
                                          double inv_scale) {
-  unsigned long long s = 0ull;  // two's-complement wrap-around is exact
+  unsigned long long s = 0ull;  // two's-complement wrap-around is exact, in any order
+#pragma unroll 8
   for (int b = 0; b < nblocks; ++b) s += (unsigned long long)P[(size_t)b * nd + k];
   return (double)(long long)s * inv_scale;
 }
 
-// rhs (exact row sum, or `direct`) -> M^{-1} rhs in LDS -> out.  One workgroup.
+// rhs (exact row sum, or `direct`) -> M^{-1} rhs -> out.  One workgroup of
+// kSolveThreads.  M = M1 (x) M2, so the coefficient array (i1 fastest) is
+// C = M1^{-1} B M2^{-1}: two small dense products with the precomputed
+// symmetric 1D inverses, phase one held in registers across a barrier so B
+// and M1^{-1} B share one LDS array of n_dofs doubles.
+constexpr int kSolveThreads = 1024;
+constexpr int kSolvePer = 8;  // n_dofs <= kSolvePer * kSolveThreads
+
 __device__ __forceinline__ void solve_body(const MeshView& m, double* X, const long long* __restrict__ P, int nblocks,
                                            double inv_scale, const double* __restrict__ direct,
                                            double* __restrict__ out) {
-  const int nd = m.n_dofs;
+  const int nd = m.n_dofs, nd1 = m.nd1, nd2 = m.nd2;
   if (P) {
     for (int k = threadIdx.x; k < nd; k += blockDim.x) X[k] = rhs_of(P, nblocks, nd, k, inv_scale);
   } else {
     for (int k = threadIdx.x; k < nd; k += blockDim.x) X[k] = direct[k];
   }
   __syncthreads();
-  // direction 1: one line per i2 (contiguous along i1)
-  for (int l = threadIdx.x; l < m.nd2; l += blockDim.x) band_solve_line(X + (size_t)l * m.nd1, m.nd1, 1, m.chol1);
+  double t[kSolvePer];
+#pragma unroll
+  for (int q = 0; q < kSolvePer; ++q) {
+    const int k = threadIdx.x + q * kSolveThreads;
+    t[q] = 0.0;
+    if (k < nd) {
+      const int i1 = k % nd1, i2 = k - (k % nd1);
+      const double* row = m.minv1 + (size_t)i1 * nd1;
+      double s = 0.0;
+      for (int j = 0; j < nd1; ++j) s += row[j] * X[i2 + j];
+      t[q] = s;
+    }
+  }
   __syncthreads();
-  // direction 2: one line per i1 (stride nd1)
-  for (int l = threadIdx.x; l < m.nd1; l += blockDim.x) band_solve_line(X + l, m.nd2, m.nd1, m.chol2);
+#pragma unroll
+  for (int q = 0; q < kSolvePer; ++q) {
+    const int k = threadIdx.x + q * kSolveThreads;
+    if (k < nd) X[k] = t[q];
+  }
   __syncthreads();
-  for (int k = threadIdx.x; k < nd; k += blockDim.x) out[k] = X[k];
+  for (int k = threadIdx.x; k < nd; k += blockDim.x) {
+    const int i1 = k % nd1, i2 = k / nd1;
+    const double* col = m.minv2 + (size_t)i2 * nd2;
+    double s = 0.0;
+    for (int j = 0; j < nd2; ++j) s += X[i1 + (size_t)nd1 * j] * col[j];
+    out[k] = s;
+  }
 }
 
 // Quadrature pass 1 of one block (one thread per cell and Gauss point).
@@ -171,28 +198,29 @@ __device__ __forceinline__ void quad1_body(const MeshView& m, double* red, const
   }
 }
 
-// Quadrature pass 2 for dof k: gather over the cells of its support.
-__device__ __forceinline__ void quad2_body(const MeshView& m, const double* __restrict__ iota, double* __restrict__ r,
-                                           int k) {
-  if (k >= m.n_dofs) return;
+// Quadrature pass 2 for dof k, one wave: the lanes stride over the (cell,
+// point) pairs of the dof's support (at most 3 x 3 cells x nq^2 points), then
+// a fixed shuffle tree.  Lane 0 writes r[k].
+__device__ __forceinline__ void quad2_wave(const MeshView& m, const double* __restrict__ iota, double* __restrict__ r,
+                                           int k, int lane) {
+  if (k >= m.n_dofs) return;  // wave-uniform
   const int i1 = k % m.nd1, i2 = k / m.nd1;
   const int npt = m.nq * m.nq;
+  const int e1lo = m.erange1[2 * i1], n1 = m.erange1[2 * i1 + 1] - e1lo + 1;
+  const int e2lo = m.erange2[2 * i2], n2 = m.erange2[2 * i2 + 1] - e2lo + 1;
+  const int total = n1 * n2 * npt;
   double acc = 0.0;
-  for (int e2 = m.erange2[2 * i2]; e2 <= m.erange2[2 * i2 + 1]; ++e2) {
-    const int j2 = i2 - m.bs2[e2];
-    const double* p2 = m.phiq2 + (3 * e2 + j2) * m.nq;
-    for (int e1 = m.erange1[2 * i1]; e1 <= m.erange1[2 * i1 + 1]; ++e1) {
-      const int j1 = i1 - m.bs1[e1];
-      const double* p1 = m.phiq1 + (3 * e1 + j1) * m.nq;
-      const double* io = iota + (size_t)(e1 + e2 * m.ne1) * npt;
-      for (int b = 0; b < m.nq; ++b) {
-        double row = 0.0;
-        for (int a = 0; a < m.nq; ++a) row += p1[a] * io[a + m.nq * b];
-        acc += row * p2[b];
-      }
-    }
+  for (int idx = lane; idx < total; idx += 64) {
+    const int cell = idx / npt, pt = idx - cell * npt;
+    const int e2 = e2lo + cell / n1, e1 = e1lo + cell % n1;
+    const int a = pt % m.nq, b = pt / m.nq;
+    const double p1 = m.phiq1[(3 * e1 + i1 - m.bs1[e1]) * m.nq + a];
+    const double p2 = m.phiq2[(3 * e2 + i2 - m.bs2[e2]) * m.nq + b];
+    acc += p1 * iota[(size_t)(e1 + e2 * m.ne1) * npt + pt] * p2;
   }
-  r[k] = acc;
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) acc += __shfl_xor(acc, off, 64);
+  if (lane == 0) r[k] = acc;
 }
 
 // sum_{j < n} base[j * jstride]: fixed strided partition over threads, then the
@@ -313,8 +341,8 @@ __global__ __launch_bounds__(kQuadThreads) void k_quad1(MeshView m, const double
              spart ? spart + (size_t)field * gridDim.x + blockIdx.x : nullptr, blockIdx.x);
 }
 
-__global__ void k_quad2(MeshView m, const double* __restrict__ iota, double* __restrict__ r) {
-  quad2_body(m, iota, r, blockIdx.x * blockDim.x + threadIdx.x);
+__global__ __launch_bounds__(256) void k_quad2(MeshView m, const double* __restrict__ iota, double* __restrict__ r) {
+  quad2_wave(m, iota, r, blockIdx.x * 4 + (threadIdx.x >> 6), threadIdx.x & 63);
 }
 
 // rhs without the solve (collop_l2_rhs)
@@ -399,12 +427,14 @@ __global__ __launch_bounds__(kQuadThreads) void k_quad1_b(MeshView m, EnsView E,
              spart ? spart + ((size_t)r * 2 + field) * gridDim.x + blockIdx.x : nullptr, blockIdx.x);
 }
 
-// grid = (dof blocks, R): r_out = vec[(r * 4 + 2) * n_dofs]
-__global__ void k_quad2_b(MeshView m, EnsView E, const double* __restrict__ iota, double* __restrict__ vec) {
+// grid = (dof blocks of 4, R): r_out = vec[(r * 4 + 2) * n_dofs]
+__global__ __launch_bounds__(256) void k_quad2_b(MeshView m, EnsView E, const double* __restrict__ iota,
+                                                 double* __restrict__ vec) {
   const int r = blockIdx.y;
   if (ens_skip(E, r)) return;
   const size_t npts = (size_t)m.ne1 * m.ne2 * m.nq * m.nq;
-  quad2_body(m, iota + r * npts, vec + ((size_t)r * 4 + 2) * m.n_dofs, blockIdx.x * blockDim.x + threadIdx.x);
+  quad2_wave(m, iota + r * npts, vec + ((size_t)r * 4 + 2) * m.n_dofs, blockIdx.x * 4 + (threadIdx.x >> 6),
+             threadIdx.x & 63);
 }
 
 // grid = (R, count): out[r * ostride + c] = sum_{j < cnt[r]} part[row0[r] + c * istride + j * jstride]
@@ -476,7 +506,7 @@ void launch_quad1(const MeshView& m, const double* c0, const double* c1, int fie
 }
 
 void launch_quad2(const MeshView& m, const double* iota, double* r, hipStream_t st) {
-  hipLaunchKernelGGL(k_quad2, dim3((m.n_dofs + 127) / 128), dim3(128), 0, st, m, iota, r);
+  hipLaunchKernelGGL(k_quad2, dim3((m.n_dofs + 3) / 4), dim3(256), 0, st, m, iota, r);
 }
 
 void launch_sum_rows(const double* part, int n, int count, int istride, int jstride, double* out, hipStream_t st) {
@@ -517,7 +547,7 @@ void launch_quad1_b(const MeshView& m, const EnsView& E, const double* c, int rs
 }
 
 void launch_quad2_b(const MeshView& m, const EnsView& E, const double* iota, double* vec, hipStream_t st) {
-  hipLaunchKernelGGL(k_quad2_b, dim3((m.n_dofs + 127) / 128, E.R), dim3(128), 0, st, m, E, iota, vec);
+  hipLaunchKernelGGL(k_quad2_b, dim3((m.n_dofs + 3) / 4, E.R), dim3(256), 0, st, m, E, iota, vec);
 }
 
 void launch_sum_rows_b(const EnsView& E, const double* part, const int* row0, const int* cnt, int count, int istride,

@@ -22,20 +22,21 @@ __device__ __forceinline__ void geff_body(const double* __restrict__ vmid, const
                                           const double* __restrict__ dpart, int nparts, const double* __restrict__ S1,
                                           const double* __restrict__ S0, int use_gonzalez, double lo1, double hi1,
                                           double lo2, double hi2, Src64* __restrict__ s64, Src32* __restrict__ s32,
-                                          double* __restrict__ lambda_out, double* lam_sh, int lb,
+                                          double* __restrict__ lambda_out, double* lam_sh, double* red, int lb,
                                           double* __restrict__ geff_out = nullptr,
                                           const double* __restrict__ Gk = nullptr,
                                           const double* __restrict__ A = nullptr, double* __restrict__ D = nullptr) {
-  if (threadIdx.x == 0) {
-    double lam = 0.0;
-    if (use_gonzalez) {
-      double dot = 0.0, nrm2 = 0.0;
-      for (int b = 0; b < nparts; ++b) {
-        dot += dpart[2 * b];
-        nrm2 += dpart[2 * b + 1];
-      }
-      lam = nrm2 > 1e-30 ? (S1[0] - S0[0] - dot) / nrm2 : 0.0;
+  // the dot products over the gather partials, in a fixed partition and tree
+  double sums[2] = {0.0, 0.0};
+  if (use_gonzalez)
+    for (int b = threadIdx.x; b < nparts; b += blockDim.x) {
+      sums[0] += dpart[2 * b];
+      sums[1] += dpart[2 * b + 1];
     }
+  block_sum_n<kEwThreads, 2>(sums, red);
+  if (threadIdx.x == 0) {
+    const double dot = sums[0], nrm2 = sums[1];
+    const double lam = (use_gonzalez && nrm2 > 1e-30) ? (S1[0] - S0[0] - dot) / nrm2 : 0.0;
     *lam_sh = lam;
     if (lb == 0 && lambda_out) lambda_out[0] = lam;
   }
@@ -124,13 +125,10 @@ __device__ __forceinline__ void epilogue_body(const EpiArgs& A, int N, int ld, s
       }
     }
   }
+  block_sum_n<kEwThreads, kStatWidth>(st, red);
+  if (threadIdx.x == 0) {
 #pragma unroll
-  for (int j = 0; j < kStatWidth; ++j) {
-    const bool used = j < 3 || (A.col >= 0 && j >= 3 && j < 3 + A.hist) ||
-                      (j >= 3 + kMaxWindow && j < 3 + kMaxWindow + A.hist);
-    if (!used) continue;  // uniform across the block
-    const double t = block_sum<kEwThreads>(st[j], red);
-    if (threadIdx.x == 0) A.stat_part[j] = t;
+    for (int j = 0; j < kStatWidth; ++j) A.stat_part[j] = st[j];
   }
 }
 
@@ -208,12 +206,13 @@ __global__ __launch_bounds__(kEwThreads) void k_geff_pack(const double* __restri
                                                           const double* __restrict__ Gk,
                                                           const double* __restrict__ A, double* __restrict__ D) {
   __shared__ double lam_sh;
+  __shared__ double red[2 * kEwThreads / 64];
   geff_body(vmid, dS, dv, w, N, N, dpart, nparts, S1, S0, use_gonzalez, lo1, hi1, lo2, hi2, s64, s32, lambda_out,
-            &lam_sh, blockIdx.x, geff_out, Gk, A, D);
+            &lam_sh, red, blockIdx.x, geff_out, Gk, A, D);
 }
 
 __global__ __launch_bounds__(kEwThreads) void k_epilogue(EpiArgs A, int N) {
-  __shared__ double red[kEwThreads / 64];
+  __shared__ double red[kStatWidth * kEwThreads / 64];
   EpiArgs a = A;
   a.stat_part = A.stat_part + (size_t)blockIdx.x * kStatWidth;
   epilogue_body(a, N, N, (size_t)2 * N, red, blockIdx.x);
@@ -245,11 +244,12 @@ __global__ __launch_bounds__(kEwThreads) void k_geff_pack_b(EnsView E, const dou
   const RunCtl& c = E.ctl[t.x];
   if (!c.active) return;
   __shared__ double lam_sh;
+  __shared__ double red[2 * kEwThreads / 64];
   const EnsRun& R = E.runs[t.x];
   double* sc = scal + 8 * t.x;
   geff_body(vmid + R.off, dS + R.off, dv + R.off, w + R.off, R.n, E.NT, dpart + 2 * R.gblk0, R.ngb, sc + 1, sc + 0,
             use_gonzalez, lo1, hi1, lo2, hi2, c.kind == 0 ? s64 + R.off : nullptr, c.kind == 1 ? s32 + R.off : nullptr,
-            sc + 2, &lam_sh, t.y);
+            sc + 2, &lam_sh, red, t.y);
 }
 
 
@@ -257,7 +257,7 @@ __global__ __launch_bounds__(kEwThreads) void k_epilogue_b(EnsView E, EnsBufs B,
   const int2 t = E.eblk[blockIdx.x];
   const RunCtl& c = E.ctl[t.x];
   if (!c.active) return;
-  __shared__ double red[kEwThreads / 64];
+  __shared__ double red[kStatWidth * kEwThreads / 64];
   const EnsRun& R = E.runs[t.x];
   EpiArgs a;
   a.part = B.part + R.poff;

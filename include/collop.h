@@ -178,6 +178,11 @@ typedef struct {
                                the step's first outer iterate are evaluated
                                exactly inside the frozen-metric map (0 = none);
                                the fixed point is unchanged                     */
+  int32_t dc_metric_reuse;  /* DEFECT extension: which outer maps compute the
+                               metric A (the others run the plain pair sum and
+                               keep the last A): 0 = every one (the reference),
+                               1 = all but the FP64 maps after an FP32c phase,
+                               2 = the step's first only                        */
 } collop_step_params;
 
 typedef struct {

@@ -111,9 +111,10 @@ def test_defect_param_errors(gpu_ctx):
 
 @pytest.mark.parametrize("dt", [0.005, 0.02])
 def test_defect_f32c_outer_maps_reach_the_fp64_root(gpu_ctx, dt):
-    """FP32-compensated outer maps (F and A) until the FP32 floor, FP64 outer maps
-    to finish: the same root as the all-FP64 defect solve."""
-    p = collop.config("bimodal_n65536", N_PARTICLES=8192, DT=dt, solver="defect", dc_outer_m=3, dc_near_theta=0.04)
+    """The chosen path (FP32-compensated outer maps until the FP32 floor, FP64
+    outer maps reusing the FP32c metric, outer Anderson, near field) lands on
+    the root of the all-FP64 solve."""
+    p = collop.chosen_config("bimodal_n65536", N_PARTICLES=8192, DT=dt)
     v, w = collop.initial_markers(p)
     ws = collop.build_workspace(p, ctx=gpu_ctx)
     a = collop.Simulation(ws, v, w)
