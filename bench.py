@@ -67,12 +67,13 @@ def log(msg: str):
     print(msg, file=sys.stderr, flush=True)
 
 
-def run_path(ws, p, v, w, precision, warmup, steps, solver=None):
+def run_path(ws, p, v, w, precision, warmup, steps, solver=None, warmstart=None):
     import collop
     sim = collop.Simulation(ws, v, w)
-    prm = p.step_params(precision=precision, solver=solver)
+    prm = p.step_params(precision=precision, solver=solver, warmstart=warmstart)
     log(f"  knobs: solver {prm.solver} precision {prm.precision} dc_outer_m {prm.dc_outer_m} "
-        f"dc_near_theta {prm.dc_near_theta} dc_metric_reuse {prm.dc_metric_reuse} dc_eta {prm.dc_eta}")
+        f"dc_near_theta {prm.dc_near_theta} dc_metric_reuse {prm.dc_metric_reuse} dc_eta {prm.dc_eta} "
+        f"warmstart {prm.warmstart}")
     E0 = 0.5 * float(np.sum(w * (v[:, 0] ** 2 + v[:, 1] ** 2)))
     for i in range(warmup):
         st = sim.run(prm, 1)[0]
@@ -104,6 +105,7 @@ def run_path(ws, p, v, w, precision, warmup, steps, solver=None):
                 mean_iters=float(np.mean([s["iters"] for s in stats])),
                 mean_iters_f32=float(np.mean([s["iters_f32"] for s in stats])),
                 mean_inner=float(np.mean([s["inner_maps"] for s in stats])),
+                mean_r0=float(np.mean([s["r0"] for s in stats])),
                 all_converged=all(s["exit_reason"] == 0 for s in stats), markers=sim.markers())
 
 
@@ -242,6 +244,53 @@ def kernel_probe(args):
     log(f"[probe] pair kernels x3 (FP64, FP32c, FP64 + metric, FP32c + metric) at N={N}; copy {nbytes} bytes")
 
 
+def warmstart_report(args):
+    """Learned warm start: record 8 seeded N = 65536 runs (the three ICs in turn,
+    32 x 32 mesh, dt 0.02) for `--train-steps` steps each, train the MLP with
+    Adam (lr 1e-3) on the device, then step a held-out seed `--steps` times cold
+    (Euler predictor) and warm (Euler + MLP), with the plain Anderson solver and
+    with the chosen path, to the same tolerance."""
+    import collop
+    from collop import warmstart as W
+    p = collop.config("bimodal_n65536")
+    ws = collop.build_workspace(p, device=0)
+    runs = W.training_runs(8, 65536, seed0=100)
+    rec_params = collop.chosen_config("bimodal_n65536").step_params()
+    t0 = time.perf_counter()
+    n = W.record(ws, runs, args.train_steps, rec_params, log=log)
+    t_rec = time.perf_counter() - t0
+    log(f"[warmstart] recorded {n} samples in {t_rec:.1f} s")
+    ml = W.WarmStart(ws)
+    ml.set_params(W.glorot_init(0))
+    mu, sd, os_ = ml.fit_norm()
+    t0 = time.perf_counter()
+    losses = ml.train(args.epochs, batch=65536, lr=1e-3, seed=1)
+    t_train = time.perf_counter() - t0
+    log(f"[warmstart] {args.epochs} epochs in {t_train:.1f} s, loss per epoch {losses}")
+    if args.save_model:
+        ml.save(args.save_model)
+    v, w = collop.markers("bimodal", 65536, 999, sigma1=0.5 ** 0.5, sigma2=0.5 ** 0.5, peak=2.0)  # held out
+    out = {"report": "warmstart", "train_samples": n, "record_s": t_rec, "train_s": t_train,
+           "epochs": args.epochs, "loss_per_epoch": losses, "held_out": "bimodal seed 999", "steps": args.steps,
+           "norm": {"mu": mu.tolist(), "sd": sd.tolist(), "out_scale": os_.tolist()}}
+    for name, base in (("anderson_fp64", collop.config("bimodal_n65536")),
+                       ("chosen", collop.chosen_config("bimodal_n65536"))):
+        res = {}
+        for warm in (collop.WARM_EULER, collop.WARM_MLP):
+            prm = base.step_params(warmstart=warm)
+            r = W.evaluate(ws, v, w, prm, args.steps)
+            res["mlp" if warm else "euler"] = r
+            log(f"  {name} {'mlp' if warm else 'euler'}: maps {r['mean_maps']:.2f} inner {r['mean_inner']:.1f} "
+                f"r0 {r['mean_r0']:.3e} {r['steps_per_s']:.2f} steps/s")
+        a, b = res["euler"], res["mlp"]
+        out[name] = {k: {kk: vv for kk, vv in d.items() if kk != "markers"} for k, d in res.items()}
+        out[name]["maps_ratio_mlp_vs_euler"] = b["mean_maps"] / a["mean_maps"]
+        out[name]["r0_ratio_mlp_vs_euler"] = b["mean_r0"] / a["mean_r0"]
+        out[name]["steps_per_s_ratio"] = b["steps_per_s"] / a["steps_per_s"]
+        out[name]["max_abs_dv_mlp_vs_euler"] = float(np.max(np.abs(b["markers"] - a["markers"])))
+    print(json.dumps(out), flush=True)
+
+
 def ensemble_bench(args):
     """Batched ensemble: 48 runs, N log-uniform in [2048, 65536], random IC,
     shared 32x32 mesh, FP32c pair sums (FP64 finish), dt 0.02.  Times the
@@ -303,13 +352,17 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--chosen-solver", choices=["anderson", "defect"], default="defect",
                     help="solver of the chosen path (the plain FP64 path always runs Anderson)")
+    ap.add_argument("--no-mlp", action="store_true", help="chosen path without the learned warm start")
     ap.add_argument("--no-variants", action="store_true",
                     help="skip the FP32c + Anderson comparison path")
     ap.add_argument("--config", default="bimodal_n65536",
                     help="bimodal_n65536 (headline), gaussian_n4096, bimaxwellian_n262144 or ensemble48")
     ap.add_argument("--kernel-probe", action="store_true",
                     help="run only the pair kernels and a 256 MiB copy (rocprofv3 counter runs)")
-    ap.add_argument("--report", choices=["precision"], default=None,
+    ap.add_argument("--train-steps", type=int, default=200, help="warmstart report: recorded steps per run")
+    ap.add_argument("--epochs", type=int, default=3, help="warmstart report: Adam epochs")
+    ap.add_argument("--save-model", default=None, help="warmstart report: write the trained model (.npz)")
+    ap.add_argument("--report", choices=["precision", "warmstart"], default=None,
                     help="precision: FP32c vs FP64 field errors and drift (use --config bimaxwellian_n262144)")
     args = ap.parse_args()
     if args.gpus != 1:
@@ -319,6 +372,9 @@ def main():
 
     if args.report == "precision":
         precision_report(args)
+        return
+    if args.report == "warmstart":
+        warmstart_report(args)
         return
     if args.kernel_probe:
         kernel_probe(args)
@@ -342,16 +398,26 @@ def main():
     log(f"[bench] plain FP64: {r64['steps_per_s']:.3f} steps/s, mean iters {r64['mean_iters']:.1f}")
     log(f"[bench] chosen path (FP32-compensated maps + FP64 finish, {args.chosen_solver} solver)")
     pch = collop.chosen_config(args.config) if args.chosen_solver == "defect" else p
-    rch = run_path(ws, pch, v, w, collop.PREC_F32C, args.warmup, args.steps,
-                   solver=collop._lib.SOLVER_DEFECT if args.chosen_solver == "defect" else collop._lib.SOLVER_ANDERSON)
+    chosen_solver = collop._lib.SOLVER_DEFECT if args.chosen_solver == "defect" else collop._lib.SOLVER_ANDERSON
+    warm = collop.WARM_EULER
+    if not args.no_mlp:
+        from collop import warmstart as W
+        W.load_packaged(ws)  # the learned warm start (trained by --report warmstart on other seeds)
+        warm = collop.WARM_MLP
+    rch = run_path(ws, pch, v, w, collop.PREC_F32C, args.warmup, args.steps, solver=chosen_solver, warmstart=warm)
     log(f"[bench] chosen: {rch['steps_per_s']:.3f} steps/s, mean iters {rch['mean_iters']:.1f} "
         f"(f32 {rch['mean_iters_f32']:.1f})")
 
-    rva = None
+    rva = rve = None
     if not args.no_variants and args.chosen_solver == "defect":
         log("[bench] comparison path: FP32c maps + FP64 finish with the Anderson solver")
         rva = run_path(ws, p, v, w, collop.PREC_F32C, args.warmup, args.steps, solver=collop._lib.SOLVER_ANDERSON)
         log(f"[bench] FP32c + Anderson: {rva['steps_per_s']:.3f} steps/s, mean iters {rva['mean_iters']:.1f}")
+        if warm == collop.WARM_MLP:
+            log("[bench] comparison path: the chosen path with the Euler predictor only")
+            rve = run_path(ws, pch, v, w, collop.PREC_F32C, args.warmup, args.steps, solver=chosen_solver,
+                           warmstart=collop.WARM_EULER)
+            log(f"[bench] chosen, Euler predictor: {rve['steps_per_s']:.3f} steps/s, mean iters {rve['mean_iters']:.1f}")
 
     # roofline of the dominant kernel in the chosen path's timed region
     pairs = n_in * (n_in - 1)
@@ -400,15 +466,22 @@ def main():
         "ratio_vs_fp64": round(ratio, 4),
         "chosen_path": ("FP32c outer maps (F and metric A) then FP64 outer maps, frozen-metric defect correction "
                         "(step_defect!) with an outer Anderson window of 3 and exact near pairs (w/|d| > 0.02), "
-                        "FP32c Euler predictor" if args.chosen_solver == "defect" else
+                        "FP32c Euler predictor" + (" + learned MLP warm start" if warm == collop.WARM_MLP else "")
+                        if args.chosen_solver == "defect" else
                         "FP32c maps then FP64 maps, Anderson (step_anderson!), Euler predictor"),
         "plain_path": "FP64 maps, Anderson (step_anderson!), Euler predictor",
+        "chosen_euler_predictor": (None if rve is None else
+                                   {"steps_per_s": round(rve["steps_per_s"], 4),
+                                    "ratio_vs_fp64": round(rve["steps_per_s"] / r64["steps_per_s"], 4),
+                                    "mean_iters": rve["mean_iters"], "mean_inner": rve["mean_inner"],
+                                    "mean_r0": rve["mean_r0"], "all_converged": rve["all_converged"]}),
         "f32c_anderson": (None if rva is None else
                           {"steps_per_s": round(rva["steps_per_s"], 4),
                            "ratio_vs_fp64": round(rva["steps_per_s"] / r64["steps_per_s"], 4),
                            "mean_iters": rva["mean_iters"], "all_converged": rva["all_converged"]}),
         "mean_iters": {"fp64": r64["mean_iters"], "chosen": rch["mean_iters"], "chosen_f32": rch["mean_iters_f32"],
                        "chosen_inner": rch["mean_inner"]},
+        "mean_r0": {"fp64": r64["mean_r0"], "chosen": rch["mean_r0"]},
         "conservation": {"fp64": {"max_dE_rel": r64["max_dE_rel"], "max_dP": r64["max_dP"], "min_dS": r64["min_dS"]},
                          "chosen": {"max_dE_rel": rch["max_dE_rel"], "max_dP": rch["max_dP"],
                                     "min_dS": rch["min_dS"]}},

@@ -62,6 +62,7 @@ class SimParameters:
 
     def step_params(self, precision: int | None = None, warmstart: int | None = None,
                     solver: int | None = None) -> _lib.StepParams:
+        """The C step parameters; precision / warmstart / solver override the fields."""
         prec = precision if precision is not None else (_lib.PREC_F32C if self.gpu_fp32 else _lib.PREC_F64)
         warm = warmstart if warmstart is not None else (_lib.WARM_MLP if self.warmstart == "nn" else _lib.WARM_EULER)
         if solver is None:

@@ -116,3 +116,62 @@ class WarmStart:
         d = np.load(path, allow_pickle=False)
         self.set_params(d["params"])
         self.set_norm(d["mu"], d["sd"], d["out_scale"])
+
+
+def training_runs(n_runs: int = 8, N: int = 65536, seed0: int = 100):
+    """The seeded training runs: the isotropic, bimodal (+-2, cov 0.5 I) and
+    bi-Maxwellian (T_x = 2, T_y = 0.5) ICs in turn, equal weights."""
+    from .ic import ENSEMBLE_ICS, markers
+    out = []
+    for r in range(n_runs):
+        ic, s1, s2 = ENSEMBLE_ICS[r % 3]
+        v, w = markers(ic, N, seed0 + r, sigma1=s1, sigma2=s2, peak=2.0)
+        out.append((ic, v, w))
+    return out
+
+
+def record(ws: Workspace, runs, steps: int, params: _lib.StepParams, log=None) -> int:
+    """Step every run `steps` times with record_training = 1 (one sample per
+    marker per step appended to the device dataset); returns the sample count."""
+    from .sim import Simulation
+    ml = WarmStart(ws)
+    N = runs[0][1].shape[0]
+    ml.dataset_alloc(len(runs) * steps * N)
+    prm = _lib.StepParams.from_buffer_copy(params)
+    prm.record_training = 1
+    prm.warmstart = _lib.WARM_EULER
+    for i, (ic, v, w) in enumerate(runs):
+        sim = Simulation(ws, v, w)
+        st = sim.run(prm, steps)
+        if log:
+            log(f"  run {i} ({ic}): {steps} steps, mean maps {np.mean([s['iters'] for s in st]):.1f}")
+    return ml.dataset_size()
+
+
+def evaluate(ws: Workspace, v, w, params: _lib.StepParams, steps: int) -> dict:
+    """`steps` steps from (v, w): per-step maps, initial residual r0, wall time."""
+    import time
+
+    from .sim import Simulation
+    sim = Simulation(ws, v, w)
+    t0 = time.perf_counter()
+    st = sim.run(params, steps)
+    el = time.perf_counter() - t0
+    return {"steps_per_s": steps / el, "mean_maps": float(np.mean([s["iters"] for s in st])),
+            "mean_inner": float(np.mean([s["inner_maps"] for s in st])),
+            "mean_r0": float(np.mean([s["r0"] for s in st])),
+            "median_r0": float(np.median([s["r0"] for s in st])),
+            "all_converged": all(s["exit_reason"] == 0 for s in st), "markers": sim.markers()}
+
+
+MODEL_DIR = __import__("os").path.join(__import__("os").path.dirname(__file__), "models")
+# trained by `bench.py --report warmstart` (8 runs x 200 recorded steps at N = 65536 on
+# the 32 x 32 mesh over [-6, 6]^2, dt = 0.02; 20 Adam epochs, lr 1e-3)
+PACKAGED_MODEL = __import__("os").path.join(MODEL_DIR, "warmstart_mlp_32x32.npz")
+
+
+def load_packaged(ws: Workspace, path: str | None = None) -> "WarmStart":
+    """Load the packaged warm-start model onto ws's context."""
+    ml = WarmStart(ws)
+    ml.load(path or PACKAGED_MODEL)
+    return ml

@@ -83,3 +83,47 @@ def test_adam_lowers_loss(wsm):
     prm.warmstart = collop.WARM_MLP
     st = sim.run(prm, 1)[0]
     assert st["exit_reason"] == 0
+
+
+def test_trained_warm_start_gain(gpu_ctx):
+    """The learned warm start end to end at a small size: record solver steps on
+    the device, fit the normalisation, train with the library's own Adam, then
+    step a held-out seed cold (Euler) and warm (Euler + MLP).  The network must
+    shrink the predictor's error -- the first map's residual r0 -- and both
+    solves must reach the same root (only the iteration changes)."""
+    from collop import warmstart as W
+    p = collop.config("bimodal_n65536", N_PARTICLES=4096, DT=0.005)
+    ws = collop.Workspace(p.bp1, p.bp2, ctx=gpu_ctx)
+    runs = W.training_runs(3, 4096, seed0=300)
+    n = W.record(ws, runs, 40, p.step_params())
+    assert n == 3 * 40 * 4096
+    ml = W.WarmStart(ws)
+    ml.set_params(W.glorot_init(0))
+    ml.fit_norm()
+    losses = ml.train(30, batch=4096, lr=1e-3, seed=2)
+    print("loss per epoch", [round(x, 4) for x in losses[::5]])
+    assert losses[-1] < losses[0]
+    v, w = collop.markers("bimodal", 4096, 777, sigma1=0.5 ** 0.5, sigma2=0.5 ** 0.5, peak=2.0)
+    cold = W.evaluate(ws, v, w, p.step_params(warmstart=collop.WARM_EULER), 5)
+    warm = W.evaluate(ws, v, w, p.step_params(warmstart=collop.WARM_MLP), 5)
+    print(f"held-out: r0 cold {cold['mean_r0']:.3e} warm {warm['mean_r0']:.3e} "
+          f"(ratio {warm['mean_r0'] / cold['mean_r0']:.3f}); maps cold {cold['mean_maps']:.1f} warm {warm['mean_maps']:.1f}")
+    assert cold["all_converged"] and warm["all_converged"]
+    assert warm["mean_r0"] < cold["mean_r0"]
+    # one step from the same state: the same root (over several steps the
+    # trajectories may settle on different roots of later steps' systems)
+    c1 = W.evaluate(ws, v, w, p.step_params(warmstart=collop.WARM_EULER), 1)
+    w1 = W.evaluate(ws, v, w, p.step_params(warmstart=collop.WARM_MLP), 1)
+    assert np.max(np.abs(w1["markers"] - c1["markers"])) <= 1e-8
+
+
+def test_packaged_model_loads_and_steps(gpu_ctx):
+    from collop import warmstart as W
+    p = collop.chosen_config("bimodal_n65536", N_PARTICLES=8192)
+    ws = collop.build_workspace(p, ctx=gpu_ctx)
+    ml = W.load_packaged(ws)
+    assert ml.params().shape == (W.NPARAM,) and np.all(np.isfinite(ml.params()))
+    v, w = collop.initial_markers(p)
+    sim = collop.Simulation(ws, v, w)
+    st = sim.run(p.step_params(warmstart=collop.WARM_MLP), 2)
+    assert all(s["exit_reason"] == 0 for s in st)
