@@ -60,7 +60,9 @@ PAIR_KERNELS = [("k_pair_f64<1,1,false>", FP64_VECTOR_PEAK_TFLOPS, 23),
 # HBM bytes per pair-sum launch at bimodal_n65536 from rocprofv3 FETCH_SIZE (x2.0)
 # and WRITE_SIZE (x1.0), both scaled by the counters of a 256 MiB copy; see
 # profiles/pair_sum_bimodal_n65536.md.
-PAIR_TRAFFIC_BYTES = {"k_pair_f64<1,1,false>": 68646 * 1024, "k_pair_f32c<1,32,false>": 50717 * 1024}
+PAIR_TRAFFIC_BYTES = {"k_pair_f64<1,1,false>": 68646 * 1024, "k_pair_f32c<1,32,false>": 50730 * 1024,
+                      "k_pair_f64<1,1,true> (pair sum + metric)": 117802 * 1024,
+                      "k_pair_f32c<1,32,true> (pair sum + metric)": 99890 * 1024}
 
 
 def log(msg: str):
@@ -151,6 +153,8 @@ def precision_report(args):
     steps = args.steps
     out = {"workload": args.config, "N": N, "steps": steps, "dt": p.DT}
     runs = {}
+    base = collop.chosen_config(args.config)   # the same solver for both: only the precision differs
+    out["solver"] = "defect correction with outer Anderson window 3, near pairs w/|d| > 0.02 (chosen path's knobs)"
     for name, prec in (("fp64", collop.PREC_F64), ("f32c", collop.PREC_F32C)):
         sim = collop.Simulation(ws, v, w)
         E0 = 0.5 * float(np.sum(w * (v[:, 0] ** 2 + v[:, 1] ** 2)))
@@ -160,11 +164,11 @@ def precision_report(args):
         drift = []
         t0 = time.perf_counter()
         for i in range(steps):
-            st = sim.run(p.step_params(precision=prec), 1)[0]
+            st = sim.run(base.step_params(precision=prec), 1)[0]
             drift.append({"dE_rel": abs(st["energy"] - E_prev) / E0,
                           "dP": float(np.max(np.abs(np.array(st["momentum"]) - P_prev))),
                           "dS": st["entropy"] - S_prev, "iters": st["iters"], "iters_f32": st["iters_f32"],
-                          "exit": st["exit_reason"]})
+                          "inner": st["inner_maps"], "residual": st["residual"], "exit": st["exit_reason"]})
             E_prev, P_prev, S_prev = st["energy"], np.array(st["momentum"]), st["entropy"]
             log(f"  {name} step {i}: iters {st['iters']} (f32 {st['iters_f32']}) exit {st['exit_reason']} "
                 f"dE/E {drift[-1]['dE_rel']:.1e} dS {drift[-1]['dS']:.2e}")
