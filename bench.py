@@ -10,24 +10,32 @@ energy / momentum of the new state.  The markers are resident in HBM before the
 timed region starts.
 
 Paths, on identical markers and to the same tolerance:
-  * plain FP64 (the reference's GPU path, restated): every Picard map in FP64,
+  * plain FP64 (the reference's GPU path, restated): every Picard map in FP64
+    with the ordered pair sum (every ordered pair, collision_gpu.jl:49-107),
     Anderson solver (step_anderson!, solver.jl:300-423);
   * chosen: frozen-metric defect correction (the reference's step_defect!,
     solver.jl:569-634) whose outer maps run the FP32-compensated pair sum with
     the metric A until the FP32 rounding floor, then FP64 outer maps to finish
-    (every accepted state comes from an FP64 map);
-  * comparison: FP32-compensated maps + FP64 finish with the Anderson solver.
+    (every accepted state comes from an FP64 map); every pair sum in the
+    symmetric form (each unordered pair evaluated once, applied to both
+    markers);
+  * comparisons: the plain FP64 path with the symmetric pair sum (the kernel's
+    share of the gain), FP32-compensated maps + FP64 finish with the Anderson
+    solver, and the chosen path with the Euler predictor only.
 `value` is the chosen path's steps/s; the plain FP64 figure and the ratio are
 reported beside it.  Per step, |E_{n+1} - E_n| / E_0 and |P_{n+1} - P_n| are
 checked against 1e-10 and the entropy against non-decrease.
 
 The roofline object covers the dominant kernel of the chosen path's timed region
-(the pair sum with the metric): algorithmic work 23 flops per ordered in-domain
-pair for the pair sum (functions.jl:221-231), 38 with the metric
-(functions.jl:276-281), over the mean launch duration measured with HIP events
-on the library's stream.  The CPU baseline is the oracle (a C++ FP64
-restatement of the reference CPU path, OpenMP over targets like
-functions.jl:210) timing whole Picard maps at the same N on this host.
+over its mean launch duration, measured with HIP events on the library's
+stream.  Algorithmic work: the ordered sum costs 23 flops per ordered in-domain
+pair (functions.jl:221-231), 38 with the metric (functions.jl:276-281); the
+symmetric sum costs, per unordered pair, the same term once plus its
+application to the second marker, 28 flops (+5: w_g times the term, two
+products with d, two sums) and 50 with the metric (+7 more: w_g |d|^-3 and
+three multiply-adds).  The CPU baseline is the oracle (a C++ FP64 restatement
+of the reference CPU path, OpenMP over targets like functions.jl:210) timing
+whole Picard maps at the same N on this host.
 
 Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu]
        python bench.py --report precision --config bimaxwellian_n262144 --steps 20
@@ -50,13 +58,24 @@ sys.path.insert(0, os.path.join(ROOT, "collisionoperators.jl_amd"))
 FP64_VECTOR_PEAK_TFLOPS = 78.6     # MI355X published FP64 vector peak
 FP32_VECTOR_PEAK_TFLOPS = 157.3    # MI355X published FP32 vector peak (packed)
 FLOPS_PER_PAIR = 23
-# Pair kernels by profiling kind (collop_profile): name, peak TFLOP/s, flops per
-# ordered pair.  The metric pass (functions.jl:276-281) adds 15 flops to the 23
-# of the pair sum (functions.jl:221-231).
-PAIR_KERNELS = [("k_pair_f64<1,1,false>", FP64_VECTOR_PEAK_TFLOPS, 23),
-                ("k_pair_f32c<1,32,false>", FP32_VECTOR_PEAK_TFLOPS, 23),
-                ("k_pair_f64<1,1,true> (pair sum + metric)", FP64_VECTOR_PEAK_TFLOPS, 38),
-                ("k_pair_f32c<1,32,true> (pair sum + metric)", FP32_VECTOR_PEAK_TFLOPS, 38)]
+
+
+def pair_kernels(tiles: int):
+    """Pair kernels by profiling kind (collop_profile): name, peak TFLOP/s, flops
+    per pair unit, unit.  tiles = 0: the ordered sum (flops per ordered pair: 23
+    for the pair sum, functions.jl:221-231, 38 with the metric,
+    functions.jl:276-281); tiles = R: the symmetric sum with R waves per block
+    (flops per unordered pair: 28, 50 with the metric)."""
+    if not tiles:
+        return [("k_pair_f64<1,1,false>", FP64_VECTOR_PEAK_TFLOPS, 23, "ordered"),
+                ("k_pair_f32c<1,32,false>", FP32_VECTOR_PEAK_TFLOPS, 23, "ordered"),
+                ("k_pair_f64<1,1,true> (pair sum + metric)", FP64_VECTOR_PEAK_TFLOPS, 38, "ordered"),
+                ("k_pair_f32c<1,32,true> (pair sum + metric)", FP32_VECTOR_PEAK_TFLOPS, 38, "ordered")]
+    R = tiles
+    return [(f"k_pair_sym_f64<{R},false,false>", FP64_VECTOR_PEAK_TFLOPS, 28, "unordered"),
+            (f"k_pair_sym_f32c<{R},false,false>", FP32_VECTOR_PEAK_TFLOPS, 28, "unordered"),
+            (f"k_pair_sym_f64<{R},true,false> (pair sum + metric)", FP64_VECTOR_PEAK_TFLOPS, 50, "unordered"),
+            (f"k_pair_sym_f32c<{R},true,false> (pair sum + metric)", FP32_VECTOR_PEAK_TFLOPS, 50, "unordered")]
 # HBM bytes per pair-sum launch at bimodal_n65536 from rocprofv3 FETCH_SIZE (x2.0)
 # and WRITE_SIZE (x1.0), both scaled by the counters of a 256 MiB copy; see
 # profiles/pair_sum_bimodal_n65536.md.
@@ -69,13 +88,15 @@ def log(msg: str):
     print(msg, file=sys.stderr, flush=True)
 
 
-def run_path(ws, p, v, w, precision, warmup, steps, solver=None, warmstart=None):
+def run_path(ws, p, v, w, precision, warmup, steps, solver=None, warmstart=None, symmetric=True):
     import collop
+    collop.set_pair_mode(ws, symmetric)
+    tiles = collop.pair_mode(ws, v.shape[0])[1]
     sim = collop.Simulation(ws, v, w)
     prm = p.step_params(precision=precision, solver=solver, warmstart=warmstart)
     log(f"  knobs: solver {prm.solver} precision {prm.precision} dc_outer_m {prm.dc_outer_m} "
         f"dc_near_theta {prm.dc_near_theta} dc_metric_reuse {prm.dc_metric_reuse} dc_eta {prm.dc_eta} "
-        f"warmstart {prm.warmstart}")
+        f"warmstart {prm.warmstart} pair sum {'symmetric, tiles %d' % tiles if tiles else 'ordered'}")
     E0 = 0.5 * float(np.sum(w * (v[:, 0] ** 2 + v[:, 1] ** 2)))
     for i in range(warmup):
         st = sim.run(prm, 1)[0]
@@ -102,7 +123,7 @@ def run_path(ws, p, v, w, precision, warmup, steps, solver=None, warmstart=None)
         dP = max(dP, float(np.max(np.abs(np.array(st["momentum"]) - P_prev))))
         dS_min = min(dS_min, st["entropy"] - S_prev)
         E_prev, P_prev, S_prev = st["energy"], np.array(st["momentum"]), st["entropy"]
-    return dict(steps_per_s=steps / elapsed, elapsed=elapsed, stats=stats, pair_ms=ms.tolist(),
+    return dict(steps_per_s=steps / elapsed, elapsed=elapsed, stats=stats, pair_ms=ms.tolist(), tiles=tiles,
                 pair_launches=launches, max_dE_rel=dE, max_dP=dP, min_dS=float(dS_min),
                 mean_iters=float(np.mean([s["iters"] for s in stats])),
                 mean_iters_f32=float(np.mean([s["iters_f32"] for s in stats])),
@@ -209,7 +230,7 @@ def precision_report(args):
 
 def kernel_probe(args):
     """Three launches of each pair kernel (plain and with the metric, FP64 and
-    FP32c) at the workload, then one 256 MiB
+    FP32c; the ordered form, then the symmetric one) at the workload, then one 256 MiB
     device-to-device copy (the known-size calibration of the rocprofv3
     FETCH_SIZE / WRITE_SIZE counters)."""
     import ctypes as C
@@ -226,15 +247,17 @@ def kernel_probe(args):
     G = np.zeros((N, 2), order="F")
     collop.compute_G_gpu(ws, G, v, L)
     out = np.zeros((N, 2), order="F")
-    for _ in range(3):
-        collop.compute_collision_gpu(ws, out, v, w, G)
-    for _ in range(3):
-        collop.compute_collision_gpu32(ws, out, v, w, G)
     A = np.zeros((N, 3), order="F")
-    for _ in range(3):
-        collop.compute_collision_metric_gpu(ws, out, A, v, w, G)
-    for _ in range(3):
-        collop.compute_collision_metric_gpu(ws, out, A, v, w, G, fp32c=True)
+    for symmetric in (False, True):
+        collop.set_pair_mode(ws, symmetric)
+        for _ in range(3):
+            collop.compute_collision_gpu(ws, out, v, w, G)
+        for _ in range(3):
+            collop.compute_collision_gpu32(ws, out, v, w, G)
+        for _ in range(3):
+            collop.compute_collision_metric_gpu(ws, out, A, v, w, G)
+        for _ in range(3):
+            collop.compute_collision_metric_gpu(ws, out, A, v, w, G, fp32c=True)
     hip = C.CDLL("libamdhip64.so")
     nbytes = 256 << 20
     a, b = C.c_void_p(), C.c_void_p()
@@ -245,7 +268,8 @@ def kernel_probe(args):
     hip.hipDeviceSynchronize()
     hip.hipFree(a)
     hip.hipFree(b)
-    log(f"[probe] pair kernels x3 (FP64, FP32c, FP64 + metric, FP32c + metric) at N={N}; copy {nbytes} bytes")
+    log(f"[probe] pair kernels x3 (FP64, FP32c, FP64 + metric, FP32c + metric; ordered, then symmetric) at N={N}; "
+        f"copy {nbytes} bytes")
 
 
 def warmstart_report(args):
@@ -397,8 +421,8 @@ def main():
     n_in = int(inside.sum())
 
     log(f"[bench] {args.config}: N={N} (in domain {n_in}), mesh {len(p.bp1) - 1}x{len(p.bp2) - 1}, dt={p.DT}")
-    log("[bench] plain FP64 path")
-    r64 = run_path(ws, p, v, w, collop.PREC_F64, args.warmup, args.steps)
+    log("[bench] plain FP64 path (ordered pair sum)")
+    r64 = run_path(ws, p, v, w, collop.PREC_F64, args.warmup, args.steps, symmetric=False)
     log(f"[bench] plain FP64: {r64['steps_per_s']:.3f} steps/s, mean iters {r64['mean_iters']:.1f}")
     log(f"[bench] chosen path (FP32-compensated maps + FP64 finish, {args.chosen_solver} solver)")
     pch = collop.chosen_config(args.config) if args.chosen_solver == "defect" else p
@@ -412,7 +436,11 @@ def main():
     log(f"[bench] chosen: {rch['steps_per_s']:.3f} steps/s, mean iters {rch['mean_iters']:.1f} "
         f"(f32 {rch['mean_iters_f32']:.1f})")
 
-    rva = rve = None
+    rva = rve = rs64 = None
+    if not args.no_variants:
+        log("[bench] comparison path: the plain FP64 path with the symmetric pair sum")
+        rs64 = run_path(ws, p, v, w, collop.PREC_F64, args.warmup, args.steps)
+        log(f"[bench] FP64 + symmetric pair sum: {rs64['steps_per_s']:.3f} steps/s, mean iters {rs64['mean_iters']:.1f}")
     if not args.no_variants and args.chosen_solver == "defect":
         log("[bench] comparison path: FP32c maps + FP64 finish with the Anderson solver")
         rva = run_path(ws, p, v, w, collop.PREC_F32C, args.warmup, args.steps, solver=collop._lib.SOLVER_ANDERSON)
@@ -426,23 +454,28 @@ def main():
     # roofline of the dominant kernel in the chosen path's timed region
     pairs = n_in * (n_in - 1)
     kinds = []  # every pair kernel either path launched: name, peak, flops per pair, avg ms, launches
-    for path, res in (("chosen", rch), ("plain_fp64", r64)):
-        for k, (kname, peak, fpp) in enumerate(PAIR_KERNELS):
+    units = {"ordered": pairs, "unordered": pairs // 2}
+    for path, res in (("chosen", rch), ("plain_fp64", r64), ("fp64_symmetric", rs64)):
+        if res is None:
+            continue
+        for k, (kname, peak, fpp, unit) in enumerate(pair_kernels(res["tiles"])):
             if res["pair_launches"][k]:
                 avg = res["pair_ms"][k] / res["pair_launches"][k]
                 kinds.append({"path": path, "kernel": kname, "launches": res["pair_launches"][k],
                               "total_ms": round(res["pair_ms"][k], 3), "avg_ms": round(avg, 4),
-                              "tflops": round(fpp * pairs / (avg * 1e-3) / 1e12, 3),
-                              "frac": round(fpp * pairs / (avg * 1e-3) / 1e12 / peak, 4)})
+                              "tflops": round(fpp * units[unit] / (avg * 1e-3) / 1e12, 3),
+                              "frac": round(fpp * units[unit] / (avg * 1e-3) / 1e12 / peak, 4),
+                              "ordered_pairs_per_s": round(pairs / (avg * 1e-3), 1)})
     dom = int(np.argmax(rch["pair_ms"]))
-    kname, peak, fpp = PAIR_KERNELS[dom]
+    kname, peak, fpp, unit = pair_kernels(rch["tiles"])[dom]
     avg_ms = rch["pair_ms"][dom] / max(rch["pair_launches"][dom], 1)
-    flops = fpp * pairs
+    flops = fpp * units[unit]
     achieved = flops / (avg_ms * 1e-3) / 1e12
     traffic = PAIR_TRAFFIC_BYTES.get(kname) if args.config == "bimodal_n65536" else None
     roofline = {"bound": "valu", "kernel": kname, "achieved": round(achieved, 3), "peak": peak, "unit": "TFLOP/s",
                 "frac": round(achieved / peak, 4), "traffic": traffic, "avg_launch_ms": round(avg_ms, 4),
-                "flops_per_pair": fpp, "flops_per_launch": flops, "pair_kernels": kinds}
+                "flops_per_pair": fpp, "pair_unit": unit, "pairs_per_launch": units[unit],
+                "flops_per_launch": flops, "pair_kernels": kinds}
 
     cpu = None
     if not args.no_cpu:
@@ -471,9 +504,14 @@ def main():
         "chosen_path": ("FP32c outer maps (F and metric A) then FP64 outer maps, frozen-metric defect correction "
                         "(step_defect!) with an outer Anderson window of 3 and exact near pairs (w/|d| > 0.02), "
                         "FP32c Euler predictor" + (" + learned MLP warm start" if warm == collop.WARM_MLP else "")
+                        + "; symmetric pair sum (each unordered pair once)"
                         if args.chosen_solver == "defect" else
-                        "FP32c maps then FP64 maps, Anderson (step_anderson!), Euler predictor"),
-        "plain_path": "FP64 maps, Anderson (step_anderson!), Euler predictor",
+                        "FP32c maps then FP64 maps, Anderson (step_anderson!), Euler predictor, symmetric pair sum"),
+        "plain_path": "FP64 maps, Anderson (step_anderson!), Euler predictor, ordered pair sum",
+        "fp64_symmetric": (None if rs64 is None else
+                           {"steps_per_s": round(rs64["steps_per_s"], 4),
+                            "ratio_vs_fp64": round(rs64["steps_per_s"] / r64["steps_per_s"], 4),
+                            "mean_iters": rs64["mean_iters"], "all_converged": rs64["all_converged"]}),
         "chosen_euler_predictor": (None if rve is None else
                                    {"steps_per_s": round(rve["steps_per_s"], 4),
                                     "ratio_vs_fp64": round(rve["steps_per_s"] / r64["steps_per_s"], 4),

@@ -14,5 +14,5 @@ from .hooks import (compute_collision_gpu, compute_collision_gpu32, compute_coll
 from .ic import ensemble_markers, initial_markers, markers  # noqa: F401
 from .mesh import Workspace, build_workspace, extraction_p2c1  # noqa: F401
 from .params import CHOSEN, CONFIGS, SimParameters, chosen_config, config, parse_overrides  # noqa: F401
-from .sim import Ensemble, Simulation, profile, set_pair_tile  # noqa: F401
+from .sim import Ensemble, Simulation, pair_mode, profile, set_pair_mode, set_pair_tile  # noqa: F401
 from .warmstart import WarmStart, glorot_init  # noqa: F401

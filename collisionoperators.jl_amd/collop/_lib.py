@@ -29,6 +29,8 @@ WARM_EULER = 0
 WARM_MLP = 1
 SOLVER_ANDERSON = 0
 SOLVER_DEFECT = 1
+PAIR_ORDERED = 0
+PAIR_SYMMETRIC = 1
 
 
 class CollopError(RuntimeError):
@@ -147,6 +149,9 @@ SIGNATURES = [
     ("collop_ensemble_set_tile", C.c_int, [_vp, C.c_int, C.c_int, C.c_int]),
     ("collop_ensemble_tune", C.c_int, [_vp, C.POINTER(C.c_int), C.POINTER(C.c_int), _dp]),
     ("collop_set_pair_tile", C.c_int, [_vp, C.c_int, C.c_int, C.c_int]),
+    ("collop_set_pair_mode", C.c_int, [_vp, C.c_int, C.c_int]),
+    ("collop_get_pair_mode", C.c_int, [_vp, C.c_int64, C.POINTER(C.c_int), C.POINTER(C.c_int),
+                                      C.POINTER(C.c_int)]),
     ("collop_profile", C.c_int, [_vp, C.c_int, _dp, C.POINTER(C.c_long)]),
 ]
 

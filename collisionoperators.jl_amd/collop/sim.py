@@ -60,8 +60,25 @@ def profile(ws: Workspace, enable: bool):
 
 
 def set_pair_tile(ws: Workspace, gamma_per_lane: int = 1, slices: int = 0, newton: int = 1):
-    """Pin the pair-sum tile (slices = 0: automatic); see collop_set_pair_tile."""
+    """Pin the ordered pair-sum tile (slices = 0: automatic) and select the
+    ordered sum; see collop_set_pair_tile."""
     ws.ctx.check(ws.ctx.lib.collop_set_pair_tile(ws.ctx.h, gamma_per_lane, slices, newton))
+
+
+def set_pair_mode(ws: Workspace, symmetric: bool = True, tiles: int = 0):
+    """Ordered pair sum (every ordered pair, as the reference's GPU kernel) or
+    the symmetric one (every unordered pair once, the default); tiles = waves
+    per block of the symmetric sum (0: automatic).  See collop_set_pair_mode."""
+    mode = _lib.PAIR_SYMMETRIC if symmetric else _lib.PAIR_ORDERED
+    ws.ctx.check(ws.ctx.lib.collop_set_pair_mode(ws.ctx.h, mode, tiles))
+
+
+def pair_mode(ws: Workspace, N: int):
+    """(symmetric, tiles, slots) a pair sum over N markers runs with now."""
+    mode, tiles, slots = C.c_int(), C.c_int(), C.c_int()
+    ws.ctx.check(ws.ctx.lib.collop_get_pair_mode(ws.ctx.h, int(N), C.byref(mode), C.byref(tiles),
+                                                 C.byref(slots)))
+    return mode.value == _lib.PAIR_SYMMETRIC, tiles.value, slots.value
 
 
 class Ensemble:

@@ -171,20 +171,43 @@ int check_N(collop_ctx* ctx, int64_t N) {
   return COLLOP_OK;
 }
 
-void choose_tile(collop_ctx* ctx, int N) {
-  if (!ctx->tile_auto) return;
-  // Enough workgroups (>= 8192, i.e. ~32 waves per SIMD on 256 CUs, measured
-  // best at N = 65536) by cutting the source range into slices; multiples of 8
-  // keep one slice set per XCD.
-  for (PairTile* t : {&ctx->tile64, &ctx->tile32}) {
-    const int per = 256 * t->gamma_per_lane;
+// Symmetric pair sum: R waves per block.  Automatic: 2 below N = 4096, else 4
+// (measured fastest at N = 65536: 8256 blocks of 256 threads), or 8 where 4
+// would need more than kSymMaxSlots partial slots.  0 (the ordered sum) when
+// it is switched off or N needs more slots than that.
+int sym_tiles_for(const collop_ctx* ctx, int64_t N) {
+  if (!ctx->pair_sym || N <= 0) return 0;
+  int R = ctx->sym_tiles;
+  if (!R) R = N < 4096 ? 2 : (sym_rows((int)N, 4) <= kSymMaxSlots ? 4 : 8);
+  return sym_rows((int)N, R) <= kSymMaxSlots ? R : 0;
+}
+
+// The tile a pair sum over N markers runs with, from the context's settings.
+PairTile tile_for(const collop_ctx* ctx, const PairTile& cur, int N) {
+  PairTile t = cur;
+  t.sym = sym_tiles_for(ctx, N);
+  if (t.sym) {
+    t.slices = sym_rows(N, t.sym);
+  } else if (!ctx->tile_auto) {
+    t.slices = ctx->manual_slices;
+  } else {
+    // Enough workgroups (>= 8192, i.e. ~32 waves per SIMD on 256 CUs, measured
+    // best at N = 65536) by cutting the source range into slices; multiples of 8
+    // keep one slice set per XCD.
+    const int per = 256 * t.gamma_per_lane;
     const int gb = std::max(1, (N + per - 1) / per);
     int s = (8192 + gb - 1) / gb;
     if (s >= 8) s = (s + 7) / 8 * 8;
     s = std::max(1, std::min(s, 64));
     while (s > 1 && (N + s - 1) / s < 16) s /= 2;  // keep slices non-trivial
-    t->slices = s;
+    t.slices = s;
   }
+  return t;
+}
+
+void choose_tile(collop_ctx* ctx, int N) {
+  ctx->tile64 = tile_for(ctx, ctx->tile64, N);
+  ctx->tile32 = tile_for(ctx, ctx->tile32, N);
 }
 
 // ---- profiling of the pair kernel (HIP events on the launch stream) -------
@@ -247,7 +270,8 @@ int ensure_scratch(collop_ctx* ctx, int64_t N) {
   ENSURE(ctx->b_dS, n2 * 8);
   ENSURE(ctx->b_s64, (n2 / 2) * sizeof(Src64));
   ENSURE(ctx->b_s32, (n2 / 2) * sizeof(Src32));
-  const int Smax = 64;
+  const int Rs = sym_tiles_for(ctx, N);
+  const int Smax = std::max(64, Rs ? sym_rows((int)N, Rs) : 0);
   ENSURE(ctx->b_part, (size_t)Smax * (n2 / 2) * 5 * 8);
   const int nsb = scatter_blocks((int)N);
   ENSURE(ctx->b_spart_sc, (size_t)2 * nsb * m.n_dofs * 8);
@@ -1534,8 +1558,8 @@ int collop_mlp_eval(collop_ctx* ctx, const float* rec, int64_t n, int mode, floa
 }
 
 // ---- extras (not part of the reference's hook set) ----------------------
-// Pair-sum tile: gamma_per_lane in {1,2}, slices in [1, 64], newton in {1,2};
-// slices = 0 restores the automatic choice.
+// Ordered pair-sum tile (selects the ordered sum): gamma_per_lane in {1,2},
+// slices in [1, 64], newton in {1,2}; slices = 0 restores the automatic choice.
 int collop_set_pair_tile(collop_ctx* ctx, int gamma_per_lane, int slices, int newton) {
   if (!ctx) return COLLOP_ERR_INVALID;
   if (gamma_per_lane != 1 && gamma_per_lane != 2) return fail(ctx, COLLOP_ERR_INVALID, "gamma_per_lane must be 1 or 2");
@@ -1544,8 +1568,40 @@ int collop_set_pair_tile(collop_ctx* ctx, int gamma_per_lane, int slices, int ne
   ctx->tile64.gamma_per_lane = ctx->tile32.gamma_per_lane = gamma_per_lane;
   ctx->tile64.newton = newton;
   ctx->tile_auto = slices == 0;
-  if (slices > 0) ctx->tile64.slices = ctx->tile32.slices = slices;
+  ctx->manual_slices = slices;
+  ctx->pair_sym = false;
   if (ctx->has_state) choose_tile(ctx, (int)ctx->N);
+  return COLLOP_OK;
+}
+
+int collop_get_pair_mode(collop_ctx* ctx, int64_t N, int* mode, int* tiles, int* slots) {
+  if (!ctx) return COLLOP_ERR_INVALID;
+  if (N < 0 || N > (int64_t)1 << 27) return fail(ctx, COLLOP_ERR_INVALID, "N must be in [0, 2^27]");
+  const int R = sym_tiles_for(ctx, N);
+  if (mode) *mode = R ? COLLOP_PAIR_SYMMETRIC : COLLOP_PAIR_ORDERED;
+  if (tiles) *tiles = R;
+  if (slots) *slots = tile_for(ctx, ctx->tile64, (int)N).slices;
+  return COLLOP_OK;
+}
+
+int collop_set_pair_mode(collop_ctx* ctx, int mode, int tiles) {
+  if (!ctx) return COLLOP_ERR_INVALID;
+  if (mode != COLLOP_PAIR_ORDERED && mode != COLLOP_PAIR_SYMMETRIC)
+    return fail(ctx, COLLOP_ERR_INVALID, "mode must be COLLOP_PAIR_ORDERED or COLLOP_PAIR_SYMMETRIC");
+  if (tiles != 0 && tiles != 2 && tiles != 4 && tiles != 8)
+    return fail(ctx, COLLOP_ERR_INVALID, "tiles must be 0 (automatic), 2, 4 or 8");
+  ctx->pair_sym = mode == COLLOP_PAIR_SYMMETRIC;
+  ctx->sym_tiles = tiles;
+  if (mode == COLLOP_PAIR_ORDERED) {  // the ordered sum with its automatic tile
+    ctx->tile_auto = true;
+    ctx->tile64.gamma_per_lane = ctx->tile32.gamma_per_lane = 1;
+    ctx->tile64.newton = 1;
+  }
+  if (ctx->has_state) {
+    int r;
+    if ((r = ensure_scratch(ctx, ctx->N))) return r;
+    choose_tile(ctx, (int)ctx->N);
+  }
   return COLLOP_OK;
 }
 

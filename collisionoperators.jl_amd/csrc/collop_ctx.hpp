@@ -58,6 +58,9 @@ struct collop_ctx {
   collop::PairTile tile64{1, 32, 1};
   collop::PairTile tile32{1, 32, 0};
   bool tile_auto = true;
+  int manual_slices = 0;
+  bool pair_sym = true;  // symmetric pair sum (each unordered pair once)
+  int sym_tiles = 0;     // its waves per block (0: automatic)
   bool profile = false;
   std::vector<std::pair<hipEvent_t, hipEvent_t>> ev_pool;
   size_t ev_used = 0;

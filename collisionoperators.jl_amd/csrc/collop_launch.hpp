@@ -60,20 +60,6 @@ struct EnsView {
 };
 
 // ---- pair sum (pair.hip) ----------------------------------------------------
-struct PairTile {
-  int gamma_per_lane;  // 1 or 2 targets per lane
-  int slices;          // alpha slices (partials per target)
-  int newton;          // FP64 rsqrt refinement steps (1 or 2)
-};
-
-// Pack (v, G, w) into Src64/Src32 records.  Markers outside the open box get
-// w_masked = 0 and in_domain = 0 (functions.jl:212-213, :219-220).
-void launch_pack_src(const double* v, const double* G, const double* w, int N, double lo1, double hi1, double lo2,
-                     double hi2, Src64* s64, Src32* s32, hipStream_t st);
-// part[S][2N] <- slice sums of the FP64 pair sum
-void launch_pair_f64(const Src64* src, int N, const PairTile& t, double* part, hipStream_t st);
-// part[S][2N] <- slice sums of the FP32 pair sum (compensated, written in FP64)
-void launch_pair_f32c(const Src32* src, int N, const PairTile& t, double* part, hipStream_t st);
 // Near-pair detection during a metric pass: every ordered in-domain pair with
 // w_a / |d| > theta (th2 = theta^2) is appended to list[] as (gamma, alpha);
 // count receives the number found (entries beyond cap are dropped, the count
@@ -85,6 +71,32 @@ struct NearOut {
   int cap;
 };
 constexpr int kNearCap = 1 << 16;  // near pairs kept per step (beyond it the near field is skipped)
+struct PairTile {
+  int gamma_per_lane;  // 1 or 2 targets per lane
+  int slices;          // alpha slices (partials per target)
+  int newton;          // FP64 rsqrt refinement steps (1 or 2)
+  int sym;             // 0: ordered sum; R in {2, 4, 8}: symmetric sum, R waves per block (slices = sym_rows)
+};
+
+// ---- symmetric pair sum (pair_sym.hip) -----------------------------------------
+// Each unordered pair once; super-rows of 128 R markers, one block of R waves
+// per pair of super-rows.  Writes part[S][C][N] with S = sym_rows(N, R) slots
+// (C = 2, or 5 with the metric), every slot of every marker exactly once.
+__host__ __device__ inline int sym_rows(int N, int R) { return (N + 128 * R - 1) / (128 * R); }
+constexpr int kSymMaxSlots = 256;  // larger N falls back to the ordered sum
+void launch_pair_sym_f64(const Src64* src, int N, int R, bool metric, const NearOut* near, double* part,
+                         hipStream_t st);
+void launch_pair_sym_f32c(const Src32* src, int N, int R, bool metric, const NearOut* near, double* part,
+                          hipStream_t st);
+
+// Pack (v, G, w) into Src64/Src32 records.  Markers outside the open box get
+// w_masked = 0 and in_domain = 0 (functions.jl:212-213, :219-220).
+void launch_pack_src(const double* v, const double* G, const double* w, int N, double lo1, double hi1, double lo2,
+                     double hi2, Src64* s64, Src32* s32, hipStream_t st);
+// part[S][2N] <- slice sums of the FP64 pair sum
+void launch_pair_f64(const Src64* src, int N, const PairTile& t, double* part, hipStream_t st);
+// part[S][2N] <- slice sums of the FP32 pair sum (compensated, written in FP64)
+void launch_pair_f32c(const Src32* src, int N, const PairTile& t, double* part, hipStream_t st);
 // part[S][5N] <- slice sums of F (2 comps) and A (3 comps); near != null also
 // collects the near pairs (one target per lane, one Newton step)
 void launch_pair_metric_f64(const Src64* src, int N, const PairTile& t, double* part, hipStream_t st,

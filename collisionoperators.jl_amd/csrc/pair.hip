@@ -322,6 +322,7 @@ void launch_pack_src(const double* v, const double* G, const double* w, int N, d
 
 void launch_pair_f64(const Src64* src, int N, const PairTile& t, double* part, hipStream_t st) {
   if (N <= 0) return;
+  if (t.sym) return launch_pair_sym_f64(src, N, t.sym, false, nullptr, part, st);
   const int chunk = (N + t.slices - 1) / t.slices;
   const dim3 grid(grid_for(N, t)), block(kThreads);
   if (t.gamma_per_lane == 2) {
@@ -340,6 +341,7 @@ void launch_pair_f64(const Src64* src, int N, const PairTile& t, double* part, h
 void launch_pair_metric_f64(const Src64* src, int N, const PairTile& t, double* part, hipStream_t st,
                             const NearOut* near) {
   if (N <= 0) return;
+  if (t.sym) return launch_pair_sym_f64(src, N, t.sym, true, near, part, st);
   const int chunk = (N + t.slices - 1) / t.slices;
   const dim3 grid(grid_for(N, t)), block(kThreads);
   if (near) {  // one target per lane, one Newton step
@@ -362,6 +364,7 @@ void launch_pair_metric_f64(const Src64* src, int N, const PairTile& t, double* 
 
 void launch_pair_f32c(const Src32* src, int N, const PairTile& t, double* part, hipStream_t st) {
   if (N <= 0) return;
+  if (t.sym) return launch_pair_sym_f32c(src, N, t.sym, false, nullptr, part, st);
   const int chunk = (N + t.slices - 1) / t.slices;
   const dim3 grid(grid_for(N, t)), block(kThreads);
   if (t.gamma_per_lane == 2)
@@ -373,6 +376,7 @@ void launch_pair_f32c(const Src32* src, int N, const PairTile& t, double* part, 
 void launch_pair_metric_f32c(const Src32* src, int N, const PairTile& t, double* part, hipStream_t st,
                              const NearOut* near) {
   if (N <= 0) return;
+  if (t.sym) return launch_pair_sym_f32c(src, N, t.sym, true, near, part, st);
   const int chunk = (N + t.slices - 1) / t.slices;
   const dim3 grid(grid_for(N, t)), block(kThreads);
   if (near) {

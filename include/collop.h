@@ -270,9 +270,27 @@ int collop_ensemble_set_tile(collop_ctx* ctx, int gamma_tile, int waves_per_cu, 
 int collop_ensemble_tune(collop_ctx* ctx, int* gamma_tile, int* waves_per_cu, double* ms);
 
 /* ---- tuning and measurement (no counterpart in the reference) -----------
- * Pair-sum tile: gamma_per_lane in {1, 2} targets per lane, slices in [1, 64]
- * source slices (0 = automatic), newton in {1, 2} FP64 rsqrt refinements. */
+ * Ordered pair-sum tile (and selects the ordered sum): gamma_per_lane in {1, 2}
+ * targets per lane, slices in [1, 64] source slices (0 = automatic), newton in
+ * {1, 2} FP64 rsqrt refinements. */
 int collop_set_pair_tile(collop_ctx* ctx, int gamma_per_lane, int slices, int newton);
+
+typedef enum {
+  COLLOP_PAIR_ORDERED = 0,   /* every ordered pair evaluated, as collision_gpu.jl:49-107 */
+  COLLOP_PAIR_SYMMETRIC = 1  /* every unordered pair evaluated once and applied to
+                                both markers (the default)                           */
+} collop_pair_mode;
+/* Pair-sum form for every later pair sum of the context (hooks, steps);
+ * tiles = waves per block of the symmetric sum, 0 (automatic), 2, 4 or 8.
+ * Both forms sum the same terms in different orders: results agree to
+ * rounding.  The symmetric form needs ceil(N / (128 tiles)) <= 256 partial
+ * slots and falls back to the ordered sum beyond that.  No counterpart in the
+ * reference. */
+int collop_set_pair_mode(collop_ctx* ctx, int mode, int tiles);
+/* The form (mode), waves per block (tiles, 0 for the ordered sum) and partial
+ * slots a pair sum over N markers runs with under the current settings; each
+ * output pointer may be NULL. */
+int collop_get_pair_mode(collop_ctx* ctx, int64_t N, int* mode, int* tiles, int* slots);
 /* HIP-event timing of every pair-kernel launch on the context's stream.
  * Returns the totals accumulated since the previous call (ms_total[4],
  * launches[4]; index 0 = FP64, 1 = FP32c, 2 = FP64 with the metric A,

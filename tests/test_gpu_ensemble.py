@@ -57,7 +57,7 @@ def single_reference(ws, runs, prec, steps):
         st = sim.run(params(prec), steps)
         out.append(sim.markers())
         iters.append([s["iters"] for s in st])
-    collop.set_pair_tile(ws, 1, 0, 1)
+    collop.set_pair_mode(ws, True)
     return out, iters
 
 

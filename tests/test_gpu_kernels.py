@@ -59,6 +59,7 @@ def test_collision_f64_tiles(c1, tile):
         collop.compute_collision_gpu(c1["ws"], out, c1["v"], c1["w"], c1["G"])
     finally:
         collop.set_pair_tile(c1["ws"], 1, 0, 1)
+        collop.set_pair_mode(c1["ws"], True)
     assert relerr(out, ref) <= 1e-12
 
 

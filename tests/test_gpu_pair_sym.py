@@ -1,0 +1,163 @@
+"""The symmetric pair sum (every unordered pair evaluated once and applied to
+both markers) against the oracle's ordered sum (functions.jl:205-237,
+:249-290), for every block shape, ragged N (partial super-rows and tiles),
+markers outside the box and coincident pairs in different super-rows, and
+against the ordered GPU kernel.  Bars as for the ordered kernel: FP64 1e-12,
+FP32-compensated 2e-6 (metric A 1e-4, see test_gpu_kernels.py).
+"""
+import numpy as np
+import pytest
+
+import collop
+import oracle as O
+from conftest import relerr, uniform_bp
+
+pytestmark = pytest.mark.gpu
+
+
+def chain(ic, N, ne, seed=42):
+    sig = {"gaussian": (1.0, 1.0), "bimodal": (np.sqrt(0.5), np.sqrt(0.5))}[ic]
+    v, w = collop.markers(ic, N, seed, sigma1=sig[0], sigma2=sig[1], peak=2.0, weight_jitter=0.3)
+    bp = uniform_bp(-6.0, 6.0, ne)
+    m = O.Mesh(bp, bp)
+    fc = m.l2_project(v, w)
+    G = m.compute_G(v, m.mass_solve(m.compute_r(fc)))
+    return v, w, bp, m, G
+
+
+def all_four(ws, v, w, G):
+    N = v.shape[0]
+    F64 = np.zeros((N, 2), order="F")
+    F32 = np.zeros((N, 2), order="F")
+    MF = np.zeros((N, 2), order="F")
+    MA = np.zeros((N, 3), order="F")
+    MF32 = np.zeros((N, 2), order="F")
+    MA32 = np.zeros((N, 3), order="F")
+    collop.compute_collision_gpu(ws, F64, v, w, G)
+    collop.compute_collision_gpu32(ws, F32, v, w, G)
+    collop.compute_collision_metric_gpu(ws, MF, MA, v, w, G)
+    collop.compute_collision_metric_gpu(ws, MF32, MA32, v, w, G, fp32c=True)
+    return F64, F32, MF, MA, MF32, MA32
+
+
+@pytest.fixture(autouse=True)
+def default_pair_mode(gpu_ctx):
+    """The pair form is a setting of the shared context: restore the default."""
+    yield
+    gpu_ctx.check(gpu_ctx.lib.collop_set_pair_mode(gpu_ctx.h, collop._lib.PAIR_SYMMETRIC, 0))
+
+
+@pytest.fixture(scope="module")
+def sym_ws(gpu_ctx):
+    return collop.Workspace(uniform_bp(-6.0, 6.0, 16), uniform_bp(-6.0, 6.0, 16), ctx=gpu_ctx)
+
+
+@pytest.mark.parametrize("tiles", [2, 4, 8])
+@pytest.mark.parametrize("N", [1, 2, 129, 1000, 4096, 5003])
+def test_symmetric_matches_oracle(sym_ws, tiles, N):
+    v, w, bp, m, G = chain("gaussian", N, 16, seed=N)
+    collop.set_pair_mode(sym_ws, True, tiles)
+    F64, F32, MF, MA, MF32, MA32 = all_four(sym_ws, v, w, G)
+    ref = m.collision(v, w, G)
+    Fr, Ar = O.collision_metric(m.box, v, w, G)
+    e = [relerr(F64, ref), relerr(F32, ref), relerr(MF, Fr), relerr(MA, Ar), relerr(MF32, Fr), relerr(MA32, Ar)]
+    print(f"tiles {tiles} N {N}: f64 {e[0]:.1e} f32c {e[1]:.1e} metric F {e[2]:.1e} A {e[3]:.1e} "
+          f"f32c metric F {e[4]:.1e} A {e[5]:.1e}")
+    if N == 1:  # a single marker has no partner
+        assert np.all(F64 == 0.0) and np.all(MA == 0.0)
+        return
+    assert e[0] <= 1e-12 and e[2] <= 1e-12 and e[3] <= 1e-12
+    assert e[1] <= 2e-6 and e[4] <= 2e-6 and e[5] <= 1e-4
+
+
+def test_symmetric_out_of_domain_and_coincident(gpu_ctx):
+    bp = uniform_bp(-2.0, 2.0, 8)
+    m = O.Mesh(bp, bp)
+    ws = collop.Workspace(bp, bp, ctx=gpu_ctx)
+    collop.set_pair_mode(ws, True, 2)  # super-rows of 256: three of them at N = 700
+    rng = np.random.default_rng(12)
+    N = 700
+    v = np.asfortranarray(rng.uniform(-2.5, 2.5, (N, 2)))
+    v[5] = v[600]                   # coincident pair in different super-rows: skipped
+    v[7] = v[8]                     # and in the same tile
+    v[9] = (-2.0, 0.3)              # on the boundary: outside the open box
+    w = rng.uniform(0.5, 1.5, N) / N
+    G = np.asfortranarray(rng.standard_normal((N, 2)))
+    ref = m.collision(v, w, G)
+    Fr, Ar = O.collision_metric(m.box, v, w, G)
+    F64, F32, MF, MA, MF32, MA32 = all_four(ws, v, w, G)
+    collop.set_pair_mode(ws, False)
+    o = all_four(ws, v, w, G)
+    assert relerr(F64, ref) <= 1e-12 and relerr(MF, Fr) <= 1e-12 and relerr(MA, Ar) <= 1e-12
+    # G here is noise, not a projected gradient: G_a - G_g stays O(1) for close
+    # pairs, so the FP32 rounding of the positions (identical in both kernels:
+    # d is formed the same way) is not cancelled and both FP32c kernels sit
+    # ~3e-6 from the FP64 oracle.  Against each other they differ by summation
+    # order only.
+    e32 = [relerr(F32, ref), relerr(F32, o[1]), relerr(MF32, o[4]), relerr(MA32, o[5])]
+    print("f32c: vs oracle %.1e, vs ordered kernel F %.1e metric F %.1e A %.1e" % tuple(e32))
+    assert e32[0] <= 1e-5 and relerr(o[1], ref) <= 1e-5
+    assert e32[1] <= 1e-6 and e32[2] <= 1e-6 and e32[3] <= 1e-6
+    outside = (v[:, 0] <= -2) | (v[:, 0] >= 2) | (v[:, 1] <= -2) | (v[:, 1] >= 2)
+    assert outside.sum() > 20
+    for X in (F64, F32, MF, MA, MF32, MA32):
+        assert np.all(X[outside] == 0.0)
+
+
+def test_symmetric_vs_ordered_bimodal_n65536(gpu_ctx):
+    v, w, bp, m, G = chain("bimodal", 65536, 32)
+    ws = collop.Workspace(bp, bp, ctx=gpu_ctx)
+    collop.set_pair_mode(ws, False)
+    o = all_four(ws, v, w, G)
+    collop.set_pair_mode(ws, True)
+    s = all_four(ws, v, w, G)
+    ref = O.collision_rows(m.box, v, w, G, 0, 2000)
+    e_sym = relerr(s[0][:2000], ref[:2000])
+    e_ord = relerr(o[0][:2000], ref[:2000])
+    d = [relerr(a, b) for a, b in zip(s, o)]
+    print(f"N=65536: oracle rows 0..2000 sym {e_sym:.2e} ordered {e_ord:.2e}; sym vs ordered "
+          + " ".join(f"{x:.1e}" for x in d))
+    assert e_sym <= 1e-12 and e_ord <= 1e-12
+    assert d[0] <= 1e-13 and d[2] <= 1e-13 and d[3] <= 1e-13
+    assert d[1] <= 2e-6 and d[4] <= 2e-6 and d[5] <= 1e-4
+    # momentum: each unordered pair adds w_g (w_a t) - w_a (w_g t) = O(rounding) to sum w F
+    scale = np.abs(w[:, None] * s[0]).sum(axis=0)
+    p_sym = np.abs((w[:, None] * s[0]).sum(axis=0)) / scale
+    p_ord = np.abs((w[:, None] * o[0]).sum(axis=0)) / scale
+    print(f"relative momentum of the pair sum: sym {p_sym} ordered {p_ord}")
+    assert np.all(p_sym <= 1e-12)
+
+
+def test_symmetric_deterministic_and_fallback(gpu_ctx):
+    v, w, bp, m, G = chain("gaussian", 66000, 16, seed=3)
+    ws = collop.Workspace(bp, bp, ctx=gpu_ctx)
+    collop.set_pair_mode(ws, True, 8)
+    a = np.zeros((66000, 2), order="F")
+    b = np.zeros((66000, 2), order="F")
+    collop.compute_collision_gpu(ws, a, v, w, G)
+    collop.compute_collision_gpu(ws, b, v, w, G)
+    np.testing.assert_array_equal(a, b)
+    # 66000 markers need 258 super-rows of 256 at tiles = 2: the ordered sum runs
+    collop.set_pair_mode(ws, True, 2)
+    c = np.zeros((66000, 2), order="F")
+    collop.compute_collision_gpu(ws, c, v, w, G)
+    ref = O.collision_rows(m.box, v, w, G, 0, 1500)
+    assert relerr(c[:1500], ref[:1500]) <= 1e-12
+    assert relerr(a[:1500], ref[:1500]) <= 1e-12
+    assert relerr(a, c) <= 1e-13
+    with pytest.raises(collop.CollopError):
+        collop.set_pair_mode(ws, True, 3)
+
+
+def test_pair_mode_choice(sym_ws):
+    assert collop.pair_mode(sym_ws, 1000) == (True, 2, 4)
+    assert collop.pair_mode(sym_ws, 65536) == (True, 4, 128)
+    assert collop.pair_mode(sym_ws, 262144) == (True, 8, 256)
+    assert collop.pair_mode(sym_ws, 300000) == (False, 0, 7)    # beyond 256 slots: ordered
+    collop.set_pair_mode(sym_ws, True, 2)
+    assert collop.pair_mode(sym_ws, 65536) == (True, 2, 256)
+    collop.set_pair_mode(sym_ws, False)
+    assert collop.pair_mode(sym_ws, 65536) == (False, 0, 32)
+    collop.set_pair_tile(sym_ws, 2, 16, 1)
+    assert collop.pair_mode(sym_ws, 65536) == (False, 0, 16)
+    collop.set_pair_tile(sym_ws, 1, 0, 1)
