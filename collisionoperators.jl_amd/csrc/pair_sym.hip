@@ -80,7 +80,16 @@ template <int R, bool METRIC, bool NEAR>
 __global__ __launch_bounds__(64 * R) void k_pair_sym_f64(const Src64* __restrict__ src, int N, long long n_off,
                                                          double* __restrict__ part, NearOut near) {
   constexpr int E = 128 * R, NTL = 2 * R, C = METRIC ? 5 : 2;
-  __shared__ double L[5][E];     // sources of super-row b: v1, v2, G1, G2, w_masked
+  // sources of super-row b by tile: (v1, v2), (G1, G2), w_masked of its 64
+  // sources, each stored twice in a row so that source (i + k) mod 64 sits at
+  // offset i + k and the fully unrolled step loop reads it with two 16-byte and
+  // one 8-byte LDS load at immediate offsets from one address
+  struct Tile {
+    double2 v[128];
+    double2 g[128];
+    double w[128];
+  };
+  __shared__ Tile L[NTL];
   __shared__ double Lacc[C][E];  // their accumulators
   __shared__ int Lin[NEAR ? E : 1];
   int a, b;
@@ -155,18 +164,22 @@ __global__ __launch_bounds__(64 * R) void k_pair_sym_f64(const Src64* __restrict
       w = s.w;
       in = s.in_domain != 0.0;
     }
-    L[0][e] = x1;
-    L[1][e] = x2;
-    L[2][e] = y1;
-    L[3][e] = y2;
-    L[4][e] = w;
+    Tile& t = L[e >> 6];
+    const int j = e & 63;
+    t.v[j] = t.v[j + 64] = make_double2(x1, x2);
+    t.g[j] = t.g[j + 64] = make_double2(y1, y2);
+    t.w[j] = t.w[j + 64] = w;
     if constexpr (NEAR) Lin[e] = in;
 #pragma unroll
     for (int c = 0; c < C; ++c) Lacc[c][e] = 0.0;
   }
   __syncthreads();
   for (int p = 0; p < NTL; ++p) {
-    const int base = ((2 * r + p) % NTL) * 64;
+    const int tile = (2 * r + p) % NTL;
+    const int base = tile * 64;
+    const double2* pv = &L[tile].v[lane];
+    const double2* pg = &L[tile].g[lane];
+    const double* pw = &L[tile].w[lane];
     // step k: lane i meets source base + (i + k) mod 64 and holds its accumulator
     double b1 = Lacc[0][base + lane], b2 = Lacc[1][base + lane];
     double n11 = 0.0, n12 = 0.0, n22 = 0.0;
@@ -175,10 +188,10 @@ __global__ __launch_bounds__(64 * R) void k_pair_sym_f64(const Src64* __restrict
       n12 = Lacc[3][base + lane];
       n22 = Lacc[4][base + lane];
     }
-#pragma unroll 4
-    for (int k = 0; k < 64; ++k) {
-      const int ia = base + ((lane + k) & 63);
-      const double av1 = L[0][ia], av2 = L[1][ia], ag1 = L[2][ia], ag2 = L[3][ia], aw = L[4][ia];
+    auto step = [&](int k) {
+      const int ia = base + ((lane + k) & 63);  // source index (near pairs only)
+      const double2 av = pv[k], ag = pg[k];
+      const double av1 = av.x, av2 = av.y, ag1 = ag.x, ag2 = ag.y, aw = pw[k];
 #pragma unroll
       for (int q = 0; q < 2; ++q) {
         const double d1 = gv1[q] - av1;
@@ -226,6 +239,13 @@ __global__ __launch_bounds__(64 * R) void k_pair_sym_f64(const Src64* __restrict
         n12 = rot1(n12);
         n22 = rot1(n22);
       }
+    };
+    if constexpr (NEAR) {  // runs once per step of the solver: keep the code small
+#pragma unroll 2
+      for (int k = 0; k < 64; ++k) step(k);
+    } else {
+#pragma unroll
+      for (int k = 0; k < 64; ++k) step(k);
     }
     // after 64 rotations lane i holds source base + i again
     Lacc[0][base + lane] = b1;
