@@ -102,28 +102,52 @@ __device__ __forceinline__ void scatter_body(const MeshView& m, unsigned long lo
 // Exact sum of the fixed-point partial rows of one field, back to FP64.
 __device__ __forceinline__ double rhs_of(const long long* __restrict__ P, int nblocks, int nd, int k,
                                          double inv_scale) {
-  unsigned long long s = 0ull;  // two's-complement wrap-around is exact, in any order
-#pragma unroll 8
-  for (int b = 0; b < nblocks; ++b) s += (unsigned long long)P[(size_t)b * nd + k];
-  return (double)(long long)s * inv_scale;
+  // two's-complement wrap-around is exact, in any order: four independent
+  // partial sums keep 32 loads in flight per thread
+  unsigned long long s0 = 0ull, s1 = 0ull, s2 = 0ull, s3 = 0ull;
+  int b = 0;
+  for (; b + 32 <= nblocks; b += 32) {
+#pragma unroll
+    for (int j = 0; j < 32; j += 4) {
+      s0 += (unsigned long long)P[(size_t)(b + j) * nd + k];
+      s1 += (unsigned long long)P[(size_t)(b + j + 1) * nd + k];
+      s2 += (unsigned long long)P[(size_t)(b + j + 2) * nd + k];
+      s3 += (unsigned long long)P[(size_t)(b + j + 3) * nd + k];
+    }
+  }
+  for (; b < nblocks; ++b) s0 += (unsigned long long)P[(size_t)b * nd + k];
+  return (double)(long long)(s0 + s1 + s2 + s3) * inv_scale;
 }
 
 // rhs (exact row sum, or `direct`) -> M^{-1} rhs -> out.  One workgroup of
 // kSolveThreads.  M = M1 (x) M2, so the coefficient array (i1 fastest) is
 // C = M1^{-1} B M2^{-1}: two small dense products with the precomputed
 // symmetric 1D inverses, phase one held in registers across a barrier so B
-// and M1^{-1} B share one LDS array of n_dofs doubles.
+// and M1^{-1} B share one LDS array of n_dofs doubles.  The inverses are read
+// from LDS copies when `stage` (they fit beside X), else from global memory.
 constexpr int kSolveThreads = 1024;
 constexpr int kSolvePer = 8;  // n_dofs <= kSolvePer * kSolveThreads
 
-__device__ __forceinline__ void solve_body(const MeshView& m, double* X, const long long* __restrict__ P, int nblocks,
-                                           double inv_scale, const double* __restrict__ direct,
-                                           double* __restrict__ out) {
+// LDS doubles of a solve workgroup: X, and the two inverses when staged.
+__host__ __device__ inline int solve_stage(const MeshView& m) {
+  return (m.n_dofs + m.nd1 * m.nd1 + m.nd2 * m.nd2) * 8 <= 48 * 1024;
+}
+inline size_t solve_lds(const MeshView& m) {
+  return (size_t)(m.n_dofs + (solve_stage(m) ? m.nd1 * m.nd1 + m.nd2 * m.nd2 : 0)) * sizeof(double);
+}
+
+// X (LDS, n_dofs) holds B on entry; out <- M1^{-1} B M2^{-1}.
+__device__ __forceinline__ void solve_core(const MeshView& m, double* X, double* __restrict__ out) {
   const int nd = m.n_dofs, nd1 = m.nd1, nd2 = m.nd2;
-  if (P) {
-    for (int k = threadIdx.x; k < nd; k += blockDim.x) X[k] = rhs_of(P, nblocks, nd, k, inv_scale);
-  } else {
-    for (int k = threadIdx.x; k < nd; k += blockDim.x) X[k] = direct[k];
+  const double* I1 = m.minv1;
+  const double* I2 = m.minv2;
+  if (solve_stage(m)) {  // X is followed by room for both inverses
+    double* L1 = X + nd;
+    double* L2 = L1 + nd1 * nd1;
+    for (int k = threadIdx.x; k < nd1 * nd1; k += blockDim.x) L1[k] = I1[k];
+    for (int k = threadIdx.x; k < nd2 * nd2; k += blockDim.x) L2[k] = I2[k];
+    I1 = L1;
+    I2 = L2;
   }
   __syncthreads();
   double t[kSolvePer];
@@ -133,7 +157,7 @@ __device__ __forceinline__ void solve_body(const MeshView& m, double* X, const l
     t[q] = 0.0;
     if (k < nd) {
       const int i1 = k % nd1, i2 = k - (k % nd1);
-      const double* row = m.minv1 + (size_t)i1 * nd1;
+      const double* row = I1 + (size_t)i1 * nd1;
       double s = 0.0;
       for (int j = 0; j < nd1; ++j) s += row[j] * X[i2 + j];
       t[q] = s;
@@ -148,11 +172,23 @@ __device__ __forceinline__ void solve_body(const MeshView& m, double* X, const l
   __syncthreads();
   for (int k = threadIdx.x; k < nd; k += blockDim.x) {
     const int i1 = k % nd1, i2 = k / nd1;
-    const double* col = m.minv2 + (size_t)i2 * nd2;
+    const double* col = I2 + (size_t)i2 * nd2;
     double s = 0.0;
     for (int j = 0; j < nd2; ++j) s += X[i1 + (size_t)nd1 * j] * col[j];
     out[k] = s;
   }
+}
+
+__device__ __forceinline__ void solve_body(const MeshView& m, double* X, const long long* __restrict__ P, int nblocks,
+                                           double inv_scale, const double* __restrict__ direct,
+                                           double* __restrict__ out) {
+  const int nd = m.n_dofs;
+  if (P) {
+    for (int k = threadIdx.x; k < nd; k += blockDim.x) X[k] = rhs_of(P, nblocks, nd, k, inv_scale);
+  } else {
+    for (int k = threadIdx.x; k < nd; k += blockDim.x) X[k] = direct[k];
+  }
+  solve_core(m, X, out);
 }
 
 // Quadrature pass 1 of one block (one thread per cell and Gauss point).
@@ -320,7 +356,7 @@ __global__ __launch_bounds__(kScatterThreads) void k_scatter(MeshView m, const d
                blockIdx.x, gridDim.x);
 }
 
-// One workgroup per field.  X lives in dynamic LDS (n_dofs doubles).
+// One workgroup per field.  X lives in dynamic LDS (solve_lds).
 __global__ __launch_bounds__(kSolveThreads) void k_reduce_solve(MeshView m, const long long* __restrict__ part,
                                                                 int nblocks, double inv_scale,
                                                                 const double* __restrict__ direct,
@@ -483,7 +519,7 @@ void launch_scatter(const MeshView& m, const double* v0, const double* vin, cons
 
 void launch_reduce_solve(const MeshView& m, const long long* part, int nblocks, double w_abs_sum,
                          const double* direct, int fields, double* out, hipStream_t st) {
-  const size_t lds = (size_t)m.n_dofs * sizeof(double);
+  const size_t lds = solve_lds(m);
   hipLaunchKernelGGL(k_reduce_solve, dim3(fields), dim3(kSolveThreads), lds, st, m, part, nblocks,
                      1.0 / scatter_scale(w_abs_sum), direct, out);
 }
@@ -535,7 +571,7 @@ void launch_scatter_b(const MeshView& m, const EnsView& E, const double* v0, con
 
 void launch_solve_b(const MeshView& m, const EnsView& E, const long long* part, int fields, int in_slot, int out_slot,
                     double* vec, double* fc, hipStream_t st) {
-  const size_t lds = (size_t)m.n_dofs * sizeof(double);
+  const size_t lds = solve_lds(m);
   hipLaunchKernelGGL(k_solve_b, dim3(E.R, fields), dim3(kSolveThreads), lds, st, m, E, part, in_slot, out_slot, vec,
                      fc);
 }

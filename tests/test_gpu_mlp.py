@@ -7,6 +7,7 @@ import pytest
 
 import collop
 import mlp_ref as M
+import oracle as O
 from conftest import uniform_bp
 
 pytestmark = pytest.mark.gpu
@@ -110,11 +111,25 @@ def test_trained_warm_start_gain(gpu_ctx):
           f"(ratio {warm['mean_r0'] / cold['mean_r0']:.3f}); maps cold {cold['mean_maps']:.1f} warm {warm['mean_maps']:.1f}")
     assert cold["all_converged"] and warm["all_converged"]
     assert warm["mean_r0"] < cold["mean_r0"]
-    # one step from the same state: the same root (over several steps the
-    # trajectories may settle on different roots of later steps' systems)
+    # one step from the same state: both results are roots of the oracle's
+    # implicit system to the solver tolerance.  They need not be the same
+    # markers: this system (N = 4096, dt = 0.005) has states ~1e-5 apart whose
+    # residuals both meet the tolerance (several roots, or a nearly neutral
+    # direction of the map), and which one the solver reaches depends on the
+    # initial guess
     c1 = W.evaluate(ws, v, w, p.step_params(warmstart=collop.WARM_EULER), 1)
     w1 = W.evaluate(ws, v, w, p.step_params(warmstart=collop.WARM_MLP), 1)
-    assert np.max(np.abs(w1["markers"] - c1["markers"])) <= 1e-8
+    assert c1["all_converged"] and w1["all_converged"]
+    m = O.Mesh(p.bp1, p.bp2)
+    fc = m.l2_project(v, w)
+    S0 = m.entropy(fc)
+    P = O.default_params(p.DT, m=5, nthreads=0)
+    for res in (c1, w1):
+        x = res["markers"]
+        tol = max(1e-12 * np.linalg.norm(x), 1e-10)
+        assert np.linalg.norm(m.picard_map(P, x, v, w, S0) - x) <= 20 * tol
+    diff = np.max(np.abs(w1["markers"] - c1["markers"]), axis=1)
+    print(f"one step: max |dv| {diff.max():.2e}, markers beyond 1e-8: {int((diff > 1e-8).sum())}")
 
 
 def test_packaged_model_loads_and_steps(gpu_ctx):
