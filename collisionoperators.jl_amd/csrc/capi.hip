@@ -825,7 +825,9 @@ int outer_map_dev(collop_ctx* ctx, const collop_step_params& p, int kind, bool f
     return epilogue_dev(ctx, P<double>(ctx->s_FA), 1, P<double>(ctx->s_v), p.dt, P<double>(ctx->s_v1),
                         P<double>(ctx->s_Gvo), -1, 0);
   }
-  NearOut near{p.dc_near_theta * p.dc_near_theta, P<int>(ctx->b_near_count), P<int2>(ctx->b_near_list), kNearCap};
+  const double th2 = p.dc_near_theta * p.dc_near_theta;
+  NearOut near{th2, P<int>(ctx->b_near_count), P<int2>(ctx->b_near_list), kNearCap,
+               th2 > 0.0 ? ctx->w_max * ctx->w_max / th2 : 0.0};
   if (find_near) HIPC(hipMemsetAsync(ctx->b_near_count.p, 0, sizeof(int), ctx->stream));
   prof_begin(ctx, 2 + kind);
   if (kind == 0)
@@ -1282,11 +1284,12 @@ int collop_state_set(collop_ctx* ctx, const double* v, const double* w, int64_t 
   if ((r = check_N(ctx, N))) return r;
   if (N < 2) return fail(ctx, COLLOP_ERR_INVALID, "the implicit step needs N >= 2 markers");
   if (!v || !w) return fail(ctx, COLLOP_ERR_INVALID, "null array");
-  double wsum = 0.0;
+  double wsum = 0.0, wmax = 0.0;
   for (int64_t i = 0; i < N; ++i) {
     if (!(w[i] > 0.0) || !std::isfinite(w[i]))
       return fail(ctx, COLLOP_ERR_INVALID, "weights must be positive and finite (G_eff divides by w)");
     wsum += w[i];
+    wmax = std::max(wmax, w[i]);
   }
   HIPC(hipSetDevice(ctx->device));
   if ((r = ensure_scratch(ctx, N))) return r;
@@ -1309,6 +1312,7 @@ int collop_state_set(collop_ctx* ctx, const double* v, const double* w, int64_t 
   ENSURE(ctx->s_fc, (size_t)ctx->mv.n_dofs * 8);
   ctx->N = N;
   ctx->w_abs_sum = wsum;
+  ctx->w_max = wmax;
   choose_tile(ctx, (int)N);
   HIPC(hipMemcpyAsync(ctx->s_v.p, v, n2b, hipMemcpyHostToDevice, ctx->stream));
   HIPC(hipMemcpyAsync(ctx->s_w.p, w, (size_t)N * 8, hipMemcpyHostToDevice, ctx->stream));

@@ -41,6 +41,7 @@ struct collop_ctx {
   bool has_state = false;
   int64_t N = 0;
   double w_abs_sum = 0.0;  // sum |w| of the state (fixed-point scale of the scatter)
+  double w_max = 0.0;      // max w of the state (near-pair prefilter)
   DBuf s_v, s_w, s_v1, s_Gv, s_r, s_rp, s_Gp, s_best, s_dF, s_dG, s_fc, s_dotv;
   DBuf s_Gvo, s_Gk, s_FA, s_D;  // defect correction: G(v_k), G_eff(v_k), [F, A], A (G_eff - G_k)
   DBuf s_Ge, s_vmk;             // G_eff and v_mid of the inner iterate / of v_k (near field)

@@ -63,12 +63,14 @@ struct EnsView {
 // Near-pair detection during a metric pass: every ordered in-domain pair with
 // w_a / |d| > theta (th2 = theta^2) is appended to list[] as (gamma, alpha);
 // count receives the number found (entries beyond cap are dropped, the count
-// still grows).
+// still grows).  qmax = max(w)^2 / theta^2: no pair with |d|^2 >= qmax is near
+// in either direction, so the kernels test that first.
 struct NearOut {
   double th2;
   int* count;
   int2* list;
   int cap;
+  double qmax;
 };
 constexpr int kNearCap = 1 << 16;  // near pairs kept per step (beyond it the near field is skipped)
 struct PairTile {

@@ -68,7 +68,7 @@ __global__ __launch_bounds__(kThreads) void k_pair_f64(const Src64* __restrict__
       }
       if (NEAR) {  // near pair: w_a / |d| > theta
         const double q = __builtin_fma(d2, d2, d1 * d1);
-        if (act[k] && q >= 1e-24 && near.th2 * q < aw * aw) {
+        if (q < near.qmax && act[k] && q >= 1e-24 && near.th2 * q < aw * aw) {
           const int g = gblk * (kThreads * K) + k * kThreads + (int)threadIdx.x;
           const int idx = atomicAdd(near.count, 1);
           if (idx < near.cap) near.list[idx] = make_int2(g, a);
@@ -138,7 +138,7 @@ __global__ __launch_bounds__(kThreads) void k_pair_f32c(const Src32* __restrict_
           c22[k] = __builtin_fmaf(td1, d1, c22[k]);
           if (NEAR) {  // near pair: w_a / |d| > theta
             const float q = __builtin_fmaf(d2, d2, d1 * d1);
-            if (act[k] && q >= 1e-12f && (float)near.th2 * q < aw * aw) {
+            if (q < (float)near.qmax && act[k] && q >= 1e-12f && (float)near.th2 * q < aw * aw) {
               const int g = gblk * (kThreads * K) + k * kThreads + (int)threadIdx.x;
               const int idx = atomicAdd(near.count, 1);
               if (idx < near.cap) near.list[idx] = make_int2(g, a);

@@ -146,7 +146,7 @@ __global__ __launch_bounds__(64 * R) void k_pair_sym_f64(const Src64* __restrict
         }
         if constexpr (NEAR) {
           const double qd = __builtin_fma(d2, d2, d1 * d1);
-          if (gin[q] && qd >= 1e-24 && near.th2 * qd < aw * aw) near_push(near, gm[q], s);
+          if (qd < near.qmax && gin[q] && qd >= 1e-24 && near.th2 * qd < aw * aw) near_push(near, gm[q], s);
         }
       }
     }
@@ -237,7 +237,7 @@ __global__ __launch_bounds__(64 * R) void k_pair_sym_f64(const Src64* __restrict
           n22 = __builtin_fma(sa, u22, n22);
         }
         if constexpr (NEAR) {  // near pairs w_source / |d| > theta, both directions
-          if (qd >= 1e-24) {
+          if (qd < near.qmax && qd >= 1e-24) {
             const int ma = b * E + ia;
             if (gin[q] && near.th2 * qd < aw * aw) near_push(near, gm[q], ma);
             if (Lin[ia] && near.th2 * qd < gw[q] * gw[q]) near_push(near, ma, gm[q]);
@@ -345,7 +345,8 @@ __global__ __launch_bounds__(64 * R) void k_pair_sym_f32c(const Src32* __restric
             bs[q][4] = __builtin_fmaf(td1, d1, bs[q][4]);
             if constexpr (NEAR) {
               const float qd = __builtin_fmaf(d2, d2, d1 * d1);
-              if (gin[q] && qd >= 1e-12f && (float)near.th2 * qd < aw * aw) near_push(near, gm[q], s);
+              if (qd < (float)near.qmax && gin[q] && qd >= 1e-12f && (float)near.th2 * qd < aw * aw)
+                near_push(near, gm[q], s);
             }
           } else {
             pair_term_f32(gv1[q], gv2[q], gg1[q], gg2[q], av1, av2, ag1, ag2, aw, bs[q][0], bs[q][1]);
@@ -434,7 +435,7 @@ __global__ __launch_bounds__(64 * R) void k_pair_sym_f32c(const Src32* __restric
             ab[4] = __builtin_fmaf(sa, u22, ab[4]);
           }
           if constexpr (NEAR) {
-            if (qd >= 1e-12f) {
+            if (qd < (float)near.qmax && qd >= 1e-12f) {
               const int ma = b * E + ia;
               const float th2 = (float)near.th2;
               if (gin[q] && th2 * qd < aw * aw) near_push(near, gm[q], ma);
