@@ -81,7 +81,11 @@ def pair_kernels(tiles: int):
 # profiles/pair_sum_bimodal_n65536.md.
 PAIR_TRAFFIC_BYTES = {"k_pair_f64<1,1,false>": 68646 * 1024, "k_pair_f32c<1,32,false>": 50730 * 1024,
                       "k_pair_f64<1,1,true> (pair sum + metric)": 117802 * 1024,
-                      "k_pair_f32c<1,32,true> (pair sum + metric)": 99890 * 1024}
+                      "k_pair_f32c<1,32,true> (pair sum + metric)": 99890 * 1024,
+                      "k_pair_sym_f64<4,false,false>": 233132 * 1024,
+                      "k_pair_sym_f32c<4,false,false>": 161728 * 1024,
+                      "k_pair_sym_f64<4,true,false> (pair sum + metric)": 485372 * 1024,
+                      "k_pair_sym_f32c<4,true,false> (pair sum + metric)": 397245 * 1024}
 
 
 def log(msg: str):
