@@ -327,8 +327,9 @@ def ensemble_bench(args):
     """Batched ensemble: 48 runs, N log-uniform in [2048, 65536], random IC,
     shared 32x32 mesh, FP32c pair sums (FP64 finish), dt 0.02.  Times the
     batched step (one persistent pair launch per iteration, hipGraph replay)
-    and, for comparison, the same runs stepped one after another by the
-    single-run path."""
+    with the symmetric pair sum (`value`) and with the ordered one, and, for
+    comparison, the same runs stepped one after another by the single-run path
+    (symmetric pair sum)."""
     import collop
     runs = collop.ensemble_markers(48, seed=7)
     ws = collop.Workspace(np.linspace(-6, 6, 33), np.linspace(-6, 6, 33), device=0)
@@ -336,43 +337,53 @@ def ensemble_bench(args):
     n = np.array([r[3] for r in runs])
     pairs = float((n.astype(float) * (n - 1)).sum())
     log(f"[ensemble] {len(runs)} runs, N in [{n.min()}, {n.max()}], sum N {n.sum()}, {pairs:.3e} pairs per iteration")
-    ens = collop.Ensemble(ws, [(v, w) for v, w, _, _ in runs])
-    t0 = time.perf_counter()
-    bt, waves, ms = ens.tune()
-    log(f"[ensemble] autotuner ({time.perf_counter() - t0:.1f} s): gamma tile {bt}, waves/CU {waves}; "
-        f"FP32c+FP64 pair launch ms rows gamma 64..512, cols waves 4..32:\n{np.array2string(ms, precision=3)}")
-    for i in range(args.warmup):
-        st = ens.step(p, 1)[0]
-        log(f"  warmup step {i}: max iters {max(s['iters'] for s in st)} {st[0]['seconds'] * 1e3:.0f} ms")
-    t0 = time.perf_counter()
-    stats = ens.step(p, args.steps)
-    el = time.perf_counter() - t0
-    for i, st in enumerate(stats):
-        log(f"  step {i}: iters mean {np.mean([s['iters'] for s in st]):.1f} max {max(s['iters'] for s in st)} "
-            f"(f32 mean {np.mean([s['iters_f32'] for s in st]):.1f}) {st[0]['seconds'] * 1e3:.0f} ms")
-    conv = all(s["exit_reason"] == 0 for st in stats for s in st)
-    batched = len(runs) * args.steps / el
-    # the same runs stepped one after another (single-run path), a bounded sample of steps
+    res = {}
+    for form, symmetric in (("symmetric", True), ("ordered", False)):
+        collop.set_pair_mode(ws, symmetric, 4 if symmetric else 0)
+        ens = collop.Ensemble(ws, [(v, w) for v, w, _, _ in runs])
+        t0 = time.perf_counter()
+        bt, waves, ms = ens.tune()
+        log(f"[ensemble {form}] autotuner ({time.perf_counter() - t0:.1f} s): gamma tile {bt}, waves/CU {waves}; "
+            f"FP32c+FP64 pair launch ms rows gamma 64..512 (symmetric: row 0 only), cols waves 4..32:\n"
+            f"{np.array2string(ms, precision=3)}")
+        for i in range(args.warmup):
+            st = ens.step(p, 1)[0]
+            log(f"  warmup step {i}: max iters {max(s['iters'] for s in st)} {st[0]['seconds'] * 1e3:.0f} ms")
+        t0 = time.perf_counter()
+        stats = ens.step(p, args.steps)
+        el = time.perf_counter() - t0
+        for i, st in enumerate(stats):
+            log(f"  step {i}: iters mean {np.mean([s['iters'] for s in st]):.1f} max {max(s['iters'] for s in st)} "
+                f"(f32 mean {np.mean([s['iters_f32'] for s in st]):.1f}) {st[0]['seconds'] * 1e3:.0f} ms")
+        res[form] = dict(run_steps_per_s=len(runs) * args.steps / el, el=el, gamma_tile=bt, waves_per_cu=waves,
+                         tuner_ms=ms.tolist(), all_converged=all(s["exit_reason"] == 0 for st in stats for s in st))
+        log(f"[ensemble {form}] {res[form]['run_steps_per_s']:.2f} run-steps/s")
+    # the same runs stepped one after another (single-run path, symmetric pair
+    # sum), a bounded sample of steps
+    collop.set_pair_mode(ws, True)
     seq_steps = max(1, min(args.steps, 2))
     t0 = time.perf_counter()
     for v, w, _, nn in runs:
-        collop.set_pair_tile(ws, 1, collop.Ensemble.slices(nn), 1)
         sim = collop.Simulation(ws, v, w)
         sim.run(p, seq_steps)
     seq_el = time.perf_counter() - t0
-    collop.set_pair_tile(ws, 1, 0, 1)
     sequential = len(runs) * seq_steps / seq_el
+    sym = res["symmetric"]
     out = {"metric": "ensemble run-steps/s (48 runs, N log-uniform [2048, 65536], FP32c + FP64 finish, dt 0.02)",
-           "value": round(batched, 4), "unit": "run-steps/s", "n_gpus": 1, "steps": args.steps,
-           "warmup": args.warmup, "ms_per_step": round(1e3 * el / args.steps, 3), "higher_is_better": True,
+           "value": round(sym["run_steps_per_s"], 4), "unit": "run-steps/s", "n_gpus": 1, "steps": args.steps,
+           "warmup": args.warmup, "ms_per_step": round(1e3 * sym["el"] / args.steps, 3), "higher_is_better": True,
            "scaling": "weak", "vs_baseline": None, "dtype": "f32c+f64",
            "data": "synthetic (seeded: N, IC and markers from PCG64 seed 7)",
            "config": {"workload": "ensemble48", "runs": len(runs), "sum_N": int(n.sum()), "N_min": int(n.min()),
-                      "N_max": int(n.max()), "mesh": "32x32", "dt": p.dt},
-           "pairs_per_iteration": pairs, "gamma_tile": bt, "waves_per_cu": waves,
-           "all_converged": conv, "sequential_run_steps_per_s": round(sequential, 4),
-           "batched_vs_sequential": round(batched / sequential, 3),
-           "tuner_ms": ms.tolist()}
+                      "N_max": int(n.max()), "mesh": "32x32", "dt": p.dt, "pair_sum": "symmetric, 4 waves per block"},
+           "pairs_per_iteration": pairs, "waves_per_cu": sym["waves_per_cu"],
+           "all_converged": sym["all_converged"] and res["ordered"]["all_converged"],
+           "ordered_pair_sum": {"run_steps_per_s": round(res["ordered"]["run_steps_per_s"], 4),
+                                "gamma_tile": res["ordered"]["gamma_tile"],
+                                "waves_per_cu": res["ordered"]["waves_per_cu"]},
+           "sequential_run_steps_per_s": round(sequential, 4),
+           "batched_vs_sequential": round(sym["run_steps_per_s"] / sequential, 3),
+           "tuner_ms": {"symmetric": sym["tuner_ms"][0], "ordered": res["ordered"]["tuner_ms"]}}
     print(json.dumps(out), flush=True)
 
 

@@ -224,13 +224,17 @@ void launch_copy_b(const EnsView& E, const double* src, double* dst, hipStream_t
 // over the list.  force_f64: every run, FP64 (predictor); otherwise the runs
 // active in E.ctl, in their own precision.
 struct EnsTiles {
-  const int4* tiles;  // {run, first target, slice, -}
+  const int4* tiles;  // ordered: {run, first target, slice, -}; symmetric: {run, a, b, -}
   int ntiles;
   int bt;
   int grid;
+  int sym;  // 0: ordered tiles; R: symmetric blocks of R waves
 };
 void launch_pack_b(const EnsView& E, const double* v, const double* G, const double* w, double lo1, double hi1,
                    double lo2, double hi2, Src64* s64, Src32* s32, hipStream_t st);
+// the symmetric blocks of T (T.sym != 0) over every active run
+void launch_pair_sym_b(const EnsView& E, const EnsTiles& T, const Src64* s64, const Src32* s32, int force_f64,
+                       double* part, hipStream_t st);
 void launch_pair_b(const EnsView& E, const EnsTiles& T, const Src64* s64, const Src32* s32, int force_f64,
                    double* part, hipStream_t st);
 

@@ -47,6 +47,7 @@ struct EnsembleState {
   double* h_stat = nullptr;  // pinned [R][kStatWidth]
   double* h_scal = nullptr;  // pinned [R][8]
   int ntiles = 0, bt = 256, waves = 8, n_cu = 256;
+  int sym = 0;  // symmetric pair sum with sym waves per block (0: ordered tiles)
   bool use_graph = true;
   hipGraph_t graph = nullptr;
   hipGraphExec_t exec = nullptr;
@@ -114,12 +115,31 @@ EnsBufs bufs(EnsembleState* E) {
 
 const int* rows(EnsembleState* E, int kind) { return P<int>(E->d_rows) + (size_t)kind * E->R; }
 
-// Tile list of the persistent pair sum: runs in decreasing slice length
-// (longest-processing-time order), each run's tiles in (target block, slice)
-// order.  All tiles of a run cost the same: bt targets x one slice of sources.
+// Tile list of the persistent pair sum.  Ordered form: runs in decreasing
+// slice length (longest-processing-time order), each run's tiles in (target
+// block, slice) order; all tiles of a run cost the same (bt targets x one
+// slice of sources).  Symmetric form: the off-diagonal blocks (run, a, b) of
+// every run, largest runs first, then the diagonal blocks; every block but a
+// run's last super-row costs the same.
 int build_tiles(collop_ctx* ctx, EnsembleState* E, int bt) {
   std::vector<int> order(E->R);
   std::iota(order.begin(), order.end(), 0);
+  if (E->sym) {
+    std::stable_sort(order.begin(), order.end(), [&](int a, int b) { return E->runs[a].n > E->runs[b].n; });
+    std::vector<int4> t;
+    for (int r : order) {
+      const int S = sym_rows(E->runs[r].n, E->sym);
+      for (int b = 1; b < S; ++b)
+        for (int a = 0; a < b; ++a) t.push_back(make_int4(r, a, b, 0));
+    }
+    for (int r : order)
+      for (int a = 0; a < sym_rows(E->runs[r].n, E->sym); ++a) t.push_back(make_int4(r, a, a, 0));
+    ENSURE(E->tiles, t.size() * sizeof(int4));
+    HIPC(hipMemcpy(E->tiles.p, t.data(), t.size() * sizeof(int4), hipMemcpyHostToDevice));
+    E->ntiles = (int)t.size();
+    E->bt = bt;
+    return COLLOP_OK;
+  }
   auto chunk = [&](int r) { return (E->runs[r].n + E->runs[r].S - 1) / E->runs[r].S; };
   std::stable_sort(order.begin(), order.end(), [&](int a, int b) { return chunk(a) > chunk(b); });
   std::vector<int4> t;
@@ -140,7 +160,8 @@ EnsTiles tiles(EnsembleState* E) {
   T.tiles = P<int4>(E->tiles);
   T.ntiles = E->ntiles;
   T.bt = E->bt;
-  T.grid = std::max(1, E->waves * E->n_cu / (E->bt / 64));
+  T.sym = E->sym;
+  T.grid = std::max(1, E->waves * E->n_cu / (E->sym ? E->sym : E->bt / 64));
   return T;
 }
 
@@ -468,6 +489,13 @@ int collop_ensemble_set(collop_ctx* ctx, int n_runs, const int64_t* n, const dou
   const MeshView& m = ctx->mv;
   E->R = n_runs;
   E->NT = (int)padded;
+  // pair-sum form: the context's; symmetric with its tiles (or 4) when every
+  // run fits in kSymMaxSlots partial slots
+  if (ctx->pair_sym) {
+    E->sym = ctx->sym_tiles ? ctx->sym_tiles : 4;
+    for (int r = 0; r < n_runs; ++r)
+      if (sym_rows((int)n[r], E->sym) > kSymMaxSlots) E->sym = 0;
+  }
   E->runs.resize(n_runs);
   std::vector<int2> sb, gb, pb, eb;
   std::vector<int> rowtab((size_t)kRowsKinds * n_runs);
@@ -478,7 +506,7 @@ int collop_ensemble_set(collop_ctx* ctx, int n_runs, const int64_t* n, const dou
     EnsRun& q = E->runs[r];
     q.n = (int)n[r];
     q.off = off;
-    q.S = slices_for(n[r]);
+    q.S = E->sym ? sym_rows(q.n, E->sym) : slices_for(n[r]);
     q.sblk0 = (int)sb.size();
     q.nsb = scatter_blocks(q.n);
     for (int b = 0; b < q.nsb; ++b) sb.push_back(make_int2(r, b));
@@ -645,7 +673,10 @@ int collop_ensemble_tune(collop_ctx* ctx, int* gamma_tile, int* waves_per_cu, do
   const int bts[4] = {64, 128, 256, 512}, ws[4] = {4, 8, 16, 32};
   double best = 1e300;
   int best_bt = E->bt, best_w = E->waves, idx = 0;
+  if (ms)
+    for (int i = 0; i < 16; ++i) ms[i] = 0.0;
   for (int bt : bts) {
+    if (E->sym && bt != bts[0]) break;  // symmetric blocks have no gamma tile: waves only
     if ((r = build_tiles(ctx, E, bt))) return r;
     for (int wv : ws) {
       E->waves = wv;
@@ -662,7 +693,7 @@ int collop_ensemble_tune(collop_ctx* ctx, int* gamma_tile, int* waves_per_cu, do
   }
   E->waves = best_w;
   if ((r = build_tiles(ctx, E, best_bt))) return r;
-  if (gamma_tile) *gamma_tile = best_bt;
+  if (gamma_tile) *gamma_tile = E->sym ? 0 : best_bt;
   if (waves_per_cu) *waves_per_cu = best_w;
   return COLLOP_OK;
 }

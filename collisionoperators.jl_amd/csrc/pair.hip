@@ -403,6 +403,7 @@ void launch_pack_b(const EnsView& E, const double* v, const double* G, const dou
 void launch_pair_b(const EnsView& E, const EnsTiles& T, const Src64* s64, const Src32* s32, int force_f64,
                    double* part, hipStream_t st) {
   if (T.ntiles <= 0) return;
+  if (T.sym) return launch_pair_sym_b(E, T, s64, s32, force_f64, part, st);
   const dim3 grid(T.grid);
   switch (T.bt) {
     case 64:

@@ -89,24 +89,33 @@ __device__ __forceinline__ void near_push(const NearOut& near, int g, int a) {
   if (idx < near.cap) near.list[idx] = make_int2(g, a);
 }
 
+// LDS of one FP64 block: the sources of super-row b by tile -- (v1, v2),
+// (G1, G2), w_masked of its 64 sources, each stored twice in a row so that
+// source (i + k) mod 64 sits at offset i + k and the fully unrolled step loop
+// reads it with two 16-byte and one 8-byte LDS load at immediate offsets from
+// one address -- and their accumulators.
 template <int R, bool METRIC, bool NEAR>
-__global__ __launch_bounds__(64 * R) void k_pair_sym_f64(const Src64* __restrict__ src, int N, long long n_off,
-                                                         double* __restrict__ part, NearOut near) {
-  constexpr int E = 128 * R, NTL = 2 * R, C = METRIC ? 5 : 2;
-  // sources of super-row b by tile: (v1, v2), (G1, G2), w_masked of its 64
-  // sources, each stored twice in a row so that source (i + k) mod 64 sits at
-  // offset i + k and the fully unrolled step loop reads it with two 16-byte and
-  // one 8-byte LDS load at immediate offsets from one address
+struct SymLds64 {
   struct Tile {
     double2 v[128];
     double2 g[128];
     double w[128];
   };
-  __shared__ Tile L[NTL];
-  __shared__ double Lacc[C][E];  // their accumulators
-  __shared__ int Lin[NEAR ? E : 1];
-  int a, b;
-  block_rows(blockIdx.x, n_off, a, b);
+  Tile L[2 * R];
+  double Lacc[METRIC ? 5 : 2][128 * R];
+  int Lin[NEAR ? 128 * R : 1];
+};
+
+// Block (a, b) of the symmetric FP64 sum over src[0, N) into part[S][C][N].
+template <int R, bool METRIC, bool NEAR>
+__device__ __forceinline__ void sym_block_f64(const Src64* __restrict__ src, int N, int a, int b,
+                                              double* __restrict__ part, SymLds64<R, METRIC, NEAR>& S,
+                                              const NearOut& near) {
+  constexpr int E = 128 * R, NTL = 2 * R, C = METRIC ? 5 : 2;
+  using Tile = typename SymLds64<R, METRIC, NEAR>::Tile;
+  auto& L = S.L;
+  auto& Lacc = S.Lacc;
+  auto& Lin = S.Lin;
   const int lane = threadIdx.x & 63, r = threadIdx.x >> 6;
   const size_t cs = (size_t)C * N;
   double gv1[2], gv2[2], gg1[2], gg2[2], gw[2];
@@ -292,18 +301,34 @@ __global__ __launch_bounds__(64 * R) void k_pair_sym_f64(const Src64* __restrict
   }
 }
 
+template <int R, bool METRIC, bool NEAR>
+__global__ __launch_bounds__(64 * R) void k_pair_sym_f64(const Src64* __restrict__ src, int N, long long n_off,
+                                                         double* __restrict__ part, NearOut near) {
+  __shared__ SymLds64<R, METRIC, NEAR> S;
+  int a, b;
+  block_rows(blockIdx.x, n_off, a, b);
+  sym_block_f64<R, METRIC, NEAR>(src, N, a, b, part, S, near);
+}
+
 // FP32 terms, compensated sums: every 32 steps each accumulator's block sum is
 // folded into a (hi, lo) pair with TwoSum (the targets' in VGPRs, the sources'
 // in LDS), as in k_pair_f32c.
 template <int R, bool METRIC, bool NEAR>
-__global__ __launch_bounds__(64 * R) void k_pair_sym_f32c(const Src32* __restrict__ src, int N, long long n_off,
-                                                          double* __restrict__ part, NearOut near) {
+struct SymLds32 {
+  float L[5][128 * R];  // sources of super-row b: v1, v2, G1, G2, w_masked
+  float Lh[METRIC ? 5 : 2][128 * R], Ll[METRIC ? 5 : 2][128 * R];  // their (hi, lo) sums
+  int Lin[NEAR ? 128 * R : 1];
+};
+
+template <int R, bool METRIC, bool NEAR>
+__device__ __forceinline__ void sym_block_f32c(const Src32* __restrict__ src, int N, int a, int b,
+                                               double* __restrict__ part, SymLds32<R, METRIC, NEAR>& S,
+                                               const NearOut& near) {
   constexpr int E = 128 * R, NTL = 2 * R, C = METRIC ? 5 : 2;
-  __shared__ float L[5][E];
-  __shared__ float Lh[C][E], Ll[C][E];
-  __shared__ int Lin[NEAR ? E : 1];
-  int a, b;
-  block_rows(blockIdx.x, n_off, a, b);
+  auto& L = S.L;
+  auto& Lh = S.Lh;
+  auto& Ll = S.Ll;
+  auto& Lin = S.Lin;
   const int lane = threadIdx.x & 63, r = threadIdx.x >> 6;
   const size_t cs = (size_t)C * N;
   float gv1[2], gv2[2], gg1[2], gg2[2], gw[2];
@@ -482,6 +507,42 @@ __global__ __launch_bounds__(64 * R) void k_pair_sym_f32c(const Src32* __restric
   }
 }
 
+template <int R, bool METRIC, bool NEAR>
+__global__ __launch_bounds__(64 * R) void k_pair_sym_f32c(const Src32* __restrict__ src, int N, long long n_off,
+                                                          double* __restrict__ part, NearOut near) {
+  __shared__ SymLds32<R, METRIC, NEAR> S;
+  int a, b;
+  block_rows(blockIdx.x, n_off, a, b);
+  sym_block_f32c<R, METRIC, NEAR>(src, N, a, b, part, S, near);
+}
+
+// Batched ensemble: a grid-stride walk over the blocks (run, a, b) of every
+// run's symmetric sum, FP64 or FP32c per run; each block runs the single-run
+// block body on that run's records and partial slots, so a run's partials
+// equal those of k_pair_sym_<R> over its markers alone.
+template <int R>
+__global__ __launch_bounds__(64 * R) void k_pair_sym_b(EnsView E, const int4* __restrict__ blocks, int nblocks,
+                                                       const Src64* __restrict__ s64, const Src32* __restrict__ s32,
+                                                       int force_f64, double* __restrict__ part) {
+  constexpr size_t b64 = sizeof(SymLds64<R, false, false>), b32 = sizeof(SymLds32<R, false, false>);
+  __shared__ __align__(16) unsigned char smem[b64 > b32 ? b64 : b32];
+  const NearOut nz{};
+  for (int t = blockIdx.x; t < nblocks; t += gridDim.x) {
+    const int4 bl = blocks[t];
+    const int r = bl.x;
+    if (!force_f64 && !E.ctl[r].active) continue;
+    const bool f64 = force_f64 || E.ctl[r].kind == 0;
+    const EnsRun& Rn = E.runs[r];
+    if (f64)
+      sym_block_f64<R, false, false>(s64 + Rn.off, Rn.n, bl.y, bl.z, part + Rn.poff,
+                                     *reinterpret_cast<SymLds64<R, false, false>*>(smem), nz);
+    else
+      sym_block_f32c<R, false, false>(s32 + Rn.off, Rn.n, bl.y, bl.z, part + Rn.poff,
+                                      *reinterpret_cast<SymLds32<R, false, false>*>(smem), nz);
+    __syncthreads();  // the next block reuses the LDS
+  }
+}
+
 template <int R>
 void sym_f64(const Src64* src, int N, bool metric, const NearOut* near, double* part, hipStream_t st) {
   const long long S = sym_rows(N, R), n_off = S * (S - 1) / 2;
@@ -517,6 +578,23 @@ void launch_pair_sym_f64(const Src64* src, int N, int R, bool metric, const Near
     case 2: sym_f64<2>(src, N, metric, near, part, st); break;
     case 4: sym_f64<4>(src, N, metric, near, part, st); break;
     default: sym_f64<8>(src, N, metric, near, part, st); break;
+  }
+}
+
+void launch_pair_sym_b(const EnsView& E, const EnsTiles& T, const Src64* s64, const Src32* s32, int force_f64,
+                       double* part, hipStream_t st) {
+  if (T.ntiles <= 0) return;
+  const dim3 grid(T.grid);
+  switch (T.sym) {
+    case 2:
+      hipLaunchKernelGGL(k_pair_sym_b<2>, grid, dim3(128), 0, st, E, T.tiles, T.ntiles, s64, s32, force_f64, part);
+      break;
+    case 8:
+      hipLaunchKernelGGL(k_pair_sym_b<8>, grid, dim3(512), 0, st, E, T.tiles, T.ntiles, s64, s32, force_f64, part);
+      break;
+    default:
+      hipLaunchKernelGGL(k_pair_sym_b<4>, grid, dim3(256), 0, st, E, T.tiles, T.ntiles, s64, s32, force_f64, part);
+      break;
   }
 }
 

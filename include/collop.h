@@ -244,9 +244,12 @@ int collop_mlp_eval(collop_ctx* ctx, const float* rec, int64_t n, int mode, floa
  * iteration of the whole ensemble is one replay of a captured hipGraph whose
  * pair sum is ONE persistent launch over the tiles of every active run.  Each
  * run follows exactly the solve of collop_run_steps (own Anderson window,
- * precision switch and exit) and its markers equal, bit for bit, those of the
- * same run stepped alone on a context with
- * collop_set_pair_tile(ctx, 1, collop_ensemble_slices(N), 1).
+ * precision switch and exit).  The pair-sum form is the context's at the time
+ * of this call: symmetric with the context's tiles (4 when automatic) if every
+ * run fits in 256 partial slots, else ordered.  A run's markers then equal,
+ * bit for bit, those of the same run stepped alone on a context with
+ * collop_set_pair_mode(ctx, COLLOP_PAIR_SYMMETRIC, tiles) (symmetric), or with
+ * collop_set_pair_tile(ctx, 1, collop_ensemble_slices(N), 1) (ordered).
  * v: the runs' N_r x 2 column-major blocks one after another (2 sum N_r
  * doubles); w: their weights (sum N_r). */
 int collop_ensemble_set(collop_ctx* ctx, int n_runs, const int64_t* n, const double* v, const double* w);
@@ -266,7 +269,9 @@ int collop_ensemble_set_tile(collop_ctx* ctx, int gamma_tile, int waves_per_cu, 
 /* autotuner: times the persistent pair sum (one FP32c and one FP64 launch with
  * every run active) for gamma_tile in {64, 128, 256, 512} x waves_per_cu in
  * {4, 8, 16, 32} on the current ensemble, keeps the fastest; ms[16] optional
- * receives each candidate's time in that order. */
+ * receives each candidate's time in that order.  In the symmetric form only
+ * waves_per_cu is tuned: ms[0..3], *gamma_tile = 0 (collop_ensemble_set_tile
+ * then ignores gamma_tile). */
 int collop_ensemble_tune(collop_ctx* ctx, int* gamma_tile, int* waves_per_cu, double* ms);
 
 /* ---- tuning and measurement (no counterpart in the reference) -----------

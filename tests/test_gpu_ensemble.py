@@ -49,10 +49,25 @@ def single_ws():
     return collop.Workspace(bp, bp)  # its own context
 
 
-def single_reference(ws, runs, prec, steps):
+@pytest.fixture(autouse=True)
+def default_pair_mode(ens_ws):
+    """The pair form is a setting of the shared context: restore the default."""
+    yield
+    collop.set_pair_mode(ens_ws, True)
+
+
+def set_form(ws, symmetric):
+    """The ensemble's pair form follows its context's at collop_ensemble_set."""
+    collop.set_pair_mode(ws, symmetric, 4 if symmetric else 0)
+
+
+def single_reference(ws, runs, prec, steps, symmetric):
     out, iters = [], []
     for v, w in runs:
-        collop.set_pair_tile(ws, 1, collop.Ensemble.slices(v.shape[0]), 1)
+        if symmetric:
+            collop.set_pair_mode(ws, True, 4)
+        else:
+            collop.set_pair_tile(ws, 1, collop.Ensemble.slices(v.shape[0]), 1)
         sim = collop.Simulation(ws, v, w)
         st = sim.run(params(prec), steps)
         out.append(sim.markers())
@@ -61,11 +76,13 @@ def single_reference(ws, runs, prec, steps):
     return out, iters
 
 
+@pytest.mark.parametrize("symmetric", [False, True], ids=["ordered", "symmetric"])
 @pytest.mark.parametrize("prec", [collop.PREC_F64, collop.PREC_F32C])
-def test_ensemble_matches_single_runs_bitwise(ens_ws, single_ws, prec):
+def test_ensemble_matches_single_runs_bitwise(ens_ws, single_ws, prec, symmetric):
     runs = make_runs()
     steps = 2
-    ref, ref_iters = single_reference(single_ws, runs, prec, steps)
+    ref, ref_iters = single_reference(single_ws, runs, prec, steps, symmetric)
+    set_form(ens_ws, symmetric)
     ens = collop.Ensemble(ens_ws, runs)
     st = ens.step(params(prec), steps)
     got = ens.markers()
@@ -78,9 +95,11 @@ def test_ensemble_matches_single_runs_bitwise(ens_ws, single_ws, prec):
     assert all(s["exit_reason"] == 0 for row in st for s in row)
 
 
-def test_graph_and_tiles_do_not_change_results(ens_ws):
+@pytest.mark.parametrize("symmetric", [False, True], ids=["ordered", "symmetric"])
+def test_graph_and_tiles_do_not_change_results(ens_ws, symmetric):
     runs = make_runs()
     p = params(collop.PREC_F32C)
+    set_form(ens_ws, symmetric)
     results = []
     for tile, waves, graph in [(256, 8, True), (256, 8, False), (64, 4, True), (128, 16, True), (512, 32, True)]:
         ens = collop.Ensemble(ens_ws, runs)
@@ -92,12 +111,19 @@ def test_graph_and_tiles_do_not_change_results(ens_ws):
             np.testing.assert_array_equal(a, b)
 
 
-def test_autotune(ens_ws):
+@pytest.mark.parametrize("symmetric", [False, True], ids=["ordered", "symmetric"])
+def test_autotune(ens_ws, symmetric):
+    set_form(ens_ws, symmetric)
     ens = collop.Ensemble(ens_ws, make_runs())
     bt, waves, ms = ens.tune()
     print(f"tuned gamma tile {bt}, waves/CU {waves}; ms (rows 64..512, cols 4..32 waves):\n{ms}")
-    assert bt in (64, 128, 256, 512) and waves in (4, 8, 16, 32)
-    assert np.all(ms > 0) and np.isclose(ms.min(), ms[[64, 128, 256, 512].index(bt), [4, 8, 16, 32].index(waves)])
+    assert waves in (4, 8, 16, 32)
+    if symmetric:  # waves only: row 0
+        assert bt == 0 and np.all(ms[0] > 0) and np.all(ms[1:] == 0)
+        assert np.isclose(ms[0].min(), ms[0, [4, 8, 16, 32].index(waves)])
+    else:
+        assert bt in (64, 128, 256, 512)
+        assert np.all(ms > 0) and np.isclose(ms.min(), ms[[64, 128, 256, 512].index(bt), [4, 8, 16, 32].index(waves)])
     st = ens.step(params(collop.PREC_F32C), 1)[0]
     assert all(s["exit_reason"] == 0 for s in st)
 
