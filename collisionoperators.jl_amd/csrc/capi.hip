@@ -1052,7 +1052,14 @@ int defect_solve(collop_ctx* ctx, const collop_step_params& p, collop_step_stats
     for (int j = 1; j <= p.dc_max_inner; ++j) {
       const int col = j == 1 ? -1 : slot % mw;
       const int hist_new = j == 1 ? 0 : std::min(hist + 1, mw);
-      if ((r = frozen_map_dev(ctx, p, col, hist_new))) return r;
+      if (j == 1) {
+        // Phi_k(v_k) = G(v_k): at u = v_k the frozen-metric correction and the
+        // near-pair differences vanish exactly, so the first inner iterate's map
+        // value, residual and statistics are the outer map's (s_Gvo, s_r, h_stat)
+        if ((r = copy_d2d(ctx, ctx->s_Gv, ctx->s_Gvo, n2b))) return r;
+      } else if ((r = frozen_map_dev(ctx, p, col, hist_new))) {
+        return r;
+      }
       inner++;
       const double* st = ctx->h_stat;
       const double ri = std::sqrt(st[0]);
