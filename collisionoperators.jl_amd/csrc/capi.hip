@@ -283,6 +283,7 @@ int ensure_scratch(collop_ctx* ctx, int64_t N) {
   ENSURE(ctx->b_scal, 16 * 8);                    // S0, S1, lambda, E, P1, P2, ...
   ENSURE(ctx->b_statp, (size_t)epi_blocks((int)N) * kStatWidth * 8);
   ENSURE(ctx->b_stat, kStatWidth * 8);
+  ENSURE(ctx->b_stmp, (size_t)2 * m.n_dofs * 8);
   return COLLOP_OK;
 }
 
@@ -324,7 +325,7 @@ int collop_destroy(collop_ctx* ctx) {
   free_mesh(ctx);
   for (DBuf* b : {&ctx->b_v, &ctx->b_w, &ctx->b_G, &ctx->b_out, &ctx->b_vmid, &ctx->b_dv, &ctx->b_dS, &ctx->b_s64,
                   &ctx->b_s32, &ctx->b_part, &ctx->b_spart_sc, &ctx->b_mom, &ctx->b_dpart, &ctx->b_iota,
-                  &ctx->b_spart_q, &ctx->b_vec, &ctx->b_scal, &ctx->b_statp, &ctx->b_stat, &ctx->s_v, &ctx->s_w,
+                  &ctx->b_spart_q, &ctx->b_vec, &ctx->b_scal, &ctx->b_statp, &ctx->b_stat, &ctx->b_stmp, &ctx->s_v, &ctx->s_w,
                   &ctx->s_v1, &ctx->s_Gv, &ctx->s_r, &ctx->s_rp, &ctx->s_Gp, &ctx->s_best, &ctx->s_dF, &ctx->s_dG,
                   &ctx->s_fc, &ctx->s_dotv, &ctx->s_Gvo, &ctx->s_Gk, &ctx->s_FA, &ctx->s_D, &ctx->s_Ge, &ctx->s_vmk, &ctx->s_xk, &ctx->s_oU,
                   &ctx->s_oUp, &ctx->s_of, &ctx->s_ofp, &ctx->s_odF, &ctx->s_odG, &ctx->b_near_list, &ctx->b_near_count,
@@ -709,8 +710,15 @@ int geff_dev(collop_ctx* ctx, const collop_step_params& p, const double* vin, Sr
   launch_scatter(m, P<double>(ctx->s_v), vin, P<double>(ctx->s_w), n, 2, p.use_gonzalez ? 2 : 1, ctx->w_abs_sum,
                  P<double>(ctx->b_vmid), P<double>(ctx->b_dv), P<long long>(ctx->b_spart_sc), nullptr, ctx->stream);
   CHECK_LAUNCH();
-  launch_reduce_solve(m, P<long long>(ctx->b_spart_sc), nsb, ctx->w_abs_sum, nullptr, p.use_gonzalez ? 2 : 1, vec,
-                      ctx->stream);
+  // the mass solves run split over the columns and rows of the coefficient
+  // array (bitwise the single-workgroup solve) when the 1D spaces allow it
+  const bool split = m.nd1 <= 256 && m.nd2 <= 256;
+  if (split)
+    launch_solve_split(m, P<long long>(ctx->b_spart_sc), nsb, ctx->w_abs_sum, nullptr, p.use_gonzalez ? 2 : 1, vec,
+                       P<double>(ctx->b_stmp), ctx->stream);
+  else
+    launch_reduce_solve(m, P<long long>(ctx->b_spart_sc), nsb, ctx->w_abs_sum, nullptr, p.use_gonzalez ? 2 : 1,
+                        vec, ctx->stream);
   CHECK_LAUNCH();
   // r(f_mid) and S(f_in)
   launch_quad1(m, vec, vec + m.n_dofs, p.use_gonzalez ? 2 : 1, p.use_logsq, P<double>(ctx->b_iota),
@@ -718,12 +726,19 @@ int geff_dev(collop_ctx* ctx, const collop_step_params& p, const double* vin, Sr
   CHECK_LAUNCH();
   launch_quad2(m, P<double>(ctx->b_iota), vec + 2 * m.n_dofs, ctx->stream);
   CHECK_LAUNCH();
-  launch_reduce_solve(m, nullptr, 0, 1.0, vec + 2 * m.n_dofs, 1, vec + 3 * m.n_dofs, ctx->stream);
-  CHECK_LAUNCH();
   const int nqb = quad_blocks(m);
-  if (p.use_gonzalez) {
-    launch_sum_rows(P<double>(ctx->b_spart_q) + nqb, nqb, 1, 0, 1, sc + 1, ctx->stream);
+  if (split) {  // with the Gonzalez entropy S(f_in) summed in the same launch
+    launch_solve_split(m, nullptr, 0, 1.0, vec + 2 * m.n_dofs, 1, vec + 3 * m.n_dofs, P<double>(ctx->b_stmp),
+                       ctx->stream, p.use_gonzalez ? P<double>(ctx->b_spart_q) + nqb : nullptr, nqb,
+                       p.use_gonzalez ? sc + 1 : nullptr);
     CHECK_LAUNCH();
+  } else {
+    launch_reduce_solve(m, nullptr, 0, 1.0, vec + 2 * m.n_dofs, 1, vec + 3 * m.n_dofs, ctx->stream);
+    CHECK_LAUNCH();
+    if (p.use_gonzalez) {
+      launch_sum_rows(P<double>(ctx->b_spart_q) + nqb, nqb, 1, 0, 1, sc + 1, ctx->stream);
+      CHECK_LAUNCH();
+    }
   }
   // dS = -w grad L(v_mid) and the partial sums of dv.dS, |dv|^2
   launch_gather(m, P<double>(ctx->b_vmid), vec + 3 * m.n_dofs, P<double>(ctx->s_w), P<double>(ctx->b_dv), n, 1, 0.0,

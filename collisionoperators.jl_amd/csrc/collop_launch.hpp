@@ -126,6 +126,13 @@ void launch_scatter(const MeshView& m, const double* v0, const double* vin, cons
 // workgroup per field.
 void launch_reduce_solve(const MeshView& m, const long long* part, int nblocks, double w_abs_sum,
                          const double* direct, int fields, double* out, hipStream_t st);
+// The same solve as two launches over nd2 and nd1 workgroups per field (tmp:
+// fields x n_dofs doubles), bitwise equal to launch_reduce_solve; with sum_out,
+// also *sum_out = the fixed-order sum of sum_part[0, sum_n) (as launch_sum_rows).
+// Needs nd1, nd2 <= 256.
+void launch_solve_split(const MeshView& m, const long long* part, int nblocks, double w_abs_sum,
+                        const double* direct, int fields, double* out, double* tmp, hipStream_t st,
+                        const double* sum_part = nullptr, int sum_n = 0, double* sum_out = nullptr);
 // rhs (no solve) from the fixed-point partial rows of field 0
 void launch_rhs(const MeshView& m, const long long* part, int nblocks, double w_abs_sum, double* out,
                 hipStream_t st);

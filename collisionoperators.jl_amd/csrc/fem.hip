@@ -159,7 +159,7 @@ __device__ __forceinline__ void solve_core(const MeshView& m, double* X, double*
       const int i1 = k % nd1, i2 = k - (k % nd1);
       const double* row = I1 + (size_t)i1 * nd1;
       double s = 0.0;
-      for (int j = 0; j < nd1; ++j) s += row[j] * X[i2 + j];
+      for (int j = 0; j < nd1; ++j) s = __builtin_fma(row[j], X[i2 + j], s);
       t[q] = s;
     }
   }
@@ -174,7 +174,7 @@ __device__ __forceinline__ void solve_core(const MeshView& m, double* X, double*
     const int i1 = k % nd1, i2 = k / nd1;
     const double* col = I2 + (size_t)i2 * nd2;
     double s = 0.0;
-    for (int j = 0; j < nd2; ++j) s += X[i1 + (size_t)nd1 * j] * col[j];
+    for (int j = 0; j < nd2; ++j) s = __builtin_fma(X[i1 + (size_t)nd1 * j], col[j], s);
     out[k] = s;
   }
 }
@@ -368,6 +368,57 @@ __global__ __launch_bounds__(kSolveThreads) void k_reduce_solve(MeshView m, cons
              direct ? direct + (size_t)field * nd : nullptr, out + (size_t)field * nd);
 }
 
+// The same solve split over many workgroups, in the same arithmetic order per
+// coefficient (so results equal solve_body's bit for bit): k_solve_cols takes
+// one column i2 of B per workgroup -- the exact row sum (or `direct`) of its
+// nd1 entries, then T = M1^{-1} B on that column -- and k_solve_rows one row
+// i1 of T -- out = T M2^{-1} on that row.  Grid (nd2 [+1], fields) and (nd1,
+// fields) of kSplitThreads.  The optional extra workgroup of k_solve_cols
+// (blockIdx.x == nd2, field 0) sums sum_n partials into *sum_out exactly as
+// k_sum_rows does.
+constexpr int kSplitThreads = kSumThreads;
+
+__global__ __launch_bounds__(kSplitThreads) void k_solve_cols(MeshView m, const long long* __restrict__ part,
+                                                              int nblocks, double inv_scale,
+                                                              const double* __restrict__ direct,
+                                                              double* __restrict__ T,
+                                                              const double* __restrict__ sum_part, int sum_n,
+                                                              double* __restrict__ sum_out) {
+  __shared__ double B[kSplitThreads];
+  __shared__ double red[kSumThreads / 64];
+  const int nd = m.n_dofs, nd1 = m.nd1, field = blockIdx.y, i2 = blockIdx.x;
+  if (i2 == m.nd2) {  // the fused partial sum
+    const double t = sum_body(sum_part, sum_n, 1, red);
+    if (threadIdx.x == 0) *sum_out = t;
+    return;
+  }
+  const int k = i2 * nd1 + threadIdx.x;
+  if (threadIdx.x < nd1)
+    B[threadIdx.x] = part ? rhs_of(part + (size_t)field * nblocks * nd, nblocks, nd, k, inv_scale)
+                          : direct[(size_t)field * nd + k];
+  __syncthreads();
+  if (threadIdx.x < nd1) {
+    const double* row = m.minv1 + (size_t)threadIdx.x * nd1;
+    double s = 0.0;
+    for (int j = 0; j < nd1; ++j) s = __builtin_fma(row[j], B[j], s);
+    T[(size_t)field * nd + k] = s;
+  }
+}
+
+__global__ __launch_bounds__(kSplitThreads) void k_solve_rows(MeshView m, const double* __restrict__ T,
+                                                              double* __restrict__ out) {
+  __shared__ double Trow[kSplitThreads];
+  const int nd = m.n_dofs, nd1 = m.nd1, nd2 = m.nd2, field = blockIdx.y, i1 = blockIdx.x;
+  if (threadIdx.x < nd2) Trow[threadIdx.x] = T[(size_t)field * nd + i1 + (size_t)nd1 * threadIdx.x];
+  __syncthreads();
+  if (threadIdx.x < nd2) {
+    const double* col = m.minv2 + (size_t)threadIdx.x * nd2;
+    double s = 0.0;
+    for (int j = 0; j < nd2; ++j) s = __builtin_fma(Trow[j], col[j], s);
+    out[(size_t)field * nd + i1 + (size_t)nd1 * threadIdx.x] = s;
+  }
+}
+
 __global__ __launch_bounds__(kQuadThreads) void k_quad1(MeshView m, const double* __restrict__ c0,
                                                         const double* __restrict__ c1, int use_logsq,
                                                         double* __restrict__ iota, double* __restrict__ spart) {
@@ -522,6 +573,15 @@ void launch_reduce_solve(const MeshView& m, const long long* part, int nblocks, 
   const size_t lds = solve_lds(m);
   hipLaunchKernelGGL(k_reduce_solve, dim3(fields), dim3(kSolveThreads), lds, st, m, part, nblocks,
                      1.0 / scatter_scale(w_abs_sum), direct, out);
+}
+
+void launch_solve_split(const MeshView& m, const long long* part, int nblocks, double w_abs_sum,
+                        const double* direct, int fields, double* out, double* tmp, hipStream_t st,
+                        const double* sum_part, int sum_n, double* sum_out) {
+  const int extra = sum_out ? 1 : 0;
+  hipLaunchKernelGGL(k_solve_cols, dim3(m.nd2 + extra, fields), dim3(kSplitThreads), 0, st, m, part, nblocks,
+                     1.0 / scatter_scale(w_abs_sum), direct, tmp, sum_part, sum_n, sum_out);
+  hipLaunchKernelGGL(k_solve_rows, dim3(m.nd1, fields), dim3(kSplitThreads), 0, st, m, tmp, out);
 }
 
 void launch_rhs(const MeshView& m, const long long* part, int nblocks, double w_abs_sum, double* out,
