@@ -9,10 +9,13 @@ interface: the Workspace tables (MantisWrappers.jl), the GPU hook functions
 from ._lib import (CollopError, Context, StepParams, StepStats, load, PREC_F64, PREC_F32C, WARM_EULER,  # noqa: F401
                    WARM_MLP)
 from .hooks import (compute_collision_gpu, compute_collision_gpu32, compute_collision_metric_gpu,  # noqa: F401
-                    compute_G_gpu, compute_r_and_entropy_gpu, enable_gpu, eval_loggrad_gpu, l2_project_gpu,
-                    l2_rhs_gpu, mass_solve_gpu)
+                    compute_fs_minus_fp_l2_gpu, compute_G_gpu, compute_negative_part_l1_gpu,
+                    compute_r_and_entropy_gpu, enable_gpu, eval_loggrad_gpu, l2_project_gpu, l2_rhs_gpu,
+                    mass_solve_gpu)
 from .ic import ensemble_markers, initial_markers, markers  # noqa: F401
 from .mesh import Workspace, build_workspace, extraction_p2c1  # noqa: F401
 from .params import CHOSEN, CONFIGS, SimParameters, chosen_config, config, parse_overrides  # noqa: F401
+from .records import (CONS_COLS, STATS_COLS, RunRecord, checkpoint_path, cons_time_at,  # noqa: F401
+                      load_checkpoint, migrate_cons_csv, run_recorded, save_checkpoint, truncate_csv_after)
 from .sim import Ensemble, Simulation, pair_mode, profile, set_pair_mode, set_pair_tile  # noqa: F401
 from .warmstart import WarmStart, glorot_init  # noqa: F401

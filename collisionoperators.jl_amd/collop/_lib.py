@@ -128,9 +128,12 @@ SIGNATURES = [
     ("collop_loggrad_gather", C.c_int, [_vp, _dp, C.c_int64, _dp, C.c_double, C.c_double, _dp]),
     ("collop_quadrature", C.c_int, [_vp, _dp, C.c_int, _dp, _dp]),
     ("collop_mass_solve", C.c_int, [_vp, _dp, _dp]),
+    ("collop_negative_part_l1", C.c_int, [_vp, _dp, _dp]),
+    ("collop_fs_minus_fp_l2", C.c_int, [_vp, _dp, _dp, C.c_int64, _dp, _dp]),
     ("collop_state_set", C.c_int, [_vp, _dp, _dp, C.c_int64]),
     ("collop_state_get", C.c_int, [_vp, _dp]),
     ("collop_state_coeffs", C.c_int, [_vp, _dp]),
+    ("collop_state_diagnostics", C.c_int, [_vp, _dp]),
     ("collop_run_steps", C.c_int, [_vp, C.POINTER(StepParams), C.c_int, C.POINTER(StepStats)]),
     ("collop_step", C.c_int, [_vp, C.POINTER(StepParams), _dp, _dp, C.c_int64, C.POINTER(StepStats)]),
     ("collop_mlp_set_params", C.c_int, [_vp, _fp, C.c_int]),

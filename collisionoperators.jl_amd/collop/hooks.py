@@ -143,6 +143,28 @@ def mass_solve_gpu(ws: Workspace, b):
     return x
 
 
+def compute_negative_part_l1_gpu(ws: Workspace, f_coeffs) -> float:
+    """compute_negative_part_l1 (functions.jl:300-313) of f_s = sum_k f_coeffs[k] phi_k."""
+    ws.activate()
+    fc = _vec(f_coeffs, ws.n_dofs)
+    out = np.zeros(1)
+    ws.ctx.check(ws.ctx.lib.collop_negative_part_l1(ws.ctx.h, _lib.dptr(fc), _lib.dptr(out)))
+    return float(out[0])
+
+
+def compute_fs_minus_fp_l2_gpu(ws: Workspace, f_coeffs, v_parts, w_parts) -> float:
+    """compute_fs_minus_fp_l2 (functions.jl:320-343): ||f_s - f_p||_2 with f_p the
+    markers' cellwise histogram density."""
+    ws.activate()
+    fc = _vec(f_coeffs, ws.n_dofs)
+    v, N = _markers(v_parts)
+    w = _vec(w_parts, N)
+    out = np.zeros(1)
+    ws.ctx.check(ws.ctx.lib.collop_fs_minus_fp_l2(ws.ctx.h, _lib.dptr(v), _lib.dptr(w), N, _lib.dptr(fc),
+                                                  _lib.dptr(out)))
+    return float(out[0])
+
+
 def enable_gpu(hooks: dict, gpu_fp32: bool = False) -> dict:
     """Mirror of enable_gpu! (main.jl:390-423): repoint a hook table at the device routines."""
     hooks["L2PROJ_FN"] = l2_project_gpu

@@ -20,6 +20,7 @@ class Simulation:
         if v.ndim != 2 or v.shape[1] != 2 or w.shape != (v.shape[0],):
             raise ValueError("v must be (N, 2) and w (N,)")
         self.N = v.shape[0]
+        self.w = w
         ws.activate()
         ws.ctx.check(ws.ctx.lib.collop_state_set(ws.ctx.h, _lib.dptr(v), _lib.dptr(w), self.N))
         ws.ctx.state_owner = self
@@ -47,6 +48,14 @@ class Simulation:
         fc = np.empty(self.ws.n_dofs)
         self.ws.ctx.check(self.ws.ctx.lib.collop_state_coeffs(self.ws.ctx.h, _lib.dptr(fc)))
         return fc
+
+    def diagnostics(self) -> dict:
+        """The run record's fp_minus_fs and neg_part of the resident state
+        (main.jl:306-307, functions.jl:300-343)."""
+        self._own()
+        out = np.zeros(2)
+        self.ws.ctx.check(self.ws.ctx.lib.collop_state_diagnostics(self.ws.ctx.h, _lib.dptr(out)))
+        return {"fp_minus_fs": float(out[0]), "neg_part": float(out[1])}
 
 
 def profile(ws: Workspace, enable: bool):

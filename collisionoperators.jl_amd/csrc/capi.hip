@@ -633,6 +633,67 @@ int collop_quadrature(collop_ctx* ctx, const double* f_coeffs, int use_logsq, do
   return COLLOP_OK;
 }
 
+}  // extern "C"
+
+namespace {
+
+// Run-record diagnostics of f_s = sum fc_k phi_k (functions.jl:300-343) from
+// device arrays: out[1] = int max(-f_s, 0); with markers (v, w), also out[0] =
+// ||f_s - f_p||_2, f_p their cell histogram density.
+int diagnostics_dev(collop_ctx* ctx, const double* v, const double* w, int n, double w_abs_sum, const double* fc,
+                    double* out) {
+  double* sc = P<double>(ctx->b_scal);
+  launch_fs_diagnostics(ctx->mv, v, w, n, w_abs_sum, fc, P<long long>(ctx->b_spart_sc), P<double>(ctx->b_stmp),
+                        P<double>(ctx->b_spart_q), sc + 8, ctx->stream);
+  CHECK_LAUNCH();
+  double h[2];
+  HIPC(hipMemcpyAsync(h, sc + 8, sizeof(h), hipMemcpyDeviceToHost, ctx->stream));
+  HIPC(hipStreamSynchronize(ctx->stream));
+  if (v) out[0] = std::sqrt(h[1]);
+  out[1] = h[0];
+  return COLLOP_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int collop_negative_part_l1(collop_ctx* ctx, const double* f_coeffs, double* neg_part) {
+  int r = need_mesh(ctx);
+  if (r) return r;
+  if (!f_coeffs || !neg_part) return fail(ctx, COLLOP_ERR_INVALID, "null array");
+  HIPC(hipSetDevice(ctx->device));
+  if ((r = ensure_scratch(ctx, 1))) return r;
+  double* vec = P<double>(ctx->b_vec);
+  HIPC(hipMemcpyAsync(vec, f_coeffs, (size_t)ctx->mv.n_dofs * 8, hipMemcpyHostToDevice, ctx->stream));
+  double out[2] = {0.0, 0.0};
+  if ((r = diagnostics_dev(ctx, nullptr, nullptr, 0, 0.0, vec, out))) return r;
+  *neg_part = out[1];
+  return COLLOP_OK;
+}
+
+int collop_fs_minus_fp_l2(collop_ctx* ctx, const double* v, const double* w, int64_t N, const double* f_coeffs,
+                          double* fs_minus_fp) {
+  int r = need_mesh(ctx);
+  if (r) return r;
+  if ((r = check_N(ctx, N))) return r;
+  if (!f_coeffs || !fs_minus_fp || (N > 0 && (!v || !w))) return fail(ctx, COLLOP_ERR_INVALID, "null array");
+  HIPC(hipSetDevice(ctx->device));
+  if ((r = ensure_scratch(ctx, N))) return r;
+  double wsum = 0.0;
+  for (int64_t i = 0; i < N; ++i) wsum += std::fabs(w[i]);
+  if (N > 0) {
+    HIPC(hipMemcpyAsync(ctx->b_v.p, v, 2 * N * 8, hipMemcpyHostToDevice, ctx->stream));
+    HIPC(hipMemcpyAsync(ctx->b_w.p, w, N * 8, hipMemcpyHostToDevice, ctx->stream));
+  }
+  double* vec = P<double>(ctx->b_vec);
+  HIPC(hipMemcpyAsync(vec, f_coeffs, (size_t)ctx->mv.n_dofs * 8, hipMemcpyHostToDevice, ctx->stream));
+  double out[2] = {0.0, 0.0};
+  if ((r = diagnostics_dev(ctx, P<double>(ctx->b_v), P<double>(ctx->b_w), (int)N, wsum, vec, out))) return r;
+  *fs_minus_fp = out[0];
+  return COLLOP_OK;
+}
+
 int collop_mass_solve(collop_ctx* ctx, const double* b, double* x) {
   int r = need_mesh(ctx);
   if (r) return r;
@@ -1364,6 +1425,15 @@ int collop_state_coeffs(collop_ctx* ctx, double* f_coeffs) {
   HIPC(hipMemcpyAsync(f_coeffs, ctx->s_fc.p, (size_t)ctx->mv.n_dofs * 8, hipMemcpyDeviceToHost, ctx->stream));
   HIPC(hipStreamSynchronize(ctx->stream));
   return COLLOP_OK;
+}
+
+int collop_state_diagnostics(collop_ctx* ctx, double* out) {
+  if (!ctx) return COLLOP_ERR_INVALID;
+  if (!ctx->has_state) return fail(ctx, COLLOP_ERR_NO_STATE, "no device-resident state (collop_state_set)");
+  if (!out) return fail(ctx, COLLOP_ERR_INVALID, "null array");
+  HIPC(hipSetDevice(ctx->device));
+  return diagnostics_dev(ctx, P<double>(ctx->s_v), P<double>(ctx->s_w), (int)ctx->N, ctx->w_abs_sum,
+                         P<double>(ctx->s_fc), out);
 }
 
 int collop_run_steps(collop_ctx* ctx, const collop_step_params* params, int n_steps, collop_step_stats* stats) {

@@ -140,6 +140,13 @@ void launch_rhs(const MeshView& m, const long long* part, int nblocks, double w_
 // non-null) gets (1 + log f) w_q |e| (0 where f is below the floor) and spart[y]
 // (if non-null) partial sums of -f log f w_q |e|.  coeff[y] points at n_dofs.
 int quad_blocks(const MeshView& m);
+// Run-record diagnostics of f_s = sum c_k phi_k: out[0] = int max(-f_s, 0) and,
+// when v is given, out[1] = ||f_s - f_p||_2^2 with f_p the markers' cell
+// histogram density (device pointers; part: scatter_blocks(N) x ne1*ne2,
+// cell_mass: ne1*ne2, spart: 2 x quad_blocks).
+void launch_fs_diagnostics(const MeshView& m, const double* v, const double* w, int N, double w_abs_sum,
+                           const double* c, long long* part, double* cell_mass, double* spart, double* out,
+                           hipStream_t st);
 void launch_quad1(const MeshView& m, const double* c0, const double* c1, int fields, int use_logsq, double* iota,
                   double* spart, hipStream_t st);
 // r_k = sum over cells / points of phi_k iota  (compute_r!, functions.jl:158-179)

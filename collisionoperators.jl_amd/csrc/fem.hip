@@ -194,6 +194,23 @@ __device__ __forceinline__ void solve_body(const MeshView& m, double* X, const l
 // Quadrature pass 1 of one block (one thread per cell and Gauss point).
 constexpr int kQuadThreads = 256;
 
+// f_s = sum_k c_k phi_k at Gauss point (a, b) of cell (e1, e2): rows over j1
+// inside, j2 outside.
+__device__ __forceinline__ double field_at_qp(const MeshView& m, const double* __restrict__ c, int e1, int e2, int a,
+                                              int b) {
+  const int s1 = m.bs1[e1], s2 = m.bs2[e2];
+  double f = 0.0;
+#pragma unroll
+  for (int j2 = 0; j2 < 3; ++j2) {
+    const double p2 = m.phiq2[(3 * e2 + j2) * m.nq + b];
+    double row = 0.0;
+#pragma unroll
+    for (int j1 = 0; j1 < 3; ++j1) row += c[(s1 + j1) + (s2 + j2) * m.nd1] * m.phiq1[(3 * e1 + j1) * m.nq + a];
+    f += row * p2;
+  }
+  return f;
+}
+
 __device__ __forceinline__ void quad1_body(const MeshView& m, double* red, const double* __restrict__ c, int use_logsq,
                                            double* __restrict__ iota, double* __restrict__ spart_out, int lb) {
   const int npt = m.nq * m.nq;
@@ -204,16 +221,7 @@ __device__ __forceinline__ void quad1_body(const MeshView& m, double* red, const
     const int e = tid / npt, pt = tid - e * npt;
     const int e1 = e % m.ne1, e2 = e / m.ne1;
     const int a = pt % m.nq, b = pt / m.nq;
-    const int s1 = m.bs1[e1], s2 = m.bs2[e2];
-    double f = 0.0;
-#pragma unroll
-    for (int j2 = 0; j2 < 3; ++j2) {
-      const double p2 = m.phiq2[(3 * e2 + j2) * m.nq + b];
-      double row = 0.0;
-#pragma unroll
-      for (int j1 = 0; j1 < 3; ++j1) row += c[(s1 + j1) + (s2 + j2) * m.nd1] * m.phiq1[(3 * e1 + j1) * m.nq + a];
-      f += row * p2;
-    }
+    const double f = field_at_qp(m, c, e1, e2, a, b);
     const double jac = (m.bp1[e1 + 1] - m.bp1[e1]) * (m.bp2[e2 + 1] - m.bp2[e2]);
     const double wq = m.qw[a] * m.qw[b] * jac;
     double lg = 0.0;
@@ -461,6 +469,63 @@ __global__ __launch_bounds__(kGatherThreads) void k_gather(MeshView m, const dou
 }
 
 // ---------------------------------------------------------------------------
+// Run-record diagnostics (functions.jl:300-343).
+
+// Cell histogram of the marker weights, m_e = sum of w_a over the markers that
+// locate_particle puts in cell e (functions.jl:326-330), as fixed-point partial
+// rows like the scatter's: exact and independent of thread order.
+// Dynamic LDS: ne1 * ne2 u64.
+__global__ __launch_bounds__(kScatterThreads) void k_cell_mass(MeshView m, const double* __restrict__ v,
+                                                               const double* __restrict__ w, int N, double scale,
+                                                               long long* __restrict__ part) {
+  extern __shared__ unsigned long long hist[];
+  __shared__ double sb1[kMaxBp], sb2[kMaxBp];
+  const int nel = m.ne1 * m.ne2;
+  for (int k = threadIdx.x; k < nel; k += blockDim.x) hist[k] = 0ull;
+  stage_bp(m, sb1, sb2);
+  __syncthreads();
+  for (int a = blockIdx.x * blockDim.x + threadIdx.x; a < N; a += gridDim.x * blockDim.x) {
+    Located L;
+    if (!locate(m, sb1, sb2, v[a], v[N + a], L)) continue;
+    atomicAdd(&hist[L.i + L.j * m.ne1], (unsigned long long)__double2ll_rn(w[a] * scale));
+  }
+  __syncthreads();
+  for (int k = threadIdx.x; k < nel; k += blockDim.x) part[(size_t)blockIdx.x * nel + k] = (long long)hist[k];
+}
+
+// One thread per (cell, Gauss point): f = f_s there, then the block sums of
+// max(-f, 0) w_q |e| (compute_negative_part_l1) and, when the cell masses are
+// given, (f - m_e / |e|)^2 w_q |e| (compute_fs_minus_fp_l2) into spart[b] and
+// spart[gridDim.x + b].
+__global__ __launch_bounds__(kQuadThreads) void k_diag_quad(MeshView m, const double* __restrict__ c,
+                                                            const double* __restrict__ cell_mass,
+                                                            double* __restrict__ spart) {
+  __shared__ double red[kQuadThreads / 64];
+  const int npt = m.nq * m.nq;
+  const int tid = blockIdx.x * blockDim.x + threadIdx.x;
+  double neg = 0.0, sq = 0.0;
+  if (tid < m.ne1 * m.ne2 * npt) {
+    const int e = tid / npt, pt = tid - e * npt;
+    const int e1 = e % m.ne1, e2 = e / m.ne1;
+    const int a = pt % m.nq, b = pt / m.nq;
+    const double f = field_at_qp(m, c, e1, e2, a, b);
+    const double jac = (m.bp1[e1 + 1] - m.bp1[e1]) * (m.bp2[e2 + 1] - m.bp2[e2]);
+    const double wq = m.qw[a] * m.qw[b];
+    if (f < 0.0) neg = (-f) * wq * jac;
+    if (cell_mass) {
+      const double d = f - cell_mass[e] / jac;
+      sq = d * d * wq * jac;
+    }
+  }
+  const double s1 = block_sum<kQuadThreads>(neg, red);
+  const double s2 = block_sum<kQuadThreads>(sq, red);
+  if (threadIdx.x == 0) {
+    spart[blockIdx.x] = s1;
+    spart[gridDim.x + blockIdx.x] = s2;
+  }
+}
+
+// ---------------------------------------------------------------------------
 // Batched ensemble kernels.  Block tables map blockIdx.x to (run, local block);
 // runs whose control block says inactive return before any barrier.
 __device__ __forceinline__ bool ens_skip(const EnsView& E, int r) { return E.ctl && !E.ctl[r].active; }
@@ -608,6 +673,21 @@ void launch_quad2(const MeshView& m, const double* iota, double* r, hipStream_t 
 void launch_sum_rows(const double* part, int n, int count, int istride, int jstride, double* out, hipStream_t st) {
   if (count <= 0) return;
   hipLaunchKernelGGL(k_sum_rows, dim3(count), dim3(kSumThreads), 0, st, part, n, istride, jstride, out);
+}
+
+void launch_fs_diagnostics(const MeshView& m, const double* v, const double* w, int N, double w_abs_sum,
+                           const double* c, long long* part, double* cell_mass, double* spart, double* out,
+                           hipStream_t st) {
+  const int nel = m.ne1 * m.ne2, qb = quad_blocks(m);
+  if (v) {
+    const int nb = scatter_blocks(N);
+    hipLaunchKernelGGL(k_cell_mass, dim3(nb), dim3(kScatterThreads), (size_t)nel * sizeof(unsigned long long), st,
+                       m, v, w, N, scatter_scale(w_abs_sum), part);
+    hipLaunchKernelGGL(k_rhs, dim3((nel + 255) / 256), dim3(256), 0, st, part, nb, nel,
+                       1.0 / scatter_scale(w_abs_sum), cell_mass);
+  }
+  hipLaunchKernelGGL(k_diag_quad, dim3(qb), dim3(kQuadThreads), 0, st, m, c, v ? cell_mass : nullptr, spart);
+  hipLaunchKernelGGL(k_sum_rows, dim3(2), dim3(kSumThreads), 0, st, spart, qb, qb, 1, out);
 }
 
 

@@ -114,6 +114,17 @@ int collop_quadrature(collop_ctx* ctx, const double* f_coeffs, int use_logsq, do
 /* x = M^{-1} b (the reference's ws.M_lu \ b, functions.jl:132, solver.jl:67). */
 int collop_mass_solve(collop_ctx* ctx, const double* b, double* x);
 
+/* Run-record diagnostics, the conservation CSV's neg_part and fp_minus_fs
+ * columns (io.jl:106-107, written per step at main.jl:306-307):
+ *   collop_negative_part_l1: int max(-f_s, 0) dv by the N_QUAD^2 Gauss rule per
+ *     cell; replaces compute_negative_part_l1 (functions.jl:300-313).
+ *   collop_fs_minus_fp_l2: ||f_s - f_p||_2 with f_p = m_e / |e| the cellwise
+ *     histogram density of the markers (v, w) in the open box; replaces
+ *     compute_fs_minus_fp_l2 (functions.jl:320-343). */
+int collop_negative_part_l1(collop_ctx* ctx, const double* f_coeffs, double* neg_part);
+int collop_fs_minus_fp_l2(collop_ctx* ctx, const double* v, const double* w, int64_t N, const double* f_coeffs,
+                          double* fs_minus_fp);
+
 /* ---- device-resident implicit step ---------------------------------------
  * The step of main.jl:226-307 (Landau branch, solver = :anderson) with the
  * Picard map of solver.jl:176-227 and the Anderson solve of solver.jl:300-423
@@ -206,6 +217,10 @@ int collop_state_set(collop_ctx* ctx, const double* v, const double* w, int64_t 
 int collop_state_get(collop_ctx* ctx, double* v);
 /* Current projection coefficients f_s (n_dofs). */
 int collop_state_coeffs(collop_ctx* ctx, double* f_coeffs);
+/* Diagnostics of the resident state and its projection (the same two as
+ * collop_fs_minus_fp_l2 and collop_negative_part_l1, main.jl:306-307):
+ * out[0] = fp_minus_fs, out[1] = neg_part. */
+int collop_state_diagnostics(collop_ctx* ctx, double* out);
 /* Advance n_steps implicit steps on the device; stats[n_steps] optional. */
 int collop_run_steps(collop_ctx* ctx, const collop_step_params* params, int n_steps, collop_step_stats* stats);
 /* One step on host arrays: state_set + run_steps(1) + state_get. */

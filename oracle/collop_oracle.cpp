@@ -550,6 +550,53 @@ double orc_entropy(const orc_mesh* ms, const double* fc, int use_logsq) {
   return S;
 }
 
+// functions.jl:300-313 (compute_negative_part_l1): sum over cells and Gauss
+// points of (-f) w_q |e| where f < 0.
+double orc_negative_part_l1(const orc_mesh* ms, const double* fc) {
+  double neg = 0.0;
+  std::vector<double> fq, basis;
+  std::vector<int> gids;
+  for (int e2 = 0; e2 < ms->d2.ne; ++e2)
+    for (int e1 = 0; e1 < ms->d1.ne; ++e1) {
+      const double jac = (ms->d1.bp[e1 + 1] - ms->d1.bp[e1]) * (ms->d2.bp[e2 + 1] - ms->d2.bp[e2]);
+      element_eval(ms, e1, e2, fc, fq, basis, gids);
+      for (int b = 0; b < ms->nq; ++b)
+        for (int a = 0; a < ms->nq; ++a) {
+          const double f = fq[a + ms->nq * b];
+          if (f < 0.0) neg += (-f) * (ms->qw[a] * ms->qw[b]) * jac;
+        }
+    }
+  return neg;
+}
+
+// functions.jl:320-343 (compute_fs_minus_fp_l2): the cell masses m_e = sum of
+// w over the markers locate_particle places in cell e, in marker order; then
+// sqrt(sum over cells and Gauss points of (f - m_e / |e|)^2 w_q |e|).
+double orc_fs_minus_fp_l2(const orc_mesh* ms, const double* fc, const double* v, const double* w, int64_t N) {
+  const int ne1 = ms->d1.ne, ne2 = ms->d2.ne;
+  std::vector<double> mass((size_t)ne1 * ne2, 0.0);
+  for (int64_t a = 0; a < N; ++a) {
+    Loc L;
+    if (!locate(ms, v[a], v[N + a], L)) continue;
+    mass[(size_t)L.i + (size_t)ne1 * L.j] += w[a];
+  }
+  double sumsq = 0.0;
+  std::vector<double> fq, basis;
+  std::vector<int> gids;
+  for (int e2 = 0; e2 < ne2; ++e2)
+    for (int e1 = 0; e1 < ne1; ++e1) {
+      const double jac = (ms->d1.bp[e1 + 1] - ms->d1.bp[e1]) * (ms->d2.bp[e2 + 1] - ms->d2.bp[e2]);
+      const double fp = mass[(size_t)e1 + (size_t)ne1 * e2] / jac;
+      element_eval(ms, e1, e2, fc, fq, basis, gids);
+      for (int b = 0; b < ms->nq; ++b)
+        for (int a = 0; a < ms->nq; ++a) {
+          const double d = fq[a + ms->nq * b] - fp;
+          sumsq += d * d * (ms->qw[a] * ms->qw[b]) * jac;
+        }
+    }
+  return std::sqrt(sumsq);
+}
+
 // functions.jl:158-179 (compute_r!)
 void orc_compute_r(const orc_mesh* ms, const double* fc, int use_logsq, double* r) {
   std::fill(r, r + ms->n_dofs, 0.0);

@@ -95,9 +95,12 @@ def lib():
         L.orc_mesh_build.argtypes = [_dp, C.c_int, _dp, C.c_int, C.c_int, C.c_int, C.c_int]
         L.orc_mesh_free.argtypes = [C.c_void_p]
         L.orc_entropy.restype = C.c_double
+        L.orc_negative_part_l1.restype = C.c_double
+        L.orc_fs_minus_fp_l2.restype = C.c_double
         L.orc_landau_geff.restype = C.c_double
         for name in ("orc_mesh_info", "orc_mesh_tables", "orc_mass_dense", "orc_mass_solve",
                      "orc_eval_particle", "orc_l2_rhs", "orc_l2_project", "orc_entropy",
+                     "orc_negative_part_l1", "orc_fs_minus_fp_l2",
                      "orc_compute_r", "orc_compute_G", "orc_loggrad", "orc_landau_geff",
                      "orc_picard_map", "orc_predictor", "orc_step_anderson", "orc_step_defect", "orc_time_step"):
             getattr(L, name).argtypes = None
@@ -220,6 +223,18 @@ class Mesh:
     def entropy(self, fc, use_logsq: bool = False) -> float:
         fc = np.ascontiguousarray(fc, dtype=np.float64)
         return float(lib().orc_entropy(self.h, _p(fc), C.c_int(int(use_logsq))))
+
+    def negative_part_l1(self, fc) -> float:
+        """compute_negative_part_l1 (functions.jl:300-313)."""
+        fc = np.ascontiguousarray(fc, dtype=np.float64)
+        return float(lib().orc_negative_part_l1(self.h, _p(fc)))
+
+    def fs_minus_fp_l2(self, fc, v, w) -> float:
+        """compute_fs_minus_fp_l2 (functions.jl:320-343)."""
+        fc = np.ascontiguousarray(fc, dtype=np.float64)
+        v = colmajor(v)
+        w = np.ascontiguousarray(w, dtype=np.float64)
+        return float(lib().orc_fs_minus_fp_l2(self.h, _p(fc), _p(v), _p(w), C.c_int64(v.shape[0])))
 
     def compute_r(self, fc, use_logsq: bool = False):
         fc = np.ascontiguousarray(fc, dtype=np.float64)

@@ -221,3 +221,94 @@ def test_eval_particle_matches_scipy():
             assert abs(dx1[k] / h[0] - b1.derivative()(x) * b2(y)) < 1e-13
             assert abs(dx2[k] / h[1] - b1(x) * b2.derivative()(y)) < 1e-13
     assert m.eval_particle(6.0, 0.0) is None and m.eval_particle(0.0, -6.0) is None
+
+
+# ---- run-record diagnostics (functions.jl:300-343) ---------------------------
+
+def _gauss_sum(bp1, bp2, fc, fn):
+    """sum over cells and 6x6 Gauss points of fn(f, cell) w_q |e|, f from scipy
+    B-splines: an implementation independent of the oracle's extraction."""
+    xg, wg = np.polynomial.legendre.leggauss(6)
+    t1, t2 = knots(bp1, 2, 1), knots(bp2, 2, 1)
+    nd1, nd2 = len(t1) - 3, len(t2) - 3
+    C = np.asarray(fc).reshape(nd2, nd1)           # gid = i1 + nd1 * i2
+    tot = 0.0
+    for j in range(len(bp2) - 1):
+        for i in range(len(bp1) - 1):
+            x = bp1[i] + (xg + 1) / 2 * (bp1[i + 1] - bp1[i])
+            y = bp2[j] + (xg + 1) / 2 * (bp2[j + 1] - bp2[j])
+            jac = (bp1[i + 1] - bp1[i]) * (bp2[j + 1] - bp2[j])
+            Bx = np.array([BSpline(t1, np.eye(nd1)[k], 2)(x) for k in range(nd1)])
+            By = np.array([BSpline(t2, np.eye(nd2)[k], 2)(y) for k in range(nd2)])
+            F = By.T @ C @ Bx                        # F[b, a] = f(x_a, y_b)
+            W = np.outer(wg, wg).T / 4               # W[b, a] = w_a w_b
+            tot += np.sum(fn(F, i, j) * W) * jac
+    return tot
+
+
+def test_negative_part_closed_forms():
+    bp1, bp2 = NONUNI, uniform_bp(-2, 2, 4)
+    m = O.Mesh(bp1, bp2)
+    area = (bp1[-1] - bp1[0]) * (bp2[-1] - bp2[0])
+    assert m.negative_part_l1(np.full(m.n_dofs, 0.4)) == 0.0
+    assert abs(m.negative_part_l1(np.full(m.n_dofs, -0.4)) - 0.4 * area) < 1e-13
+    # f = x on [-2, 2]^2 with a breakpoint at the kink: int max(-x, 0) = 2 * 4
+    bp = uniform_bp(-2, 2, 4)
+    m2 = O.Mesh(bp, bp)
+    assert abs(m2.negative_part_l1(_coeffs_of(bp, bp, "x", "1")) - 8.0) < 1e-13
+
+
+def test_negative_part_vs_scipy_gauss_sum():
+    rng = np.random.default_rng(5)
+    bp1, bp2 = NONUNI, uniform_bp(-3, 3, 7)
+    m = O.Mesh(bp1, bp2)
+    fc = rng.standard_normal(m.n_dofs) * 0.3 + 0.1
+    ref = _gauss_sum(bp1, bp2, fc, lambda F, i, j: np.maximum(-F, 0.0))
+    assert ref > 0.1
+    assert abs(m.negative_part_l1(fc) - ref) < 1e-13 * max(1.0, ref)
+
+
+def test_fs_minus_fp_closed_forms():
+    bp1, bp2 = NONUNI, uniform_bp(-2, 2, 4)
+    m = O.Mesh(bp1, bp2)
+    c = 0.25
+    fc = np.full(m.n_dofs, c)                    # f_s = c (partition of unity)
+    h1, h2 = np.diff(bp1), np.diff(bp2)
+    cx = (bp1[:-1] + bp1[1:]) / 2
+    cy = (bp2[:-1] + bp2[1:]) / 2
+    X, Y = np.meshgrid(cx, cy, indexing="xy")     # one marker per cell, mass c |e|
+    v = np.c_[X.ravel(), Y.ravel()]
+    w = (c * np.outer(h2, h1)).ravel()
+    assert m.fs_minus_fp_l2(fc, v, w) < 1e-14
+    # markers outside the open box (and on its boundary) are not counted
+    v_out = np.r_[v, [[bp1[0], 0.0], [9.0, 0.1], [0.2, bp2[-1]]]]
+    w_out = np.r_[w, [5.0, 7.0, 3.0]]
+    assert m.fs_minus_fp_l2(fc, v_out, w_out) < 1e-14
+    # move marker 0's mass into cell 1: |d| = dm / |e| on both cells
+    dm = 0.3 * w[0]
+    w2 = w.copy()
+    w2[0] -= dm
+    w2[1] += dm
+    e0, e1 = h1[0] * h2[0], h1[1] * h2[0]
+    exact = np.sqrt(dm ** 2 / e0 + dm ** 2 / e1)
+    assert abs(m.fs_minus_fp_l2(fc, v, w2) - exact) < 1e-14 * max(1.0, exact) + 1e-15
+
+
+def test_fs_minus_fp_vs_numpy_histogram():
+    rng = np.random.default_rng(8)
+    bp1, bp2 = NONUNI, uniform_bp(-3, 3, 7)
+    m = O.Mesh(bp1, bp2)
+    N = 3000
+    v = np.c_[rng.uniform(-7, 7, N), rng.uniform(-3.5, 3.5, N)]   # some outside
+    w = rng.uniform(0.5, 1.5, N) / N
+    fc = m.l2_project(v, w)
+    inside = (v[:, 0] > bp1[0]) & (v[:, 0] < bp1[-1]) & (v[:, 1] > bp2[0]) & (v[:, 1] < bp2[-1])
+    i = np.searchsorted(bp1, v[inside, 0], side="right") - 1
+    j = np.searchsorted(bp2, v[inside, 1], side="right") - 1
+    ne1 = len(bp1) - 1
+    mass = np.bincount(i + ne1 * j, weights=w[inside], minlength=ne1 * (len(bp2) - 1))
+    jac = np.outer(np.diff(bp2), np.diff(bp1)).ravel()
+    fp = mass / jac
+    ref = np.sqrt(_gauss_sum(bp1, bp2, fc, lambda F, ii, jj: (F - fp[ii + ne1 * jj]) ** 2))
+    assert ref > 1e-3
+    assert abs(m.fs_minus_fp_l2(fc, v, w) - ref) < 1e-12 * ref
