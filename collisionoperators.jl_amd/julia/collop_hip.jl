@@ -138,3 +138,23 @@ function eval_loggrad_gpu!(ws, g, v_parts, f_coeffs)
     g .= out
     return nothing
 end
+
+# Run-record diagnostics (main.jl:306-307): device versions of
+# compute_negative_part_l1 (functions.jl:300-313) and compute_fs_minus_fp_l2
+# (functions.jl:320-343), taking the coefficient vector the field is built from.
+function compute_negative_part_l1_gpu(ws, f_coeffs)
+    ctx = _collop_mesh!(ws)
+    out = Ref{Float64}(0.0)
+    _collop_check(ccall((:collop_negative_part_l1, _COLLOP_LIB), Cint,
+        (Ptr{Cvoid}, Ptr{Float64}, Ref{Float64}), ctx, Vector{Float64}(f_coeffs), out))
+    return out[]
+end
+
+function compute_fs_minus_fp_l2_gpu(ws, f_coeffs, v_parts, w_parts)
+    ctx = _collop_mesh!(ws)
+    out = Ref{Float64}(0.0)
+    _collop_check(ccall((:collop_fs_minus_fp_l2, _COLLOP_LIB), Cint,
+        (Ptr{Cvoid}, Ptr{Float64}, Ptr{Float64}, Int64, Ptr{Float64}, Ref{Float64}),
+        ctx, _colmajor(v_parts), Vector{Float64}(w_parts), size(v_parts, 1), Vector{Float64}(f_coeffs), out))
+    return out[]
+end
