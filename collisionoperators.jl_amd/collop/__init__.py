@@ -6,10 +6,10 @@ interface: the Workspace tables (MantisWrappers.jl), the GPU hook functions
 (collision_gpu.jl, projection_gpu.jl) and the device-resident implicit step
 (solver.jl, main.jl).
 """
-from ._lib import (CollopError, Context, StepParams, StepStats, load, PREC_F64, PREC_F32C, WARM_EULER,  # noqa: F401
-                   WARM_MLP)
+from ._lib import (CollopError, Context, StepParams, StepStats, load, MODEL_LANDAU, MODEL_LB, PREC_F64,  # noqa: F401
+                   PREC_F32C, WARM_EULER, WARM_MLP)
 from .hooks import (compute_collision_gpu, compute_collision_gpu32, compute_collision_metric_gpu,  # noqa: F401
-                    compute_fs_minus_fp_l2_gpu, compute_G_gpu, compute_negative_part_l1_gpu,
+                    compute_fs_minus_fp_l2_gpu, compute_G_gpu, compute_lb_rhs_gpu, compute_negative_part_l1_gpu,
                     compute_r_and_entropy_gpu, enable_gpu, eval_loggrad_gpu, l2_project_gpu, l2_rhs_gpu,
                     mass_solve_gpu)
 from .ic import ensemble_markers, initial_markers, markers  # noqa: F401

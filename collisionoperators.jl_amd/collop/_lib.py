@@ -29,6 +29,8 @@ WARM_EULER = 0
 WARM_MLP = 1
 SOLVER_ANDERSON = 0
 SOLVER_DEFECT = 1
+MODEL_LANDAU = 0
+MODEL_LB = 1
 PAIR_ORDERED = 0
 PAIR_SYMMETRIC = 1
 
@@ -74,6 +76,8 @@ class StepParams(C.Structure):
         ("dc_outer_m", C.c_int32),
         ("dc_near_theta", C.c_double),
         ("dc_metric_reuse", C.c_int32),
+        ("collision_model", C.c_int32),
+        ("nu", C.c_double),
     ]
 
 
@@ -128,6 +132,7 @@ SIGNATURES = [
     ("collop_loggrad_gather", C.c_int, [_vp, _dp, C.c_int64, _dp, C.c_double, C.c_double, _dp]),
     ("collop_quadrature", C.c_int, [_vp, _dp, C.c_int, _dp, _dp]),
     ("collop_mass_solve", C.c_int, [_vp, _dp, _dp]),
+    ("collop_lb_rhs", C.c_int, [_vp, _dp, _dp, _dp, C.c_int64, C.c_double, _dp, _dp]),
     ("collop_negative_part_l1", C.c_int, [_vp, _dp, _dp]),
     ("collop_fs_minus_fp_l2", C.c_int, [_vp, _dp, _dp, C.c_int64, _dp, _dp]),
     ("collop_state_set", C.c_int, [_vp, _dp, _dp, C.c_int64]),

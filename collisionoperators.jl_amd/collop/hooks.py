@@ -143,6 +143,23 @@ def mass_solve_gpu(ws: Workspace, b):
     return x
 
 
+def compute_lb_rhs_gpu(ws: Workspace, dot_v, v_parts, w_parts, g, nu: float = 1.0):
+    """compute_moments + compute_drift_multipliers + compute_LB_velocity!
+    (functions.jl:66-115) on the device: writes dot_v = -nu (g + A + B v) and
+    returns (A1, A2, B)."""
+    ws.activate()
+    v, N = _markers(v_parts)
+    w = _vec(w_parts, N)
+    g = _colmajor(g, (N, 2))
+    tmp = _out(dot_v, 2, N)
+    mult = np.zeros(3)
+    ws.ctx.check(ws.ctx.lib.collop_lb_rhs(ws.ctx.h, _lib.dptr(v), _lib.dptr(w), _lib.dptr(g), N, float(nu),
+                                          _lib.dptr(tmp), _lib.dptr(mult)))
+    if tmp is not dot_v:
+        dot_v[...] = tmp
+    return float(mult[0]), float(mult[1]), float(mult[2])
+
+
 def compute_negative_part_l1_gpu(ws: Workspace, f_coeffs) -> float:
     """compute_negative_part_l1 (functions.jl:300-313) of f_s = sum_k f_coeffs[k] phi_k."""
     ws.activate()

@@ -59,6 +59,8 @@ class SimParameters:
     dc_outer_m: int = 0              # extension: outer Anderson window (0 = the reference's outer iteration)
     dc_near_theta: float = 0.0       # extension: exact near pairs in the frozen map (0 = none)
     dc_metric_reuse: int = 0         # extension: 0 A at every outer map, 1 FP64 maps reuse the FP32c A, 2 first only
+    collision_model: str = "landau"  # :landau | :lb  (Parameters.jl:50)
+    nu: float = 1.0                  # LB collision frequency (Parameters.jl:53)
 
     def step_params(self, precision: int | None = None, warmstart: int | None = None,
                     solver: int | None = None) -> _lib.StepParams:
@@ -69,6 +71,8 @@ class SimParameters:
             if self.solver not in ("anderson", "defect"):
                 raise ValueError(f"unknown solver={self.solver} (anderson | defect)")
             solver = _lib.SOLVER_DEFECT if self.solver == "defect" else _lib.SOLVER_ANDERSON
+        if self.collision_model not in ("landau", "lb"):
+            raise ValueError(f"unknown collision_model={self.collision_model} (landau | lb)")
         return _lib.StepParams(
             dt=self.DT, m_anderson=self.m_anderson, max_iter=self.max_iter, tol=self.tol,
             abs_floor=self.abs_floor, stag_window=self.stag_window, stag_rel_tol=self.stag_rel_tol,
@@ -78,7 +82,8 @@ class SimParameters:
             exit_picard_step=int(self.exit_picard_step), use_logsq=int(self.use_logsq),
             precision=prec, warmstart=warm, f32_floor_factor=16.0, f32_keep_window=1, record_training=0,
             solver=solver, dc_eta=self.dc_eta, dc_max_inner=self.dc_max_inner, dc_stag_window=self.dc_stag_window,
-            dc_outer_m=self.dc_outer_m, dc_near_theta=self.dc_near_theta, dc_metric_reuse=self.dc_metric_reuse)
+            dc_outer_m=self.dc_outer_m, dc_near_theta=self.dc_near_theta, dc_metric_reuse=self.dc_metric_reuse,
+            collision_model=_lib.MODEL_LB if self.collision_model == "lb" else _lib.MODEL_LANDAU, nu=self.nu)
 
 
 def parse_overrides(p: SimParameters, args: list[str]) -> SimParameters:
@@ -125,6 +130,12 @@ CONFIGS = {
     "bimaxwellian_n262144": dict(N_PARTICLES=262144, ic="bimaxwellian", σ1=math.sqrt(2.0), σ2=math.sqrt(0.5),
                                  weight_jitter=0.2, bp1=_uniform(-8, 8, 64), bp2=_uniform(-8, 8, 64), DT=0.01,
                                  N_STEPS=20, seed=42),
+    # Lenard-Bernstein on the bimodal IC of parameters_LB_bimodal_v1.jl: N(+-2, 1) x N(0, 1), nu = 1,
+    # non-uniform breakpoints (0.4 cells over [-5, 5] in v1, [-4.5, 4.5] in v2, end cells to +-6), dt 0.001, m 8
+    "lb_bimodal_n40000": dict(N_PARTICLES=40_000, ic="bimodal", v1_peak=2.0, σ1=1.0, σ2=1.0,
+                              bp1=np.r_[-6.0, np.linspace(-5.0, 5.0, 26), 6.0],
+                              bp2=np.r_[-6.0, np.arange(-4.5, 4.5 + 1e-9, 0.4), 6.0],
+                              collision_model="lb", nu=1.0, DT=0.001, N_STEPS=25000, m_anderson=8, seed=42),
 }
 
 

@@ -284,6 +284,7 @@ int ensure_scratch(collop_ctx* ctx, int64_t N) {
   ENSURE(ctx->b_statp, (size_t)epi_blocks((int)N) * kStatWidth * 8);
   ENSURE(ctx->b_stat, kStatWidth * 8);
   ENSURE(ctx->b_stmp, (size_t)2 * m.n_dofs * 8);
+  ENSURE(ctx->b_lbpart, (size_t)lb_blocks((int)N) * kLbSums * 8);
   return COLLOP_OK;
 }
 
@@ -325,7 +326,7 @@ int collop_destroy(collop_ctx* ctx) {
   free_mesh(ctx);
   for (DBuf* b : {&ctx->b_v, &ctx->b_w, &ctx->b_G, &ctx->b_out, &ctx->b_vmid, &ctx->b_dv, &ctx->b_dS, &ctx->b_s64,
                   &ctx->b_s32, &ctx->b_part, &ctx->b_spart_sc, &ctx->b_mom, &ctx->b_dpart, &ctx->b_iota,
-                  &ctx->b_spart_q, &ctx->b_vec, &ctx->b_scal, &ctx->b_statp, &ctx->b_stat, &ctx->b_stmp, &ctx->s_v, &ctx->s_w,
+                  &ctx->b_spart_q, &ctx->b_vec, &ctx->b_scal, &ctx->b_statp, &ctx->b_stat, &ctx->b_stmp, &ctx->b_lbpart, &ctx->s_v, &ctx->s_w,
                   &ctx->s_v1, &ctx->s_Gv, &ctx->s_r, &ctx->s_rp, &ctx->s_Gp, &ctx->s_best, &ctx->s_dF, &ctx->s_dG,
                   &ctx->s_fc, &ctx->s_dotv, &ctx->s_Gvo, &ctx->s_Gk, &ctx->s_FA, &ctx->s_D, &ctx->s_Ge, &ctx->s_vmk, &ctx->s_xk, &ctx->s_oU,
                   &ctx->s_oUp, &ctx->s_of, &ctx->s_ofp, &ctx->s_odF, &ctx->s_odG, &ctx->b_near_list, &ctx->b_near_count,
@@ -694,6 +695,36 @@ int collop_fs_minus_fp_l2(collop_ctx* ctx, const double* v, const double* w, int
   return COLLOP_OK;
 }
 
+int collop_lb_rhs(collop_ctx* ctx, const double* v, const double* w, const double* g, int64_t N, double nu,
+                  double* dot_v, double* mult) {
+  int r = need_mesh(ctx);
+  if (r) return r;
+  if ((r = check_N(ctx, N))) return r;
+  if (N == 0) return COLLOP_OK;
+  if (!v || !w || !g || !dot_v) return fail(ctx, COLLOP_ERR_INVALID, "null array");
+  if (!std::isfinite(nu)) return fail(ctx, COLLOP_ERR_INVALID, "nu must be finite");
+  HIPC(hipSetDevice(ctx->device));
+  if ((r = ensure_scratch(ctx, N))) return r;
+  HIPC(hipMemcpyAsync(ctx->b_v.p, v, 2 * N * 8, hipMemcpyHostToDevice, ctx->stream));
+  HIPC(hipMemcpyAsync(ctx->b_w.p, w, N * 8, hipMemcpyHostToDevice, ctx->stream));
+  HIPC(hipMemcpyAsync(ctx->b_G.p, g, 2 * N * 8, hipMemcpyHostToDevice, ctx->stream));
+  double* coef = P<double>(ctx->b_scal) + 10;
+  launch_lb_rhs(ctx->mv, P<double>(ctx->b_v), nullptr, P<double>(ctx->b_w), nullptr, (int)N, nullptr, 0,
+                P<double>(ctx->b_scal), nu, P<double>(ctx->b_G), P<double>(ctx->b_lbpart), coef,
+                P<double>(ctx->b_out), ctx->stream);
+  CHECK_LAUNCH();
+  double h[5];
+  HIPC(hipMemcpyAsync(dot_v, ctx->b_out.p, 2 * N * 8, hipMemcpyDeviceToHost, ctx->stream));
+  HIPC(hipMemcpyAsync(h, coef, sizeof(h), hipMemcpyDeviceToHost, ctx->stream));
+  HIPC(hipStreamSynchronize(ctx->stream));
+  if (mult) {
+    mult[0] = h[1];
+    mult[1] = h[2];
+    mult[2] = h[3];
+  }
+  return COLLOP_OK;
+}
+
 int collop_mass_solve(collop_ctx* ctx, const double* b, double* x) {
   int r = need_mesh(ctx);
   if (r) return r;
@@ -937,6 +968,52 @@ int frozen_map_dev(collop_ctx* ctx, const collop_step_params& p, int col, int hi
                       P<double>(ctx->s_Gv), col, hist);
 }
 
+// One Lenard-Bernstein Picard map (solver.jl:176-227, LB branch): the
+// projections of v_mid (and of v_in for Gonzalez), g = clamped grad f / f at
+// v_mid (LOGGRAD_FN), lambda and the drift multipliers, v_dot = -nu (G_eff + A
+// + B v_mid), then Gv = v0 + dt v_dot with the Anderson epilogue.
+int lb_map_dev(collop_ctx* ctx, const collop_step_params& p, const double* vin, int col, int hist) {
+  const MeshView& m = ctx->mv;
+  const int n = (int)ctx->N;
+  const int nsb = scatter_blocks(n);
+  double* vec = P<double>(ctx->b_vec);
+  double* sc = P<double>(ctx->b_scal);
+  const int fields = p.use_gonzalez ? 2 : 1;
+  launch_scatter(m, P<double>(ctx->s_v), vin, P<double>(ctx->s_w), n, 2, fields, ctx->w_abs_sum,
+                 P<double>(ctx->b_vmid), P<double>(ctx->b_dv), P<long long>(ctx->b_spart_sc), nullptr, ctx->stream);
+  CHECK_LAUNCH();
+  if (m.nd1 <= 256 && m.nd2 <= 256)
+    launch_solve_split(m, P<long long>(ctx->b_spart_sc), nsb, ctx->w_abs_sum, nullptr, fields, vec,
+                       P<double>(ctx->b_stmp), ctx->stream);
+  else
+    launch_reduce_solve(m, P<long long>(ctx->b_spart_sc), nsb, ctx->w_abs_sum, nullptr, fields, vec, ctx->stream);
+  CHECK_LAUNCH();
+  if (p.use_gonzalez) {  // S(f_in) partials (solver.jl:196-197)
+    launch_quad1(m, vec + m.n_dofs, nullptr, 1, p.use_logsq, nullptr, P<double>(ctx->b_spart_q), ctx->stream);
+    CHECK_LAUNCH();
+  }
+  launch_lb_rhs(m, P<double>(ctx->b_vmid), vec, P<double>(ctx->s_w), P<double>(ctx->b_dv), n,
+                p.use_gonzalez ? P<double>(ctx->b_spart_q) : nullptr, p.use_gonzalez ? quad_blocks(m) : 0, sc + 0,
+                p.nu, P<double>(ctx->b_G), P<double>(ctx->b_lbpart), sc + 10, P<double>(ctx->b_dS), ctx->stream);
+  CHECK_LAUNCH();
+  return epilogue_dev(ctx, P<double>(ctx->b_dS), 1, P<double>(ctx->s_v), p.dt, vin, P<double>(ctx->s_Gv), col, hist);
+}
+
+// The LB predictor (main.jl:233-238): g at v from the state's projection, the
+// multipliers at v, v1 = v + dt v_dot.
+int lb_predictor_dev(collop_ctx* ctx, const collop_step_params& p) {
+  const int n = (int)ctx->N;
+  double* sc = P<double>(ctx->b_scal);
+  launch_lb_rhs(ctx->mv, P<double>(ctx->s_v), P<double>(ctx->s_fc), P<double>(ctx->s_w), nullptr, n, nullptr, 0,
+                sc + 0, p.nu, P<double>(ctx->b_G), P<double>(ctx->b_lbpart), sc + 10, P<double>(ctx->b_dS),
+                ctx->stream);
+  CHECK_LAUNCH();
+  launch_euler(P<double>(ctx->b_dS), 1, P<double>(ctx->s_v), p.dt, n, P<double>(ctx->s_v1), P<double>(ctx->s_dotv),
+               ctx->stream);
+  CHECK_LAUNCH();
+  return COLLOP_OK;
+}
+
 }  // namespace
 
 namespace collop {
@@ -1033,6 +1110,14 @@ int validate_params(collop_ctx* ctx, const collop_step_params* p) {
        !std::isfinite(p->dc_near_theta) || p->dc_metric_reuse < 0 || p->dc_metric_reuse > 2))
     return fail(ctx, COLLOP_ERR_INVALID,
                 "defect correction needs 0 <= dc_outer_m <= 8, dc_near_theta >= 0 and dc_metric_reuse in {0, 1, 2}");
+  if (p->collision_model != COLLOP_MODEL_LANDAU && p->collision_model != COLLOP_MODEL_LB)
+    return fail(ctx, COLLOP_ERR_INVALID, "unknown collision_model");
+  if (p->collision_model == COLLOP_MODEL_LB &&
+      (p->solver != COLLOP_SOLVER_ANDERSON || p->precision != COLLOP_PREC_F64 || p->warmstart != COLLOP_WARM_EULER ||
+       p->record_training || !std::isfinite(p->nu)))
+    return fail(ctx, COLLOP_ERR_INVALID,
+                "the Lenard-Bernstein model runs the Anderson solver with FP64 maps and the Euler predictor "
+                "(no pair sum, no metric, no learned warm start) and needs a finite nu");
   if (p->record_training && ctx->m_size + ctx->N > ctx->m_cap)
     return fail(ctx, COLLOP_ERR_INVALID, "training dataset full (collop_mlp_dataset_alloc)");
   return COLLOP_OK;
@@ -1237,18 +1322,23 @@ int step_once(collop_ctx* ctx, const collop_step_params& p, collop_step_stats* o
   const int n = (int)ctx->N;
   const size_t n2b = (size_t)2 * n * 8;
   int r;
-  // ---- predictor: v1 = v + dt vdot(v, G), G = grad(M^{-1} r(f_s(v)))
-  if ((r = entropy_seed_gradient(ctx, P<double>(ctx->s_v), P<double>(ctx->s_fc), p.use_logsq))) return r;
-  // (the predictor is only the initial guess: F32C runs take its pair sum in FP32c)
   const MeshView& m = ctx->mv;
-  const int pk = p.precision == COLLOP_PREC_F32C ? 1 : 0;
-  launch_pack_src(P<double>(ctx->s_v), P<double>(ctx->b_G), P<double>(ctx->s_w), n, m.lo1, m.hi1, m.lo2, m.hi2,
-                  pk == 0 ? P<Src64>(ctx->b_s64) : nullptr, pk == 1 ? P<Src32>(ctx->b_s32) : nullptr, ctx->stream);
-  CHECK_LAUNCH();
-  if ((r = run_pair(ctx, pk, n, P<double>(ctx->b_part)))) return r;
-  launch_euler(P<double>(ctx->b_part), pk == 0 ? ctx->tile64.slices : ctx->tile32.slices, P<double>(ctx->s_v), p.dt,
-               n, P<double>(ctx->s_v1), P<double>(ctx->s_dotv), ctx->stream);
-  CHECK_LAUNCH();
+  const bool lb = p.collision_model == COLLOP_MODEL_LB;
+  if (lb) {
+    if ((r = lb_predictor_dev(ctx, p))) return r;
+  } else {
+    // ---- predictor: v1 = v + dt vdot(v, G), G = grad(M^{-1} r(f_s(v)))
+    if ((r = entropy_seed_gradient(ctx, P<double>(ctx->s_v), P<double>(ctx->s_fc), p.use_logsq))) return r;
+    // (the predictor is only the initial guess: F32C runs take its pair sum in FP32c)
+    const int pk = p.precision == COLLOP_PREC_F32C ? 1 : 0;
+    launch_pack_src(P<double>(ctx->s_v), P<double>(ctx->b_G), P<double>(ctx->s_w), n, m.lo1, m.hi1, m.lo2, m.hi2,
+                    pk == 0 ? P<Src64>(ctx->b_s64) : nullptr, pk == 1 ? P<Src32>(ctx->b_s32) : nullptr, ctx->stream);
+    CHECK_LAUNCH();
+    if ((r = run_pair(ctx, pk, n, P<double>(ctx->b_part)))) return r;
+    launch_euler(P<double>(ctx->b_part), pk == 0 ? ctx->tile64.slices : ctx->tile32.slices, P<double>(ctx->s_v),
+                 p.dt, n, P<double>(ctx->s_v1), P<double>(ctx->s_dotv), ctx->stream);
+    CHECK_LAUNCH();
+  }
   // ---- warm-start hook (main.jl:249-254): features of v, optional record,
   // optional MLP correction of the predictor
   float* rec = nullptr;
@@ -1301,7 +1391,11 @@ int step_once(collop_ctx* ctx, const collop_step_params& p, collop_step_stats* o
   const char* trace = std::getenv("COLLOP_TRACE") ? "run" : nullptr;
   while (p.solver == COLLOP_SOLVER_ANDERSON && !an.done) {
     const AndersonPlan pl = an.plan();
-    if ((r = picard_map_dev(ctx, p, pl.kind, P<double>(ctx->s_v1), pl.col, pl.hist))) return r;
+    if (lb) {
+      if ((r = lb_map_dev(ctx, p, P<double>(ctx->s_v1), pl.col, pl.hist))) return r;
+    } else if ((r = picard_map_dev(ctx, p, pl.kind, P<double>(ctx->s_v1), pl.col, pl.hist))) {
+      return r;
+    }
     AndersonAct act;
     if (an.after_map(ctx->h_stat, act, trace) != COLLOP_OK)
       return fail(ctx, COLLOP_ERR_NONFINITE, "non-finite residual in Picard map " + std::to_string(an.maps()));

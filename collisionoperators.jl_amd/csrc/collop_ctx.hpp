@@ -36,7 +36,7 @@ struct collop_ctx {
   std::vector<DBuf*> mesh_bufs;
   // ---- scratch buffers by name (grow-only)
   DBuf b_v, b_w, b_G, b_out, b_vmid, b_dv, b_dS, b_s64, b_s32, b_part, b_spart_sc, b_mom, b_dpart, b_iota, b_spart_q,
-      b_vec, b_scal, b_statp, b_stat, b_stmp;
+      b_vec, b_scal, b_statp, b_stat, b_stmp, b_lbpart;
   // ---- device-resident state
   bool has_state = false;
   int64_t N = 0;

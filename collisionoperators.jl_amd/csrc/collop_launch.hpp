@@ -140,6 +140,16 @@ void launch_rhs(const MeshView& m, const long long* part, int nblocks, double w_
 // non-null) gets (1 + log f) w_q |e| (0 where f is below the floor) and spart[y]
 // (if non-null) partial sums of -f log f w_q |e|.  coeff[y] points at n_dofs.
 int quad_blocks(const MeshView& m);
+// Lenard-Bernstein right-hand side at x (device pointers): with coeff, g = the
+// clamped log-gradient of f = sum coeff phi at x (fs_floor 1e-30, g_max 100),
+// else g is given; then the block sums (part: lb_blocks(N) x kLbSums), the
+// reduction, lambda (qpart: nq entropy partials of S(f_in), nq = 0: none), the
+// 3x3 drift multipliers (coef = lambda, A1, A2, B, S1) and dot = -nu (G_eff + A + B x).
+constexpr int kLbSums = 12;
+int lb_blocks(int N);
+void launch_lb_rhs(const MeshView& m, const double* x, const double* coeff, const double* w, const double* dv, int N,
+                   const double* qpart, int nq, const double* S0, double nu, double* g, double* part, double* coef,
+                   double* dot, hipStream_t st);
 // Run-record diagnostics of f_s = sum c_k phi_k: out[0] = int max(-f_s, 0) and,
 // when v is given, out[1] = ||f_s - f_p||_2^2 with f_p the markers' cell
 // histogram density (device pointers; part: scatter_blocks(N) x ne1*ne2,

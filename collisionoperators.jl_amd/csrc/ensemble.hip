@@ -632,6 +632,8 @@ int collop_ensemble_step(collop_ctx* ctx, const collop_step_params* params, int 
   int r = need_ens(ctx);
   if (r) return r;
   if ((r = check_params(ctx, params))) return r;
+  if (params->collision_model != COLLOP_MODEL_LANDAU)
+    return fail(ctx, COLLOP_ERR_UNSUPPORTED, "the batched ensemble steps the Landau model only");
   if (n_steps < 0) return fail(ctx, COLLOP_ERR_INVALID, "n_steps must be >= 0");
   HIPC(hipSetDevice(ctx->device));
   for (int s = 0; s < n_steps; ++s)

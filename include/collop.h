@@ -114,6 +114,15 @@ int collop_quadrature(collop_ctx* ctx, const double* f_coeffs, int use_logsq, do
 /* x = M^{-1} b (the reference's ws.M_lu \ b, functions.jl:132, solver.jl:67). */
 int collop_mass_solve(collop_ctx* ctx, const double* b, double* x);
 
+/* Lenard-Bernstein right-hand side at markers v with log-density gradient g
+ * (N x 2, e.g. from collop_loggrad_gather): the moments n, U, Q of (v, w)
+ * (compute_moments, functions.jl:66-82), the drift multipliers (A1, A2, B) of
+ * the 3x3 system enforcing momentum and energy conservation
+ * (compute_drift_multipliers, functions.jl:84-106) and dot_v = -nu (g + A +
+ * B v) (compute_LB_velocity!, functions.jl:108-115).  mult[3] may be NULL. */
+int collop_lb_rhs(collop_ctx* ctx, const double* v, const double* w, const double* g, int64_t N, double nu,
+                  double* dot_v, double* mult);
+
 /* Run-record diagnostics, the conservation CSV's neg_part and fp_minus_fs
  * columns (io.jl:106-107, written per step at main.jl:306-307):
  *   collop_negative_part_l1: int max(-f_s, 0) dv by the N_QUAD^2 Gauss rule per
@@ -149,6 +158,14 @@ typedef enum {
                                  full map with the metric A; an inner Anderson
                                  solve on the frozen-metric map needs no pair sum */
 } collop_solver;
+
+typedef enum {
+  COLLOP_MODEL_LANDAU = 0,  /* the Landau pair sum (collision_model = :landau)  */
+  COLLOP_MODEL_LB = 1       /* Lenard-Bernstein drift, O(N) per map
+                               (collision_model = :lb; functions.jl:66-115,
+                               solver.jl:185-222): Anderson solver, FP64,
+                               Euler predictor only                             */
+} collop_model;
 
 typedef struct {
   double dt;                /* DT                                               */
@@ -194,6 +211,8 @@ typedef struct {
                                keep the last A): 0 = every one (the reference),
                                1 = all but the FP64 maps after an FP32c phase,
                                2 = the step's first only                        */
+  int32_t collision_model;  /* collop_model (Parameters.jl:50)                  */
+  double nu;                /* LB collision frequency (Parameters.jl:53, 1.0)   */
 } collop_step_params;
 
 typedef struct {

@@ -293,6 +293,8 @@ struct orc_solver_params {
   int dc_stag_window;
   int dc_outer_m;      // extension: Anderson window of the outer iteration (0 = plain outer iteration)
   double dc_near_theta;  // extension: pairs with w_a / |d| > theta are exact in the frozen map (0 = none)
+  int collision_model;   // 0 :landau, 1 :lb (Parameters.jl:50)
+  double nu;             // LB collision frequency (Parameters.jl:53)
 };
 
 struct orc_step_stats {
@@ -853,9 +855,111 @@ double landau_geff(const orc_mesh* ms, double* Geff, const double* vin, const do
   return corr;
 }
 
-// solver.jl:176-227 (picard_map!, Landau branch)
+// ---- Lenard-Bernstein (functions.jl:66-115) ----------------------------------
+
+// functions.jl:66-82 (compute_moments): n, U1, U2, Q in marker order
+void lb_moments(const double* v, const double* w, int64_t N, double* m4) {
+  double n = 0.0, U1 = 0.0, U2 = 0.0, Q = 0.0;
+  for (int64_t a = 0; a < N; ++a) {
+    const double wa = w[a], x = v[a], y = v[N + a];
+    n += wa;
+    U1 += wa * x;
+    U2 += wa * y;
+    Q += wa * (x * x + y * y);
+  }
+  m4[0] = n;
+  m4[1] = U1;
+  m4[2] = U2;
+  m4[3] = Q;
+}
+
+// functions.jl:84-106 (compute_drift_multipliers): Sg, P in marker order, then
+// Mmat \ rhs, which for a square dense matrix is LU with partial pivoting.
+void lb_multipliers(const double* v, const double* w, const double* g, int64_t N, const double* m4, double* AB) {
+  double Sg1 = 0.0, Sg2 = 0.0, Pd = 0.0;
+  for (int64_t a = 0; a < N; ++a) {
+    const double g1 = g[a], g2 = g[N + a], wa = w[a];
+    Sg1 += wa * g1;
+    Sg2 += wa * g2;
+    Pd += wa * (v[a] * g1 + v[N + a] * g2);
+  }
+  double M[3][3] = {{m4[0], 0.0, m4[1]}, {0.0, m4[0], m4[2]}, {m4[1], m4[2], m4[3]}};
+  double r[3] = {-Sg1, -Sg2, -Pd};
+  for (int k = 0; k < 3; ++k) {
+    int pr = k;
+    for (int i = k + 1; i < 3; ++i)
+      if (std::fabs(M[i][k]) > std::fabs(M[pr][k])) pr = i;
+    if (pr != k) {
+      for (int j = 0; j < 3; ++j) std::swap(M[k][j], M[pr][j]);
+      std::swap(r[k], r[pr]);
+    }
+    for (int i = k + 1; i < 3; ++i) {
+      const double f = M[i][k] / M[k][k];
+      for (int j = k; j < 3; ++j) M[i][j] -= f * M[k][j];
+      r[i] -= f * r[k];
+    }
+  }
+  for (int i = 2; i >= 0; --i) {
+    double t = r[i];
+    for (int j = i + 1; j < 3; ++j) t -= M[i][j] * AB[j];
+    AB[i] = t / M[i][i];
+  }
+}
+
+// functions.jl:108-115 (compute_LB_velocity!)
+void lb_velocity(double* dot_v, const double* v, const double* g, int64_t N, const double* AB, double nu) {
+  for (int64_t a = 0; a < N; ++a) {
+    dot_v[a] = -nu * (g[a] + AB[0] + AB[2] * v[a]);
+    dot_v[N + a] = -nu * (g[N + a] + AB[1] + AB[2] * v[N + a]);
+  }
+}
+
+// The right-hand side at v with gradient g: moments, multipliers, velocity.
+void lb_rhs(double* dot_v, const double* v, const double* w, const double* g, int64_t N, double nu, double* AB) {
+  double m4[4];
+  lb_moments(v, w, N, m4);
+  lb_multipliers(v, w, g, N, m4, AB);
+  lb_velocity(dot_v, v, g, N, AB, nu);
+}
+
+// solver.jl:185-210 (picard_map!, LB branch): G_eff = clamped grad f / f of the
+// projection of v_mid (LOGGRAD_FN), with the Gonzalez correction
+// G_eff -= lambda dv / w, lambda = (S(v_in) - S0 + sum w dv.G_eff) / |dv|^2.
+void lb_geff(const orc_mesh* ms, double* Geff, const double* vin, const double* v0, const double* w, int64_t N,
+             double S0, Scratch& s, int use_gonzalez, int use_logsq) {
+  for (int64_t i = 0; i < 2 * N; ++i) {
+    s.vmid[i] = 0.5 * (v0[i] + vin[i]);
+    s.dv[i] = vin[i] - v0[i];
+  }
+  orc_l2_project(ms, s.vmid.data(), w, N, s.fbuf.data());
+  orc_loggrad(ms, s.vmid.data(), N, s.fbuf.data(), 1e-30, 100.0, Geff);
+  if (!use_gonzalez) return;
+  orc_l2_project(ms, vin, w, N, s.fbuf.data());
+  const double S1 = orc_entropy(ms, s.fbuf.data(), use_logsq);
+  double dot = 0.0, nrm2 = 0.0;
+  for (int64_t a = 0; a < N; ++a) {
+    dot -= w[a] * (s.dv[a] * Geff[a] + s.dv[N + a] * Geff[N + a]);
+    nrm2 += s.dv[a] * s.dv[a] + s.dv[N + a] * s.dv[N + a];
+  }
+  const double lam = nrm2 > 1e-30 ? (S1 - S0 - dot) / nrm2 : 0.0;
+  for (int64_t a = 0; a < N; ++a) {
+    const double iw = 1.0 / w[a];
+    Geff[a] -= lam * s.dv[a] * iw;
+    Geff[N + a] -= lam * s.dv[N + a] * iw;
+  }
+}
+
+// solver.jl:176-227 (picard_map!): the Landau branch, or the LB branch
+// (solver.jl:185-210, 218-222) when collision_model = 1
 void picard_map(const orc_mesh* ms, double* vout, const double* vin, const double* v0, const double* w, int64_t N,
                 double S0, double dt, Scratch& s, const orc_solver_params& P) {
+  if (P.collision_model == 1) {
+    lb_geff(ms, s.Geff.data(), vin, v0, w, N, S0, s, P.use_gonzalez, P.use_logsq);
+    double AB[3];
+    lb_rhs(s.dotv.data(), s.vmid.data(), w, s.Geff.data(), N, P.nu, AB);
+    for (int64_t i = 0; i < 2 * N; ++i) vout[i] = v0[i] + dt * s.dotv[i];
+    return;
+  }
   landau_geff(ms, s.Geff.data(), vin, v0, w, N, S0, s, P.use_gonzalez, P.use_logsq);
   double box[4];
   box_of(ms, box);
@@ -939,6 +1043,19 @@ double orc_landau_geff(const orc_mesh* ms, const double* vin, const double* v0, 
   return lam;
 }
 
+// functions.jl:66-115: moments, drift multipliers (A1, A2, B) and the LB
+// velocity at v with log-density gradient g.
+void orc_lb_rhs(const double* v, const double* w, const double* g, int64_t N, double nu, double* dot_v, double* AB) {
+  lb_rhs(dot_v, v, w, g, N, nu, AB);
+}
+
+// solver.jl:185-210: the LB G_eff at v_mid = (v0 + vin) / 2
+void orc_lb_geff(const orc_mesh* ms, const double* vin, const double* v0, const double* w, int64_t N, double S0,
+                 int use_gonzalez, int use_logsq, double* Geff) {
+  Scratch s(N, ms->n_dofs);
+  lb_geff(ms, Geff, vin, v0, w, N, S0, s, use_gonzalez, use_logsq);
+}
+
 // solver.jl:176-227
 void orc_picard_map(const orc_mesh* ms, const orc_solver_params* P, const double* vin, const double* v0,
                     const double* w, int64_t N, double S0, double* vout) {
@@ -951,6 +1068,14 @@ void orc_picard_map(const orc_mesh* ms, const orc_solver_params* P, const double
 void orc_predictor(const orc_mesh* ms, const orc_solver_params* P, const double* v, const double* w, int64_t N,
                    const double* fc, double* v1, double* dot_v_out) {
   std::vector<double> r(ms->n_dofs), L(ms->n_dofs), G(2 * N), dv(2 * N);
+  if (P->collision_model == 1) {  // main.jl:233-238: the direct clamped gradient at v
+    orc_loggrad(ms, v, N, fc, 1e-30, 100.0, G.data());
+    double AB[3];
+    lb_rhs(dv.data(), v, w, G.data(), N, P->nu, AB);
+    for (int64_t i = 0; i < 2 * N; ++i) v1[i] = v[i] + P->dt * dv[i];
+    if (dot_v_out) std::memcpy(dot_v_out, dv.data(), sizeof(double) * 2 * N);
+    return;
+  }
   orc_compute_r(ms, fc, P->use_logsq, r.data());
   orc_mass_solve(ms, r.data(), L.data());
   orc_compute_G(ms, v, N, L.data(), G.data());

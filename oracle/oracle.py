@@ -51,6 +51,8 @@ class SolverParams(C.Structure):
         ("dc_stag_window", C.c_int),
         ("dc_outer_m", C.c_int),
         ("dc_near_theta", C.c_double),
+        ("collision_model", C.c_int),
+        ("nu", C.c_double),
     ]
 
 
@@ -77,7 +79,7 @@ def default_params(dt: float, m: int = 5, nthreads: int = 0, **kw) -> SolverPara
         damp_decay_factor=0.5, restart_factor=float("inf"), damping=0.7,
         reg_factor=1e-10, use_anderson=1, use_gonzalez=1, exit_picard_step=1,
         use_logsq=0, nthreads=nthreads, solver=0, dc_eta=0.05, dc_max_inner=60, dc_stag_window=5,
-        dc_outer_m=0, dc_near_theta=0.0,
+        dc_outer_m=0, dc_near_theta=0.0, collision_model=0, nu=1.0,
     )
     for k, v in kw.items():
         setattr(p, k, v)
@@ -100,7 +102,7 @@ def lib():
         L.orc_landau_geff.restype = C.c_double
         for name in ("orc_mesh_info", "orc_mesh_tables", "orc_mass_dense", "orc_mass_solve",
                      "orc_eval_particle", "orc_l2_rhs", "orc_l2_project", "orc_entropy",
-                     "orc_negative_part_l1", "orc_fs_minus_fp_l2",
+                     "orc_negative_part_l1", "orc_fs_minus_fp_l2", "orc_lb_rhs", "orc_lb_geff",
                      "orc_compute_r", "orc_compute_G", "orc_loggrad", "orc_landau_geff",
                      "orc_picard_map", "orc_predictor", "orc_step_anderson", "orc_step_defect", "orc_time_step"):
             getattr(L, name).argtypes = None
@@ -260,6 +262,15 @@ class Mesh:
     def collision(self, v, w, G, nthreads: int = 0):
         return collision(self.box, v, w, G, nthreads)
 
+    def lb_geff(self, vin, v0, w, S0, use_gonzalez=True, use_logsq=False):
+        """G_eff of the LB Picard map (solver.jl:185-210)."""
+        vin, v0 = colmajor(vin), colmajor(v0)
+        w = np.ascontiguousarray(w, dtype=np.float64)
+        G = np.zeros((vin.shape[0], 2), order="F")
+        lib().orc_lb_geff(self.h, _p(vin), _p(v0), _p(w), C.c_int64(vin.shape[0]), C.c_double(S0),
+                          C.c_int(int(use_gonzalez)), C.c_int(int(use_logsq)), _p(G))
+        return G
+
     def landau_geff(self, vin, v0, w, S0, use_gonzalez=True, use_logsq=False):
         vin, v0 = colmajor(vin), colmajor(v0)
         w = np.ascontiguousarray(w, dtype=np.float64)
@@ -350,6 +361,17 @@ def collision_metric(box, v, w, G, nthreads: int = 0):
     A = np.zeros((N, 3), order="F")
     lib().orc_collision_metric(_p(box), _p(v), _p(w), _p(G), C.c_int64(N), _p(out), _p(A), C.c_int(nthreads))
     return out, A
+
+
+def lb_rhs(v, w, g, nu: float = 1.0):
+    """(dot_v, (A1, A2, B)): compute_moments, compute_drift_multipliers and
+    compute_LB_velocity! (functions.jl:66-115)."""
+    v, g = colmajor(v), colmajor(g)
+    w = np.ascontiguousarray(w, dtype=np.float64)
+    dv = np.zeros((v.shape[0], 2), order="F")
+    AB = np.zeros(3)
+    lib().orc_lb_rhs(_p(v), _p(w), _p(g), C.c_int64(v.shape[0]), C.c_double(nu), _p(dv), _p(AB))
+    return dv, AB
 
 
 def momentum_energy(v, w):
