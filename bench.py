@@ -40,6 +40,7 @@ whole Picard maps at the same N on this host.
 Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu]
        python bench.py --report precision --config bimaxwellian_n262144 --steps 20
        python bench.py --config ensemble48 --steps 3
+       python bench.py --config lb_bimodal_n40000 --steps 100
        python bench.py --kernel-probe        (pair kernels + a 256 MiB copy, for rocprofv3 counters)
 """
 from __future__ import annotations
@@ -387,6 +388,71 @@ def ensemble_bench(args):
     print(json.dumps(out), flush=True)
 
 
+def lb_bench(args):
+    """Lenard-Bernstein workload (parameters_LB_bimodal_v1.jl: N = 40000,
+    N(+-2, 1) x N(0, 1), non-uniform 27x24 cells, nu = 1, dt = 0.001, m = 8):
+    device-resident implicit steps/s, with the oracle's LB step (serial C++)
+    timed on a bounded sample of the same steps beside it."""
+    import collop
+    p = collop.config(args.config)
+    v, w = collop.initial_markers(p)
+    ws = collop.build_workspace(p, device=0)
+    N = v.shape[0]
+    prm = p.step_params()
+    sim = collop.Simulation(ws, v, w)
+    log(f"[lb] {args.config}: N={N}, mesh {len(p.bp1) - 1}x{len(p.bp2) - 1}, dt={p.DT}, nu={p.nu}, m={p.m_anderson}")
+    for i in range(args.warmup):
+        st = sim.run(prm, 1)[0]
+        log(f"  warmup step {i}: iters {st['iters']} {st['seconds'] * 1e3:.1f} ms")
+    vc = sim.markers()
+    E0 = 0.5 * float(np.sum(w * (vc[:, 0] ** 2 + vc[:, 1] ** 2)))
+    E_prev, P_prev = E0, (w[:, None] * vc).sum(axis=0)
+    S_prev = collop.compute_r_and_entropy_gpu(ws, sim.coeffs())[1]
+    t0 = time.perf_counter()
+    stats = sim.run(prm, args.steps)
+    el = time.perf_counter() - t0
+    dE = dP = 0.0
+    dS_min = np.inf
+    for i, st in enumerate(stats):
+        log(f"  step {i}: iters {st['iters']} exit {st['exit_reason']} res {st['residual']:.2e} "
+            f"{st['seconds'] * 1e3:.2f} ms")
+        dE = max(dE, abs(st["energy"] - E_prev) / E0)
+        dP = max(dP, float(np.max(np.abs(np.array(st["momentum"]) - P_prev))))
+        dS_min = min(dS_min, st["entropy"] - S_prev)
+        E_prev, P_prev, S_prev = st["energy"], np.array(st["momentum"]), st["entropy"]
+    maps = float(np.mean([s["iters"] for s in stats]))
+    sps = args.steps / el
+    log(f"[lb] {sps:.2f} steps/s, {maps:.1f} maps per step, {1e3 * el / args.steps / maps:.3f} ms per map")
+    cpu = None
+    if not args.no_cpu:
+        sys.path.insert(0, os.path.join(ROOT, "oracle"))
+        import oracle as O
+        m = O.Mesh(p.bp1, p.bp2)
+        fc = m.l2_project(v, w)
+        S = m.entropy(fc)
+        P = O.default_params(p.DT, m=p.m_anderson, nthreads=1, collision_model=1, nu=p.nu)
+        vv, t_all, done, it_cpu = v, time.perf_counter(), 0, 0
+        while done < 3 and time.perf_counter() - t_all < 20.0:
+            vv, fc, st = m.time_step(P, vv, w, fc, S)
+            S, done, it_cpu = st.entropy, done + 1, it_cpu + st.iters
+            log(f"  cpu step {done}: iters {st.iters} exit {st.exit_reason}")
+        t_cpu = (time.perf_counter() - t_all) / done
+        cpu = {"value": round(1.0 / t_cpu, 4), "unit": "steps/s", "cores": 1, "kind": "port",
+               "sample": f"the oracle's first {done} LB steps from the same markers (serial C++, {it_cpu / done:.1f} "
+                         f"maps per step), {t_cpu:.3f} s each"}
+    out = {"metric": "Lenard-Bernstein implicit steps/s (N = 40000, 27x24 non-uniform cells, nu 1, dt 0.001)",
+           "value": round(sps, 4), "unit": "steps/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+           "ms_per_step": round(1e3 * el / args.steps, 3), "higher_is_better": True, "scaling": "weak",
+           "vs_baseline": None, "dtype": "f64", "data": "synthetic (seeded PCG64 markers, seed 42)",
+           "config": {"workload": args.config, "N": N, "mesh": f"{len(p.bp1) - 1}x{len(p.bp2) - 1}", "dt": p.DT,
+                      "nu": p.nu, "m": p.m_anderson, "solver": "anderson"},
+           "mean_maps_per_step": maps, "ms_per_map": round(1e3 * el / args.steps / maps, 4),
+           "all_converged": all(s["exit_reason"] == 0 for s in stats),
+           "conservation": {"max_dE_rel": dE, "max_dP": dP, "min_dS": float(dS_min)},
+           "roofline": None, "cpu_baseline": cpu}
+    print(json.dumps(out), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -399,7 +465,8 @@ def main():
     ap.add_argument("--no-variants", action="store_true",
                     help="skip the FP32c + Anderson comparison path")
     ap.add_argument("--config", default="bimodal_n65536",
-                    help="bimodal_n65536 (headline), gaussian_n4096, bimaxwellian_n262144 or ensemble48")
+                    help="bimodal_n65536 (headline), gaussian_n4096, bimaxwellian_n262144, ensemble48 or "
+                         "lb_bimodal_n40000 (Lenard-Bernstein)")
     ap.add_argument("--kernel-probe", action="store_true",
                     help="run only the pair kernels and a 256 MiB copy (rocprofv3 counter runs)")
     ap.add_argument("--train-steps", type=int, default=200, help="warmstart report: recorded steps per run")
@@ -424,6 +491,9 @@ def main():
         return
     if args.config == "ensemble48":
         ensemble_bench(args)
+        return
+    if args.config.startswith("lb_"):
+        lb_bench(args)
         return
 
     import collop
