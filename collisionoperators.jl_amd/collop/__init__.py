@@ -10,7 +10,7 @@ from ._lib import (CollopError, Context, StepParams, StepStats, load, MODEL_LAND
                    PREC_F32C, WARM_EULER, WARM_MLP)
 from .hooks import (compute_collision_gpu, compute_collision_gpu32, compute_collision_metric_gpu,  # noqa: F401
                     compute_fs_minus_fp_l2_gpu, compute_G_gpu, compute_lb_rhs_gpu, compute_negative_part_l1_gpu,
-                    compute_r_and_entropy_gpu, enable_gpu, eval_loggrad_gpu, l2_project_gpu, l2_rhs_gpu,
+                    compute_r_and_entropy_gpu, enable_gpu, eval_loggrad_gpu, gmres_dense_gpu, l2_project_gpu, l2_rhs_gpu,
                     mass_solve_gpu)
 from .ic import ensemble_markers, initial_markers, markers  # noqa: F401
 from .mesh import Workspace, build_workspace, extraction_p2c1  # noqa: F401

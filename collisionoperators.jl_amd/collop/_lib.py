@@ -29,6 +29,7 @@ WARM_EULER = 0
 WARM_MLP = 1
 SOLVER_ANDERSON = 0
 SOLVER_DEFECT = 1
+SOLVER_NEWTON = 2
 MODEL_LANDAU = 0
 MODEL_LB = 1
 PAIR_ORDERED = 0
@@ -78,6 +79,10 @@ class StepParams(C.Structure):
         ("dc_metric_reuse", C.c_int32),
         ("collision_model", C.c_int32),
         ("nu", C.c_double),
+        ("nk_krylov_max", C.c_int32),
+        ("nk_eta_max", C.c_double),
+        ("nk_fd_h", C.c_double),
+        ("nk_fd_rel", C.c_double),
     ]
 
 
@@ -133,6 +138,8 @@ SIGNATURES = [
     ("collop_quadrature", C.c_int, [_vp, _dp, C.c_int, _dp, _dp]),
     ("collop_mass_solve", C.c_int, [_vp, _dp, _dp]),
     ("collop_lb_rhs", C.c_int, [_vp, _dp, _dp, _dp, C.c_int64, C.c_double, _dp, _dp]),
+    ("collop_gmres_dense", C.c_int, [_vp, C.c_int, _dp, _dp, C.c_double, C.c_double, C.c_int, C.c_int, _dp,
+                                     C.POINTER(C.c_int), _dp]),
     ("collop_negative_part_l1", C.c_int, [_vp, _dp, _dp]),
     ("collop_fs_minus_fp_l2", C.c_int, [_vp, _dp, _dp, C.c_int64, _dp, _dp]),
     ("collop_state_set", C.c_int, [_vp, _dp, _dp, C.c_int64]),

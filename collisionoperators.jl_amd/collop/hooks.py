@@ -160,6 +160,23 @@ def compute_lb_rhs_gpu(ws: Workspace, dot_v, v_parts, w_parts, g, nu: float = 1.
     return float(mult[0]), float(mult[1]), float(mult[2])
 
 
+def gmres_dense_gpu(ctx, A, b, rtol: float, atol: float = 0.0, itmax: int | None = None, mem: int | None = None):
+    """gmres! (newton_krylov.jl:60-131) of the device Newton-Krylov engine on a
+    dense operator: (x, products, GMRES residual norm)."""
+    import ctypes as C
+    A = np.ascontiguousarray(A, dtype=np.float64)
+    b = _vec(b)
+    n = b.shape[0]
+    mem = min(n, 64) if mem is None else mem
+    itmax = mem if itmax is None else itmax
+    x = np.zeros(n)
+    k = C.c_int(0)
+    r = np.zeros(1)
+    ctx.check(ctx.lib.collop_gmres_dense(ctx.h, n, _lib.dptr(A), _lib.dptr(b), float(rtol), float(atol), int(itmax),
+                                         int(mem), _lib.dptr(x), C.byref(k), _lib.dptr(r)))
+    return x, int(k.value), float(r[0])
+
+
 def compute_negative_part_l1_gpu(ws: Workspace, f_coeffs) -> float:
     """compute_negative_part_l1 (functions.jl:300-313) of f_s = sum_k f_coeffs[k] phi_k."""
     ws.activate()

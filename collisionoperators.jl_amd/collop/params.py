@@ -52,7 +52,7 @@ class SimParameters:
     seed: int = 42
     ic: str = "gaussian"             # gaussian | bimodal | bimaxwellian
     weight_jitter: float = 0.0       # w ∝ 1 + jitter u, u ~ U(-1, 1)
-    solver: str = "anderson"         # :anderson | :defect  (Parameters.jl:80-88)
+    solver: str = "anderson"         # :anderson | :defect | :newton  (Parameters.jl:78-88)
     dc_eta: float = 0.05             # Parameters.jl:89-91
     dc_max_inner: int = 60
     dc_stag_window: int = 5
@@ -61,6 +61,10 @@ class SimParameters:
     dc_metric_reuse: int = 0         # extension: 0 A at every outer map, 1 FP64 maps reuse the FP32c A, 2 first only
     collision_model: str = "landau"  # :landau | :lb  (Parameters.jl:50)
     nu: float = 1.0                  # LB collision frequency (Parameters.jl:53)
+    nk_krylov_max: int = 30          # Parameters.jl:92-100 (solver = :newton)
+    nk_eta_max: float = 0.9
+    nk_fd_h: float = 0.0             # 0 = 1e-6 (FP64 maps)
+    nk_fd_rel: float = 0.0
 
     def step_params(self, precision: int | None = None, warmstart: int | None = None,
                     solver: int | None = None) -> _lib.StepParams:
@@ -68,9 +72,10 @@ class SimParameters:
         prec = precision if precision is not None else (_lib.PREC_F32C if self.gpu_fp32 else _lib.PREC_F64)
         warm = warmstart if warmstart is not None else (_lib.WARM_MLP if self.warmstart == "nn" else _lib.WARM_EULER)
         if solver is None:
-            if self.solver not in ("anderson", "defect"):
-                raise ValueError(f"unknown solver={self.solver} (anderson | defect)")
-            solver = _lib.SOLVER_DEFECT if self.solver == "defect" else _lib.SOLVER_ANDERSON
+            solvers = {"anderson": _lib.SOLVER_ANDERSON, "defect": _lib.SOLVER_DEFECT, "newton": _lib.SOLVER_NEWTON}
+            if self.solver not in solvers:
+                raise ValueError(f"unknown solver={self.solver} (anderson | defect | newton)")
+            solver = solvers[self.solver]
         if self.collision_model not in ("landau", "lb"):
             raise ValueError(f"unknown collision_model={self.collision_model} (landau | lb)")
         return _lib.StepParams(
@@ -83,7 +88,9 @@ class SimParameters:
             precision=prec, warmstart=warm, f32_floor_factor=16.0, f32_keep_window=1, record_training=0,
             solver=solver, dc_eta=self.dc_eta, dc_max_inner=self.dc_max_inner, dc_stag_window=self.dc_stag_window,
             dc_outer_m=self.dc_outer_m, dc_near_theta=self.dc_near_theta, dc_metric_reuse=self.dc_metric_reuse,
-            collision_model=_lib.MODEL_LB if self.collision_model == "lb" else _lib.MODEL_LANDAU, nu=self.nu)
+            collision_model=_lib.MODEL_LB if self.collision_model == "lb" else _lib.MODEL_LANDAU, nu=self.nu,
+            nk_krylov_max=self.nk_krylov_max, nk_eta_max=self.nk_eta_max, nk_fd_h=self.nk_fd_h,
+            nk_fd_rel=self.nk_fd_rel)
 
 
 def parse_overrides(p: SimParameters, args: list[str]) -> SimParameters:

@@ -22,10 +22,13 @@
 #include "../../include/collop.h"
 #include "anderson.hpp"
 #include "collop_ctx.hpp"
+#include "nk.hpp"
 
 using namespace collop;
 
 namespace {
+
+int nk_setup(collop_ctx* ctx, int n, int mem, NkDevice& E, double** extra, int nextra);
 
 // Gauss-Legendre rule on [0,1]: Newton's method on the three-term recurrence
 // of P_n (any textbook; e.g. Press et al., Numerical Recipes, 4.6).
@@ -326,7 +329,8 @@ int collop_destroy(collop_ctx* ctx) {
   free_mesh(ctx);
   for (DBuf* b : {&ctx->b_v, &ctx->b_w, &ctx->b_G, &ctx->b_out, &ctx->b_vmid, &ctx->b_dv, &ctx->b_dS, &ctx->b_s64,
                   &ctx->b_s32, &ctx->b_part, &ctx->b_spart_sc, &ctx->b_mom, &ctx->b_dpart, &ctx->b_iota,
-                  &ctx->b_spart_q, &ctx->b_vec, &ctx->b_scal, &ctx->b_statp, &ctx->b_stat, &ctx->b_stmp, &ctx->b_lbpart, &ctx->s_v, &ctx->s_w,
+                  &ctx->b_spart_q, &ctx->b_vec, &ctx->b_scal, &ctx->b_statp, &ctx->b_stat, &ctx->b_stmp, &ctx->b_lbpart, &ctx->nk_V, &ctx->nk_vec,
+                  &ctx->nk_part, &ctx->nk_h, &ctx->nk_A, &ctx->s_v, &ctx->s_w,
                   &ctx->s_v1, &ctx->s_Gv, &ctx->s_r, &ctx->s_rp, &ctx->s_Gp, &ctx->s_best, &ctx->s_dF, &ctx->s_dG,
                   &ctx->s_fc, &ctx->s_dotv, &ctx->s_Gvo, &ctx->s_Gk, &ctx->s_FA, &ctx->s_D, &ctx->s_Ge, &ctx->s_vmk, &ctx->s_xk, &ctx->s_oU,
                   &ctx->s_oUp, &ctx->s_of, &ctx->s_ofp, &ctx->s_odF, &ctx->s_odG, &ctx->b_near_list, &ctx->b_near_count,
@@ -725,6 +729,36 @@ int collop_lb_rhs(collop_ctx* ctx, const double* v, const double* w, const doubl
   return COLLOP_OK;
 }
 
+int collop_gmres_dense(collop_ctx* ctx, int n, const double* A, const double* b, double rtol, double atol, int itmax,
+                       int mem, double* x, int* products, double* rnorm) {
+  if (!ctx) return COLLOP_ERR_INVALID;
+  if (n < 1 || !A || !b || !x) return fail(ctx, COLLOP_ERR_INVALID, "n >= 1 and non-null arrays required");
+  if (mem < 1 || mem > kNkMaxKrylov || itmax < 0 || !(rtol >= 0.0) || !(atol >= 0.0))
+    return fail(ctx, COLLOP_ERR_INVALID, "need 1 <= mem <= 64, itmax >= 0, rtol, atol >= 0");
+  HIPC(hipSetDevice(ctx->device));
+  NkDevice E;
+  double* ex[2];
+  int r = nk_setup(ctx, n, mem, E, ex, 2);
+  if (r) return r;
+  ENSURE(ctx->nk_A, (size_t)n * n * 8);
+  HIPC(hipMemcpyAsync(ctx->nk_A.p, A, (size_t)n * n * 8, hipMemcpyHostToDevice, ctx->stream));
+  HIPC(hipMemcpyAsync(ex[1], b, (size_t)n * 8, hipMemcpyHostToDevice, ctx->stream));
+  const double* Ad = P<double>(ctx->nk_A);
+  NkDevice::Op op = [&](double* y, const double* u) -> int {
+    launch_dense_mv(y, Ad, u, n, ctx->stream);
+    return hipGetLastError() == hipSuccess ? 0 : -1;
+  };
+  int k = 0;
+  double rn = 0.0;
+  if (E.gmres(ex[0], op, ex[1], mem, rtol, atol, itmax, &k, &rn))
+    return fail(ctx, COLLOP_ERR_HIP, std::string("GMRES: ") + hipGetErrorString(hipGetLastError()));
+  HIPC(hipMemcpyAsync(x, ex[0], (size_t)n * 8, hipMemcpyDeviceToHost, ctx->stream));
+  HIPC(hipStreamSynchronize(ctx->stream));
+  if (products) *products = k;
+  if (rnorm) *rnorm = rn;
+  return COLLOP_OK;
+}
+
 int collop_mass_solve(collop_ctx* ctx, const double* b, double* x) {
   int r = need_mesh(ctx);
   if (r) return r;
@@ -1014,6 +1048,80 @@ int lb_predictor_dev(collop_ctx* ctx, const collop_step_params& p) {
   return COLLOP_OK;
 }
 
+// The device Newton-Krylov engine over vectors of length n with a basis of mem
+// vectors; nvec extra work vectors follow the engine's own in nk_vec.
+int nk_setup(collop_ctx* ctx, int n, int mem, NkDevice& E, double** extra, int nextra) {
+  const size_t nb = (size_t)std::max(n, 1) * 8;
+  ENSURE(ctx->nk_V, nb * (size_t)(mem + 1));
+  ENSURE(ctx->nk_vec, nb * (size_t)(8 + nextra));
+  ENSURE(ctx->nk_part, nk_part_size() * 8);
+  ENSURE(ctx->nk_h, (size_t)(kNkMaxKrylov + 1) * 8);
+  double* v = P<double>(ctx->nk_vec);
+  const size_t ld = (size_t)std::max(n, 1);
+  E.n = n;
+  E.st = ctx->stream;
+  E.V = P<double>(ctx->nk_V);
+  E.ldv = ld;
+  E.part = P<double>(ctx->nk_part);
+  E.hdev = P<double>(ctx->nk_h);
+  E.w = v;
+  E.Ft = v + ld;
+  E.xt = v + 2 * ld;
+  E.xp = v + 3 * ld;
+  E.Fp = v + 4 * ld;
+  E.d = v + 5 * ld;
+  E.xb = v + 6 * ld;
+  E.Fb = v + 7 * ld;
+  for (int i = 0; i < nextra; ++i) extra[i] = v + (8 + i) * ld;
+  return COLLOP_OK;
+}
+
+// step_newton! (solver.jl:437-486): inexact Newton on F(v) = v - G(v) from the
+// initial guess in s_v1, every residual one device Picard map (Landau or LB);
+// with exit_picard_step, v1 <- v - F(v) = G(v).
+int newton_solve(collop_ctx* ctx, const collop_step_params& p, collop_step_stats* out) {
+  const int n2 = 2 * (int)ctx->N;
+  const bool lb = p.collision_model == COLLOP_MODEL_LB;
+  NkDevice E;
+  double* ex[2];
+  int r = nk_setup(ctx, n2, p.nk_krylov_max, E, ex, 2);
+  if (r) return r;
+  double* x = ex[0];
+  double* F = ex[1];
+  HIPC(hipMemcpyAsync(x, ctx->s_v1.p, (size_t)n2 * 8, hipMemcpyDeviceToDevice, ctx->stream));
+  double nv;
+  if (E.norm(x, &nv)) return fail(ctx, COLLOP_ERR_HIP, "Newton-Krylov: device norm failed");
+  const double eff_tol = std::max(p.tol * (nv + 1e-30), p.abs_floor);
+  const double fd_h = p.nk_fd_rel > 0 ? p.nk_fd_rel * std::max(nv, 1.0) : (p.nk_fd_h > 0 ? p.nk_fd_h : 1e-6);
+  int maps = 0;
+  NkDevice::Resid Fn = [&](const double* xv, double* Fv, double* nrm) -> int {
+    const int rr = lb ? lb_map_dev(ctx, p, xv, -1, 0) : picard_map_dev(ctx, p, 0, xv, -1, 0);
+    if (rr) return rr;
+    ++maps;
+    launch_neg(Fv, P<double>(ctx->s_r), n2, ctx->stream);  // F = v - G(v) = -(G(v) - v)
+    *nrm = std::sqrt(ctx->h_stat[0]);
+    return std::isfinite(*nrm) ? 0 : -2;
+  };
+  NkResult res{};
+  const int st = E.newton(x, Fn, F, p.nk_krylov_max, eff_tol, p.max_iter, fd_h, p.nk_eta_max, &res);
+  if (st == -2) return fail(ctx, COLLOP_ERR_NONFINITE, "non-finite residual in a Newton-Krylov map " +
+                                                             std::to_string(maps));
+  if (st) return fail(ctx, COLLOP_ERR_HIP, std::string("Newton-Krylov: ") + hipGetErrorString(hipGetLastError()));
+  if (p.exit_picard_step)
+    launch_axpby(P<double>(ctx->s_v1), x, -1.0, F, n2, ctx->stream);
+  else
+    HIPC(hipMemcpyAsync(ctx->s_v1.p, x, (size_t)n2 * 8, hipMemcpyDeviceToDevice, ctx->stream));
+  CHECK_LAUNCH();
+  out->iters = res.n_evals;
+  out->iters_f32 = 0;
+  out->n_restart = res.n_newton;
+  out->exit_reason = res.status;
+  out->residual = res.rnorm;
+  out->r0 = res.rnorm0;
+  out->inner_maps = 0;
+  return COLLOP_OK;
+}
+
 }  // namespace
 
 namespace collop {
@@ -1100,8 +1208,15 @@ int validate_params(collop_ctx* ctx, const collop_step_params* p) {
   if (p->warmstart == COLLOP_WARM_MLP && !(ctx->m_has_params && ctx->m_has_norm))
     return fail(ctx, COLLOP_ERR_NO_MODEL, "no warm-start model loaded on this context (collop_mlp_set_params / "
                                           "collop_mlp_set_norm)");
-  if (p->solver != COLLOP_SOLVER_ANDERSON && p->solver != COLLOP_SOLVER_DEFECT)
+  if (p->solver != COLLOP_SOLVER_ANDERSON && p->solver != COLLOP_SOLVER_DEFECT && p->solver != COLLOP_SOLVER_NEWTON)
     return fail(ctx, COLLOP_ERR_INVALID, "unknown solver");
+  if (p->solver == COLLOP_SOLVER_NEWTON &&
+      (p->precision != COLLOP_PREC_F64 || p->nk_krylov_max < 1 || p->nk_krylov_max > kNkMaxKrylov ||
+       !(p->nk_eta_max > 0.0 && p->nk_eta_max < 1.0) || !(p->nk_fd_h >= 0.0) || !(p->nk_fd_rel >= 0.0) ||
+       !std::isfinite(p->nk_fd_h) || !std::isfinite(p->nk_fd_rel)))
+    return fail(ctx, COLLOP_ERR_INVALID,
+                "Newton-Krylov runs FP64 maps and needs 1 <= nk_krylov_max <= 64, 0 < nk_eta_max < 1 and finite "
+                "nk_fd_h, nk_fd_rel >= 0");
   if (p->solver == COLLOP_SOLVER_DEFECT &&
       (!(p->dc_eta > 0.0 && p->dc_eta < 1.0) || p->dc_max_inner < 1 || p->dc_stag_window < 1))
     return fail(ctx, COLLOP_ERR_INVALID, "defect correction needs 0 < dc_eta < 1, dc_max_inner >= 1, dc_stag_window >= 1");
@@ -1113,11 +1228,11 @@ int validate_params(collop_ctx* ctx, const collop_step_params* p) {
   if (p->collision_model != COLLOP_MODEL_LANDAU && p->collision_model != COLLOP_MODEL_LB)
     return fail(ctx, COLLOP_ERR_INVALID, "unknown collision_model");
   if (p->collision_model == COLLOP_MODEL_LB &&
-      (p->solver != COLLOP_SOLVER_ANDERSON || p->precision != COLLOP_PREC_F64 || p->warmstart != COLLOP_WARM_EULER ||
+      (p->solver == COLLOP_SOLVER_DEFECT || p->precision != COLLOP_PREC_F64 || p->warmstart != COLLOP_WARM_EULER ||
        p->record_training || !std::isfinite(p->nu)))
     return fail(ctx, COLLOP_ERR_INVALID,
-                "the Lenard-Bernstein model runs the Anderson solver with FP64 maps and the Euler predictor "
-                "(no pair sum, no metric, no learned warm start) and needs a finite nu");
+                "the Lenard-Bernstein model runs the Anderson or Newton-Krylov solver with FP64 maps and the Euler "
+                "predictor (no pair sum, no metric, no learned warm start) and needs a finite nu");
   if (p->record_training && ctx->m_size + ctx->N > ctx->m_cap)
     return fail(ctx, COLLOP_ERR_INVALID, "training dataset full (collop_mlp_dataset_alloc)");
   return COLLOP_OK;
@@ -1382,6 +1497,8 @@ int step_once(collop_ctx* ctx, const collop_step_params& p, collop_step_stats* o
   collop_step_stats dc{};
   if (p.solver == COLLOP_SOLVER_DEFECT) {
     if ((r = defect_solve(ctx, p, &dc))) return r;
+  } else if (p.solver == COLLOP_SOLVER_NEWTON) {
+    if ((r = newton_solve(ctx, p, &dc))) return r;
   }
   // ---- Anderson (solver.jl:300-423)
   AndersonRun an;
@@ -1431,7 +1548,7 @@ int step_once(collop_ctx* ctx, const collop_step_params& p, collop_step_stats* o
   HIPC(hipStreamSynchronize(ctx->stream));
   prof_collect(ctx);
   if (out) {
-    if (p.solver == COLLOP_SOLVER_DEFECT) {
+    if (p.solver != COLLOP_SOLVER_ANDERSON) {
       *out = dc;
     } else {
       out->iters = an.k_done;

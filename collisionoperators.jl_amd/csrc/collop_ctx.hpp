@@ -37,6 +37,7 @@ struct collop_ctx {
   // ---- scratch buffers by name (grow-only)
   DBuf b_v, b_w, b_G, b_out, b_vmid, b_dv, b_dS, b_s64, b_s32, b_part, b_spart_sc, b_mom, b_dpart, b_iota, b_spart_q,
       b_vec, b_scal, b_statp, b_stat, b_stmp, b_lbpart;
+  DBuf nk_V, nk_vec, nk_part, nk_h, nk_A;  // Newton-Krylov: basis, work vectors, dot partials, scalars, dense A
   // ---- device-resident state
   bool has_state = false;
   int64_t N = 0;

@@ -114,6 +114,14 @@ int collop_quadrature(collop_ctx* ctx, const double* f_coeffs, int use_logsq, do
 /* x = M^{-1} b (the reference's ws.M_lu \ b, functions.jl:132, solver.jl:67). */
 int collop_mass_solve(collop_ctx* ctx, const double* b, double* x);
 
+/* gmres! (newton_krylov.jl:60-131) of the device Newton-Krylov engine on a
+ * dense operator: x = the GMRES solution of A x = b (A row-major n x n) from
+ * x0 = 0 with basis size mem (<= 64), stopping at ||b - A x|| <= atol + rtol
+ * ||b||, on breakdown or after itmax products.  products and rnorm (the GMRES
+ * residual norm) may be NULL.  For small systems and for testing the engine. */
+int collop_gmres_dense(collop_ctx* ctx, int n, const double* A, const double* b, double rtol, double atol, int itmax,
+                       int mem, double* x, int* products, double* rnorm);
+
 /* Lenard-Bernstein right-hand side at markers v with log-density gradient g
  * (N x 2, e.g. from collop_loggrad_gather): the moments n, U, Q of (v, w)
  * (compute_moments, functions.jl:66-82), the drift multipliers (A1, A2, B) of
@@ -153,10 +161,14 @@ typedef enum {
 
 typedef enum {
   COLLOP_SOLVER_ANDERSON = 0, /* step_anderson! (solver.jl:300-423)              */
-  COLLOP_SOLVER_DEFECT = 1    /* frozen-metric defect correction, step_defect!
+  COLLOP_SOLVER_DEFECT = 1,   /* frozen-metric defect correction, step_defect!
                                  (solver.jl:569-634): each outer iteration is one
                                  full map with the metric A; an inner Anderson
                                  solve on the frozen-metric map needs no pair sum */
+  COLLOP_SOLVER_NEWTON = 2    /* Jacobian-free Newton-Krylov, step_newton!
+                                 (solver.jl:437-486, newton_krylov.jl): GMRES on
+                                 forward-difference products, one Picard map each;
+                                 FP64 maps                                       */
 } collop_solver;
 
 typedef enum {
@@ -213,6 +225,12 @@ typedef struct {
                                2 = the step's first only                        */
   int32_t collision_model;  /* collop_model (Parameters.jl:50)                  */
   double nu;                /* LB collision frequency (Parameters.jl:53, 1.0)   */
+  int32_t nk_krylov_max;    /* NEWTON: GMRES basis per Newton step (<= 64; 30)  */
+  double nk_eta_max;        /* NEWTON: Eisenstat-Walker forcing cap (0.9)       */
+  double nk_fd_h;           /* NEWTON: absolute finite-difference step of J u
+                               (Parameters.jl:98; 0 = 1e-6)                     */
+  double nk_fd_rel;         /* NEWTON: legacy relative step nk_fd_rel max(|v|, 1);
+                               > 0 overrides nk_fd_h                            */
 } collop_step_params;
 
 typedef struct {

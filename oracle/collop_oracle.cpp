@@ -35,6 +35,7 @@
 //                 is projection_gpu.jl:66)
 
 #include <algorithm>
+#include <functional>
 #include <cmath>
 #include <cstdint>
 #include <cstdlib>
@@ -295,6 +296,10 @@ struct orc_solver_params {
   double dc_near_theta;  // extension: pairs with w_a / |d| > theta are exact in the frozen map (0 = none)
   int collision_model;   // 0 :landau, 1 :lb (Parameters.jl:50)
   double nu;             // LB collision frequency (Parameters.jl:53)
+  int nk_krylov_max;     // solver 2 (step_newton!): GMRES basis per Newton step (Parameters.jl:92)
+  double nk_eta_max;     // Eisenstat-Walker cap (Parameters.jl:93)
+  double nk_fd_h;        // absolute finite-difference step (Parameters.jl:98; 0 = 1e-6)
+  double nk_fd_rel;      // legacy relative step (> 0 overrides nk_fd_h)
 };
 
 struct orc_step_stats {
@@ -1390,6 +1395,259 @@ void orc_step_defect(const orc_mesh* ms, const orc_solver_params* Pp, double* v1
 
 // One full time step of main.jl:226-307 (Landau, Anderson): predictor,
 // step_anderson!, projection, entropy / energy / momentum of the new state.
+}  // extern "C"
+
+// ---------------------------------------------------------------------------
+// Matrix-free GMRES and Jacobian-free Newton-Krylov (newton_krylov.jl), generic
+// over a residual closure on plain vectors.
+namespace {
+
+using Vec = std::vector<double>;
+using Resid = std::function<void(Vec& F, const Vec& x)>;
+
+double vdot(const Vec& a, const Vec& b) {
+  double s = 0.0;
+  for (size_t i = 0; i < a.size(); ++i) s += a[i] * b[i];
+  return s;
+}
+
+// newton_krylov.jl:37-58 (_sym_givens): [c s; s -c] [a; b] = [rho; 0]
+void sym_givens(double a, double b, double& c, double& s, double& rho) {
+  auto sgn = [](double x) { return x > 0.0 ? 1.0 : (x < 0.0 ? -1.0 : 0.0); };
+  if (b == 0.0) {
+    c = sgn(a) + (a == 0.0 ? 1.0 : 0.0);
+    s = 0.0;
+    rho = std::fabs(a);
+  } else if (a == 0.0) {
+    c = 0.0;
+    s = sgn(b);
+    rho = std::fabs(b);
+  } else if (std::fabs(b) > std::fabs(a)) {
+    const double t = a / b;
+    s = sgn(b) / std::sqrt(1.0 + t * t);
+    c = s * t;
+    rho = b / s;
+  } else {
+    const double t = b / a;
+    c = sgn(a) / std::sqrt(1.0 + t * t);
+    s = c * t;
+    rho = a / c;
+  }
+}
+
+// newton_krylov.jl:60-131 (gmres!): from x0 = 0, unrestarted, modified
+// Gram-Schmidt, Givens QR of the Hessenberg matrix; returns the product count.
+int gmres(Vec& x, const std::function<void(Vec& y, const Vec& u)>& A, const Vec& b, int mem, double rtol,
+          double atol, int itmax, double* rnorm_out) {
+  const size_t n = b.size();
+  mem = std::min(itmax, mem);
+  std::fill(x.begin(), x.end(), 0.0);
+  const double beta = std::sqrt(vdot(b, b));
+  if (beta == 0.0) {
+    *rnorm_out = 0.0;
+    return 0;
+  }
+  const double eps_tol = atol + rtol * beta;
+  const double btol = std::pow(2.220446049250313e-16, 0.75);
+  std::vector<Vec> V(mem + 1, Vec(n));
+  std::vector<double> R((size_t)mem * mem, 0.0), c(mem), sn(mem), z(mem + 1, 0.0);
+  Vec w(n);
+  z[0] = beta;
+  for (size_t i = 0; i < n; ++i) V[0][i] = b[i] / beta;
+  int k = 0;
+  double rnorm = beta;
+  auto Rm = [&](int i, int j) -> double& { return R[(size_t)i * mem + j]; };
+  while (k < mem) {
+    ++k;
+    A(w, V[k - 1]);
+    for (int i = 0; i < k; ++i) {
+      const double h = vdot(V[i], w);
+      Rm(i, k - 1) = h;
+      for (size_t j = 0; j < n; ++j) w[j] -= h * V[i][j];
+    }
+    const double hnext = std::sqrt(vdot(w, w));
+    for (int i = 0; i < k - 1; ++i) {
+      const double tmp = c[i] * Rm(i, k - 1) + sn[i] * Rm(i + 1, k - 1);
+      Rm(i + 1, k - 1) = sn[i] * Rm(i, k - 1) - c[i] * Rm(i + 1, k - 1);
+      Rm(i, k - 1) = tmp;
+    }
+    sym_givens(Rm(k - 1, k - 1), hnext, c[k - 1], sn[k - 1], Rm(k - 1, k - 1));
+    z[k] = sn[k - 1] * z[k - 1];
+    z[k - 1] = c[k - 1] * z[k - 1];
+    rnorm = std::fabs(z[k]);
+    if (rnorm <= eps_tol || hnext <= btol) break;
+    if (k < mem)
+      for (size_t j = 0; j < n; ++j) V[k][j] = w[j] / hnext;
+  }
+  for (int i = k - 1; i >= 0; --i) {
+    double acc = z[i];
+    for (int j = i + 1; j < k; ++j) acc -= Rm(i, j) * z[j];
+    z[i] = std::fabs(Rm(i, i)) <= btol ? 0.0 : acc / Rm(i, i);
+  }
+  for (int i = 0; i < k; ++i)
+    for (size_t j = 0; j < n; ++j) x[j] += z[i] * V[i][j];
+  *rnorm_out = rnorm;
+  return k;
+}
+
+// newton_krylov.jl:189-303 (newton_krylov!): status 0 converged, 1 stalled,
+// 2 budget; F holds F(x) on return.
+int newton_krylov(Vec& x, const Resid& Fn, Vec& F, int krylov_max, double tol, int max_evals, double fd_h,
+                  double eta_max, int* n_evals_out, double* rnorm_out, int* n_newton_out, double* rnorm0_out) {
+  const double eta_gamma = 0.9, armijo = 1e-4;
+  const int max_backtrack = 8;
+  const size_t n = x.size();
+  Vec Ft(n), xt(n), xp(n), Fp(n), d(n), xb(x), Fb(n);
+  Fn(F, x);
+  int n_evals = 1;
+  double rnorm = std::sqrt(vdot(F, F));
+  const double rnorm0 = rnorm;
+  double rnorm_best = rnorm;
+  Fb = F;
+  auto jvp = [&](Vec& y, const Vec& u) {
+    const double h = fd_h / std::sqrt(vdot(u, u));
+    for (size_t i = 0; i < n; ++i) xp[i] = x[i] + h * u[i];
+    Fn(Fp, xp);
+    ++n_evals;
+    for (size_t i = 0; i < n; ++i) y[i] = (Fp[i] - F[i]) / h;
+  };
+  double eta = eta_max, rnorm_prev = rnorm;
+  int n_newton = 0, status = 2;
+  while (true) {
+    if (rnorm < tol) {
+      status = 0;
+      break;
+    }
+    const int budget = max_evals - n_evals;
+    if (budget < 2) break;
+    if (n_newton > 0) {
+      double ew = eta_gamma * (rnorm / rnorm_prev) * (rnorm / rnorm_prev);
+      const double safe = eta_gamma * eta * eta;
+      if (safe > 0.1) ew = std::max(ew, safe);
+      eta = std::min(std::max(ew, 0.5 * tol / rnorm), eta_max);
+    }
+    double lin_r;
+    gmres(d, jvp, F, krylov_max, eta, 0.0, std::min(krylov_max, budget - 1), &lin_r);
+    ++n_newton;
+    double lam = 1.0, rnorm_t = std::numeric_limits<double>::infinity();
+    bool accepted = false;
+    for (int bt = 0; bt <= max_backtrack; ++bt) {
+      for (size_t i = 0; i < n; ++i) xt[i] = x[i] - lam * d[i];
+      Fn(Ft, xt);
+      ++n_evals;
+      rnorm_t = std::sqrt(vdot(Ft, Ft));
+      if (rnorm_t <= (1 - armijo * lam * (1 - eta)) * rnorm) {
+        accepted = true;
+        break;
+      }
+      if (n_evals >= max_evals) break;
+      lam /= 2;
+    }
+    if (rnorm_t < rnorm_best) {
+      rnorm_best = rnorm_t;
+      xb = xt;
+      Fb = Ft;
+    }
+    if (!accepted) {
+      status = n_evals >= max_evals ? 2 : 1;
+      break;
+    }
+    x = xt;
+    F = Ft;
+    rnorm_prev = rnorm;
+    rnorm = rnorm_t;
+  }
+  if (status != 0 && rnorm_best < rnorm) {
+    x = xb;
+    F = Fb;
+    rnorm = rnorm_best;
+  }
+  *n_evals_out = n_evals;
+  *rnorm_out = rnorm;
+  *n_newton_out = n_newton;
+  *rnorm0_out = rnorm0;
+  return status;
+}
+
+}  // namespace
+
+extern "C" {
+
+// gmres! on the dense operator y = A u (A row-major n x n); returns the product count.
+int orc_gmres_dense(int n, const double* A, const double* b, double rtol, double atol, int itmax, int mem, double* x,
+                    double* rnorm) {
+  Vec bv(b, b + n), xv(n);
+  auto op = [&](Vec& y, const Vec& u) {
+    for (int i = 0; i < n; ++i) {
+      double s = 0.0;
+      for (int j = 0; j < n; ++j) s += A[(size_t)i * n + j] * u[j];
+      y[i] = s;
+    }
+  };
+  const int k = gmres(xv, op, bv, mem, rtol, atol, itmax, rnorm);
+  std::memcpy(x, xv.data(), sizeof(double) * n);
+  return k;
+}
+
+// newton_krylov! on F(x) = x - x0 - dt f((x0 + x) / 2), f(y) = tanh(B y) - y^3 / 2
+// (the implicit-midpoint-like problem of the reference's own solver tests).
+// out[0..3] = n_evals, rnorm, n_newton, rnorm0; returns the status.
+int orc_newton_midpoint(int n, const double* B, const double* x0, double dt, double tol, int max_evals, double fd_h,
+                        int krylov_max, double* x, double* F_out, double* out) {
+  auto Fn = [&](Vec& F, const Vec& xv) {
+    Vec y(n);
+    for (int i = 0; i < n; ++i) y[i] = 0.5 * (x0[i] + xv[i]);
+    for (int i = 0; i < n; ++i) {
+      double s = 0.0;
+      for (int j = 0; j < n; ++j) s += B[(size_t)i * n + j] * y[j];
+      F[i] = xv[i] - x0[i] - dt * (std::tanh(s) - 0.5 * y[i] * y[i] * y[i]);
+    }
+  };
+  Vec xv(x, x + n), F(n);
+  int ne, nn;
+  double rn, r0;
+  const int st = newton_krylov(xv, Fn, F, krylov_max, tol, max_evals, fd_h, 0.9, &ne, &rn, &nn, &r0);
+  std::memcpy(x, xv.data(), sizeof(double) * n);
+  std::memcpy(F_out, F.data(), sizeof(double) * n);
+  out[0] = ne;
+  out[1] = rn;
+  out[2] = nn;
+  out[3] = r0;
+  return st;
+}
+
+// solver.jl:437-486 (step_newton!): F(v) = v - G(v) with G the Picard map;
+// v1 holds the initial guess on entry and the solution (G(v), with
+// exit_picard_step) on exit.
+void orc_step_newton(const orc_mesh* ms, const orc_solver_params* Pp, double* v1, const double* v0, const double* w,
+                     int64_t N, double S0, orc_step_stats* st) {
+  const orc_solver_params& P = *Pp;
+  const int64_t n2 = 2 * N;
+  Scratch s(N, ms->n_dofs);
+  Vec Gv(n2);
+  auto Fn = [&](Vec& F, const Vec& x) {
+    picard_map(ms, Gv.data(), x.data(), v0, w, N, S0, P.dt, s, P);
+    for (int64_t i = 0; i < n2; ++i) F[i] = x[i] - Gv[i];
+  };
+  Vec x(v1, v1 + n2), F(n2);
+  const double nv = std::sqrt(vdot(x, x));
+  const double eff_tol = std::max(P.tol * (nv + 1e-30), P.abs_floor);
+  const double fd_h = P.nk_fd_rel > 0 ? P.nk_fd_rel * std::max(nv, 1.0) : (P.nk_fd_h > 0 ? P.nk_fd_h : 1e-6);
+  int ne, nn;
+  double rn, r0;
+  const int status = newton_krylov(x, Fn, F, P.nk_krylov_max, eff_tol, P.max_iter, fd_h, P.nk_eta_max, &ne, &rn, &nn,
+                                   &r0);
+  if (P.exit_picard_step)
+    for (int64_t i = 0; i < n2; ++i) x[i] -= F[i];
+  std::memcpy(v1, x.data(), sizeof(double) * n2);
+  st->iters = ne;
+  st->n_restart = nn;
+  st->residual = rn;
+  st->r0 = r0;
+  st->exit_reason = status;
+  st->inner = 0;
+}
+
 // v (in/out), fc (in: projection of v; out: projection of the new v),
 // S0 = entropy of fc on entry.
 void orc_time_step(const orc_mesh* ms, const orc_solver_params* P, double* v, const double* w, int64_t N, double* fc,
@@ -1399,6 +1657,8 @@ void orc_time_step(const orc_mesh* ms, const orc_solver_params* P, double* v, co
   orc_predictor(ms, P, v, w, N, fc, v1.data(), nullptr);
   if (P->solver == 1)
     orc_step_defect(ms, P, v1.data(), v, w, N, S0, st, nullptr);
+  else if (P->solver == 2)
+    orc_step_newton(ms, P, v1.data(), v, w, N, S0, st);
   else
     orc_step_anderson(ms, P, v1.data(), v, w, N, S0, st, nullptr);
   std::memcpy(v, v1.data(), sizeof(double) * n2);

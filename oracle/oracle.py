@@ -53,6 +53,10 @@ class SolverParams(C.Structure):
         ("dc_near_theta", C.c_double),
         ("collision_model", C.c_int),
         ("nu", C.c_double),
+        ("nk_krylov_max", C.c_int),
+        ("nk_eta_max", C.c_double),
+        ("nk_fd_h", C.c_double),
+        ("nk_fd_rel", C.c_double),
     ]
 
 
@@ -80,6 +84,7 @@ def default_params(dt: float, m: int = 5, nthreads: int = 0, **kw) -> SolverPara
         reg_factor=1e-10, use_anderson=1, use_gonzalez=1, exit_picard_step=1,
         use_logsq=0, nthreads=nthreads, solver=0, dc_eta=0.05, dc_max_inner=60, dc_stag_window=5,
         dc_outer_m=0, dc_near_theta=0.0, collision_model=0, nu=1.0,
+        nk_krylov_max=30, nk_eta_max=0.9, nk_fd_h=0.0, nk_fd_rel=0.0,
     )
     for k, v in kw.items():
         setattr(p, k, v)
@@ -102,7 +107,8 @@ def lib():
         L.orc_landau_geff.restype = C.c_double
         for name in ("orc_mesh_info", "orc_mesh_tables", "orc_mass_dense", "orc_mass_solve",
                      "orc_eval_particle", "orc_l2_rhs", "orc_l2_project", "orc_entropy",
-                     "orc_negative_part_l1", "orc_fs_minus_fp_l2", "orc_lb_rhs", "orc_lb_geff",
+                     "orc_negative_part_l1", "orc_fs_minus_fp_l2", "orc_lb_rhs", "orc_lb_geff", "orc_gmres_dense", "orc_newton_midpoint",
+                     "orc_step_newton",
                      "orc_compute_r", "orc_compute_G", "orc_loggrad", "orc_landau_geff",
                      "orc_picard_map", "orc_predictor", "orc_step_anderson", "orc_step_defect", "orc_time_step"):
             getattr(L, name).argtypes = None
@@ -372,6 +378,35 @@ def lb_rhs(v, w, g, nu: float = 1.0):
     AB = np.zeros(3)
     lib().orc_lb_rhs(_p(v), _p(w), _p(g), C.c_int64(v.shape[0]), C.c_double(nu), _p(dv), _p(AB))
     return dv, AB
+
+
+def gmres_dense(A, b, rtol: float, atol: float = 0.0, itmax: int | None = None, mem: int | None = None):
+    """gmres! (newton_krylov.jl:60-131) on y = A u from x0 = 0: (x, products, residual norm)."""
+    A = np.ascontiguousarray(A, dtype=np.float64)
+    b = np.ascontiguousarray(b, dtype=np.float64)
+    n = b.shape[0]
+    mem = n if mem is None else mem
+    itmax = mem if itmax is None else itmax
+    x = np.zeros(n)
+    r = C.c_double(0.0)
+    k = lib().orc_gmres_dense(C.c_int(n), _p(A), _p(b), C.c_double(rtol), C.c_double(atol), C.c_int(itmax),
+                              C.c_int(mem), _p(x), C.byref(r))
+    return x, int(k), float(r.value)
+
+
+def newton_midpoint(B, x0, dt: float, x, tol: float, max_evals: int, fd_h: float, krylov_max: int = 20):
+    """newton_krylov! (newton_krylov.jl:189-303) on F(x) = x - x0 - dt f((x0 + x)/2),
+    f(y) = tanh(B y) - y^3 / 2.  Returns (x, F, status, n_evals, rnorm, n_newton, rnorm0);
+    status 0 converged, 1 stalled, 2 budget."""
+    B = np.ascontiguousarray(B, dtype=np.float64)
+    x0 = np.ascontiguousarray(x0, dtype=np.float64)
+    x = np.array(x, dtype=np.float64)
+    n = x0.shape[0]
+    F = np.zeros(n)
+    out = np.zeros(4)
+    st = lib().orc_newton_midpoint(C.c_int(n), _p(B), _p(x0), C.c_double(dt), C.c_double(tol), C.c_int(max_evals),
+                                   C.c_double(fd_h), C.c_int(krylov_max), _p(x), _p(F), _p(out))
+    return x, F, int(st), int(out[0]), float(out[1]), int(out[2]), float(out[3])
 
 
 def momentum_energy(v, w):
