@@ -521,11 +521,14 @@ def main():
     log(f"[bench] chosen: {rch['steps_per_s']:.3f} steps/s, mean iters {rch['mean_iters']:.1f} "
         f"(f32 {rch['mean_iters_f32']:.1f})")
 
-    rva = rve = rs64 = None
+    rva = rve = rs64 = rnk = None
     if not args.no_variants:
         log("[bench] comparison path: the plain FP64 path with the symmetric pair sum")
         rs64 = run_path(ws, p, v, w, collop.PREC_F64, args.warmup, args.steps)
         log(f"[bench] FP64 + symmetric pair sum: {rs64['steps_per_s']:.3f} steps/s, mean iters {rs64['mean_iters']:.1f}")
+        log("[bench] comparison path: Jacobian-free Newton-Krylov (step_newton!), FP64 maps, symmetric pair sum")
+        rnk = run_path(ws, p, v, w, collop.PREC_F64, args.warmup, args.steps, solver=collop._lib.SOLVER_NEWTON)
+        log(f"[bench] FP64 Newton-Krylov: {rnk['steps_per_s']:.3f} steps/s, mean maps {rnk['mean_iters']:.1f}")
     if not args.no_variants and args.chosen_solver == "defect":
         log("[bench] comparison path: FP32c maps + FP64 finish with the Anderson solver")
         rva = run_path(ws, p, v, w, collop.PREC_F32C, args.warmup, args.steps, solver=collop._lib.SOLVER_ANDERSON)
@@ -597,6 +600,12 @@ def main():
                            {"steps_per_s": round(rs64["steps_per_s"], 4),
                             "ratio_vs_fp64": round(rs64["steps_per_s"] / r64["steps_per_s"], 4),
                             "mean_iters": rs64["mean_iters"], "all_converged": rs64["all_converged"]}),
+        "fp64_newton_krylov": (None if rnk is None else
+                               {"steps_per_s": round(rnk["steps_per_s"], 4),
+                                "ratio_vs_fp64": round(rnk["steps_per_s"] / r64["steps_per_s"], 4),
+                                "mean_maps": rnk["mean_iters"],
+                                "mean_newton_steps": float(np.mean([st["n_restart"] for st in rnk["stats"]])),
+                                "all_converged": rnk["all_converged"]}),
         "chosen_euler_predictor": (None if rve is None else
                                    {"steps_per_s": round(rve["steps_per_s"], 4),
                                     "ratio_vs_fp64": round(rve["steps_per_s"] / r64["steps_per_s"], 4),
