@@ -285,7 +285,6 @@ int ensure_scratch(collop_ctx* ctx, int64_t N) {
   ENSURE(ctx->b_vec, (size_t)4 * m.n_dofs * 8);  // [f_mid, f_in, r, L]
   ENSURE(ctx->b_scal, 16 * 8);                    // S0, S1, lambda, E, P1, P2, ...
   ENSURE(ctx->b_statp, (size_t)epi_blocks((int)N) * kStatWidth * 8);
-  ENSURE(ctx->b_stat, kStatWidth * 8);
   ENSURE(ctx->b_stmp, (size_t)2 * m.n_dofs * 8);
   ENSURE(ctx->b_lbpart, (size_t)lb_blocks((int)N) * kLbSums * 8);
   return COLLOP_OK;
@@ -316,6 +315,12 @@ int collop_create(int device, collop_ctx** out) {
     delete c;
     return COLLOP_ERR_HIP;
   }
+  if (hipHostGetDevicePointer((void**)&c->h_stat_dev, c->h_stat, 0) != hipSuccess) {
+    (void)hipHostFree(c->h_stat);
+    (void)hipStreamDestroy(c->stream);
+    delete c;
+    return COLLOP_ERR_HIP;
+  }
   *out = c;
   return COLLOP_OK;
 }
@@ -329,12 +334,12 @@ int collop_destroy(collop_ctx* ctx) {
   free_mesh(ctx);
   for (DBuf* b : {&ctx->b_v, &ctx->b_w, &ctx->b_G, &ctx->b_out, &ctx->b_vmid, &ctx->b_dv, &ctx->b_dS, &ctx->b_s64,
                   &ctx->b_s32, &ctx->b_part, &ctx->b_spart_sc, &ctx->b_mom, &ctx->b_dpart, &ctx->b_iota,
-                  &ctx->b_spart_q, &ctx->b_vec, &ctx->b_scal, &ctx->b_statp, &ctx->b_stat, &ctx->b_stmp, &ctx->b_lbpart, &ctx->nk_V, &ctx->nk_vec,
+                  &ctx->b_spart_q, &ctx->b_vec, &ctx->b_scal, &ctx->b_statp, &ctx->b_stmp, &ctx->b_lbpart, &ctx->nk_V, &ctx->nk_vec,
                   &ctx->nk_part, &ctx->nk_h, &ctx->nk_A, &ctx->s_v, &ctx->s_w,
                   &ctx->s_v1, &ctx->s_Gv, &ctx->s_r, &ctx->s_rp, &ctx->s_Gp, &ctx->s_best, &ctx->s_dF, &ctx->s_dG,
                   &ctx->s_fc, &ctx->s_dotv, &ctx->s_Gvo, &ctx->s_Gk, &ctx->s_FA, &ctx->s_D, &ctx->s_Ge, &ctx->s_vmk, &ctx->s_xk, &ctx->s_oU,
                   &ctx->s_oUp, &ctx->s_of, &ctx->s_ofp, &ctx->s_odF, &ctx->s_odG, &ctx->b_near_list, &ctx->b_near_count,
-                  &ctx->b_ntg, &ctx->b_noff, &ctx->b_nal, &ctx->m_params, &ctx->m_adam_m, &ctx->m_adam_v, &ctx->m_data, &ctx->m_rec,
+                  &ctx->b_ntg, &ctx->b_noff, &ctx->b_nal, &ctx->b_nidx, &ctx->s_v1n, &ctx->m_params, &ctx->m_adam_m, &ctx->m_adam_v, &ctx->m_data, &ctx->m_rec,
                   &ctx->m_y, &ctx->m_gpart, &ctx->m_lpart, &ctx->m_tiles, &ctx->m_eval, &ctx->m_grad})
     release(*b);
   for (auto& e : ctx->ev_pool) {
@@ -883,10 +888,9 @@ int epilogue_run(collop_ctx* ctx, const EpiArgs& a) {
   const int n = (int)ctx->N;
   launch_epilogue(a, n, ctx->stream);
   CHECK_LAUNCH();
-  launch_sum_rows(P<double>(ctx->b_statp), epi_blocks(n), kStatWidth, 1, kStatWidth, P<double>(ctx->b_stat),
-                  ctx->stream);
+  // the reduction writes the statistics straight into the pinned host buffer
+  launch_sum_rows(P<double>(ctx->b_statp), epi_blocks(n), kStatWidth, 1, kStatWidth, ctx->h_stat_dev, ctx->stream);
   CHECK_LAUNCH();
-  HIPC(hipMemcpyAsync(ctx->h_stat, ctx->b_stat.p, kStatWidth * 8, hipMemcpyDeviceToHost, ctx->stream));
   HIPC(hipStreamSynchronize(ctx->stream));
   prof_collect(ctx);
   return COLLOP_OK;
@@ -935,6 +939,8 @@ int build_near(collop_ctx* ctx) {
   HIPC(hipMemcpy(ctx->b_noff.p, off.data(), off.size() * 4, hipMemcpyHostToDevice));
   HIPC(hipMemcpy(ctx->b_nal.p, al.data(), al.size() * 4, hipMemcpyHostToDevice));
   ctx->near_nt = (int)tg.size();
+  launch_near_index(ctx->near_nt, P<int>(ctx->b_ntg), (int)ctx->N, P<int>(ctx->b_nidx), ctx->stream);
+  CHECK_LAUNCH();
   return COLLOP_OK;
 }
 
@@ -986,6 +992,79 @@ int outer_map_dev(collop_ctx* ctx, const collop_step_params& p, int kind, bool f
 
 // Inner map of defect correction (solver.jl:598-607): Phi(u) = G(v_k) - dt A_k
 // (G_eff(u) - G_k), no pair sum; Phi(u) lands in s_Gv with the Anderson epilogue.
+// The same map with its small kernels fused (split mass solves only): the
+// pending Anderson update applied by the scatter (the updated iterate lands in
+// s_v1n, then takes s_v1's place), quadrature passes 1 and 2 in one launch, and
+// G_eff, D = A (G_eff - G_k) and the near-pair corrections inside the
+// epilogue.  The same arithmetic as the unfused map.
+int frozen_map_fused(collop_ctx* ctx, const collop_step_params& p, int col, int hist, const UpdateArgs* upd) {
+  const MeshView& m = ctx->mv;
+  const int n = (int)ctx->N;
+  const int nsb = scatter_blocks(n);
+  double* vec = P<double>(ctx->b_vec);
+  double* sc = P<double>(ctx->b_scal);
+  const int fields = p.use_gonzalez ? 2 : 1;
+  launch_scatter(m, P<double>(ctx->s_v), P<double>(ctx->s_v1), P<double>(ctx->s_w), n, 2, fields, ctx->w_abs_sum,
+                 P<double>(ctx->b_vmid), P<double>(ctx->b_dv), P<long long>(ctx->b_spart_sc), nullptr, ctx->stream,
+                 upd, upd ? P<double>(ctx->s_v1n) : nullptr);
+  CHECK_LAUNCH();
+  if (upd) std::swap(ctx->s_v1, ctx->s_v1n);
+  launch_solve_split(m, P<long long>(ctx->b_spart_sc), nsb, ctx->w_abs_sum, nullptr, fields, vec,
+                     P<double>(ctx->b_stmp), ctx->stream);
+  CHECK_LAUNCH();
+  const int nqb = quad_blocks(m);
+  launch_quad_r(m, vec, p.use_gonzalez ? vec + m.n_dofs : nullptr, p.use_logsq, vec + 2 * m.n_dofs,
+                P<double>(ctx->b_spart_q), ctx->stream);
+  CHECK_LAUNCH();
+  launch_solve_split(m, nullptr, 0, 1.0, vec + 2 * m.n_dofs, 1, vec + 3 * m.n_dofs, P<double>(ctx->b_stmp),
+                     ctx->stream, p.use_gonzalez ? P<double>(ctx->b_spart_q) : nullptr, nqb,
+                     p.use_gonzalez ? sc + 1 : nullptr);
+  CHECK_LAUNCH();
+  launch_gather(m, P<double>(ctx->b_vmid), vec + 3 * m.n_dofs, P<double>(ctx->s_w), P<double>(ctx->b_dv), n, 1, 0.0,
+                0.0, P<double>(ctx->b_dS), P<double>(ctx->b_dpart), ctx->stream);
+  CHECK_LAUNCH();
+  EpiArgs a{};
+  a.part = nullptr;
+  a.S = 0;
+  a.v0 = P<double>(ctx->s_Gvo);
+  a.v1 = P<double>(ctx->s_v1);
+  a.dt = -p.dt;
+  a.Gv = P<double>(ctx->s_Gv);
+  a.r = P<double>(ctx->s_r);
+  a.rp = P<double>(ctx->s_rp);
+  a.Gp = P<double>(ctx->s_Gp);
+  a.dF = P<double>(ctx->s_dF);
+  a.dG = P<double>(ctx->s_dG);
+  a.col = col;
+  a.hist = hist;
+  a.stat_part = P<double>(ctx->b_statp);
+  FrozenArgs f{};
+  f.dS = P<double>(ctx->b_dS);
+  f.dv = P<double>(ctx->b_dv);
+  f.w = P<double>(ctx->s_w);
+  f.Gk = P<double>(ctx->s_Gk);
+  f.A = P<double>(ctx->s_FA) + 2 * (size_t)n;
+  f.dpart = P<double>(ctx->b_dpart);
+  f.nparts = gather_blocks(n);
+  f.S1 = sc + 1;
+  f.S0 = sc + 0;
+  f.use_gonzalez = p.use_gonzalez;
+  if (ctx->near_nt > 0) {
+    f.nidx = P<int>(ctx->b_nidx);
+    f.off = P<int>(ctx->b_noff);
+    f.al = P<int>(ctx->b_nal);
+    f.vm = P<double>(ctx->b_vmid);
+    f.vmk = P<double>(ctx->s_vmk);
+  }
+  launch_epilogue_frozen(a, f, n, ctx->stream);
+  CHECK_LAUNCH();
+  launch_sum_rows(P<double>(ctx->b_statp), epi_blocks(n), kStatWidth, 1, kStatWidth, ctx->h_stat_dev, ctx->stream);
+  CHECK_LAUNCH();
+  HIPC(hipStreamSynchronize(ctx->stream));
+  prof_collect(ctx);
+  return COLLOP_OK;
+}
+
 int frozen_map_dev(collop_ctx* ctx, const collop_step_params& p, int col, int hist) {
   const int n = (int)ctx->N;
   const bool near = ctx->near_nt > 0;
@@ -1261,6 +1340,7 @@ int defect_solve(collop_ctx* ctx, const collop_step_params& p, collop_step_stats
   // outer Anderson window (extension): Anderson with beta = 1 on v -> T(v), the
   // inner solution, with the epilogue's column bookkeeping
   const int mo = p.dc_outer_m;
+  const bool fused = ctx->mv.nd1 <= 256 && ctx->mv.nd2 <= 256;  // the fused frozen map needs the split solves
   int ohist = 0, oslot = 0;
   bool oprev = false;
   double oATA[kMaxWindow * kMaxWindow] = {0};
@@ -1324,6 +1404,8 @@ int defect_solve(collop_ctx* ctx, const collop_step_params& p, collop_step_stats
     // inner Anderson on u = Phi_k(u), u = s_v1 (solver.jl:507-545)
     const double tol_in = std::max(p.dc_eta * nr, 0.5 * p.abs_floor);
     int hist = 0, slot = 0;
+    UpdateArgs pend{};
+    bool has_pend = false;
     double ATA[kMaxWindow * kMaxWindow] = {0};
     for (int j = 1; j <= p.dc_max_inner; ++j) {
       const int col = j == 1 ? -1 : slot % mw;
@@ -1333,6 +1415,9 @@ int defect_solve(collop_ctx* ctx, const collop_step_params& p, collop_step_stats
         // near-pair differences vanish exactly, so the first inner iterate's map
         // value, residual and statistics are the outer map's (s_Gvo, s_r, h_stat)
         if ((r = copy_d2d(ctx, ctx->s_Gv, ctx->s_Gvo, n2b))) return r;
+      } else if (fused) {
+        if ((r = frozen_map_fused(ctx, p, col, hist_new, has_pend ? &pend : nullptr))) return r;
+        has_pend = false;
       } else if ((r = frozen_map_dev(ctx, p, col, hist_new))) {
         return r;
       }
@@ -1344,10 +1429,22 @@ int defect_solve(collop_ctx* ctx, const collop_step_params& p, collop_step_stats
         std::swap(ctx->s_v1, ctx->s_Gv);  // u <- Phi(u)
         break;
       }
+      // the next frozen map's scatter applies the update (the loop's last
+      // iterate is updated here, as is every one when the maps are not fused)
+      const bool defer = fused && j < p.dc_max_inner;
       if (j == 1) {
-        launch_update(P<double>(ctx->s_v1), P<double>(ctx->s_Gv), P<double>(ctx->s_dG), nullptr, 0, p.damping, n,
-                      ctx->stream);
-        CHECK_LAUNCH();
+        if (defer) {
+          pend = UpdateArgs{};
+          pend.Gv = P<double>(ctx->s_Gv);
+          pend.dG = P<double>(ctx->s_dG);
+          pend.hist = 0;
+          pend.beta = p.damping;
+          has_pend = true;
+        } else {
+          launch_update(P<double>(ctx->s_v1), P<double>(ctx->s_Gv), P<double>(ctx->s_dG), nullptr, 0, p.damping, n,
+                        ctx->stream);
+          CHECK_LAUNCH();
+        }
       } else {
         slot = slot % mw + 1;
         hist = hist_new;
@@ -1365,13 +1462,24 @@ int defect_solve(collop_ctx* ctx, const collop_step_params& p, collop_step_stats
         const double l2 = p.reg_factor * dm / hist + 1e-30;
         for (int i = 0; i < hist; ++i) A[i * hist + i] += l2;
         small_solve(hist, A, b, g);
-        launch_update(P<double>(ctx->s_v1), P<double>(ctx->s_Gv), P<double>(ctx->s_dG), g, hist, p.damping, n,
-                      ctx->stream);
-        CHECK_LAUNCH();
+        if (defer) {
+          pend = UpdateArgs{};
+          pend.Gv = P<double>(ctx->s_Gv);
+          pend.dG = P<double>(ctx->s_dG);
+          for (int i = 0; i < hist; ++i) pend.g[i] = g[i];
+          pend.hist = hist;
+          pend.beta = p.damping;
+          has_pend = true;
+        } else {
+          launch_update(P<double>(ctx->s_v1), P<double>(ctx->s_Gv), P<double>(ctx->s_dG), g, hist, p.damping, n,
+                        ctx->stream);
+          CHECK_LAUNCH();
+        }
       }
       std::swap(ctx->s_rp, ctx->s_r);
       std::swap(ctx->s_Gp, ctx->s_Gv);
     }
+    if (has_pend) return fail(ctx, COLLOP_ERR_HIP, "internal: an Anderson update was left pending");
     if (mo > 0) {  // outer Anderson: f = U - x_k with U the inner solution
       EpiArgs a{};
       a.part = nullptr;
@@ -1600,6 +1708,8 @@ int collop_state_set(collop_ctx* ctx, const double* v, const double* w, int64_t 
   ENSURE(ctx->b_ntg, (size_t)kNearCap * 4);
   ENSURE(ctx->b_noff, (size_t)(kNearCap + 1) * 4);
   ENSURE(ctx->b_nal, (size_t)kNearCap * 4);
+  ENSURE(ctx->b_nidx, (size_t)N * 4);
+  ENSURE(ctx->s_v1n, n2b);
   ENSURE(ctx->s_w, (size_t)N * 8);
   ENSURE(ctx->s_dF, n2b * kMaxWindow);
   ENSURE(ctx->s_dG, n2b * kMaxWindow);

@@ -36,7 +36,7 @@ struct collop_ctx {
   std::vector<DBuf*> mesh_bufs;
   // ---- scratch buffers by name (grow-only)
   DBuf b_v, b_w, b_G, b_out, b_vmid, b_dv, b_dS, b_s64, b_s32, b_part, b_spart_sc, b_mom, b_dpart, b_iota, b_spart_q,
-      b_vec, b_scal, b_statp, b_stat, b_stmp, b_lbpart;
+      b_vec, b_scal, b_statp, b_stmp, b_lbpart;
   DBuf nk_V, nk_vec, nk_part, nk_h, nk_A;  // Newton-Krylov: basis, work vectors, dot partials, scalars, dense A
   // ---- device-resident state
   bool has_state = false;
@@ -48,8 +48,11 @@ struct collop_ctx {
   DBuf s_Ge, s_vmk;             // G_eff and v_mid of the inner iterate / of v_k (near field)
   DBuf s_xk, s_oU, s_oUp, s_of, s_ofp, s_odF, s_odG;  // outer Anderson window
   DBuf b_near_list, b_near_count, b_ntg, b_noff, b_nal;  // near pairs (list, CSR by target)
+  DBuf b_nidx;                  // near-target row of every marker (-1: none)
+  DBuf s_v1n;                   // the inner iterate after a scatter-applied Anderson update
   int near_nt = 0;              // near targets of the current step
   double* h_stat = nullptr;  // pinned, kStatWidth doubles
+  double* h_stat_dev = nullptr;  // its device-side address: the statistics reduction writes there
   // ---- warm-start MLP
   DBuf m_params, m_adam_m, m_adam_v, m_data, m_rec, m_y, m_gpart, m_lpart, m_tiles, m_eval, m_grad;
   bool m_has_params = false, m_has_norm = false;

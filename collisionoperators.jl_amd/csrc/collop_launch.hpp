@@ -118,9 +118,19 @@ void launch_slice_sum(const double* part, int S, int64_t len, double* out, hipSt
 // field 0 also write partial sums (w v1, w v2, w |x|^2) of the scattered
 // positions to mom[blocks][3].
 double scatter_scale(double w_abs_sum);  // 2^62 / W (fixed-point scale of the scatter)
+// An Anderson update v1 <- beta (Gv - sum_j dG_j g_j) + (1 - beta) v1
+// (solver.jl:383-412) that a scatter applies to its input markers as it reads
+// them (single run: components N apart, dG columns 2N apart).
+struct UpdateArgs {
+  const double* Gv;
+  const double* dG;
+  double g[kMaxWindow];
+  int hist;
+  double beta;
+};
 void launch_scatter(const MeshView& m, const double* v0, const double* vin, const double* w, int N, int fields,
                     int grid_fields, double w_abs_sum, double* vmid, double* dv, long long* part, double* mom,
-                    hipStream_t st);
+                    hipStream_t st, const UpdateArgs* upd = nullptr, double* vin_out = nullptr);
 // c[field] = M^{-1} rhs[field]; rhs is the exact sum of the fixed-point partial
 // rows, or read from `direct[field * n_dofs]` when part == nullptr.  One
 // workgroup per field.
@@ -161,6 +171,11 @@ void launch_quad1(const MeshView& m, const double* c0, const double* c1, int fie
                   double* spart, hipStream_t st);
 // r_k = sum over cells / points of phi_k iota  (compute_r!, functions.jl:158-179)
 void launch_quad2(const MeshView& m, const double* iota, double* r, hipStream_t st);
+// Quadrature passes 1 and 2 in one launch: r = compute_r!(c0) with the pass-1
+// integrand evaluated where pass 2 reads it (the same arithmetic), and, when
+// c1 is given, the block partials of S(c1) into spart[0 .. quad_blocks).
+void launch_quad_r(const MeshView& m, const double* c0, const double* c1, int use_logsq, double* r, double* spart,
+                   hipStream_t st);
 // out[i] = sum_{j<n} part[i * istride + j * jstride] in fixed order, i < count
 void launch_sum_rows(const double* part, int n, int count, int istride, int jstride, double* out, hipStream_t st);
 // Gather: G (mode 0), or dS = -w G plus partial (dv.dS, |dv|^2) (mode 1),
@@ -216,6 +231,29 @@ void launch_update(double* v1, const double* Gv, const double* dG, const double*
 // tg[nt], partners al[off[t] .. off[t+1])): D -= sum (exact - frozen) pair terms.
 void launch_near(int nt, const int* tg, const int* off, const int* al, int N, const double* w, const double* vm,
                  const double* Ge, const double* vmk, const double* Gk, double* D, hipStream_t st);
+// The frozen-metric map's G_eff, D = A (G_eff - G_k) and near-pair correction
+// (k_geff_pack + k_near), computed inside the Anderson epilogue instead of in
+// launches of their own.  nidx[a] = the near-target row of marker a, or -1.
+struct FrozenArgs {
+  const double* dS;
+  const double* dv;
+  const double* w;
+  const double* Gk;
+  const double* A;      // metric, 3N: A11, A12, A22
+  const double* dpart;  // gather partials (dv . dS, |dv|^2)
+  int nparts;
+  const double* S1;
+  const double* S0;
+  int use_gonzalez;
+  const int* nidx;      // null: no near field
+  const int* off;
+  const int* al;
+  const double* vm;     // v_mid of the inner iterate
+  const double* vmk;    // v_mid of v_k
+};
+void launch_epilogue_frozen(const EpiArgs& a, const FrozenArgs& f, int N, hipStream_t st);
+// nidx[tg[t]] = t for t < nt (nidx cleared to -1 first)
+void launch_near_index(int nt, const int* tg, int N, int* nidx, hipStream_t st);
 // out = v + dt * (sum over slices of part)  (the Euler predictor, main.jl:245)
 void launch_euler(const double* part, int S, const double* v, double dt, int N, double* out, double* dotv,
                   hipStream_t st);

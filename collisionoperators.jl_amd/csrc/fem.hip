@@ -38,7 +38,8 @@ __device__ __forceinline__ void scatter_body(const MeshView& m, unsigned long lo
                                              const double* __restrict__ vin, const double* __restrict__ w, int N,
                                              int ld, int fields, int field, double scale, double* __restrict__ vmid,
                                              double* __restrict__ dv, long long* __restrict__ out_row,
-                                             double* __restrict__ mom_out, int lb, int nbk) {
+                                             double* __restrict__ mom_out, int lb, int nbk,
+                                             const UpdateArgs* upd = nullptr, double* vin_out = nullptr) {
   for (int k = threadIdx.x; k < m.n_dofs; k += blockDim.x) rhs[k] = 0ull;
   stage_bp(m, sb1, sb2);
   __syncthreads();
@@ -47,7 +48,22 @@ __device__ __forceinline__ void scatter_body(const MeshView& m, unsigned long lo
   for (int a = lb * blockDim.x + threadIdx.x; a < N; a += stride) {
     double x, y;
     if (fields == 2) {
-      const double a1 = v0[a], a2 = v0[ld + a], b1 = vin[a], b2 = vin[ld + a];
+      const double a1 = v0[a], a2 = v0[ld + a];
+      double b1 = vin[a], b2 = vin[ld + a];
+      if (upd) {  // the pending Anderson update of the input, written once (field 0) to vin_out
+        const size_t cs = (size_t)2 * N;
+        double x1 = upd->Gv[a], x2 = upd->Gv[ld + a];
+        for (int j = 0; j < upd->hist; ++j) {
+          x1 -= upd->dG[j * cs + a] * upd->g[j];
+          x2 -= upd->dG[j * cs + ld + a] * upd->g[j];
+        }
+        b1 = upd->beta * x1 + (1.0 - upd->beta) * b1;
+        b2 = upd->beta * x2 + (1.0 - upd->beta) * b2;
+        if (field == 0) {
+          vin_out[a] = b1;
+          vin_out[ld + a] = b2;
+        }
+      }
       const double m1 = 0.5 * (a1 + b1), m2 = 0.5 * (a2 + b2);
       if (field == 0) {
         vmid[a] = m1;
@@ -242,6 +258,49 @@ __device__ __forceinline__ void quad1_body(const MeshView& m, double* red, const
   }
 }
 
+// The pass-1 integrand (1 + log f) w_q |e| at Gauss point (a, b) of cell (e1, e2)
+// (0 where f is below the floor), with f = sum c phi.
+__device__ __forceinline__ double iota_at(const MeshView& m, const double* __restrict__ c, int e1, int e2, int a,
+                                          int b, int use_logsq) {
+  const double f = field_at_qp(m, c, e1, e2, a, b);
+  const double jac = (m.bp1[e1 + 1] - m.bp1[e1]) * (m.bp2[e2 + 1] - m.bp2[e2]);
+  const double wq = m.qw[a] * m.qw[b] * jac;
+  double lg = 0.0;
+  bool ok;
+  if (use_logsq) {
+    ok = f * f > 1e-60;
+    if (ok) lg = 0.5 * log(f * f);
+  } else {
+    ok = f > 1e-30;
+    if (ok) lg = log(f);
+  }
+  return ok ? (1.0 + lg) * wq : 0.0;
+}
+
+// Quadrature pass 2 for dof k with the pass-1 integrand evaluated in place
+// from the coefficients c (same points, same order as quad2_wave).
+__device__ __forceinline__ void quad_r_wave(const MeshView& m, const double* __restrict__ c, int use_logsq,
+                                            double* __restrict__ r, int k, int lane) {
+  if (k >= m.n_dofs) return;  // wave-uniform
+  const int i1 = k % m.nd1, i2 = k / m.nd1;
+  const int npt = m.nq * m.nq;
+  const int e1lo = m.erange1[2 * i1], n1 = m.erange1[2 * i1 + 1] - e1lo + 1;
+  const int e2lo = m.erange2[2 * i2], n2 = m.erange2[2 * i2 + 1] - e2lo + 1;
+  const int total = n1 * n2 * npt;
+  double acc = 0.0;
+  for (int idx = lane; idx < total; idx += 64) {
+    const int cell = idx / npt, pt = idx - cell * npt;
+    const int e2 = e2lo + cell / n1, e1 = e1lo + cell % n1;
+    const int a = pt % m.nq, b = pt / m.nq;
+    const double p1 = m.phiq1[(3 * e1 + i1 - m.bs1[e1]) * m.nq + a];
+    const double p2 = m.phiq2[(3 * e2 + i2 - m.bs2[e2]) * m.nq + b];
+    acc += p1 * iota_at(m, c, e1, e2, a, b, use_logsq) * p2;
+  }
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) acc += __shfl_xor(acc, off, 64);
+  if (lane == 0) r[k] = acc;
+}
+
 // Quadrature pass 2 for dof k, one wave: the lanes stride over the (cell,
 // point) pairs of the dof's support (at most 3 x 3 cells x nq^2 points), then
 // a fixed shuffle tree.  Lane 0 writes r[k].
@@ -354,14 +413,15 @@ __global__ __launch_bounds__(kScatterThreads) void k_scatter(MeshView m, const d
                                                              double scale, double* __restrict__ vmid,
                                                              double* __restrict__ dv,
                                                              long long* __restrict__ part,
-                                                             double* __restrict__ mom) {
+                                                             double* __restrict__ mom, UpdateArgs U, int has_upd,
+                                                             double* __restrict__ vin_out) {
   extern __shared__ unsigned long long lds_u[];
   __shared__ double sb1[kMaxBp], sb2[kMaxBp];
   __shared__ double red[kScatterThreads / 64];
   const int field = blockIdx.y;
   scatter_body(m, lds_u, sb1, sb2, red, v0, vin, w, N, N, fields, field, scale, vmid, dv,
                part + ((size_t)field * gridDim.x + blockIdx.x) * m.n_dofs, mom ? mom + 3 * blockIdx.x : nullptr,
-               blockIdx.x, gridDim.x);
+               blockIdx.x, gridDim.x, has_upd ? &U : nullptr, vin_out);
 }
 
 // One workgroup per field.  X lives in dynamic LDS (solve_lds).
@@ -434,6 +494,21 @@ __global__ __launch_bounds__(kQuadThreads) void k_quad1(MeshView m, const double
   const int field = blockIdx.y;
   quad1_body(m, red, field == 0 ? c0 : c1, use_logsq, field == 0 ? iota : nullptr,
              spart ? spart + (size_t)field * gridDim.x + blockIdx.x : nullptr, blockIdx.x);
+}
+
+// grid: (n_dofs + 3) / 4 blocks of pass-2 waves, then, when c1 is given,
+// quad_blocks(m) blocks of pass 1 over c1 for the partials of S(c1).
+__global__ __launch_bounds__(256) void k_quad_r(MeshView m, const double* __restrict__ c0,
+                                                const double* __restrict__ c1, int use_logsq, double* __restrict__ r,
+                                                double* __restrict__ spart) {
+  const int nr = (m.n_dofs + 3) / 4;
+  if ((int)blockIdx.x < nr) {
+    quad_r_wave(m, c0, use_logsq, r, blockIdx.x * 4 + (threadIdx.x >> 6), threadIdx.x & 63);
+    return;
+  }
+  __shared__ double red[kQuadThreads / 64];
+  const int lb = blockIdx.x - nr;
+  quad1_body(m, red, c1, use_logsq, nullptr, spart + lb, lb);
 }
 
 __global__ __launch_bounds__(256) void k_quad2(MeshView m, const double* __restrict__ iota, double* __restrict__ r) {
@@ -626,11 +701,13 @@ double scatter_scale(double w_abs_sum) {
 
 void launch_scatter(const MeshView& m, const double* v0, const double* vin, const double* w, int N, int fields,
                     int grid_fields, double w_abs_sum, double* vmid, double* dv, long long* part, double* mom,
-                    hipStream_t st) {
+                    hipStream_t st, const UpdateArgs* upd, double* vin_out) {
   const int nb = scatter_blocks(N);
   const size_t lds = (size_t)m.n_dofs * sizeof(unsigned long long);
+  UpdateArgs U{};
+  if (upd) U = *upd;
   hipLaunchKernelGGL(k_scatter, dim3(nb, grid_fields), dim3(kScatterThreads), lds, st, m, v0, vin, w, N, fields,
-                     scatter_scale(w_abs_sum), vmid, dv, part, mom);
+                     scatter_scale(w_abs_sum), vmid, dv, part, mom, U, upd ? 1 : 0, vin_out);
 }
 
 void launch_reduce_solve(const MeshView& m, const long long* part, int nblocks, double w_abs_sum,
@@ -668,6 +745,12 @@ void launch_quad1(const MeshView& m, const double* c0, const double* c1, int fie
 
 void launch_quad2(const MeshView& m, const double* iota, double* r, hipStream_t st) {
   hipLaunchKernelGGL(k_quad2, dim3((m.n_dofs + 3) / 4), dim3(256), 0, st, m, iota, r);
+}
+
+void launch_quad_r(const MeshView& m, const double* c0, const double* c1, int use_logsq, double* r, double* spart,
+                   hipStream_t st) {
+  const int nb = (m.n_dofs + 3) / 4 + (c1 ? quad_blocks(m) : 0);
+  hipLaunchKernelGGL(k_quad_r, dim3(nb), dim3(256), 0, st, m, c0, c1, use_logsq, r, spart);
 }
 
 void launch_sum_rows(const double* part, int n, int count, int istride, int jstride, double* out, hipStream_t st) {

@@ -89,7 +89,8 @@ __device__ __forceinline__ void geff_body(const double* __restrict__ vmid, const
 // Anderson column (solver.jl:391-394) and the block's partial sums of |r|^2,
 // |v1|^2, |Gv - v0|^2, dF_col . dF_j and dF_j . r (solver.jl:398-399).  The
 // partials `part` are compact [S][2n]; dF / dG columns are `cs` apart.
-__device__ __forceinline__ void epilogue_body(const EpiArgs& A, int N, int ld, size_t cs, double* red, int lb) {
+__device__ __forceinline__ void epilogue_body(const EpiArgs& A, int N, int ld, size_t cs, double* red, int lb,
+                                              const double* s_given = nullptr) {
   const int n2 = 2 * N;
   const int i = lb * blockDim.x + threadIdx.x;
   double st[kStatWidth];
@@ -98,7 +99,10 @@ __device__ __forceinline__ void epilogue_body(const EpiArgs& A, int N, int ld, s
   if (i < n2) {
     const size_t e = el(i, N, ld);
     double s = 0.0;
-    for (int k = 0; k < A.S; ++k) s += A.part[(size_t)k * n2 + i];
+    if (s_given)
+      s = *s_given;
+    else
+      for (int k = 0; k < A.S; ++k) s += A.part[(size_t)k * n2 + i];
     const double gv = A.v0[e] + A.dt * s;
     A.Gv[e] = gv;
     const double x1 = A.v1[e];
@@ -191,7 +195,73 @@ __global__ void k_near(int nt, const int* __restrict__ tg, const int* __restrict
   D[N + g] -= c2;
 }
 
+// The frozen-metric map's value at flattened element i (solver.jl:598-607):
+// G_eff of its marker from the gather's dS and the Gonzalez lambda (geff_body's
+// arithmetic), D = A (G_eff - G_k), minus the near-pair corrections of k_near
+// (the partners' G_eff formed the same way).
+__device__ __forceinline__ double geff_of(const FrozenArgs& F, double lam, int a, int N, double& g2) {
+  const double iw = 1.0 / F.w[a];
+  g2 = -(F.dS[N + a] + lam * F.dv[N + a]) * iw;
+  return -(F.dS[a] + lam * F.dv[a]) * iw;
+}
+
+__device__ __forceinline__ double frozen_value(const FrozenArgs& F, double lam, int i, int N) {
+  const int a = i < N ? i : i - N;
+  double g2;
+  const double g1 = geff_of(F, lam, a, N, g2);
+  const double d1 = g1 - F.Gk[a];
+  const double d2 = g2 - F.Gk[N + a];
+  double D = i < N ? F.A[a] * d1 + F.A[N + a] * d2 : F.A[N + a] * d1 + F.A[2 * (size_t)N + a] * d2;
+  if (F.nidx) {
+    const int t = F.nidx[a];
+    if (t >= 0) {
+      double c1 = 0.0, c2 = 0.0;
+      for (int p = F.off[t]; p < F.off[t + 1]; ++p) {
+        const int q = F.al[p];
+        double h2;
+        const double h1 = geff_of(F, lam, q, N, h2);
+        double e1, e2, f1, f2;
+        uterm(F.w[q], F.vm[a] - F.vm[q], F.vm[N + a] - F.vm[N + q], h1 - g1, h2 - g2, e1, e2);
+        uterm(F.w[q], F.vmk[a] - F.vmk[q], F.vmk[N + a] - F.vmk[N + q], F.Gk[q] - g1, F.Gk[N + q] - g2, f1, f2);
+        c1 += e1 - f1;
+        c2 += e2 - f2;
+      }
+      D -= i < N ? c1 : c2;
+    }
+  }
+  return D;
+}
+
 // ---- single-run kernels ---------------------------------------------------------
+__global__ __launch_bounds__(kEwThreads) void k_epilogue_frozen(EpiArgs A, FrozenArgs F, int N) {
+  __shared__ double red[kStatWidth * kEwThreads / 64];
+  __shared__ double lam_sh;
+  // the Gonzalez lambda from the gather partials, in geff_body's fixed order
+  double sums[2] = {0.0, 0.0};
+  if (F.use_gonzalez)
+    for (int b = threadIdx.x; b < F.nparts; b += blockDim.x) {
+      sums[0] += F.dpart[2 * b];
+      sums[1] += F.dpart[2 * b + 1];
+    }
+  block_sum_n<kEwThreads, 2>(sums, red);
+  if (threadIdx.x == 0) {
+    const double dot = sums[0], nrm2 = sums[1];
+    lam_sh = (F.use_gonzalez && nrm2 > 1e-30) ? (F.S1[0] - F.S0[0] - dot) / nrm2 : 0.0;
+  }
+  __syncthreads();
+  const double lam = lam_sh;
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  const double s = i < 2 * N ? frozen_value(F, lam, i, N) : 0.0;
+  EpiArgs a = A;
+  a.stat_part = A.stat_part + (size_t)blockIdx.x * kStatWidth;
+  epilogue_body(a, N, N, (size_t)2 * N, red, blockIdx.x, &s);
+}
+
+__global__ void k_near_index(int nt, const int* __restrict__ tg, int* __restrict__ nidx) {
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t < nt) nidx[tg[t]] = t;
+}
+
 __global__ __launch_bounds__(kEwThreads) void k_geff_pack(const double* __restrict__ vmid,
                                                           const double* __restrict__ dS,
                                                           const double* __restrict__ dv,
@@ -333,6 +403,15 @@ int epi_blocks(int N) { return (2 * N + kEwThreads - 1) / kEwThreads; }
 
 void launch_epilogue(const EpiArgs& a, int N, hipStream_t st) {
   hipLaunchKernelGGL(k_epilogue, dim3(epi_blocks(N)), dim3(kEwThreads), 0, st, a, N);
+}
+
+void launch_epilogue_frozen(const EpiArgs& a, const FrozenArgs& f, int N, hipStream_t st) {
+  hipLaunchKernelGGL(k_epilogue_frozen, dim3(epi_blocks(N)), dim3(kEwThreads), 0, st, a, f, N);
+}
+
+void launch_near_index(int nt, const int* tg, int N, int* nidx, hipStream_t st) {
+  (void)hipMemsetAsync(nidx, 0xff, (size_t)N * sizeof(int), st);
+  if (nt > 0) hipLaunchKernelGGL(k_near_index, dim3((nt + 255) / 256), dim3(256), 0, st, nt, tg, nidx);
 }
 
 void launch_update(double* v1, const double* Gv, const double* dG, const double* gam, int hist, double beta, int N,
