@@ -1360,11 +1360,13 @@ int defect_solve(collop_ctx* ctx, const collop_step_params& p, collop_step_stats
   int exit_reason = 2, k_done = p.max_iter, inner = 0, r;
   const bool trace = std::getenv("COLLOP_TRACE") != nullptr;
   // F32C: FP32-compensated outer maps (F and A) until the outer residual reaches
-  // the FP32 rounding floor of the step increment (or stops improving); then
-  // FP64 outer maps, which alone decide convergence, stagnation and the result.
+  // the FP32 rounding floor of the step increment, is predicted to reach it at
+  // the next map, or stops improving; then FP64 outer maps, which alone decide
+  // convergence, stagnation and the result.
   bool use32 = p.precision == COLLOP_PREC_F32C;
   const double floor_factor = p.f32_floor_factor > 0.0 ? p.f32_floor_factor : 16.0;
-  double best32 = std::numeric_limits<double>::infinity();
+  double best32 = std::numeric_limits<double>::infinity(), prev32 = 0.0;
+  bool switch_after = false;
   int since_best32 = 0, iters32 = 0, k64 = 0;
   // outer Anderson window (extension): Anderson with beta = 1 on v -> T(v), the
   // inner solution, with the epilogue's column bookkeeping
@@ -1404,6 +1406,12 @@ int defect_solve(collop_ctx* ctx, const collop_step_params& p, collop_step_stats
         ohist = oslot = 0;
         continue;
       }
+      // When the outer contraction so far, nr / prev32, predicts that the next
+      // FP32c map would only reach the FP32 floor, finish this iteration and
+      // evaluate the next iterate with an FP64 map: an FP32c map at the floor
+      // would serve only to detect the floor.
+      if (prev32 > 0.0 && nr < prev32 && nr * (nr / prev32) < floor32) switch_after = true;
+      prev32 = nr;
     } else {
       k64++;
       if (nr < nrm_best) {
@@ -1551,6 +1559,12 @@ int defect_solve(collop_ctx* ctx, const collop_step_params& p, collop_step_stats
       std::swap(ctx->s_of, ctx->s_ofp);
       std::swap(ctx->s_oU, ctx->s_oUp);
       oprev = true;
+    }
+    if (switch_after) {  // the predicted switch: FP64 outer maps from the next iterate on
+      switch_after = false;
+      use32 = false;
+      oprev = false;
+      ohist = oslot = 0;
     }
   }
   if (exit_reason == 2) {

@@ -206,7 +206,9 @@ typedef struct {
   int32_t precision;        /* collop_precision                                 */
   int32_t warmstart;        /* collop_warmstart                                 */
   double f32_floor_factor;  /* F32C: FP32 maps run until ||r|| < factor * 2^-24
-                               * ||Gv - v0|| (0 = 16), then FP64 maps finish    */
+                               * ||Gv - v0|| (0 = 16), then FP64 maps finish;
+                               DEFECT also switches one outer map early when the
+                               contraction so far predicts the floor next      */
   int32_t f32_keep_window;  /* F32C: 1 = keep the Anderson window across the
                                switch (the last FP32 iterate is re-evaluated in
                                FP64 first), 0 = restart it                      */
