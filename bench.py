@@ -102,6 +102,7 @@ def run_path(ws, p, v, w, precision, warmup, steps, solver=None, warmstart=None,
     prm = p.step_params(precision=precision, solver=solver, warmstart=warmstart)
     log(f"  knobs: solver {prm.solver} precision {prm.precision} dc_outer_m {prm.dc_outer_m} "
         f"dc_near_theta {prm.dc_near_theta} dc_metric_reuse {prm.dc_metric_reuse} dc_eta {prm.dc_eta} "
+        f"dc_eta_f64 {prm.dc_eta_f64} "
         f"warmstart {prm.warmstart} pair sum {'symmetric, tiles %d' % tiles if tiles else 'ordered'}")
     E0 = 0.5 * float(np.sum(w * (v[:, 0] ** 2 + v[:, 1] ** 2)))
     for i in range(warmup):

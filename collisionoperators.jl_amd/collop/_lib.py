@@ -79,6 +79,7 @@ class StepParams(C.Structure):
         ("dc_metric_reuse", C.c_int32),
         ("collision_model", C.c_int32),
         ("nu", C.c_double),
+        ("dc_eta_f64", C.c_double),
         ("nk_krylov_max", C.c_int32),
         ("nk_eta_max", C.c_double),
         ("nk_fd_h", C.c_double),

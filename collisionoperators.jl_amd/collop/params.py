@@ -59,6 +59,7 @@ class SimParameters:
     dc_outer_m: int = 0              # extension: outer Anderson window (0 = the reference's outer iteration)
     dc_near_theta: float = 0.0       # extension: exact near pairs in the frozen map (0 = none)
     dc_metric_reuse: int = 0         # extension: 0 A at every outer map, 1 FP64 maps reuse the FP32c A, 2 first only
+    dc_eta_f64: float = 0.0          # extension: inner tolerance of the FP64 outer iterations (0 = dc_eta)
     collision_model: str = "landau"  # :landau | :lb  (Parameters.jl:50)
     nu: float = 1.0                  # LB collision frequency (Parameters.jl:53)
     nk_krylov_max: int = 30          # Parameters.jl:92-100 (solver = :newton)
@@ -89,7 +90,7 @@ class SimParameters:
             solver=solver, dc_eta=self.dc_eta, dc_max_inner=self.dc_max_inner, dc_stag_window=self.dc_stag_window,
             dc_outer_m=self.dc_outer_m, dc_near_theta=self.dc_near_theta, dc_metric_reuse=self.dc_metric_reuse,
             collision_model=_lib.MODEL_LB if self.collision_model == "lb" else _lib.MODEL_LANDAU, nu=self.nu,
-            nk_krylov_max=self.nk_krylov_max, nk_eta_max=self.nk_eta_max, nk_fd_h=self.nk_fd_h,
+            dc_eta_f64=self.dc_eta_f64, nk_krylov_max=self.nk_krylov_max, nk_eta_max=self.nk_eta_max, nk_fd_h=self.nk_fd_h,
             nk_fd_rel=self.nk_fd_rel)
 
 
@@ -147,7 +148,7 @@ CONFIGS = {
 
 
 # The chosen path: FP32-compensated outer maps then FP64 outer maps, frozen-metric
-# defect correction (step_defect!) with the two extensions of this library (an
+# defect correction (step_defect!) with the extensions of this library (an
 # outer Anderson window of 3 and exact near pairs with w / |d| > 0.02), the
 # FP64 outer maps reusing the metric A of the last FP32c outer map.
 CHOSEN = dict(gpu_fp32=True, solver="defect", dc_outer_m=3, dc_near_theta=0.02, dc_metric_reuse=1)

@@ -1326,8 +1326,11 @@ int validate_params(collop_ctx* ctx, const collop_step_params* p) {
                 "Newton-Krylov runs FP64 maps and needs 1 <= nk_krylov_max <= 64, 0 < nk_eta_max < 1 and finite "
                 "nk_fd_h, nk_fd_rel >= 0");
   if (p->solver == COLLOP_SOLVER_DEFECT &&
-      (!(p->dc_eta > 0.0 && p->dc_eta < 1.0) || p->dc_max_inner < 1 || p->dc_stag_window < 1))
-    return fail(ctx, COLLOP_ERR_INVALID, "defect correction needs 0 < dc_eta < 1, dc_max_inner >= 1, dc_stag_window >= 1");
+      (!(p->dc_eta > 0.0 && p->dc_eta < 1.0) || p->dc_max_inner < 1 || p->dc_stag_window < 1 ||
+       !(p->dc_eta_f64 == 0.0 || (p->dc_eta_f64 > 0.0 && p->dc_eta_f64 < 1.0))))
+    return fail(ctx, COLLOP_ERR_INVALID,
+                "defect correction needs 0 < dc_eta < 1, dc_eta_f64 = 0 or in (0, 1), dc_max_inner >= 1, "
+                "dc_stag_window >= 1");
   if (p->solver == COLLOP_SOLVER_DEFECT &&
       (p->dc_outer_m < 0 || p->dc_outer_m > kMaxWindow || !(p->dc_near_theta >= 0.0) ||
        !std::isfinite(p->dc_near_theta) || p->dc_metric_reuse < 0 || p->dc_metric_reuse > 2))
@@ -1439,7 +1442,8 @@ int defect_solve(collop_ctx* ctx, const collop_step_params& p, collop_step_stats
     if (ctx->near_nt > 0 && (r = copy_d2d(ctx, ctx->s_vmk, ctx->b_vmid, n2b))) return r;  // v_mid of v_k
     if (mo > 0 && (r = copy_d2d(ctx, ctx->s_xk, ctx->s_v1, n2b))) return r;
     // inner Anderson on u = Phi_k(u), u = s_v1 (solver.jl:507-545)
-    const double tol_in = std::max(p.dc_eta * nr, 0.5 * p.abs_floor);
+    const double eta = (!use32 && p.dc_eta_f64 > 0.0) ? p.dc_eta_f64 : p.dc_eta;
+    const double tol_in = std::max(eta * nr, 0.5 * p.abs_floor);
     int hist = 0, slot = 0;
     UpdateArgs pend{};
     bool has_pend = false;

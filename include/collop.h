@@ -234,6 +234,9 @@ typedef struct {
                                2 = the step's first only                        */
   int32_t collision_model;  /* collop_model (Parameters.jl:50)                  */
   double nu;                /* LB collision frequency (Parameters.jl:53, 1.0)   */
+  double dc_eta_f64;        /* DEFECT extension: inner tolerance of the FP64
+                               outer iterations (0 = dc_eta); a tighter one
+                               trades inner maps for FP64 pair sums            */
   int32_t nk_krylov_max;    /* NEWTON: GMRES basis per Newton step (<= 64; 30)  */
   double nk_eta_max;        /* NEWTON: Eisenstat-Walker forcing cap (0.9)       */
   double nk_fd_h;           /* NEWTON: absolute finite-difference step of J u

@@ -107,6 +107,32 @@ def test_defect_param_errors(gpu_ctx):
     bad.solver = 7
     with pytest.raises(collop.CollopError):
         sim.run(bad, 1)
+    bad = collop.config("gaussian_n4096", solver="defect", dc_eta_f64=1.5).step_params()
+    with pytest.raises(collop.CollopError):
+        sim.run(bad, 1)
+
+
+def test_defect_eta_f64_matches_oracle(gpu_ctx):
+    """dc_eta_f64 sets the inner tolerance of the FP64 outer iterations; with
+    FP64 maps throughout that is every outer iteration, i.e. the oracle's
+    step_defect! with dc_eta = dc_eta_f64."""
+    v, w = collop.markers("gaussian", 1024, 42)
+    bp = uniform_bp(-6, 6, 16)
+    m = O.Mesh(bp, bp)
+    fc = m.l2_project(v, w)
+    S0 = m.entropy(fc)
+    P = O.default_params(0.005, m=5, nthreads=0, solver=1, dc_eta=0.002)
+    v_ref, _, st_ref = m.time_step(P, v, w, fc, S0)
+    ws = collop.Workspace(bp, bp, ctx=gpu_ctx)
+    sim = collop.Simulation(ws, v, w)
+    st = sim.run(collop.config("gaussian_n4096", DT=0.005, solver="defect", dc_eta_f64=0.002).step_params(), 1)[0]
+    dv = float(np.max(np.abs(sim.markers() - v_ref)))
+    print(f"eta_f64 0.002: oracle outer {st_ref.iters} inner {st_ref.inner}; gpu outer {st['iters']} "
+          f"inner {st['inner_maps']}; max|dv| {dv:.2e}")
+    assert st["exit_reason"] == 0 and st_ref.exit_reason == 0
+    assert abs(st["iters"] - st_ref.iters) <= 1
+    assert abs(st["inner_maps"] - st_ref.inner) <= max(3, st_ref.inner // 5)
+    assert dv <= 1e-9
 
 
 @pytest.mark.parametrize("dt", [0.005, 0.02])
