@@ -90,6 +90,70 @@ def pair_mode(ws: Workspace, N: int):
     return mode.value == _lib.PAIR_SYMMETRIC, tiles.value, slots.value
 
 
+class StreamEnsemble:
+    """Independent runs on one mesh advanced by the single-run device step,
+    several at once: each of ``workers`` contexts owns its own HIP stream and
+    buffers, and a host thread steps that context's share of the runs (largest
+    N² first, to the least-loaded worker), so the small O(N) kernels and the
+    pair sums of different runs overlap on the GPU.  Unlike :class:`Ensemble`
+    (one persistent launch per Anderson iteration) every solver of the
+    single-run step is available, defect correction included, and each run's
+    result is bitwise that of a :class:`Simulation` of its markers alone."""
+
+    def __init__(self, bp1, bp2, runs, workers: int = 4, device: int = 0, warm_start=None):
+        if workers < 1:
+            raise ValueError("workers must be >= 1")
+        self.v, self.w = [], []
+        for v, w in runs:
+            v = np.asfortranarray(np.asarray(v, dtype=np.float64))
+            w = np.ascontiguousarray(np.asarray(w, dtype=np.float64))
+            if v.ndim != 2 or v.shape[1] != 2 or w.shape != (v.shape[0],):
+                raise ValueError("each run needs v (N, 2) and w (N,)")
+            self.v.append(v)
+            self.w.append(w)
+        self.workers = min(workers, max(1, len(self.v)))
+        self.ws = [Workspace(bp1, bp2, device=device) for _ in range(self.workers)]
+        if warm_start is not None:
+            for ws in self.ws:
+                warm_start(ws)
+        load = [0.0] * self.workers
+        self.share: list[list[int]] = [[] for _ in range(self.workers)]
+        for r in sorted(range(len(self.v)), key=lambda r: -float(self.v[r].shape[0]) ** 2):
+            k = int(np.argmin(load))
+            self.share[k].append(r)
+            load[k] += float(self.v[r].shape[0]) ** 2
+
+    @property
+    def n_runs(self) -> int:
+        return len(self.v)
+
+    def step(self, params: _lib.StepParams, n_steps: int = 1) -> list[list[dict]]:
+        """n_steps steps of every run; returns stats[step][run]."""
+        from concurrent.futures import ThreadPoolExecutor
+
+        stats: list[list[dict]] = [[{} for _ in self.v] for _ in range(n_steps)]
+
+        def work(k: int):
+            for r in self.share[k]:
+                sim = Simulation(self.ws[k], self.v[r], self.w[r])
+                st = sim.run(params, n_steps)
+                for s in range(n_steps):
+                    stats[s][r] = st[s]
+                self.v[r] = sim.markers()
+
+        with ThreadPoolExecutor(self.workers) as ex:
+            for f in [ex.submit(work, k) for k in range(self.workers)]:
+                f.result()
+        return stats
+
+    def markers(self) -> list[np.ndarray]:
+        return [v.copy(order="F") for v in self.v]
+
+    def close(self):
+        for ws in self.ws:
+            ws.ctx.close()
+
+
 class Ensemble:
     """Independent runs with uneven N on one workspace, advanced together
     (collop_ensemble_*).  Each run solves exactly as a :class:`Simulation` of its
