@@ -17,6 +17,6 @@ from .mesh import Workspace, build_workspace, extraction_p2c1  # noqa: F401
 from .params import CHOSEN, CONFIGS, SimParameters, chosen_config, config, parse_overrides  # noqa: F401
 from .records import (CONS_COLS, STATS_COLS, RunRecord, checkpoint_path, cons_time_at,  # noqa: F401
                       load_checkpoint, migrate_cons_csv, run_recorded, save_checkpoint, truncate_csv_after)
-from .sim import (Ensemble, Simulation, StreamEnsemble, pair_mode, profile, set_pair_mode,  # noqa: F401
-                  set_pair_tile)
+from .sim import (Ensemble, Simulation, StreamEnsemble, pair_mode, profile, set_fused_maps,  # noqa: F401
+                  set_pair_mode, set_pair_tile)
 from .warmstart import WarmStart, glorot_init  # noqa: F401

@@ -82,6 +82,13 @@ def set_pair_mode(ws: Workspace, symmetric: bool = True, tiles: int = 0):
     ws.ctx.check(ws.ctx.lib.collop_set_pair_mode(ws.ctx.h, mode, tiles))
 
 
+def set_fused_maps(ws: Workspace, on: bool = True):
+    """Fused small kernels (the default) or one launch per stage in the
+    frozen-metric inner maps and the Lenard-Bernstein Anderson maps; both give
+    the same bits.  See collop_set_fused_maps."""
+    ws.ctx.check(ws.ctx.lib.collop_set_fused_maps(ws.ctx.h, 1 if on else 0))
+
+
 def pair_mode(ws: Workspace, N: int):
     """(symmetric, tiles, slots) a pair sum over N markers runs with now."""
     mode, tiles, slots = C.c_int(), C.c_int(), C.c_int()

@@ -1141,6 +1141,64 @@ int lb_map_dev(collop_ctx* ctx, const collop_step_params& p, const double* vin, 
   return epilogue_dev(ctx, P<double>(ctx->b_dS), 1, P<double>(ctx->s_v), p.dt, vin, P<double>(ctx->s_Gv), col, hist);
 }
 
+// The same LB map with its small kernels fused (split mass solves only): the
+// entropy quadrature's blocks inside the gather launch, and the reduction,
+// lambda, multipliers and v_dot inside the epilogue.  The same arithmetic as
+// lb_map_dev on s_v1.  (The Anderson update stays a launch of its own: with the
+// LB window of 8 columns, applying it inside the scatter, whose two projections
+// would each re-read the columns, costs more than the launch it saves.)
+int lb_map_fused(collop_ctx* ctx, const collop_step_params& p, int col, int hist) {
+  const MeshView& m = ctx->mv;
+  const int n = (int)ctx->N;
+  double* vec = P<double>(ctx->b_vec);
+  double* sc = P<double>(ctx->b_scal);
+  const int fields = p.use_gonzalez ? 2 : 1;
+  launch_scatter(m, P<double>(ctx->s_v), P<double>(ctx->s_v1), P<double>(ctx->s_w), n, 2, fields, ctx->w_abs_sum,
+                 P<double>(ctx->b_vmid), P<double>(ctx->b_dv), P<long long>(ctx->b_spart_sc), nullptr, ctx->stream);
+  CHECK_LAUNCH();
+  launch_solve_split(m, P<long long>(ctx->b_spart_sc), scatter_blocks(n), ctx->w_abs_sum, nullptr, fields, vec,
+                     P<double>(ctx->b_stmp), ctx->stream);
+  CHECK_LAUNCH();
+  launch_lb_gather_q(m, P<double>(ctx->b_vmid), vec, p.use_gonzalez ? vec + m.n_dofs : nullptr, p.use_logsq,
+                     P<double>(ctx->s_w), P<double>(ctx->b_dv), n, P<double>(ctx->b_G), P<double>(ctx->b_lbpart),
+                     P<double>(ctx->b_spart_q), ctx->stream);
+  CHECK_LAUNCH();
+  EpiArgs a{};
+  a.part = nullptr;
+  a.S = 0;
+  a.v0 = P<double>(ctx->s_v);
+  a.v1 = P<double>(ctx->s_v1);
+  a.dt = p.dt;
+  a.Gv = P<double>(ctx->s_Gv);
+  a.r = P<double>(ctx->s_r);
+  a.rp = P<double>(ctx->s_rp);
+  a.Gp = P<double>(ctx->s_Gp);
+  a.dF = P<double>(ctx->s_dF);
+  a.dG = P<double>(ctx->s_dG);
+  a.col = col;
+  a.hist = hist;
+  a.stat_part = P<double>(ctx->b_statp);
+  LbEpiArgs l{};
+  l.x = P<double>(ctx->b_vmid);
+  l.g = P<double>(ctx->b_G);
+  l.dv = P<double>(ctx->b_dv);
+  l.w = P<double>(ctx->s_w);
+  l.part = P<double>(ctx->b_lbpart);
+  l.nparts = lb_blocks(n);
+  l.qpart = p.use_gonzalez ? P<double>(ctx->b_spart_q) : nullptr;
+  l.nq = p.use_gonzalez ? quad_blocks(m) : 0;
+  l.S0 = sc + 0;
+  l.nu = p.nu;
+  l.coef = sc + 10;
+  launch_epilogue_lb(a, l, n, ctx->stream);
+  CHECK_LAUNCH();
+  launch_sum_rows(P<double>(ctx->b_statp), epi_blocks(n), kStatWidth, 1, kStatWidth, ctx->h_stat_dev, ctx->stream);
+  CHECK_LAUNCH();
+  HIPC(hipStreamSynchronize(ctx->stream));
+  prof_collect(ctx);
+  return COLLOP_OK;
+}
+
 // The LB predictor (main.jl:233-238): g at v from the state's projection, the
 // multipliers at v, v1 = v + dt v_dot.
 int lb_predictor_dev(collop_ctx* ctx, const collop_step_params& p) {
@@ -1374,7 +1432,8 @@ int defect_solve(collop_ctx* ctx, const collop_step_params& p, collop_step_stats
   // outer Anderson window (extension): Anderson with beta = 1 on v -> T(v), the
   // inner solution, with the epilogue's column bookkeeping
   const int mo = p.dc_outer_m;
-  const bool fused = ctx->mv.nd1 <= 256 && ctx->mv.nd2 <= 256;  // the fused frozen map needs the split solves
+  // the fused frozen map needs the split solves
+  const bool fused = ctx->fused_maps && ctx->mv.nd1 <= 256 && ctx->mv.nd2 <= 256;
   int ohist = 0, oslot = 0;
   bool oprev = false;
   double oATA[kMaxWindow * kMaxWindow] = {0};
@@ -1661,9 +1720,12 @@ int step_once(collop_ctx* ctx, const collop_step_params& p, collop_step_stats* o
   // The best iterate starts as the initial guess (solver.jl:331).
   if (p.solver == COLLOP_SOLVER_ANDERSON && (r = copy_d2d(ctx, ctx->s_best, ctx->s_v1, n2b))) return r;
   const char* trace = std::getenv("COLLOP_TRACE") ? "run" : nullptr;
+  const bool lb_fused = lb && ctx->fused_maps && m.nd1 <= 256 && m.nd2 <= 256;
   while (p.solver == COLLOP_SOLVER_ANDERSON && !an.done) {
     const AndersonPlan pl = an.plan();
-    if (lb) {
+    if (lb_fused) {
+      if ((r = lb_map_fused(ctx, p, pl.col, pl.hist))) return r;
+    } else if (lb) {
       if ((r = lb_map_dev(ctx, p, P<double>(ctx->s_v1), pl.col, pl.hist))) return r;
     } else if ((r = picard_map_dev(ctx, p, pl.kind, P<double>(ctx->s_v1), pl.col, pl.hist))) {
       return r;
@@ -2066,6 +2128,13 @@ int collop_set_pair_mode(collop_ctx* ctx, int mode, int tiles) {
     if ((r = ensure_scratch(ctx, ctx->N))) return r;
     choose_tile(ctx, (int)ctx->N);
   }
+  return COLLOP_OK;
+}
+
+int collop_set_fused_maps(collop_ctx* ctx, int on) {
+  if (!ctx) return COLLOP_ERR_INVALID;
+  if (on != 0 && on != 1) return fail(ctx, COLLOP_ERR_INVALID, "on must be 0 or 1");
+  ctx->fused_maps = on == 1;
   return COLLOP_OK;
 }
 

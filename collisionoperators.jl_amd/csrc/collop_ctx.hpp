@@ -66,6 +66,7 @@ struct collop_ctx {
   int manual_slices = 0;
   bool pair_sym = true;  // symmetric pair sum (each unordered pair once)
   int sym_tiles = 0;     // its waves per block (0: automatic)
+  bool fused_maps = true;  // fused small-kernel maps where the split mass solves apply
   bool profile = false;
   std::vector<std::pair<hipEvent_t, hipEvent_t>> ev_pool;
   size_t ev_used = 0;

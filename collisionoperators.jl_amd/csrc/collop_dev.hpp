@@ -97,6 +97,11 @@ __device__ __forceinline__ void basis3_grad(const double* __restrict__ C, int ce
   }
 }
 
+// a b + c d rounded as fma(a, b, c d).  Sums of two products that more than one
+// kernel forms use it, so that the compiler's choice of which product to fuse
+// cannot differ between those kernels.
+__device__ __forceinline__ double dot2(double a, double b, double c, double d) { return __builtin_fma(a, b, c * d); }
+
 // Deterministic block sum (fixed shuffle tree, then fixed order over waves).
 // All threads of the block must call it; the result is valid in thread 0.
 template <int NT>

@@ -166,6 +166,12 @@ int lb_blocks(int N);
 void launch_lb_rhs(const MeshView& m, const double* x, const double* coeff, const double* w, const double* dv, int N,
                    const double* qpart, int nq, const double* S0, double nu, double* g, double* part, double* coef,
                    double* dot, hipStream_t st);
+// The gather of launch_lb_rhs (coeff given) in one launch with, when cin is
+// given, the quadrature partials qpart (quad_blocks(m)) of S(cin); its
+// reduction, lambda and v_dot then come from launch_epilogue_lb.
+void launch_lb_gather_q(const MeshView& m, const double* x, const double* coeff, const double* cin, int use_logsq,
+                        const double* w, const double* dv, int N, double* g, double* part, double* qpart,
+                        hipStream_t st);
 // Run-record diagnostics of f_s = sum c_k phi_k: out[0] = int max(-f_s, 0) and,
 // when v is given, out[1] = ||f_s - f_p||_2^2 with f_p the markers' cell
 // histogram density (device pointers; part: scatter_blocks(N) x ne1*ne2,
@@ -258,6 +264,25 @@ struct FrozenArgs {
   const double* vmk;    // v_mid of v_k
 };
 void launch_epilogue_frozen(const EpiArgs& a, const FrozenArgs& f, int N, hipStream_t st);
+// The Lenard-Bernstein map's value inside the Anderson epilogue: every block
+// reduces the gather partials (and the entropy partials) in the fixed order of
+// launch_lb_rhs, solves for lambda and the drift multipliers, and forms
+// v_dot = -nu (G_eff + A + B x) of its elements; block 0 also writes coef =
+// (lambda, A1, A2, B, S1).  The same bits as launch_lb_rhs + launch_epilogue.
+struct LbEpiArgs {
+  const double* x;      // v_mid (2N)
+  const double* g;      // clamped log-gradient at v_mid (2N)
+  const double* dv;     // v_in - v (2N)
+  const double* w;
+  const double* part;   // nparts = lb_blocks(N) rows of kLbSums gather partials
+  int nparts;
+  const double* qpart;  // nq partials of S(f_in); nq = 0: no Gonzalez lambda
+  int nq;
+  const double* S0;
+  double nu;
+  double* coef;
+};
+void launch_epilogue_lb(const EpiArgs& a, const LbEpiArgs& l, int N, hipStream_t st);
 // nidx[tg[t]] = t for t < nt (nidx cleared to -1 first)
 void launch_near_index(int nt, const int* tg, int N, int* nidx, hipStream_t st);
 // out = v + dt * (sum over slices of part)  (the Euler predictor, main.jl:245)

@@ -21,6 +21,7 @@
 // solves along direction 1 followed by nd1 along direction 2, all in LDS in a
 // single workgroup.
 #include "collop_launch.hpp"
+#include "lb_dev.hpp"
 
 namespace collop {
 namespace {
@@ -57,8 +58,8 @@ __device__ __forceinline__ void scatter_body(const MeshView& m, unsigned long lo
           x1 -= upd->dG[j * cs + a] * upd->g[j];
           x2 -= upd->dG[j * cs + ld + a] * upd->g[j];
         }
-        b1 = upd->beta * x1 + (1.0 - upd->beta) * b1;
-        b2 = upd->beta * x2 + (1.0 - upd->beta) * b2;
+        b1 = dot2(upd->beta, x1, 1.0 - upd->beta, b1);
+        b2 = dot2(upd->beta, x2, 1.0 - upd->beta, b2);
         if (field == 0) {
           vin_out[a] = b1;
           vin_out[ld + a] = b2;
@@ -511,6 +512,26 @@ __global__ __launch_bounds__(256) void k_quad_r(MeshView m, const double* __rest
   quad1_body(m, red, c1, use_logsq, nullptr, spart + lb, lb);
 }
 
+// The Lenard-Bernstein gather of the Anderson map (lb_gather_body over nb
+// blocks), then, when cin is given, quad_blocks(m) blocks of quadrature pass 1
+// over cin for the partials qpart of the Gonzalez entropy S(f_in).
+__global__ __launch_bounds__(kLbThreads) void k_lb_gather_q(MeshView m, const double* __restrict__ x,
+                                                            const double* __restrict__ coeff,
+                                                            const double* __restrict__ cin, int use_logsq,
+                                                            const double* __restrict__ w,
+                                                            const double* __restrict__ dv, int N, int nb,
+                                                            double* __restrict__ g, double* __restrict__ part,
+                                                            double* __restrict__ qpart) {
+  __shared__ double sb1[kMaxBp], sb2[kMaxBp];
+  __shared__ double red[kLbSums * kLbThreads / 64];
+  if ((int)blockIdx.x < nb) {
+    lb_gather_body(m, x, coeff, w, dv, N, 1e-30, 100.0, g, part, sb1, sb2, red, blockIdx.x, nb);
+    return;
+  }
+  const int lb = blockIdx.x - nb;
+  quad1_body(m, red, cin, use_logsq, nullptr, qpart + lb, lb);
+}
+
 __global__ __launch_bounds__(256) void k_quad2(MeshView m, const double* __restrict__ iota, double* __restrict__ r) {
   quad2_wave(m, iota, r, blockIdx.x * 4 + (threadIdx.x >> 6), threadIdx.x & 63);
 }
@@ -735,6 +756,15 @@ void launch_rhs(const MeshView& m, const long long* part, int nblocks, double w_
 int quad_blocks(const MeshView& m) {
   const int tot = m.ne1 * m.ne2 * m.nq * m.nq;
   return (tot + kQuadThreads - 1) / kQuadThreads;
+}
+
+void launch_lb_gather_q(const MeshView& m, const double* x, const double* coeff, const double* cin, int use_logsq,
+                        const double* w, const double* dv, int N, double* g, double* part, double* qpart,
+                        hipStream_t st) {
+  static_assert(kQuadThreads == kLbThreads, "one block size for both halves of k_lb_gather_q");
+  const int nb = lb_blocks(N);
+  hipLaunchKernelGGL(k_lb_gather_q, dim3(nb + (cin ? quad_blocks(m) : 0)), dim3(kLbThreads), 0, st, m, x, coeff, cin,
+                     use_logsq, w, dv, N, nb, g, part, qpart);
 }
 
 void launch_quad1(const MeshView& m, const double* c0, const double* c1, int fields, int use_logsq, double* iota,

@@ -4,6 +4,7 @@
 // All reductions are deterministic: per-block partials in a fixed shuffle tree,
 // then a fixed-order sum over blocks.
 #include "collop_launch.hpp"
+#include "lb_dev.hpp"
 
 namespace collop {
 namespace {
@@ -54,8 +55,8 @@ __device__ __forceinline__ void geff_body(const double* __restrict__ vmid, const
   if (D) {  // frozen-metric product A (G_eff - G_k) of the defect-correction inner map (solver.jl:598-607)
     const double d1 = g1 - Gk[i];
     const double d2 = g2 - Gk[ld + i];
-    D[i] = A[i] * d1 + A[ld + i] * d2;
-    D[ld + i] = A[ld + i] * d1 + A[2 * (size_t)ld + i] * d2;
+    D[i] = dot2(A[i], d1, A[ld + i], d2);
+    D[ld + i] = dot2(A[ld + i], d1, A[2 * (size_t)ld + i], d2);
   }
   const double a = vmid[i], b = vmid[ld + i];
   const bool in = in_box(a, b, lo1, hi1, lo2, hi2);
@@ -145,7 +146,7 @@ __device__ __forceinline__ double update_value(const double* __restrict__ Gv, co
                                                size_t cs, const double* g, int hist, double beta, double v1) {
   double x = Gv[e];
   for (int j = 0; j < hist; ++j) x -= dG[j * cs + e] * g[j];
-  return beta * x + (1.0 - beta) * v1;
+  return dot2(beta, x, 1.0 - beta, v1);
 }
 
 // v_out = v + dt * (slice sum) for flattened element i < 2n  (main.jl:245)
@@ -164,8 +165,8 @@ __device__ __forceinline__ void euler_body(const double* __restrict__ part, int 
 // ---- near field of the frozen-metric map ---------------------------------------
 // w_a U(d) g, U(d) g = (g - d (d.g) / |d|^2) / |d|  (functions.jl:221-231 as written)
 __device__ __forceinline__ void uterm(double wa, double d1, double d2, double g1, double g2, double& o1, double& o2) {
-  const double q = d1 * d1 + d2 * d2;
-  const double dg = (d1 * g1 + d2 * g2) / q;
+  const double q = dot2(d1, d1, d2, d2);
+  const double dg = dot2(d1, g1, d2, g2) / q;
   const double inv = 1.0 / sqrt(q);
   o1 = wa * (g1 - d1 * dg) * inv;
   o2 = wa * (g2 - d2 * dg) * inv;
@@ -211,7 +212,7 @@ __device__ __forceinline__ double frozen_value(const FrozenArgs& F, double lam, 
   const double g1 = geff_of(F, lam, a, N, g2);
   const double d1 = g1 - F.Gk[a];
   const double d2 = g2 - F.Gk[N + a];
-  double D = i < N ? F.A[a] * d1 + F.A[N + a] * d2 : F.A[N + a] * d1 + F.A[2 * (size_t)N + a] * d2;
+  double D = i < N ? dot2(F.A[a], d1, F.A[N + a], d2) : dot2(F.A[N + a], d1, F.A[2 * (size_t)N + a], d2);
   if (F.nidx) {
     const int t = F.nidx[a];
     if (t >= 0) {
@@ -252,6 +253,24 @@ __global__ __launch_bounds__(kEwThreads) void k_epilogue_frozen(EpiArgs A, Froze
   const double lam = lam_sh;
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   const double s = i < 2 * N ? frozen_value(F, lam, i, N) : 0.0;
+  EpiArgs a = A;
+  a.stat_part = A.stat_part + (size_t)blockIdx.x * kStatWidth;
+  epilogue_body(a, N, N, (size_t)2 * N, red, blockIdx.x, &s);
+}
+
+// The Lenard-Bernstein map's epilogue: the reduction, lambda and the drift
+// multipliers, recomputed by every block in launch_lb_rhs's fixed order, then
+// v_dot of element i and the Anderson epilogue of v0 + dt v_dot.
+__global__ __launch_bounds__(kEwThreads) void k_epilogue_lb(EpiArgs A, LbEpiArgs L, int N) {
+  static_assert(kEwThreads == kLbThreads, "lb_coef_body reduces over kLbThreads threads");
+  __shared__ double red[kStatWidth * kEwThreads / 64];
+  static_assert(kStatWidth >= kLbSums + 1, "red also holds the LB reduction");
+  __shared__ double coef[5];
+  lb_coef_body(L.part, L.nparts, L.qpart, L.nq, L.S0, red, coef);
+  __syncthreads();
+  if (blockIdx.x == 0 && threadIdx.x < 5) L.coef[threadIdx.x] = coef[threadIdx.x];
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  const double s = i < 2 * N ? lb_dot_value(L.x, L.g, L.dv, L.w, coef, L.nu, N, i) : 0.0;
   EpiArgs a = A;
   a.stat_part = A.stat_part + (size_t)blockIdx.x * kStatWidth;
   epilogue_body(a, N, N, (size_t)2 * N, red, blockIdx.x, &s);
@@ -407,6 +426,10 @@ void launch_epilogue(const EpiArgs& a, int N, hipStream_t st) {
 
 void launch_epilogue_frozen(const EpiArgs& a, const FrozenArgs& f, int N, hipStream_t st) {
   hipLaunchKernelGGL(k_epilogue_frozen, dim3(epi_blocks(N)), dim3(kEwThreads), 0, st, a, f, N);
+}
+
+void launch_epilogue_lb(const EpiArgs& a, const LbEpiArgs& l, int N, hipStream_t st) {
+  hipLaunchKernelGGL(k_epilogue_lb, dim3(epi_blocks(N)), dim3(kEwThreads), 0, st, a, l, N);
 }
 
 void launch_near_index(int nt, const int* tg, int N, int* nidx, hipStream_t st) {

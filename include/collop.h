@@ -356,6 +356,12 @@ typedef enum {
  * slots and falls back to the ordered sum beyond that.  No counterpart in the
  * reference. */
 int collop_set_pair_mode(collop_ctx* ctx, int mode, int tiles);
+/* Fused (on = 1, the default) or launch-per-stage (0) small kernels in the
+ * frozen-metric inner maps of defect correction and in the Lenard-Bernstein
+ * Anderson maps.  Both compute the same bits.  The fused maps need at most 256
+ * basis functions per direction (larger meshes always run the staged maps).
+ * No counterpart in the reference. */
+int collop_set_fused_maps(collop_ctx* ctx, int on);
 /* The form (mode), waves per block (tiles, 0 for the ordered sum) and partial
  * slots a pair sum over N markers runs with under the current settings; each
  * output pointer may be NULL. */
