@@ -53,6 +53,9 @@ struct collop_ctx {
   int near_nt = 0;              // near targets of the current step
   double* h_stat = nullptr;  // pinned, kStatWidth doubles
   double* h_stat_dev = nullptr;  // its device-side address: the statistics reduction writes there
+  unsigned* h_seq = nullptr;      // pinned: sequence number published after each h_stat write
+  unsigned* h_seq_dev = nullptr;  // its device-side address
+  unsigned seq = 0;               // the last sequence number launched
   // ---- warm-start MLP
   DBuf m_params, m_adam_m, m_adam_v, m_data, m_rec, m_y, m_gpart, m_lpart, m_tiles, m_eval, m_grad;
   bool m_has_params = false, m_has_norm = false;

@@ -236,6 +236,10 @@ struct EpiArgs {
 int epi_blocks(int N);
 constexpr int kStatWidth = 3 + 2 * kMaxWindow;
 void launch_epilogue(const EpiArgs& a, int N, hipStream_t st);
+// The statistics of n rows of epilogue partials (launch_sum_rows' sums and
+// order) into the pinned host buffer out (device address), then seq into
+// *seq_out (pinned, device address) once they are visible to the host.
+void launch_stat_publish(const double* part, int n, double* out, unsigned* seq_out, unsigned seq, hipStream_t st);
 // v1 = beta (Gv - sum_j dG_j gamma_j) + (1 - beta) v1
 void launch_update(double* v1, const double* Gv, const double* dG, const double* gam, int hist, double beta, int N,
                    hipStream_t st);

@@ -276,6 +276,28 @@ __global__ __launch_bounds__(kEwThreads) void k_epilogue_lb(EpiArgs A, LbEpiArgs
   epilogue_body(a, N, N, (size_t)2 * N, red, blockIdx.x, &s);
 }
 
+// The Anderson statistics of the epilogue partials part[n][kStatWidth]: the sums
+// of launch_sum_rows (the same partition over threads, the same tree), written
+// by thread 0 to the pinned host buffer out, then seq to *seq_out with
+// system-scope release, so a host that reads seq also reads the statistics.
+__global__ __launch_bounds__(kEwThreads) void k_stat_publish(const double* __restrict__ part, int n, double* out,
+                                                             unsigned* seq_out, unsigned seq) {
+  __shared__ double red[kStatWidth * kEwThreads / 64];
+  double s[kStatWidth];
+#pragma unroll
+  for (int c = 0; c < kStatWidth; ++c) s[c] = 0.0;
+  for (int j = threadIdx.x; j < n; j += kEwThreads)
+#pragma unroll
+    for (int c = 0; c < kStatWidth; ++c) s[c] += part[(size_t)j * kStatWidth + c];
+  block_sum_n<kEwThreads, kStatWidth>(s, red);
+  if (threadIdx.x == 0) {
+#pragma unroll
+    for (int c = 0; c < kStatWidth; ++c) out[c] = s[c];
+    __threadfence_system();
+    __hip_atomic_store(seq_out, seq, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_SYSTEM);
+  }
+}
+
 __global__ void k_near_index(int nt, const int* __restrict__ tg, int* __restrict__ nidx) {
   const int t = blockIdx.x * blockDim.x + threadIdx.x;
   if (t < nt) nidx[tg[t]] = t;
@@ -430,6 +452,11 @@ void launch_epilogue_frozen(const EpiArgs& a, const FrozenArgs& f, int N, hipStr
 
 void launch_epilogue_lb(const EpiArgs& a, const LbEpiArgs& l, int N, hipStream_t st) {
   hipLaunchKernelGGL(k_epilogue_lb, dim3(epi_blocks(N)), dim3(kEwThreads), 0, st, a, l, N);
+}
+
+void launch_stat_publish(const double* part, int n, double* out, unsigned* seq_out, unsigned seq, hipStream_t st) {
+  static_assert(kEwThreads == 256, "the partition of launch_sum_rows (256 threads)");
+  hipLaunchKernelGGL(k_stat_publish, dim3(1), dim3(kEwThreads), 0, st, part, n, out, seq_out, seq);
 }
 
 void launch_near_index(int nt, const int* tg, int N, int* nidx, hipStream_t st) {
