@@ -613,7 +613,14 @@ def main():
         cpu = cpu_baseline(p, v, w, r64["mean_iters"] + 1.0)
 
     ratio = rch["steps_per_s"] / r64["steps_per_s"]
-    dv = float(np.max(np.abs(rch["markers"] - r64["markers"])))
+    dvm = np.max(np.abs(rch["markers"] - r64["markers"]), axis=1)
+    dv = float(np.max(dvm))
+    dv_note = {
+        "markers_differing_above_1e-6": float(np.mean(dvm > 1e-6)),
+        "relative_l2": float(np.linalg.norm(rch["markers"] - r64["markers"]) / np.linalg.norm(r64["markers"])),
+        "why": "at dt 0.02 the implicit systems have several roots around close pairs; both paths end every "
+               "step on a root within the FP64 tolerance, not always the same one",
+    }
     out = {
         "metric": "Landau implicit steps/s at N=65536, chosen vs plain FP64 GPU, |ΔE|/E≤1e-10",
         "value": round(rch["steps_per_s"], 4),
@@ -665,6 +672,7 @@ def main():
                                     "min_dS": rch["min_dS"]}},
         "all_converged": r64["all_converged"] and rch["all_converged"],
         "chosen_vs_fp64_max_abs_dv": dv,
+        "chosen_vs_fp64_dv": dv_note,
         "pair_interactions_per_s": round(pairs / (r64["pair_ms"][0] / max(r64["pair_launches"][0], 1) * 1e-3), 1),
         "roofline": roofline,
         "cpu_baseline": cpu,
