@@ -619,7 +619,8 @@ def main():
         "markers_differing_above_1e-6": float(np.mean(dvm > 1e-6)),
         "relative_l2": float(np.linalg.norm(rch["markers"] - r64["markers"]) / np.linalg.norm(r64["markers"])),
         "why": "at dt 0.02 the implicit systems have several roots around close pairs; both paths end every "
-               "step on a root within the FP64 tolerance, not always the same one",
+               "step on a root within the FP64 tolerance, not always the same one, and a different root at a "
+               "close pair moves every other marker slightly through the projected entropy gradient",
     }
     out = {
         "metric": "Landau implicit steps/s at N=65536, chosen vs plain FP64 GPU, |ΔE|/E≤1e-10",
