@@ -81,6 +81,43 @@ def test_lb_step_matches_oracle(gpu_ctx, N, dt):
         Sr = m.entropy(fr)
 
 
+@pytest.mark.parametrize("gonzalez,logsq,max_iter", [(0, 0, 4), (1, 1, 2000)], ids=["no_gonzalez", "logsq"])
+def test_lb_step_variants_match_oracle(gpu_ctx, gonzalez, logsq, max_iter):
+    """The LB map without the Gonzalez correction (plain midpoint G = g(v_mid)),
+    and with the 1/2 log f^2 entropy (use_logsq), against the oracle's step.
+    Without Gonzalez the oracle's own Anderson solve of this IC stagnates at
+    residuals near 1e-2 (N = 800 to 16000), so that case compares the first
+    four maps (predictor, three Anderson updates) through the iteration cap."""
+    p = collop.config("lb_bimodal_n40000", N_PARTICLES=800, DT=0.001)
+    v, w = collop.initial_markers(p)
+    m = O.Mesh(p.bp1, p.bp2)
+    fc = m.l2_project(v, w)
+    S0 = m.entropy(fc, use_logsq=bool(logsq))
+    P = O.default_params(0.001, m=8, nthreads=0, collision_model=1, nu=1.0, use_gonzalez=gonzalez,
+                         use_logsq=logsq, max_iter=max_iter)
+    ws = collop.Workspace(p.bp1, p.bp2, ctx=gpu_ctx)
+    sim = collop.Simulation(ws, v, w)
+    sp = p.step_params()
+    sp.use_gonzalez = gonzalez
+    sp.use_logsq = logsq
+    sp.max_iter = max_iter
+    vr, fr, st_ref = m.time_step(P, v, w, fc, S0)
+    st = sim.run(sp, 1)[0]
+    vg = sim.markers()
+    dmax = float(np.max(np.abs(vg - vr)))
+    print(f"LB gonzalez={gonzalez} logsq={logsq}: oracle iters {st_ref.iters}, gpu iters {st['iters']}, "
+          f"max|dv| {dmax:.2e}")
+    assert st["exit_reason"] == st_ref.exit_reason
+    if max_iter < 2000:  # the same maps, cut at the cap
+        assert st["iters"] == st_ref.iters == max_iter
+        assert dmax <= 1e-11
+    else:
+        assert st["exit_reason"] == 0
+        assert abs(st["iters"] - st_ref.iters) <= 3
+        assert dmax <= 1e-9
+    assert relerr(sim.coeffs(), fr) <= 1e-9
+
+
 def test_lb_param_errors(gpu_ctx):
     p = collop.config("lb_bimodal_n40000", N_PARTICLES=600)
     v, w = collop.initial_markers(p)
