@@ -111,6 +111,40 @@ def test_f32c_reaches_fp64_solution(gpu_ctx, dt, strict):
     assert _oracle_residual(m, P, vb, v, w, S0) <= 20 * max(1e-12 * np.linalg.norm(vb), 1e-10)
 
 
+@pytest.mark.parametrize("kw", [dict(use_logsq=1), dict(exit_picard_step=0), dict(use_gonzalez=0),
+                                dict(use_anderson=0), dict(m_anderson=2)],
+                         ids=["logsq", "no_exit_picard", "no_gonzalez", "damped_picard", "window2"])
+def test_step_switches_match_oracle(gpu_ctx, kw):
+    """The solver switches of SimParameters (Parameters.jl:21-77): the 1/2 log f^2
+    entropy, returning v rather than G(v) at convergence, the plain midpoint G
+    without the Gonzalez correction, damped Picard without Anderson, and a
+    window of 2; each against the oracle's step on a well-conditioned case."""
+    v, w = collop.markers("gaussian", 1024, 42, sigma1=1.0, sigma2=1.0, peak=2.0)
+    bp = uniform_bp(-6, 6, 16)
+    m = O.Mesh(bp, bp)
+    fc = m.l2_project(v, w)
+    kw = dict(kw)
+    mw = kw.pop("m_anderson", 5)
+    logsq = kw.get("use_logsq", 0)
+    P = O.default_params(0.005, m=mw, nthreads=0, **kw)
+    v_ref, fc_ref, st_ref = m.time_step(P, v, w, fc, m.entropy(fc, use_logsq=bool(logsq)))
+    ws = collop.Workspace(bp, bp, ctx=gpu_ctx)
+    sim = collop.Simulation(ws, v, w)
+    sp = _params(0.005)
+    sp.m_anderson = mw
+    for k, val in kw.items():
+        setattr(sp, k, val)
+    st = sim.run(sp, 1)[0]
+    v_gpu = sim.markers()
+    dmax = float(np.max(np.abs(v_gpu - v_ref)))
+    print(f"{kw} m={mw}: oracle iters {st_ref.iters}, gpu iters {st['iters']}, max|dv| {dmax:.2e}")
+    assert st["exit_reason"] == 0 and st_ref.exit_reason == 0
+    assert abs(st["iters"] - st_ref.iters) <= 2
+    assert dmax <= 1e-9
+    assert abs(st["entropy"] - st_ref.entropy) <= 1e-10
+    assert relerr(sim.coeffs(), fc_ref) <= 1e-9
+
+
 def test_step_deterministic(gpu_ctx):
     p = collop.config("gaussian_n4096", N_PARTICLES=2048)
     v, w = collop.initial_markers(p)
