@@ -145,6 +145,29 @@ def test_step_switches_match_oracle(gpu_ctx, kw):
     assert relerr(sim.coeffs(), fc_ref) <= 1e-9
 
 
+def test_step_with_markers_outside_the_box(gpu_ctx):
+    """2.4 % of the markers lie outside the open box [-2.5, 2.5]^2: they get no
+    projection weight and v_dot = 0 (functions.jl:205-237), so the step leaves
+    them exactly where they are; the others match the oracle's step."""
+    v, w = collop.markers("gaussian", 1024, 42, sigma1=1.0, sigma2=1.0, peak=2.0)
+    bp = uniform_bp(-2.5, 2.5, 12)
+    outside = (np.abs(v) >= 2.5).any(axis=1)
+    assert outside.sum() >= 20
+    m = O.Mesh(bp, bp)
+    fc = m.l2_project(v, w)
+    v_ref, fc_ref, st_ref = m.time_step(O.default_params(0.005, m=5, nthreads=0), v, w, fc, m.entropy(fc))
+    ws = collop.Workspace(bp, bp, ctx=gpu_ctx)
+    sim = collop.Simulation(ws, v, w)
+    st = sim.run(_params(0.005), 1)[0]
+    v_gpu = sim.markers()
+    print(f"outside {outside.sum()}: oracle iters {st_ref.iters}, gpu iters {st['iters']}, "
+          f"max|dv| {np.max(np.abs(v_gpu - v_ref)):.2e}")
+    assert st["exit_reason"] == 0 and abs(st["iters"] - st_ref.iters) <= 2
+    assert np.array_equal(v_gpu[outside], v[outside])
+    assert np.max(np.abs(v_gpu - v_ref)) <= 1e-9
+    assert relerr(sim.coeffs(), fc_ref) <= 1e-9
+
+
 def test_step_deterministic(gpu_ctx):
     p = collop.config("gaussian_n4096", N_PARTICLES=2048)
     v, w = collop.initial_markers(p)
