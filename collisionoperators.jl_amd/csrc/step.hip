@@ -277,22 +277,45 @@ __global__ __launch_bounds__(kEwThreads) void k_epilogue_lb(EpiArgs A, LbEpiArgs
 }
 
 // The Anderson statistics of the epilogue partials part[n][kStatWidth]: the sums
-// of launch_sum_rows (the same partition over threads, the same tree), written
-// by thread 0 to the pinned host buffer out, then seq to *seq_out with
-// system-scope release, so a host that reads seq also reads the statistics.
-__global__ __launch_bounds__(kEwThreads) void k_stat_publish(const double* __restrict__ part, int n, double* out,
-                                                             unsigned* seq_out, unsigned seq) {
-  __shared__ double red[kStatWidth * kEwThreads / 64];
-  double s[kStatWidth];
+// of launch_sum_rows, written by thread 0 to the pinned host buffer out, then
+// seq to *seq_out with system-scope release, so a host that reads seq also
+// reads the statistics.  Four groups of 256 threads share the columns (group
+// g takes columns g, g + 4, ...); within a group, thread t sums rows t, t + 256,
+// ... and the four waves reduce by the shuffle tree of block_sum, then in wave
+// order -- the partition and tree of launch_sum_rows' 256-thread blocks, so the
+// sums are bitwise the same.
+constexpr int kPubGroups = 4;
+__global__ __launch_bounds__(kPubGroups * 256) void k_stat_publish(const double* __restrict__ part, int n,
+                                                                   double* out, unsigned* seq_out, unsigned seq) {
+  constexpr int kPer = (kStatWidth + kPubGroups - 1) / kPubGroups;
+  __shared__ double wsum[kStatWidth][4];
+  const int g = threadIdx.x >> 8, t = threadIdx.x & 255;
+  const int wave = t >> 6, lane = t & 63;
+  double s[kPer];
 #pragma unroll
-  for (int c = 0; c < kStatWidth; ++c) s[c] = 0.0;
-  for (int j = threadIdx.x; j < n; j += kEwThreads)
+  for (int q = 0; q < kPer; ++q) s[q] = 0.0;
+  for (int j = t; j < n; j += 256)
 #pragma unroll
-    for (int c = 0; c < kStatWidth; ++c) s[c] += part[(size_t)j * kStatWidth + c];
-  block_sum_n<kEwThreads, kStatWidth>(s, red);
+    for (int q = 0; q < kPer; ++q) {
+      const int c = g + q * kPubGroups;
+      if (c < kStatWidth) s[q] += part[(size_t)j * kStatWidth + c];
+    }
+#pragma unroll
+  for (int q = 0; q < kPer; ++q) {
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) s[q] += __shfl_xor(s[q], off, 64);
+    const int c = g + q * kPubGroups;
+    if (lane == 0 && c < kStatWidth) wsum[c][wave] = s[q];
+  }
+  __syncthreads();
   if (threadIdx.x == 0) {
 #pragma unroll
-    for (int c = 0; c < kStatWidth; ++c) out[c] = s[c];
+    for (int c = 0; c < kStatWidth; ++c) {
+      double v = 0.0;
+#pragma unroll
+      for (int i = 0; i < 4; ++i) v += wsum[c][i];
+      out[c] = v;
+    }
     __threadfence_system();
     __hip_atomic_store(seq_out, seq, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_SYSTEM);
   }
@@ -455,8 +478,7 @@ void launch_epilogue_lb(const EpiArgs& a, const LbEpiArgs& l, int N, hipStream_t
 }
 
 void launch_stat_publish(const double* part, int n, double* out, unsigned* seq_out, unsigned seq, hipStream_t st) {
-  static_assert(kEwThreads == 256, "the partition of launch_sum_rows (256 threads)");
-  hipLaunchKernelGGL(k_stat_publish, dim3(1), dim3(kEwThreads), 0, st, part, n, out, seq_out, seq);
+  hipLaunchKernelGGL(k_stat_publish, dim3(1), dim3(kPubGroups * 256), 0, st, part, n, out, seq_out, seq);
 }
 
 void launch_near_index(int nt, const int* tg, int N, int* nidx, hipStream_t st) {
