@@ -1751,7 +1751,11 @@ int step_once(collop_ctx* ctx, const collop_step_params& p, collop_step_stats* o
     AndersonAct act;
     if (an.after_map(ctx->h_stat, act, trace) != COLLOP_OK)
       return fail(ctx, COLLOP_ERR_NONFINITE, "non-finite residual in Picard map " + std::to_string(an.maps()));
-    if (act.copy_best && (r = copy_d2d(ctx, ctx->s_best, p.exit_picard_step ? ctx->s_Gv : ctx->s_v1, n2b))) return r;
+    // the best iterate is saved by the update's launch when one follows, else by a copy
+    const bool best_in_update = act.copy_best && act.exit == kContinue && act.update;
+    if (act.copy_best && !best_in_update &&
+        (r = copy_d2d(ctx, ctx->s_best, p.exit_picard_step ? ctx->s_Gv : ctx->s_v1, n2b)))
+      return r;
     if (act.exit == kConverged) {
       if (p.exit_picard_step) std::swap(ctx->s_v1, ctx->s_Gv);
       break;
@@ -1762,7 +1766,8 @@ int step_once(collop_ctx* ctx, const collop_step_params& p, collop_step_stats* o
     }
     if (act.update) {
       launch_update(P<double>(ctx->s_v1), P<double>(ctx->s_Gv), P<double>(ctx->s_dG), act.gamma, act.h, act.beta, n,
-                    ctx->stream);
+                    ctx->stream, best_in_update ? P<double>(p.exit_picard_step ? ctx->s_Gv : ctx->s_v1) : nullptr,
+                    best_in_update ? P<double>(ctx->s_best) : nullptr);
       CHECK_LAUNCH();
     }
     if (act.rotate) {  // rp <- r, Gp <- Gv (pointer rotation instead of copies)

@@ -240,9 +240,10 @@ void launch_epilogue(const EpiArgs& a, int N, hipStream_t st);
 // order) into the pinned host buffer out (device address), then seq into
 // *seq_out (pinned, device address) once they are visible to the host.
 void launch_stat_publish(const double* part, int n, double* out, unsigned* seq_out, unsigned seq, hipStream_t st);
-// v1 = beta (Gv - sum_j dG_j gamma_j) + (1 - beta) v1
+// v1 = beta (Gv - sum_j dG_j gamma_j) + (1 - beta) v1; with best_dst, first
+// best_dst <- best_src (2N values; best_src may be v1 itself)
 void launch_update(double* v1, const double* Gv, const double* dG, const double* gam, int hist, double beta, int N,
-                   hipStream_t st);
+                   hipStream_t st, const double* best_src = nullptr, double* best_dst = nullptr);
 // Near-field correction of the frozen-metric map (pairs in CSR order: targets
 // tg[nt], partners al[off[t] .. off[t+1])): D -= sum (exact - frozen) pair terms.
 void launch_near(int nt, const int* tg, const int* off, const int* al, int N, const double* w, const double* vm,

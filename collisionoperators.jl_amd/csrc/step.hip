@@ -352,11 +352,15 @@ __global__ __launch_bounds__(kEwThreads) void k_epilogue(EpiArgs A, int N) {
   epilogue_body(a, N, N, (size_t)2 * N, red, blockIdx.x);
 }
 
-__global__ void k_update(double* __restrict__ v1, const double* __restrict__ Gv, const double* __restrict__ dG,
-                         Gamma gam, int hist, double beta, int n2) {
+// bdst given: bdst <- bsrc first (the best iterate, saved in the same launch;
+// bsrc may be v1, read before it is updated).
+__global__ void k_update(double* v1, const double* __restrict__ Gv, const double* __restrict__ dG, Gamma gam,
+                         int hist, double beta, int n2, const double* bsrc, double* __restrict__ bdst) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n2) return;
-  v1[i] = update_value(Gv, dG, i, n2, gam.g, hist, beta, v1[i]);
+  const double x = v1[i];
+  if (bdst) bdst[i] = bsrc == v1 ? x : bsrc[i];
+  v1[i] = update_value(Gv, dG, i, n2, gam.g, hist, beta, x);
 }
 
 __global__ void k_euler(const double* __restrict__ part, int S, const double* __restrict__ v, double dt, int N,
@@ -487,11 +491,12 @@ void launch_near_index(int nt, const int* tg, int N, int* nidx, hipStream_t st) 
 }
 
 void launch_update(double* v1, const double* Gv, const double* dG, const double* gam, int hist, double beta, int N,
-                   hipStream_t st) {
+                   hipStream_t st, const double* best_src, double* best_dst) {
   Gamma g{};
   for (int j = 0; j < hist && j < kMaxWindow; ++j) g.g[j] = gam[j];
   const int n2 = 2 * N;
-  hipLaunchKernelGGL(k_update, dim3((n2 + 255) / 256), dim3(256), 0, st, v1, Gv, dG, g, hist, beta, n2);
+  hipLaunchKernelGGL(k_update, dim3((n2 + 255) / 256), dim3(256), 0, st, v1, Gv, dG, g, hist, beta, n2, best_src,
+                     best_dst);
 }
 
 void launch_near(int nt, const int* tg, const int* off, const int* al, int N, const double* w, const double* vm,
