@@ -42,7 +42,6 @@ Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu]
        python bench.py --config ensemble48 --steps 3
        python bench.py --config lb_bimodal_n40000 --steps 100
        python bench.py --kernel-probe        (pair kernels + a 256 MiB copy, for rocprofv3 counters)
-       python bench.py --issue-probe         (issue cost of the pair kernels' instruction kinds)
 """
 from __future__ import annotations
 
@@ -409,22 +408,6 @@ def ensemble_bench(args):
     print(json.dumps(out), flush=True)
 
 
-def issue_probe(args):
-    """Issue cost of the instruction kinds the pair kernels are made of
-    (collop_issue_probe): clocks per wave64 instruction per SIMD at 2.4 GHz."""
-    import collop
-    ctx = collop.Context(0)
-    names = ["v_fma_f32", "v_sub_f32", "v_rsq_f32", "v_mov_b32_dpp", "v_fma_f64", "v_rsq_f64"]
-    out = {}
-    for k, nm in enumerate(names):
-        clk, ms = np.zeros(1), np.zeros(1)
-        ctx.check(ctx.lib.collop_issue_probe(ctx.h, k, 20000, collop._lib.dptr(clk), collop._lib.dptr(ms)))
-        out[nm] = round(float(clk[0]), 3)
-        log(f"  {nm:16s} {clk[0]:6.2f} clocks per wave64 instruction per SIMD ({ms[0]:.3f} ms)")
-    print(json.dumps({"issue_clocks_per_wave64_instruction": out, "clock_GHz_assumed": 2.4,
-                      "waves_per_simd": 8, "chains_per_lane": 16}), flush=True)
-
-
 def lb_bench(args):
     """Lenard-Bernstein workload (parameters_LB_bimodal_v1.jl: N = 40000,
     N(+-2, 1) x N(0, 1), non-uniform 27x24 cells, nu = 1, dt = 0.001, m = 8):
@@ -504,8 +487,6 @@ def main():
     ap.add_argument("--config", default="bimodal_n65536",
                     help="bimodal_n65536 (headline), gaussian_n4096, bimaxwellian_n262144, ensemble48 or "
                          "lb_bimodal_n40000 (Lenard-Bernstein)")
-    ap.add_argument("--issue-probe", action="store_true",
-                    help="issue cost of the VALU instruction kinds of the pair kernels (collop_issue_probe)")
     ap.add_argument("--kernel-probe", action="store_true",
                     help="run only the pair kernels and a 256 MiB copy (rocprofv3 counter runs)")
     ap.add_argument("--train-steps", type=int, default=200, help="warmstart report: recorded steps per run")
@@ -527,9 +508,6 @@ def main():
         return
     if args.kernel_probe:
         kernel_probe(args)
-        return
-    if args.issue_probe:
-        issue_probe(args)
         return
     if args.config == "ensemble48":
         ensemble_bench(args)

@@ -139,7 +139,6 @@ SIGNATURES = [
     ("collop_quadrature", C.c_int, [_vp, _dp, C.c_int, _dp, _dp]),
     ("collop_mass_solve", C.c_int, [_vp, _dp, _dp]),
     ("collop_lb_rhs", C.c_int, [_vp, _dp, _dp, _dp, C.c_int64, C.c_double, _dp, _dp]),
-    ("collop_issue_probe", C.c_int, [_vp, C.c_int, C.c_int, _dp, _dp]),
     ("collop_gmres_dense", C.c_int, [_vp, C.c_int, _dp, _dp, C.c_double, C.c_double, C.c_int, C.c_int, _dp,
                                      C.POINTER(C.c_int), _dp]),
     ("collop_negative_part_l1", C.c_int, [_vp, _dp, _dp]),

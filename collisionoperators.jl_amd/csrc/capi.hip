@@ -795,35 +795,6 @@ int collop_gmres_dense(collop_ctx* ctx, int n, const double* A, const double* b,
   return COLLOP_OK;
 }
 
-int collop_issue_probe(collop_ctx* ctx, int kind, int iters, double* clocks, double* ms) {
-  if (!ctx) return COLLOP_ERR_INVALID;
-  if (kind < 0 || kind > 5 || iters < 1 || iters > 1000000 || !clocks)
-    return fail(ctx, COLLOP_ERR_INVALID, "kind must be in [0, 5] and iters in [1, 1e6]");
-  HIPC(hipSetDevice(ctx->device));
-  hipDeviceProp_t prop;
-  HIPC(hipGetDeviceProperties(&prop, ctx->device));
-  const int ncu = prop.multiProcessorCount;
-  const int blocks = ncu * 8;  // 8 waves per SIMD
-  ENSURE(ctx->nk_A, (size_t)blocks * 256 * 8);
-  hipEvent_t e0, e1;
-  HIPC(hipEventCreate(&e0));
-  HIPC(hipEventCreate(&e1));
-  float best = 1e30f;
-  int r = 0;
-  for (int rep = 0; rep < 2 && r == 0; ++rep) {
-    float t = 0.0f;
-    r = issue_probe(kind, iters, blocks, ctx->nk_A.p, ctx->stream, e0, e1, &t);
-    if (r == 0 && t < best) best = t;
-  }
-  (void)hipEventDestroy(e0);
-  (void)hipEventDestroy(e1);
-  if (r) return fail(ctx, COLLOP_ERR_HIP, std::string("issue probe: ") + hipGetErrorString(hipGetLastError()));
-  const double per_simd = (double)blocks * 4 * iters * issue_probe_chains() / (ncu * 4.0);
-  *clocks = best * 1e-3 * 2.4e9 / per_simd;
-  if (ms) *ms = best;
-  return COLLOP_OK;
-}
-
 int collop_mass_solve(collop_ctx* ctx, const double* b, double* x) {
   int r = need_mesh(ctx);
   if (r) return r;

@@ -150,12 +150,6 @@ void launch_rhs(const MeshView& m, const long long* part, int nblocks, double w_
 // non-null) gets (1 + log f) w_q |e| (0 where f is below the floor) and spart[y]
 // (if non-null) partial sums of -f log f w_q |e|.  coeff[y] points at n_dofs.
 int quad_blocks(const MeshView& m);
-// Issue-cost probe (probe.hip): kind 0 v_fma_f32, 1 v_sub_f32, 2 v_rsq_f32, 3 DPP
-// v_mov_b32, 4 v_fma_f64, 5 v_rsq_f64; `blocks` workgroups of 4 waves, each
-// lane running issue_probe_chains() independent chains for `iters` rounds
-// into out (blocks * 256 doubles).  *ms = the launch's event time.
-int issue_probe(int kind, int iters, int blocks, void* out, hipStream_t st, hipEvent_t e0, hipEvent_t e1, float* ms);
-int issue_probe_chains();
 // Lenard-Bernstein right-hand side at x (device pointers): with coeff, g = the
 // clamped log-gradient of f = sum coeff phi at x (fs_floor 1e-30, g_max 100),
 // else g is given; then the block sums (part: lb_blocks(N) x kLbSums), the

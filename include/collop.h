@@ -122,13 +122,6 @@ int collop_mass_solve(collop_ctx* ctx, const double* b, double* x);
 int collop_gmres_dense(collop_ctx* ctx, int n, const double* A, const double* b, double rtol, double atol, int itmax,
                        int mem, double* x, int* products, double* rnorm);
 
-/* Issue cost of one instruction kind on this GPU, the denominator of the pair
- * kernels' issue-bound estimates (no counterpart in the reference): kind 0
- * v_fma_f32, 1 v_sub_f32, 2 v_rsq_f32, 3 DPP v_mov_b32, 4 v_fma_f64, 5
- * v_rsq_f64.  Independent chains at 8 waves per SIMD; clocks = clocks per wave64
- * instruction per SIMD at 2.4 GHz (best of two launches), ms = that launch's time. */
-int collop_issue_probe(collop_ctx* ctx, int kind, int iters, double* clocks, double* ms);
-
 /* Lenard-Bernstein right-hand side at markers v with log-density gradient g
  * (N x 2, e.g. from collop_loggrad_gather): the moments n, U, Q of (v, w)
  * (compute_moments, functions.jl:66-82), the drift multipliers (A1, A2, B) of
