@@ -84,6 +84,9 @@ class StepParams(C.Structure):
         ("nk_eta_max", C.c_double),
         ("nk_fd_h", C.c_double),
         ("nk_fd_rel", C.c_double),
+        ("cl_theta", C.c_double),
+        ("cl_kmax", C.c_int32),
+        ("cl_pad", C.c_int32),
     ]
 
 
@@ -102,6 +105,8 @@ class StepStats(C.Structure):
         ("momentum", C.c_double * 2),
         ("seconds", C.c_double),
         ("inner_maps", C.c_int32),
+        ("n_clusters", C.c_int32),
+        ("cl_fail", C.c_int32),
         ("pad", C.c_int32),
     ]
 
@@ -111,7 +116,7 @@ class StepStats(C.Structure):
             "exit_reason": self.exit_reason, "residual": self.residual, "r0": self.r0,
             "entropy": self.entropy, "energy": self.energy,
             "momentum": (self.momentum[0], self.momentum[1]), "seconds": self.seconds,
-            "inner_maps": self.inner_maps,
+            "inner_maps": self.inner_maps, "n_clusters": self.n_clusters, "cl_fail": self.cl_fail,
         }
 
 

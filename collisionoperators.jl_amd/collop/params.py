@@ -66,6 +66,8 @@ class SimParameters:
     nk_eta_max: float = 0.9
     nk_fd_h: float = 0.0             # 0 = 1e-6 (FP64 maps)
     nk_fd_rel: float = 0.0
+    cl_theta: float = 0.0            # extension: stiff clusters max(w_i, w_j) / |d| > cl_theta solved exactly (0 = off)
+    cl_kmax: int = 32                # largest stiff cluster (markers)
 
     def step_params(self, precision: int | None = None, warmstart: int | None = None,
                     solver: int | None = None) -> _lib.StepParams:
@@ -91,7 +93,7 @@ class SimParameters:
             dc_outer_m=self.dc_outer_m, dc_near_theta=self.dc_near_theta, dc_metric_reuse=self.dc_metric_reuse,
             collision_model=_lib.MODEL_LB if self.collision_model == "lb" else _lib.MODEL_LANDAU, nu=self.nu,
             dc_eta_f64=self.dc_eta_f64, nk_krylov_max=self.nk_krylov_max, nk_eta_max=self.nk_eta_max, nk_fd_h=self.nk_fd_h,
-            nk_fd_rel=self.nk_fd_rel)
+            nk_fd_rel=self.nk_fd_rel, cl_theta=self.cl_theta, cl_kmax=self.cl_kmax)
 
 
 def parse_overrides(p: SimParameters, args: list[str]) -> SimParameters:

@@ -46,8 +46,11 @@ struct AndersonAct {
 
 class AndersonRun {
  public:
-  void start(const collop_step_params& p) {
+  // clusters: the maps carry stiff-cluster rows (statistic kStatTrue); the
+  // FP32 window is then not kept across the switch (its maps had none)
+  void start(const collop_step_params& p, bool clusters = false) {
     p_ = p;
+    clusters_ = clusters;
     mw_ = p.m_anderson;
     history_ = slot_ = n_restart = 0;
     nrm_r0 = 0.0;
@@ -86,7 +89,10 @@ class AndersonRun {
     const double nrm_r = std::sqrt(st[0]);
     const double nrm_v1 = std::sqrt(st[1]);
     if (!std::isfinite(nrm_r) || !std::isfinite(nrm_v1)) return COLLOP_ERR_NONFINITE;
-    if (k == 1) nrm_r0 = nrm_r;
+    // the plain map's residual: the exit, best-iterate and stagnation tests
+    // (with stiff clusters, nrm_r is that of the map Anderson mixes)
+    const double nrm_map = std::sqrt(std::max(st[0] + st[kStatTrue], 0.0));
+    if (k == 1) nrm_r0 = nrm_map;
     const double eff_tol = std::max(p_.tol * (nrm_v1 + 1e-30), p_.abs_floor);
     if (use32_) {
       iters32++;
@@ -108,7 +114,7 @@ class AndersonRun {
         std::fprintf(stderr, "  [%s f32] k=%d |r|=%.3e floor=%.3e best=%.3e\n", trace_tag, k, nrm_r, floor32,
                      best32_);
       if (switch_now && k >= p_.max_iter) return finish_max_iter(a);
-      if (switch_now && p_.f32_keep_window && p_.use_anderson && history_ > 0) {
+      if (switch_now && p_.f32_keep_window && !clusters_ && p_.use_anderson && history_ > 0) {
         // Keep the secant window built from FP32 maps (its columns are
         // differences far above FP32 rounding) and re-evaluate this iterate
         // with an FP64 map before the next update: no column mixes the two.
@@ -128,16 +134,17 @@ class AndersonRun {
         return COLLOP_OK;
       }
     } else {
-      if (trace_tag) std::fprintf(stderr, "  [%s f64] k=%d |r|=%.3e tol=%.3e\n", trace_tag, k, nrm_r, eff_tol);
-      if (nrm_r < nrm_best_) {
-        nrm_best_ = nrm_r;
+      if (trace_tag)
+        std::fprintf(stderr, "  [%s f64] k=%d |r|=%.3e (mixed %.3e) tol=%.3e\n", trace_tag, k, nrm_map, nrm_r, eff_tol);
+      if (nrm_map < nrm_best_) {
+        nrm_best_ = nrm_map;
         a.copy_best = true;
       }
-      if (nrm_r < eff_tol) {
+      if (nrm_map < eff_tol) {
         a.exit = kConverged;
         exit_reason = 0;
         k_done = k;
-        res_out = nrm_r;
+        res_out = nrm_map;
         done = true;
         return COLLOP_OK;
       }
@@ -219,7 +226,7 @@ class AndersonRun {
   int mw_ = 1, history_ = 0, slot_ = 0, since_reset_ = 0, since_best32_ = 0, k_ = 0;
   double nrm_best_ = 0.0, nrm_best_window_ = 0.0, best32_ = 0.0, floor_factor_ = 16.0;
   double ATA_[kMaxWindow * kMaxWindow] = {0};
-  bool use32_ = false, rebase_ = false;
+  bool use32_ = false, rebase_ = false, clusters_ = false;
   AndersonPlan plan_{0, -1, 0};
 };
 

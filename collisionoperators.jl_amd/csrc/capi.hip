@@ -370,7 +370,10 @@ int collop_destroy(collop_ctx* ctx) {
                   &ctx->s_fc, &ctx->s_dotv, &ctx->s_Gvo, &ctx->s_Gk, &ctx->s_FA, &ctx->s_D, &ctx->s_Ge, &ctx->s_vmk, &ctx->s_xk, &ctx->s_oU,
                   &ctx->s_oUp, &ctx->s_of, &ctx->s_ofp, &ctx->s_odF, &ctx->s_odG, &ctx->b_near_list, &ctx->b_near_count,
                   &ctx->b_ntg, &ctx->b_noff, &ctx->b_nal, &ctx->b_nidx, &ctx->s_v1n, &ctx->m_params, &ctx->m_adam_m, &ctx->m_adam_v, &ctx->m_data, &ctx->m_rec,
-                  &ctx->m_y, &ctx->m_gpart, &ctx->m_lpart, &ctx->m_tiles, &ctx->m_eval, &ctx->m_grad})
+                  &ctx->m_y, &ctx->m_gpart, &ctx->m_lpart, &ctx->m_tiles, &ctx->m_eval, &ctx->m_grad,
+                  &ctx->b_cell_of, &ctx->b_cell_cnt, &ctx->b_cell_start, &ctx->b_cell_sorted, &ctx->b_links,
+                  &ctx->b_link_count, &ctx->b_cl_off, &ctx->b_cl_idx, &ctx->b_cl_row, &ctx->b_cl_x, &ctx->b_cl_true,
+                  &ctx->b_cl_root, &ctx->b_cl_have, &ctx->b_cl_fail, &ctx->b_cl_its})
     release(*b);
   for (auto& e : ctx->ev_pool) {
     (void)hipEventDestroy(e.first);
@@ -923,8 +926,13 @@ int epilogue_run(collop_ctx* ctx, const EpiArgs& a) {
 }
 
 int epilogue_dev(collop_ctx* ctx, const double* part, int S, const double* v0e, double dte, const double* vin,
-                 double* Gv, int col, int hist) {
+                 double* Gv, int col, int hist, bool clusters = false) {
   EpiArgs a{};
+  if (clusters) {
+    a.cl_row = P<int>(ctx->b_cl_row);
+    a.cl_x = P<double>(ctx->b_cl_x);
+    a.cl_true = P<double>(ctx->b_cl_true);
+  }
   a.part = part;
   a.S = S;
   a.v0 = v0e;
@@ -970,15 +978,128 @@ int build_near(collop_ctx* ctx) {
   return COLLOP_OK;
 }
 
+// Stiff clusters of the step (extension): every pair of the state's markers
+// with max(w_i, w_j) / |d| > theta, found cell by cell on the device, joined
+// closest pair first unless the cluster would exceed kmax markers; clusters of
+// two or more markers in CSR, members ascending, ordered by first member.  Too
+// many links or members for the buffers: no clusters this step.
+int build_clusters_dev(collop_ctx* ctx, double theta, int kmax) {
+  const MeshView& m = ctx->mv;
+  const int n = (int)ctx->N;
+  if (ctx->cl_total > 0) {
+    launch_cluster_rows(P<int>(ctx->b_cl_idx), ctx->cl_total, false, P<int>(ctx->b_cl_row), ctx->stream);
+    CHECK_LAUNCH();
+  }
+  ctx->ncl = ctx->cl_total = 0;
+  if (!(theta > 0.0)) return COLLOP_OK;
+  launch_cell_sort(m, P<double>(ctx->s_v), n, P<int>(ctx->b_cell_of), P<int>(ctx->b_cell_cnt),
+                   P<int>(ctx->b_cell_start), P<int>(ctx->b_cell_sorted), ctx->stream);
+  CHECK_LAUNCH();
+  launch_near_cells(m, P<double>(ctx->s_v), P<double>(ctx->s_w), n, theta, ctx->w_max, P<int>(ctx->b_cell_start),
+                    P<int>(ctx->b_cell_sorted), P<NearLink>(ctx->b_links), P<int>(ctx->b_link_count), kLinkCap,
+                    ctx->stream);
+  CHECK_LAUNCH();
+  int cnt = 0;
+  HIPC(hipMemcpyAsync(&cnt, ctx->b_link_count.p, sizeof(int), hipMemcpyDeviceToHost, ctx->stream));
+  HIPC(hipStreamSynchronize(ctx->stream));
+  if (cnt <= 0 || cnt > kLinkCap) return COLLOP_OK;
+  std::vector<NearLink> L(cnt);
+  HIPC(hipMemcpy(L.data(), ctx->b_links.p, (size_t)cnt * sizeof(NearLink), hipMemcpyDeviceToHost));
+  std::sort(L.begin(), L.end(), [](const NearLink& a, const NearLink& b) {
+    return a.q != b.q ? a.q < b.q : (a.i != b.i ? a.i < b.i : a.j < b.j);
+  });
+  std::vector<int> nodes;
+  nodes.reserve(2 * cnt);
+  for (const NearLink& l : L) {
+    nodes.push_back(l.i);
+    nodes.push_back(l.j);
+  }
+  std::sort(nodes.begin(), nodes.end());
+  nodes.erase(std::unique(nodes.begin(), nodes.end()), nodes.end());
+  auto id = [&](int a) { return (int)(std::lower_bound(nodes.begin(), nodes.end(), a) - nodes.begin()); };
+  std::vector<int> par(nodes.size()), sz(nodes.size(), 1);
+  for (size_t t = 0; t < par.size(); ++t) par[t] = (int)t;
+  auto find = [&](int a) {
+    while (par[a] != a) a = par[a] = par[par[a]];
+    return a;
+  };
+  const int cap = std::min(std::max(2, kmax), kClusterMax);
+  for (const NearLink& l : L) {
+    int a = find(id(l.i)), b = find(id(l.j));
+    if (a == b || sz[a] + sz[b] > cap) continue;
+    if (a > b) std::swap(a, b);
+    par[b] = a;
+    sz[a] += sz[b];
+  }
+  std::vector<std::vector<int>> groups(nodes.size());
+  for (size_t t = 0; t < nodes.size(); ++t) groups[find((int)t)].push_back(nodes[t]);
+  std::vector<int> off(1, 0), idx;
+  for (auto& g : groups) {  // groups[root] lists members ascending; roots are first members
+    if (g.size() < 2) continue;
+    idx.insert(idx.end(), g.begin(), g.end());
+    off.push_back((int)idx.size());
+  }
+  if (idx.empty() || (int)idx.size() > kClMembersCap) return COLLOP_OK;
+  ctx->ncl = (int)off.size() - 1;
+  ctx->cl_total = (int)idx.size();
+  HIPC(hipMemcpyAsync(ctx->b_cl_off.p, off.data(), off.size() * 4, hipMemcpyHostToDevice, ctx->stream));
+  HIPC(hipMemcpyAsync(ctx->b_cl_idx.p, idx.data(), idx.size() * 4, hipMemcpyHostToDevice, ctx->stream));
+  launch_cluster_rows(P<int>(ctx->b_cl_idx), ctx->cl_total, true, P<int>(ctx->b_cl_row), ctx->stream);
+  CHECK_LAUNCH();
+  launch_fill_int(P<int>(ctx->b_cl_have), ctx->ncl, 0, ctx->stream);
+  CHECK_LAUNCH();
+  launch_fill_int(P<int>(ctx->b_cl_its), ctx->ncl, 0, ctx->stream);
+  CHECK_LAUNCH();
+  launch_fill_int(P<int>(ctx->b_cl_fail), 1, 0, ctx->stream);
+  CHECK_LAUNCH();
+  // the host lists must outlive the asynchronous copies
+  HIPC(hipStreamSynchronize(ctx->stream));
+  return COLLOP_OK;
+}
+
+// The cluster solves of an FP64 map (after its pair sum): every cluster's rows
+// solved exactly with the map's v_mid, G_eff, lambda, L and partials.
+int cluster_solve_dev(collop_ctx* ctx, const collop_step_params& p, const double* part, int S) {
+  ClusterArgs a{};
+  a.m = ctx->mv;
+  a.off = P<int>(ctx->b_cl_off);
+  a.idx = P<int>(ctx->b_cl_idx);
+  a.N = (int)ctx->N;
+  a.dt = p.dt;
+  a.v0 = P<double>(ctx->s_v);
+  a.w = P<double>(ctx->s_w);
+  a.vmid = P<double>(ctx->b_vmid);
+  a.dS = P<double>(ctx->b_dS);
+  a.dv = P<double>(ctx->b_dv);
+  a.lam = P<double>(ctx->b_scal) + 2;
+  a.L = P<double>(ctx->b_vec) + 3 * ctx->mv.n_dofs;
+  a.part = part;
+  a.S = S;
+  a.mroot = P<double>(ctx->b_cl_root);
+  a.have_root = P<int>(ctx->b_cl_have);
+  a.x_out = P<double>(ctx->b_cl_x);
+  a.fail_count = P<int>(ctx->b_cl_fail);
+  a.newton_its = P<int>(ctx->b_cl_its);
+  launch_cluster_solve(a, ctx->ncl, ctx->stream);
+  CHECK_LAUNCH();
+  return COLLOP_OK;
+}
+
 // One Picard map (solver.jl:176-227, Landau branch): Gv = v0 + dt vdot(v_mid, G_eff),
 // followed by the epilogue of the Anderson iteration.
+// With stiff clusters (FP64 maps only), each cluster's rows take the cluster
+// solve's value (Anderson mixes that map) and the plain map's residual goes to
+// the statistics beside it.
 int picard_map_dev(collop_ctx* ctx, const collop_step_params& p, int kind, const double* vin, int col, int hist) {
   int r = geff_dev(ctx, p, vin, kind == 0 ? P<Src64>(ctx->b_s64) : nullptr, kind == 1 ? P<Src32>(ctx->b_s32) : nullptr,
                    nullptr, nullptr, nullptr, nullptr);
   if (r) return r;
   if ((r = run_pair(ctx, kind, (int)ctx->N, P<double>(ctx->b_part)))) return r;
-  return epilogue_dev(ctx, P<double>(ctx->b_part), kind == 0 ? ctx->tile64.slices : ctx->tile32.slices,
-                      P<double>(ctx->s_v), p.dt, vin, P<double>(ctx->s_Gv), col, hist);
+  const int S = kind == 0 ? ctx->tile64.slices : ctx->tile32.slices;
+  const bool cl = kind == 0 && ctx->ncl > 0;
+  if (cl && (r = cluster_solve_dev(ctx, p, P<double>(ctx->b_part), S))) return r;
+  return epilogue_dev(ctx, P<double>(ctx->b_part), S, P<double>(ctx->s_v), p.dt, vin, P<double>(ctx->s_Gv), col, hist,
+                      cl);
 }
 
 // Outer map of defect correction (solver.jl:590-594): G(v1) and the metric A in
@@ -1697,6 +1818,11 @@ int step_once(collop_ctx* ctx, const collop_step_params& p, collop_step_stats* o
     CHECK_LAUNCH();
   }
 
+  // stiff clusters of the step (Anderson, Landau): found on the state's markers
+  const bool want_cl = !lb && p.solver == COLLOP_SOLVER_ANDERSON && p.cl_theta > 0.0;
+  if (want_cl || ctx->cl_total > 0) {
+    if ((r = build_clusters_dev(ctx, want_cl ? p.cl_theta : 0.0, p.cl_kmax))) return r;
+  }
   collop_step_stats dc{};
   if (p.solver == COLLOP_SOLVER_DEFECT) {
     if ((r = defect_solve(ctx, p, &dc))) return r;
@@ -1705,7 +1831,7 @@ int step_once(collop_ctx* ctx, const collop_step_params& p, collop_step_stats* o
   }
   // ---- Anderson (solver.jl:300-423)
   AndersonRun an;
-  an.start(p);
+  an.start(p, ctx->ncl > 0);
   // The best iterate starts as the initial guess (solver.jl:331).
   if (p.solver == COLLOP_SOLVER_ANDERSON && (r = copy_d2d(ctx, ctx->s_best, ctx->s_v1, n2b))) return r;
   const char* trace = std::getenv("COLLOP_TRACE") ? "run" : nullptr;
@@ -1728,7 +1854,15 @@ int step_once(collop_ctx* ctx, const collop_step_params& p, collop_step_stats* o
         (r = copy_d2d(ctx, ctx->s_best, p.exit_picard_step ? ctx->s_Gv : ctx->s_v1, n2b)))
       return r;
     if (act.exit == kConverged) {
-      if (p.exit_picard_step) std::swap(ctx->s_v1, ctx->s_Gv);
+      if (p.exit_picard_step) {
+        std::swap(ctx->s_v1, ctx->s_Gv);
+        // the exit map is the plain map G(v) (solver.jl:351): its cluster rows too
+        if (ctx->ncl > 0 && pl.kind == 0) {
+          launch_cluster_fix(P<double>(ctx->s_v1), n, P<int>(ctx->b_cl_idx), ctx->cl_total,
+                             P<double>(ctx->b_cl_true), ctx->stream);
+          CHECK_LAUNCH();
+        }
+      }
       break;
     }
     if (act.exit != kContinue) {
@@ -1755,7 +1889,9 @@ int step_once(collop_ctx* ctx, const collop_step_params& p, collop_step_stats* o
   }
   if ((r = project_state(ctx, p.use_logsq))) return r;
   double sc[6];
+  int cl_fail = 0;
   HIPC(hipMemcpyAsync(sc, ctx->b_scal.p, 6 * 8, hipMemcpyDeviceToHost, ctx->stream));
+  if (ctx->ncl > 0) HIPC(hipMemcpyAsync(&cl_fail, ctx->b_cl_fail.p, sizeof(int), hipMemcpyDeviceToHost, ctx->stream));
   HIPC(hipStreamSynchronize(ctx->stream));
   prof_collect(ctx);
   if (out) {
@@ -1770,6 +1906,8 @@ int step_once(collop_ctx* ctx, const collop_step_params& p, collop_step_stats* o
       out->r0 = an.nrm_r0;
       out->inner_maps = 0;
     }
+    out->n_clusters = ctx->ncl;
+    out->cl_fail = cl_fail;
     out->entropy = sc[0];
     out->energy = 0.5 * sc[5];
     out->momentum[0] = sc[3];
@@ -1817,6 +1955,27 @@ int collop_state_set(collop_ctx* ctx, const double* v, const double* w, int64_t 
   ENSURE(ctx->s_dF, n2b * kMaxWindow);
   ENSURE(ctx->s_dG, n2b * kMaxWindow);
   ENSURE(ctx->s_fc, (size_t)ctx->mv.n_dofs * 8);
+  // stiff clusters: no cluster until a step finds them
+  const int ncell = ctx->mv.ne1 * ctx->mv.ne2;
+  ENSURE(ctx->b_cell_of, (size_t)N * 4);
+  ENSURE(ctx->b_cell_sorted, (size_t)N * 4);
+  ENSURE(ctx->b_cell_cnt, (size_t)ncell * 4);
+  ENSURE(ctx->b_cell_start, (size_t)(ncell + 1) * 4);
+  ENSURE(ctx->b_links, (size_t)kLinkCap * sizeof(NearLink));
+  ENSURE(ctx->b_link_count, 64);
+  ENSURE(ctx->b_cl_idx, (size_t)kClMembersCap * 4);
+  ENSURE(ctx->b_cl_off, (size_t)(kClMembersCap / 2 + 1) * 4);
+  ENSURE(ctx->b_cl_x, (size_t)2 * kClMembersCap * 8);
+  ENSURE(ctx->b_cl_true, (size_t)2 * kClMembersCap * 8);
+  ENSURE(ctx->b_cl_root, (size_t)2 * kClMembersCap * 8);
+  ENSURE(ctx->b_cl_have, (size_t)(kClMembersCap / 2) * 4);
+  ENSURE(ctx->b_cl_its, (size_t)(kClMembersCap / 2) * 4);
+  ENSURE(ctx->b_cl_fail, 64);
+  ENSURE(ctx->b_cl_row, (size_t)N * 4);
+  launch_fill_int(P<int>(ctx->b_cl_row), (int)N, -1, ctx->stream);
+  CHECK_LAUNCH();
+  ctx->cl_row_n = N;
+  ctx->ncl = ctx->cl_total = 0;
   ctx->N = N;
   ctx->w_abs_sum = wsum;
   ctx->w_max = wmax;

@@ -51,6 +51,12 @@ struct collop_ctx {
   DBuf b_nidx;                  // near-target row of every marker (-1: none)
   DBuf s_v1n;                   // the inner iterate after a scatter-applied Anderson update
   int near_nt = 0;              // near targets of the current step
+  // stiff clusters of the current step (cluster.hip): cell buckets, the near
+  // links, the clusters in CSR, the member row of every marker, map values
+  DBuf b_cell_of, b_cell_cnt, b_cell_start, b_cell_sorted, b_links, b_link_count;
+  DBuf b_cl_off, b_cl_idx, b_cl_row, b_cl_x, b_cl_true, b_cl_root, b_cl_have, b_cl_fail, b_cl_its;
+  int ncl = 0, cl_total = 0;    // clusters and their members
+  int64_t cl_row_n = 0;         // markers b_cl_row covers (all -1 outside the members)
   double* h_stat = nullptr;  // pinned, kStatWidth doubles
   double* h_stat_dev = nullptr;  // its device-side address: the statistics reduction writes there
   unsigned* h_seq = nullptr;      // pinned: sequence number published after each h_stat write

@@ -225,10 +225,15 @@ struct EpiArgs {
   const double* v0; const double* v1; double dt;
   double* Gv; double* r; const double* rp; const double* Gp;
   double* dF; double* dG; int col; int hist;
-  double* stat_part;  // [blocks][kStatWidth]: |r|^2, |v1|^2, |Gv-v0|^2, ATA row, ATr
+  double* stat_part;  // [blocks][kStatWidth]: |r|^2, |v1|^2, |Gv-v0|^2, ATA row, ATr, |r_map|^2 - |r|^2
+  // stiff clusters (null: none): row of marker i in the cluster members (-1:
+  // none); the map value of such a row is cl_x[2 row + comp] (the cluster
+  // solve's), and the plain map value goes to cl_true[2 row + comp]
+  const int* cl_row; const double* cl_x; double* cl_true;
 };
 int epi_blocks(int N);
-constexpr int kStatWidth = 3 + 2 * kMaxWindow;
+constexpr int kStatTrue = 3 + 2 * kMaxWindow;  // |Gv_map - v1|^2 - |Gv - v1|^2 over the cluster rows
+constexpr int kStatWidth = kStatTrue + 1;
 void launch_epilogue(const EpiArgs& a, int N, hipStream_t st);
 // The statistics of n rows of epilogue partials (launch_sum_rows' sums and
 // order) into the pinned host buffer out (device address), then seq into
@@ -354,5 +359,52 @@ void launch_targets(const double* v_old, const double* v_new, const double* dotv
                     hipStream_t st);
 void launch_warm_apply(const double* v, const double* dotv, const float* y, float os0, float os1, double dt, int N,
                        double* v1, hipStream_t st);
+
+
+// ---- stiff clusters (cluster.hip) --------------------------------------------
+constexpr int kClusterMax = 32;         // largest cluster solved exactly (markers)
+constexpr int kLinkCap = 1 << 17;       // near links kept per step (beyond it: no clusters that step)
+constexpr int kClMembersCap = 1 << 15;  // cluster members per step (beyond it: no clusters that step)
+struct NearLink {
+  int i, j;  // i < j
+  double q;  // |v_i - v_j|^2
+};
+// x[0 .. n) <- value
+void launch_fill_int(int* x, int n, int value, hipStream_t st);
+// Markers of v (N) bucketed by mesh cell: cell_of[N] (-1 outside the box),
+// start[ncell + 1] (cnt[ncell] is scratch), sorted[] = marker ids cell by cell.
+void launch_cell_sort(const MeshView& m, const double* v, int N, int* cell_of, int* cnt, int* start, int* sorted,
+                      hipStream_t st);
+// Every unordered in-box pair with |d|^2 >= 1e-24 and max(w_i, w_j) / |d| >
+// theta (w_max = max w bounds the reach across cell edges) into out[] (count
+// receives the number found; entries beyond cap are dropped).
+void launch_near_cells(const MeshView& m, const double* v, const double* w, int N, double theta, double w_max,
+                       const int* start, const int* sorted, NearLink* out, int* count, int cap, hipStream_t st);
+// row[idx[r]] = r (set) or -1 (reset) for r < total
+void launch_cluster_rows(const int* idx, int total, bool set, int* row, hipStream_t st);
+// v[idx[r]], v[N + idx[r]] <- x[2 r], x[2 r + 1]
+void launch_cluster_fix(double* v, int N, const int* idx, int total, const double* x, hipStream_t st);
+struct ClusterArgs {
+  MeshView m;
+  const int* off;        // clusters: members idx[off[c] .. off[c + 1]) (<= kClusterMax each)
+  const int* idx;
+  int N;
+  double dt;
+  const double* v0;      // state (start of the step)
+  const double* w;
+  const double* vmid;    // the map's v_mid, dS = -w grad L(v_mid), dv = v_in - v0
+  const double* dS;
+  const double* dv;
+  const double* lam;     // the map's Gonzalez lambda (device scalar)
+  const double* L;       // the map's M^-1 r (n_dofs)
+  const double* part;    // the map's pair-sum partials [S][2N]
+  int S;
+  double* mroot;         // 2 per member: the cluster's last root (warm start)
+  int* have_root;        // per cluster
+  double* x_out;         // 2 per member: the map value of the member's rows
+  int* fail_count;       // += clusters whose solve reached no root
+  int* newton_its;       // per cluster: += Newton iterations
+};
+void launch_cluster_solve(const ClusterArgs& a, int ncl, hipStream_t st);
 
 }  // namespace collop

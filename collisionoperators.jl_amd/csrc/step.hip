@@ -104,9 +104,21 @@ __device__ __forceinline__ void epilogue_body(const EpiArgs& A, int N, int ld, s
       s = *s_given;
     else
       for (int k = 0; k < A.S; ++k) s += A.part[(size_t)k * n2 + i];
-    const double gv = A.v0[e] + A.dt * s;
-    A.Gv[e] = gv;
+    double gv = A.v0[e] + A.dt * s;
     const double x1 = A.v1[e];
+    if (A.cl_row) {  // a stiff cluster's row: the cluster solve's value replaces the map's
+      const int mk = i < N ? i : i - N;
+      const int row = A.cl_row[mk];
+      if (row >= 0) {
+        const int cp = 2 * row + (i < N ? 0 : 1);
+        A.cl_true[cp] = gv;
+        const double rt = gv - x1;
+        gv = A.cl_x[cp];
+        const double rh = gv - x1;
+        st[kStatTrue] = rt * rt - rh * rh;
+      }
+    }
+    A.Gv[e] = gv;
     const double ri = gv - x1;
     A.r[e] = ri;
     st[0] = ri * ri;
@@ -412,6 +424,9 @@ __global__ __launch_bounds__(kEwThreads) void k_epilogue_b(EnsView E, EnsBufs B,
   a.col = c.col;
   a.hist = c.hist;
   a.stat_part = B.statp + (size_t)(R.eblk0 + t.y) * kStatWidth;
+  a.cl_row = nullptr;
+  a.cl_x = nullptr;
+  a.cl_true = nullptr;
   epilogue_body(a, R.n, E.NT, (size_t)2 * E.NT, red, t.y);
 }
 

@@ -236,6 +236,14 @@ typedef struct {
                                (Parameters.jl:98; 0 = 1e-6)                     */
   double nk_fd_rel;         /* NEWTON: legacy relative step nk_fd_rel max(|v|, 1);
                                > 0 overrides nk_fd_h                            */
+  double cl_theta;          /* ANDERSON extension (Landau): markers with
+                               max(w_i, w_j) / |d| > cl_theta at the start of
+                               the step form stiff clusters whose rows every
+                               FP64 map solves exactly (nonlinear elimination);
+                               the exit test stays on the plain map, so the
+                               root is the same; 0 = off                       */
+  int32_t cl_kmax;          /* largest stiff cluster (markers, <= 32)          */
+  int32_t cl_pad;
 } collop_step_params;
 
 typedef struct {
@@ -250,6 +258,9 @@ typedef struct {
   double momentum[2];       /* P = sum w v                                     */
   double seconds;           /* wall time of the step                           */
   int32_t inner_maps;       /* DEFECT: frozen-metric maps (no pair sum)         */
+  int32_t n_clusters;       /* ANDERSON: stiff clusters solved in each FP64 map */
+  int32_t cl_fail;          /* cluster solves that reached no root (those maps
+                               kept the plain rows of the cluster)             */
   int32_t pad;
 } collop_step_stats;
 

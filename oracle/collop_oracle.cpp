@@ -300,6 +300,8 @@ struct orc_solver_params {
   double nk_eta_max;     // Eisenstat-Walker cap (Parameters.jl:93)
   double nk_fd_h;        // absolute finite-difference step (Parameters.jl:98; 0 = 1e-6)
   double nk_fd_rel;      // legacy relative step (> 0 overrides nk_fd_h)
+  double cl_theta;       // extension: stiff clusters of pairs with max(w_i, w_j) / |d| > cl_theta (0 = off)
+  int cl_kmax;           // largest cluster solved exactly (markers)
 };
 
 struct orc_step_stats {
@@ -310,6 +312,8 @@ struct orc_step_stats {
   int exit_reason;  // 0 converged, 1 stagnation, 2 max_iter
   double entropy, energy, p1, p2;
   int inner;        // step_defect!: frozen-metric maps
+  int n_clusters;   // stiff clusters solved exactly inside each map
+  int cl_fail;      // cluster solves that failed (their rows keep the plain map)
 };
 
 namespace {
@@ -956,20 +960,22 @@ void lb_geff(const orc_mesh* ms, double* Geff, const double* vin, const double* 
 
 // solver.jl:176-227 (picard_map!): the Landau branch, or the LB branch
 // (solver.jl:185-210, 218-222) when collision_model = 1
-void picard_map(const orc_mesh* ms, double* vout, const double* vin, const double* v0, const double* w, int64_t N,
-                double S0, double dt, Scratch& s, const orc_solver_params& P) {
+// Returns the Gonzalez lambda of the Landau branch (0 for LB).
+double picard_map(const orc_mesh* ms, double* vout, const double* vin, const double* v0, const double* w, int64_t N,
+                  double S0, double dt, Scratch& s, const orc_solver_params& P) {
   if (P.collision_model == 1) {
     lb_geff(ms, s.Geff.data(), vin, v0, w, N, S0, s, P.use_gonzalez, P.use_logsq);
     double AB[3];
     lb_rhs(s.dotv.data(), s.vmid.data(), w, s.Geff.data(), N, P.nu, AB);
     for (int64_t i = 0; i < 2 * N; ++i) vout[i] = v0[i] + dt * s.dotv[i];
-    return;
+    return 0.0;
   }
-  landau_geff(ms, s.Geff.data(), vin, v0, w, N, S0, s, P.use_gonzalez, P.use_logsq);
+  const double lam = landau_geff(ms, s.Geff.data(), vin, v0, w, N, S0, s, P.use_gonzalez, P.use_logsq);
   double box[4];
   box_of(ms, box);
   orc_collision(box, s.vmid.data(), w, s.Geff.data(), N, s.dotv.data(), P.nthreads);
   for (int64_t i = 0; i < 2 * N; ++i) vout[i] = v0[i] + dt * s.dotv[i];
+  return lam;
 }
 
 double nrm(const double* x, int64_t n) {
@@ -1091,6 +1097,563 @@ void orc_predictor(const orc_mesh* ms, const orc_solver_params* P, const double*
   if (dot_v_out) std::memcpy(dot_v_out, dv.data(), sizeof(double) * 2 * N);
 }
 
+}  // extern "C"
+
+// ---------------------------------------------------------------------------
+// Stiff clusters (extension; no counterpart in the reference).
+//
+// Markers closer than max(w_i, w_j) / theta form clusters.  Inside a cluster
+// the pair terms w U(d) (G_a - G_g) turn the midpoint system very stiff (the
+// stiffness of a pair grows like dt w |Hess L| / |d|), and Anderson over the
+// whole map stalls once there are more such modes than window columns.  Each
+// map therefore replaces a cluster's rows by the exact solution of the
+// cluster's own midpoint equations, everything outside the cluster held at
+// the map's values (nonlinear elimination):
+//   m_i = v0_i + dt/2 (ext_i + sum_{j in C} w_j U(m_i - m_j) (g_j - g_i)),
+//   g_i = grad L(m_i) - 2 lambda (m_i - v0_i) / w_i,   x_i = 2 m_i - v0_i,
+// with ext_i = the map's vdot_i minus the cluster's own terms at the iterate,
+// and L, lambda those of the map.  At a fixed point of the modified map the
+// cluster rows equal the plain map's, so the root is the implicit system's.
+// The cluster equations can have several roots; the one taken is the end of
+// the homotopy in the coupling strength t (the cluster's own terms scaled by
+// t) from t = 0, where m = v0 + dt/2 ext, to t = 1, each point corrected by
+// Newton with the analytic Jacobian; a warm start from the previous map's root
+// of the same cluster is tried first.  The weighted centre of m does not move
+// along the path (the cluster's own terms carry no momentum).
+namespace {
+
+struct Link {
+  double q;
+  int i, j;
+};
+
+// Unordered in-domain pairs i < j with |d|^2 >= 1e-24 and max(w_i, w_j) / |d| > theta,
+// ordered by (|d|^2, i, j).
+std::vector<Link> find_links(const double* box, const double* v, const double* w, int64_t N, double theta) {
+  std::vector<Link> out;
+  if (!(theta > 0.0)) return out;
+  double wmax = 0.0;
+  for (int64_t a = 0; a < N; ++a) wmax = std::max(wmax, w[a]);
+  const double qmax = wmax * wmax / (theta * theta);
+  // sweep in v1 order: only pairs with |dv1| < sqrt(qmax) can be near
+  std::vector<int> ord(N);
+  for (int64_t a = 0; a < N; ++a) ord[a] = (int)a;
+  std::sort(ord.begin(), ord.end(), [&](int a, int b) { return v[a] != v[b] ? v[a] < v[b] : a < b; });
+  const double r = std::sqrt(qmax);
+  for (int64_t x = 0; x < N; ++x) {
+    const int a = ord[x];
+    if (!in_box(v[a], v[N + a], box[0], box[1], box[2], box[3])) continue;
+    for (int64_t y = x + 1; y < N && v[ord[y]] - v[a] < r; ++y) {
+      const int b = ord[y];
+      if (!in_box(v[b], v[N + b], box[0], box[1], box[2], box[3])) continue;
+      const double d1 = v[a] - v[b], d2 = v[N + a] - v[N + b];
+      const double q = d1 * d1 + d2 * d2;
+      const double wm = std::max(w[a], w[b]);
+      if (q >= 1e-24 && theta * theta * q < wm * wm) out.push_back({q, std::min(a, b), std::max(a, b)});
+    }
+  }
+  std::sort(out.begin(), out.end(), [](const Link& x, const Link& y) {
+    return x.q != y.q ? x.q < y.q : (x.i != y.i ? x.i < y.i : x.j < y.j);
+  });
+  return out;
+}
+
+// Links joined closest first (union-find) unless the merged cluster would
+// exceed kmax markers; clusters of >= 2 markers with members ascending, ordered
+// by their first member.  CSR: off[c] .. off[c+1] into idx.
+void build_clusters(const std::vector<Link>& links, int kmax, std::vector<int>& off, std::vector<int>& idx) {
+  std::vector<int> nodes;
+  for (const Link& l : links) {
+    nodes.push_back(l.i);
+    nodes.push_back(l.j);
+  }
+  std::sort(nodes.begin(), nodes.end());
+  nodes.erase(std::unique(nodes.begin(), nodes.end()), nodes.end());
+  auto id = [&](int a) { return (int)(std::lower_bound(nodes.begin(), nodes.end(), a) - nodes.begin()); };
+  std::vector<int> par(nodes.size()), sz(nodes.size(), 1);
+  for (size_t t = 0; t < par.size(); ++t) par[t] = (int)t;
+  std::function<int(int)> find = [&](int a) { return par[a] == a ? a : (par[a] = find(par[a])); };
+  for (const Link& l : links) {
+    int a = find(id(l.i)), b = find(id(l.j));
+    if (a == b || sz[a] + sz[b] > kmax) continue;
+    if (a > b) std::swap(a, b);
+    par[b] = a;
+    sz[a] += sz[b];
+  }
+  std::vector<std::vector<int>> groups(nodes.size());
+  for (size_t t = 0; t < nodes.size(); ++t) groups[find((int)t)].push_back(nodes[t]);
+  std::vector<std::vector<int>> cl;
+  for (auto& g : groups)
+    if (g.size() >= 2) cl.push_back(g);
+  std::sort(cl.begin(), cl.end(), [](const std::vector<int>& a, const std::vector<int>& b) { return a[0] < b[0]; });
+  off.assign(1, 0);
+  idx.clear();
+  for (auto& g : cl) {
+    idx.insert(idx.end(), g.begin(), g.end());
+    off.push_back((int)idx.size());
+  }
+}
+
+// grad L and the Hessian (H11, H12, H22) of L = sum L_k phi_k at (x, y); false outside the box
+bool grad_hess(const orc_mesh* ms, const double* L, double x, double y, double* g, double* H) {
+  Loc lc;
+  if (!locate(ms, x, y, lc)) return false;
+  const int q = ms->p + 1;
+  double B1[16], dB1[16], B2[16], dB2[16];
+  bernstein_eval(ms->p, lc.xi1, B1, dB1);
+  bernstein_eval(ms->p, lc.xi2, B2, dB2);
+  // second derivatives of the Bernstein polynomials: p (p-1) (B^{p-2}_{i-2} - 2 B^{p-2}_{i-1} + B^{p-2}_i)
+  auto d2b = [&](double xi, double* out) {
+    const int p = ms->p;
+    double Bm[16] = {0};
+    for (int i = 0; i <= p - 2; ++i) Bm[i] = binom(p - 2, i) * std::pow(xi, i) * std::pow(1.0 - xi, p - 2 - i);
+    for (int i = 0; i <= p; ++i) {
+      const double a = (i >= 2 && i - 2 <= p - 2) ? Bm[i - 2] : 0.0;
+      const double b = (i >= 1 && i - 1 <= p - 2) ? Bm[i - 1] : 0.0;
+      const double c = (i <= p - 2) ? Bm[i] : 0.0;
+      out[i] = p * (p - 1) * (a - 2.0 * b + c);
+    }
+  };
+  double ddB1[16], ddB2[16];
+  d2b(lc.xi1, ddB1);
+  d2b(lc.xi2, ddB2);
+  double f1[16], df1[16], ddf1[16], f2[16], df2[16], ddf2[16];
+  const double* C1 = &ms->d1.C[(size_t)q * q * lc.i];
+  const double* C2 = &ms->d2.C[(size_t)q * q * lc.j];
+  for (int j = 0; j < q; ++j) {
+    f1[j] = df1[j] = ddf1[j] = f2[j] = df2[j] = ddf2[j] = 0.0;
+    for (int i = 0; i < q; ++i) {
+      f1[j] += C1[i + q * j] * B1[i];
+      df1[j] += C1[i + q * j] * dB1[i];
+      ddf1[j] += C1[i + q * j] * ddB1[i];
+      f2[j] += C2[i + q * j] * B2[i];
+      df2[j] += C2[i + q * j] * dB2[i];
+      ddf2[j] += C2[i + q * j] * ddB2[i];
+    }
+  }
+  const int s1 = ms->d1.bs[lc.i], s2 = ms->d2.bs[lc.j];
+  double gx = 0, gy = 0, hxx = 0, hxy = 0, hyy = 0;
+  for (int j2 = 0; j2 < q; ++j2)
+    for (int j1 = 0; j1 < q; ++j1) {
+      const double c = L[(s1 + j1) + (s2 + j2) * ms->d1.nd];
+      gx += c * df1[j1] * f2[j2];
+      gy += c * f1[j1] * df2[j2];
+      hxx += c * ddf1[j1] * f2[j2];
+      hxy += c * df1[j1] * df2[j2];
+      hyy += c * f1[j1] * ddf2[j2];
+    }
+  g[0] = gx / lc.h1;
+  g[1] = gy / lc.h2;
+  H[0] = hxx / (lc.h1 * lc.h1);
+  H[1] = hxy / (lc.h1 * lc.h2);
+  H[2] = hyy / (lc.h2 * lc.h2);
+  return true;
+}
+
+// The cluster's equations Phi(m; t) (interleaved m[2i], m[2i+1]) and, if J is
+// non-null, their Jacobian (n x n row-major, n = 2k).  False if a marker left
+// the box or two coincide.
+struct ClusterEq {
+  const orc_mesh* ms;
+  const double* L;
+  double lam, dt;
+  int k;
+  const double* v0;   // 2k interleaved
+  const double* w;    // k
+  const double* ext;  // 2k interleaved
+
+  bool eval(const double* m, double t, double* F, double* J) const {
+    const int n = 2 * k;
+    std::vector<double> g(n), H(3 * k);
+    for (int i = 0; i < k; ++i) {
+      if (!grad_hess(ms, L, m[2 * i], m[2 * i + 1], &g[2 * i], &H[3 * i])) return false;
+      g[2 * i] -= 2.0 * lam * (m[2 * i] - v0[2 * i]) / w[i];
+      g[2 * i + 1] -= 2.0 * lam * (m[2 * i + 1] - v0[2 * i + 1]) / w[i];
+    }
+    const double s = t * 0.5 * dt;
+    if (J) std::fill(J, J + (size_t)n * n, 0.0);
+    for (int i = 0; i < k; ++i) {
+      double acc0 = 0.0, acc1 = 0.0;
+      for (int j = 0; j < k; ++j) {
+        if (j == i) continue;
+        const double d0 = m[2 * i] - m[2 * j], d1 = m[2 * i + 1] - m[2 * j + 1];
+        const double y0 = g[2 * j] - g[2 * i], y1 = g[2 * j + 1] - g[2 * i + 1];
+        const double qd = d0 * d0 + d1 * d1;
+        if (!(qd >= 1e-24)) return false;
+        const double r = std::sqrt(qd), r3 = qd * r;
+        const double dy = d0 * y0 + d1 * y1;
+        // U(d) y = y / r - d (d.y) / r^3
+        acc0 += w[j] * (y0 / r - d0 * dy / r3);
+        acc1 += w[j] * (y1 / r - d1 * dy / r3);
+        if (J) {
+          // Dd[a][b] = d(U(d) y)_a / d d_b = -(y_a d_b + d_a y_b + (d.y) delta_ab) / r^3 + 3 d_a (d.y) d_b / r^5
+          const double dd[2] = {d0, d1}, yy[2] = {y0, y1};
+          double Dd[2][2], U[2][2];
+          for (int a = 0; a < 2; ++a)
+            for (int b = 0; b < 2; ++b) {
+              Dd[a][b] = -(yy[a] * dd[b] + dd[a] * yy[b] + (a == b ? dy : 0.0)) / r3 + 3.0 * dd[a] * dy * dd[b] / (r3 * qd);
+              U[a][b] = ((a == b ? 1.0 : 0.0) - dd[a] * dd[b] / qd) / r;
+            }
+          // B_i = Hess L(m_i) - 2 lambda / w_i I
+          const double Bi[2][2] = {{H[3 * i] - 2.0 * lam / w[i], H[3 * i + 1]}, {H[3 * i + 1], H[3 * i + 2] - 2.0 * lam / w[i]}};
+          const double Bj[2][2] = {{H[3 * j] - 2.0 * lam / w[j], H[3 * j + 1]}, {H[3 * j + 1], H[3 * j + 2] - 2.0 * lam / w[j]}};
+          for (int a = 0; a < 2; ++a)
+            for (int b = 0; b < 2; ++b) {
+              const double UBi = U[a][0] * Bi[0][b] + U[a][1] * Bi[1][b];
+              const double UBj = U[a][0] * Bj[0][b] + U[a][1] * Bj[1][b];
+              // dPhi_i/dm_i -= s w_j (Dd - U B_i);  dPhi_i/dm_j -= s w_j (-Dd + U B_j)
+              J[(size_t)(2 * i + a) * n + 2 * i + b] -= s * w[j] * (Dd[a][b] - UBi);
+              J[(size_t)(2 * i + a) * n + 2 * j + b] -= s * w[j] * (-Dd[a][b] + UBj);
+            }
+        }
+      }
+      F[2 * i] = m[2 * i] - v0[2 * i] - 0.5 * dt * ext[2 * i] - s * acc0;
+      F[2 * i + 1] = m[2 * i + 1] - v0[2 * i + 1] - 0.5 * dt * ext[2 * i + 1] - s * acc1;
+      if (J) {
+        J[(size_t)(2 * i) * n + 2 * i] += 1.0;
+        J[(size_t)(2 * i + 1) * n + 2 * i + 1] += 1.0;
+      }
+    }
+    return true;
+  }
+};
+
+// LU with partial pivoting: solves A x = b in place (A n x n row-major); false if singular
+bool lu_solve(int n, std::vector<double>& A, std::vector<double>& b) {
+  for (int c = 0; c < n; ++c) {
+    int pr = c;
+    for (int r = c + 1; r < n; ++r)
+      if (std::fabs(A[(size_t)r * n + c]) > std::fabs(A[(size_t)pr * n + c])) pr = r;
+    if (!(std::fabs(A[(size_t)pr * n + c]) > 0.0)) return false;
+    if (pr != c) {
+      for (int j = 0; j < n; ++j) std::swap(A[(size_t)c * n + j], A[(size_t)pr * n + j]);
+      std::swap(b[c], b[pr]);
+    }
+    const double inv = 1.0 / A[(size_t)c * n + c];
+    for (int r = c + 1; r < n; ++r) {
+      const double f = A[(size_t)r * n + c] * inv;
+      if (f == 0.0) continue;
+      for (int j = c; j < n; ++j) A[(size_t)r * n + j] -= f * A[(size_t)c * n + j];
+      b[r] -= f * b[c];
+    }
+  }
+  for (int r = n - 1; r >= 0; --r) {
+    double t = b[r];
+    for (int j = r + 1; j < n; ++j) t -= A[(size_t)r * n + j] * b[j];
+    b[r] = t / A[(size_t)r * n + r];
+  }
+  return true;
+}
+
+// Newton on Phi(m; t) = 0 from m (in/out): converged when |Phi| < tol_abs.
+// A step is shortened so that no two markers move by more than half their
+// distance relative to each other (factor a <= 1); it fails on a singular
+// Jacobian or a residual that does not fall to (1 - a / 10) of the last one
+// (unless already within 1e3 tol_abs).
+bool cl_newton(const ClusterEq& E, std::vector<double>& m, double t, int maxit, double tol_abs, int& its) {
+  const int k = E.k, n = 2 * k;
+  std::vector<double> F(n), Fn(n), J((size_t)n * n), mn(n);
+  if (!E.eval(m.data(), t, F.data(), nullptr)) return false;
+  double nf = nrm(F.data(), n);
+  for (int it = 0; it < maxit; ++it) {
+    if (nf < tol_abs) return true;
+    if (!E.eval(m.data(), t, F.data(), J.data())) return false;
+    std::vector<double> dlt(F);
+    for (double& x : dlt) x = -x;
+    ++its;
+    if (!lu_solve(n, J, dlt)) return false;
+    double a = 1.0;
+    for (int i = 0; i < k; ++i)
+      for (int j = i + 1; j < k; ++j) {
+        const double r0 = dlt[2 * i] - dlt[2 * j], r1 = dlt[2 * i + 1] - dlt[2 * j + 1];
+        const double d0 = m[2 * i] - m[2 * j], d1 = m[2 * i + 1] - m[2 * j + 1];
+        const double rr = r0 * r0 + r1 * r1, dd = d0 * d0 + d1 * d1;
+        if (rr > 0.25 * dd) a = std::min(a, 0.5 * std::sqrt(dd / rr));
+      }
+    for (int c = 0; c < n; ++c) mn[c] = m[c] + a * dlt[c];
+    if (!E.eval(mn.data(), t, Fn.data(), nullptr)) return false;
+    const double nn = nrm(Fn.data(), n);
+    if (!(nn <= (1.0 - 0.1 * a) * nf) && nf > 1e3 * tol_abs) return false;
+    m = mn;
+    nf = nn;
+  }
+  return nf < tol_abs;
+}
+
+// A two-marker cluster reduces to one unknown.  With W = w_g + w_a, the
+// weighted centre c of (m_g, m_a) is explicit and d = m_g - m_a solves
+// d = a + dt/2 W U(d) (g_a - g_g), a = v0_g - v0_a + dt/2 (ext_g - ext_a).
+// U(d) y is orthogonal to d, so d . (d - a) = 0: d lies on the circle over a,
+// d = |a| cos(psi) e(phi_a + psi), |psi| < pi/2, and the remaining equation is
+//   F(psi) = -|a|^2 / 2 sin(2 psi) + dt/2 W e_perp . (g_a - g_g) = 0.
+// At psi -> +-pi/2 (d -> 0) F tends to +-dt W lambda (a . (v0_g - v0_a)) / (|a|
+// w) for equal weights, so a root exists whenever that is non-zero.  The root
+// taken is the one nearest psi_ref (0, the uncoupled solution d = a, or the
+// previous map's root): sign changes on a fixed grid of 64 points, then
+// bisection.  Returns false if the grid shows no sign change.
+struct PairEq {
+  const ClusterEq& E;
+  double c[2], a[2], na, pa;
+  explicit PairEq(const ClusterEq& e) : E(e) {
+    const double* v0 = E.v0;
+    const double* x = E.ext;
+    const double wg = E.w[0], wa = E.w[1], W = wg + wa;
+    for (int q = 0; q < 2; ++q) {
+      c[q] = (wg * v0[q] + wa * v0[2 + q]) / W + 0.5 * E.dt * (wg * x[q] + wa * x[2 + q]) / W;
+      a[q] = v0[q] - v0[2 + q] + 0.5 * E.dt * (x[q] - x[2 + q]);
+    }
+    na = std::sqrt(a[0] * a[0] + a[1] * a[1]);
+    pa = std::atan2(a[1], a[0]);
+  }
+  // midpoints of psi; false outside the box
+  void mid(double psi, double* m) const {
+    const double wg = E.w[0], wa = E.w[1], W = wg + wa;
+    const double ph = pa + psi, rho = na * std::cos(psi);
+    const double d0 = rho * std::cos(ph), d1 = rho * std::sin(ph);
+    m[0] = c[0] + (wa / W) * d0;
+    m[1] = c[1] + (wa / W) * d1;
+    m[2] = c[0] - (wg / W) * d0;
+    m[3] = c[1] - (wg / W) * d1;
+  }
+  bool F(double psi, double& f) const {
+    double m[4], g[4], H[3];
+    mid(psi, m);
+    for (int i = 0; i < 2; ++i) {
+      if (!grad_hess(E.ms, E.L, m[2 * i], m[2 * i + 1], &g[2 * i], H)) return false;
+      g[2 * i] -= 2.0 * E.lam * (m[2 * i] - E.v0[2 * i]) / E.w[i];
+      g[2 * i + 1] -= 2.0 * E.lam * (m[2 * i + 1] - E.v0[2 * i + 1]) / E.w[i];
+    }
+    const double ph = pa + psi;
+    const double ep0 = -std::sin(ph), ep1 = std::cos(ph);
+    f = -0.5 * na * na * std::sin(2.0 * psi) + 0.5 * E.dt * (E.w[0] + E.w[1]) * (ep0 * (g[2] - g[0]) + ep1 * (g[3] - g[1]));
+    return true;
+  }
+};
+
+constexpr int kPairGrid = 64;
+inline double pair_grid(int j) { return -0.5 * M_PI + M_PI * (j + 0.5) / kPairGrid; }
+
+bool pair_root(const ClusterEq& E, double psi_ref, double& psi_out, std::vector<double>& m) {
+  PairEq P(E);
+  if (!(P.na > 0.0)) return false;
+  double f[kPairGrid];
+  for (int j = 0; j < kPairGrid; ++j)
+    if (!P.F(pair_grid(j), f[j])) return false;
+  int best = -1;
+  double bd = 1e300;
+  for (int j = 0; j + 1 < kPairGrid; ++j) {
+    if ((f[j] < 0.0) == (f[j + 1] < 0.0) && f[j] != 0.0) continue;
+    const double dd = std::fabs(0.5 * (pair_grid(j) + pair_grid(j + 1)) - psi_ref);
+    if (dd < bd) {
+      bd = dd;
+      best = j;
+    }
+  }
+  if (best < 0) return false;
+  double lo = pair_grid(best), hi = pair_grid(best + 1), flo = f[best];
+  for (int it = 0; it < 200 && hi - lo > 1e-15; ++it) {
+    const double md = 0.5 * (lo + hi);
+    double fm;
+    if (!P.F(md, fm)) return false;
+    if ((fm < 0.0) == (flo < 0.0) && fm != 0.0) {
+      lo = md;
+      flo = fm;
+    } else {
+      hi = md;
+    }
+  }
+  psi_out = 0.5 * (lo + hi);
+  m.resize(4);
+  P.mid(psi_out, m.data());
+  return true;
+}
+
+// The cluster's root (midpoints m, interleaved): Newton from mprev if given,
+// else (or if that fails) the homotopy from t = 0.  Returns false if no root
+// was reached within the budget; its counts Newton iterations.
+bool cluster_root(const ClusterEq& E, const double* mprev, std::vector<double>& m, int& its) {
+  const int n = 2 * E.k;
+  if (E.k == 2) {  // the one-unknown form; the reference angle is the previous root's
+    double psi_ref = 0.0, psi = 0.0;
+    its = 0;
+    if (mprev) {
+      PairEq P(E);
+      const double d0 = mprev[0] - mprev[2], d1 = mprev[1] - mprev[3];
+      double r = std::atan2(d1, d0) - P.pa;
+      while (r > M_PI) r -= 2.0 * M_PI;
+      while (r < -M_PI) r += 2.0 * M_PI;
+      if (std::fabs(r) < 0.5 * M_PI) psi_ref = r;
+    }
+    return pair_root(E, psi_ref, psi, m);
+  }
+  double scale = 1.0;
+  for (int c = 0; c < n; ++c) scale = std::max(scale, std::fabs(E.v0[c]) + 1.0);
+  const double tol_final = 1e-13 * scale, tol_path = 1e-12 * scale;
+  its = 0;
+  if (mprev) {
+    m.assign(mprev, mprev + n);
+    if (cl_newton(E, m, 1.0, 10, tol_final, its)) return true;
+  }
+  // t = 0: the cluster drifts with ext alone
+  m.resize(n);
+  for (int c = 0; c < n; ++c) m[c] = E.v0[c] + 0.5 * E.dt * E.ext[c];
+  const std::vector<double> m_t0(m);
+  std::vector<double> mp(m), guess(n);
+  double t = 0.0, tp = 0.0, dtau = 0.05;
+  bool have_prev = false, path_ok = true;
+  while (t < 1.0) {
+    if (its > 400) {
+      path_ok = false;
+      break;
+    }
+    const double t1 = std::min(1.0, t + dtau);
+    for (int c = 0; c < n; ++c) guess[c] = have_prev ? m[c] + (m[c] - mp[c]) * (t1 - t) / (t - tp) : m[c];
+    if (cl_newton(E, guess, t1, 8, tol_path, its)) {
+      mp = m;
+      tp = t;
+      m = guess;
+      t = t1;
+      have_prev = true;
+      dtau = std::min(1.5 * dtau, 0.5);
+    } else {
+      dtau *= 0.5;
+      if (dtau < 1e-7) {
+        path_ok = false;
+        break;
+      }
+    }
+  }
+  if (path_ok && cl_newton(E, m, 1.0, 20, tol_final, its)) return true;
+  // The path folded back before t = 1.  In the stiff limit the cluster's own
+  // dynamics lines its markers up along an eigenvector of Hess L (every pair
+  // term vanishes there), so Newton starts from the t = 0 configuration
+  // projected on the line through its centre along each eigenvector in turn.
+  double W = 0.0, c0 = 0.0, c1 = 0.0;
+  for (int i = 0; i < E.k; ++i) {
+    W += E.w[i];
+    c0 += E.w[i] * m_t0[2 * i];
+    c1 += E.w[i] * m_t0[2 * i + 1];
+  }
+  c0 /= W;
+  c1 /= W;
+  double g[2], H[3];
+  if (!grad_hess(E.ms, E.L, c0, c1, g, H)) return false;
+  // eigenvectors of [[H0, H1], [H1, H2]], smaller eigenvalue first
+  const double th = 0.5 * std::atan2(2.0 * H[1], H[0] - H[2]);
+  const double ca = std::cos(th), sa = std::sin(th);
+  const double l_a = H[0] * ca * ca + 2.0 * H[1] * ca * sa + H[2] * sa * sa;
+  const double l_b = H[0] * sa * sa - 2.0 * H[1] * ca * sa + H[2] * ca * ca;
+  double e[2][2] = {{ca, sa}, {-sa, ca}};
+  if (l_b < l_a) std::swap(e[0], e[1]);
+  for (int q = 0; q < 2; ++q) {
+    if (its > 600) return false;
+    for (int i = 0; i < E.k; ++i) {
+      const double p = (m_t0[2 * i] - c0) * e[q][0] + (m_t0[2 * i + 1] - c1) * e[q][1];
+      m[2 * i] = c0 + p * e[q][0];
+      m[2 * i + 1] = c1 + p * e[q][1];
+    }
+    if (cl_newton(E, m, 1.0, 40, tol_final, its)) return true;
+  }
+  return false;
+}
+
+// The cluster rows of a map: for every cluster, ext = vdot minus the cluster's
+// own terms at the iterate (v_mid, G_eff of the map), then the root; Ghat's
+// cluster rows <- 2 m - v0 (rows of failed clusters keep Gv).  mroot holds each
+// cluster's last root (2 * cluster size values per cluster, empty = none).
+int cluster_rows(const orc_mesh* ms, const std::vector<int>& off, const std::vector<int>& idx, const double* v0,
+                 const double* w, int64_t N, const Scratch& s, double lam, double dt, double* Ghat,
+                 std::vector<std::vector<double>>& mroot) {
+  int fails = 0;
+  const int ncl = (int)off.size() - 1;
+  for (int c = 0; c < ncl; ++c) {
+    const int k = off[c + 1] - off[c];
+    const int* id = &idx[off[c]];
+    std::vector<double> v0c(2 * k), wc(k), ext(2 * k), m;
+    for (int i = 0; i < k; ++i) {
+      const int a = id[i];
+      v0c[2 * i] = v0[a];
+      v0c[2 * i + 1] = v0[N + a];
+      wc[i] = w[a];
+      double e0 = s.dotv[a], e1 = s.dotv[N + a];
+      for (int j = 0; j < k; ++j) {
+        if (j == i) continue;
+        const int b = id[j];
+        const double d0 = s.vmid[a] - s.vmid[b], d1 = s.vmid[N + a] - s.vmid[N + b];
+        const double y0 = s.Geff[b] - s.Geff[a], y1 = s.Geff[N + b] - s.Geff[N + a];
+        const double qd = d0 * d0 + d1 * d1;
+        if (!(qd >= 1e-24)) continue;
+        const double dy = (d0 * y0 + d1 * y1) / qd, inv = 1.0 / std::sqrt(qd);
+        e0 -= w[b] * (y0 - d0 * dy) * inv;
+        e1 -= w[b] * (y1 - d1 * dy) * inv;
+      }
+      ext[2 * i] = e0;
+      ext[2 * i + 1] = e1;
+    }
+    ClusterEq E{ms, s.L.data(), lam, dt, k, v0c.data(), wc.data(), ext.data()};
+    int its = 0;
+    const bool ok = cluster_root(E, mroot[c].empty() ? nullptr : mroot[c].data(), m, its);
+    if (!ok) {
+      fails++;
+      mroot[c].clear();
+      continue;
+    }
+    mroot[c] = m;
+    for (int i = 0; i < k; ++i) {
+      Ghat[id[i]] = 2.0 * m[2 * i] - v0c[2 * i];
+      Ghat[N + id[i]] = 2.0 * m[2 * i + 1] - v0c[2 * i + 1];
+    }
+  }
+  return fails;
+}
+
+}  // namespace
+
+extern "C" {
+
+// Stiff clusters of the markers v (N x 2): the CSR (off[ncl + 1], idx) of
+// find_links + build_clusters.  Returns ncl, or -1 if the arrays are too small.
+int orc_clusters(const orc_mesh* ms, const double* v, const double* w, int64_t N, double theta, int kmax, int* off,
+                 int off_cap, int* idx, int idx_cap) {
+  double box[4];
+  box_of(ms, box);
+  std::vector<int> o, ix;
+  build_clusters(find_links(box, v, w, N, theta), kmax, o, ix);
+  if ((int)o.size() > off_cap || (int)ix.size() > idx_cap) return -1;
+  std::copy(o.begin(), o.end(), off);
+  std::copy(ix.begin(), ix.end(), idx);
+  return (int)o.size() - 1;
+}
+
+// One cluster's root for given inputs (interleaved v0, ext; mprev may be null):
+// m_out (2k), returns 1 on success, 0 if no root was reached; *its = Newton iterations.
+int orc_cluster_root(const orc_mesh* ms, const double* L, double lam, double dt, int k, const double* v0,
+                     const double* w, const double* ext, const double* mprev, double* m_out, int* its) {
+  ClusterEq E{ms, L, lam, dt, k, v0, w, ext};
+  std::vector<double> m;
+  int it = 0;
+  const bool ok = cluster_root(E, mprev, m, it);
+  if (its) *its = it;
+  if (ok) std::copy(m.begin(), m.end(), m_out);
+  return ok ? 1 : 0;
+}
+
+// The cluster equations' residual at m (interleaved) and t; returns 0 if undefined.
+int orc_cluster_residual(const orc_mesh* ms, const double* L, double lam, double dt, int k, const double* v0,
+                         const double* w, const double* ext, const double* m, double t, double* F, double* J) {
+  ClusterEq E{ms, L, lam, dt, k, v0, w, ext};
+  return E.eval(m, t, F, J) ? 1 : 0;
+}
+
+// grad L and Hess L (H11, H12, H22) at (x, y); returns 0 outside the box
+int orc_grad_hess(const orc_mesh* ms, const double* L, double x, double y, double* g, double* H) {
+  return grad_hess(ms, L, x, y, g, H) ? 1 : 0;
+}
+
+}  // extern "C"
+
+extern "C" {
+
 // solver.jl:300-423 (step_anderson!).  v1 holds the initial guess on entry and
 // the solution on exit.  iter_res (optional, length max_iter) receives ||r_k||.
 void orc_step_anderson(const orc_mesh* ms, const orc_solver_params* Pp, double* v1, const double* v0, const double* w,
@@ -1103,11 +1666,33 @@ void orc_step_anderson(const orc_mesh* ms, const orc_solver_params* Pp, double* 
   int history = 0, slot = 0, n_restart = 0;
   double nrm_r0 = 0.0, nrm_r = 0.0, nrm_best = std::numeric_limits<double>::infinity();
   double nrm_best_window = std::numeric_limits<double>::infinity();
+  // stiff clusters (extension): found once per step on v0; Anderson mixes the
+  // map with the clusters' rows solved exactly (Gh, rh), while the exit, best
+  // iterate and stagnation tests use the plain map
+  std::vector<int> coff, cidx;
+  const bool clusters = P.cl_theta > 0.0 && P.collision_model == 0;
+  if (clusters) {
+    double box[4];
+    box_of(ms, box);
+    build_clusters(find_links(box, v0, w, N, P.cl_theta), std::max(2, P.cl_kmax), coff, cidx);
+  }
+  const int ncl = clusters ? (int)coff.size() - 1 : 0;
+  std::vector<std::vector<double>> mroot(ncl);
+  std::vector<double> Gh(ncl ? n2 : 0), rh(ncl ? n2 : 0);
+  int cl_fail = 0;
   for (int k = 1; k <= P.max_iter; ++k) {
     std::memcpy(vold.data(), v1, sizeof(double) * n2);
-    picard_map(ms, Gv.data(), v1, v0, w, N, S0, P.dt, s, P);
+    const double lam = picard_map(ms, Gv.data(), v1, v0, w, N, S0, P.dt, s, P);
     for (int64_t i = 0; i < n2; ++i) r[i] = Gv[i] - v1[i];
     nrm_r = nrm(r.data(), n2);
+    if (ncl) {
+      std::memcpy(Gh.data(), Gv.data(), sizeof(double) * n2);
+      cl_fail += cluster_rows(ms, coff, cidx, v0, w, N, s, lam, P.dt, Gh.data(), mroot);
+      for (int64_t i = 0; i < n2; ++i) rh[i] = Gh[i] - v1[i];
+    }
+    // the vectors Anderson mixes
+    const std::vector<double>& Gm = ncl ? Gh : Gv;
+    const std::vector<double>& rm = ncl ? rh : r;
     if (iter_res) iter_res[k - 1] = nrm_r;
     if (k == 1) nrm_r0 = nrm_r;
     if (nrm_r < nrm_best) {
@@ -1117,14 +1702,14 @@ void orc_step_anderson(const orc_mesh* ms, const orc_solver_params* Pp, double* 
     const double eff_tol = std::max(P.tol * (nrm(v1, n2) + 1e-30), P.abs_floor);
     if (nrm_r < eff_tol) {
       if (P.exit_picard_step) std::memcpy(v1, Gv.data(), sizeof(double) * n2);
-      *st = {k, n_restart, nrm_r, nrm_r0, 0, 0, 0, 0, 0, 0};
+      *st = {k, n_restart, nrm_r, nrm_r0, 0, 0, 0, 0, 0, 0, ncl, cl_fail};
       return;
     }
     if (k > P.stag_window && k % P.stag_window == 0) {
       const double rel = (nrm_best_window - nrm_best) / (nrm_best_window + 1e-30);
       if (rel < P.stag_rel_tol) {
         std::memcpy(v1, vbest.data(), sizeof(double) * n2);
-        *st = {k, n_restart, nrm_best, nrm_r0, 1, 0, 0, 0, 0, 0};
+        *st = {k, n_restart, nrm_best, nrm_r0, 1, 0, 0, 0, 0, 0, ncl, cl_fail};
         return;
       }
       nrm_best_window = nrm_best;
@@ -1138,15 +1723,15 @@ void orc_step_anderson(const orc_mesh* ms, const orc_solver_params* Pp, double* 
     }
     const double beta = k > P.damp_decay_start ? P.damping * P.damp_decay_factor : P.damping;
     if (k == 1 || just_restarted || !P.use_anderson) {
-      for (int64_t i = 0; i < n2; ++i) v1[i] = beta * Gv[i] + (1 - beta) * vold[i];
+      for (int64_t i = 0; i < n2; ++i) v1[i] = beta * Gm[i] + (1 - beta) * vold[i];
     } else {
       slot = slot % m + 1;
       history = std::min(history + 1, m);
       double* F = &dF[(size_t)(slot - 1) * n2];
       double* Gc = &dG[(size_t)(slot - 1) * n2];
       for (int64_t i = 0; i < n2; ++i) {
-        F[i] = r[i] - rp[i];
-        Gc[i] = Gv[i] - Gp[i];
+        F[i] = rm[i] - rp[i];
+        Gc[i] = Gm[i] - Gp[i];
       }
       const int h = history;
       std::vector<double> ATA(h * h), ATr(h), gam(h);
@@ -1160,7 +1745,7 @@ void orc_step_anderson(const orc_mesh* ms, const orc_solver_params* Pp, double* 
         }
         double t = 0.0;
         const double* Fa = &dF[(size_t)a * n2];
-        for (int64_t i = 0; i < n2; ++i) t += Fa[i] * r[i];
+        for (int64_t i = 0; i < n2; ++i) t += Fa[i] * rm[i];
         ATr[a] = t;
       }
       double dm = 0.0;
@@ -1170,16 +1755,16 @@ void orc_step_anderson(const orc_mesh* ms, const orc_solver_params* Pp, double* 
       for (int j = 0; j < h; ++j) ATA[j * h + j] += l2;
       small_solve(h, ATA, ATr, gam.data());
       for (int64_t i = 0; i < n2; ++i) {
-        double x = Gv[i];
+        double x = Gm[i];
         for (int j = 0; j < h; ++j) x -= dG[(size_t)j * n2 + i] * gam[j];
         v1[i] = beta * x + (1 - beta) * vold[i];
       }
     }
-    std::memcpy(rp.data(), r.data(), sizeof(double) * n2);
-    std::memcpy(Gp.data(), Gv.data(), sizeof(double) * n2);
+    std::memcpy(rp.data(), rm.data(), sizeof(double) * n2);
+    std::memcpy(Gp.data(), Gm.data(), sizeof(double) * n2);
   }
   std::memcpy(v1, vbest.data(), sizeof(double) * n2);
-  *st = {P.max_iter, n_restart, nrm_best, nrm_r0, 2, 0, 0, 0, 0, 0};
+  *st = {P.max_iter, n_restart, nrm_best, nrm_r0, 2, 0, 0, 0, 0, 0, ncl, cl_fail};
 }
 
 }  // extern "C"
@@ -1317,13 +1902,13 @@ void orc_step_defect(const orc_mesh* ms, const orc_solver_params* Pp, double* v1
     }
     if (nr < std::max(P.tol * (nrm(v1, n2) + 1e-30), P.abs_floor)) {
       std::memcpy(v1, Gv.data(), sizeof(double) * n2);
-      *st = {k, 0, nr, nrm0, 0, 0, 0, 0, 0, inner};
+      *st = {k, 0, nr, nrm0, 0, 0, 0, 0, 0, inner, 0, 0};
       return;
     }
     if (k > P.dc_stag_window && k % P.dc_stag_window == 0) {
       if ((nrm_best_window - nrm_best) / (nrm_best_window + 1e-30) < P.stag_rel_tol) {
         std::memcpy(v1, vbest.data(), sizeof(double) * n2);
-        *st = {k, 0, nrm_best, nrm0, 1, 0, 0, 0, 0, inner};
+        *st = {k, 0, nrm_best, nrm0, 1, 0, 0, 0, 0, inner, 0, 0};
         return;
       }
       nrm_best_window = nrm_best;
@@ -1390,7 +1975,7 @@ void orc_step_defect(const orc_mesh* ms, const orc_solver_params* Pp, double* v1
     }
   }
   std::memcpy(v1, vbest.data(), sizeof(double) * n2);
-  *st = {P.max_iter, 0, nrm_best, nrm0, 2, 0, 0, 0, 0, inner};
+  *st = {P.max_iter, 0, nrm_best, nrm0, 2, 0, 0, 0, 0, inner, 0, 0};
 }
 
 // One full time step of main.jl:226-307 (Landau, Anderson): predictor,
@@ -1646,6 +2231,8 @@ void orc_step_newton(const orc_mesh* ms, const orc_solver_params* Pp, double* v1
   st->r0 = r0;
   st->exit_reason = status;
   st->inner = 0;
+  st->n_clusters = 0;
+  st->cl_fail = 0;
 }
 
 // v (in/out), fc (in: projection of v; out: projection of the new v),

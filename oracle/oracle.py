@@ -57,6 +57,8 @@ class SolverParams(C.Structure):
         ("nk_eta_max", C.c_double),
         ("nk_fd_h", C.c_double),
         ("nk_fd_rel", C.c_double),
+        ("cl_theta", C.c_double),
+        ("cl_kmax", C.c_int),
     ]
 
 
@@ -72,6 +74,8 @@ class StepStats(C.Structure):
         ("p1", C.c_double),
         ("p2", C.c_double),
         ("inner", C.c_int),
+        ("n_clusters", C.c_int),
+        ("cl_fail", C.c_int),
     ]
 
 
@@ -84,7 +88,7 @@ def default_params(dt: float, m: int = 5, nthreads: int = 0, **kw) -> SolverPara
         reg_factor=1e-10, use_anderson=1, use_gonzalez=1, exit_picard_step=1,
         use_logsq=0, nthreads=nthreads, solver=0, dc_eta=0.05, dc_max_inner=60, dc_stag_window=5,
         dc_outer_m=0, dc_near_theta=0.0, collision_model=0, nu=1.0,
-        nk_krylov_max=30, nk_eta_max=0.9, nk_fd_h=0.0, nk_fd_rel=0.0,
+        nk_krylov_max=30, nk_eta_max=0.9, nk_fd_h=0.0, nk_fd_rel=0.0, cl_theta=0.0, cl_kmax=32,
     )
     for k, v in kw.items():
         setattr(p, k, v)
@@ -110,7 +114,8 @@ def lib():
                      "orc_negative_part_l1", "orc_fs_minus_fp_l2", "orc_lb_rhs", "orc_lb_geff", "orc_gmres_dense", "orc_newton_midpoint",
                      "orc_step_newton",
                      "orc_compute_r", "orc_compute_G", "orc_loggrad", "orc_landau_geff",
-                     "orc_picard_map", "orc_predictor", "orc_step_anderson", "orc_step_defect", "orc_time_step"):
+                     "orc_picard_map", "orc_predictor", "orc_step_anderson", "orc_step_defect", "orc_time_step",
+                     "orc_clusters", "orc_cluster_root", "orc_cluster_residual", "orc_grad_hess"):
             getattr(L, name).argtypes = None
         _LIB = L
     return _LIB
@@ -313,6 +318,53 @@ class Mesh:
         lib().orc_step_anderson(self.h, C.byref(params), _p(v1), _p(v0), _p(w), C.c_int64(v0.shape[0]),
                                 C.c_double(S0), C.byref(st), _p(res))
         return v1, st, res[: st.iters]
+
+    def clusters(self, v, w, theta: float, kmax: int = 32):
+        """Stiff clusters of markers v (extension): list of sorted index arrays."""
+        v = colmajor(v)
+        w = np.ascontiguousarray(w, dtype=np.float64)
+        N = v.shape[0]
+        off = np.zeros(N + 1, dtype=np.int32)
+        idx = np.zeros(N, dtype=np.int32)
+        n = lib().orc_clusters(self.h, _p(v), _p(w), C.c_int64(N), C.c_double(theta), C.c_int(kmax),
+                               off.ctypes.data_as(_ip), C.c_int(N + 1), idx.ctypes.data_as(_ip), C.c_int(N))
+        if n < 0:
+            raise ValueError("cluster arrays too small")
+        return [idx[off[c]:off[c + 1]].copy() for c in range(n)]
+
+    def grad_hess(self, L, x: float, y: float):
+        """(grad L, (H11, H12, H22)) at (x, y), or None outside the box."""
+        L = np.ascontiguousarray(L, dtype=np.float64)
+        g, H = np.zeros(2), np.zeros(3)
+        ok = lib().orc_grad_hess(self.h, _p(L), C.c_double(x), C.c_double(y), _p(g), _p(H))
+        return (g, H) if ok else None
+
+    def cluster_residual(self, L, lam, dt, v0c, wc, ext, m, t=1.0, jacobian=False):
+        """The cluster equations Phi(m; t) (interleaved (k, 2) arrays) and optionally their Jacobian."""
+        v0c, ext, m = (np.ascontiguousarray(a, dtype=np.float64).ravel() for a in (v0c, ext, m))
+        wc = np.ascontiguousarray(wc, dtype=np.float64)
+        k = wc.shape[0]
+        F = np.zeros(2 * k)
+        J = np.zeros((2 * k, 2 * k)) if jacobian else None
+        ok = lib().orc_cluster_residual(self.h, _p(np.ascontiguousarray(L, dtype=np.float64)), C.c_double(lam),
+                                        C.c_double(dt), C.c_int(k), _p(v0c), _p(wc), _p(ext), _p(m), C.c_double(t),
+                                        _p(F), _p(J) if jacobian else None)
+        if not ok:
+            return None
+        return (F, J) if jacobian else F
+
+    def cluster_root(self, L, lam, dt, v0c, wc, ext, mprev=None):
+        """One cluster's midpoint root (k, 2), or None; and the Newton iteration count."""
+        v0c, ext = (np.ascontiguousarray(a, dtype=np.float64).ravel() for a in (v0c, ext))
+        wc = np.ascontiguousarray(wc, dtype=np.float64)
+        k = wc.shape[0]
+        m = np.zeros(2 * k)
+        its = C.c_int(0)
+        mp = None if mprev is None else np.ascontiguousarray(mprev, dtype=np.float64).ravel()
+        ok = lib().orc_cluster_root(self.h, _p(np.ascontiguousarray(L, dtype=np.float64)), C.c_double(lam),
+                                    C.c_double(dt), C.c_int(k), _p(v0c), _p(wc), _p(ext),
+                                    _p(mp) if mp is not None else None, _p(m), C.byref(its))
+        return (m.reshape(k, 2) if ok else None), its.value
 
     def step_defect(self, params: SolverParams, v1, v0, w, S0):
         """step_defect! (solver.jl:569-634); returns (v1, stats, outer residuals)."""
