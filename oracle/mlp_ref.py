@@ -29,16 +29,18 @@ C0 = np.sqrt(2.0 / np.pi)
 
 
 def unpack(p):
+    """Parameter blocks; float64 parameters stay float64 (the exact path), others become float32."""
+    dt = np.float64 if np.asarray(p).dtype == np.float64 else np.float32
     out, o = {}, 0
     for name, shape in SIZES:
         n = int(np.prod(shape))
-        out[name] = np.asarray(p[o:o + n], dtype=np.float32).reshape(shape)
+        out[name] = np.asarray(p[o:o + n], dtype=dt).reshape(shape)
         o += n
     return out
 
 
-def pack(d):
-    return np.concatenate([np.asarray(d[name], dtype=np.float32).reshape(-1) for name, _ in SIZES])
+def pack(d, dtype=np.float32):
+    return np.concatenate([np.asarray(d[name], dtype=dtype).reshape(-1) for name, _ in SIZES])
 
 
 def bf16(a):
@@ -75,15 +77,23 @@ def init(seed: int = 0):
     return pack(d)
 
 
+def _rounding(emulate_bf16):
+    """Operand rounding: bfloat16 (the device's matrix-core operands) or none."""
+    return bf16 if emulate_bf16 else (lambda a: np.asarray(a, np.float64))
+
+
 def forward(p, x, emulate_bf16=True):
-    """x: (B, 6) float32 -> y (B, 2) and the cache for backward (float64 math)."""
+    """x: (B, 6) -> y (B, 2) and the cache for backward (float64 math).  With
+    emulate_bf16=False nothing is rounded: float64 parameters and inputs give
+    the exact network."""
     P = unpack(p)
-    q = bf16 if emulate_bf16 else (lambda a: np.asarray(a, np.float32))
+    q = _rounding(emulate_bf16)
+    c32 = (lambda a: a.astype(np.float32)) if emulate_bf16 else (lambda a: a)  # noqa: E731
     xb = q(x).astype(np.float64)
     z1 = xb @ q(P["W1"]).astype(np.float64).T + P["b1"]
-    h1 = q(gelu(z1).astype(np.float32)).astype(np.float64)
+    h1 = q(c32(gelu(z1))).astype(np.float64)
     z2 = h1 @ q(P["W2"]).astype(np.float64).T + P["b2"]
-    h2 = q(gelu(z2).astype(np.float32)).astype(np.float64)
+    h2 = q(c32(gelu(z2))).astype(np.float64)
     y = h2 @ q(P["W3"]).astype(np.float64).T + P["b3"]
     return y, (xb, z1, h1, z2, h2)
 
@@ -95,31 +105,34 @@ def loss_and_grad(p, x, t, emulate_bf16=True):
     bfloat16 (weights, x, h1, h2, dy, dz2, dz1), products accumulate in FP32
     (here FP64), element-wise factors and bias gradients stay unrounded."""
     P = unpack(p)
-    q = bf16 if emulate_bf16 else (lambda a: np.asarray(a, np.float32))
+    q = _rounding(emulate_bf16)
+    c32 = (lambda a: a.astype(np.float32)) if emulate_bf16 else (lambda a: a)  # noqa: E731
     f64 = lambda a: np.asarray(a, dtype=np.float64)  # noqa: E731
     y, (xb, z1, h1, z2, h2) = forward(p, x, emulate_bf16)
     B = x.shape[0]
     e = y - t
     loss = float(np.sum(e * e) / B)
     dy = 2.0 * e / B
-    dyb = f64(q(dy.astype(np.float32)))
+    dyb = f64(q(c32(dy)))
     g = {}
     g["W3"] = dyb.T @ h2
     g["b3"] = dy.sum(axis=0)
     dh2 = dyb @ f64(q(P["W3"]))
     dz2 = dh2 * gelu_grad(z2)
-    dz2b = f64(q(dz2.astype(np.float32)))
+    dz2b = f64(q(c32(dz2)))
     g["W2"] = dz2b.T @ h1
     g["b2"] = dz2.sum(axis=0)
     dh1 = dz2b @ f64(q(P["W2"]))
     dz1 = dh1 * gelu_grad(z1)
-    dz1b = f64(q(dz1.astype(np.float32)))
+    dz1b = f64(q(c32(dz1)))
     g["W1"] = dz1b.T @ xb
     g["b1"] = dz1.sum(axis=0)
-    return loss, pack({k: v.astype(np.float32) for k, v in g.items()})
+    out_dt = np.float64 if np.asarray(p).dtype == np.float64 else np.float32
+    return loss, pack(g, out_dt)
 
 
 def adam(p, g, m, v, step, lr=1e-3, b1=0.9, b2=0.999, eps=1e-8):
+    """One Adam step (Kingma & Ba 2015, Algorithm 1) in float32, as the device."""
     m = b1 * m + (1 - b1) * g
     v = b2 * v + (1 - b2) * g * g
     mh = m / (1 - b1 ** step)

@@ -59,8 +59,9 @@ def test_forward_loss_grad_match_numpy(wsm):
                      ("b2", slice(4544, 4608)), ("W3", slice(4608, 4736)), ("b3", slice(4736, 4738))]:
         e = np.max(np.abs(g[sl] - g_ref[sl])) / np.max(np.abs(g_ref[sl]))
         assert e < 2e-2, (name, e)
-    # against exact (unrounded) finite differences the bf16 gradient is close too
-    _, g64 = M.loss_and_grad(p, x, t, emulate_bf16=False)
+    # the exact (unrounded float64) gradient of the restatement, itself pinned by
+    # finite differences (tests/test_oracle_mlp.py), points the same way
+    _, g64 = M.loss_and_grad(p.astype(np.float64), x.astype(np.float64), t.astype(np.float64), emulate_bf16=False)
     assert np.dot(g, g64) / (np.linalg.norm(g) * np.linalg.norm(g64)) > 0.995
 
 
@@ -84,6 +85,40 @@ def test_adam_lowers_loss(wsm):
     prm.warmstart = collop.WARM_MLP
     st = sim.run(prm, 1)[0]
     assert st["exit_reason"] == 0
+
+
+def test_device_adam_matches_oracle(gpu_ctx):
+    """Three Adam steps of collop_mlp_train on a one-tile dataset (16 samples,
+    batch 16: one step per epoch) against mlp_ref.adam fed the device's own
+    gradient of that tile (collop_mlp_eval), so that the update rule is what is
+    compared: moments, bias corrections and the parameter update."""
+    bp = uniform_bp(-6, 6, 16)
+    ws = collop.Workspace(bp, bp, ctx=gpu_ctx)
+    w = collop.WarmStart(ws)
+    sim = collop.Simulation(ws, *collop.markers("gaussian", 16, 3))
+    w.dataset_alloc(16)
+    prm = collop.config("gaussian_n4096", DT=0.01).step_params()
+    prm.record_training = 1
+    sim.run(prm, 1)
+    assert w.dataset_size() == 16
+    rec = w.dataset_read(0, 16)
+    p = M.init(4)
+    w.set_params(p)
+    w.fit_norm()
+    m = np.zeros(M.NPARAM, np.float32)
+    v = np.zeros(M.NPARAM, np.float32)
+    lr = 1e-3
+    for step in range(1, 4):
+        _, _, g = w.eval(rec, 2)
+        p_ref, m, v = M.adam(p, g.astype(np.float32), m, v, step, lr=lr)
+        w.train(epochs=1, batch=16, lr=lr, seed=step)
+        p_dev = w.params()
+        err = float(np.max(np.abs(p_dev - p_ref)))
+        moved = float(np.max(np.abs(p_ref - p)))
+        print(f"Adam step {step}: max |device - oracle| {err:.2e} (step size {moved:.2e})")
+        assert moved > 0.5 * lr
+        assert err <= 1e-3 * lr
+        p = p_dev
 
 
 def test_trained_warm_start_gain(gpu_ctx):
