@@ -107,7 +107,7 @@ class StepStats(C.Structure):
         ("inner_maps", C.c_int32),
         ("n_clusters", C.c_int32),
         ("cl_fail", C.c_int32),
-        ("pad", C.c_int32),
+        ("stat_syncs", C.c_int32),
     ]
 
     def as_dict(self) -> dict:
@@ -117,6 +117,7 @@ class StepStats(C.Structure):
             "entropy": self.entropy, "energy": self.energy,
             "momentum": (self.momentum[0], self.momentum[1]), "seconds": self.seconds,
             "inner_maps": self.inner_maps, "n_clusters": self.n_clusters, "cl_fail": self.cl_fail,
+            "stat_syncs": self.stat_syncs,
         }
 
 

@@ -149,18 +149,44 @@ CONFIGS = {
 }
 
 
-# The chosen path: FP32-compensated outer maps then FP64 outer maps, frozen-metric
-# defect correction (step_defect!) with the extensions of this library (an
-# outer Anderson window of 3 and exact near pairs with w / |d| > 0.02), the
-# FP64 outer maps reusing the metric A of the last FP32c outer map.
-CHOSEN = dict(gpu_fp32=True, solver="defect", dc_outer_m=3, dc_near_theta=0.02, dc_metric_reuse=1)
+# The defect-correction path: FP32-compensated outer maps then FP64 outer maps,
+# frozen-metric defect correction (step_defect!) with the extensions of this
+# library (an outer Anderson window of 3 and exact near pairs with w / |d| >
+# 0.02), the FP64 outer maps reusing the metric A of the last FP32c outer map.
+DEFECT = dict(gpu_fp32=True, solver="defect", dc_outer_m=3, dc_near_theta=0.02, dc_metric_reuse=1)
+
+# The chosen path: FP32-compensated Anderson maps until the FP32 floor, then FP64
+# maps (step_anderson!), with stiff-cluster elimination: markers closer than
+# max(w_i, w_j) / |d| > CLUSTER_TAU / dt at the start of a step form clusters
+# whose rows every FP64 map solves exactly.  The threshold keeps dt w / |d|
+# fixed, the factor the stiffness of a pair scales with.
+CLUSTER_TAU = 0.005
+CHOSEN = dict(gpu_fp32=True, solver="anderson", cl_kmax=32)
 
 
 def chosen_config(name: str, **over) -> SimParameters:
     """config(name) with the chosen path's solver knobs (CHOSEN), then `over`."""
     kw = dict(CHOSEN)
     kw.update(over)
+    p = config(name, **kw)
+    if "cl_theta" not in over:
+        p = dataclasses.replace(p, cl_theta=CLUSTER_TAU / p.DT)
+    return p
+
+
+def defect_config(name: str, **over) -> SimParameters:
+    """config(name) with the defect-correction path's knobs (DEFECT), then `over`."""
+    kw = dict(DEFECT)
+    kw.update(over)
     return config(name, **kw)
+
+
+def chosen_description(name: str, mlp: bool) -> str:
+    p = chosen_config(name)
+    return ("FP32c maps until the FP32 rounding floor, then FP64 maps, Anderson (step_anderson!) with stiff-cluster "
+            f"elimination (max(w_i, w_j) / |d| > {p.cl_theta:g} at the step's start, clusters of <= {p.cl_kmax} "
+            "markers solved exactly in every FP64 map), FP32c Euler predictor"
+            + (" + learned MLP warm start" if mlp else "") + "; symmetric pair sum (each unordered pair once)")
 
 
 def config(name: str, **over) -> SimParameters:

@@ -249,7 +249,9 @@ void prof_collect(collop_ctx* ctx) {
 // The map's Anderson statistics: one kernel reduces the epilogue partials into
 // the pinned h_stat and then publishes a new sequence number; the host polls
 // that number instead of synchronising the stream, a round trip about 5 us
-// shorter.  After 20 ms (long pair sums), or when profiling, it synchronises.
+// shorter (both live in coherent pinned memory, so the device's writes reach the
+// host without a synchronisation).  After 20 ms (long maps), or when profiling,
+// it synchronises; stat_syncs counts the polls that ran out.
 int publish_stat(collop_ctx* ctx, int nparts) {
   launch_stat_publish(P<double>(ctx->b_statp), nparts, ctx->h_stat_dev, ctx->h_seq_dev, ++ctx->seq, ctx->stream);
   CHECK_LAUNCH();
@@ -259,6 +261,7 @@ int publish_stat(collop_ctx* ctx, int nparts) {
       if (__atomic_load_n(ctx->h_seq, __ATOMIC_ACQUIRE) == ctx->seq) return COLLOP_OK;
       if ((it & 255) == 0 && std::chrono::steady_clock::now() - t0 > std::chrono::milliseconds(20)) break;
     }
+    ctx->stat_syncs++;
   }
   HIPC(hipStreamSynchronize(ctx->stream));
   if (__atomic_load_n(ctx->h_seq, __ATOMIC_ACQUIRE) != ctx->seq)
@@ -331,13 +334,13 @@ int collop_create(int device, collop_ctx** out) {
     delete c;
     return COLLOP_ERR_HIP;
   }
-  if (hipHostMalloc((void**)&c->h_stat, kStatWidth * sizeof(double), hipHostMallocDefault) != hipSuccess) {
+  if (hipHostMalloc((void**)&c->h_stat, kStatWidth * sizeof(double), hipHostMallocCoherent) != hipSuccess) {
     (void)hipStreamDestroy(c->stream);
     delete c;
     return COLLOP_ERR_HIP;
   }
   if (hipHostGetDevicePointer((void**)&c->h_stat_dev, c->h_stat, 0) != hipSuccess ||
-      hipHostMalloc((void**)&c->h_seq, 64, hipHostMallocDefault) != hipSuccess) {
+      hipHostMalloc((void**)&c->h_seq, 64, hipHostMallocCoherent) != hipSuccess) {
     (void)hipHostFree(c->h_stat);
     (void)hipStreamDestroy(c->stream);
     delete c;
@@ -372,8 +375,8 @@ int collop_destroy(collop_ctx* ctx) {
                   &ctx->b_ntg, &ctx->b_noff, &ctx->b_nal, &ctx->b_nidx, &ctx->s_v1n, &ctx->m_params, &ctx->m_adam_m, &ctx->m_adam_v, &ctx->m_data, &ctx->m_rec,
                   &ctx->m_y, &ctx->m_gpart, &ctx->m_lpart, &ctx->m_tiles, &ctx->m_eval, &ctx->m_grad,
                   &ctx->b_cell_of, &ctx->b_cell_cnt, &ctx->b_cell_start, &ctx->b_cell_sorted, &ctx->b_links,
-                  &ctx->b_link_count, &ctx->b_cl_off, &ctx->b_cl_idx, &ctx->b_cl_row, &ctx->b_cl_x, &ctx->b_cl_true,
-                  &ctx->b_cl_root, &ctx->b_cl_have, &ctx->b_cl_fail, &ctx->b_cl_its})
+                  &ctx->b_link_count, &ctx->b_cl_off, &ctx->b_cl_idx, &ctx->b_cl_row, &ctx->b_cl_x,
+                  &ctx->b_cl_root, &ctx->b_cl_have, &ctx->b_cl_fail, &ctx->b_cl_its, &ctx->b_cl_next})
     release(*b);
   for (auto& e : ctx->ev_pool) {
     (void)hipEventDestroy(e.first);
@@ -931,7 +934,6 @@ int epilogue_dev(collop_ctx* ctx, const double* part, int S, const double* v0e, 
   if (clusters) {
     a.cl_row = P<int>(ctx->b_cl_row);
     a.cl_x = P<double>(ctx->b_cl_x);
-    a.cl_true = P<double>(ctx->b_cl_true);
   }
   a.part = part;
   a.S = S;
@@ -1052,6 +1054,9 @@ int build_clusters_dev(collop_ctx* ctx, double theta, int kmax) {
   CHECK_LAUNCH();
   launch_fill_int(P<int>(ctx->b_cl_fail), 1, 0, ctx->stream);
   CHECK_LAUNCH();
+  launch_fill_int(P<int>(ctx->b_cl_next), ctx->ncl, 0, ctx->stream);
+  CHECK_LAUNCH();
+  ctx->cl_map = 0;
   // the host lists must outlive the asynchronous copies
   HIPC(hipStreamSynchronize(ctx->stream));
   return COLLOP_OK;
@@ -1080,6 +1085,8 @@ int cluster_solve_dev(collop_ctx* ctx, const collop_step_params& p, const double
   a.x_out = P<double>(ctx->b_cl_x);
   a.fail_count = P<int>(ctx->b_cl_fail);
   a.newton_its = P<int>(ctx->b_cl_its);
+  a.next_try = P<int>(ctx->b_cl_next);
+  a.map_k = ctx->cl_map++;
   launch_cluster_solve(a, ctx->ncl, ctx->stream);
   CHECK_LAUNCH();
   return COLLOP_OK;
@@ -1758,6 +1765,7 @@ int defect_solve(collop_ctx* ctx, const collop_step_params& p, collop_step_stats
 // step_anderson!, solver.jl:300-423).
 int step_once(collop_ctx* ctx, const collop_step_params& p, collop_step_stats* out) {
   const auto t0 = std::chrono::steady_clock::now();
+  const long syncs0 = ctx->stat_syncs;
   const int n = (int)ctx->N;
   const size_t n2b = (size_t)2 * n * 8;
   int r;
@@ -1854,15 +1862,11 @@ int step_once(collop_ctx* ctx, const collop_step_params& p, collop_step_stats* o
         (r = copy_d2d(ctx, ctx->s_best, p.exit_picard_step ? ctx->s_Gv : ctx->s_v1, n2b)))
       return r;
     if (act.exit == kConverged) {
-      if (p.exit_picard_step) {
-        std::swap(ctx->s_v1, ctx->s_Gv);
-        // the exit map is the plain map G(v) (solver.jl:351): its cluster rows too
-        if (ctx->ncl > 0 && pl.kind == 0) {
-          launch_cluster_fix(P<double>(ctx->s_v1), n, P<int>(ctx->b_cl_idx), ctx->cl_total,
-                             P<double>(ctx->b_cl_true), ctx->stream);
-          CHECK_LAUNCH();
-        }
-      }
+      // the exit Picard step (solver.jl:351).  With stiff clusters it is the
+      // map Anderson mixes: its cluster rows are the clusters' exact solutions
+      // at the final iterate, where the plain map would amplify the final
+      // residual by the clusters' stiffness.
+      if (p.exit_picard_step) std::swap(ctx->s_v1, ctx->s_Gv);
       break;
     }
     if (act.exit != kContinue) {
@@ -1908,6 +1912,7 @@ int step_once(collop_ctx* ctx, const collop_step_params& p, collop_step_stats* o
     }
     out->n_clusters = ctx->ncl;
     out->cl_fail = cl_fail;
+    out->stat_syncs = (int32_t)(ctx->stat_syncs - syncs0);
     out->entropy = sc[0];
     out->energy = 0.5 * sc[5];
     out->momentum[0] = sc[3];
@@ -1966,10 +1971,10 @@ int collop_state_set(collop_ctx* ctx, const double* v, const double* w, int64_t 
   ENSURE(ctx->b_cl_idx, (size_t)kClMembersCap * 4);
   ENSURE(ctx->b_cl_off, (size_t)(kClMembersCap / 2 + 1) * 4);
   ENSURE(ctx->b_cl_x, (size_t)2 * kClMembersCap * 8);
-  ENSURE(ctx->b_cl_true, (size_t)2 * kClMembersCap * 8);
   ENSURE(ctx->b_cl_root, (size_t)2 * kClMembersCap * 8);
   ENSURE(ctx->b_cl_have, (size_t)(kClMembersCap / 2) * 4);
   ENSURE(ctx->b_cl_its, (size_t)(kClMembersCap / 2) * 4);
+  ENSURE(ctx->b_cl_next, (size_t)(kClMembersCap / 2) * 4);
   ENSURE(ctx->b_cl_fail, 64);
   ENSURE(ctx->b_cl_row, (size_t)N * 4);
   launch_fill_int(P<int>(ctx->b_cl_row), (int)N, -1, ctx->stream);

@@ -19,10 +19,12 @@
 //   k_near_cells: one workgroup per cell tests the cell's pairs and, for
 //     markers within reach of a cell edge, the pairs across it (FP64);
 //   k_cluster_solve: one wavefront per cluster (<= 32 markers, 64 unknowns):
-//     the homotopy in the coupling strength t from t = 0 (m = v0 + dt/2 ext)
-//     to t = 1, Newton with the analytic Jacobian and partial-pivot LU in LDS
-//     at every point; a warm start from the cluster's previous root first, and
-//     Newton from the eigen-line configurations of Hess L if the path folds.
+//     a pair is one unknown angle (grid of 64 angles, one per lane, then
+//     bisection); larger clusters take the homotopy in the coupling strength t
+//     from t = 0 (m = v0 + dt/2 ext) to t = 1, Newton with the analytic
+//     Jacobian and partial-pivot LU in LDS at every point; a warm start from
+//     the cluster's previous root first, and Newton from the eigen-line
+//     configurations of Hess L if the path folds.
 #include "collop_launch.hpp"
 
 namespace collop {
@@ -150,15 +152,6 @@ __global__ __launch_bounds__(256) void k_near_cells(MeshView m, const double* __
 __global__ void k_cluster_rows(const int* __restrict__ idx, int total, int set, int* __restrict__ row) {
   const int r = blockIdx.x * blockDim.x + threadIdx.x;
   if (r < total) row[idx[r]] = set ? r : -1;
-}
-
-// v[idx[r]], v[N + idx[r]] <- x[2 r], x[2 r + 1]
-__global__ void k_cluster_fix(double* __restrict__ v, int N, const int* __restrict__ idx, int total,
-                              const double* __restrict__ x) {
-  const int r = blockIdx.x * blockDim.x + threadIdx.x;
-  if (r >= total) return;
-  v[idx[r]] = x[2 * r];
-  v[N + idx[r]] = x[2 * r + 1];
 }
 
 // ---- the cluster solve: one wavefront per cluster --------------------------------
@@ -384,6 +377,104 @@ __device__ bool cl_newton(ClusterLds& S, const ClusterArgs& A, int k, double lam
   return nf < tol;
 }
 
+// ---- two-marker clusters: one unknown ---------------------------------------------
+// With W = w_g + w_a the weighted centre c of (m_g, m_a) is explicit and
+// d = m_g - m_a solves d = a + dt/2 W U(d) (g_a - g_g), a = v0_g - v0_a +
+// dt/2 (ext_g - ext_a).  U(d) y is orthogonal to d, so d . (d - a) = 0: d lies
+// on the circle over a, d = |a| cos(psi) e(phi_a + psi), |psi| < pi/2, and
+//   F(psi) = -|a|^2 / 2 sin(2 psi) + dt/2 W e_perp . (g_a - g_g) = 0.
+// The root taken is the one nearest psi_ref: sign changes of F on a fixed grid
+// of 64 angles (one per lane), then bisection.
+struct PairGeo {
+  double c0, c1, a0, a1, na, pa, W;
+};
+
+__device__ PairGeo pair_geo(const ClusterLds& S, double dt) {
+  PairGeo G;
+  const double wg = S.w[0], wa = S.w[1];
+  G.W = wg + wa;
+  G.c0 = (wg * S.v0[0] + wa * S.v0[2]) / G.W + 0.5 * dt * (wg * S.ext[0] + wa * S.ext[2]) / G.W;
+  G.c1 = (wg * S.v0[1] + wa * S.v0[3]) / G.W + 0.5 * dt * (wg * S.ext[1] + wa * S.ext[3]) / G.W;
+  G.a0 = S.v0[0] - S.v0[2] + 0.5 * dt * (S.ext[0] - S.ext[2]);
+  G.a1 = S.v0[1] - S.v0[3] + 0.5 * dt * (S.ext[1] - S.ext[3]);
+  G.na = sqrt(G.a0 * G.a0 + G.a1 * G.a1);
+  G.pa = atan2(G.a1, G.a0);
+  return G;
+}
+
+__device__ __forceinline__ void pair_mid(const PairGeo& G, const ClusterLds& S, double psi, double* m) {
+  const double ph = G.pa + psi, rho = G.na * cos(psi);
+  const double d0 = rho * cos(ph), d1 = rho * sin(ph);
+  m[0] = G.c0 + (S.w[1] / G.W) * d0;
+  m[1] = G.c1 + (S.w[1] / G.W) * d1;
+  m[2] = G.c0 - (S.w[0] / G.W) * d0;
+  m[3] = G.c1 - (S.w[0] / G.W) * d1;
+}
+
+// F(psi) for this lane's psi; false outside the box
+__device__ bool pair_F(const PairGeo& G, const ClusterLds& S, const ClusterArgs& A, double lam, double psi,
+                       double& f) {
+  double m[4], g[4], H[3];
+  pair_mid(G, S, psi, m);
+  for (int i = 0; i < 2; ++i) {
+    if (!grad_hess_dev(A.m, S.sb1, S.sb2, A.L, m[2 * i], m[2 * i + 1], &g[2 * i], H)) return false;
+    g[2 * i] -= 2.0 * lam * (m[2 * i] - S.v0[2 * i]) / S.w[i];
+    g[2 * i + 1] -= 2.0 * lam * (m[2 * i + 1] - S.v0[2 * i + 1]) / S.w[i];
+  }
+  const double ph = G.pa + psi;
+  f = -0.5 * G.na * G.na * sin(2.0 * psi) + 0.5 * A.dt * G.W * (-sin(ph) * (g[2] - g[0]) + cos(ph) * (g[3] - g[1]));
+  return true;
+}
+
+// The pair's root into S.m (m_g, m_a interleaved); wave-uniform result
+__device__ bool pair_solve(ClusterLds& S, const ClusterArgs& A, double lam, double psi_ref) {
+  const int lane = threadIdx.x;
+  const PairGeo G = pair_geo(S, A.dt);
+  if (!(G.na > 0.0)) return false;
+  const double pi = 3.141592653589793;
+  const double psi_j = -0.5 * pi + pi * (lane + 0.5) / 64.0;
+  double fj = 0.0;
+  const bool okj = pair_F(G, S, A, lam, psi_j, fj);
+  if (__any(!okj)) return false;
+  const double fn = __shfl_down(fj, 1, 64);
+  // candidate bracket [psi_j, psi_j+1]: a sign change (or a zero at psi_j)
+  double dd = 1e300;
+  if (lane < 63 && !((fj < 0.0) == (fn < 0.0) && fj != 0.0)) {
+    const double md = 0.5 * (psi_j + (-0.5 * pi + pi * (lane + 1.5) / 64.0));
+    dd = fabs(md - psi_ref);
+  }
+  int best = lane;
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) {
+    const double od = __shfl_xor(dd, off, 64);
+    const int ob = __shfl_xor(best, off, 64);
+    if (od < dd || (od == dd && ob < best)) {
+      dd = od;
+      best = ob;
+    }
+  }
+  if (!(dd < 1e300)) return false;
+  double lo = -0.5 * pi + pi * (best + 0.5) / 64.0, hi = -0.5 * pi + pi * (best + 1.5) / 64.0;
+  double flo = __shfl(fj, best, 64);
+  for (int it = 0; it < 200 && hi - lo > 1e-15; ++it) {
+    const double md = 0.5 * (lo + hi);
+    double fm = 0.0;
+    const bool okm = pair_F(G, S, A, lam, md, fm);
+    if (!okm) return false;  // every lane evaluates the same point
+    if ((fm < 0.0) == (flo < 0.0) && fm != 0.0) {
+      lo = md;
+      flo = fm;
+    } else {
+      hi = md;
+    }
+  }
+  double m[4];
+  pair_mid(G, S, 0.5 * (lo + hi), m);
+  if (lane < 4) S.m[lane] = m[lane];
+  __syncthreads();
+  return true;
+}
+
 __global__ __launch_bounds__(64) void k_cluster_solve(ClusterArgs A) {
   __shared__ ClusterLds S;
   const int c = blockIdx.x, lane = threadIdx.x;
@@ -438,12 +529,27 @@ __global__ __launch_bounds__(64) void k_cluster_solve(ClusterArgs A) {
   int its = 0;
   bool ok = false;
   double* mr = A.mroot + 2 * o;
-  if (A.have_root[c]) {
+  // have_root: 1 = a root of an earlier map, 0 = none, -f = f failed solves this
+  // step.  After the f-th failure the cluster keeps the plain rows for 2^f maps
+  // (at most 32) before it is tried again, from a map closer to the root.
+  const int state = A.have_root[c];
+  const bool skip = state < 0 && A.map_k < A.next_try[c];
+  if (k == 2 && !skip) {  // the one-unknown form; the reference angle is the previous root's
+    double psi_ref = 0.0;
+    if (state > 0) {
+      const PairGeo G = pair_geo(S, A.dt);
+      double r = atan2(mr[1] - mr[3], mr[0] - mr[2]) - G.pa;
+      while (r > 3.141592653589793) r -= 2.0 * 3.141592653589793;
+      while (r < -3.141592653589793) r += 2.0 * 3.141592653589793;
+      if (fabs(r) < 0.5 * 3.141592653589793) psi_ref = r;
+    }
+    ok = pair_solve(S, A, lam, psi_ref);
+  } else if (state > 0) {
     if (lane < n) S.m[lane] = mr[lane];
     __syncthreads();
     ok = cl_newton(S, A, k, lam, S.m, 1.0, 10, tol_final, its);
   }
-  if (!ok) {
+  if (!ok && !skip && k > 2) {
     // the homotopy in the coupling strength from t = 0: the cluster drifts with ext alone
     if (lane < n) {
       S.m[lane] = S.v0[lane] + 0.5 * A.dt * S.ext[lane];
@@ -454,7 +560,7 @@ __global__ __launch_bounds__(64) void k_cluster_solve(ClusterArgs A) {
     double t = 0.0, tp = 0.0, dtau = 0.05;
     bool have_prev = false, path_ok = true;
     while (t < 1.0) {
-      if (its > 400) {
+      if (its > 200) {
         path_ok = false;
         break;
       }
@@ -501,7 +607,7 @@ __global__ __launch_bounds__(64) void k_cluster_solve(ClusterArgs A) {
         const double ca = cos(th), sa = sin(th);
         const double la = H[0] * ca * ca + 2.0 * H[1] * ca * sa + H[2] * sa * sa;
         const double lb = H[0] * sa * sa - 2.0 * H[1] * ca * sa + H[2] * ca * ca;
-        for (int q = 0; q < 2 && !ok && its <= 600; ++q) {
+        for (int q = 0; q < 2 && !ok && its <= 300; ++q) {
           const bool first_a = la <= lb;
           const bool use_a = (q == 0) == first_a;
           const double e0 = use_a ? ca : -sa, e1 = use_a ? sa : ca;
@@ -531,8 +637,14 @@ __global__ __launch_bounds__(64) void k_cluster_solve(ClusterArgs A) {
     }
   }
   if (lane == 0) {
-    A.have_root[c] = ok ? 1 : 0;
-    if (!ok) atomicAdd(A.fail_count, 1);
+    if (!skip) {
+      const int f = ok ? 0 : (state > 0 ? 1 : 1 - state);
+      A.have_root[c] = ok ? 1 : -f;
+      if (!ok) {
+        A.next_try[c] = A.map_k + (1 << min(f, 5));
+        atomicAdd(A.fail_count, 1);
+      }
+    }
     A.newton_its[c] += its;
   }
 }
@@ -564,11 +676,6 @@ void launch_fill_int(int* x, int n, int value, hipStream_t st) {
 void launch_cluster_rows(const int* idx, int total, bool set, int* row, hipStream_t st) {
   if (total <= 0) return;
   hipLaunchKernelGGL(k_cluster_rows, dim3((total + 255) / 256), dim3(256), 0, st, idx, total, set ? 1 : 0, row);
-}
-
-void launch_cluster_fix(double* v, int N, const int* idx, int total, const double* x, hipStream_t st) {
-  if (total <= 0) return;
-  hipLaunchKernelGGL(k_cluster_fix, dim3((total + 255) / 256), dim3(256), 0, st, v, N, idx, total, x);
 }
 
 void launch_cluster_solve(const ClusterArgs& a, int ncl, hipStream_t st) {

@@ -54,7 +54,8 @@ struct collop_ctx {
   // stiff clusters of the current step (cluster.hip): cell buckets, the near
   // links, the clusters in CSR, the member row of every marker, map values
   DBuf b_cell_of, b_cell_cnt, b_cell_start, b_cell_sorted, b_links, b_link_count;
-  DBuf b_cl_off, b_cl_idx, b_cl_row, b_cl_x, b_cl_true, b_cl_root, b_cl_have, b_cl_fail, b_cl_its;
+  DBuf b_cl_off, b_cl_idx, b_cl_row, b_cl_x, b_cl_root, b_cl_have, b_cl_fail, b_cl_its, b_cl_next;
+  int cl_map = 0;               // FP64 maps of the step so far (the cluster retry clock)
   int ncl = 0, cl_total = 0;    // clusters and their members
   int64_t cl_row_n = 0;         // markers b_cl_row covers (all -1 outside the members)
   double* h_stat = nullptr;  // pinned, kStatWidth doubles
@@ -62,6 +63,7 @@ struct collop_ctx {
   unsigned* h_seq = nullptr;      // pinned: sequence number published after each h_stat write
   unsigned* h_seq_dev = nullptr;  // its device-side address
   unsigned seq = 0;               // the last sequence number launched
+  long stat_syncs = 0;            // statistics polls that ran out (a stream synchronisation instead)
   // ---- warm-start MLP
   DBuf m_params, m_adam_m, m_adam_v, m_data, m_rec, m_y, m_gpart, m_lpart, m_tiles, m_eval, m_grad;
   bool m_has_params = false, m_has_norm = false;

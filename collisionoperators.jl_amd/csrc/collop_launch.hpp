@@ -228,8 +228,8 @@ struct EpiArgs {
   double* stat_part;  // [blocks][kStatWidth]: |r|^2, |v1|^2, |Gv-v0|^2, ATA row, ATr, |r_map|^2 - |r|^2
   // stiff clusters (null: none): row of marker i in the cluster members (-1:
   // none); the map value of such a row is cl_x[2 row + comp] (the cluster
-  // solve's), and the plain map value goes to cl_true[2 row + comp]
-  const int* cl_row; const double* cl_x; double* cl_true;
+  // solve's); the plain map's residual there enters statistic kStatTrue
+  const int* cl_row; const double* cl_x;
 };
 int epi_blocks(int N);
 constexpr int kStatTrue = 3 + 2 * kMaxWindow;  // |Gv_map - v1|^2 - |Gv - v1|^2 over the cluster rows
@@ -382,8 +382,6 @@ void launch_near_cells(const MeshView& m, const double* v, const double* w, int 
                        const int* start, const int* sorted, NearLink* out, int* count, int cap, hipStream_t st);
 // row[idx[r]] = r (set) or -1 (reset) for r < total
 void launch_cluster_rows(const int* idx, int total, bool set, int* row, hipStream_t st);
-// v[idx[r]], v[N + idx[r]] <- x[2 r], x[2 r + 1]
-void launch_cluster_fix(double* v, int N, const int* idx, int total, const double* x, hipStream_t st);
 struct ClusterArgs {
   MeshView m;
   const int* off;        // clusters: members idx[off[c] .. off[c + 1]) (<= kClusterMax each)
@@ -400,7 +398,9 @@ struct ClusterArgs {
   const double* part;    // the map's pair-sum partials [S][2N]
   int S;
   double* mroot;         // 2 per member: the cluster's last root (warm start)
-  int* have_root;        // per cluster
+  int* have_root;        // per cluster: 1 root of an earlier map, 0 none, -f after f failed solves
+  int* next_try;         // per cluster: the first map that tries it again after a failure
+  int map_k;             // this map's index in the step
   double* x_out;         // 2 per member: the map value of the member's rows
   int* fail_count;       // += clusters whose solve reached no root
   int* newton_its;       // per cluster: += Newton iterations

@@ -111,7 +111,6 @@ __device__ __forceinline__ void epilogue_body(const EpiArgs& A, int N, int ld, s
       const int row = A.cl_row[mk];
       if (row >= 0) {
         const int cp = 2 * row + (i < N ? 0 : 1);
-        A.cl_true[cp] = gv;
         const double rt = gv - x1;
         gv = A.cl_x[cp];
         const double rh = gv - x1;
@@ -426,7 +425,6 @@ __global__ __launch_bounds__(kEwThreads) void k_epilogue_b(EnsView E, EnsBufs B,
   a.stat_part = B.statp + (size_t)(R.eblk0 + t.y) * kStatWidth;
   a.cl_row = nullptr;
   a.cl_x = nullptr;
-  a.cl_true = nullptr;
   epilogue_body(a, R.n, E.NT, (size_t)2 * E.NT, red, t.y);
 }
 

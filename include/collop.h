@@ -261,7 +261,8 @@ typedef struct {
   int32_t n_clusters;       /* ANDERSON: stiff clusters solved in each FP64 map */
   int32_t cl_fail;          /* cluster solves that reached no root (those maps
                                kept the plain rows of the cluster)             */
-  int32_t pad;
+  int32_t stat_syncs;       /* maps whose statistics needed a stream
+                               synchronisation (the 20 ms poll ran out)        */
 } collop_step_stats;
 
 /* Upload markers (v: N x 2 column-major, w: N) and project them; keeps S0. */
