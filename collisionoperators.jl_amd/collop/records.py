@@ -30,10 +30,27 @@ _HISTORIES = ("entropy_history", "energy_history", "momentum_history", "iter_his
 
 
 def _num(x) -> str:
-    """Shortest round-trip text of a number (Julia's `string` of a Float64 or Int)."""
+    """Text of a number as Julia's string interpolation writes it: integers
+    plainly; a Float64 as its shortest round-trip digits, in plain decimal with
+    at least one fractional digit (``1.0``, ``0.0001``, ``123456.0``) when
+    1e-4 <= |x| < 1e6 or x == 0, and otherwise as ``d.ddde-x`` / ``d.ddde+x``
+    with Julia's exponent style (``1.0e-5``, ``1.0e16``: no ``+``, no padding)."""
     if isinstance(x, (int, np.integer)):
         return str(int(x))
-    return repr(float(x))
+    x = float(x)
+    if not np.isfinite(x):
+        return "NaN" if np.isnan(x) else ("Inf" if x > 0 else "-Inf")
+    r = repr(x)
+    if x == 0.0 or 1e-4 <= abs(x) < 1e6:
+        if "e" in r:  # repr switches to exponents outside [1e-4, 1e16)
+            r = np.format_float_positional(x, unique=True, trim="0")
+        return r if ("." in r or "e" in r) else r + ".0"
+    m, e = np.format_float_scientific(x, unique=True, trim="0").split("e")
+    if "." not in m:
+        m += ".0"
+    elif m.endswith("."):
+        m += "0"
+    return f"{m}e{int(e)}"
 
 
 def _read_lines(fname: str) -> list[str]:
@@ -75,37 +92,33 @@ def migrate_cons_csv(fname: str) -> None:
     if "dt" in old:
         return None
     rows = [dict(zip(old, ln.split(","))) for ln in lines[1:]]
-    dt_at: dict[int, float] = {}
+    step_dt: dict[int, float] = {}
     for r in rows:
         try:
-            st = int(r.get("step", ""))
-            t = float(r.get("time", ""))
+            k = int(r.get("step", ""))
+            tk = float(r.get("time", ""))
         except ValueError:
             continue
-        if st < 0:
-            continue
-        dt_at[st] = 0.0 if st == 0 else t / st
-    if not dt_at:
+        if k >= 0:
+            step_dt[k] = tk / k if k else 0.0
+    if not step_dt:
         return None
-    t_at, acc = {}, 0.0
-    for st in range(0, max(dt_at) + 1):
-        acc += dt_at.get(st, 0.0)
-        t_at[st] = acc
-    tmp = fname + ".migrating"
-    with open(tmp, "w") as f:
+    step_time = dict(zip(range(max(step_dt) + 1),
+                         np.cumsum([step_dt.get(k, 0.0) for k in range(max(step_dt) + 1)]).tolist()))
+    upgraded = fname + ".upgrade.tmp"
+    with open(upgraded, "w") as f:
         f.write(",".join(CONS_COLS) + "\n")
         for r in rows:
             try:
-                st = int(r.get("step", ""))
+                k = int(r.get("step", ""))
             except ValueError:
                 continue
-            if st not in t_at:
-                continue
-            r["time"] = _num(t_at[st])
-            r["dt"] = _num(dt_at[st])
-            f.write(",".join(r.get(c, "0.0") for c in CONS_COLS) + "\n")
-    os.replace(tmp, fname)
-    print(f"Migrated {fname} to accumulated `time` + `dt` ({len(old)} -> {len(CONS_COLS)} columns)")
+            if k in step_time:
+                r["time"], r["dt"] = _num(step_time[k]), _num(step_dt[k])
+                f.write(",".join(r.get(c, "0.0") for c in CONS_COLS) + "\n")
+    os.replace(upgraded, fname)
+    print(f"{fname}: conservation CSV upgraded from {len(old)} to {len(CONS_COLS)} columns "
+          f"(accumulated time and per-step dt)")
     return None
 
 
@@ -212,9 +225,10 @@ class RunRecord:
                     with open(fn, "w") as f:
                         f.write(",".join(cols) + "\n")
 
-    def write_initial(self, entropy: float, energy: float, momentum, fp_minus_fs: float, neg_part: float):
-        """Row of step 0: the initial state (main.jl:116-126,189-195)."""
-        self._cons_row(0, 0.0, entropy, energy, momentum, 0, 0.0, fp_minus_fs, neg_part, 0.0, 0.0)
+    def write_initial(self, entropy: float, energy: float, momentum):
+        """Row of step 0: the initial state, with zeros in the diagnostic, residual
+        and dt columns as the reference writes it (main.jl:189-194)."""
+        self._cons_row(0, 0.0, entropy, energy, momentum, 0, 0.0, 0.0, 0.0, 0.0, 0.0)
 
     def write_step(self, step: int, t: float, stats: dict, diag: dict, dt: float, t_solve: float, t_step: float):
         """Rows of one step from the device step's statistics and diagnostics."""
@@ -246,7 +260,7 @@ def run_recorded(sim, params, n_steps: int, record: RunRecord, step0: int = 0, t
         t_solve = time.perf_counter() - t_step0
         diag = sim.diagnostics()
         t += params.dt
-        record.write_step(step, t, st, diag, params.dt, st["seconds"], time.perf_counter() - t_step0)
+        record.write_step(step, t, st, diag, params.dt, t_solve, time.perf_counter() - t_step0)
         h["entropy_history"].append(st["entropy"])
         h["energy_history"].append(st["energy"])
         h["momentum_history"].append(st["momentum"])

@@ -376,7 +376,7 @@ int collop_destroy(collop_ctx* ctx) {
                   &ctx->m_y, &ctx->m_gpart, &ctx->m_lpart, &ctx->m_tiles, &ctx->m_eval, &ctx->m_grad,
                   &ctx->b_cell_of, &ctx->b_cell_cnt, &ctx->b_cell_start, &ctx->b_cell_sorted, &ctx->b_links,
                   &ctx->b_link_count, &ctx->b_cl_off, &ctx->b_cl_idx, &ctx->b_cl_row, &ctx->b_cl_x,
-                  &ctx->b_cl_root, &ctx->b_cl_have, &ctx->b_cl_fail, &ctx->b_cl_its, &ctx->b_cl_next})
+                  &ctx->b_cl_root, &ctx->b_cl_have, &ctx->b_cl_fail, &ctx->b_cl_its})
     release(*b);
   for (auto& e : ctx->ev_pool) {
     (void)hipEventDestroy(e.first);
@@ -1054,9 +1054,7 @@ int build_clusters_dev(collop_ctx* ctx, double theta, int kmax) {
   CHECK_LAUNCH();
   launch_fill_int(P<int>(ctx->b_cl_fail), 1, 0, ctx->stream);
   CHECK_LAUNCH();
-  launch_fill_int(P<int>(ctx->b_cl_next), ctx->ncl, 0, ctx->stream);
-  CHECK_LAUNCH();
-  ctx->cl_map = 0;
+
   // the host lists must outlive the asynchronous copies
   HIPC(hipStreamSynchronize(ctx->stream));
   return COLLOP_OK;
@@ -1085,8 +1083,6 @@ int cluster_solve_dev(collop_ctx* ctx, const collop_step_params& p, const double
   a.x_out = P<double>(ctx->b_cl_x);
   a.fail_count = P<int>(ctx->b_cl_fail);
   a.newton_its = P<int>(ctx->b_cl_its);
-  a.next_try = P<int>(ctx->b_cl_next);
-  a.map_k = ctx->cl_map++;
   launch_cluster_solve(a, ctx->ncl, ctx->stream);
   CHECK_LAUNCH();
   return COLLOP_OK;
@@ -1974,7 +1970,6 @@ int collop_state_set(collop_ctx* ctx, const double* v, const double* w, int64_t 
   ENSURE(ctx->b_cl_root, (size_t)2 * kClMembersCap * 8);
   ENSURE(ctx->b_cl_have, (size_t)(kClMembersCap / 2) * 4);
   ENSURE(ctx->b_cl_its, (size_t)(kClMembersCap / 2) * 4);
-  ENSURE(ctx->b_cl_next, (size_t)(kClMembersCap / 2) * 4);
   ENSURE(ctx->b_cl_fail, 64);
   ENSURE(ctx->b_cl_row, (size_t)N * 4);
   launch_fill_int(P<int>(ctx->b_cl_row), (int)N, -1, ctx->stream);

@@ -530,10 +530,10 @@ __global__ __launch_bounds__(64) void k_cluster_solve(ClusterArgs A) {
   bool ok = false;
   double* mr = A.mroot + 2 * o;
   // have_root: 1 = a root of an earlier map, 0 = none, -f = f failed solves this
-  // step.  After the f-th failure the cluster keeps the plain rows for 2^f maps
-  // (at most 32) before it is tried again, from a map closer to the root.
+  // step.  After kClusterTries failures the cluster keeps the plain rows for the
+  // rest of the step.
   const int state = A.have_root[c];
-  const bool skip = state < 0 && A.map_k < A.next_try[c];
+  const bool skip = state <= -kClusterTries;
   if (k == 2 && !skip) {  // the one-unknown form; the reference angle is the previous root's
     double psi_ref = 0.0;
     if (state > 0) {
@@ -560,7 +560,7 @@ __global__ __launch_bounds__(64) void k_cluster_solve(ClusterArgs A) {
     double t = 0.0, tp = 0.0, dtau = 0.05;
     bool have_prev = false, path_ok = true;
     while (t < 1.0) {
-      if (its > 200) {
+      if (its > kClusterPathIts) {
         path_ok = false;
         break;
       }
@@ -607,7 +607,7 @@ __global__ __launch_bounds__(64) void k_cluster_solve(ClusterArgs A) {
         const double ca = cos(th), sa = sin(th);
         const double la = H[0] * ca * ca + 2.0 * H[1] * ca * sa + H[2] * sa * sa;
         const double lb = H[0] * sa * sa - 2.0 * H[1] * ca * sa + H[2] * ca * ca;
-        for (int q = 0; q < 2 && !ok && its <= 300; ++q) {
+        for (int q = 0; q < 2 && !ok && its <= kClusterIts; ++q) {
           const bool first_a = la <= lb;
           const bool use_a = (q == 0) == first_a;
           const double e0 = use_a ? ca : -sa, e1 = use_a ? sa : ca;
@@ -617,7 +617,7 @@ __global__ __launch_bounds__(64) void k_cluster_solve(ClusterArgs A) {
             S.m[2 * lane + 1] = cy + pr * e1;
           }
           __syncthreads();
-          ok = cl_newton(S, A, k, lam, S.m, 1.0, 40, tol_final, its);
+          ok = cl_newton(S, A, k, lam, S.m, 1.0, 20, tol_final, its);
         }
       }
     }
@@ -638,12 +638,8 @@ __global__ __launch_bounds__(64) void k_cluster_solve(ClusterArgs A) {
   }
   if (lane == 0) {
     if (!skip) {
-      const int f = ok ? 0 : (state > 0 ? 1 : 1 - state);
-      A.have_root[c] = ok ? 1 : -f;
-      if (!ok) {
-        A.next_try[c] = A.map_k + (1 << min(f, 5));
-        atomicAdd(A.fail_count, 1);
-      }
+      A.have_root[c] = ok ? 1 : (state > 0 ? -1 : state - 1);
+      if (!ok) atomicAdd(A.fail_count, 1);
     }
     A.newton_its[c] += its;
   }

@@ -54,8 +54,7 @@ struct collop_ctx {
   // stiff clusters of the current step (cluster.hip): cell buckets, the near
   // links, the clusters in CSR, the member row of every marker, map values
   DBuf b_cell_of, b_cell_cnt, b_cell_start, b_cell_sorted, b_links, b_link_count;
-  DBuf b_cl_off, b_cl_idx, b_cl_row, b_cl_x, b_cl_root, b_cl_have, b_cl_fail, b_cl_its, b_cl_next;
-  int cl_map = 0;               // FP64 maps of the step so far (the cluster retry clock)
+  DBuf b_cl_off, b_cl_idx, b_cl_row, b_cl_x, b_cl_root, b_cl_have, b_cl_fail, b_cl_its;
   int ncl = 0, cl_total = 0;    // clusters and their members
   int64_t cl_row_n = 0;         // markers b_cl_row covers (all -1 outside the members)
   double* h_stat = nullptr;  // pinned, kStatWidth doubles

@@ -365,6 +365,9 @@ void launch_warm_apply(const double* v, const double* dotv, const float* y, floa
 constexpr int kClusterMax = 32;         // largest cluster solved exactly (markers)
 constexpr int kLinkCap = 1 << 17;       // near links kept per step (beyond it: no clusters that step)
 constexpr int kClMembersCap = 1 << 15;  // cluster members per step (beyond it: no clusters that step)
+constexpr int kClusterTries = 2;        // failed solves of a cluster per step before it keeps the plain rows
+constexpr int kClusterPathIts = 60;     // Newton iterations along a cluster's homotopy path
+constexpr int kClusterIts = 90;         // ... including the eigen-line starts
 struct NearLink {
   int i, j;  // i < j
   double q;  // |v_i - v_j|^2
@@ -399,8 +402,6 @@ struct ClusterArgs {
   int S;
   double* mroot;         // 2 per member: the cluster's last root (warm start)
   int* have_root;        // per cluster: 1 root of an earlier map, 0 none, -f after f failed solves
-  int* next_try;         // per cluster: the first map that tries it again after a failure
-  int map_k;             // this map's index in the step
   double* x_out;         // 2 per member: the map value of the member's rows
   int* fail_count;       // += clusters whose solve reached no root
   int* newton_its;       // per cluster: += Newton iterations

@@ -1502,7 +1502,7 @@ bool cluster_root(const ClusterEq& E, const double* mprev, std::vector<double>& 
   double t = 0.0, tp = 0.0, dtau = 0.05;
   bool have_prev = false, path_ok = true;
   while (t < 1.0) {
-    if (its > 200) {
+    if (its > 60) {
       path_ok = false;
       break;
     }
@@ -1546,13 +1546,13 @@ bool cluster_root(const ClusterEq& E, const double* mprev, std::vector<double>& 
   double e[2][2] = {{ca, sa}, {-sa, ca}};
   if (l_b < l_a) std::swap(e[0], e[1]);
   for (int q = 0; q < 2; ++q) {
-    if (its > 300) return false;
+    if (its > 90) return false;
     for (int i = 0; i < E.k; ++i) {
       const double p = (m_t0[2 * i] - c0) * e[q][0] + (m_t0[2 * i + 1] - c1) * e[q][1];
       m[2 * i] = c0 + p * e[q][0];
       m[2 * i + 1] = c1 + p * e[q][1];
     }
-    if (cl_newton(E, m, 1.0, 40, tol_final, its)) return true;
+    if (cl_newton(E, m, 1.0, 20, tol_final, its)) return true;
   }
   return false;
 }
@@ -1561,17 +1561,16 @@ bool cluster_root(const ClusterEq& E, const double* mprev, std::vector<double>& 
 // own terms at the iterate (v_mid, G_eff of the map), then the root; Ghat's
 // cluster rows <- 2 m - v0 (rows of failed clusters keep Gv).  mroot holds each
 // cluster's last root (2 * cluster size values per cluster, empty = none).
-// nfail counts a cluster's failed solves since its last root: after the f-th
-// it keeps the plain rows for 2^min(f, 5) maps (next_try) before it is tried
-// again.  map_k is the map's index in the step.
+// nfail counts a cluster's failed solves since its last root: after two it
+// keeps the plain rows for the rest of the step.
+constexpr int kClusterTries = 2;
 int cluster_rows(const orc_mesh* ms, const std::vector<int>& off, const std::vector<int>& idx, const double* v0,
                  const double* w, int64_t N, const Scratch& s, double lam, double dt, double* Ghat,
-                 std::vector<std::vector<double>>& mroot, std::vector<int>& nfail, std::vector<int>& next_try,
-                 int map_k) {
+                 std::vector<std::vector<double>>& mroot, std::vector<int>& nfail) {
   int fails = 0;
   const int ncl = (int)off.size() - 1;
   for (int c = 0; c < ncl; ++c) {
-    if (nfail[c] > 0 && map_k < next_try[c]) continue;
+    if (nfail[c] >= kClusterTries) continue;
     const int k = off[c + 1] - off[c];
     const int* id = &idx[off[c]];
     std::vector<double> v0c(2 * k), wc(k), ext(2 * k), m;
@@ -1601,7 +1600,6 @@ int cluster_rows(const orc_mesh* ms, const std::vector<int>& off, const std::vec
     if (!ok) {
       fails++;
       nfail[c] = mroot[c].empty() ? nfail[c] + 1 : 1;
-      next_try[c] = map_k + (1 << std::min(nfail[c], 5));
       mroot[c].clear();
       continue;
     }
@@ -1686,8 +1684,7 @@ void orc_step_anderson(const orc_mesh* ms, const orc_solver_params* Pp, double* 
   }
   const int ncl = clusters ? (int)coff.size() - 1 : 0;
   std::vector<std::vector<double>> mroot(ncl);
-  std::vector<int> nfail(ncl, 0), next_try(ncl, 0);
-  int map_k = 0;
+  std::vector<int> nfail(ncl, 0);
   std::vector<double> Gh(ncl ? n2 : 0), rh(ncl ? n2 : 0);
   int cl_fail = 0;
   for (int k = 1; k <= P.max_iter; ++k) {
@@ -1697,7 +1694,7 @@ void orc_step_anderson(const orc_mesh* ms, const orc_solver_params* Pp, double* 
     nrm_r = nrm(r.data(), n2);
     if (ncl) {
       std::memcpy(Gh.data(), Gv.data(), sizeof(double) * n2);
-      cl_fail += cluster_rows(ms, coff, cidx, v0, w, N, s, lam, P.dt, Gh.data(), mroot, nfail, next_try, map_k++);
+      cl_fail += cluster_rows(ms, coff, cidx, v0, w, N, s, lam, P.dt, Gh.data(), mroot, nfail);
       for (int64_t i = 0; i < n2; ++i) rh[i] = Gh[i] - v1[i];
     }
     // the vectors Anderson mixes

@@ -60,7 +60,7 @@ def test_state_diagnostics_and_run_record(gpu_ctx, tmp_path):
     params = collop.config("gaussian_n4096", DT=0.01).step_params()
     rec = collop.RunRecord("gpu", str(tmp_path))
     (p1, p2), E0 = O.momentum_energy(v, w)
-    rec.write_initial(m.entropy(fc0), E0, (p1, p2), d0["fp_minus_fs"], d0["neg_part"])
+    rec.write_initial(m.entropy(fc0), E0, (p1, p2))
     out = collop.run_recorded(sim, params, 3, rec, checkpoint_every=2, checkpoint_dir=str(tmp_path))
     assert [o["exit_reason"] for o in out] == [0, 0, 0]
     vg, fcg = sim.markers(), sim.coeffs()

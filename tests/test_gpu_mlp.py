@@ -177,3 +177,54 @@ def test_packaged_model_loads_and_steps(gpu_ctx):
     sim = collop.Simulation(ws, v, w)
     st = sim.run(p.step_params(warmstart=collop.WARM_MLP), 2)
     assert all(s["exit_reason"] == 0 for s in st)
+
+
+def test_warm_start_at_n65536(gpu_ctx):
+    """The learned warm start at the workload's size: the packaged model on a
+    held-out bimodal seed (999), N = 65536 on 32 x 32 cells, dt 0.02, two steps
+    of the chosen path with and without it.  Each state is a root of the
+    oracle's implicit system, the model lowers the first residual r0 below the
+    Euler predictor's, and the fixed point does not depend on the predictor.
+    Then a short recording and training pass at the same size: the loss is
+    finite and falls."""
+    from collop import warmstart as W
+    p = collop.chosen_config("bimodal_n65536")
+    ws = collop.build_workspace(p, ctx=gpu_ctx)
+    W.load_packaged(ws)
+    v, w = collop.markers("bimodal", 65536, 999, sigma1=0.5 ** 0.5, sigma2=0.5 ** 0.5, peak=2.0)
+    m = O.Mesh(p.bp1, p.bp2)
+    P = O.default_params(p.DT, m=5)
+    r0, last = {}, {}
+    for k in ("euler", "mlp"):  # one simulation owns the context's state at a time
+        sim = collop.Simulation(ws, v, w)
+        prev = v
+        r0[k] = []
+        for step in range(2):
+            st = sim.run(p.step_params(warmstart=collop.WARM_MLP if k == "mlp" else collop.WARM_EULER), 1)[0]
+            x = sim.markers()
+            fc = m.l2_project(prev, w)
+            res = float(np.linalg.norm(m.picard_map(P, x, prev, w, m.entropy(fc)) - x))
+            tol = max(1e-12 * np.linalg.norm(x), 1e-10)
+            print(f"step {step} {k}: {st['iters']} maps, r0 {st['r0']:.3e}, exit {st['exit_reason']}, "
+                  f"oracle residual {res:.2e} (tol {tol:.2e})", flush=True)
+            assert st["exit_reason"] == 0 and res <= 20 * tol
+            r0[k].append(st["r0"])
+            prev = x
+        last[k] = prev
+    assert all(a < b for a, b in zip(r0["mlp"], r0["euler"]))
+    print(f"euler vs warm-started state after 2 steps: max|dv| {np.max(np.abs(last['euler'] - last['mlp'])):.2e}")
+    # recording and training at N = 65536
+    bp = p.bp1
+    ws2 = collop.Workspace(bp, bp, ctx=gpu_ctx)
+    ml = collop.WarmStart(ws2)
+    ml.dataset_alloc(2 * 65536)
+    vr, wr = collop.markers("bimodal", 65536, 100, sigma1=0.5 ** 0.5, sigma2=0.5 ** 0.5, peak=2.0)
+    prm = p.step_params()
+    prm.record_training = 1
+    collop.Simulation(ws2, vr, wr).run(prm, 2)
+    assert ml.dataset_size() == 2 * 65536
+    ml.set_params(W.glorot_init(0))
+    ml.fit_norm()
+    losses = ml.train(epochs=3, batch=65536, lr=1e-3, seed=1)
+    print("training loss per epoch", losses)
+    assert all(np.isfinite(losses)) and losses[-1] < losses[0]

@@ -154,7 +154,7 @@ def test_checkpoint_round_trip_and_auto(tmp_path):
 
 def test_run_record_rows_and_resume(tmp_path):
     rec = R.RunRecord("bim", str(tmp_path))
-    rec.write_initial(1.5, 2.0, (0.0, 0.0), 0.3, 0.0)
+    rec.write_initial(1.5, 2.0, (0.0, 0.0))
     st = {"entropy": 1.6, "energy": 2.0, "momentum": (1e-17, -2e-17), "iters": 9, "residual": 3e-13, "r0": 0.5,
           "n_restart": 0, "inner_maps": 44}
     for k in (1, 2, 3):
@@ -162,8 +162,10 @@ def test_run_record_rows_and_resume(tmp_path):
     cons = tmp_path / "conservation_history_bim.csv"
     lines = cons.read_text().splitlines()
     assert lines[0] == ",".join(R.CONS_COLS) and len(lines) == 5
-    assert lines[2].split(",") == ["1", "0.02", "1.6", "2.0", "1e-17", "-2e-17", "9", "3e-13", "0.31", "1e-05", "0.5",
-                                   "0.02"]
+    # the step-0 row as main.jl:191-194 writes it, byte for byte
+    assert lines[1] == "0,0.0,1.5,2.0,0.0,0.0,0,0.0,0.0,0.0,0.0,0.0"
+    assert lines[2].split(",") == ["1", "0.02", "1.6", "2.0", "1.0e-17", "-2.0e-17", "9", "3.0e-13", "0.31",
+                                   "1.0e-5", "0.5", "0.02"]
     stats = (tmp_path / "solver_stats_bim.csv").read_text().splitlines()
     assert stats[0] == ",".join(R.STATS_COLS) and stats[1].split(",")[:3] == ["1", "9", "44"]
     # resume from step 2: rows past it go, the next row appends
@@ -172,3 +174,17 @@ def test_run_record_rows_and_resume(tmp_path):
     assert list(read_col(cons, "step")) == [0.0, 1.0, 2.0, 3.0]
     assert read_col(cons, "fp_minus_fs")[-1] == 0.32
     assert list(read_col(tmp_path / "solver_stats_bim.csv", "step")) == [1.0, 2.0, 3.0]
+
+
+def test_number_text_follows_julia():
+    """Julia's string interpolation of Float64 / Int: shortest round-trip digits,
+    plain decimal in [1e-4, 1e6) with a fractional digit, else d.ddde+-x without
+    '+' or padding."""
+    cases = {0.0: "0.0", 1.0: "1.0", -2.5: "-2.5", 0.02: "0.02", 0.0001: "0.0001", 1e-5: "1.0e-5",
+             123456.0: "123456.0", 1e6: "1.0e6", 1.5e-7: "1.5e-7", 1e16: "1.0e16", -3.2e-12: "-3.2e-12",
+             0.1 + 0.2: "0.30000000000000004", 2.9999999999999995e-10: "2.9999999999999995e-10"}
+    for x, t in cases.items():
+        assert R._num(x) == t, (x, R._num(x), t)
+    assert R._num(7) == "7" and R._num(np.int64(-3)) == "-3"
+    for x in (1e-5, 3.14159e8, 0.30000000000000004, 6.02e23):
+        assert float(R._num(x)) == x
