@@ -138,6 +138,7 @@ SIGNATURES = [
     ("collop_collision_f32c", C.c_int, [_vp, _dp, _dp, _dp, _dp, C.c_int64]),
     ("collop_collision_metric_f64", C.c_int, [_vp, _dp, _dp, _dp, _dp, _dp, C.c_int64]),
     ("collop_collision_metric_f32c", C.c_int, [_vp, _dp, _dp, _dp, _dp, _dp, C.c_int64]),
+    ("collop_collision_f64_noalloc", C.c_int, [_vp, _vp, _vp, _vp, _vp, C.c_int64, _vp]),
     ("collop_l2_rhs", C.c_int, [_vp, _dp, _dp, C.c_int64, _dp]),
     ("collop_l2_project", C.c_int, [_vp, _dp, _dp, C.c_int64, _dp]),
     ("collop_grad_gather", C.c_int, [_vp, _dp, C.c_int64, _dp, _dp]),

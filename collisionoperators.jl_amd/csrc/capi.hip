@@ -26,6 +26,17 @@
 
 using namespace collop;
 
+// ---- scratch of the allocation-free pair sum (collop_collision_f64_noalloc) ---
+// The packed source records (8 doubles per marker) followed by kNoAllocSlices
+// slices of 2N partial sums, for N <= kNoAllocMaxN: 21 MB, one copy per GPU,
+// held until the process ends.  Its device address on each GPU is looked up
+// once (collop_create) and only that address is used from the host.
+constexpr int64_t kNoAllocMaxN = 65536;
+constexpr int kNoAllocSlices = 16;
+static __device__ __align__(256) double scratch[(size_t)kNoAllocMaxN * (8 + 2 * kNoAllocSlices)];
+constexpr int kMaxDevices = 64;
+static double* g_scratch_dev[kMaxDevices] = {nullptr};
+
 namespace {
 
 int nk_setup(collop_ctx* ctx, int n, int mem, NkDevice& E, double** extra, int nextra);
@@ -354,6 +365,11 @@ int collop_create(int device, collop_ctx** out) {
     delete c;
     return COLLOP_ERR_HIP;
   }
+  // the allocation-free pair sum's scratch on this device (the current one)
+  if (device < kMaxDevices && !g_scratch_dev[device]) {
+    double* ptr = nullptr;
+    if (hipGetSymbolAddress((void**)&ptr, HIP_SYMBOL(scratch)) == hipSuccess && ptr) g_scratch_dev[device] = ptr;
+  }
   *out = c;
   return COLLOP_OK;
 }
@@ -585,6 +601,36 @@ int collop_collision_metric_f64(collop_ctx* ctx, const double* v, const double* 
 int collop_collision_metric_f32c(collop_ctx* ctx, const double* v, const double* w, const double* G, double* dot_v,
                                  double* A, int64_t N) {
   return collision_common(ctx, 3, v, w, G, dot_v, A, N);
+}
+
+// The pair sum of collop_collision_f64 on caller-owned device arrays, with no
+// allocation: the packed sources and the slice partials live in `scratch`
+// (ordered form, kNoAllocSlices slices), launched on `stream` (null: the
+// context's) without synchronisation.  Calls that overlap on one GPU race.
+int collop_collision_f64_noalloc(collop_ctx* ctx, const double* v, const double* w, const double* G, double* dot_v,
+                                 int64_t N, void* stream) {
+  int r = need_mesh(ctx);
+  if (r) return r;
+  if (N < 0 || N > kNoAllocMaxN)
+    return fail(ctx, COLLOP_ERR_INVALID,
+                "N = " + std::to_string(N) + " exceeds the allocation-free scratch (at most 65536 markers)");
+  if (N == 0) return COLLOP_OK;
+  if (!v || !w || !G || !dot_v) return fail(ctx, COLLOP_ERR_INVALID, "null array");
+  double* base = ctx->device < kMaxDevices ? g_scratch_dev[ctx->device] : nullptr;
+  if (!base) return fail(ctx, COLLOP_ERR_HIP, "the scratch array's device address is unknown on this GPU");
+  HIPC(hipSetDevice(ctx->device));
+  hipStream_t st = stream ? static_cast<hipStream_t>(stream) : ctx->stream;
+  const int n = (int)N;
+  const MeshView& m = ctx->mv;
+  Src64* src = reinterpret_cast<Src64*>(base);
+  double* part = base + (size_t)8 * N;
+  launch_pack_src(v, G, w, n, m.lo1, m.hi1, m.lo2, m.hi2, src, nullptr, st);
+  CHECK_LAUNCH();
+  launch_pair_f64(src, n, PairTile{1, kNoAllocSlices, 1, 0}, part, st);
+  CHECK_LAUNCH();
+  launch_slice_sum(part, kNoAllocSlices, 2 * N, dot_v, st);
+  CHECK_LAUNCH();
+  return COLLOP_OK;
 }
 
 static int rhs_common(collop_ctx* ctx, const double* v, const double* w, int64_t N, double* out, bool solve) {

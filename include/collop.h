@@ -86,6 +86,18 @@ int collop_collision_metric_f64(collop_ctx* ctx, const double* v, const double* 
 int collop_collision_metric_f32c(collop_ctx* ctx, const double* v, const double* w, const double* G, double* dot_v,
                                  double* A, int64_t N);
 
+/* Allocation-free variant of collop_collision_f64 (no counterpart in the
+ * reference; for callers that cannot allocate: code under HIP graph capture,
+ * serving loops, harnesses that forbid hipMalloc).  v, w, G and dot_v are
+ * DEVICE arrays on the context's GPU, in the same layouts; the scratch (packed
+ * sources and slice partials, 21 MB) is a static device array of the library,
+ * so N is at most 65536 (COLLOP_ERR_INVALID beyond).  The work is launched on
+ * `stream` (a hipStream_t; NULL = the context's stream) without
+ * synchronisation.  Every call on one GPU shares that scratch: calls that
+ * overlap on one GPU race. */
+int collop_collision_f64_noalloc(collop_ctx* ctx, const double* v, const double* w, const double* G, double* dot_v,
+                                 int64_t N, void* stream);
+
 /* L2PROJ_FN, scatter only: rhs_k = sum_alpha w_alpha phi_k(v_alpha) over
  * in-domain markers (functions.jl:121-131; _l2_scatter_kernel!,
  * projection_gpu.jl:47-71).  A caller wanting bitwise host-solve parity
