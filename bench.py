@@ -41,7 +41,6 @@ Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu]
        python bench.py --report precision --config bimaxwellian_n262144 --steps 20
        python bench.py --config ensemble48 --steps 3
        python bench.py --config lb_bimodal_n40000 --steps 100
-       python bench.py --kernel-probe        (pair kernels + a 256 MiB copy, for rocprofv3 counters)
 """
 from __future__ import annotations
 
@@ -272,50 +271,6 @@ def precision_report(args):
     print(json.dumps(out), flush=True)
 
 
-def kernel_probe(args):
-    """Three launches of each pair kernel (plain and with the metric, FP64 and
-    FP32c; the ordered form, then the symmetric one) at the workload, then one 256 MiB
-    device-to-device copy (the known-size calibration of the rocprofv3
-    FETCH_SIZE / WRITE_SIZE counters)."""
-    import ctypes as C
-
-    import collop
-    p = collop.config(args.config)
-    v, w = collop.initial_markers(p)
-    ws = collop.build_workspace(p, device=0)
-    N = v.shape[0]
-    fc = np.zeros(ws.n_dofs)
-    collop.l2_project_gpu(ws, fc, v, w)
-    r, _ = collop.compute_r_and_entropy_gpu(ws, fc)
-    L = collop.mass_solve_gpu(ws, r)
-    G = np.zeros((N, 2), order="F")
-    collop.compute_G_gpu(ws, G, v, L)
-    out = np.zeros((N, 2), order="F")
-    A = np.zeros((N, 3), order="F")
-    for symmetric in (False, True):
-        collop.set_pair_mode(ws, symmetric)
-        for _ in range(3):
-            collop.compute_collision_gpu(ws, out, v, w, G)
-        for _ in range(3):
-            collop.compute_collision_gpu32(ws, out, v, w, G)
-        for _ in range(3):
-            collop.compute_collision_metric_gpu(ws, out, A, v, w, G)
-        for _ in range(3):
-            collop.compute_collision_metric_gpu(ws, out, A, v, w, G, fp32c=True)
-    hip = C.CDLL("libamdhip64.so")
-    nbytes = 256 << 20
-    a, b = C.c_void_p(), C.c_void_p()
-    if hip.hipMalloc(C.byref(a), C.c_size_t(nbytes)) or hip.hipMalloc(C.byref(b), C.c_size_t(nbytes)):
-        raise RuntimeError("hipMalloc failed")
-    hip.hipMemset(a, 1, C.c_size_t(nbytes))
-    hip.hipMemcpy(b, a, C.c_size_t(nbytes), 3)  # device to device
-    hip.hipDeviceSynchronize()
-    hip.hipFree(a)
-    hip.hipFree(b)
-    log(f"[probe] pair kernels x3 (FP64, FP32c, FP64 + metric, FP32c + metric; ordered, then symmetric) at N={N}; "
-        f"copy {nbytes} bytes")
-
-
 def warmstart_report(args):
     """Learned warm start: record 8 seeded N = 65536 runs (the three ICs in turn,
     32 x 32 mesh, dt 0.02) for `--train-steps` steps each, train the MLP with
@@ -522,8 +477,6 @@ def main():
     ap.add_argument("--config", default="bimodal_n65536",
                     help="bimodal_n65536 (headline), gaussian_n4096, bimaxwellian_n262144, ensemble48 or "
                          "lb_bimodal_n40000 (Lenard-Bernstein)")
-    ap.add_argument("--kernel-probe", action="store_true",
-                    help="run only the pair kernels and a 256 MiB copy (rocprofv3 counter runs)")
     ap.add_argument("--train-steps", type=int, default=200, help="warmstart report: recorded steps per run")
     ap.add_argument("--epochs", type=int, default=3, help="warmstart report: Adam epochs")
     ap.add_argument("--save-model", default=None, help="warmstart report: write the trained model (.npz)")
@@ -540,9 +493,6 @@ def main():
         return
     if args.report == "warmstart":
         warmstart_report(args)
-        return
-    if args.kernel_probe:
-        kernel_probe(args)
         return
     if args.config == "ensemble48":
         ensemble_bench(args)
@@ -582,9 +532,9 @@ def main():
         variants = [
             ("fp64_symmetric", "plain FP64 Anderson with the symmetric pair sum", p, collop.PREC_F64, None,
              collop.WARM_EULER),
-            ("fp64_clusters", "FP64 Anderson with the symmetric pair sum and stiff-cluster elimination",
-             collop.config(args.config, cl_theta=pch.cl_theta, cl_kmax=pch.cl_kmax), collop.PREC_F64, None,
-             collop.WARM_EULER),
+            ("chosen_clusters", "the chosen path with stiff-cluster elimination (clusters of "
+             f"max(w_i, w_j) / |d| > {collop.cluster_config(args.config).cl_theta:g} solved exactly in every FP64 map)",
+             collop.cluster_config(args.config), collop.PREC_F32C, None, warm),
             ("defect_correction", "FP32c then FP64 outer maps, frozen-metric defect correction with an outer "
              "Anderson window of 3 and exact near pairs, symmetric pair sum", collop.defect_config(args.config),
              collop.PREC_F32C, None, warm),

@@ -15,9 +15,9 @@ from .hooks import (compute_collision_gpu, compute_collision_gpu32, compute_coll
 from .ic import ensemble_markers, initial_markers, markers  # noqa: F401
 from .mesh import Workspace, build_workspace, extraction_p2c1  # noqa: F401
 from .params import (CHOSEN, CONFIGS, DEFECT, SimParameters, chosen_config, chosen_description,  # noqa: F401
-                     config, defect_config, parse_overrides)
+                     cluster_config, config, defect_config, parse_overrides)
 from .records import (CONS_COLS, STATS_COLS, RunRecord, checkpoint_path, cons_time_at,  # noqa: F401
                       load_checkpoint, migrate_cons_csv, run_recorded, save_checkpoint, truncate_csv_after)
-from .sim import (Ensemble, Simulation, StreamEnsemble, pair_mode, profile, set_fused_maps,  # noqa: F401
+from .sim import (Ensemble, Simulation, StreamEnsemble, pair_mode, profile, set_device_solve, set_fused_maps,  # noqa: F401
                   set_pair_mode, set_pair_tile)
 from .warmstart import WarmStart, glorot_init  # noqa: F401

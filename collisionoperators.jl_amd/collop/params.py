@@ -157,18 +157,28 @@ DEFECT = dict(gpu_fp32=True, solver="defect", dc_outer_m=3, dc_near_theta=0.02, 
 
 # The chosen path: FP32-compensated Anderson maps until the FP32 floor, then FP64
 # maps (step_anderson!), with stiff-cluster elimination: markers closer than
+# The chosen path: FP32c maps until their rounding floor, then FP64 maps, the
+# Anderson solve of step_anderson! with its control on the device, symmetric
+# pair sum.
+CHOSEN = dict(gpu_fp32=True, solver="anderson")
+# Stiff-cluster elimination (an opt-in extension): pairs with
 # max(w_i, w_j) / |d| > CLUSTER_TAU / dt at the start of a step form clusters
 # whose rows every FP64 map solves exactly.  The threshold keeps dt w / |d|
 # fixed, the factor the stiffness of a pair scales with.
 CLUSTER_TAU = 0.005
-CHOSEN = dict(gpu_fp32=True, solver="anderson", cl_kmax=32)
+CLUSTER_KMAX = 32
 
 
 def chosen_config(name: str, **over) -> SimParameters:
     """config(name) with the chosen path's solver knobs (CHOSEN), then `over`."""
     kw = dict(CHOSEN)
     kw.update(over)
-    p = config(name, **kw)
+    return config(name, **kw)
+
+
+def cluster_config(name: str, **over) -> SimParameters:
+    """chosen_config(name) with stiff-cluster elimination, then `over`."""
+    p = chosen_config(name, cl_kmax=CLUSTER_KMAX, **over)
     if "cl_theta" not in over:
         p = dataclasses.replace(p, cl_theta=CLUSTER_TAU / p.DT)
     return p
@@ -182,10 +192,8 @@ def defect_config(name: str, **over) -> SimParameters:
 
 
 def chosen_description(name: str, mlp: bool) -> str:
-    p = chosen_config(name)
-    return ("FP32c maps until the FP32 rounding floor, then FP64 maps, Anderson (step_anderson!) with stiff-cluster "
-            f"elimination (max(w_i, w_j) / |d| > {p.cl_theta:g} at the step's start, clusters of <= {p.cl_kmax} "
-            "markers solved exactly in every FP64 map), FP32c Euler predictor"
+    return ("FP32c maps until the FP32 rounding floor, then FP64 maps, Anderson (step_anderson!) with its control "
+            "(least squares, exit and stagnation tests) on the device, FP32c Euler predictor"
             + (" + learned MLP warm start" if mlp else "") + "; symmetric pair sum (each unordered pair once)")
 
 

@@ -89,6 +89,13 @@ def set_fused_maps(ws: Workspace, on: bool = True):
     ws.ctx.check(ws.ctx.lib.collop_set_fused_maps(ws.ctx.h, 1 if on else 0))
 
 
+def set_device_solve(ws: Workspace, on: bool = True):
+    """Anderson control of Landau steps on the device (the default: the host
+    only launches maps) or on the host (every map's statistics read back);
+    both give the same bits.  See collop_set_device_solve."""
+    ws.ctx.check(ws.ctx.lib.collop_set_device_solve(ws.ctx.h, 1 if on else 0))
+
+
 def pair_mode(ws: Workspace, N: int):
     """(symmetric, tiles, slots) a pair sum over N markers runs with now."""
     mode, tiles, slots = C.c_int(), C.c_int(), C.c_int()

@@ -1,7 +1,10 @@
-// anderson.hpp -- host-side state machine of the implicit solve
-// (step_anderson!, solver.jl:300-423), shared by the single-run step and every
-// run of the batched ensemble.  It decides, from the reduced statistics of each
-// Picard map, what the device does next; it owns no device memory.
+// anderson.hpp -- the state machine of the implicit solve (step_anderson!,
+// solver.jl:300-423), shared by the single-run step and every run of the
+// batched ensemble.  It decides, from the reduced statistics of each Picard
+// map, what the device does next; it owns no device memory.  The same code runs
+// on the host (ensemble, traced and cluster solves) and inside the control
+// kernel of the device-resident solve (anderson_dev.hip); floating-point
+// contraction is off in it, so both give bitwise the same decisions.
 //
 // Per map k the caller:
 //   1. asks plan() for the map's precision, Anderson column and window width,
@@ -14,7 +17,6 @@
 #pragma once
 #include <algorithm>
 #include <cmath>
-#include <cstdio>
 #include <limits>
 
 #include "../../include/collop.h"
@@ -23,8 +25,128 @@
 namespace collop {
 
 // Small SPD solve (Cholesky; Julia's factorize picks it for the exactly
-// symmetric ATA), with partial-pivot LU as the fallback.
-void small_solve(int n, const double* A0, const double* b0, double* x);
+// symmetric ATA), with partial-pivot LU as the fallback.  n <= kMaxWindow.
+__host__ __device__ inline void small_solve(int n, const double* A0, const double* b0, double* x) {
+#pragma clang fp contract(off)
+  double A[kMaxWindow * kMaxWindow], L[kMaxWindow * kMaxWindow], b[kMaxWindow];
+  for (int i = 0; i < n * n; ++i) A[i] = L[i] = A0[i];
+  for (int i = 0; i < n; ++i) b[i] = b0[i];
+  bool ok = true;
+  for (int j = 0; j < n && ok; ++j) {
+    double d = L[j * n + j];
+    for (int k = 0; k < j; ++k) d -= L[j * n + k] * L[j * n + k];
+    if (!(d > 0.0)) {
+      ok = false;
+      break;
+    }
+    d = sqrt(d);
+    L[j * n + j] = d;
+    for (int i = j + 1; i < n; ++i) {
+      double t = L[i * n + j];
+      for (int k = 0; k < j; ++k) t -= L[i * n + k] * L[j * n + k];
+      L[i * n + j] = t / d;
+    }
+  }
+  if (ok) {
+    double y[kMaxWindow];
+    for (int i = 0; i < n; ++i) {
+      double t = b[i];
+      for (int k = 0; k < i; ++k) t -= L[i * n + k] * y[k];
+      y[i] = t / L[i * n + i];
+    }
+    for (int i = n - 1; i >= 0; --i) {
+      double t = y[i];
+      for (int k = i + 1; k < n; ++k) t -= L[k * n + i] * x[k];
+      x[i] = t / L[i * n + i];
+    }
+    return;
+  }
+  for (int k = 0; k < n; ++k) {
+    int pr = k;
+    for (int i = k + 1; i < n; ++i)
+      if (fabs(A[i * n + k]) > fabs(A[pr * n + k])) pr = i;
+    if (pr != k) {
+      for (int j = 0; j < n; ++j) {
+        const double t = A[k * n + j];
+        A[k * n + j] = A[pr * n + j];
+        A[pr * n + j] = t;
+      }
+      const double t = b[k];
+      b[k] = b[pr];
+      b[pr] = t;
+    }
+    for (int i = k + 1; i < n; ++i) {
+      const double f = A[i * n + k] / A[k * n + k];
+      for (int j = k; j < n; ++j) A[i * n + j] -= f * A[k * n + j];
+      b[i] -= f * b[k];
+    }
+  }
+  for (int i = n - 1; i >= 0; --i) {
+    double t = b[i];
+    for (int j = i + 1; j < n; ++j) t -= A[i * n + j] * x[j];
+    x[i] = t / A[i * n + i];
+  }
+}
+
+// The window's least squares: the same solve on an n x n matrix stored with
+// row stride kMaxWindow (n <= kMaxWindow), padded to kMaxWindow with the
+// identity, so that every loop has constant bounds and the device keeps the
+// factor in registers.  The padding adds exact zeros to every sum of the
+// leading n x n solve, whose result it therefore leaves unchanged.  A matrix
+// that is not positive definite goes to small_solve's pivoted LU.
+__host__ __device__ __forceinline__ void window_solve(int n, const double* A0, const double* b0, double* x) {
+#pragma clang fp contract(off)
+  constexpr int M = kMaxWindow;
+  double L[M * M], y[M], xs[M];
+  bool ok = true;
+#pragma unroll
+  for (int i = 0; i < M; ++i)
+#pragma unroll
+    for (int j = 0; j < M; ++j) L[i * M + j] = (i < n && j < n) ? A0[i * M + j] : (i == j ? 1.0 : 0.0);
+#pragma unroll
+  for (int j = 0; j < M; ++j) {
+    double d = L[j * M + j];
+#pragma unroll
+    for (int k = 0; k < j; ++k) d -= L[j * M + k] * L[j * M + k];
+    ok = ok && d > 0.0;
+    d = sqrt(d);
+    L[j * M + j] = d;
+#pragma unroll
+    for (int i = j + 1; i < M; ++i) {
+      double t = L[i * M + j];
+#pragma unroll
+      for (int k = 0; k < j; ++k) t -= L[i * M + k] * L[j * M + k];
+      L[i * M + j] = t / d;
+    }
+  }
+  if (ok) {
+#pragma unroll
+    for (int i = 0; i < M; ++i) {
+      double t = i < n ? b0[i] : 0.0;
+#pragma unroll
+      for (int k = 0; k < i; ++k) t -= L[i * M + k] * y[k];
+      y[i] = t / L[i * M + i];
+    }
+#pragma unroll
+    for (int i = M - 1; i >= 0; --i) {
+      double t = y[i];
+#pragma unroll
+      for (int k = i + 1; k < M; ++k) t -= L[k * M + i] * xs[k];
+      xs[i] = t / L[i * M + i];
+    }
+#pragma unroll
+    for (int i = 0; i < M; ++i)
+      if (i < n) x[i] = xs[i];
+    return;
+  }
+  double A[M * M], b[M];
+  for (int i = 0; i < n; ++i) {
+    for (int j = 0; j < n; ++j) A[i * n + j] = A0[i * M + j];
+    b[i] = b0[i];
+  }
+  small_solve(n, A, b, x);
+}
+
 
 struct AndersonPlan {
   int kind;  // 0 FP64 map, 1 FP32c map
@@ -70,30 +192,36 @@ class AndersonRun {
     done = false;
   }
 
-  AndersonPlan plan() {
+  __host__ __device__ __forceinline__ AndersonPlan plan() {
     const bool anderson_step = p_.use_anderson && since_reset_ > 0 && !rebase_;
     AndersonPlan pl;
     pl.col = anderson_step ? slot_ % mw_ : -1;
-    pl.hist = anderson_step ? std::min(history_ + 1, mw_) : (rebase_ ? history_ : 0);
+    pl.hist = anderson_step ? (history_ + 1 < mw_ ? history_ + 1 : mw_) : (rebase_ ? history_ : 0);
     pl.kind = use32_ ? 1 : 0;
     plan_ = pl;
     return pl;
   }
 
   // Returns COLLOP_OK or COLLOP_ERR_NONFINITE.
-  int after_map(const double* st, AndersonAct& a, const char* trace_tag = nullptr) {
+  __host__ __device__ __forceinline__ int after_map(const double* st, AndersonAct& a) {
+#pragma clang fp contract(off)
     a = AndersonAct{};
     const int k = ++k_;
     const int col = plan_.col, hist_new = plan_.hist;
     since_reset_++;
-    const double nrm_r = std::sqrt(st[0]);
-    const double nrm_v1 = std::sqrt(st[1]);
-    if (!std::isfinite(nrm_r) || !std::isfinite(nrm_v1)) return COLLOP_ERR_NONFINITE;
+    const double nrm_r = sqrt(st[0]);
+    const double nrm_v1 = sqrt(st[1]);
+    last_nrm_r = nrm_r;
+    last_eff_tol = 0.0;
+    if (!__builtin_isfinite(nrm_r) || !__builtin_isfinite(nrm_v1)) return COLLOP_ERR_NONFINITE;
     // the plain map's residual: the exit, best-iterate and stagnation tests
     // (with stiff clusters, nrm_r is that of the map Anderson mixes)
-    const double nrm_map = std::sqrt(std::max(st[0] + st[kStatTrue], 0.0));
+    const double nrm_map = sqrt(fmax(st[0] + st[kStatTrue], 0.0));
+    last_nrm_map = nrm_map;
+    last_f32 = use32_;
     if (k == 1) nrm_r0 = nrm_map;
-    const double eff_tol = std::max(p_.tol * (nrm_v1 + 1e-30), p_.abs_floor);
+    const double eff_tol = fmax(p_.tol * (nrm_v1 + 1e-30), p_.abs_floor);
+    last_eff_tol = eff_tol;
     if (use32_) {
       iters32++;
       // FP32 maps run until their own rounding floor (a few FP32 ulps of the
@@ -102,17 +230,15 @@ class AndersonRun {
       // exit map are FP64.  Switching well inside the basin keeps the FP64
       // maps on the root the plain FP64 solve finds (the implicit system can
       // have several roots at large dt).
-      const double floor32 = floor_factor_ * 5.96e-8 * std::sqrt(st[2]);
+      const double floor32 = floor_factor_ * 5.96e-8 * sqrt(st[2]);
+      last_floor32 = floor32;
       if (nrm_r < best32_) {
         best32_ = nrm_r;
         since_best32_ = 0;
       } else {
         since_best32_++;
       }
-      const bool switch_now = nrm_r < std::max(floor32, eff_tol) || since_best32_ >= 4 || k >= p_.max_iter / 2;
-      if (trace_tag)
-        std::fprintf(stderr, "  [%s f32] k=%d |r|=%.3e floor=%.3e best=%.3e\n", trace_tag, k, nrm_r, floor32,
-                     best32_);
+      const bool switch_now = nrm_r < fmax(floor32, eff_tol) || since_best32_ >= 4 || k >= p_.max_iter / 2;
       if (switch_now && k >= p_.max_iter) return finish_max_iter(a);
       if (switch_now && p_.f32_keep_window && !clusters_ && p_.use_anderson && history_ > 0) {
         // Keep the secant window built from FP32 maps (its columns are
@@ -134,8 +260,6 @@ class AndersonRun {
         return COLLOP_OK;
       }
     } else {
-      if (trace_tag)
-        std::fprintf(stderr, "  [%s f64] k=%d |r|=%.3e (mixed %.3e) tol=%.3e\n", trace_tag, k, nrm_map, nrm_r, eff_tol);
       if (nrm_map < nrm_best_) {
         nrm_best_ = nrm_map;
         a.copy_best = true;
@@ -182,38 +306,51 @@ class AndersonRun {
     if (!rebased) {
       slot_ = slot_ % mw_ + 1;
       history_ = hist_new;
-      for (int j = 0; j < history_; ++j) {
-        ATA_[col * kMaxWindow + j] = st[3 + j];
-        ATA_[j * kMaxWindow + col] = st[3 + j];
-      }
+      // new Gram row and column (loops over the full window with constant
+      // indices, so that the device keeps the matrix in registers)
+#pragma unroll
+      for (int i = 0; i < kMaxWindow; ++i)
+#pragma unroll
+        for (int j = 0; j < kMaxWindow; ++j) {
+          if (j < history_ && (i == col)) ATA_[i * kMaxWindow + j] = st[3 + j];
+          if (i < history_ && (j == col)) ATA_[i * kMaxWindow + j] = st[3 + i];
+        }
     }
     const int h = history_;
     double A[kMaxWindow * kMaxWindow], b[kMaxWindow];
     double dm = 0.0;
-    for (int i = 0; i < h; ++i) {
-      for (int c = 0; c < h; ++c) A[i * h + c] = ATA_[i * kMaxWindow + c];
-      b[i] = st[3 + kMaxWindow + i];
-      dm += A[i * h + i];
+#pragma unroll
+    for (int i = 0; i < kMaxWindow; ++i) {
+#pragma unroll
+      for (int c = 0; c < kMaxWindow; ++c) A[i * kMaxWindow + c] = (i < h && c < h) ? ATA_[i * kMaxWindow + c] : 0.0;
+      b[i] = i < h ? st[3 + kMaxWindow + i] : 0.0;
+      if (i < h) dm += A[i * kMaxWindow + i];
     }
     dm /= h;
     const double l2 = p_.reg_factor * dm + 1e-30;
-    for (int i = 0; i < h; ++i) A[i * h + i] += l2;
-    small_solve(h, A, b, a.gamma);
+#pragma unroll
+    for (int i = 0; i < kMaxWindow; ++i)
+      if (i < h) A[i * kMaxWindow + i] += l2;
+    window_solve(h, A, b, a.gamma);
     a.h = h;
     return COLLOP_OK;
   }
 
-  int maps() const { return k_; }
+  __host__ __device__ int maps() const { return k_; }
+  __host__ __device__ double best32() const { return best32_; }
 
   // results of the step
   int iters32 = 0, n_restart = 0, exit_reason = 2, k_done = 0;
   double nrm_r0 = 0.0, res_out = 0.0;
   bool done = false;
+  // the last map's figures, for tracing
+  double last_nrm_r = 0.0, last_nrm_map = 0.0, last_eff_tol = 0.0, last_floor32 = 0.0;
+  bool last_f32 = false;
 
  private:
   // The map budget is spent: the best iterate (the initial guess if no FP64
   // map improved on it) is the result.
-  int finish_max_iter(AndersonAct& a) {
+  __host__ __device__ __forceinline__ int finish_max_iter(AndersonAct& a) {
     a.exit = kMaxIter;
     exit_reason = 2;
     k_done = p_.max_iter;
@@ -229,5 +366,26 @@ class AndersonRun {
   bool use32_ = false, rebase_ = false, clusters_ = false;
   AndersonPlan plan_{0, -1, 0};
 };
+
+// Control block of the device-resident solve (one per context, in device
+// memory): the kernels of map k read gate = {live, kind, col, hist}, which the
+// control kernel of map k - 1 wrote.
+struct AndCtl {
+  int gate[4];
+  int maps;  // maps whose control ran
+  int err;   // non-finite residual in map `maps`
+  AndersonRun run;
+  AndersonAct act;  // the action of map `maps`
+  double st[kStatWidth];
+};
+
+// Control kernel of map k (one workgroup): reduce the epilogue partials
+// part[n][kStatWidth], run the state machine, plan map k + 1; publishes
+// pub[1..3] = {live, kind, err}, then pub[0] = seq0 + k (pinned memory,
+// device-side address).
+void launch_anderson_ctl(const double* part, int n, AndCtl* c, int k, unsigned* pub, unsigned seq0, hipStream_t st);
+// The action of map k on the 2N components (v1, rp, Gp, best).
+void launch_anderson_apply(const AndCtl* c, int k, double* v1, const double* Gv, const double* dG, const double* r,
+                           double* rp, double* Gp, double* best, int N, int exit_picard, hipStream_t st);
 
 }  // namespace collop

@@ -233,8 +233,10 @@ void prof_begin(collop_ctx* ctx, int kind) {
     (void)hipEventCreate(&b);
     ctx->ev_pool.push_back({a, b});
     ctx->ev_kind.push_back(0);
+    ctx->ev_map.push_back(0);
   }
   ctx->ev_kind[ctx->ev_used] = kind;
+  ctx->ev_map[ctx->ev_used] = ctx->prof_map;
   (void)hipEventRecord(ctx->ev_pool[ctx->ev_used].first, ctx->stream);
 }
 
@@ -244,10 +246,13 @@ void prof_end(collop_ctx* ctx) {
   ctx->ev_used++;
 }
 
-// Called after a stream synchronisation.
-void prof_collect(collop_ctx* ctx) {
+// Called after a stream synchronisation.  Launches of maps after map_cut of a
+// device-resident solve (launched, then skipped once the solve had ended) are
+// not counted.
+void prof_collect(collop_ctx* ctx, int map_cut = 1 << 30) {
   if (!ctx->profile) return;
   for (size_t i = 0; i < ctx->ev_used; ++i) {
+    if (ctx->ev_map[i] > map_cut) continue;
     float ms = 0.0f;
     if (hipEventElapsedTime(&ms, ctx->ev_pool[i].first, ctx->ev_pool[i].second) == hipSuccess) {
       ctx->prof_ms[ctx->ev_kind[i]] += ms;
@@ -257,39 +262,43 @@ void prof_collect(collop_ctx* ctx) {
   ctx->ev_used = 0;
 }
 
+// Waits until the pinned sequence number reaches `target` (published by the
+// device after its writes to pinned memory, which is coherent, so they reach
+// the host without a synchronisation): the host polls instead of synchronising
+// the stream, a round trip about 5 us shorter.  After 20 ms (long maps) it
+// synchronises the stream; stat_syncs counts the polls that ran out.
+int wait_seq(collop_ctx* ctx, unsigned target) {
+  const auto t0 = std::chrono::steady_clock::now();
+  for (unsigned it = 1;; ++it) {
+    if ((int)(__atomic_load_n(ctx->h_seq, __ATOMIC_ACQUIRE) - target) >= 0) return COLLOP_OK;
+    if ((it & 255) == 0 && std::chrono::steady_clock::now() - t0 > std::chrono::milliseconds(20)) break;
+  }
+  ctx->stat_syncs++;
+  HIPC(hipStreamSynchronize(ctx->stream));
+  return COLLOP_OK;
+}
+
 // The map's Anderson statistics: one kernel reduces the epilogue partials into
-// the pinned h_stat and then publishes a new sequence number; the host polls
-// that number instead of synchronising the stream, a round trip about 5 us
-// shorter (both live in coherent pinned memory, so the device's writes reach the
-// host without a synchronisation).  After 20 ms (long maps), or when profiling,
-// it synchronises; stat_syncs counts the polls that ran out.
+// the pinned h_stat and then publishes a new sequence number (wait_seq).
 int publish_stat(collop_ctx* ctx, int nparts) {
   launch_stat_publish(P<double>(ctx->b_statp), nparts, ctx->h_stat_dev, ctx->h_seq_dev, ++ctx->seq, ctx->stream);
   CHECK_LAUNCH();
-  if (!ctx->profile) {
-    const auto t0 = std::chrono::steady_clock::now();
-    for (unsigned it = 1;; ++it) {
-      if (__atomic_load_n(ctx->h_seq, __ATOMIC_ACQUIRE) == ctx->seq) return COLLOP_OK;
-      if ((it & 255) == 0 && std::chrono::steady_clock::now() - t0 > std::chrono::milliseconds(20)) break;
-    }
-    ctx->stat_syncs++;
-  }
-  HIPC(hipStreamSynchronize(ctx->stream));
+  int r;
+  if ((r = wait_seq(ctx, ctx->seq))) return r;
   if (__atomic_load_n(ctx->h_seq, __ATOMIC_ACQUIRE) != ctx->seq)
     return fail(ctx, COLLOP_ERR_HIP, "the statistics' sequence number was not published");
-  prof_collect(ctx);
   return COLLOP_OK;
 }
 
 // ---- building blocks on device buffers -------------------------------------
 // part <- pair sum of the packed sources (FP64: kind 0; FP32c: kind 1)
-int run_pair(collop_ctx* ctx, int kind, int N, double* part) {
+int run_pair(collop_ctx* ctx, int kind, int N, double* part, const int* live = nullptr) {
   if (kind == 0) {
     prof_begin(ctx, 0);
-    launch_pair_f64(P<Src64>(ctx->b_s64), N, ctx->tile64, part, ctx->stream);
+    launch_pair_f64(P<Src64>(ctx->b_s64), N, ctx->tile64, part, ctx->stream, live);
   } else {
     prof_begin(ctx, 1);
-    launch_pair_f32c(P<Src32>(ctx->b_s32), N, ctx->tile32, part, ctx->stream);
+    launch_pair_f32c(P<Src32>(ctx->b_s32), N, ctx->tile32, part, ctx->stream, live);
   }
   CHECK_LAUNCH();
   prof_end(ctx);
@@ -1447,71 +1456,15 @@ int newton_solve(collop_ctx* ctx, const collop_step_params& p, collop_step_stats
   return COLLOP_OK;
 }
 
-}  // namespace
-
-namespace collop {
-
-// Small SPD solve (Cholesky; Julia's factorize picks it for the exactly
-// symmetric ATA), with partial-pivot LU as the fallback.
-void small_solve(int n, const double* A0, const double* b0, double* x) {
-  double A[kMaxWindow * kMaxWindow], L[kMaxWindow * kMaxWindow], b[kMaxWindow];
-  std::memcpy(A, A0, sizeof(double) * n * n);
-  std::memcpy(b, b0, sizeof(double) * n);
-  std::memcpy(L, A0, sizeof(double) * n * n);
-  bool ok = true;
-  for (int j = 0; j < n && ok; ++j) {
-    double d = L[j * n + j];
-    for (int k = 0; k < j; ++k) d -= L[j * n + k] * L[j * n + k];
-    if (!(d > 0.0)) {
-      ok = false;
-      break;
-    }
-    d = std::sqrt(d);
-    L[j * n + j] = d;
-    for (int i = j + 1; i < n; ++i) {
-      double t = L[i * n + j];
-      for (int k = 0; k < j; ++k) t -= L[i * n + k] * L[j * n + k];
-      L[i * n + j] = t / d;
-    }
-  }
-  if (ok) {
-    double y[kMaxWindow];
-    for (int i = 0; i < n; ++i) {
-      double t = b[i];
-      for (int k = 0; k < i; ++k) t -= L[i * n + k] * y[k];
-      y[i] = t / L[i * n + i];
-    }
-    for (int i = n - 1; i >= 0; --i) {
-      double t = y[i];
-      for (int k = i + 1; k < n; ++k) t -= L[k * n + i] * x[k];
-      x[i] = t / L[i * n + i];
-    }
-    return;
-  }
-  for (int k = 0; k < n; ++k) {
-    int pr = k;
-    for (int i = k + 1; i < n; ++i)
-      if (std::fabs(A[i * n + k]) > std::fabs(A[pr * n + k])) pr = i;
-    if (pr != k) {
-      for (int j = 0; j < n; ++j) std::swap(A[k * n + j], A[pr * n + j]);
-      std::swap(b[k], b[pr]);
-    }
-    for (int i = k + 1; i < n; ++i) {
-      const double f = A[i * n + k] / A[k * n + k];
-      for (int j = k; j < n; ++j) A[i * n + j] -= f * A[k * n + j];
-      b[i] -= f * b[k];
-    }
-  }
-  for (int i = n - 1; i >= 0; --i) {
-    double t = b[i];
-    for (int j = i + 1; j < n; ++j) t -= A[i * n + j] * x[j];
-    x[i] = t / A[i * n + i];
-  }
+// One line per map of a traced solve (COLLOP_TRACE set).
+void trace_map(const AndersonRun& an, const char* tag) {
+  if (an.last_f32)
+    std::fprintf(stderr, "  [%s f32] k=%d |r|=%.3e floor=%.3e best=%.3e\n", tag, an.maps(), an.last_nrm_r,
+                 an.last_floor32, an.best32());
+  else
+    std::fprintf(stderr, "  [%s f64] k=%d |r|=%.3e (mixed %.3e) tol=%.3e\n", tag, an.maps(), an.last_nrm_map,
+                 an.last_nrm_r, an.last_eff_tol);
 }
-
-}  // namespace collop
-
-namespace {
 
 int copy_d2d(collop_ctx* ctx, DBuf& dst, const DBuf& src, size_t bytes) {
   HIPC(hipMemcpyAsync(dst.p, src.p, bytes, hipMemcpyDeviceToDevice, ctx->stream));
@@ -1803,6 +1756,85 @@ int defect_solve(collop_ctx* ctx, const collop_step_params& p, collop_step_stats
   return COLLOP_OK;
 }
 
+// The Anderson solve of a Landau step with its control on the device
+// (solver.jl:300-423; no stiff clusters): per map the host only launches work.
+// The kernels of map k read the plan {live, kind, col, hist} that the control
+// kernel of map k - 1 wrote, and the control kernel of map k runs the state
+// machine (the window's least squares included) and plans map k + 1.  While the
+// maps are FP32c the host launches map k + 1 once map k has published (its
+// precision is decided there); once they are FP64, which they then stay, it
+// launches map k + 1 while map k runs.  The launches of a map after the exit
+// return at once.  Same arithmetic, and so the same bits, as the host-controlled
+// loop.
+int anderson_device(collop_ctx* ctx, const collop_step_params& p, AndersonRun& an) {
+  const int n = (int)ctx->N;
+  ENSURE(ctx->b_actl, sizeof(AndCtl));
+  AndCtl* dc = P<AndCtl>(ctx->b_actl);
+  ctx->h_actl.resize(sizeof(AndCtl));
+  const AndersonPlan pl0 = an.plan();
+  AndCtl h{};
+  h.gate[0] = 1;
+  h.gate[1] = pl0.kind;
+  h.gate[2] = pl0.col;
+  h.gate[3] = pl0.hist;
+  h.maps = 0;
+  h.err = 0;
+  h.run = an;
+  std::memcpy(ctx->h_actl.data(), &h, sizeof(AndCtl));
+  HIPC(hipMemcpyAsync(dc, ctx->h_actl.data(), sizeof(AndCtl), hipMemcpyHostToDevice, ctx->stream));
+  const unsigned seq0 = ctx->seq;
+  volatile unsigned* pub = ctx->h_seq;
+  bool f32 = pl0.kind == 1;
+  int k = 0, seen = 0, r;
+  while (k < p.max_iter) {
+    const int need = f32 ? k : k - 1;
+    if (need > seen) {
+      if ((r = wait_seq(ctx, seq0 + (unsigned)need))) return r;
+      seen = (int)(__atomic_load_n(ctx->h_seq, __ATOMIC_ACQUIRE) - seq0);
+      if (pub[1] == 0 || pub[3] != 0) break;  // the solve has ended
+      if (seen >= k) f32 = pub[2] == 1;       // precision of map seen + 1
+    }
+    const int kind = f32 ? 1 : 0;
+    ctx->prof_map = ++k;
+    if ((r = geff_dev(ctx, p, P<double>(ctx->s_v1), kind == 0 ? P<Src64>(ctx->b_s64) : nullptr,
+                      kind == 1 ? P<Src32>(ctx->b_s32) : nullptr, nullptr, nullptr, nullptr, nullptr)))
+      return r;
+    if ((r = run_pair(ctx, kind, n, P<double>(ctx->b_part), &dc->gate[0]))) return r;
+    EpiArgs a{};
+    a.part = P<double>(ctx->b_part);
+    a.S = ctx->tile64.slices;
+    a.S32 = ctx->tile32.slices;
+    a.gate = &dc->gate[0];
+    a.v0 = P<double>(ctx->s_v);
+    a.v1 = P<double>(ctx->s_v1);
+    a.dt = p.dt;
+    a.Gv = P<double>(ctx->s_Gv);
+    a.r = P<double>(ctx->s_r);
+    a.rp = P<double>(ctx->s_rp);
+    a.Gp = P<double>(ctx->s_Gp);
+    a.dF = P<double>(ctx->s_dF);
+    a.dG = P<double>(ctx->s_dG);
+    a.stat_part = P<double>(ctx->b_statp);
+    launch_epilogue(a, n, ctx->stream);
+    CHECK_LAUNCH();
+    launch_anderson_ctl(P<double>(ctx->b_statp), epi_blocks(n), dc, k, ctx->h_seq_dev, seq0, ctx->stream);
+    CHECK_LAUNCH();
+    launch_anderson_apply(dc, k, P<double>(ctx->s_v1), P<double>(ctx->s_Gv), P<double>(ctx->s_dG),
+                          P<double>(ctx->s_r), P<double>(ctx->s_rp), P<double>(ctx->s_Gp), P<double>(ctx->s_best), n,
+                          p.exit_picard_step, ctx->stream);
+    CHECK_LAUNCH();
+  }
+  ctx->prof_map = 0;
+  HIPC(hipMemcpyAsync(ctx->h_actl.data(), dc, sizeof(AndCtl), hipMemcpyDeviceToHost, ctx->stream));
+  HIPC(hipStreamSynchronize(ctx->stream));
+  std::memcpy(&h, ctx->h_actl.data(), sizeof(AndCtl));
+  ctx->seq = seq0 + (unsigned)h.maps;
+  prof_collect(ctx, h.maps);
+  if (h.err) return fail(ctx, COLLOP_ERR_NONFINITE, "non-finite residual in Picard map " + std::to_string(h.maps));
+  an = h.run;
+  return COLLOP_OK;
+}
+
 // One implicit step on the device-resident state (main.jl:226-307 with
 // step_anderson!, solver.jl:300-423).
 int step_once(collop_ctx* ctx, const collop_step_params& p, collop_step_stats* out) {
@@ -1886,6 +1918,9 @@ int step_once(collop_ctx* ctx, const collop_step_params& p, collop_step_stats* o
   if (p.solver == COLLOP_SOLVER_ANDERSON && (r = copy_d2d(ctx, ctx->s_best, ctx->s_v1, n2b))) return r;
   const char* trace = std::getenv("COLLOP_TRACE") ? "run" : nullptr;
   const bool lb_fused = lb && ctx->fused_maps && m.nd1 <= 256 && m.nd2 <= 256;
+  if (p.solver == COLLOP_SOLVER_ANDERSON && ctx->dev_solve && !lb && ctx->ncl == 0 && !trace) {
+    if ((r = anderson_device(ctx, p, an))) return r;
+  }
   while (p.solver == COLLOP_SOLVER_ANDERSON && !an.done) {
     const AndersonPlan pl = an.plan();
     if (lb_fused) {
@@ -1896,8 +1931,9 @@ int step_once(collop_ctx* ctx, const collop_step_params& p, collop_step_stats* o
       return r;
     }
     AndersonAct act;
-    if (an.after_map(ctx->h_stat, act, trace) != COLLOP_OK)
+    if (an.after_map(ctx->h_stat, act) != COLLOP_OK)
       return fail(ctx, COLLOP_ERR_NONFINITE, "non-finite residual in Picard map " + std::to_string(an.maps()));
+    if (trace) trace_map(an, trace);
     // the best iterate is saved by the update's launch when one follows, else by a copy
     const bool best_in_update = act.copy_best && act.exit == kContinue && act.update;
     if (act.copy_best && !best_in_update &&
@@ -1925,6 +1961,25 @@ int step_once(collop_ctx* ctx, const collop_step_params& p, collop_step_stats* o
       std::swap(ctx->s_rp, ctx->s_r);
       std::swap(ctx->s_Gp, ctx->s_Gv);
     }
+  }
+  if (trace && ctx->ncl > 0) {  // per cluster size: clusters, Newton iterations of the step, failures
+    std::vector<int> off(ctx->ncl + 1), its(ctx->ncl), have(ctx->ncl);
+    HIPC(hipMemcpy(off.data(), ctx->b_cl_off.p, off.size() * 4, hipMemcpyDeviceToHost));
+    HIPC(hipMemcpy(its.data(), ctx->b_cl_its.p, its.size() * 4, hipMemcpyDeviceToHost));
+    HIPC(hipMemcpy(have.data(), ctx->b_cl_have.p, have.size() * 4, hipMemcpyDeviceToHost));
+    int cnt[kClusterMax + 1] = {0}, mx[kClusterMax + 1] = {0}, bad[kClusterMax + 1] = {0};
+    long sum[kClusterMax + 1] = {0};
+    for (int c = 0; c < ctx->ncl; ++c) {
+      const int k = off[c + 1] - off[c];
+      cnt[k]++;
+      sum[k] += its[c];
+      mx[k] = std::max(mx[k], its[c]);
+      bad[k] += have[c] < 0;
+    }
+    for (int k = 2; k <= kClusterMax; ++k)
+      if (cnt[k])
+        std::fprintf(stderr, "  [clusters] size %2d: %4d clusters, Newton its mean %.1f max %d, unsolved at exit %d\n", k,
+                     cnt[k], (double)sum[k] / cnt[k], mx[k], bad[k]);
   }
   // v <- v1 and the new projection / entropy / moments (main.jl:296-303)
   std::swap(ctx->s_v, ctx->s_v1);
@@ -2334,6 +2389,13 @@ int collop_set_fused_maps(collop_ctx* ctx, int on) {
   if (!ctx) return COLLOP_ERR_INVALID;
   if (on != 0 && on != 1) return fail(ctx, COLLOP_ERR_INVALID, "on must be 0 or 1");
   ctx->fused_maps = on == 1;
+  return COLLOP_OK;
+}
+
+int collop_set_device_solve(collop_ctx* ctx, int on) {
+  if (!ctx) return COLLOP_ERR_INVALID;
+  if (on != 0 && on != 1) return fail(ctx, COLLOP_ERR_INVALID, "on must be 0 or 1");
+  ctx->dev_solve = on == 1;
   return COLLOP_OK;
 }
 

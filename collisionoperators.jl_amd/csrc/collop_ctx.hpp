@@ -63,6 +63,10 @@ struct collop_ctx {
   unsigned* h_seq_dev = nullptr;  // its device-side address
   unsigned seq = 0;               // the last sequence number launched
   long stat_syncs = 0;            // statistics polls that ran out (a stream synchronisation instead)
+  // ---- device-resident Anderson solve (control block AndCtl in b_actl)
+  bool dev_solve = true;               // Landau Anderson steps keep their control on the device
+  DBuf b_actl;
+  std::vector<unsigned char> h_actl;   // host image of the control block (upload / read-back)
   // ---- warm-start MLP
   DBuf m_params, m_adam_m, m_adam_v, m_data, m_rec, m_y, m_gpart, m_lpart, m_tiles, m_eval, m_grad;
   bool m_has_params = false, m_has_norm = false;
@@ -81,6 +85,8 @@ struct collop_ctx {
   std::vector<std::pair<hipEvent_t, hipEvent_t>> ev_pool;
   size_t ev_used = 0;
   std::vector<int> ev_kind;
+  std::vector<int> ev_map;  // map of the step each event pair times (device-resident solve)
+  int prof_map = 0;         // map the next events belong to (0: not in a device-resident solve)
   double prof_ms[4] = {0.0, 0.0, 0.0, 0.0};  // FP64, FP32c, FP64 + metric, FP32c + metric
   long prof_n[4] = {0, 0, 0, 0};
   // ---- batched ensemble (ensemble.hip)

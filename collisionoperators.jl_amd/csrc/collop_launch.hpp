@@ -71,6 +71,9 @@ struct NearOut {
   int2* list;
   int cap;
   double qmax;
+  // device flag of the device-resident solve: 0 = the solve has ended, the
+  // launch returns at once (nullptr: always run)
+  const int* live = nullptr;
 };
 constexpr int kNearCap = 1 << 16;  // near pairs kept per step (beyond it the near field is skipped)
 struct PairTile {
@@ -87,18 +90,20 @@ struct PairTile {
 __host__ __device__ inline int sym_rows(int N, int R) { return (N + 128 * R - 1) / (128 * R); }
 constexpr int kSymMaxSlots = 256;  // larger N falls back to the ordered sum
 void launch_pair_sym_f64(const Src64* src, int N, int R, bool metric, const NearOut* near, double* part,
-                         hipStream_t st);
+                         hipStream_t st, const int* live = nullptr);
 void launch_pair_sym_f32c(const Src32* src, int N, int R, bool metric, const NearOut* near, double* part,
-                          hipStream_t st);
+                          hipStream_t st, const int* live = nullptr);
 
 // Pack (v, G, w) into Src64/Src32 records.  Markers outside the open box get
 // w_masked = 0 and in_domain = 0 (functions.jl:212-213, :219-220).
 void launch_pack_src(const double* v, const double* G, const double* w, int N, double lo1, double hi1, double lo2,
                      double hi2, Src64* s64, Src32* s32, hipStream_t st);
 // part[S][2N] <- slice sums of the FP64 pair sum
-void launch_pair_f64(const Src64* src, int N, const PairTile& t, double* part, hipStream_t st);
+void launch_pair_f64(const Src64* src, int N, const PairTile& t, double* part, hipStream_t st,
+                     const int* live = nullptr);
 // part[S][2N] <- slice sums of the FP32 pair sum (compensated, written in FP64)
-void launch_pair_f32c(const Src32* src, int N, const PairTile& t, double* part, hipStream_t st);
+void launch_pair_f32c(const Src32* src, int N, const PairTile& t, double* part, hipStream_t st,
+                      const int* live = nullptr);
 // part[S][5N] <- slice sums of F (2 comps) and A (3 comps); near != null also
 // collects the near pairs (one target per lane, one Newton step)
 void launch_pair_metric_f64(const Src64* src, int N, const PairTile& t, double* part, hipStream_t st,
@@ -230,6 +235,11 @@ struct EpiArgs {
   // none); the map value of such a row is cl_x[2 row + comp] (the cluster
   // solve's); the plain map's residual there enters statistic kStatTrue
   const int* cl_row; const double* cl_x;
+  // device-resident solve (null: host control): gate = {live, kind, col, hist}
+  // of the map, written by the control kernel of the previous map; the launch
+  // returns at once when live = 0, takes col / hist from it, and S32 slices
+  // instead of S when kind = 1 (FP32c)
+  const int* gate; int S32;
 };
 int epi_blocks(int N);
 constexpr int kStatTrue = 3 + 2 * kMaxWindow;  // |Gv_map - v1|^2 - |Gv - v1|^2 over the cluster rows

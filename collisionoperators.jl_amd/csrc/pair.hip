@@ -29,6 +29,7 @@ constexpr int kThreads = 256;
 template <int K, int NEWTON, bool METRIC, bool NEAR = false>
 __global__ __launch_bounds__(kThreads) void k_pair_f64(const Src64* __restrict__ src, int N, int S, int chunk,
                                                         double* __restrict__ part, NearOut near = NearOut{}) {
+  if (near.live && *near.live == 0) return;
   const int bid = blockIdx.x;
   const int slice = bid % S;
   const int gblk = bid / S;
@@ -96,6 +97,7 @@ __global__ __launch_bounds__(kThreads) void k_pair_f64(const Src64* __restrict__
 template <int K, int BLK, bool METRIC, bool NEAR = false>
 __global__ __launch_bounds__(kThreads) void k_pair_f32c(const Src32* __restrict__ src, int N, int S, int chunk,
                                                          double* __restrict__ part, NearOut near = NearOut{}) {
+  if (near.live && *near.live == 0) return;
   const int bid = blockIdx.x;
   const int slice = bid % S;
   const int gblk = bid / S;
@@ -320,21 +322,23 @@ void launch_pack_src(const double* v, const double* G, const double* w, int N, d
   hipLaunchKernelGGL(k_pack, dim3((N + 255) / 256), dim3(256), 0, st, v, G, w, N, lo1, hi1, lo2, hi2, s64, s32);
 }
 
-void launch_pair_f64(const Src64* src, int N, const PairTile& t, double* part, hipStream_t st) {
+void launch_pair_f64(const Src64* src, int N, const PairTile& t, double* part, hipStream_t st, const int* live) {
   if (N <= 0) return;
-  if (t.sym) return launch_pair_sym_f64(src, N, t.sym, false, nullptr, part, st);
+  if (t.sym) return launch_pair_sym_f64(src, N, t.sym, false, nullptr, part, st, live);
   const int chunk = (N + t.slices - 1) / t.slices;
   const dim3 grid(grid_for(N, t)), block(kThreads);
+  NearOut g{};
+  g.live = live;
   if (t.gamma_per_lane == 2) {
     if (t.newton == 1)
-      hipLaunchKernelGGL((k_pair_f64<2, 1, false>), grid, block, 0, st, src, N, t.slices, chunk, part);
+      hipLaunchKernelGGL((k_pair_f64<2, 1, false>), grid, block, 0, st, src, N, t.slices, chunk, part, g);
     else
-      hipLaunchKernelGGL((k_pair_f64<2, 2, false>), grid, block, 0, st, src, N, t.slices, chunk, part);
+      hipLaunchKernelGGL((k_pair_f64<2, 2, false>), grid, block, 0, st, src, N, t.slices, chunk, part, g);
   } else {
     if (t.newton == 1)
-      hipLaunchKernelGGL((k_pair_f64<1, 1, false>), grid, block, 0, st, src, N, t.slices, chunk, part);
+      hipLaunchKernelGGL((k_pair_f64<1, 1, false>), grid, block, 0, st, src, N, t.slices, chunk, part, g);
     else
-      hipLaunchKernelGGL((k_pair_f64<1, 2, false>), grid, block, 0, st, src, N, t.slices, chunk, part);
+      hipLaunchKernelGGL((k_pair_f64<1, 2, false>), grid, block, 0, st, src, N, t.slices, chunk, part, g);
   }
 }
 
@@ -362,15 +366,17 @@ void launch_pair_metric_f64(const Src64* src, int N, const PairTile& t, double* 
   }
 }
 
-void launch_pair_f32c(const Src32* src, int N, const PairTile& t, double* part, hipStream_t st) {
+void launch_pair_f32c(const Src32* src, int N, const PairTile& t, double* part, hipStream_t st, const int* live) {
   if (N <= 0) return;
-  if (t.sym) return launch_pair_sym_f32c(src, N, t.sym, false, nullptr, part, st);
+  if (t.sym) return launch_pair_sym_f32c(src, N, t.sym, false, nullptr, part, st, live);
   const int chunk = (N + t.slices - 1) / t.slices;
   const dim3 grid(grid_for(N, t)), block(kThreads);
+  NearOut g{};
+  g.live = live;
   if (t.gamma_per_lane == 2)
-    hipLaunchKernelGGL((k_pair_f32c<2, 32, false>), grid, block, 0, st, src, N, t.slices, chunk, part);
+    hipLaunchKernelGGL((k_pair_f32c<2, 32, false>), grid, block, 0, st, src, N, t.slices, chunk, part, g);
   else
-    hipLaunchKernelGGL((k_pair_f32c<1, 32, false>), grid, block, 0, st, src, N, t.slices, chunk, part);
+    hipLaunchKernelGGL((k_pair_f32c<1, 32, false>), grid, block, 0, st, src, N, t.slices, chunk, part, g);
 }
 
 void launch_pair_metric_f32c(const Src32* src, int N, const PairTile& t, double* part, hipStream_t st,

@@ -305,6 +305,7 @@ template <int R, bool METRIC, bool NEAR>
 __global__ __launch_bounds__(64 * R) void k_pair_sym_f64(const Src64* __restrict__ src, int N, long long n_off,
                                                          double* __restrict__ part, NearOut near) {
   __shared__ SymLds64<R, METRIC, NEAR> S;
+  if (near.live && *near.live == 0) return;
   int a, b;
   block_rows(blockIdx.x, n_off, a, b);
   sym_block_f64<R, METRIC, NEAR>(src, N, a, b, part, S, near);
@@ -511,6 +512,7 @@ template <int R, bool METRIC, bool NEAR>
 __global__ __launch_bounds__(64 * R) void k_pair_sym_f32c(const Src32* __restrict__ src, int N, long long n_off,
                                                           double* __restrict__ part, NearOut near) {
   __shared__ SymLds32<R, METRIC, NEAR> S;
+  if (near.live && *near.live == 0) return;
   int a, b;
   block_rows(blockIdx.x, n_off, a, b);
   sym_block_f32c<R, METRIC, NEAR>(src, N, a, b, part, S, near);
@@ -544,40 +546,52 @@ __global__ __launch_bounds__(64 * R) void k_pair_sym_b(EnsView E, const int4* __
 }
 
 template <int R>
-void sym_f64(const Src64* src, int N, bool metric, const NearOut* near, double* part, hipStream_t st) {
+void sym_f64(const Src64* src, int N, bool metric, const NearOut* near, double* part, hipStream_t st,
+             const int* live) {
   const long long S = sym_rows(N, R), n_off = S * (S - 1) / 2;
   const dim3 grid((unsigned)(n_off + S)), block(64 * R);
-  const NearOut nz{};
+  NearOut nz{};
+  nz.live = live;
   if (!metric)
     hipLaunchKernelGGL((k_pair_sym_f64<R, false, false>), grid, block, 0, st, src, N, n_off, part, nz);
   else if (!near)
     hipLaunchKernelGGL((k_pair_sym_f64<R, true, false>), grid, block, 0, st, src, N, n_off, part, nz);
   else
-    hipLaunchKernelGGL((k_pair_sym_f64<R, true, true>), grid, block, 0, st, src, N, n_off, part, *near);
+  {
+    NearOut nn = *near;
+    nn.live = live;
+    hipLaunchKernelGGL((k_pair_sym_f64<R, true, true>), grid, block, 0, st, src, N, n_off, part, nn);
+  }
 }
 
 template <int R>
-void sym_f32c(const Src32* src, int N, bool metric, const NearOut* near, double* part, hipStream_t st) {
+void sym_f32c(const Src32* src, int N, bool metric, const NearOut* near, double* part, hipStream_t st,
+             const int* live) {
   const long long S = sym_rows(N, R), n_off = S * (S - 1) / 2;
   const dim3 grid((unsigned)(n_off + S)), block(64 * R);
-  const NearOut nz{};
+  NearOut nz{};
+  nz.live = live;
   if (!metric)
     hipLaunchKernelGGL((k_pair_sym_f32c<R, false, false>), grid, block, 0, st, src, N, n_off, part, nz);
   else if (!near)
     hipLaunchKernelGGL((k_pair_sym_f32c<R, true, false>), grid, block, 0, st, src, N, n_off, part, nz);
   else
-    hipLaunchKernelGGL((k_pair_sym_f32c<R, true, true>), grid, block, 0, st, src, N, n_off, part, *near);
+  {
+    NearOut nn = *near;
+    nn.live = live;
+    hipLaunchKernelGGL((k_pair_sym_f32c<R, true, true>), grid, block, 0, st, src, N, n_off, part, nn);
+  }
 }
 
 }  // namespace
 
 void launch_pair_sym_f64(const Src64* src, int N, int R, bool metric, const NearOut* near, double* part,
-                         hipStream_t st) {
+                         hipStream_t st, const int* live) {
   if (N <= 0) return;
   switch (R) {
-    case 2: sym_f64<2>(src, N, metric, near, part, st); break;
-    case 4: sym_f64<4>(src, N, metric, near, part, st); break;
-    default: sym_f64<8>(src, N, metric, near, part, st); break;
+    case 2: sym_f64<2>(src, N, metric, near, part, st, live); break;
+    case 4: sym_f64<4>(src, N, metric, near, part, st, live); break;
+    default: sym_f64<8>(src, N, metric, near, part, st, live); break;
   }
 }
 
@@ -599,12 +613,12 @@ void launch_pair_sym_b(const EnsView& E, const EnsTiles& T, const Src64* s64, co
 }
 
 void launch_pair_sym_f32c(const Src32* src, int N, int R, bool metric, const NearOut* near, double* part,
-                          hipStream_t st) {
+                          hipStream_t st, const int* live) {
   if (N <= 0) return;
   switch (R) {
-    case 2: sym_f32c<2>(src, N, metric, near, part, st); break;
-    case 4: sym_f32c<4>(src, N, metric, near, part, st); break;
-    default: sym_f32c<8>(src, N, metric, near, part, st); break;
+    case 2: sym_f32c<2>(src, N, metric, near, part, st, live); break;
+    case 4: sym_f32c<4>(src, N, metric, near, part, st, live); break;
+    default: sym_f32c<8>(src, N, metric, near, part, st, live); break;
   }
 }
 

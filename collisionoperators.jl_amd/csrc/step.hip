@@ -3,6 +3,7 @@
 //
 // All reductions are deterministic: per-block partials in a fixed shuffle tree,
 // then a fixed-order sum over blocks.
+#include "anderson.hpp"
 #include "collop_launch.hpp"
 #include "lb_dev.hpp"
 
@@ -296,8 +297,12 @@ __global__ __launch_bounds__(kEwThreads) void k_epilogue_lb(EpiArgs A, LbEpiArgs
 // order -- the partition and tree of launch_sum_rows' 256-thread blocks, so the
 // sums are bitwise the same.
 constexpr int kPubGroups = 4;
-__global__ __launch_bounds__(kPubGroups * 256) void k_stat_publish(const double* __restrict__ part, int n,
-                                                                   double* out, unsigned* seq_out, unsigned seq) {
+// Thread 0 of a G * 256 workgroup ends with out[c] = the sum over the n rows of
+// part[n][kStatWidth]; group g sums the columns c = g mod G, each with the
+// same partition and tree, so the sums do not depend on G.
+template <int G>
+__device__ __forceinline__ void stat_rows_sum(const double* __restrict__ part, int n, double* out) {
+  constexpr int kPubGroups = G;
   constexpr int kPer = (kStatWidth + kPubGroups - 1) / kPubGroups;
   __shared__ double wsum[kStatWidth][4];
   const int g = threadIdx.x >> 8, t = threadIdx.x & 255;
@@ -327,8 +332,104 @@ __global__ __launch_bounds__(kPubGroups * 256) void k_stat_publish(const double*
       for (int i = 0; i < 4; ++i) v += wsum[c][i];
       out[c] = v;
     }
+  }
+}
+
+__global__ __launch_bounds__(kPubGroups * 256) void k_stat_publish(const double* __restrict__ part, int n,
+                                                                   double* out, unsigned* seq_out, unsigned seq) {
+  stat_rows_sum<kPubGroups>(part, n, out);
+  if (threadIdx.x == 0) {
     __threadfence_system();
     __hip_atomic_store(seq_out, seq, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_SYSTEM);
+  }
+}
+
+// ---- device-resident Anderson solve -------------------------------------------
+// Control of map k: the statistics of its epilogue, the state machine's
+// decision (the least squares for gamma included) and the plan of map k + 1,
+// then pub = {live, kind, err} and seq = seq0 + k published to the host.  A
+// launch whose map did not run (the solve ended before it) changes nothing.
+// One group of 256 threads: the state machine of thread 0 then has the
+// register file of a whole SIMD lane to itself.
+__global__ __launch_bounds__(256) void k_anderson_ctl(const double* __restrict__ part, int n, AndCtl* __restrict__ c,
+                                                      int k, unsigned* pub, unsigned seq0) {
+  // the state machine runs on a copy in LDS (thread 0), written back after
+  constexpr int kWords = (int)(sizeof(AndersonRun) / sizeof(int));
+  static_assert(sizeof(AndersonRun) % sizeof(int) == 0, "AndersonRun is copied as ints");
+  __shared__ double st[kStatWidth];
+  __shared__ int run_map;
+  __shared__ __align__(16) int run_lds[kWords];
+  if (threadIdx.x == 0) run_map = c->gate[0] != 0 && c->maps == k - 1;
+  __syncthreads();
+  if (!run_map) return;
+  const int* src = reinterpret_cast<const int*>(&c->run);
+  for (int i = threadIdx.x; i < kWords; i += blockDim.x) run_lds[i] = src[i];
+  stat_rows_sum<1>(part, n, st);
+  __syncthreads();  // st (written by thread 0) and run_lds complete
+  if (threadIdx.x == 0) {
+    AndersonRun& run = *reinterpret_cast<AndersonRun*>(run_lds);
+    AndersonAct act;
+    const int e = run.after_map(st, act);
+    c->maps = k;
+    int live = 1, kind = 0;
+    if (e != COLLOP_OK) {
+      c->err = 1;
+      live = 0;
+    } else {
+      c->act = act;
+      if (act.exit != kContinue) {
+        live = 0;
+      } else {
+        const AndersonPlan pl = run.plan();
+        kind = pl.kind;
+        c->gate[1] = pl.kind;
+        c->gate[2] = pl.col;
+        c->gate[3] = pl.hist;
+      }
+    }
+    c->gate[0] = live;
+    for (int j = 0; j < kStatWidth; ++j) c->st[j] = st[j];
+    pub[1] = (unsigned)live;
+    pub[2] = (unsigned)kind;
+    pub[3] = (unsigned)c->err;
+  }
+  __syncthreads();
+  int* dst = reinterpret_cast<int*>(&c->run);
+  for (int i = threadIdx.x; i < kWords; i += blockDim.x) dst[i] = run_lds[i];
+  __threadfence_system();
+  __syncthreads();
+  if (threadIdx.x == 0) __hip_atomic_store(pub, seq0 + (unsigned)k, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_SYSTEM);
+}
+
+// The action of map k (if its control ran): best <- (exit_picard ? Gv : v1) when
+// the map improved the best residual; then the exit (v1 <- Gv or v1 at
+// convergence, v1 <- best otherwise) or the update v1 <- beta (Gv - dG gamma) +
+// (1 - beta) v1 followed by (rp, Gp) <- (r, Gv).
+__global__ void k_anderson_apply(const AndCtl* __restrict__ c, int k, double* v1, const double* __restrict__ Gv,
+                                 const double* __restrict__ dG, const double* __restrict__ r, double* __restrict__ rp,
+                                 double* __restrict__ Gp, double* __restrict__ best, int n2, int exit_picard) {
+  if (c->maps != k || c->err) return;
+  const AndersonAct& a = c->act;
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n2) return;
+  const double x = v1[i], g = Gv[i];
+  double b = 0.0;
+  if (a.copy_best) {
+    b = exit_picard ? g : x;
+    best[i] = b;
+  }
+  if (a.exit == kConverged) {
+    v1[i] = exit_picard ? g : x;
+    return;
+  }
+  if (a.exit != kContinue) {
+    v1[i] = a.copy_best ? b : best[i];
+    return;
+  }
+  if (a.update) v1[i] = update_value(Gv, dG, i, n2, a.gamma, a.h, a.beta, x);
+  if (a.rotate) {
+    rp[i] = r[i];
+    Gp[i] = g;
   }
 }
 
@@ -359,6 +460,12 @@ __global__ __launch_bounds__(kEwThreads) void k_geff_pack(const double* __restri
 __global__ __launch_bounds__(kEwThreads) void k_epilogue(EpiArgs A, int N) {
   __shared__ double red[kStatWidth * kEwThreads / 64];
   EpiArgs a = A;
+  if (A.gate) {  // device-resident solve: the map's plan
+    if (A.gate[0] == 0) return;
+    a.col = A.gate[2];
+    a.hist = A.gate[3];
+    if (A.gate[1]) a.S = A.S32;
+  }
   a.stat_part = A.stat_part + (size_t)blockIdx.x * kStatWidth;
   epilogue_body(a, N, N, (size_t)2 * N, red, blockIdx.x);
 }
@@ -492,6 +599,17 @@ void launch_epilogue_frozen(const EpiArgs& a, const FrozenArgs& f, int N, hipStr
 
 void launch_epilogue_lb(const EpiArgs& a, const LbEpiArgs& l, int N, hipStream_t st) {
   hipLaunchKernelGGL(k_epilogue_lb, dim3(epi_blocks(N)), dim3(kEwThreads), 0, st, a, l, N);
+}
+
+void launch_anderson_ctl(const double* part, int n, AndCtl* c, int k, unsigned* pub, unsigned seq0, hipStream_t st) {
+  hipLaunchKernelGGL(k_anderson_ctl, dim3(1), dim3(256), 0, st, part, n, c, k, pub, seq0);
+}
+
+void launch_anderson_apply(const AndCtl* c, int k, double* v1, const double* Gv, const double* dG, const double* r,
+                           double* rp, double* Gp, double* best, int N, int exit_picard, hipStream_t st) {
+  const int n2 = 2 * N;
+  hipLaunchKernelGGL(k_anderson_apply, dim3((n2 + 255) / 256), dim3(256), 0, st, c, k, v1, Gv, dG, r, rp, Gp, best,
+                     n2, exit_picard);
 }
 
 void launch_stat_publish(const double* part, int n, double* out, unsigned* seq_out, unsigned seq, hipStream_t st) {

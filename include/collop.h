@@ -379,6 +379,14 @@ int collop_set_pair_mode(collop_ctx* ctx, int mode, int tiles);
  * basis functions per direction (larger meshes always run the staged maps).
  * No counterpart in the reference. */
 int collop_set_fused_maps(collop_ctx* ctx, int on);
+/* Control of the Anderson solve of Landau steps without stiff clusters: on the
+ * device (on = 1, the default: a control kernel per map runs the window's
+ * least squares, the exit and stagnation tests and plans the next map, and the
+ * host launches the next map before the current one has finished once the maps
+ * are FP64) or on the host (0: the host reads every map's statistics and
+ * decides).  Both compute the same bits.  The host-side counterpart is
+ * step_anderson! (solver.jl:300-423), whose least squares is solver.jl:396-412. */
+int collop_set_device_solve(collop_ctx* ctx, int on);
 /* The form (mode), waves per block (tiles, 0 for the ordered sum) and partial
  * slots a pair sum over N markers runs with under the current settings; each
  * output pointer may be NULL. */

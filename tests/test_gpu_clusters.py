@@ -72,10 +72,11 @@ def test_stiff_step_plain_solve_stalls(gpu_ctx):
 
 
 def test_chosen_step_at_n65536_is_an_oracle_root(gpu_ctx):
-    """One step of the chosen path (FP32c then FP64 Anderson maps, stiff
-    clusters, learned warm start) at N = 65536 on 32 x 32 cells, dt 0.02: the
-    state is a root of the oracle's implicit system; and one step of the plain
-    FP64 path is too."""
+    """One step of the chosen path (FP32c then FP64 Anderson maps with the
+    device-resident control, learned warm start) at N = 65536 on 32 x 32
+    cells, dt 0.02: the state is a root of the oracle's implicit system; and
+    one step of the same path with stiff clusters, and of the plain FP64 path,
+    is too."""
     from collop import warmstart as W
     p = collop.chosen_config("bimodal_n65536")
     v, w = collop.initial_markers(p)
@@ -83,6 +84,8 @@ def test_chosen_step_at_n65536_is_an_oracle_root(gpu_ctx):
     W.load_packaged(ws)
     m = O.Mesh(p.bp1, p.bp2)
     for name, prm in (("chosen", p.step_params(warmstart=collop.WARM_MLP)),
+                      ("chosen + clusters", collop.cluster_config("bimodal_n65536").step_params(
+                          warmstart=collop.WARM_MLP)),
                       ("plain FP64", collop.config("bimodal_n65536").step_params())):
         sim = collop.Simulation(ws, v, w)
         st = sim.run(prm, 1)[0]
