@@ -12,16 +12,17 @@ timed region starts.
 Paths, on identical markers and to the same tolerance:
   * plain FP64 (the reference's GPU path, restated): every Picard map in FP64
     with the ordered pair sum (every ordered pair, collision_gpu.jl:49-107),
-    Anderson solver (step_anderson!, solver.jl:300-423);
-  * chosen: frozen-metric defect correction (the reference's step_defect!,
-    solver.jl:569-634) whose outer maps run the FP32-compensated pair sum with
-    the metric A until the FP32 rounding floor, then FP64 outer maps to finish
-    (every accepted state comes from an FP64 map); every pair sum in the
-    symmetric form (each unordered pair evaluated once, applied to both
-    markers);
+    Anderson solver (step_anderson!, solver.jl:300-423) controlled from the
+    host map by map;
+  * chosen: the same Anderson solver with its control on the device, Picard
+    maps with the FP32-compensated pair sum until the FP32 rounding floor, then
+    FP64 maps to finish (every accepted state comes from an FP64 map), every pair
+    sum in the symmetric form (each unordered pair evaluated once, applied to
+    both markers), Euler predictor (--mlp adds the learned warm start);
   * comparisons: the plain FP64 path with the symmetric pair sum (the kernel's
-    share of the gain), FP32-compensated maps + FP64 finish with the Anderson
-    solver, and the chosen path with the Euler predictor only.
+    share of the gain), the chosen path with stiff-cluster elimination, the
+    frozen-metric defect correction (step_defect!, solver.jl:569-634), and the
+    chosen path with the learned warm start.
 `value` is the chosen path's steps/s; the plain FP64 figure and the ratio are
 reported beside it.  Per step, |E_{n+1} - E_n| / E_0 and |P_{n+1} - P_n| are
 checked against 1e-10 and the entropy against non-decrease.
@@ -41,6 +42,7 @@ Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu]
        python bench.py --report precision --config bimaxwellian_n262144 --steps 20
        python bench.py --config ensemble48 --steps 3
        python bench.py --config lb_bimodal_n40000 --steps 100
+       python bench.py --report scatter-order --steps 50   (under rocprofv3 --kernel-trace)
 """
 from __future__ import annotations
 
@@ -78,12 +80,14 @@ def pair_kernels(tiles: int):
             (f"k_pair_sym_f32c<{R},true,false> (pair sum + metric)", FP32_VECTOR_PEAK_TFLOPS, 50, "unordered")]
 # HBM bytes per pair-sum launch at bimodal_n65536 from rocprofv3 FETCH_SIZE (x2.0)
 # and WRITE_SIZE (x1.0), both scaled by the counters of a 256 MiB copy; see
-# profiles/pair_sum_bimodal_n65536.md.
-PAIR_TRAFFIC_BYTES = {"k_pair_f64<1,1,false>": 68646 * 1024, "k_pair_f32c<1,32,false>": 50730 * 1024,
+# profiles/pair_sum_bimodal_n65536.md (the metric kernels from an earlier run of
+# the same counters).
+PAIR_TRAFFIC_BYTES = {"k_pair_f64<1,1,false>": (2 * 17939 + 32768) * 1024,
+                      "k_pair_f32c<1,32,false>": 50730 * 1024,
                       "k_pair_f64<1,1,true> (pair sum + metric)": 117802 * 1024,
                       "k_pair_f32c<1,32,true> (pair sum + metric)": 99890 * 1024,
-                      "k_pair_sym_f64<4,false,false>": 233132 * 1024,
-                      "k_pair_sym_f32c<4,false,false>": 161728 * 1024,
+                      "k_pair_sym_f64<4,false,false>": (2 * 50966 + 131072) * 1024,
+                      "k_pair_sym_f32c<4,false,false>": (2 * 15446 + 131072) * 1024,
                       "k_pair_sym_f64<4,true,false> (pair sum + metric)": 485372 * 1024,
                       "k_pair_sym_f32c<4,true,false> (pair sum + metric)": 397245 * 1024}
 
@@ -92,9 +96,10 @@ def log(msg: str):
     print(msg, file=sys.stderr, flush=True)
 
 
-def run_path(ws, p, v, w, precision, warmup, steps, solver=None, warmstart=None, symmetric=True):
+def run_path(ws, p, v, w, precision, warmup, steps, solver=None, warmstart=None, symmetric=True, device_solve=True):
     import collop
     collop.set_pair_mode(ws, symmetric)
+    collop.set_device_solve(ws, device_solve)
     tiles = collop.pair_mode(ws, v.shape[0])[1]
     sim = collop.Simulation(ws, v, w)
     prm = p.step_params(precision=precision, solver=solver, warmstart=warmstart)
@@ -217,8 +222,8 @@ def precision_report(args):
     steps = args.steps
     out = {"workload": args.config, "N": N, "steps": steps, "dt": p.DT}
     runs = {}
-    base = collop.chosen_config(args.config)   # the same solver for both: only the precision differs
-    out["solver"] = "defect correction with outer Anderson window 3, near pairs w/|d| > 0.02 (chosen path's knobs)"
+    base = collop.defect_config(args.config)   # the same solver for both: only the precision differs
+    out["solver"] = "defect correction with outer Anderson window 3, near pairs w/|d| > 0.02"
     for name, prec in (("fp64", collop.PREC_F64), ("f32c", collop.PREC_F32C)):
         sim = collop.Simulation(ws, v, w)
         E0 = 0.5 * float(np.sum(w * (v[:, 0] ** 2 + v[:, 1] ** 2)))
@@ -271,18 +276,60 @@ def precision_report(args):
     print(json.dumps(out), flush=True)
 
 
+def scatter_order_report(args):
+    """Marker order and the scatter / gathers: the projection (k_scatter + the
+    mass solve) and the gradient gather (k_gather) of the workload's markers,
+    `--steps` times each in the IC's order and then sorted by mesh cell (the
+    order projection_gpu.jl's cell-sorted kernels assume), through the hook
+    entry points.  The coefficients agree bit for bit (the scatter's
+    fixed-point sums do not depend on order).  Run under rocprofv3
+    --kernel-trace: the first half of the k_scatter / k_gather launches is the
+    IC's order, the second half the sorted one."""
+    import collop
+    p = collop.config(args.config)
+    v, w = collop.initial_markers(p)
+    ws = collop.build_workspace(p, device=0)
+    N = v.shape[0]
+    h1 = (p.bp1[-1] - p.bp1[0]) / (len(p.bp1) - 1)
+    h2 = (p.bp2[-1] - p.bp2[0]) / (len(p.bp2) - 1)
+    ci = np.clip(((v[:, 0] - p.bp1[0]) / h1).astype(np.int64), 0, len(p.bp1) - 2)
+    cj = np.clip(((v[:, 1] - p.bp2[0]) / h2).astype(np.int64), 0, len(p.bp2) - 2)
+    perm = np.argsort(ci + (len(p.bp1) - 1) * cj, kind="stable")
+    out = {"workload": args.config, "N": N, "calls": args.steps}
+    res = {}
+    for name, (vv, ww) in (("ic_order", (v, w)), ("cell_sorted", (np.asfortranarray(v[perm]), w[perm]))):
+        fc = np.zeros(ws.n_dofs)
+        G = np.zeros((N, 2), order="F")
+        t0 = time.perf_counter()
+        for _ in range(args.steps):
+            collop.l2_project_gpu(ws, fc, vv, ww)
+        t_proj = (time.perf_counter() - t0) / args.steps
+        r, _ = collop.compute_r_and_entropy_gpu(ws, fc)
+        L = collop.mass_solve_gpu(ws, r)
+        t0 = time.perf_counter()
+        for _ in range(args.steps):
+            collop.compute_G_gpu(ws, G, vv, L)
+        t_gath = (time.perf_counter() - t0) / args.steps
+        res[name] = (fc.copy(), G.copy())
+        out[name] = {"projection_call_ms": round(t_proj * 1e3, 4), "gather_call_ms": round(t_gath * 1e3, 4)}
+        log(f"[scatter-order] {name}: projection call {t_proj * 1e3:.3f} ms, gather call {t_gath * 1e3:.3f} ms")
+    out["coeffs_bitwise_equal"] = bool(np.array_equal(res["ic_order"][0], res["cell_sorted"][0]))
+    out["G_max_abs_diff"] = float(np.max(np.abs(res["ic_order"][1][perm] - res["cell_sorted"][1])))
+    print(json.dumps(out), flush=True)
+
+
 def warmstart_report(args):
     """Learned warm start: record 8 seeded N = 65536 runs (the three ICs in turn,
     32 x 32 mesh, dt 0.02) for `--train-steps` steps each, train the MLP with
     Adam (lr 1e-3) on the device, then step a held-out seed `--steps` times cold
     (Euler predictor) and warm (Euler + MLP), with the plain Anderson solver and
-    with the chosen path, to the same tolerance."""
+    with defect correction (FP32c outer maps), to the same tolerance."""
     import collop
     from collop import warmstart as W
     p = collop.config("bimodal_n65536")
     ws = collop.build_workspace(p, device=0)
     runs = W.training_runs(8, 65536, seed0=100)
-    rec_params = collop.chosen_config("bimodal_n65536").step_params()
+    rec_params = collop.defect_config("bimodal_n65536").step_params()
     t0 = time.perf_counter()
     n = W.record(ws, runs, args.train_steps, rec_params, log=log)
     t_rec = time.perf_counter() - t0
@@ -301,7 +348,7 @@ def warmstart_report(args):
            "epochs": args.epochs, "loss_per_epoch": losses, "held_out": "bimodal seed 999", "steps": args.steps,
            "norm": {"mu": mu.tolist(), "sd": sd.tolist(), "out_scale": os_.tolist()}}
     for name, base in (("anderson_fp64", collop.config("bimodal_n65536")),
-                       ("chosen", collop.chosen_config("bimodal_n65536"))):
+                       ("defect_correction", collop.defect_config("bimodal_n65536"))):
         res = {}
         for warm in (collop.WARM_EULER, collop.WARM_MLP):
             prm = base.step_params(warmstart=warm)
@@ -353,10 +400,10 @@ def ensemble_bench(args):
         res[form] = dict(run_steps_per_s=len(runs) * args.steps / el, el=el, gamma_tile=bt, waves_per_cu=waves,
                          tuner_ms=ms.tolist(), all_converged=all(s["exit_reason"] == 0 for st in stats for s in st))
         log(f"[ensemble {form}] {res[form]['run_steps_per_s']:.2f} run-steps/s")
-    # the chosen single-run path (defect correction, FP32c maps, learned warm
+    # the single-run defect-correction path (FP32c outer maps, learned warm
     # start) on 4 contexts with a HIP stream each, stepped by host threads
     from collop import warmstart as W
-    pch = collop.chosen_config("bimodal_n65536").step_params(precision=collop.PREC_F32C,
+    pch = collop.defect_config("bimodal_n65536").step_params(precision=collop.PREC_F32C,
                                                              warmstart=collop.WARM_MLP)
     sens = collop.StreamEnsemble(np.linspace(-6, 6, 33), np.linspace(-6, 6, 33), [(v, w) for v, w, _, _ in runs],
                                  workers=4, warm_start=W.load_packaged)
@@ -369,7 +416,7 @@ def ensemble_bench(args):
     stream = {"run_steps_per_s": round(len(runs) * args.steps / s_el, 4), "workers": 4,
               "mean_outer_maps": float(np.mean([s["iters"] for st in sst for s in st])),
               "all_converged": all(s["exit_reason"] == 0 for st in sst for s in st)}
-    log(f"[ensemble streams] chosen single-run path on 4 streams: {stream['run_steps_per_s']:.2f} run-steps/s, "
+    log(f"[ensemble streams] defect-correction single-run path on 4 streams: {stream['run_steps_per_s']:.2f} run-steps/s, "
         f"mean outer maps {stream['mean_outer_maps']:.1f}")
     # the same runs stepped one after another (single-run path, symmetric pair
     # sum), a bounded sample of steps
@@ -472,7 +519,9 @@ def main():
     ap.add_argument("--steps", type=int, default=10)
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--no-mlp", action="store_true", help="chosen path without the learned warm start")
+    ap.add_argument("--mlp", action="store_true",
+                    help="chosen path with the learned warm start (by default it is a comparison path: in the "
+                         "20-step window after 5 warmup steps it costs more maps than the Euler predictor)")
     ap.add_argument("--no-variants", action="store_true", help="skip the comparison paths")
     ap.add_argument("--config", default="bimodal_n65536",
                     help="bimodal_n65536 (headline), gaussian_n4096, bimaxwellian_n262144, ensemble48 or "
@@ -480,7 +529,7 @@ def main():
     ap.add_argument("--train-steps", type=int, default=200, help="warmstart report: recorded steps per run")
     ap.add_argument("--epochs", type=int, default=3, help="warmstart report: Adam epochs")
     ap.add_argument("--save-model", default=None, help="warmstart report: write the trained model (.npz)")
-    ap.add_argument("--report", choices=["precision", "warmstart"], default=None,
+    ap.add_argument("--report", choices=["precision", "warmstart", "scatter-order"], default=None,
                     help="precision: FP32c vs FP64 field errors and drift (use --config bimaxwellian_n262144)")
     args = ap.parse_args()
     if args.gpus != 1:
@@ -493,6 +542,9 @@ def main():
         return
     if args.report == "warmstart":
         warmstart_report(args)
+        return
+    if args.report == "scatter-order":
+        scatter_order_report(args)
         return
     if args.config == "ensemble48":
         ensemble_bench(args)
@@ -514,14 +566,12 @@ def main():
     log(f"[bench] {args.config}: N={N} (in domain {n_in}), mesh {len(p.bp1) - 1}x{len(p.bp2) - 1}, dt={p.DT}; "
         f"{W} untimed steps, then {K} timed steps (steps {W}..{W + K - 1}, t = {W * p.DT:.2f}..{(W + K) * p.DT:.2f})")
     log("[bench] plain FP64 path (the reference's GPU path: ordered pair sum, Anderson)")
-    r64 = run_path(ws, p, v, w, collop.PREC_F64, W, K, symmetric=False)
+    r64 = run_path(ws, p, v, w, collop.PREC_F64, W, K, symmetric=False, device_solve=False)
     log(f"[bench] plain FP64: {r64['steps_per_s']:.3f} steps/s, mean iters {r64['mean_iters']:.1f}")
     pch = collop.chosen_config(args.config)
-    warm = collop.WARM_EULER
-    if not args.no_mlp:
-        from collop import warmstart as W_
-        W_.load_packaged(ws)  # the learned warm start (trained by --report warmstart on other seeds)
-        warm = collop.WARM_MLP
+    from collop import warmstart as W_
+    W_.load_packaged(ws)  # the learned warm start (trained by --report warmstart on other seeds)
+    warm = collop.WARM_MLP if args.mlp else collop.WARM_EULER
     log("[bench] chosen path")
     rch = run_path(ws, pch, v, w, collop.PREC_F32C, W, K, warmstart=warm)
     log(f"[bench] chosen: {rch['steps_per_s']:.3f} steps/s, mean iters {rch['mean_iters']:.1f} "
@@ -542,6 +592,9 @@ def main():
         if warm == collop.WARM_MLP:
             variants.append(("chosen_euler_predictor", "the chosen path with the Euler predictor only", pch,
                              collop.PREC_F32C, None, collop.WARM_EULER))
+        else:
+            variants.append(("chosen_mlp_warm_start", "the chosen path with the learned MLP warm start", pch,
+                             collop.PREC_F32C, None, collop.WARM_MLP))
         for key, what, pp, prec, solver, wm in variants:
             log(f"[bench] comparison path: {what}")
             comps[key] = (what, run_path(ws, pp, v, w, prec, W, K, solver=solver, warmstart=wm))
@@ -609,7 +662,8 @@ def main():
         "fp64_steps_per_s": round(r64["steps_per_s"], 4),
         "ratio_vs_fp64": round(ratio, 4),
         "chosen_path": collop.chosen_description(args.config, warm == collop.WARM_MLP),
-        "plain_path": "FP64 maps, Anderson (step_anderson!), Euler predictor, ordered pair sum",
+        "plain_path": "FP64 maps, Anderson (step_anderson!) controlled from the host, Euler predictor, ordered pair "
+                      "sum",
         "chosen": path_summary(rch, r64),
         "plain_fp64": path_summary(r64),
         "comparisons": {k: dict(path_summary(r, r64), path=what) for k, (what, r) in comps.items()},
