@@ -386,7 +386,8 @@ def ensemble_bench(args):
         t0 = time.perf_counter()
         bt, waves, ms = ens.tune()
         log(f"[ensemble {form}] autotuner ({time.perf_counter() - t0:.1f} s): gamma tile {bt}, waves/CU {waves}; "
-            f"FP32c+FP64 pair launch ms rows gamma 64..512 (symmetric: row 0 only), cols waves 4..32:\n"
+            f"FP32c+FP64 pair launch ms rows gamma 64..512 (symmetric: super-rows 256, 512, 1024), "
+            f"cols waves 4..32:\n"
             f"{np.array2string(ms, precision=3)}")
         for i in range(args.warmup):
             st = ens.step(p, 1)[0]

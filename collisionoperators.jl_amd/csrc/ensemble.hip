@@ -172,6 +172,26 @@ void drop_graph(EnsembleState* E) {
   E->graph = nullptr;
 }
 
+// Symmetric form with R waves per block (super-rows of 128 R markers): every
+// run's slot count and partial offset, the partial buffer and the block list
+// follow R.
+int set_sym_form(collop_ctx* ctx, EnsembleState* E, int R) {
+  for (const EnsRun& q : E->runs)
+    if (sym_rows(q.n, R) > kSymMaxSlots) return fail(ctx, COLLOP_ERR_INVALID, "too many partial slots for this tile");
+  E->sym = R;
+  long long poff = 0;
+  for (EnsRun& q : E->runs) {
+    q.S = sym_rows(q.n, R);
+    q.poff = poff;
+    poff += (long long)q.S * 2 * q.n;
+  }
+  E->part_len = poff;
+  ENSURE(E->part, (size_t)poff * 8);
+  HIPC(hipMemcpy(E->d_runs.p, E->runs.data(), E->runs.size() * sizeof(EnsRun), hipMemcpyHostToDevice));
+  drop_graph(E);
+  return build_tiles(ctx, E, E->bt);
+}
+
 // Projection of every run's state markers, its entropy and moments
 // (main.jl:298-303): fc[r], scal[r] = {S0, -, -, P1, P2, sum w|v|^2}.
 int ens_project(collop_ctx* ctx, EnsembleState* E, int use_logsq) {
@@ -672,13 +692,39 @@ int collop_ensemble_tune(collop_ctx* ctx, int* gamma_tile, int* waves_per_cu, do
   launch_pack_b(V, P<double>(E->v), P<double>(E->G), P<double>(E->w), m.lo1, m.hi1, m.lo2, m.hi2, P<Src64>(E->s64),
                 P<Src32>(E->s32), ctx->stream);
   CHECK_LAUNCH();
-  const int bts[4] = {64, 128, 256, 512}, ws[4] = {4, 8, 16, 32};
+  const int ws[4] = {4, 8, 16, 32};
   double best = 1e300;
-  int best_bt = E->bt, best_w = E->waves, idx = 0;
+  int best_bt = E->bt, best_w = E->waves, best_R = E->sym, idx = 0;
   if (ms)
     for (int i = 0; i < 16; ++i) ms[i] = 0.0;
+  if (E->sym) {
+    // symmetric form: the gamma tile is the super-row, 128 R markers for R in {2, 4, 8}
+    const int Rs[3] = {2, 4, 8};
+    for (int i = 0; i < 3; ++i) {
+      bool fits = true;
+      for (const EnsRun& q : E->runs) fits = fits && sym_rows(q.n, Rs[i]) <= kSymMaxSlots;
+      if (!fits) continue;
+      if ((r = set_sym_form(ctx, E, Rs[i]))) return r;
+      for (int j = 0; j < 4; ++j) {
+        E->waves = ws[j];
+        double t = 0.0;
+        if ((r = time_pair(ctx, E, 3, &t))) return r;
+        if (ms) ms[i * 4 + j] = t;
+        if (t < best) {
+          best = t;
+          best_R = Rs[i];
+          best_w = ws[j];
+        }
+      }
+    }
+    E->waves = best_w;
+    if ((r = set_sym_form(ctx, E, best_R))) return r;
+    if (gamma_tile) *gamma_tile = 128 * best_R;
+    if (waves_per_cu) *waves_per_cu = best_w;
+    return COLLOP_OK;
+  }
+  const int bts[4] = {64, 128, 256, 512};
   for (int bt : bts) {
-    if (E->sym && bt != bts[0]) break;  // symmetric blocks have no gamma tile: waves only
     if ((r = build_tiles(ctx, E, bt))) return r;
     for (int wv : ws) {
       E->waves = wv;
@@ -695,7 +741,7 @@ int collop_ensemble_tune(collop_ctx* ctx, int* gamma_tile, int* waves_per_cu, do
   }
   E->waves = best_w;
   if ((r = build_tiles(ctx, E, best_bt))) return r;
-  if (gamma_tile) *gamma_tile = E->sym ? 0 : best_bt;
+  if (gamma_tile) *gamma_tile = best_bt;
   if (waves_per_cu) *waves_per_cu = best_w;
   return COLLOP_OK;
 }

@@ -350,9 +350,11 @@ int collop_ensemble_set_tile(collop_ctx* ctx, int gamma_tile, int waves_per_cu, 
 /* autotuner: times the persistent pair sum (one FP32c and one FP64 launch with
  * every run active) for gamma_tile in {64, 128, 256, 512} x waves_per_cu in
  * {4, 8, 16, 32} on the current ensemble, keeps the fastest; ms[16] optional
- * receives each candidate's time in that order.  In the symmetric form only
- * waves_per_cu is tuned: ms[0..3], *gamma_tile = 0 (collop_ensemble_set_tile
- * then ignores gamma_tile). */
+ * receives each candidate's time in that order.  In the symmetric form the
+ * gamma tile is the super-row of 128 R markers, R in {2, 4, 8} waves per block
+ * (ms rows 0..2; *gamma_tile = 128 R): each run is then bitwise the same run
+ * stepped alone after collop_set_pair_mode(ctx, COLLOP_PAIR_SYMMETRIC, R).
+ * collop_ensemble_set_tile ignores gamma_tile in the symmetric form. */
 int collop_ensemble_tune(collop_ctx* ctx, int* gamma_tile, int* waves_per_cu, double* ms);
 
 /* ---- tuning and measurement (no counterpart in the reference) -----------

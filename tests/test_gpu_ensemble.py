@@ -112,20 +112,32 @@ def test_graph_and_tiles_do_not_change_results(ens_ws, symmetric):
 
 
 @pytest.mark.parametrize("symmetric", [False, True], ids=["ordered", "symmetric"])
-def test_autotune(ens_ws, symmetric):
+def test_autotune(ens_ws, single_ws, symmetric):
     set_form(ens_ws, symmetric)
     ens = collop.Ensemble(ens_ws, make_runs())
     bt, waves, ms = ens.tune()
     print(f"tuned gamma tile {bt}, waves/CU {waves}; ms (rows 64..512, cols 4..32 waves):\n{ms}")
     assert waves in (4, 8, 16, 32)
-    if symmetric:  # waves only: row 0
-        assert bt == 0 and np.all(ms[0] > 0) and np.all(ms[1:] == 0)
-        assert np.isclose(ms[0].min(), ms[0, [4, 8, 16, 32].index(waves)])
+    if symmetric:  # super-rows of 128 R markers, R = 2, 4, 8: rows 0..2
+        assert bt in (256, 512, 1024)
+        assert np.all(ms[:3] > 0) and np.all(ms[3] == 0)
+        assert np.isclose(ms[:3].min(), ms[[256, 512, 1024].index(bt), [4, 8, 16, 32].index(waves)])
     else:
         assert bt in (64, 128, 256, 512)
         assert np.all(ms > 0) and np.isclose(ms.min(), ms[[64, 128, 256, 512].index(bt), [4, 8, 16, 32].index(waves)])
+    runs = make_runs()
     st = ens.step(params(collop.PREC_F32C), 1)[0]
     assert all(s["exit_reason"] == 0 for s in st)
+    if symmetric:  # the tuned super-row: each run equals the same run alone with R = bt / 128
+        got = ens.markers()
+        collop.set_pair_mode(single_ws, True, bt // 128)
+        try:
+            for q in (0, len(runs) - 1):
+                sim = collop.Simulation(single_ws, *runs[q])
+                sim.run(params(collop.PREC_F32C), 1)
+                np.testing.assert_array_equal(got[q], sim.markers())
+        finally:
+            collop.set_pair_mode(single_ws, True)
 
 
 def test_conservation_and_entropy(ens_ws):
