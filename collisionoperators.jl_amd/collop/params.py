@@ -158,8 +158,8 @@ DEFECT = dict(gpu_fp32=True, solver="defect", dc_outer_m=3, dc_near_theta=0.02, 
 # The chosen path: FP32-compensated Anderson maps until the FP32 floor, then FP64
 # maps (step_anderson!), with stiff-cluster elimination: markers closer than
 # The chosen path: FP32c maps until their rounding floor, then FP64 maps, the
-# Anderson solve of step_anderson! with its control on the device, symmetric
-# pair sum.
+# Anderson solve of step_anderson! (the workload's window of 5) with its
+# control on the device, symmetric pair sum.
 CHOSEN = dict(gpu_fp32=True, solver="anderson")
 # Stiff-cluster elimination (an opt-in extension): pairs with
 # max(w_i, w_j) / |d| > CLUSTER_TAU / dt at the start of a step form clusters
