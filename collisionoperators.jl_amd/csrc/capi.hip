@@ -192,7 +192,7 @@ int check_N(collop_ctx* ctx, int64_t N) {
 int sym_tiles_for(const collop_ctx* ctx, int64_t N) {
   if (!ctx->pair_sym || N <= 0) return 0;
   int R = ctx->sym_tiles;
-  if (!R) R = N < 4096 ? 2 : (sym_rows((int)N, 4) <= kSymMaxSlots ? 4 : 8);
+  if (!R) R = N < 4096 ? 2 : (sym_rows((int)N, 4) <= kSymPreferSlots ? 4 : 8);
   return sym_rows((int)N, R) <= kSymMaxSlots ? R : 0;
 }
 

@@ -88,7 +88,8 @@ struct PairTile {
 // per pair of super-rows.  Writes part[S][C][N] with S = sym_rows(N, R) slots
 // (C = 2, or 5 with the metric), every slot of every marker exactly once.
 __host__ __device__ inline int sym_rows(int N, int R) { return (N + 128 * R - 1) / (128 * R); }
-constexpr int kSymMaxSlots = 256;  // larger N falls back to the ordered sum
+constexpr int kSymMaxSlots = 512;    // larger N falls back to the ordered sum
+constexpr int kSymPreferSlots = 256;  // the automatic choice takes 4 waves per block up to this many slots
 void launch_pair_sym_f64(const Src64* src, int N, int R, bool metric, const NearOut* near, double* part,
                          hipStream_t st, const int* live = nullptr);
 void launch_pair_sym_f32c(const Src32* src, int N, int R, bool metric, const NearOut* near, double* part,

@@ -7,11 +7,11 @@
 // batched kernel the block count the single-run kernel would use at N = n_r,
 // so each run's arithmetic is the single-run arithmetic.
 //
-// One Anderson iteration of the whole ensemble is one replay of a captured
-// hipGraph: upload the per-run control blocks (RunCtl) -> per-run update of the
+// One Anderson iteration of the whole ensemble: upload the per-run control
+// blocks (RunCtl), then one replay of a captured hipGraph (per-run update of the
 // iterate -> scatter / solve / quadrature / gather / G_eff of every active run ->
 // ONE persistent pair-sum launch over the tiles of every active run (FP32c or
-// FP64 per run) -> epilogue -> per-run statistics -> download.  The host then
+// FP64 per run) -> epilogue -> per-run statistics), then download.  The host then
 // advances each run's Anderson state machine (the same one the single-run step
 // uses) and writes the next control blocks.  Kernel arguments never change
 // between replays; everything that varies lives in the control blocks.
@@ -234,6 +234,8 @@ int ens_predict(collop_ctx* ctx, EnsembleState* E, const collop_step_params& p) 
       E->h_ctl[i].kind = 1;
     }
     HIPC(hipMemcpyAsync(E->d_ctl.p, E->h_ctl, (size_t)E->R * sizeof(RunCtl), hipMemcpyHostToDevice, st));
+    // the host rewrites h_ctl right after this function: the upload must have read it
+    HIPC(hipStreamSynchronize(st));
     launch_pack_b(V, P<double>(E->v), P<double>(E->G), P<double>(E->w), m.lo1, m.hi1, m.lo2, m.hi2, nullptr,
                   P<Src32>(E->s32), st);
     CHECK_LAUNCH();
@@ -263,7 +265,6 @@ int ens_iteration(collop_ctx* ctx, EnsembleState* E, const collop_step_params& p
   const int gf = p.use_gonzalez ? 2 : 1;
   double* vec = P<double>(E->vec);
   double* scal = P<double>(E->scal);
-  HIPC(hipMemcpyAsync(E->d_ctl.p, E->h_ctl, (size_t)E->R * sizeof(RunCtl), hipMemcpyHostToDevice, st));
   launch_update_b(V, B, st);
   CHECK_LAUNCH();
   launch_scatter_b(m, V, P<double>(E->v), P<double>(E->v1), P<double>(E->w), 2, gf, P<double>(E->vmid),
@@ -294,35 +295,40 @@ int ens_iteration(collop_ctx* ctx, EnsembleState* E, const collop_step_params& p
   launch_sum_rows_b(V, P<double>(E->statp), rows(E, kRowsStatR0), rows(E, kRowsStatN), kStatWidth, 1, kStatWidth,
                     P<double>(E->stat), kStatWidth, st);
   CHECK_LAUNCH();
-  HIPC(hipMemcpyAsync(E->h_stat, E->stat.p, (size_t)E->R * kStatWidth * 8, hipMemcpyDeviceToHost, st));
   return COLLOP_OK;
 }
 
+// One ensemble iteration: the control blocks go up, the iteration's kernels run
+// (captured once in a hipGraph and replayed, or launched one by one), the
+// statistics come down.  The graph holds kernel nodes only: the two copies
+// stay outside it, on the same stream.
 int ens_replay(collop_ctx* ctx, EnsembleState* E, const collop_step_params& p) {
+  HIPC(hipMemcpyAsync(E->d_ctl.p, E->h_ctl, (size_t)E->R * sizeof(RunCtl), hipMemcpyHostToDevice, ctx->stream));
   if (!E->use_graph) {
     int r = ens_iteration(ctx, E, p);
     if (r) return r;
-    HIPC(hipStreamSynchronize(ctx->stream));
-    return COLLOP_OK;
-  }
-  if (E->exec && (E->g_dt != p.dt || E->g_gonz != p.use_gonzalez || E->g_logsq != p.use_logsq)) drop_graph(E);
-  if (!E->exec) {
-    HIPC(hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeThreadLocal));
-    const int r = ens_iteration(ctx, E, p);
-    hipGraph_t g = nullptr;
-    const hipError_t ec = hipStreamEndCapture(ctx->stream, &g);
-    if (r) {
-      if (g) (void)hipGraphDestroy(g);
-      return r;
+  } else {
+    if (E->exec && (E->g_dt != p.dt || E->g_gonz != p.use_gonzalez || E->g_logsq != p.use_logsq)) drop_graph(E);
+    if (!E->exec) {
+      HIPC(hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeThreadLocal));
+      const int r = ens_iteration(ctx, E, p);
+      hipGraph_t g = nullptr;
+      const hipError_t ec = hipStreamEndCapture(ctx->stream, &g);
+      if (r) {
+        if (g) (void)hipGraphDestroy(g);
+        return r;
+      }
+      if (ec != hipSuccess)
+        return fail(ctx, COLLOP_ERR_HIP, std::string("hipStreamEndCapture: ") + hipGetErrorString(ec));
+      E->graph = g;
+      HIPC(hipGraphInstantiate(&E->exec, E->graph, nullptr, nullptr, 0));
+      E->g_dt = p.dt;
+      E->g_gonz = p.use_gonzalez;
+      E->g_logsq = p.use_logsq;
     }
-    if (ec != hipSuccess) return fail(ctx, COLLOP_ERR_HIP, std::string("hipStreamEndCapture: ") + hipGetErrorString(ec));
-    E->graph = g;
-    HIPC(hipGraphInstantiate(&E->exec, E->graph, nullptr, nullptr, 0));
-    E->g_dt = p.dt;
-    E->g_gonz = p.use_gonzalez;
-    E->g_logsq = p.use_logsq;
+    HIPC(hipGraphLaunch(E->exec, ctx->stream));
   }
-  HIPC(hipGraphLaunch(E->exec, ctx->stream));
+  HIPC(hipMemcpyAsync(E->h_stat, E->stat.p, (size_t)E->R * kStatWidth * 8, hipMemcpyDeviceToHost, ctx->stream));
   HIPC(hipStreamSynchronize(ctx->stream));
   return COLLOP_OK;
 }

@@ -532,6 +532,8 @@ __global__ __launch_bounds__(kEwThreads) void k_epilogue_b(EnsView E, EnsBufs B,
   a.stat_part = B.statp + (size_t)(R.eblk0 + t.y) * kStatWidth;
   a.cl_row = nullptr;
   a.cl_x = nullptr;
+  a.gate = nullptr;
+  a.S32 = 0;
   epilogue_body(a, R.n, E.NT, (size_t)2 * E.NT, red, t.y);
 }
 

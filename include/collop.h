@@ -327,7 +327,7 @@ int collop_mlp_eval(collop_ctx* ctx, const float* rec, int64_t n, int mode, floa
  * run follows exactly the solve of collop_run_steps (own Anderson window,
  * precision switch and exit).  The pair-sum form is the context's at the time
  * of this call: symmetric with the context's tiles (4 when automatic) if every
- * run fits in 256 partial slots, else ordered.  A run's markers then equal,
+ * run fits in 512 partial slots, else ordered.  A run's markers then equal,
  * bit for bit, those of the same run stepped alone on a context with
  * collop_set_pair_mode(ctx, COLLOP_PAIR_SYMMETRIC, tiles) (symmetric), or with
  * collop_set_pair_tile(ctx, 1, collop_ensemble_slices(N), 1) (ordered).
@@ -371,7 +371,7 @@ typedef enum {
 /* Pair-sum form for every later pair sum of the context (hooks, steps);
  * tiles = waves per block of the symmetric sum, 0 (automatic), 2, 4 or 8.
  * Both forms sum the same terms in different orders: results agree to
- * rounding.  The symmetric form needs ceil(N / (128 tiles)) <= 256 partial
+ * rounding.  The symmetric form needs ceil(N / (128 tiles)) <= 512 partial
  * slots and falls back to the ordered sum beyond that.  No counterpart in the
  * reference. */
 int collop_set_pair_mode(collop_ctx* ctx, int mode, int tiles);

@@ -137,8 +137,9 @@ def test_symmetric_deterministic_and_fallback(gpu_ctx):
     collop.compute_collision_gpu(ws, a, v, w, G)
     collop.compute_collision_gpu(ws, b, v, w, G)
     np.testing.assert_array_equal(a, b)
-    # 66000 markers need 258 super-rows of 256 at tiles = 2: the ordered sum runs
+    # 66000 markers take 258 super-rows of 256 at tiles = 2: still symmetric
     collop.set_pair_mode(ws, True, 2)
+    assert collop.pair_mode(ws, 66000) == (True, 2, 258)
     c = np.zeros((66000, 2), order="F")
     collop.compute_collision_gpu(ws, c, v, w, G)
     ref = O.collision_rows(m.box, v, w, G, 0, 1500)
@@ -149,11 +150,35 @@ def test_symmetric_deterministic_and_fallback(gpu_ctx):
         collop.set_pair_mode(ws, True, 3)
 
 
+def test_symmetric_beyond_256_slots_and_fallback(gpu_ctx):
+    """300000 markers take 293 super-rows of 1024 (more than 256 partial
+    slots): the symmetric sum against the ordered one and the oracle; 140000
+    markers at 2 waves per block would need 547 slots (more than 512): the
+    ordered sum runs."""
+    v, w, bp, m, G = chain("gaussian", 300000, 16, seed=5)
+    ws = collop.Workspace(bp, bp, ctx=gpu_ctx)
+    collop.set_pair_mode(ws, True)
+    assert collop.pair_mode(ws, 300000) == (True, 8, 293)
+    s = np.zeros((300000, 2), order="F")
+    collop.compute_collision_gpu(ws, s, v, w, G)
+    collop.set_pair_mode(ws, False)
+    o = np.zeros((300000, 2), order="F")
+    collop.compute_collision_gpu(ws, o, v, w, G)
+    ref = O.collision_rows(m.box, v, w, G, 0, 500)
+    ref2 = O.collision_rows(m.box, v, w, G, 299500, 300000)
+    e = max(relerr(s[:500], ref[:500]), relerr(s[299500:], ref2[299500:]))
+    print(f"N=300000: symmetric vs oracle (1000 rows) {e:.2e}, vs ordered {relerr(s, o):.2e}")
+    assert e <= 1e-12 and relerr(s, o) <= 1e-13
+    collop.set_pair_mode(ws, True, 2)
+    assert collop.pair_mode(ws, 140000)[:2] == (False, 0)
+
+
 def test_pair_mode_choice(sym_ws):
     assert collop.pair_mode(sym_ws, 1000) == (True, 2, 4)
     assert collop.pair_mode(sym_ws, 65536) == (True, 4, 128)
     assert collop.pair_mode(sym_ws, 262144) == (True, 8, 256)
-    assert collop.pair_mode(sym_ws, 300000) == (False, 0, 7)    # beyond 256 slots: ordered
+    assert collop.pair_mode(sym_ws, 300000) == (True, 8, 293)
+    assert collop.pair_mode(sym_ws, 600000)[:2] == (False, 0)  # beyond 512 slots: ordered
     collop.set_pair_mode(sym_ws, True, 2)
     assert collop.pair_mode(sym_ws, 65536) == (True, 2, 256)
     collop.set_pair_mode(sym_ws, False)
