@@ -64,6 +64,19 @@ def test_device_solve_bitwise(gpu_ctx, fp32, exit_picard, m):
     assert all(s["stat_syncs"] == 0 for s in dev[1])
 
 
+@pytest.mark.parametrize("gonzalez,logsq", [(0, 0), (1, 1)], ids=["no-gonzalez", "half-log-squared"])
+def test_device_solve_map_variants(gpu_ctx, gonzalez, logsq):
+    """The map without the Gonzalez correction, and with the half log f^2
+    entropy: both controls still agree bit for bit."""
+    p, v, w, ws = _setup(gpu_ctx, 4096, gpu_fp32=True)
+    prm = p.step_params()
+    prm.use_gonzalez = gonzalez
+    prm.use_logsq = logsq
+    dev = _run(ws, v, w, prm, 2, True)
+    host = _run(ws, v, w, prm, 2, False)
+    _same(dev, host)
+
+
 def test_device_solve_stagnation_and_budget(gpu_ctx):
     """Forced stagnation exits and map-budget exits end on the best iterate in both."""
     p, v, w, ws = _setup(gpu_ctx, 4096)
