@@ -401,7 +401,7 @@ int collop_destroy(collop_ctx* ctx) {
                   &ctx->m_y, &ctx->m_gpart, &ctx->m_lpart, &ctx->m_tiles, &ctx->m_eval, &ctx->m_grad,
                   &ctx->b_cell_of, &ctx->b_cell_cnt, &ctx->b_cell_start, &ctx->b_cell_sorted, &ctx->b_links,
                   &ctx->b_link_count, &ctx->b_cl_off, &ctx->b_cl_idx, &ctx->b_cl_row, &ctx->b_cl_x,
-                  &ctx->b_cl_root, &ctx->b_cl_have, &ctx->b_cl_fail, &ctx->b_cl_its})
+                  &ctx->b_cl_root, &ctx->b_cl_have, &ctx->b_cl_fail, &ctx->b_cl_its, &ctx->b_actl})
     release(*b);
   for (auto& e : ctx->ev_pool) {
     (void)hipEventDestroy(e.first);
