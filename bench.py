@@ -517,8 +517,9 @@ def lb_bench(args):
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=10)
-    ap.add_argument("--warmup", type=int, default=1)
+    # default: the workload's own run, 50 steps from the initial markers
+    ap.add_argument("--steps", type=int, default=50)
+    ap.add_argument("--warmup", type=int, default=0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--mlp", action="store_true",
                     help="chosen path with the learned warm start (by default it is a comparison path: in the "
