@@ -14,15 +14,18 @@ Paths, on identical markers and to the same tolerance:
     with the ordered pair sum (every ordered pair, collision_gpu.jl:49-107),
     Anderson solver (step_anderson!, solver.jl:300-423) controlled from the
     host map by map;
-  * chosen: the same Anderson solver with its control on the device, Picard
-    maps with the FP32-compensated pair sum until the FP32 rounding floor, then
-    FP64 maps to finish (every accepted state comes from an FP64 map), every pair
-    sum in the symmetric form (each unordered pair evaluated once, applied to
-    both markers), Euler predictor (--mlp adds the learned warm start);
+  * chosen: frozen-metric defect correction (step_defect!, solver.jl:569-634)
+    with outer maps in the FP32-compensated pair sum until the FP32 rounding
+    floor, then FP64 outer maps (every accepted state comes from an FP64 map),
+    handing a step with stiff near pairs over to the Anderson solve
+    (step_anderson!, FP64 maps, control on the device); every pair sum in the
+    symmetric form (each unordered pair evaluated once, applied to both
+    markers); Euler predictor with the learned MLP correction (--no-mlp: Euler
+    alone);
   * comparisons: the plain FP64 path with the symmetric pair sum (the kernel's
-    share of the gain), the chosen path with stiff-cluster elimination, the
-    frozen-metric defect correction (step_defect!, solver.jl:569-634), and the
-    chosen path with the learned warm start.
+    share of the gain), the Anderson solve alone (FP32c then FP64 maps, control
+    on the device), that with stiff-cluster elimination, the defect correction
+    without the hand-over, and the chosen path with the Euler predictor alone.
 `value` is the chosen path's steps/s; the plain FP64 figure and the ratio are
 reported beside it.  Per step, |E_{n+1} - E_n| / E_0 and |P_{n+1} - P_n| are
 checked against 1e-10 and the entropy against non-decrease.
@@ -122,8 +125,9 @@ def run_path(ws, p, v, w, precision, warmup, steps, solver=None, warmstart=None,
     for i in range(steps):
         st = sim.run(prm, 1)[0]
         stats.append(st)
-        log(f"  step {i}: iters {st['iters']} (f32 {st['iters_f32']}, inner {st['inner_maps']}) "
-            f"exit {st['exit_reason']} res {st['residual']:.2e} {st['seconds'] * 1e3:.1f} ms")
+        log(f"  step {i}: iters {st['iters']} (f32 {st['iters_f32']}, inner {st['inner_maps']}"
+            + (f", handed over after {st['handover']}" if st['handover'] else "")
+            + f") exit {st['exit_reason']} res {st['residual']:.2e} {st['seconds'] * 1e3:.1f} ms")
     elapsed = time.perf_counter() - t0
     ms, launches = collop.profile(ws, False)
     dE = dP = 0.0
@@ -146,6 +150,7 @@ def run_path(ws, p, v, w, precision, warmup, steps, solver=None, warmstart=None,
                 max_exit_residual=float(max(s["residual"] for s in stats)),
                 clusters_mean=float(np.mean([s["n_clusters"] for s in stats])),
                 cluster_solve_failures=int(sum(s["cl_fail"] for s in stats)),
+                handed_over_steps=int(sum(s["handover"] > 0 for s in stats)),
                 constraint_met=bool(all_conv and dE <= 1e-10 and dP <= 1e-10 and dS_min >= 0.0),
                 markers=sim.markers())
 
@@ -160,6 +165,7 @@ def path_summary(res, ref=None):
            "constraint_met": res["constraint_met"]}
     if res["mean_inner"]:
         out["mean_inner"] = res["mean_inner"]
+        out["handed_over_steps"] = res["handed_over_steps"]
     if res["clusters_mean"]:
         out["clusters_mean"] = res["clusters_mean"]
         out["cluster_solve_failures"] = res["cluster_solve_failures"]
@@ -521,9 +527,9 @@ def main():
     ap.add_argument("--steps", type=int, default=50)
     ap.add_argument("--warmup", type=int, default=0)
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--mlp", action="store_true",
-                    help="chosen path with the learned warm start (by default it is a comparison path: in the "
-                         "20-step window after 5 warmup steps it costs more maps than the Euler predictor)")
+    ap.add_argument("--no-mlp", action="store_true",
+                    help="chosen path with the Euler predictor alone (the learned warm start becomes a comparison "
+                         "path)")
     ap.add_argument("--no-variants", action="store_true", help="skip the comparison paths")
     ap.add_argument("--config", default="bimodal_n65536",
                     help="bimodal_n65536 (headline), gaussian_n4096, bimaxwellian_n262144, ensemble48 or "
@@ -573,7 +579,7 @@ def main():
     pch = collop.chosen_config(args.config)
     from collop import warmstart as W_
     W_.load_packaged(ws)  # the learned warm start (trained by --report warmstart on other seeds)
-    warm = collop.WARM_MLP if args.mlp else collop.WARM_EULER
+    warm = collop.WARM_EULER if args.no_mlp else collop.WARM_MLP
     log("[bench] chosen path")
     rch = run_path(ws, pch, v, w, collop.PREC_F32C, W, K, warmstart=warm)
     log(f"[bench] chosen: {rch['steps_per_s']:.3f} steps/s, mean iters {rch['mean_iters']:.1f} "
@@ -584,12 +590,13 @@ def main():
         variants = [
             ("fp64_symmetric", "plain FP64 Anderson with the symmetric pair sum", p, collop.PREC_F64, None,
              collop.WARM_EULER),
-            ("chosen_clusters", "the chosen path with stiff-cluster elimination (clusters of "
+            ("anderson_f32c", collop.anderson_description(), collop.anderson_config(args.config), collop.PREC_F32C,
+             None, warm),
+            ("anderson_clusters", "the Anderson path with stiff-cluster elimination (clusters of "
              f"max(w_i, w_j) / |d| > {collop.cluster_config(args.config).cl_theta:g} solved exactly in every FP64 map)",
              collop.cluster_config(args.config), collop.PREC_F32C, None, warm),
-            ("defect_correction", "FP32c then FP64 outer maps, frozen-metric defect correction with an outer "
-             "Anderson window of 3 and exact near pairs, symmetric pair sum", collop.defect_config(args.config),
-             collop.PREC_F32C, None, warm),
+            ("defect_correction", "the chosen path without the hand-over to Anderson (step_defect! alone, 60 inner "
+             "maps per outer iteration)", collop.defect_config(args.config), collop.PREC_F32C, None, warm),
         ]
         if warm == collop.WARM_MLP:
             variants.append(("chosen_euler_predictor", "the chosen path with the Euler predictor only", pch,

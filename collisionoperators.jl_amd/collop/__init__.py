@@ -14,8 +14,9 @@ from .hooks import (compute_collision_gpu, compute_collision_gpu32, compute_coll
                     mass_solve_gpu)
 from .ic import ensemble_markers, initial_markers, markers  # noqa: F401
 from .mesh import Workspace, build_workspace, extraction_p2c1  # noqa: F401
-from .params import (CHOSEN, CONFIGS, DEFECT, SimParameters, chosen_config, chosen_description,  # noqa: F401
-                     cluster_config, config, defect_config, parse_overrides)
+from .params import (ANDERSON_F32C, CHOSEN, CONFIGS, DEFECT, SimParameters, anderson_config,  # noqa: F401
+                     anderson_description, chosen_config, chosen_description, cluster_config, config, defect_config,
+                     parse_overrides)
 from .records import (CONS_COLS, STATS_COLS, RunRecord, checkpoint_path, cons_time_at,  # noqa: F401
                       load_checkpoint, migrate_cons_csv, run_recorded, save_checkpoint, truncate_csv_after)
 from .sim import (Ensemble, Simulation, StreamEnsemble, pair_mode, profile, set_device_solve, set_fused_maps,  # noqa: F401

@@ -86,7 +86,7 @@ class StepParams(C.Structure):
         ("nk_fd_rel", C.c_double),
         ("cl_theta", C.c_double),
         ("cl_kmax", C.c_int32),
-        ("cl_pad", C.c_int32),
+        ("dc_fallback", C.c_int32),
     ]
 
 
@@ -108,6 +108,7 @@ class StepStats(C.Structure):
         ("n_clusters", C.c_int32),
         ("cl_fail", C.c_int32),
         ("stat_syncs", C.c_int32),
+        ("handover", C.c_int32),
     ]
 
     def as_dict(self) -> dict:
@@ -117,7 +118,7 @@ class StepStats(C.Structure):
             "entropy": self.entropy, "energy": self.energy,
             "momentum": (self.momentum[0], self.momentum[1]), "seconds": self.seconds,
             "inner_maps": self.inner_maps, "n_clusters": self.n_clusters, "cl_fail": self.cl_fail,
-            "stat_syncs": self.stat_syncs,
+            "stat_syncs": self.stat_syncs, "handover": self.handover,
         }
 
 

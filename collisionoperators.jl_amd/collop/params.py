@@ -68,6 +68,7 @@ class SimParameters:
     nk_fd_rel: float = 0.0
     cl_theta: float = 0.0            # extension: stiff clusters max(w_i, w_j) / |d| > cl_theta solved exactly (0 = off)
     cl_kmax: int = 32                # largest stiff cluster (markers)
+    dc_fallback: bool = False        # extension: defect correction hands a stiff step to the Anderson solve
 
     def step_params(self, precision: int | None = None, warmstart: int | None = None,
                     solver: int | None = None) -> _lib.StepParams:
@@ -93,7 +94,8 @@ class SimParameters:
             dc_outer_m=self.dc_outer_m, dc_near_theta=self.dc_near_theta, dc_metric_reuse=self.dc_metric_reuse,
             collision_model=_lib.MODEL_LB if self.collision_model == "lb" else _lib.MODEL_LANDAU, nu=self.nu,
             dc_eta_f64=self.dc_eta_f64, nk_krylov_max=self.nk_krylov_max, nk_eta_max=self.nk_eta_max, nk_fd_h=self.nk_fd_h,
-            nk_fd_rel=self.nk_fd_rel, cl_theta=self.cl_theta, cl_kmax=self.cl_kmax)
+            nk_fd_rel=self.nk_fd_rel, cl_theta=self.cl_theta, cl_kmax=self.cl_kmax,
+            dc_fallback=int(self.dc_fallback))
 
 
 def parse_overrides(p: SimParameters, args: list[str]) -> SimParameters:
@@ -155,12 +157,15 @@ CONFIGS = {
 # 0.02), the FP64 outer maps reusing the metric A of the last FP32c outer map.
 DEFECT = dict(gpu_fp32=True, solver="defect", dc_outer_m=3, dc_near_theta=0.02, dc_metric_reuse=1)
 
-# The chosen path: FP32-compensated Anderson maps until the FP32 floor, then FP64
-# maps (step_anderson!), with stiff-cluster elimination: markers closer than
-# The chosen path: FP32c maps until their rounding floor, then FP64 maps, the
+# The Anderson path: FP32c maps until their rounding floor, then FP64 maps, the
 # Anderson solve of step_anderson! (the workload's window of 5) with its
 # control on the device, symmetric pair sum.
-CHOSEN = dict(gpu_fp32=True, solver="anderson")
+ANDERSON_F32C = dict(gpu_fp32=True, solver="anderson")
+# The chosen path: the defect-correction path (DEFECT) with the hand-over: when
+# a frozen-metric inner solve ends unconverged, an FP64 outer iteration does
+# not halve the residual, or the outer iteration stagnates (stiff near pairs),
+# the step continues with the Anderson solve in FP64 maps.
+CHOSEN = dict(DEFECT, dc_fallback=True)
 # Stiff-cluster elimination (an opt-in extension): pairs with
 # max(w_i, w_j) / |d| > CLUSTER_TAU / dt at the start of a step form clusters
 # whose rows every FP64 map solves exactly.  The threshold keeps dt w / |d|
@@ -176,9 +181,16 @@ def chosen_config(name: str, **over) -> SimParameters:
     return config(name, **kw)
 
 
+def anderson_config(name: str, **over) -> SimParameters:
+    """config(name) with the Anderson path's knobs (ANDERSON_F32C), then `over`."""
+    kw = dict(ANDERSON_F32C)
+    kw.update(over)
+    return config(name, **kw)
+
+
 def cluster_config(name: str, **over) -> SimParameters:
-    """chosen_config(name) with stiff-cluster elimination, then `over`."""
-    p = chosen_config(name, cl_kmax=CLUSTER_KMAX, **over)
+    """anderson_config(name) with stiff-cluster elimination, then `over`."""
+    p = anderson_config(name, cl_kmax=CLUSTER_KMAX, **over)
     if "cl_theta" not in over:
         p = dataclasses.replace(p, cl_theta=CLUSTER_TAU / p.DT)
     return p
@@ -192,9 +204,18 @@ def defect_config(name: str, **over) -> SimParameters:
 
 
 def chosen_description(name: str, mlp: bool) -> str:
+    return ("frozen-metric defect correction (step_defect!): FP32c outer maps (F and metric A) until the FP32 "
+            "rounding floor, then FP64 outer maps, an outer Anderson window of 3 and exact near pairs; an inner solve "
+            "that misses its tolerance in 60 maps, an FP64 outer iteration that does not halve the residual, or "
+            "outer stagnation hands the step to the Anderson solve (step_anderson!, FP64 maps, control on the "
+            "device); "
+            "FP32c Euler predictor" + (" + learned MLP warm start" if mlp else "")
+            + "; symmetric pair sum (each unordered pair once)")
+
+
+def anderson_description() -> str:
     return ("FP32c maps until the FP32 rounding floor, then FP64 maps, Anderson (step_anderson!) with its control "
-            "(least squares, exit and stagnation tests) on the device, FP32c Euler predictor"
-            + (" + learned MLP warm start" if mlp else "") + "; symmetric pair sum (each unordered pair once)")
+            "(least squares, exit and stagnation tests) on the device, FP32c Euler predictor; symmetric pair sum")
 
 
 def config(name: str, **over) -> SimParameters:

@@ -1506,6 +1506,7 @@ int validate_params(collop_ctx* ctx, const collop_step_params* p) {
        !std::isfinite(p->dc_near_theta) || p->dc_metric_reuse < 0 || p->dc_metric_reuse > 2))
     return fail(ctx, COLLOP_ERR_INVALID,
                 "defect correction needs 0 <= dc_outer_m <= 8, dc_near_theta >= 0 and dc_metric_reuse in {0, 1, 2}");
+  if (p->dc_fallback != 0 && p->dc_fallback != 1) return fail(ctx, COLLOP_ERR_INVALID, "dc_fallback must be 0 or 1");
   if (p->collision_model != COLLOP_MODEL_LANDAU && p->collision_model != COLLOP_MODEL_LB)
     return fail(ctx, COLLOP_ERR_INVALID, "unknown collision_model");
   if (p->collision_model == COLLOP_MODEL_LB &&
@@ -1524,6 +1525,13 @@ int validate_params(collop_ctx* ctx, const collop_step_params* p) {
 // metric A_k; then Anderson (solver.jl:507-545: window m_anderson, damping,
 // no stagnation logic) on the frozen-metric map Phi_k to
 // max(dc_eta ||G(v_k) - v_k||, abs_floor / 2), at most dc_max_inner maps.
+// exit_reason of defect_solve when the step continues with the Anderson solve
+constexpr int kDefectHandover = 3;
+// the hand-over's outer contraction test: an FP64 outer iteration must reduce
+// the residual at least by this factor (an Anderson map reduces it by about
+// half in the steps with stiff near pairs, at the cost of one pair sum)
+constexpr double kHandoverRatio = 0.5;
+
 int defect_solve(collop_ctx* ctx, const collop_step_params& p, collop_step_stats* out) {
   const int n = (int)ctx->N;
   const size_t n2b = (size_t)2 * n * 8;
@@ -1541,6 +1549,7 @@ int defect_solve(collop_ctx* ctx, const collop_step_params& p, collop_step_stats
   double best32 = std::numeric_limits<double>::infinity(), prev32 = 0.0;
   bool switch_after = false;
   int since_best32 = 0, iters32 = 0, k64 = 0;
+  double prev64 = 0.0;
   // outer Anderson window (extension): Anderson with beta = 1 on v -> T(v), the
   // inner solution, with the epilogue's column bookkeeping
   const int mo = p.dc_outer_m;
@@ -1599,11 +1608,22 @@ int defect_solve(collop_ctx* ctx, const collop_step_params& p, collop_step_stats
         res_out = nr;
         break;
       }
+      // with the hand-over, an FP64 outer iteration that does not reduce the
+      // residual by kHandoverRatio hands the step to the Anderson solve from
+      // the best iterate (its Picard image)
+      if (p.dc_fallback && k64 >= 2 && nr > kHandoverRatio * prev64) {
+        std::swap(ctx->s_v1, ctx->s_best);
+        exit_reason = kDefectHandover;
+        k_done = k;
+        res_out = nrm_best;
+        break;
+      }
+      prev64 = nr;
     }
     if (!use32 && k64 > p.dc_stag_window && k64 % p.dc_stag_window == 0) {
       if ((nrm_best_window - nrm_best) / (nrm_best_window + 1e-30) < p.stag_rel_tol) {
         std::swap(ctx->s_v1, ctx->s_best);
-        exit_reason = 1;
+        exit_reason = p.dc_fallback ? kDefectHandover : 1;
         k_done = k;
         res_out = nrm_best;
         break;
@@ -1611,7 +1631,7 @@ int defect_solve(collop_ctx* ctx, const collop_step_params& p, collop_step_stats
       nrm_best_window = nrm_best;
     }
     if (ctx->near_nt > 0 && (r = copy_d2d(ctx, ctx->s_vmk, ctx->b_vmid, n2b))) return r;  // v_mid of v_k
-    if (mo > 0 && (r = copy_d2d(ctx, ctx->s_xk, ctx->s_v1, n2b))) return r;
+    if ((mo > 0 || p.dc_fallback) && (r = copy_d2d(ctx, ctx->s_xk, ctx->s_v1, n2b))) return r;
     // inner Anderson on u = Phi_k(u), u = s_v1 (solver.jl:507-545)
     const double eta = (!use32 && p.dc_eta_f64 > 0.0) ? p.dc_eta_f64 : p.dc_eta;
     const double tol_in = std::max(eta * nr, 0.5 * p.abs_floor);
@@ -1619,6 +1639,7 @@ int defect_solve(collop_ctx* ctx, const collop_step_params& p, collop_step_stats
     UpdateArgs pend{};
     bool has_pend = false;
     double ATA[kMaxWindow * kMaxWindow] = {0};
+    bool inner_ok = false;
     for (int j = 1; j <= p.dc_max_inner; ++j) {
       const int col = j == 1 ? -1 : slot % mw;
       const int hist_new = j == 1 ? 0 : std::min(hist + 1, mw);
@@ -1639,6 +1660,7 @@ int defect_solve(collop_ctx* ctx, const collop_step_params& p, collop_step_stats
       if (!std::isfinite(ri)) return fail(ctx, COLLOP_ERR_NONFINITE, "non-finite residual in a frozen-metric map");
       if (ri < tol_in) {
         std::swap(ctx->s_v1, ctx->s_Gv);  // u <- Phi(u)
+        inner_ok = true;
         break;
       }
       // the next frozen map's scatter applies the update (the loop's last
@@ -1692,6 +1714,15 @@ int defect_solve(collop_ctx* ctx, const collop_step_params& p, collop_step_stats
       std::swap(ctx->s_Gp, ctx->s_Gv);
     }
     if (has_pend) return fail(ctx, COLLOP_ERR_HIP, "internal: an Anderson update was left pending");
+    if (!inner_ok && p.dc_fallback) {
+      // the frozen-metric solve did not converge (stiff near pairs): the
+      // Anderson solve takes over from the outer iterate x_k
+      std::swap(ctx->s_v1, ctx->s_xk);
+      exit_reason = kDefectHandover;
+      k_done = k;
+      res_out = nr;
+      break;
+    }
     if (mo > 0) {  // outer Anderson: f = U - x_k with U the inner solution
       EpiArgs a{};
       a.part = nullptr;
@@ -1911,23 +1942,35 @@ int step_once(collop_ctx* ctx, const collop_step_params& p, collop_step_stats* o
   } else if (p.solver == COLLOP_SOLVER_NEWTON) {
     if ((r = newton_solve(ctx, p, &dc))) return r;
   }
-  // ---- Anderson (solver.jl:300-423)
+  // ---- Anderson (solver.jl:300-423).  A defect-correction step that handed
+  // over (dc_fallback) continues here from its outer iterate with the
+  // iterations it left, in FP64 maps: a hand-over means stiff near pairs, where
+  // FP32c maps slow the Anderson solve down (their rounding is amplified along
+  // the stiff modes).
+  const bool handover = p.solver == COLLOP_SOLVER_DEFECT && dc.exit_reason == kDefectHandover;
+  collop_step_params pa = p;
+  if (handover) {
+    pa.solver = COLLOP_SOLVER_ANDERSON;
+    pa.max_iter = std::max(1, p.max_iter - dc.iters);
+    pa.precision = COLLOP_PREC_F64;
+  }
+  const bool anderson = pa.solver == COLLOP_SOLVER_ANDERSON;
   AndersonRun an;
-  an.start(p, ctx->ncl > 0);
+  an.start(pa, ctx->ncl > 0);
   // The best iterate starts as the initial guess (solver.jl:331).
-  if (p.solver == COLLOP_SOLVER_ANDERSON && (r = copy_d2d(ctx, ctx->s_best, ctx->s_v1, n2b))) return r;
+  if (anderson && (r = copy_d2d(ctx, ctx->s_best, ctx->s_v1, n2b))) return r;
   const char* trace = std::getenv("COLLOP_TRACE") ? "run" : nullptr;
   const bool lb_fused = lb && ctx->fused_maps && m.nd1 <= 256 && m.nd2 <= 256;
-  if (p.solver == COLLOP_SOLVER_ANDERSON && ctx->dev_solve && !lb && ctx->ncl == 0 && !trace) {
-    if ((r = anderson_device(ctx, p, an))) return r;
+  if (anderson && ctx->dev_solve && !lb && ctx->ncl == 0 && !trace) {
+    if ((r = anderson_device(ctx, pa, an))) return r;
   }
-  while (p.solver == COLLOP_SOLVER_ANDERSON && !an.done) {
+  while (anderson && !an.done) {
     const AndersonPlan pl = an.plan();
     if (lb_fused) {
-      if ((r = lb_map_fused(ctx, p, pl.col, pl.hist))) return r;
+      if ((r = lb_map_fused(ctx, pa, pl.col, pl.hist))) return r;
     } else if (lb) {
-      if ((r = lb_map_dev(ctx, p, P<double>(ctx->s_v1), pl.col, pl.hist))) return r;
-    } else if ((r = picard_map_dev(ctx, p, pl.kind, P<double>(ctx->s_v1), pl.col, pl.hist))) {
+      if ((r = lb_map_dev(ctx, pa, P<double>(ctx->s_v1), pl.col, pl.hist))) return r;
+    } else if ((r = picard_map_dev(ctx, pa, pl.kind, P<double>(ctx->s_v1), pl.col, pl.hist))) {
       return r;
     }
     AndersonAct act;
@@ -1996,7 +2039,16 @@ int step_once(collop_ctx* ctx, const collop_step_params& p, collop_step_stats* o
   HIPC(hipStreamSynchronize(ctx->stream));
   prof_collect(ctx);
   if (out) {
-    if (p.solver != COLLOP_SOLVER_ANDERSON) {
+    out->handover = handover ? dc.iters : 0;
+    if (handover) {
+      out->iters = dc.iters + an.k_done;
+      out->iters_f32 = dc.iters_f32 + an.iters32;
+      out->n_restart = an.n_restart;
+      out->exit_reason = an.exit_reason;
+      out->residual = an.res_out;
+      out->r0 = dc.r0;
+      out->inner_maps = dc.inner_maps;
+    } else if (p.solver != COLLOP_SOLVER_ANDERSON) {
       *out = dc;
     } else {
       out->iters = an.k_done;

@@ -414,6 +414,7 @@ int ens_step(collop_ctx* ctx, EnsembleState* E, const collop_step_params& p, col
       s.momentum[0] = sc[3];
       s.momentum[1] = sc[4];
       s.seconds = sec;
+      s.handover = 0;
     }
   }
   return COLLOP_OK;

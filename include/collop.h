@@ -255,7 +255,14 @@ typedef struct {
                                the exit test stays on the plain map, so the
                                root is the same; 0 = off                       */
   int32_t cl_kmax;          /* largest stiff cluster (markers, <= 32)          */
-  int32_t cl_pad;
+  int32_t dc_fallback;      /* DEFECT extension: 1 = hand the step over to the
+                               Anderson solve (step_anderson!, FP64 maps) with
+                               the iterations left when (a) an inner solve ends
+                               unconverged at dc_max_inner maps (from the outer
+                               iterate), (b) an FP64 outer iteration does not
+                               halve the residual or (c) the outer stagnation
+                               test fires (both from the best outer iterate's
+                               map); 0 = off (the reference's step_defect!)    */
 } collop_step_params;
 
 typedef struct {
@@ -275,6 +282,9 @@ typedef struct {
                                kept the plain rows of the cluster)             */
   int32_t stat_syncs;       /* maps whose statistics needed a stream
                                synchronisation (the 20 ms poll ran out)        */
+  int32_t handover;         /* DEFECT with dc_fallback: outer maps before the
+                               step continued with the Anderson solve (0 = the
+                               defect correction finished the step)           */
 } collop_step_stats;
 
 /* Upload markers (v: N x 2 column-major, w: N) and project them; keeps S0. */

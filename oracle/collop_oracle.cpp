@@ -302,6 +302,8 @@ struct orc_solver_params {
   double nk_fd_rel;      // legacy relative step (> 0 overrides nk_fd_h)
   double cl_theta;       // extension: stiff clusters of pairs with max(w_i, w_j) / |d| > cl_theta (0 = off)
   int cl_kmax;           // largest cluster solved exactly (markers)
+  int dc_fallback;       // extension: step_defect! hands a step whose inner solve fails (or whose outer
+                         // iteration stagnates) to step_anderson! from the outer iterate
 };
 
 struct orc_step_stats {
@@ -314,6 +316,7 @@ struct orc_step_stats {
   int inner;        // step_defect!: frozen-metric maps
   int n_clusters;   // stiff clusters solved exactly inside each map
   int cl_fail;      // cluster solves that failed (their rows keep the plain map)
+  int handover;     // step_defect! with dc_fallback: outer maps before step_anderson! took over (0 = none)
 };
 
 namespace {
@@ -1711,14 +1714,14 @@ void orc_step_anderson(const orc_mesh* ms, const orc_solver_params* Pp, double* 
       // the exit Picard step; with stiff clusters the map Anderson mixes (its
       // cluster rows are the clusters' exact solutions at the final iterate)
       if (P.exit_picard_step) std::memcpy(v1, Gm.data(), sizeof(double) * n2);
-      *st = {k, n_restart, nrm_r, nrm_r0, 0, 0, 0, 0, 0, 0, ncl, cl_fail};
+      *st = {k, n_restart, nrm_r, nrm_r0, 0, 0, 0, 0, 0, 0, ncl, cl_fail, 0};
       return;
     }
     if (k > P.stag_window && k % P.stag_window == 0) {
       const double rel = (nrm_best_window - nrm_best) / (nrm_best_window + 1e-30);
       if (rel < P.stag_rel_tol) {
         std::memcpy(v1, vbest.data(), sizeof(double) * n2);
-        *st = {k, n_restart, nrm_best, nrm_r0, 1, 0, 0, 0, 0, 0, ncl, cl_fail};
+        *st = {k, n_restart, nrm_best, nrm_r0, 1, 0, 0, 0, 0, 0, ncl, cl_fail, 0};
         return;
       }
       nrm_best_window = nrm_best;
@@ -1773,7 +1776,7 @@ void orc_step_anderson(const orc_mesh* ms, const orc_solver_params* Pp, double* 
     std::memcpy(Gp.data(), Gm.data(), sizeof(double) * n2);
   }
   std::memcpy(v1, vbest.data(), sizeof(double) * n2);
-  *st = {P.max_iter, n_restart, nrm_best, nrm_r0, 2, 0, 0, 0, 0, 0, ncl, cl_fail};
+  *st = {P.max_iter, n_restart, nrm_best, nrm_r0, 2, 0, 0, 0, 0, 0, ncl, cl_fail, 0};
 }
 
 }  // extern "C"
@@ -1785,15 +1788,19 @@ namespace {
 // logic.  Stops at ||Phi(u) - u|| < tol with u <- Phi(u), or after maxit maps
 // with u the last update.  Returns the number of maps.
 template <class MapF>
-int anderson_inner(MapF&& phi, double* u, int64_t n2, double tol, int maxit, int m, double damping, double reg) {
+int anderson_inner(MapF&& phi, double* u, int64_t n2, double tol, int maxit, int m, double damping, double reg,
+                   bool* converged = nullptr) {
+  if (converged) *converged = false;
   std::vector<double> Fu(n2), r(n2), rp(n2), Fp(n2), uold(n2), dF((size_t)m * n2), dG((size_t)m * n2);
   int hist = 0, slot = 0;
   for (int k = 1; k <= maxit; ++k) {
     std::memcpy(uold.data(), u, sizeof(double) * n2);
     phi(Fu.data(), u);
     for (int64_t i = 0; i < n2; ++i) r[i] = Fu[i] - u[i];
-    if (nrm(r.data(), n2) < tol) {
+    const double nr = nrm(r.data(), n2);
+    if (nr < tol) {
       std::memcpy(u, Fu.data(), sizeof(double) * n2);
+      if (converged) *converged = true;
       return k;
     }
     if (k == 1) {
@@ -1846,6 +1853,20 @@ extern "C" {
 // pass, then Anderson on Phi_k(u) = G(v_k) - dt A_k (G_eff(u) - G_eff(v_k)) to
 // max(eta ||G(v_k) - v_k||, abs_floor / 2).  The exit returns G(v) (or the best
 // G(v) after stagnation / max_iter).
+// Extension (dc_fallback): the rest of a defect-correction step that handed
+// over after k outer maps, as step_anderson! from v1 with the iterations left.
+static void defect_handover(const orc_mesh* ms, const orc_solver_params& P, double* v1, const double* v0,
+                            const double* w, int64_t N, double S0, orc_step_stats* st, int k, double nrm0, int inner) {
+  orc_solver_params Pa = P;
+  Pa.solver = 0;
+  Pa.max_iter = std::max(1, P.max_iter - k);
+  orc_step_anderson(ms, &Pa, v1, v0, w, N, S0, st, nullptr);
+  st->iters += k;
+  st->r0 = nrm0;
+  st->inner = inner;
+  st->handover = k;
+}
+
 void orc_step_defect(const orc_mesh* ms, const orc_solver_params* Pp, double* v1, const double* v0, const double* w,
                      int64_t N, double S0, orc_step_stats* st, double* iter_res) {
   const orc_solver_params& P = *Pp;
@@ -1855,7 +1876,7 @@ void orc_step_defect(const orc_mesh* ms, const orc_solver_params* Pp, double* v1
   double box[4];
   box_of(ms, box);
   double nrm0 = 0.0, nrm_best = std::numeric_limits<double>::infinity();
-  double nrm_best_window = std::numeric_limits<double>::infinity();
+  double nrm_best_window = std::numeric_limits<double>::infinity(), nr_prev = 0.0;
   int inner = 0;
   // near pairs (gamma, alpha) of the current outer iterate, sorted by gamma, and
   // the midpoints at v_k
@@ -1911,13 +1932,25 @@ void orc_step_defect(const orc_mesh* ms, const orc_solver_params* Pp, double* v1
     }
     if (nr < std::max(P.tol * (nrm(v1, n2) + 1e-30), P.abs_floor)) {
       std::memcpy(v1, Gv.data(), sizeof(double) * n2);
-      *st = {k, 0, nr, nrm0, 0, 0, 0, 0, 0, inner, 0, 0};
+      *st = {k, 0, nr, nrm0, 0, 0, 0, 0, 0, inner, 0, 0, 0};
       return;
     }
+    // hand-over extension: an outer iteration that does not halve the residual
+    // continues the step with step_anderson! from the best iterate's image
+    if (P.dc_fallback && k >= 2 && nr > 0.5 * nr_prev) {
+      std::memcpy(v1, vbest.data(), sizeof(double) * n2);
+      defect_handover(ms, P, v1, v0, w, N, S0, st, k, nrm0, inner);
+      return;
+    }
+    nr_prev = nr;
     if (k > P.dc_stag_window && k % P.dc_stag_window == 0) {
       if ((nrm_best_window - nrm_best) / (nrm_best_window + 1e-30) < P.stag_rel_tol) {
         std::memcpy(v1, vbest.data(), sizeof(double) * n2);
-        *st = {k, 0, nrm_best, nrm0, 1, 0, 0, 0, 0, inner, 0, 0};
+        if (P.dc_fallback) {
+          defect_handover(ms, P, v1, v0, w, N, S0, st, k, nrm0, inner);
+          return;
+        }
+        *st = {k, 0, nrm_best, nrm0, 1, 0, 0, 0, 0, inner, 0, 0, 0};
         return;
       }
       nrm_best_window = nrm_best;
@@ -1940,8 +1973,14 @@ void orc_step_defect(const orc_mesh* ms, const orc_solver_params* Pp, double* v1
       }
     }
     std::memcpy(xk.data(), v1, sizeof(double) * n2);
+    bool inner_ok = false;
     inner += anderson_inner(frozen, v1, n2, std::max(P.dc_eta * nr, 0.5 * P.abs_floor), P.dc_max_inner,
-                            P.m_anderson, P.damping, P.reg_factor);
+                            P.m_anderson, P.damping, P.reg_factor, &inner_ok);
+    if (!inner_ok && P.dc_fallback) {  // the Anderson solve continues from x_k
+      std::memcpy(v1, xk.data(), sizeof(double) * n2);
+      defect_handover(ms, P, v1, v0, w, N, S0, st, k, nrm0, inner);
+      return;
+    }
     if (mo > 0) {  // Anderson on x -> T(x) (the inner solution), beta = 1
       std::memcpy(U.data(), v1, sizeof(double) * n2);
       for (int64_t i = 0; i < n2; ++i) f[i] = U[i] - xk[i];
@@ -1984,7 +2023,7 @@ void orc_step_defect(const orc_mesh* ms, const orc_solver_params* Pp, double* v1
     }
   }
   std::memcpy(v1, vbest.data(), sizeof(double) * n2);
-  *st = {P.max_iter, 0, nrm_best, nrm0, 2, 0, 0, 0, 0, inner, 0, 0};
+  *st = {P.max_iter, 0, nrm_best, nrm0, 2, 0, 0, 0, 0, inner, 0, 0, 0};
 }
 
 // One full time step of main.jl:226-307 (Landau, Anderson): predictor,

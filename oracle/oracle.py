@@ -59,6 +59,7 @@ class SolverParams(C.Structure):
         ("nk_fd_rel", C.c_double),
         ("cl_theta", C.c_double),
         ("cl_kmax", C.c_int),
+        ("dc_fallback", C.c_int),
     ]
 
 
@@ -76,6 +77,7 @@ class StepStats(C.Structure):
         ("inner", C.c_int),
         ("n_clusters", C.c_int),
         ("cl_fail", C.c_int),
+        ("handover", C.c_int),
     ]
 
 
@@ -88,7 +90,7 @@ def default_params(dt: float, m: int = 5, nthreads: int = 0, **kw) -> SolverPara
         reg_factor=1e-10, use_anderson=1, use_gonzalez=1, exit_picard_step=1,
         use_logsq=0, nthreads=nthreads, solver=0, dc_eta=0.05, dc_max_inner=60, dc_stag_window=5,
         dc_outer_m=0, dc_near_theta=0.0, collision_model=0, nu=1.0,
-        nk_krylov_max=30, nk_eta_max=0.9, nk_fd_h=0.0, nk_fd_rel=0.0, cl_theta=0.0, cl_kmax=32,
+        nk_krylov_max=30, nk_eta_max=0.9, nk_fd_h=0.0, nk_fd_rel=0.0, cl_theta=0.0, cl_kmax=32, dc_fallback=0,
     )
     for k, v in kw.items():
         setattr(p, k, v)

@@ -154,3 +154,134 @@ def test_defect_f32c_outer_maps_reach_the_fp64_root(gpu_ctx, dt):
     assert sa["exit_reason"] == 0 and sb["exit_reason"] == 0
     assert sb["iters_f32"] >= 1
     assert np.max(np.abs(va - vb)) <= 1e-9
+
+
+@pytest.mark.parametrize("device_control", [True, False], ids=["device_control", "host_control"])
+@pytest.mark.parametrize("ic,N,ne,dt", [("gaussian", 1024, 16, 0.05), ("bimodal", 2048, 32, 0.02)])
+def test_defect_handover_matches_oracle(gpu_ctx, ic, N, ne, dt, device_control):
+    """The hand-over (dc_fallback, 10 inner maps per outer iteration) against
+    the oracle's: at dt 0.05 the first frozen-metric solve misses its tolerance
+    and the step continues with the Anderson solve; at dt 0.02 the defect
+    correction finishes the step itself.  FP64 maps; the Anderson part runs with
+    its control on the device or on the host."""
+    sig = (1.0, 1.0) if ic == "gaussian" else (np.sqrt(0.5), np.sqrt(0.5))
+    v, w = collop.markers(ic, N, 42, sigma1=sig[0], sigma2=sig[1], peak=2.0)
+    bp = uniform_bp(-6, 6, ne)
+    m = O.Mesh(bp, bp)
+    fc = m.l2_project(v, w)
+    S0 = m.entropy(fc)
+    P = O.default_params(dt, m=5, nthreads=0, solver=1, dc_outer_m=3, dc_near_theta=0.02, dc_max_inner=10,
+                         dc_fallback=1)
+    v_ref, fc_ref, st_ref = m.time_step(P, v, w, fc, S0)
+    ws = collop.Workspace(bp, bp, ctx=gpu_ctx)
+    collop.set_device_solve(ws, device_control)
+    sim = collop.Simulation(ws, v, w)
+    prm = collop.config("gaussian_n4096", DT=dt, solver="defect", dc_outer_m=3, dc_near_theta=0.02, dc_max_inner=10,
+                        dc_fallback=True).step_params()
+    st = sim.run(prm, 1)[0]
+    v_gpu = sim.markers()
+    dv = float(np.max(np.abs(v_gpu - v_ref)))
+    print(f"{ic} N={N} dt={dt}: oracle maps {st_ref.iters} inner {st_ref.inner} handover {st_ref.handover}; "
+          f"gpu maps {st['iters']} inner {st['inner_maps']} handover {st['handover']}; max|dv| {dv:.2e}")
+    assert st["exit_reason"] == 0 and st_ref.exit_reason == 0
+    assert st["handover"] == st_ref.handover and (st_ref.handover > 0) == (dt == 0.05)
+    assert abs(st["iters"] - st_ref.iters) <= 1
+    assert dv <= 1e-9
+    assert relerr(sim.coeffs(), fc_ref) <= 1e-9
+
+
+def test_defect_handover_with_one_inner_map_is_the_anderson_step(gpu_ctx):
+    """With one inner map per outer iteration every step hands over after its
+    first outer map, from the predictor: with FP64 maps the result is the
+    Anderson step's bit for bit, with one map more.  With FP32c maps only that
+    first outer map is FP32c (the hand-over continues in FP64), and the state
+    lands on the same root."""
+    p = collop.config("bimodal_n65536", N_PARTICLES=4096)
+    v, w = collop.initial_markers(p)
+    ws = collop.build_workspace(p, ctx=gpu_ctx)
+    a = collop.Simulation(ws, v, w)
+    sa = a.run(p.step_params(), 2)
+    va = a.markers()
+    q = collop.config("bimodal_n65536", N_PARTICLES=4096, solver="defect", dc_outer_m=3, dc_near_theta=0.02,
+                      dc_max_inner=1, dc_fallback=True)
+    b = collop.Simulation(ws, v, w)
+    sb = b.run(q.step_params(precision=collop.PREC_F64), 2)
+    vb = b.markers()
+    c = collop.Simulation(ws, v, w)
+    sc = c.run(q.step_params(precision=collop.PREC_F32C), 2)
+    vc = c.markers()
+    print("anderson maps", [s["iters"] for s in sa], "hand-over maps", [s["iters"] for s in sb],
+          "with an FP32c outer map", [(s["iters"], s["iters_f32"]) for s in sc],
+          f"max|dv| {np.max(np.abs(vc - va)):.2e}")
+    assert all(s["handover"] == 1 for s in sb + sc)
+    assert [s["iters"] + 1 for s in sa] == [s["iters"] for s in sb]
+    assert all(s["exit_reason"] == 0 for s in sa + sb + sc)
+    np.testing.assert_array_equal(va, vb)
+    assert all(s["iters_f32"] == 1 for s in sc)
+    assert np.max(np.abs(vc - va)) <= 1e-9
+
+
+def test_chosen_path_steps_bimodal_n65536(gpu_ctx):
+    """The chosen path (FP32c defect correction with the hand-over) on the
+    headline workload, N = 65536 on 32 x 32 cells, dt 0.02: the first 15 steps
+    from the initial markers all converge, some hand over to the Anderson solve,
+    energy and momentum are conserved per step to 1e-10, entropy does not fall,
+    and the 15th step's state is a root of the oracle's implicit system."""
+    p = collop.chosen_config("bimodal_n65536")
+    v, w = collop.initial_markers(p)
+    ws = collop.build_workspace(p, ctx=gpu_ctx)
+    sim = collop.Simulation(ws, v, w)
+    E0 = 0.5 * float(np.sum(w * (v[:, 0] ** 2 + v[:, 1] ** 2)))
+    E_prev, P_prev = E0, (w[:, None] * v).sum(axis=0)
+    S_prev = collop.compute_r_and_entropy_gpu(ws, sim.coeffs())[1]
+    prm = p.step_params()
+    handed = 0
+    for k in range(14):
+        s = sim.run(prm, 1)[0]
+        print(f"step {k}: maps {s['iters']} (f32 {s['iters_f32']}, inner {s['inner_maps']}, handover {s['handover']}) "
+              f"exit {s['exit_reason']} {s['seconds'] * 1e3:.1f} ms", flush=True)
+        assert s["exit_reason"] == 0
+        assert abs(s["energy"] - E_prev) / E0 <= 1e-10
+        assert np.all(np.abs(np.array(s["momentum"]) - P_prev) <= 1e-10)
+        assert s["entropy"] >= S_prev
+        E_prev, P_prev, S_prev = s["energy"], np.array(s["momentum"]), s["entropy"]
+        handed += s["handover"] > 0
+    x0 = sim.markers()
+    s = sim.run(prm, 1)[0]
+    x = sim.markers()
+    m = O.Mesh(p.bp1, p.bp2)
+    S0 = m.entropy(m.l2_project(x0, w))
+    P = O.default_params(p.DT, m=5)
+    res = float(np.linalg.norm(m.picard_map(P, x, x0, w, S0) - x))
+    tol = max(1e-12 * np.linalg.norm(x), 1e-10)
+    print(f"step 14: maps {s['iters']} handover {s['handover']} exit {s['exit_reason']}; oracle map residual "
+          f"{res:.2e} (tol {tol:.2e}); steps handed over {handed}")
+    assert s["exit_reason"] == 0 and res <= 20 * tol
+    assert handed >= 1
+
+
+def test_defect_handover_by_the_contraction_test_matches_oracle(gpu_ctx):
+    """The hand-over's outer contraction test against the oracle's: with a loose
+    inner tolerance (dc_eta = 0.8) the second FP64 outer iteration of a
+    well-conditioned step (isotropic N = 1024, dt 0.005) reduces the residual by
+    a factor 0.88 only, so the step hands over after its third outer map and
+    the Anderson solve finishes it.  FP64 maps; the hand-over point, the map
+    counts and the state match."""
+    v, w = collop.markers("gaussian", 1024, 42)
+    bp = uniform_bp(-6, 6, 16)
+    m = O.Mesh(bp, bp)
+    fc = m.l2_project(v, w)
+    S0 = m.entropy(fc)
+    P = O.default_params(0.005, m=5, nthreads=0, solver=1, dc_eta=0.8, dc_fallback=1)
+    v_ref, _, st_ref = m.time_step(P, v, w, fc, S0)
+    ws = collop.Workspace(bp, bp, ctx=gpu_ctx)
+    sim = collop.Simulation(ws, v, w)
+    st = sim.run(collop.config("gaussian_n4096", DT=0.005, solver="defect", dc_eta=0.8, dc_fallback=True).step_params(),
+                 1)[0]
+    dv = float(np.max(np.abs(sim.markers() - v_ref)))
+    print(f"oracle: handover after {st_ref.handover} outer maps, {st_ref.iters} maps; gpu: after {st['handover']}, "
+          f"{st['iters']} maps; max|dv| {dv:.2e}")
+    assert st_ref.handover == 3 and st["handover"] == 3
+    assert st["exit_reason"] == 0 and st_ref.exit_reason == 0
+    assert abs(st["iters"] - st_ref.iters) <= 1
+    assert dv <= 1e-9

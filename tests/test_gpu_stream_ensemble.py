@@ -19,7 +19,7 @@ def test_stream_ensemble_matches_single_runs(gpu_ctx):
         sig = {"gaussian": (1.0, 1.0), "bimodal": (np.sqrt(0.5), np.sqrt(0.5)),
                "bimaxwellian": (np.sqrt(2.0), np.sqrt(0.5))}[ic]
         runs.append(collop.markers(ic, n, 10 + i, sigma1=sig[0], sigma2=sig[1], peak=2.0))
-    params = collop.chosen_config("bimodal_n65536", DT=0.01).step_params(precision=collop.PREC_F32C)
+    params = collop.anderson_config("bimodal_n65536", DT=0.01).step_params(precision=collop.PREC_F32C)
     ens = collop.StreamEnsemble(bp, bp, runs, workers=2)
     try:
         stats = ens.step(params, 2)

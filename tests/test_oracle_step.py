@@ -162,3 +162,50 @@ def test_defect_extensions_keep_the_root():
     assert np.max(np.abs(va - vb)) <= 1e-10
     r = np.linalg.norm(m.picard_map(P, vb, v, w, S0) - vb)
     assert r <= 20 * max(1e-12 * np.linalg.norm(vb), 1e-10)
+
+
+def test_defect_handover_with_one_inner_map_is_the_anderson_solve():
+    """The hand-over (dc_fallback, an extension) continues the step with
+    step_anderson! from the outer iterate.  With one inner map per outer
+    iteration no frozen-metric solve can reach its tolerance (at u = v_k the
+    frozen map's residual is the outer one), so the step hands over after its
+    first outer map and must return step_anderson!'s result from the same
+    initial guess bit for bit, with one map more."""
+    v, w = collop.markers("gaussian", 512, 3)
+    bp = uniform_bp(-6, 6, 12)
+    m = O.Mesh(bp, bp)
+    fc = m.l2_project(v, w)
+    S0 = m.entropy(fc)
+    P = O.default_params(0.02, m=5)
+    v1, _ = m.predictor(P, v, w, fc)
+    va, sa, _ = m.step_anderson(P, v1, v, w, S0)
+    Q = O.default_params(0.02, m=5, solver=1, dc_max_inner=1, dc_fallback=1)
+    vd, sd, _ = m.step_defect(Q, v1, v, w, S0)
+    assert sd.handover == 1 and sa.handover == 0
+    assert sd.iters == sa.iters + 1 and sd.exit_reason == sa.exit_reason == 0
+    np.testing.assert_array_equal(vd, va)
+    Q.dc_fallback = 0                                 # without the hand-over: the reference's outer iteration
+    ve, se, _ = m.step_defect(Q, v1, v, w, S0)
+    assert se.handover == 0 and se.inner == se.iters - (se.exit_reason == 0)
+
+
+def test_defect_handover_lands_on_a_root():
+    """A step whose first frozen-metric solve needs more than dc_max_inner maps
+    (isotropic N = 1024, dt = 0.05) hands over and converges: the result is a
+    root of the full Picard map within the tolerance, with momentum exact and
+    energy to the tolerance's order."""
+    v, w = collop.markers("gaussian", 1024, 42)
+    bp = uniform_bp(-6, 6, 16)
+    m = O.Mesh(bp, bp)
+    fc = m.l2_project(v, w)
+    S0 = m.entropy(fc)
+    P = O.default_params(0.05, m=5, solver=1, dc_outer_m=3, dc_near_theta=0.02, dc_max_inner=10, dc_fallback=1)
+    vn, fcn, st = m.time_step(P, v, w, fc, S0)
+    assert st.handover >= 1 and st.exit_reason == 0
+    r = np.linalg.norm(m.picard_map(P, vn, v, w, S0) - vn)
+    assert r <= 20 * max(1e-12 * np.linalg.norm(vn), 1e-10)
+    (p1, p2), E0 = O.momentum_energy(v, w)
+    (q1, q2), E1 = O.momentum_energy(vn, w)
+    assert abs(q1 - p1) <= 1e-14 and abs(q2 - p2) <= 1e-14
+    assert abs(E1 - E0) <= 1e-11 * E0
+    assert m.entropy(fcn) >= S0
