@@ -652,6 +652,9 @@ def main():
             if a["exit_reason"] == 0 and b["exit_reason"] == 0]
     t_both_ch = sum(rch["stats"][i]["seconds"] for i in both)
     t_both_64 = sum(r64["stats"][i]["seconds"] for i in both)
+    per_step = {k: {"ms": [round(st["seconds"] * 1e3, 1) for st in r["stats"]],
+                    "maps": [st["iters"] for st in r["stats"]],
+                    "exit": [st["exit_reason"] for st in r["stats"]]} for k, r in (("chosen", rch), ("plain_fp64", r64))}
     converged_both = {"steps": [W + i for i in both],
                       "chosen_steps_per_s": round(len(both) / t_both_ch, 4) if both else None,
                       "plain_fp64_steps_per_s": round(len(both) / t_both_64, 4) if both else None,
@@ -682,6 +685,7 @@ def main():
         "fp64_steps_per_s": round(r64["steps_per_s"], 4),
         "ratio_vs_fp64": round(ratio, 4),
         "converged_on_both_paths": converged_both,
+        "per_step": per_step,
         "chosen_path": collop.chosen_description(args.config, warm == collop.WARM_MLP),
         "plain_path": "FP64 maps, Anderson (step_anderson!) controlled from the host, Euler predictor, ordered pair "
                       "sum",
