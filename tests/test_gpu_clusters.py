@@ -3,8 +3,9 @@ oracle: on a state where close pairs have formed (the oracle's own Anderson
 solve of the next step stagnates), the device step with clusters converges, it
 finds the oracle's clusters, and its result is a root of the oracle's implicit
 system; momentum and energy are conserved and entropy rises.  Then the same
-check at the headline size: one step of the chosen path at N = 65536 from the
-bimodal initial state."""
+check at the headline size: one step of the chosen path, of the Anderson path
+with clusters and of the plain FP64 path at N = 65536 from the bimodal initial
+state."""
 import os
 
 import numpy as np
@@ -72,11 +73,10 @@ def test_stiff_step_plain_solve_stalls(gpu_ctx):
 
 
 def test_chosen_step_at_n65536_is_an_oracle_root(gpu_ctx):
-    """One step of the chosen path (FP32c then FP64 Anderson maps with the
-    device-resident control, learned warm start) at N = 65536 on 32 x 32
-    cells, dt 0.02: the state is a root of the oracle's implicit system; and
-    one step of the same path with stiff clusters, and of the plain FP64 path,
-    is too."""
+    """One step of the chosen path (FP32c defect correction with the
+    hand-over, learned warm start) at N = 65536 on 32 x 32 cells, dt 0.02: the
+    state is a root of the oracle's implicit system; and one step of the
+    Anderson path with stiff clusters, and of the plain FP64 path, is too."""
     from collop import warmstart as W
     p = collop.chosen_config("bimodal_n65536")
     v, w = collop.initial_markers(p)
@@ -84,7 +84,7 @@ def test_chosen_step_at_n65536_is_an_oracle_root(gpu_ctx):
     W.load_packaged(ws)
     m = O.Mesh(p.bp1, p.bp2)
     for name, prm in (("chosen", p.step_params(warmstart=collop.WARM_MLP)),
-                      ("chosen + clusters", collop.cluster_config("bimodal_n65536").step_params(
+                      ("anderson + clusters", collop.cluster_config("bimodal_n65536").step_params(
                           warmstart=collop.WARM_MLP)),
                       ("plain FP64", collop.config("bimodal_n65536").step_params())):
         sim = collop.Simulation(ws, v, w)

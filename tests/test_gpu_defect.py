@@ -137,9 +137,9 @@ def test_defect_eta_f64_matches_oracle(gpu_ctx):
 
 @pytest.mark.parametrize("dt", [0.005, 0.02])
 def test_defect_f32c_outer_maps_reach_the_fp64_root(gpu_ctx, dt):
-    """The chosen path (FP32-compensated outer maps until the FP32 floor, FP64
-    outer maps reusing the FP32c metric, outer Anderson, near field) lands on
-    the root of the all-FP64 solve."""
+    """The defect-correction path (FP32-compensated outer maps until the FP32
+    floor, FP64 outer maps reusing the FP32c metric, outer Anderson, near field)
+    lands on the root of the all-FP64 solve."""
     p = collop.defect_config("bimodal_n65536", N_PARTICLES=8192, DT=dt)
     v, w = collop.initial_markers(p)
     ws = collop.build_workspace(p, ctx=gpu_ctx)

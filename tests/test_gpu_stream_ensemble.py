@@ -1,6 +1,6 @@
 """The multi-stream ensemble (several contexts, each with its own HIP stream,
 stepped by host threads): every run's result is bitwise that of the same run
-stepped alone, with the chosen path's solver (defect correction, FP32c maps)."""
+stepped alone, with the Anderson path's solver (FP32c then FP64 maps)."""
 import numpy as np
 import pytest
 
