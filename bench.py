@@ -407,10 +407,11 @@ def ensemble_bench(args):
         res[form] = dict(run_steps_per_s=len(runs) * args.steps / el, el=el, gamma_tile=bt, waves_per_cu=waves,
                          tuner_ms=ms.tolist(), all_converged=all(s["exit_reason"] == 0 for st in stats for s in st))
         log(f"[ensemble {form}] {res[form]['run_steps_per_s']:.2f} run-steps/s")
-    # the single-run defect-correction path (FP32c outer maps, learned warm
-    # start) on 4 contexts with a HIP stream each, stepped by host threads
+    # the single-run chosen path (FP32c defect correction with the hand-over,
+    # learned warm start) on 4 contexts with a HIP stream each, stepped by host
+    # threads
     from collop import warmstart as W
-    pch = collop.defect_config("bimodal_n65536").step_params(precision=collop.PREC_F32C,
+    pch = collop.chosen_config("bimodal_n65536").step_params(precision=collop.PREC_F32C,
                                                              warmstart=collop.WARM_MLP)
     sens = collop.StreamEnsemble(np.linspace(-6, 6, 33), np.linspace(-6, 6, 33), [(v, w) for v, w, _, _ in runs],
                                  workers=4, warm_start=W.load_packaged)
@@ -423,7 +424,7 @@ def ensemble_bench(args):
     stream = {"run_steps_per_s": round(len(runs) * args.steps / s_el, 4), "workers": 4,
               "mean_outer_maps": float(np.mean([s["iters"] for st in sst for s in st])),
               "all_converged": all(s["exit_reason"] == 0 for st in sst for s in st)}
-    log(f"[ensemble streams] defect-correction single-run path on 4 streams: {stream['run_steps_per_s']:.2f} run-steps/s, "
+    log(f"[ensemble streams] chosen single-run path on 4 streams: {stream['run_steps_per_s']:.2f} run-steps/s, "
         f"mean outer maps {stream['mean_outer_maps']:.1f}")
     # the same runs stepped one after another (single-run path, symmetric pair
     # sum), a bounded sample of steps
@@ -449,7 +450,7 @@ def ensemble_bench(args):
                                 "gamma_tile": res["ordered"]["gamma_tile"],
                                 "waves_per_cu": res["ordered"]["waves_per_cu"]},
            "sequential_run_steps_per_s": round(sequential, 4),
-           "streams_defect_correction": stream,
+           "streams_chosen_path": stream,
            "batched_vs_sequential": round(sym["run_steps_per_s"] / sequential, 3),
            "tuner_ms": {"symmetric": sym["tuner_ms"][0], "ordered": res["ordered"]["tuner_ms"]}}
     print(json.dumps(out), flush=True)
