@@ -162,9 +162,9 @@ DEFECT = dict(gpu_fp32=True, solver="defect", dc_outer_m=3, dc_near_theta=0.02, 
 # control on the device, symmetric pair sum.
 ANDERSON_F32C = dict(gpu_fp32=True, solver="anderson")
 # The chosen path: the defect-correction path (DEFECT) with the hand-over: when
-# a frozen-metric inner solve ends unconverged, an FP64 outer iteration does
-# not halve the residual, or the outer iteration stagnates (stiff near pairs),
-# the step continues with the Anderson solve in FP64 maps.
+# a frozen-metric inner solve ends unconverged, an outer iteration does not
+# reduce the residual by 0.4, or the outer iteration stagnates (stiff near
+# pairs), the step continues with the Anderson solve in FP64 maps.
 CHOSEN = dict(DEFECT, dc_fallback=True)
 # Stiff-cluster elimination (an opt-in extension): pairs with
 # max(w_i, w_j) / |d| > CLUSTER_TAU / dt at the start of a step form clusters
@@ -206,7 +206,7 @@ def defect_config(name: str, **over) -> SimParameters:
 def chosen_description(name: str, mlp: bool) -> str:
     return ("frozen-metric defect correction (step_defect!): FP32c outer maps (F and metric A) until the FP32 "
             "rounding floor, then FP64 outer maps, an outer Anderson window of 3 and exact near pairs; an inner solve "
-            "that misses its tolerance in 60 maps, an FP64 outer iteration that does not halve the residual, or "
+            "that misses its tolerance in 60 maps, an outer iteration that does not reduce the residual by 0.4, or "
             "outer stagnation hands the step to the Anderson solve (step_anderson!, FP64 maps, control on the "
             "device); "
             "FP32c Euler predictor" + (" + learned MLP warm start" if mlp else "")

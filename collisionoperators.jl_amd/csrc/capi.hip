@@ -1527,10 +1527,15 @@ int validate_params(collop_ctx* ctx, const collop_step_params* p) {
 // max(dc_eta ||G(v_k) - v_k||, abs_floor / 2), at most dc_max_inner maps.
 // exit_reason of defect_solve when the step continues with the Anderson solve
 constexpr int kDefectHandover = 3;
-// the hand-over's outer contraction test: an FP64 outer iteration must reduce
-// the residual at least by this factor (an Anderson map reduces it by about
-// half in the steps with stiff near pairs, at the cost of one pair sum)
-constexpr double kHandoverRatio = 0.5;
+// the hand-over's outer contraction test: an outer iteration must reduce the
+// residual at least by this factor, FP64 or FP32c (the latter while the
+// residual is above kHandoverFloorMargin times the FP32 floor).  An Anderson
+// map reduces it by about half in the steps with stiff near pairs, and an
+// outer iteration (a pair sum with the metric plus its inner maps) costs about
+// 1.3 Anderson maps: 0.5^1.3 = 0.4.
+constexpr double kHandoverRatio = 0.4;
+constexpr double kHandoverRatio32 = 0.4;
+constexpr double kHandoverFloorMargin = 4.0;
 
 int defect_solve(collop_ctx* ctx, const collop_step_params& p, collop_step_stats* out) {
   const int n = (int)ctx->N;
@@ -1581,6 +1586,15 @@ int defect_solve(collop_ctx* ctx, const collop_step_params& p, collop_step_stats
         since_best32 = 0;
       } else {
         since_best32++;
+      }
+      // with the hand-over, an FP32c outer iteration that does not reduce the
+      // residual by kHandoverRatio32, well above the FP32 floor, hands the step
+      // to the Anderson solve from the outer iterate
+      if (p.dc_fallback && prev32 > 0.0 && nr > kHandoverRatio32 * prev32 && nr > kHandoverFloorMargin * floor32) {
+        exit_reason = kDefectHandover;
+        k_done = k;
+        res_out = nr;
+        break;
       }
       if (nr < std::max(floor32, std::max(p.tol * (nv + 1e-30), p.abs_floor)) || since_best32 >= 2 ||
           k >= p.max_iter / 2) {

@@ -258,9 +258,11 @@ typedef struct {
   int32_t dc_fallback;      /* DEFECT extension: 1 = hand the step over to the
                                Anderson solve (step_anderson!, FP64 maps) with
                                the iterations left when (a) an inner solve ends
-                               unconverged at dc_max_inner maps (from the outer
-                               iterate), (b) an FP64 outer iteration does not
-                               halve the residual or (c) the outer stagnation
+                               unconverged at dc_max_inner maps or (b) an FP32c
+                               outer iteration, above 4x the FP32 floor, does
+                               not reduce the residual by 0.4 (both from the
+                               outer iterate), (c) an FP64 outer iteration does
+                               not reduce it by 0.4 or (d) the outer stagnation
                                test fires (both from the best outer iterate's
                                map); 0 = off (the reference's step_defect!)    */
 } collop_step_params;

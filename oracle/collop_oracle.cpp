@@ -1935,9 +1935,9 @@ void orc_step_defect(const orc_mesh* ms, const orc_solver_params* Pp, double* v1
       *st = {k, 0, nr, nrm0, 0, 0, 0, 0, 0, inner, 0, 0, 0};
       return;
     }
-    // hand-over extension: an outer iteration that does not halve the residual
-    // continues the step with step_anderson! from the best iterate's image
-    if (P.dc_fallback && k >= 2 && nr > 0.5 * nr_prev) {
+    // hand-over extension: an outer iteration that does not reduce the residual
+    // by 0.4 continues the step with step_anderson! from the best iterate's image
+    if (P.dc_fallback && k >= 2 && nr > 0.4 * nr_prev) {
       std::memcpy(v1, vbest.data(), sizeof(double) * n2);
       defect_handover(ms, P, v1, v0, w, N, S0, st, k, nrm0, inner);
       return;

@@ -264,7 +264,8 @@ def test_defect_handover_by_the_contraction_test_matches_oracle(gpu_ctx):
     """The hand-over's outer contraction test against the oracle's: with a loose
     inner tolerance (dc_eta = 0.8) the second FP64 outer iteration of a
     well-conditioned step (isotropic N = 1024, dt 0.005) reduces the residual by
-    a factor 0.88 only, so the step hands over after its third outer map and
+    a factor 0.88 only (the test asks for 0.4), so the step hands over after its
+    third outer map and
     the Anderson solve finishes it.  FP64 maps; the hand-over point, the map
     counts and the state match."""
     v, w = collop.markers("gaussian", 1024, 42)
@@ -284,4 +285,28 @@ def test_defect_handover_by_the_contraction_test_matches_oracle(gpu_ctx):
     assert st_ref.handover == 3 and st["handover"] == 3
     assert st["exit_reason"] == 0 and st_ref.exit_reason == 0
     assert abs(st["iters"] - st_ref.iters) <= 1
+    assert dv <= 1e-9
+
+
+def test_defect_handover_in_the_fp32c_phase(gpu_ctx):
+    """The contraction test also holds in the FP32c phase, while the residual is
+    well above the FP32 floor: with dc_eta = 0.8 the step of the contraction
+    test above (isotropic N = 1024, dt 0.005) hands over after its third outer
+    map with FP32c outer maps as with FP64 ones, and both land on the same
+    root."""
+    v, w = collop.markers("gaussian", 1024, 42)
+    bp = uniform_bp(-6, 6, 16)
+    ws = collop.Workspace(bp, bp, ctx=gpu_ctx)
+    q = collop.config("gaussian_n4096", DT=0.005, solver="defect", dc_eta=0.8, dc_fallback=True)
+    out = {}
+    for prec in (collop.PREC_F64, collop.PREC_F32C):
+        sim = collop.Simulation(ws, v, w)
+        out[prec] = (sim.run(q.step_params(precision=prec), 1)[0], sim.markers())
+    (s64, v64), (s32, v32) = out[collop.PREC_F64], out[collop.PREC_F32C]
+    dv = float(np.max(np.abs(v64 - v32)))
+    print(f"FP64 outer maps: handover after {s64['handover']}, {s64['iters']} maps; FP32c: after {s32['handover']} "
+          f"({s32['iters_f32']} FP32c), {s32['iters']} maps; max|dv| {dv:.2e}")
+    assert s64["handover"] == s32["handover"] == 3
+    assert s32["iters_f32"] == 3
+    assert s64["exit_reason"] == s32["exit_reason"] == 0
     assert dv <= 1e-9
