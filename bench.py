@@ -228,8 +228,9 @@ def precision_report(args):
     steps = args.steps
     out = {"workload": args.config, "N": N, "steps": steps, "dt": p.DT}
     runs = {}
-    base = collop.defect_config(args.config)   # the same solver for both: only the precision differs
-    out["solver"] = "defect correction with outer Anderson window 3, near pairs w/|d| > 0.02"
+    base = collop.chosen_config(args.config)   # the same solver for both: only the precision differs
+    out["solver"] = ("the chosen path's solver with the Euler predictor: defect correction with outer Anderson "
+                     "window 3 and near pairs w/|d| > 0.02, stiff steps handed over to the FP64 Anderson solve")
     for name, prec in (("fp64", collop.PREC_F64), ("f32c", collop.PREC_F32C)):
         sim = collop.Simulation(ws, v, w)
         E0 = 0.5 * float(np.sum(w * (v[:, 0] ** 2 + v[:, 1] ** 2)))
@@ -245,8 +246,9 @@ def precision_report(args):
                           "dS": st["entropy"] - S_prev, "iters": st["iters"], "iters_f32": st["iters_f32"],
                           "inner": st["inner_maps"], "residual": st["residual"], "exit": st["exit_reason"]})
             E_prev, P_prev, S_prev = st["energy"], np.array(st["momentum"]), st["entropy"]
-            log(f"  {name} step {i}: iters {st['iters']} (f32 {st['iters_f32']}) exit {st['exit_reason']} "
-                f"dE/E {drift[-1]['dE_rel']:.1e} dS {drift[-1]['dS']:.2e}")
+            drift[-1]["handover"] = st["handover"]
+            log(f"  {name} step {i}: iters {st['iters']} (f32 {st['iters_f32']}, handover {st['handover']}) exit "
+                f"{st['exit_reason']} dE/E {drift[-1]['dE_rel']:.1e} dS {drift[-1]['dS']:.2e}")
         runs[name] = dict(v=sim.markers(), fc=sim.coeffs(), S=S_prev, E=E_prev, P=P_prev, drift=drift,
                           seconds=time.perf_counter() - t0)
     a, b = runs["fp64"], runs["f32c"]
