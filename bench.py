@@ -130,29 +130,39 @@ def run_path(ws, p, v, w, precision, warmup, steps, solver=None, warmstart=None,
             + f") exit {st['exit_reason']} res {st['residual']:.2e} {st['seconds'] * 1e3:.1f} ms")
     elapsed = time.perf_counter() - t0
     ms, launches = collop.profile(ws, False)
+    out = dict(steps_per_s=steps / elapsed, elapsed=elapsed, stats=stats, pair_ms=ms.tolist(), tiles=tiles,
+               pair_launches=launches, markers=sim.markers())
+    out.update(constraint_summary(stats, E0, E_prev, P_prev, S_prev))
+    return out
+
+
+def constraint_summary(stats, E0, E_prev, P_prev, S_prev):
+    """The metric's constraint over the timed steps (per-step statistics of
+    collop_run_steps, in order) from the state before them (energy E_prev,
+    momentum P_prev, entropy S_prev; E0 the initial energy): every step
+    converged (exit 0, solver.jl:349-367), per step |E_n+1 - E_n| / E0 and
+    |P_n+1 - P_n| at most 1e-10, and the entropy never falls."""
     dE = dP = 0.0
     dS_min = np.inf
+    P_prev = np.asarray(P_prev, dtype=float)
     for st in stats:
         dE = max(dE, abs(st["energy"] - E_prev) / E0)
         dP = max(dP, float(np.max(np.abs(np.array(st["momentum"]) - P_prev))))
         dS_min = min(dS_min, st["entropy"] - S_prev)
         E_prev, P_prev, S_prev = st["energy"], np.array(st["momentum"]), st["entropy"]
-    n_stag = sum(1 for s in stats if s["exit_reason"] == 1)
-    n_cap = sum(1 for s in stats if s["exit_reason"] == 2)
     all_conv = all(s["exit_reason"] == 0 for s in stats)
-    return dict(steps_per_s=steps / elapsed, elapsed=elapsed, stats=stats, pair_ms=ms.tolist(), tiles=tiles,
-                pair_launches=launches, max_dE_rel=dE, max_dP=dP, min_dS=float(dS_min),
+    return dict(max_dE_rel=dE, max_dP=dP, min_dS=float(dS_min),
                 mean_iters=float(np.mean([s["iters"] for s in stats])),
                 mean_iters_f32=float(np.mean([s["iters_f32"] for s in stats])),
                 mean_inner=float(np.mean([s["inner_maps"] for s in stats])),
                 mean_r0=float(np.mean([s["r0"] for s in stats])),
-                all_converged=all_conv, stagnated_steps=n_stag, max_iter_steps=n_cap,
+                all_converged=all_conv, stagnated_steps=sum(1 for s in stats if s["exit_reason"] == 1),
+                max_iter_steps=sum(1 for s in stats if s["exit_reason"] == 2),
                 max_exit_residual=float(max(s["residual"] for s in stats)),
                 clusters_mean=float(np.mean([s["n_clusters"] for s in stats])),
                 cluster_solve_failures=int(sum(s["cl_fail"] for s in stats)),
                 handed_over_steps=int(sum(s["handover"] > 0 for s in stats)),
-                constraint_met=bool(all_conv and dE <= 1e-10 and dP <= 1e-10 and dS_min >= 0.0),
-                markers=sim.markers())
+                constraint_met=bool(all_conv and dE <= 1e-10 and dP <= 1e-10 and dS_min >= 0.0))
 
 
 def path_summary(res, ref=None):
