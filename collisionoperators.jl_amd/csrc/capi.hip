@@ -1567,6 +1567,8 @@ int defect_solve(collop_ctx* ctx, const collop_step_params& p, collop_step_stats
   // the best iterate starts as the initial guess
   if ((r = copy_d2d(ctx, ctx->s_best, ctx->s_v1, n2b))) return r;
   for (int k = 1; k <= p.max_iter; ++k) {
+    // the hand-over needs at least one iteration left for the Anderson solve
+    const bool fallback = p.dc_fallback && k < p.max_iter;
     const bool find_near = k == 1 && p.dc_near_theta > 0.0;
     // the metric A: every outer map (mode 0); FP64 maps after an FP32c phase
     // reuse the last FP32c A (mode 1); the step's first outer map only (mode 2)
@@ -1590,7 +1592,7 @@ int defect_solve(collop_ctx* ctx, const collop_step_params& p, collop_step_stats
       // with the hand-over, an FP32c outer iteration that does not reduce the
       // residual by kHandoverRatio32, well above the FP32 floor, hands the step
       // to the Anderson solve from the outer iterate
-      if (p.dc_fallback && prev32 > 0.0 && nr > kHandoverRatio32 * prev32 && nr > kHandoverFloorMargin * floor32) {
+      if (fallback && prev32 > 0.0 && nr > kHandoverRatio32 * prev32 && nr > kHandoverFloorMargin * floor32) {
         exit_reason = kDefectHandover;
         k_done = k;
         res_out = nr;
@@ -1625,7 +1627,7 @@ int defect_solve(collop_ctx* ctx, const collop_step_params& p, collop_step_stats
       // with the hand-over, an FP64 outer iteration that does not reduce the
       // residual by kHandoverRatio hands the step to the Anderson solve from
       // the best iterate (its Picard image)
-      if (p.dc_fallback && k64 >= 2 && nr > kHandoverRatio * prev64) {
+      if (fallback && k64 >= 2 && nr > kHandoverRatio * prev64) {
         std::swap(ctx->s_v1, ctx->s_best);
         exit_reason = kDefectHandover;
         k_done = k;
@@ -1637,7 +1639,7 @@ int defect_solve(collop_ctx* ctx, const collop_step_params& p, collop_step_stats
     if (!use32 && k64 > p.dc_stag_window && k64 % p.dc_stag_window == 0) {
       if ((nrm_best_window - nrm_best) / (nrm_best_window + 1e-30) < p.stag_rel_tol) {
         std::swap(ctx->s_v1, ctx->s_best);
-        exit_reason = p.dc_fallback ? kDefectHandover : 1;
+        exit_reason = fallback ? kDefectHandover : 1;
         k_done = k;
         res_out = nrm_best;
         break;
@@ -1728,7 +1730,7 @@ int defect_solve(collop_ctx* ctx, const collop_step_params& p, collop_step_stats
       std::swap(ctx->s_Gp, ctx->s_Gv);
     }
     if (has_pend) return fail(ctx, COLLOP_ERR_HIP, "internal: an Anderson update was left pending");
-    if (!inner_ok && p.dc_fallback) {
+    if (!inner_ok && fallback) {
       // the frozen-metric solve did not converge (stiff near pairs): the
       // Anderson solve takes over from the outer iterate x_k
       std::swap(ctx->s_v1, ctx->s_xk);
@@ -1965,7 +1967,7 @@ int step_once(collop_ctx* ctx, const collop_step_params& p, collop_step_stats* o
   collop_step_params pa = p;
   if (handover) {
     pa.solver = COLLOP_SOLVER_ANDERSON;
-    pa.max_iter = std::max(1, p.max_iter - dc.iters);
+    pa.max_iter = p.max_iter - dc.iters;  // >= 1: a hand-over leaves at least one
     pa.precision = COLLOP_PREC_F64;
   }
   const bool anderson = pa.solver == COLLOP_SOLVER_ANDERSON;

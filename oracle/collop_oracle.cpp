@@ -1859,7 +1859,7 @@ static void defect_handover(const orc_mesh* ms, const orc_solver_params& P, doub
                             const double* w, int64_t N, double S0, orc_step_stats* st, int k, double nrm0, int inner) {
   orc_solver_params Pa = P;
   Pa.solver = 0;
-  Pa.max_iter = std::max(1, P.max_iter - k);
+  Pa.max_iter = P.max_iter - k;
   orc_step_anderson(ms, &Pa, v1, v0, w, N, S0, st, nullptr);
   st->iters += k;
   st->r0 = nrm0;
@@ -1917,6 +1917,8 @@ void orc_step_defect(const orc_mesh* ms, const orc_solver_params* Pp, double* v1
       dGo((size_t)std::max(mo, 1) * n2);
   int ohist = 0, oslot = 0;
   for (int k = 1; k <= P.max_iter; ++k) {
+    // the hand-over needs at least one iteration left for step_anderson!
+    const bool fallback = P.dc_fallback && k < P.max_iter;
     landau_geff(ms, s.Geff.data(), v1, v0, w, N, S0, s, P.use_gonzalez, P.use_logsq);
     orc_collision_metric(box, s.vmid.data(), w, s.Geff.data(), N, s.dotv.data(), A.data(), P.nthreads);
     for (int64_t i = 0; i < n2; ++i) {
@@ -1937,7 +1939,7 @@ void orc_step_defect(const orc_mesh* ms, const orc_solver_params* Pp, double* v1
     }
     // hand-over extension: an outer iteration that does not reduce the residual
     // by 0.4 continues the step with step_anderson! from the best iterate's image
-    if (P.dc_fallback && k >= 2 && nr > 0.4 * nr_prev) {
+    if (fallback && k >= 2 && nr > 0.4 * nr_prev) {
       std::memcpy(v1, vbest.data(), sizeof(double) * n2);
       defect_handover(ms, P, v1, v0, w, N, S0, st, k, nrm0, inner);
       return;
@@ -1946,7 +1948,7 @@ void orc_step_defect(const orc_mesh* ms, const orc_solver_params* Pp, double* v1
     if (k > P.dc_stag_window && k % P.dc_stag_window == 0) {
       if ((nrm_best_window - nrm_best) / (nrm_best_window + 1e-30) < P.stag_rel_tol) {
         std::memcpy(v1, vbest.data(), sizeof(double) * n2);
-        if (P.dc_fallback) {
+        if (fallback) {
           defect_handover(ms, P, v1, v0, w, N, S0, st, k, nrm0, inner);
           return;
         }
@@ -1976,7 +1978,7 @@ void orc_step_defect(const orc_mesh* ms, const orc_solver_params* Pp, double* v1
     bool inner_ok = false;
     inner += anderson_inner(frozen, v1, n2, std::max(P.dc_eta * nr, 0.5 * P.abs_floor), P.dc_max_inner,
                             P.m_anderson, P.damping, P.reg_factor, &inner_ok);
-    if (!inner_ok && P.dc_fallback) {  // the Anderson solve continues from x_k
+    if (!inner_ok && fallback) {  // the Anderson solve continues from x_k
       std::memcpy(v1, xk.data(), sizeof(double) * n2);
       defect_handover(ms, P, v1, v0, w, N, S0, st, k, nrm0, inner);
       return;

@@ -219,6 +219,13 @@ def test_defect_handover_with_one_inner_map_is_the_anderson_step(gpu_ctx):
     np.testing.assert_array_equal(va, vb)
     assert all(s["iters_f32"] == 1 for s in sc)
     assert np.max(np.abs(vc - va)) <= 1e-9
+    # no iteration left for the Anderson solve: the iteration cap, no hand-over (as the oracle)
+    for max_iter, handover, iters in ((1, 0, 1), (2, 1, 2)):
+        prm = q.step_params(precision=collop.PREC_F64)
+        prm.max_iter = max_iter
+        d = collop.Simulation(ws, v, w)
+        st = d.run(prm, 1)[0]
+        assert (st["handover"], st["iters"], st["exit_reason"]) == (handover, iters, 2)
 
 
 def test_chosen_path_steps_bimodal_n65536(gpu_ctx):

@@ -209,3 +209,21 @@ def test_defect_handover_lands_on_a_root():
     assert abs(q1 - p1) <= 1e-14 and abs(q2 - p2) <= 1e-14
     assert abs(E1 - E0) <= 1e-11 * E0
     assert m.entropy(fcn) >= S0
+
+
+def test_defect_handover_needs_an_iteration_left():
+    """With max_iter = 1 there is no iteration left for the Anderson solve: the
+    step leaves through the iteration cap after its one outer map, without a
+    hand-over, as step_defect! does."""
+    v, w = collop.markers("gaussian", 512, 3)
+    bp = uniform_bp(-6, 6, 12)
+    m = O.Mesh(bp, bp)
+    fc = m.l2_project(v, w)
+    S0 = m.entropy(fc)
+    P = O.default_params(0.02, m=5, solver=1, dc_max_inner=1, dc_fallback=1, max_iter=1)
+    v1, _ = m.predictor(P, v, w, fc)
+    vd, sd, _ = m.step_defect(P, v1, v, w, S0)
+    assert sd.handover == 0 and sd.exit_reason == 2 and sd.iters == 1
+    P.max_iter = 2                                   # one iteration left: the hand-over runs one Anderson map
+    vd, sd, _ = m.step_defect(P, v1, v, w, S0)
+    assert sd.handover == 1 and sd.iters == 2 and sd.exit_reason == 2
