@@ -20,5 +20,5 @@ from .params import (ANDERSON_F32C, CHOSEN, CONFIGS, DEFECT, SimParameters, ande
 from .records import (CONS_COLS, STATS_COLS, RunRecord, checkpoint_path, cons_time_at,  # noqa: F401
                       load_checkpoint, migrate_cons_csv, run_recorded, save_checkpoint, truncate_csv_after)
 from .sim import (Ensemble, Simulation, StreamEnsemble, pair_mode, profile, set_device_solve, set_fused_maps,  # noqa: F401
-                  set_pair_mode, set_pair_tile)
+                  set_pair_mode, set_pair_tile, set_weight_path)
 from .warmstart import WarmStart, glorot_init  # noqa: F401

@@ -176,6 +176,7 @@ SIGNATURES = [
     ("collop_set_pair_mode", C.c_int, [_vp, C.c_int, C.c_int]),
     ("collop_set_fused_maps", C.c_int, [_vp, C.c_int]),
     ("collop_set_device_solve", C.c_int, [_vp, C.c_int]),
+    ("collop_set_weight_path", C.c_int, [_vp, C.c_int]),
     ("collop_get_pair_mode", C.c_int, [_vp, C.c_int64, C.POINTER(C.c_int), C.POINTER(C.c_int),
                                       C.POINTER(C.c_int)]),
     ("collop_profile", C.c_int, [_vp, C.c_int, _dp, C.POINTER(C.c_long)]),

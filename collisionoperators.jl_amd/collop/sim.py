@@ -89,6 +89,12 @@ def set_fused_maps(ws: Workspace, on: bool = True):
     ws.ctx.check(ws.ctx.lib.collop_set_fused_maps(ws.ctx.h, 1 if on else 0))
 
 
+def set_weight_path(ws: Workspace, on: bool = True):
+    """The equal-weight form of the FP64 symmetric pair sum (on, the default)
+    or the general form: the same bits.  See collop_set_weight_path."""
+    ws.ctx.check(ws.ctx.lib.collop_set_weight_path(ws.ctx.h, 1 if on else 0))
+
+
 def set_device_solve(ws: Workspace, on: bool = True):
     """Anderson control of Landau steps on the device (the default: the host
     only launches maps) or on the host (every map's statistics read back);

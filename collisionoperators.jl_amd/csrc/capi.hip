@@ -295,7 +295,7 @@ int publish_stat(collop_ctx* ctx, int nparts) {
 int run_pair(collop_ctx* ctx, int kind, int N, double* part, const int* live = nullptr) {
   if (kind == 0) {
     prof_begin(ctx, 0);
-    launch_pair_f64(P<Src64>(ctx->b_s64), N, ctx->tile64, part, ctx->stream, live);
+    launch_pair_f64(P<Src64>(ctx->b_s64), N, ctx->tile64, part, ctx->stream, live, ctx->w_pow2 && ctx->pow2w_path);
   } else {
     prof_begin(ctx, 1);
     launch_pair_f32c(P<Src32>(ctx->b_s32), N, ctx->tile32, part, ctx->stream, live);
@@ -2098,12 +2098,16 @@ int collop_state_set(collop_ctx* ctx, const double* v, const double* w, int64_t 
   if (N < 2) return fail(ctx, COLLOP_ERR_INVALID, "the implicit step needs N >= 2 markers");
   if (!v || !w) return fail(ctx, COLLOP_ERR_INVALID, "null array");
   double wsum = 0.0, wmax = 0.0;
+  bool same_w = true;
   for (int64_t i = 0; i < N; ++i) {
     if (!(w[i] > 0.0) || !std::isfinite(w[i]))
       return fail(ctx, COLLOP_ERR_INVALID, "weights must be positive and finite (G_eff divides by w)");
     wsum += w[i];
     wmax = std::max(wmax, w[i]);
+    same_w = same_w && w[i] == w[0];
   }
+  int w_exp = 0;
+  ctx->w_pow2 = same_w && std::frexp(w[0], &w_exp) == 0.5;
   HIPC(hipSetDevice(ctx->device));
   if ((r = ensure_scratch(ctx, N))) return r;
   const size_t n2b = (size_t)2 * N * 8;
@@ -2457,6 +2461,13 @@ int collop_set_fused_maps(collop_ctx* ctx, int on) {
   if (!ctx) return COLLOP_ERR_INVALID;
   if (on != 0 && on != 1) return fail(ctx, COLLOP_ERR_INVALID, "on must be 0 or 1");
   ctx->fused_maps = on == 1;
+  return COLLOP_OK;
+}
+
+int collop_set_weight_path(collop_ctx* ctx, int on) {
+  if (!ctx) return COLLOP_ERR_INVALID;
+  if (on != 0 && on != 1) return fail(ctx, COLLOP_ERR_INVALID, "on must be 0 or 1");
+  ctx->pow2w_path = on == 1;
   return COLLOP_OK;
 }
 

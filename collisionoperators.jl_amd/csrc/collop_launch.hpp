@@ -90,8 +90,10 @@ struct PairTile {
 __host__ __device__ inline int sym_rows(int N, int R) { return (N + 128 * R - 1) / (128 * R); }
 constexpr int kSymMaxSlots = 512;    // larger N falls back to the ordered sum
 constexpr int kSymPreferSlots = 256;  // the automatic choice takes 4 waves per block up to this many slots
+// pow2w: every marker has the same weight, a power of two (the plain sum only;
+// the same bits as without it, with two multiplies fewer per pair)
 void launch_pair_sym_f64(const Src64* src, int N, int R, bool metric, const NearOut* near, double* part,
-                         hipStream_t st, const int* live = nullptr);
+                         hipStream_t st, const int* live = nullptr, bool pow2w = false);
 void launch_pair_sym_f32c(const Src32* src, int N, int R, bool metric, const NearOut* near, double* part,
                           hipStream_t st, const int* live = nullptr);
 
@@ -101,7 +103,7 @@ void launch_pack_src(const double* v, const double* G, const double* w, int N, d
                      double hi2, Src64* s64, Src32* s32, hipStream_t st);
 // part[S][2N] <- slice sums of the FP64 pair sum
 void launch_pair_f64(const Src64* src, int N, const PairTile& t, double* part, hipStream_t st,
-                     const int* live = nullptr);
+                     const int* live = nullptr, bool pow2w = false);
 // part[S][2N] <- slice sums of the FP32 pair sum (compensated, written in FP64)
 void launch_pair_f32c(const Src32* src, int N, const PairTile& t, double* part, hipStream_t st,
                       const int* live = nullptr);

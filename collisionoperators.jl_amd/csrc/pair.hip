@@ -322,9 +322,10 @@ void launch_pack_src(const double* v, const double* G, const double* w, int N, d
   hipLaunchKernelGGL(k_pack, dim3((N + 255) / 256), dim3(256), 0, st, v, G, w, N, lo1, hi1, lo2, hi2, s64, s32);
 }
 
-void launch_pair_f64(const Src64* src, int N, const PairTile& t, double* part, hipStream_t st, const int* live) {
+void launch_pair_f64(const Src64* src, int N, const PairTile& t, double* part, hipStream_t st, const int* live,
+                     bool pow2w) {
   if (N <= 0) return;
-  if (t.sym) return launch_pair_sym_f64(src, N, t.sym, false, nullptr, part, st, live);
+  if (t.sym) return launch_pair_sym_f64(src, N, t.sym, false, nullptr, part, st, live, pow2w);
   const int chunk = (N + t.slices - 1) / t.slices;
   const dim3 grid(grid_for(N, t)), block(kThreads);
   NearOut g{};

@@ -106,11 +106,22 @@ struct SymLds64 {
   int Lin[NEAR ? 128 * R : 1];
 };
 
+// Equal power-of-two weights (POW2W): every marker carries the same weight w =
+// 2^k, so the pair's two scale factors w_a c / |d|^3 and w_g c / |d|^3 are one,
+// and w multiplies G exactly.  The off-diagonal blocks then take G' = w G and
+// c' = w c (exact scaling: the same bits as w * (y^3 c)), and drop both weight
+// products per pair.  A marker outside the box (w_masked = 0) sits at the
+// common far point (kFar, kFar) with G' = 0: its pairs with markers in the box
+// give |d|^-3 = 0 exactly (underflow), its pairs with other far markers |d| = 0
+// (the guard), so it adds nothing on either side, as the mask does.
+constexpr double kFar = 1e150;
+
 // Block (a, b) of the symmetric FP64 sum over src[0, N) into part[S][C][N].
-template <int R, bool METRIC, bool NEAR>
+template <int R, bool METRIC, bool NEAR, bool POW2W = false>
 __device__ __forceinline__ void sym_block_f64(const Src64* __restrict__ src, int N, int a, int b,
                                               double* __restrict__ part, SymLds64<R, METRIC, NEAR>& S,
                                               const NearOut& near) {
+  static_assert(!(POW2W && (METRIC || NEAR)), "the equal-weight form is the plain pair sum only");
   constexpr int E = 128 * R, NTL = 2 * R, C = METRIC ? 5 : 2;
   using Tile = typename SymLds64<R, METRIC, NEAR>::Tile;
   auto& L = S.L;
@@ -173,6 +184,15 @@ __device__ __forceinline__ void sym_block_f64(const Src64* __restrict__ src, int
     }
     return;
   }
+  if constexpr (POW2W) {
+#pragma unroll
+    for (int q = 0; q < 2; ++q) {
+      gv1[q] = gin[q] ? gv1[q] : kFar;
+      gv2[q] = gin[q] ? gv2[q] : kFar;
+      gg1[q] = gin[q] ? gw[q] * gg1[q] : 0.0;
+      gg2[q] = gin[q] ? gw[q] * gg2[q] : 0.0;
+    }
+  }
   for (int e = threadIdx.x; e < E; e += 64 * R) {
     const int m = b * E + e;
     double x1 = 0.0, x2 = 0.0, y1 = 0.0, y2 = 0.0, w = 0.0;
@@ -185,6 +205,12 @@ __device__ __forceinline__ void sym_block_f64(const Src64* __restrict__ src, int
       y2 = s.g2;
       w = s.w;
       in = s.in_domain != 0.0;
+      if constexpr (POW2W) {
+        x1 = in ? x1 : kFar;
+        x2 = in ? x2 : kFar;
+        y1 = in ? w * y1 : 0.0;
+        y2 = in ? w * y2 : 0.0;
+      }
     }
     Tile& t = L[e >> 6];
     const int j = e & 63;
@@ -222,7 +248,13 @@ __device__ __forceinline__ void sym_block_f64(const Src64* __restrict__ src, int
         const double qd = __builtin_fma(d2, d2, u22);
         const double y3 = rsqrt3_f64(guard_q64(qd));
         const double c = __builtin_fma(ag1 - gg1[q], d2, -((ag2 - gg2[q]) * d1));
-        if constexpr (!METRIC) {
+        if constexpr (POW2W) {  // c is w c: one factor serves both sides
+          const double pc = y3 * c;
+          f1[q] = __builtin_fma(pc, d2, f1[q]);
+          f2[q] = __builtin_fma(-pc, d1, f2[q]);
+          b1 = __builtin_fma(-pc, d2, b1);
+          b2 = __builtin_fma(pc, d1, b2);
+        } else if constexpr (!METRIC) {
           const double pc = y3 * c;
           const double kg = aw * pc, ka = gw[q] * pc;
           f1[q] = __builtin_fma(kg, d2, f1[q]);
@@ -301,14 +333,14 @@ __device__ __forceinline__ void sym_block_f64(const Src64* __restrict__ src, int
   }
 }
 
-template <int R, bool METRIC, bool NEAR>
+template <int R, bool METRIC, bool NEAR, bool POW2W = false>
 __global__ __launch_bounds__(64 * R) void k_pair_sym_f64(const Src64* __restrict__ src, int N, long long n_off,
                                                          double* __restrict__ part, NearOut near) {
   __shared__ SymLds64<R, METRIC, NEAR> S;
   if (near.live && *near.live == 0) return;
   int a, b;
   block_rows(blockIdx.x, n_off, a, b);
-  sym_block_f64<R, METRIC, NEAR>(src, N, a, b, part, S, near);
+  sym_block_f64<R, METRIC, NEAR, POW2W>(src, N, a, b, part, S, near);
 }
 
 // FP32 terms, compensated sums: every 32 steps each accumulator's block sum is
@@ -547,12 +579,14 @@ __global__ __launch_bounds__(64 * R) void k_pair_sym_b(EnsView E, const int4* __
 
 template <int R>
 void sym_f64(const Src64* src, int N, bool metric, const NearOut* near, double* part, hipStream_t st,
-             const int* live) {
+             const int* live, bool pow2w) {
   const long long S = sym_rows(N, R), n_off = S * (S - 1) / 2;
   const dim3 grid((unsigned)(n_off + S)), block(64 * R);
   NearOut nz{};
   nz.live = live;
-  if (!metric)
+  if (!metric && pow2w)
+    hipLaunchKernelGGL((k_pair_sym_f64<R, false, false, true>), grid, block, 0, st, src, N, n_off, part, nz);
+  else if (!metric)
     hipLaunchKernelGGL((k_pair_sym_f64<R, false, false>), grid, block, 0, st, src, N, n_off, part, nz);
   else if (!near)
     hipLaunchKernelGGL((k_pair_sym_f64<R, true, false>), grid, block, 0, st, src, N, n_off, part, nz);
@@ -586,12 +620,12 @@ void sym_f32c(const Src32* src, int N, bool metric, const NearOut* near, double*
 }  // namespace
 
 void launch_pair_sym_f64(const Src64* src, int N, int R, bool metric, const NearOut* near, double* part,
-                         hipStream_t st, const int* live) {
+                         hipStream_t st, const int* live, bool pow2w) {
   if (N <= 0) return;
   switch (R) {
-    case 2: sym_f64<2>(src, N, metric, near, part, st, live); break;
-    case 4: sym_f64<4>(src, N, metric, near, part, st, live); break;
-    default: sym_f64<8>(src, N, metric, near, part, st, live); break;
+    case 2: sym_f64<2>(src, N, metric, near, part, st, live, pow2w); break;
+    case 4: sym_f64<4>(src, N, metric, near, part, st, live, pow2w); break;
+    default: sym_f64<8>(src, N, metric, near, part, st, live, pow2w); break;
   }
 }
 

@@ -186,3 +186,32 @@ def test_pair_mode_choice(sym_ws):
     collop.set_pair_tile(sym_ws, 2, 16, 1)
     assert collop.pair_mode(sym_ws, 65536) == (False, 0, 16)
     collop.set_pair_tile(sym_ws, 1, 0, 1)
+
+
+@pytest.mark.parametrize("N,precision", [(4096, "f64"), (4096, "f32c"), (65536, "f32c")])
+def test_equal_weight_form_is_bitwise_the_general_form(gpu_ctx, N, precision):
+    """The equal-weight form of the FP64 symmetric sum (every marker w = 1/N =
+    2^-k) against the general form on the same steps: the same bits, with 3 %
+    of the markers outside the box (their pairs vanish through the far point
+    instead of the mask).  Two steps of the chosen path's solver, FP64 outer and
+    Anderson maps included."""
+    p = collop.chosen_config("bimodal_n65536", N_PARTICLES=N)
+    v, w = collop.initial_markers(p)
+    rng = np.random.default_rng(5)
+    out = rng.choice(N, N * 3 // 100, replace=False)
+    v[out] *= 6.5 / np.max(np.abs(v[out]), axis=1, keepdims=True)  # pushed onto |v|_inf = 6.5
+    assert np.all(w == w[0]) and np.frexp(w[0])[0] == 0.5
+    ws = collop.build_workspace(p, ctx=gpu_ctx)
+    prm = p.step_params(precision=collop.PREC_F64 if precision == "f64" else collop.PREC_F32C,
+                        warmstart=collop.WARM_EULER)
+    res = {}
+    for on in (True, False):
+        collop.set_weight_path(ws, on)
+        sim = collop.Simulation(ws, v, w)
+        st = sim.run(prm, 2)
+        res[on] = (sim.markers(), [(s["iters"], s["iters_f32"], s["inner_maps"], s["residual"]) for s in st])
+    collop.set_weight_path(ws, True)
+    print(f"N={N} {precision}: maps {res[True][1]}")
+    assert res[True][1] == res[False][1]
+    np.testing.assert_array_equal(res[True][0], res[False][0])
+    np.testing.assert_array_equal(res[True][0][out], res[False][0][out])
