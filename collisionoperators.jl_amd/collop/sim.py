@@ -90,8 +90,8 @@ def set_fused_maps(ws: Workspace, on: bool = True):
 
 
 def set_weight_path(ws: Workspace, on: bool = True):
-    """The equal-weight form of the FP64 symmetric pair sum (on, the default)
-    or the general form: the same bits.  See collop_set_weight_path."""
+    """The equal-weight form of the symmetric pair sums (on, the default) or
+    the general form: the same bits.  See collop_set_weight_path."""
     ws.ctx.check(ws.ctx.lib.collop_set_weight_path(ws.ctx.h, 1 if on else 0))
 
 

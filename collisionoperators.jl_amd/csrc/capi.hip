@@ -292,13 +292,26 @@ int publish_stat(collop_ctx* ctx, int nparts) {
 
 // ---- building blocks on device buffers -------------------------------------
 // part <- pair sum of the packed sources (FP64: kind 0; FP32c: kind 1)
-int run_pair(collop_ctx* ctx, int kind, int N, double* part, const int* live = nullptr) {
+// True when all N weights are one power of two in [2^-60, 2^60]: the pair sums'
+// equal-weight form then gives the same bits (scaling by 2^k is exact in FP32
+// and FP64 over that range).
+bool equal_pow2_weights(const double* w, int64_t N) {
+  int e = 0;
+  if (N <= 0 || std::frexp(w[0], &e) != 0.5 || e < -59 || e > 61) return false;
+  for (int64_t i = 1; i < N; ++i)
+    if (w[i] != w[0]) return false;
+  return true;
+}
+
+// pow2w: the markers packed in b_s64/b_s32 satisfy equal_pow2_weights
+int run_pair(collop_ctx* ctx, int kind, int N, double* part, bool pow2w, const int* live = nullptr) {
+  pow2w = pow2w && ctx->pow2w_path;
   if (kind == 0) {
     prof_begin(ctx, 0);
-    launch_pair_f64(P<Src64>(ctx->b_s64), N, ctx->tile64, part, ctx->stream, live, ctx->w_pow2 && ctx->pow2w_path);
+    launch_pair_f64(P<Src64>(ctx->b_s64), N, ctx->tile64, part, ctx->stream, live, pow2w);
   } else {
     prof_begin(ctx, 1);
-    launch_pair_f32c(P<Src32>(ctx->b_s32), N, ctx->tile32, part, ctx->stream, live);
+    launch_pair_f32c(P<Src32>(ctx->b_s32), N, ctx->tile32, part, ctx->stream, live, pow2w);
   }
   CHECK_LAUNCH();
   prof_end(ctx);
@@ -565,6 +578,7 @@ static int collision_common(collop_ctx* ctx, int kind, const double* v, const do
                   (kind == 1 || kind == 3) ? nullptr : P<Src64>(ctx->b_s64),
                   (kind == 1 || kind == 3) ? P<Src32>(ctx->b_s32) : nullptr, ctx->stream);
   CHECK_LAUNCH();
+  const bool pow2w = equal_pow2_weights(w, N);
   int S;
   if (kind == 2) {
     prof_begin(ctx, 2);
@@ -574,12 +588,13 @@ static int collision_common(collop_ctx* ctx, int kind, const double* v, const do
     S = ctx->tile64.slices;
   } else if (kind == 3) {
     prof_begin(ctx, 3);
-    launch_pair_metric_f32c(P<Src32>(ctx->b_s32), n, ctx->tile32, P<double>(ctx->b_part), ctx->stream);
+    launch_pair_metric_f32c(P<Src32>(ctx->b_s32), n, ctx->tile32, P<double>(ctx->b_part), ctx->stream, nullptr,
+                            pow2w && ctx->pow2w_path);
     CHECK_LAUNCH();
     prof_end(ctx);
     S = ctx->tile32.slices;
   } else {
-    if ((r = run_pair(ctx, kind, n, P<double>(ctx->b_part)))) return r;
+    if ((r = run_pair(ctx, kind, n, P<double>(ctx->b_part), pow2w))) return r;
     S = kind == 0 ? ctx->tile64.slices : ctx->tile32.slices;
   }
   const int64_t len = (metric ? 5 : 2) * N;
@@ -1152,7 +1167,7 @@ int picard_map_dev(collop_ctx* ctx, const collop_step_params& p, int kind, const
   int r = geff_dev(ctx, p, vin, kind == 0 ? P<Src64>(ctx->b_s64) : nullptr, kind == 1 ? P<Src32>(ctx->b_s32) : nullptr,
                    nullptr, nullptr, nullptr, nullptr);
   if (r) return r;
-  if ((r = run_pair(ctx, kind, (int)ctx->N, P<double>(ctx->b_part)))) return r;
+  if ((r = run_pair(ctx, kind, (int)ctx->N, P<double>(ctx->b_part), ctx->w_pow2))) return r;
   const int S = kind == 0 ? ctx->tile64.slices : ctx->tile32.slices;
   const bool cl = kind == 0 && ctx->ncl > 0;
   if (cl && (r = cluster_solve_dev(ctx, p, P<double>(ctx->b_part), S))) return r;
@@ -1171,7 +1186,7 @@ int outer_map_dev(collop_ctx* ctx, const collop_step_params& p, int kind, bool f
   if (r) return r;
   const PairTile& t = kind == 0 ? ctx->tile64 : ctx->tile32;
   if (!metric) {  // the plain pair sum; A of an earlier outer map stays in s_FA + 2N
-    if ((r = run_pair(ctx, kind, n, P<double>(ctx->b_part)))) return r;
+    if ((r = run_pair(ctx, kind, n, P<double>(ctx->b_part), ctx->w_pow2))) return r;
     launch_slice_sum(P<double>(ctx->b_part), t.slices, (int64_t)2 * n, P<double>(ctx->s_FA), ctx->stream);
     CHECK_LAUNCH();
     return epilogue_dev(ctx, P<double>(ctx->s_FA), 1, P<double>(ctx->s_v), p.dt, P<double>(ctx->s_v1),
@@ -1186,7 +1201,7 @@ int outer_map_dev(collop_ctx* ctx, const collop_step_params& p, int kind, bool f
     launch_pair_metric_f64(P<Src64>(ctx->b_s64), n, t, P<double>(ctx->b_part), ctx->stream, find_near ? &near : nullptr);
   else
     launch_pair_metric_f32c(P<Src32>(ctx->b_s32), n, t, P<double>(ctx->b_part), ctx->stream,
-                            find_near ? &near : nullptr);
+                            find_near ? &near : nullptr, ctx->w_pow2 && ctx->pow2w_path);
   CHECK_LAUNCH();
   prof_end(ctx);
   launch_slice_sum(P<double>(ctx->b_part), t.slices, (int64_t)5 * n, P<double>(ctx->s_FA), ctx->stream);
@@ -1846,7 +1861,7 @@ int anderson_device(collop_ctx* ctx, const collop_step_params& p, AndersonRun& a
     if ((r = geff_dev(ctx, p, P<double>(ctx->s_v1), kind == 0 ? P<Src64>(ctx->b_s64) : nullptr,
                       kind == 1 ? P<Src32>(ctx->b_s32) : nullptr, nullptr, nullptr, nullptr, nullptr)))
       return r;
-    if ((r = run_pair(ctx, kind, n, P<double>(ctx->b_part), &dc->gate[0]))) return r;
+    if ((r = run_pair(ctx, kind, n, P<double>(ctx->b_part), ctx->w_pow2, &dc->gate[0]))) return r;
     EpiArgs a{};
     a.part = P<double>(ctx->b_part);
     a.S = ctx->tile64.slices;
@@ -1902,7 +1917,7 @@ int step_once(collop_ctx* ctx, const collop_step_params& p, collop_step_stats* o
     launch_pack_src(P<double>(ctx->s_v), P<double>(ctx->b_G), P<double>(ctx->s_w), n, m.lo1, m.hi1, m.lo2, m.hi2,
                     pk == 0 ? P<Src64>(ctx->b_s64) : nullptr, pk == 1 ? P<Src32>(ctx->b_s32) : nullptr, ctx->stream);
     CHECK_LAUNCH();
-    if ((r = run_pair(ctx, pk, n, P<double>(ctx->b_part)))) return r;
+    if ((r = run_pair(ctx, pk, n, P<double>(ctx->b_part), ctx->w_pow2))) return r;
     launch_euler(P<double>(ctx->b_part), pk == 0 ? ctx->tile64.slices : ctx->tile32.slices, P<double>(ctx->s_v),
                  p.dt, n, P<double>(ctx->s_v1), P<double>(ctx->s_dotv), ctx->stream);
     CHECK_LAUNCH();
@@ -2106,8 +2121,7 @@ int collop_state_set(collop_ctx* ctx, const double* v, const double* w, int64_t 
     wmax = std::max(wmax, w[i]);
     same_w = same_w && w[i] == w[0];
   }
-  int w_exp = 0;
-  ctx->w_pow2 = same_w && std::frexp(w[0], &w_exp) == 0.5;
+  ctx->w_pow2 = same_w && equal_pow2_weights(w, 1);
   HIPC(hipSetDevice(ctx->device));
   if ((r = ensure_scratch(ctx, N))) return r;
   const size_t n2b = (size_t)2 * N * 8;

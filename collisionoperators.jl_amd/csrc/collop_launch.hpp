@@ -90,12 +90,13 @@ struct PairTile {
 __host__ __device__ inline int sym_rows(int N, int R) { return (N + 128 * R - 1) / (128 * R); }
 constexpr int kSymMaxSlots = 512;    // larger N falls back to the ordered sum
 constexpr int kSymPreferSlots = 256;  // the automatic choice takes 4 waves per block up to this many slots
-// pow2w: every marker has the same weight, a power of two (the plain sum only;
-// the same bits as without it, with two multiplies fewer per pair)
+// pow2w: every marker has the same weight, a power of two (FP64: the plain sum
+// only; FP32c: the plain and the metric sums without the near-pair search; the
+// same bits as without it, with fewer multiplies per pair)
 void launch_pair_sym_f64(const Src64* src, int N, int R, bool metric, const NearOut* near, double* part,
                          hipStream_t st, const int* live = nullptr, bool pow2w = false);
 void launch_pair_sym_f32c(const Src32* src, int N, int R, bool metric, const NearOut* near, double* part,
-                          hipStream_t st, const int* live = nullptr);
+                          hipStream_t st, const int* live = nullptr, bool pow2w = false);
 
 // Pack (v, G, w) into Src64/Src32 records.  Markers outside the open box get
 // w_masked = 0 and in_domain = 0 (functions.jl:212-213, :219-220).
@@ -106,14 +107,14 @@ void launch_pair_f64(const Src64* src, int N, const PairTile& t, double* part, h
                      const int* live = nullptr, bool pow2w = false);
 // part[S][2N] <- slice sums of the FP32 pair sum (compensated, written in FP64)
 void launch_pair_f32c(const Src32* src, int N, const PairTile& t, double* part, hipStream_t st,
-                      const int* live = nullptr);
+                      const int* live = nullptr, bool pow2w = false);
 // part[S][5N] <- slice sums of F (2 comps) and A (3 comps); near != null also
 // collects the near pairs (one target per lane, one Newton step)
 void launch_pair_metric_f64(const Src64* src, int N, const PairTile& t, double* part, hipStream_t st,
                             const NearOut* near = nullptr);
 // the same from FP32 pair terms (compensated sums, written in FP64)
 void launch_pair_metric_f32c(const Src32* src, int N, const PairTile& t, double* part, hipStream_t st,
-                             const NearOut* near = nullptr);
+                             const NearOut* near = nullptr, bool pow2w = false);
 // out[ncomp*N] <- sum over S slices in fixed order
 void launch_slice_sum(const double* part, int S, int64_t len, double* out, hipStream_t st);
 

@@ -367,9 +367,10 @@ void launch_pair_metric_f64(const Src64* src, int N, const PairTile& t, double* 
   }
 }
 
-void launch_pair_f32c(const Src32* src, int N, const PairTile& t, double* part, hipStream_t st, const int* live) {
+void launch_pair_f32c(const Src32* src, int N, const PairTile& t, double* part, hipStream_t st, const int* live,
+                      bool pow2w) {
   if (N <= 0) return;
-  if (t.sym) return launch_pair_sym_f32c(src, N, t.sym, false, nullptr, part, st, live);
+  if (t.sym) return launch_pair_sym_f32c(src, N, t.sym, false, nullptr, part, st, live, pow2w);
   const int chunk = (N + t.slices - 1) / t.slices;
   const dim3 grid(grid_for(N, t)), block(kThreads);
   NearOut g{};
@@ -381,9 +382,9 @@ void launch_pair_f32c(const Src32* src, int N, const PairTile& t, double* part, 
 }
 
 void launch_pair_metric_f32c(const Src32* src, int N, const PairTile& t, double* part, hipStream_t st,
-                             const NearOut* near) {
+                             const NearOut* near, bool pow2w) {
   if (N <= 0) return;
-  if (t.sym) return launch_pair_sym_f32c(src, N, t.sym, true, near, part, st);
+  if (t.sym) return launch_pair_sym_f32c(src, N, t.sym, true, near, part, st, nullptr, pow2w && !near);
   const int chunk = (N + t.slices - 1) / t.slices;
   const dim3 grid(grid_for(N, t)), block(kThreads);
   if (near) {

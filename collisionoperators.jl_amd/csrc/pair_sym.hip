@@ -110,8 +110,9 @@ struct SymLds64 {
 // 2^k, so the pair's two scale factors w_a c / |d|^3 and w_g c / |d|^3 are one,
 // and w multiplies G exactly.  The off-diagonal blocks then take G' = w G and
 // c' = w c (exact scaling: the same bits as w * (y^3 c)), and drop both weight
-// products per pair.  A marker outside the box (w_masked = 0) sits at the
-// common far point (kFar, kFar) with G' = 0: its pairs with markers in the box
+// products per pair.  A marker outside the box (w_masked = 0), and a padding
+// slot past N, sits at the common far point (kFar, kFar) with G' = 0: its
+// pairs with markers in the box
 // give |d|^-3 = 0 exactly (underflow), its pairs with other far markers |d| = 0
 // (the guard), so it adds nothing on either side, as the mask does.
 constexpr double kFar = 1e150;
@@ -205,12 +206,12 @@ __device__ __forceinline__ void sym_block_f64(const Src64* __restrict__ src, int
       y2 = s.g2;
       w = s.w;
       in = s.in_domain != 0.0;
-      if constexpr (POW2W) {
-        x1 = in ? x1 : kFar;
-        x2 = in ? x2 : kFar;
-        y1 = in ? w * y1 : 0.0;
-        y2 = in ? w * y2 : 0.0;
-      }
+    }
+    if constexpr (POW2W) {  // padding past N too: the weight no longer masks it
+      x1 = in ? x1 : kFar;
+      x2 = in ? x2 : kFar;
+      y1 = in ? w * y1 : 0.0;
+      y2 = in ? w * y2 : 0.0;
     }
     Tile& t = L[e >> 6];
     const int j = e & 63;
@@ -353,10 +354,15 @@ struct SymLds32 {
   int Lin[NEAR ? 128 * R : 1];
 };
 
-template <int R, bool METRIC, bool NEAR>
+// POW2W: the equal power-of-two weight form of sym_block_f64, in FP32 (2^k is
+// exact there too); with the metric it keeps one weight product, w y^3 for A.
+constexpr float kFar32 = 1e18f;
+
+template <int R, bool METRIC, bool NEAR, bool POW2W = false>
 __device__ __forceinline__ void sym_block_f32c(const Src32* __restrict__ src, int N, int a, int b,
                                                double* __restrict__ part, SymLds32<R, METRIC, NEAR>& S,
                                                const NearOut& near) {
+  static_assert(!(POW2W && NEAR), "the equal-weight form has no near-pair search");
   constexpr int E = 128 * R, NTL = 2 * R, C = METRIC ? 5 : 2;
   auto& L = S.L;
   auto& Lh = S.Lh;
@@ -428,6 +434,15 @@ __device__ __forceinline__ void sym_block_f32c(const Src32* __restrict__ src, in
     }
     return;
   }
+  if constexpr (POW2W) {
+#pragma unroll
+    for (int q = 0; q < 2; ++q) {
+      gv1[q] = gin[q] ? gv1[q] : kFar32;
+      gv2[q] = gin[q] ? gv2[q] : kFar32;
+      gg1[q] = gin[q] ? gw[q] * gg1[q] : 0.0f;
+      gg2[q] = gin[q] ? gw[q] * gg2[q] : 0.0f;
+    }
+  }
   for (int e = threadIdx.x; e < E; e += 64 * R) {
     const int m = b * E + e;
     float x1 = 0.0f, x2 = 0.0f, y1 = 0.0f, y2 = 0.0f, w = 0.0f;
@@ -440,6 +455,12 @@ __device__ __forceinline__ void sym_block_f32c(const Src32* __restrict__ src, in
       y2 = s.g2;
       w = s.w;
       in = s.in_domain != 0.0f;
+    }
+    if constexpr (POW2W) {  // padding past N too: the weight no longer masks it
+      x1 = in ? x1 : kFar32;
+      x2 = in ? x2 : kFar32;
+      y1 = in ? w * y1 : 0.0f;
+      y2 = in ? w * y2 : 0.0f;
     }
     L[0][e] = x1;
     L[1][e] = x2;
@@ -470,7 +491,13 @@ __device__ __forceinline__ void sym_block_f32c(const Src32* __restrict__ src, in
           const float y = __builtin_amdgcn_rsqf(guard_q32(qd));
           const float y3 = (y * y) * y;
           const float c = __builtin_fmaf(ag1 - gg1[q], d2, -((ag2 - gg2[q]) * d1));
-          if constexpr (!METRIC) {
+          if constexpr (POW2W && !METRIC) {  // c is w c: one factor serves both sides
+            const float pc = y3 * c;
+            bs[q][0] = __builtin_fmaf(pc, d2, bs[q][0]);
+            bs[q][1] = __builtin_fmaf(-pc, d1, bs[q][1]);
+            ab[0] = __builtin_fmaf(-pc, d2, ab[0]);
+            ab[1] = __builtin_fmaf(pc, d1, ab[1]);
+          } else if constexpr (!METRIC) {
             const float pc = y3 * c;
             const float kg = aw * pc, ka = gw[q] * pc;
             bs[q][0] = __builtin_fmaf(kg, d2, bs[q][0]);
@@ -478,8 +505,9 @@ __device__ __forceinline__ void sym_block_f32c(const Src32* __restrict__ src, in
             ab[0] = __builtin_fmaf(-ka, d2, ab[0]);
             ab[1] = __builtin_fmaf(ka, d1, ab[1]);
           } else {
-            const float sg = aw * y3, sa = gw[q] * y3;
-            const float kg = sg * c, ka = sa * c;
+            // POW2W: sa = sg = w y^3, and kg = ka = y^3 (w c), the same bits as (w y^3) c
+            const float sg = aw * y3, sa = POW2W ? sg : gw[q] * y3;
+            const float kg = POW2W ? y3 * c : sg * c, ka = POW2W ? kg : sa * c;
             bs[q][0] = __builtin_fmaf(kg, d2, bs[q][0]);
             bs[q][1] = __builtin_fmaf(-kg, d1, bs[q][1]);
             ab[0] = __builtin_fmaf(-ka, d2, ab[0]);
@@ -540,14 +568,14 @@ __device__ __forceinline__ void sym_block_f32c(const Src32* __restrict__ src, in
   }
 }
 
-template <int R, bool METRIC, bool NEAR>
+template <int R, bool METRIC, bool NEAR, bool POW2W = false>
 __global__ __launch_bounds__(64 * R) void k_pair_sym_f32c(const Src32* __restrict__ src, int N, long long n_off,
                                                           double* __restrict__ part, NearOut near) {
   __shared__ SymLds32<R, METRIC, NEAR> S;
   if (near.live && *near.live == 0) return;
   int a, b;
   block_rows(blockIdx.x, n_off, a, b);
-  sym_block_f32c<R, METRIC, NEAR>(src, N, a, b, part, S, near);
+  sym_block_f32c<R, METRIC, NEAR, POW2W>(src, N, a, b, part, S, near);
 }
 
 // Batched ensemble: a grid-stride walk over the blocks (run, a, b) of every
@@ -600,13 +628,17 @@ void sym_f64(const Src64* src, int N, bool metric, const NearOut* near, double* 
 
 template <int R>
 void sym_f32c(const Src32* src, int N, bool metric, const NearOut* near, double* part, hipStream_t st,
-             const int* live) {
+             const int* live, bool pow2w) {
   const long long S = sym_rows(N, R), n_off = S * (S - 1) / 2;
   const dim3 grid((unsigned)(n_off + S)), block(64 * R);
   NearOut nz{};
   nz.live = live;
-  if (!metric)
+  if (!metric && pow2w)
+    hipLaunchKernelGGL((k_pair_sym_f32c<R, false, false, true>), grid, block, 0, st, src, N, n_off, part, nz);
+  else if (!metric)
     hipLaunchKernelGGL((k_pair_sym_f32c<R, false, false>), grid, block, 0, st, src, N, n_off, part, nz);
+  else if (!near && pow2w)
+    hipLaunchKernelGGL((k_pair_sym_f32c<R, true, false, true>), grid, block, 0, st, src, N, n_off, part, nz);
   else if (!near)
     hipLaunchKernelGGL((k_pair_sym_f32c<R, true, false>), grid, block, 0, st, src, N, n_off, part, nz);
   else
@@ -647,12 +679,12 @@ void launch_pair_sym_b(const EnsView& E, const EnsTiles& T, const Src64* s64, co
 }
 
 void launch_pair_sym_f32c(const Src32* src, int N, int R, bool metric, const NearOut* near, double* part,
-                          hipStream_t st, const int* live) {
+                          hipStream_t st, const int* live, bool pow2w) {
   if (N <= 0) return;
   switch (R) {
-    case 2: sym_f32c<2>(src, N, metric, near, part, st, live); break;
-    case 4: sym_f32c<4>(src, N, metric, near, part, st, live); break;
-    default: sym_f32c<8>(src, N, metric, near, part, st, live); break;
+    case 2: sym_f32c<2>(src, N, metric, near, part, st, live, pow2w); break;
+    case 4: sym_f32c<4>(src, N, metric, near, part, st, live, pow2w); break;
+    default: sym_f32c<8>(src, N, metric, near, part, st, live, pow2w); break;
   }
 }
 

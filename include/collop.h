@@ -401,11 +401,13 @@ int collop_set_fused_maps(collop_ctx* ctx, int on);
  * decides).  Both compute the same bits.  The host-side counterpart is
  * step_anderson! (solver.jl:300-423), whose least squares is solver.jl:396-412. */
 int collop_set_device_solve(collop_ctx* ctx, int on);
-/* The equal-weight form of the plain FP64 symmetric pair sum in the
- * device-resident step (on = 1, the default): when every marker of the state
- * carries the same weight w = 2^k, w scales G exactly and the two weight
- * products per pair drop out; off (0) runs the general form.  Both compute the
- * same bits.  No counterpart in the reference (its weights enter every term of
+/* The equal-weight form of the symmetric pair sums (on = 1, the default): when
+ * every marker carries the same weight w = 2^k, 2^-60 <= w <= 2^60 (the state's
+ * markers in the step, the call's markers in the collision hooks), w scales G
+ * exactly and the weight products per pair drop out (FP64: the plain sum;
+ * FP32-compensated: the plain sum and the metric sum without the near-pair
+ * search); off (0) runs the general form.  Both compute the same bits.  No
+ * counterpart in the reference (its weights enter every term of
  * functions.jl:230-231). */
 int collop_set_weight_path(collop_ctx* ctx, int on);
 /* The form (mode), waves per block (tiles, 0 for the ordered sum) and partial

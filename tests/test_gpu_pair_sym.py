@@ -188,15 +188,60 @@ def test_pair_mode_choice(sym_ws):
     collop.set_pair_tile(sym_ws, 1, 0, 1)
 
 
-@pytest.mark.parametrize("N,precision", [(4096, "f64"), (4096, "f32c"), (65536, "f32c")])
+@pytest.mark.parametrize("tiles", [2, 4, 8])
+@pytest.mark.parametrize("N", [1000, 5003])
+def test_equal_weight_hooks_are_bitwise_the_general_form(sym_ws, tiles, N):
+    """The collision hooks with every weight 2^-12 (5 % of the markers outside
+    the box): the equal-weight forms (plain FP64, plain and metric
+    FP32-compensated) give the general forms' bits, and stay within the
+    oracle's bars.  Then jittered weights after a state of equal weights: the
+    hooks judge the weights of their own call, not the state's."""
+    v, _, bp, m, _ = chain("gaussian", N, 16, seed=N)
+    rng = np.random.default_rng(N)
+    out = rng.choice(N, N // 20, replace=False)
+    v[out] *= 6.5 / np.max(np.abs(v[out]), axis=1, keepdims=True)
+    w = np.full(N, 2.0 ** -12)
+    G = m.compute_G(v, m.mass_solve(m.compute_r(m.l2_project(v, w))))
+    collop.set_pair_mode(sym_ws, True, tiles)
+    res = {}
+    for on in (True, False):
+        collop.set_weight_path(sym_ws, on)
+        res[on] = all_four(sym_ws, v, w, G)
+    collop.set_weight_path(sym_ws, True)
+    for a, b in zip(res[True], res[False]):
+        np.testing.assert_array_equal(a, b)
+    F64, F32, MF, MA, MF32, MA32 = res[True]
+    ref = m.collision(v, w, G)
+    refF, refA = O.collision_metric(m.box, v, w, G)
+    print(f"N={N} R={tiles}: vs oracle {relerr(F64, ref):.1e} {relerr(F32, ref):.1e} "
+          f"{relerr(MF32, refF):.1e} {relerr(MA32, refA):.1e}")
+    assert relerr(F64, ref) <= 1e-12 and relerr(F32, ref) <= 2e-6
+    assert relerr(MF32, refF) <= 2e-6 and relerr(MA32, refA) <= 1e-4
+    sim = collop.Simulation(sym_ws, v, w)  # a state of equal power-of-two weights
+    wj = w * (1.0 + 0.3 * rng.random(N))
+    Gj = m.compute_G(v, m.mass_solve(m.compute_r(m.l2_project(v, wj))))
+    jit = all_four(sym_ws, v, wj, Gj)
+    collop.set_weight_path(sym_ws, False)
+    gen = all_four(sym_ws, v, wj, Gj)
+    collop.set_weight_path(sym_ws, True)
+    del sim
+    for a, b in zip(jit, gen):
+        np.testing.assert_array_equal(a, b)
+    assert relerr(jit[0], m.collision(v, wj, Gj)) <= 1e-12
+
+
+@pytest.mark.parametrize("N,precision", [(4096, "f64"), (4096, "f32c"), (65536, "f32c"), (5003, "f64"),
+                                         (5003, "f32c")])
 def test_equal_weight_form_is_bitwise_the_general_form(gpu_ctx, N, precision):
-    """The equal-weight form of the FP64 symmetric sum (every marker w = 1/N =
-    2^-k) against the general form on the same steps: the same bits, with 3 %
-    of the markers outside the box (their pairs vanish through the far point
-    instead of the mask).  Two steps of the chosen path's solver, FP64 outer and
-    Anderson maps included."""
+    """The equal-weight form of the symmetric sums (every marker w = 2^-k: 1/N,
+    or 2^-12 at N = 5003, whose last super-row is partial) against the general
+    form on the same steps: the same bits, with 3 % of the markers outside the
+    box (their pairs vanish through the far point instead of the mask).  Two
+    steps of the chosen path's solver, FP64 outer and Anderson maps included."""
     p = collop.chosen_config("bimodal_n65536", N_PARTICLES=N)
     v, w = collop.initial_markers(p)
+    if np.frexp(w[0])[0] != 0.5:
+        w = np.full(N, 2.0 ** -12)
     rng = np.random.default_rng(5)
     out = rng.choice(N, N * 3 // 100, replace=False)
     v[out] *= 6.5 / np.max(np.abs(v[out]), axis=1, keepdims=True)  # pushed onto |v|_inf = 6.5
