@@ -110,11 +110,11 @@ struct SymLds64 {
 // 2^k, so the pair's two scale factors w_a c / |d|^3 and w_g c / |d|^3 are one,
 // and w multiplies G exactly.  The off-diagonal blocks then take G' = w G and
 // c' = w c (exact scaling: the same bits as w * (y^3 c)), and drop both weight
-// products per pair.  A marker outside the box (w_masked = 0), and a padding
-// slot past N, sits at the common far point (kFar, kFar) with G' = 0: its
-// pairs with markers in the box
-// give |d|^-3 = 0 exactly (underflow), its pairs with other far markers |d| = 0
-// (the guard), so it adds nothing on either side, as the mask does.
+// products per pair.  A target outside the box (w_masked = 0) sits at the far
+// point (kFar, kFar), a source outside the box or a padding slot past N at
+// (-kFar, -kFar), both with G' = 0: every pair with a far marker gives
+// |d|^-3 = 0 exactly (underflow) and c' = 0, so it adds nothing on either
+// side, as the mask does, and no two far markers coincide.
 constexpr double kFar = 1e150;
 
 // Block (a, b) of the symmetric FP64 sum over src[0, N) into part[S][C][N].
@@ -208,8 +208,8 @@ __device__ __forceinline__ void sym_block_f64(const Src64* __restrict__ src, int
       in = s.in_domain != 0.0;
     }
     if constexpr (POW2W) {  // padding past N too: the weight no longer masks it
-      x1 = in ? x1 : kFar;
-      x2 = in ? x2 : kFar;
+      x1 = in ? x1 : -kFar;
+      x2 = in ? x2 : -kFar;
       y1 = in ? w * y1 : 0.0;
       y2 = in ? w * y2 : 0.0;
     }
@@ -223,6 +223,13 @@ __device__ __forceinline__ void sym_block_f64(const Src64* __restrict__ src, int
     for (int c = 0; c < C; ++c) Lacc[c][e] = 0.0;
   }
   __syncthreads();
+  // GUARD = false: the pairs run without the coincidence guard while each lane
+  // keeps the smallest high word of |d|^2; the caller reruns the block guarded
+  // if any pair came near the threshold (the same bits: the guard leaves every
+  // |d|^2 >= 1e-24 as it is).
+  int qhi_min = 0x7FFFFFFF;
+  auto tiles = [&](auto guard_tag) {
+  constexpr bool GUARD = decltype(guard_tag)::value;
   for (int p = 0; p < NTL; ++p) {
     const int tile = (2 * r + p) % NTL;
     const int base = tile * 64;
@@ -247,7 +254,13 @@ __device__ __forceinline__ void sym_block_f64(const Src64* __restrict__ src, int
         const double d2 = gv2[q] - av2;
         const double u22 = d1 * d1;
         const double qd = __builtin_fma(d2, d2, u22);
-        const double y3 = rsqrt3_f64(guard_q64(qd));
+        double y3;
+        if constexpr (GUARD) {
+          y3 = rsqrt3_f64(guard_q64(qd));
+        } else {
+          qhi_min = min(qhi_min, __double2hiint(qd));
+          y3 = rsqrt3_f64(qd);
+        }
         const double c = __builtin_fma(ag1 - gg1[q], d2, -((ag2 - gg2[q]) * d1));
         if constexpr (POW2W) {  // c is w c: one factor serves both sides
           const double pc = y3 * c;
@@ -311,6 +324,22 @@ __device__ __forceinline__ void sym_block_f64(const Src64* __restrict__ src, int
     }
     __syncthreads();
   }
+  };
+  if constexpr (METRIC || NEAR) {
+    tiles(std::true_type{});
+  } else {
+    tiles(std::false_type{});
+    // high word of 1e-24 (0x3AF357C2 299A88EA): every |d|^2 < 1e-24 has a high word <= it
+    if (__syncthreads_or(qhi_min <= 0x3AF357C2)) {
+#pragma unroll
+      for (int q = 0; q < 2; ++q) f1[q] = f2[q] = 0.0;
+      for (int e = threadIdx.x; e < E; e += 64 * R)
+#pragma unroll
+        for (int c = 0; c < C; ++c) Lacc[c][e] = 0.0;
+      __syncthreads();
+      tiles(std::true_type{});
+    }
+  }
   // targets of super-row a -> slot b; sources of super-row b -> slot a
 #pragma unroll
   for (int q = 0; q < 2; ++q) {
@@ -355,7 +384,8 @@ struct SymLds32 {
 };
 
 // POW2W: the equal power-of-two weight form of sym_block_f64, in FP32 (2^k is
-// exact there too); with the metric it keeps one weight product, w y^3 for A.
+// exact there too; far points at +-kFar32); with the metric it keeps one weight
+// product, w y^3 for A.
 constexpr float kFar32 = 1e18f;
 
 template <int R, bool METRIC, bool NEAR, bool POW2W = false>
@@ -457,8 +487,8 @@ __device__ __forceinline__ void sym_block_f32c(const Src32* __restrict__ src, in
       in = s.in_domain != 0.0f;
     }
     if constexpr (POW2W) {  // padding past N too: the weight no longer masks it
-      x1 = in ? x1 : kFar32;
-      x2 = in ? x2 : kFar32;
+      x1 = in ? x1 : -kFar32;
+      x2 = in ? x2 : -kFar32;
       y1 = in ? w * y1 : 0.0f;
       y2 = in ? w * y2 : 0.0f;
     }
@@ -472,6 +502,11 @@ __device__ __forceinline__ void sym_block_f32c(const Src32* __restrict__ src, in
     for (int c = 0; c < C; ++c) Lh[c][e] = Ll[c][e] = 0.0f;
   }
   __syncthreads();
+  // GUARD = false: unguarded pairs with the smallest |d|^2 kept per lane (its
+  // bits: |d|^2 >= 0), rerun guarded if any came below 1e-12, as in the FP64 block
+  int qbits_min = 0x7FFFFFFF;
+  auto tiles = [&](auto guard_tag) {
+  constexpr bool GUARD = decltype(guard_tag)::value;
   for (int p = 0; p < NTL; ++p) {
     const int base = ((2 * r + p) % NTL) * 64;
     for (int h = 0; h < 2; ++h) {
@@ -488,7 +523,13 @@ __device__ __forceinline__ void sym_block_f32c(const Src32* __restrict__ src, in
           const float d2 = gv2[q] - av2;
           const float u22 = d1 * d1;
           const float qd = __builtin_fmaf(d2, d2, u22);
-          const float y = __builtin_amdgcn_rsqf(guard_q32(qd));
+          float y;
+          if constexpr (GUARD) {
+            y = __builtin_amdgcn_rsqf(guard_q32(qd));
+          } else {
+            qbits_min = min(qbits_min, __float_as_int(qd));
+            y = __builtin_amdgcn_rsqf(qd);
+          }
           const float y3 = (y * y) * y;
           const float c = __builtin_fmaf(ag1 - gg1[q], d2, -((ag2 - gg2[q]) * d1));
           if constexpr (POW2W && !METRIC) {  // c is w c: one factor serves both sides
@@ -550,6 +591,23 @@ __device__ __forceinline__ void sym_block_f32c(const Src32* __restrict__ src, in
         }
     }
     __syncthreads();
+  }
+  };
+  if constexpr (NEAR) {
+    tiles(std::true_type{});
+  } else {
+    tiles(std::false_type{});
+    if (__syncthreads_or(qbits_min < 0x2B8CBCCC)) {  // bits of 1e-12f
+#pragma unroll
+      for (int q = 0; q < 2; ++q)
+#pragma unroll
+        for (int c = 0; c < C; ++c) hs[q][c] = ls[q][c] = 0.0f;
+      for (int e = threadIdx.x; e < E; e += 64 * R)
+#pragma unroll
+        for (int c = 0; c < C; ++c) Lh[c][e] = Ll[c][e] = 0.0f;
+      __syncthreads();
+      tiles(std::true_type{});
+    }
   }
 #pragma unroll
   for (int q = 0; q < 2; ++q) {

@@ -192,14 +192,17 @@ def test_pair_mode_choice(sym_ws):
 @pytest.mark.parametrize("N", [1000, 5003])
 def test_equal_weight_hooks_are_bitwise_the_general_form(sym_ws, tiles, N):
     """The collision hooks with every weight 2^-12 (5 % of the markers outside
-    the box): the equal-weight forms (plain FP64, plain and metric
-    FP32-compensated) give the general forms' bits, and stay within the
+    the box, coincident pairs in different super-rows, which send their blocks
+    through the guarded rerun): the equal-weight forms (plain FP64, plain and
+    metric FP32-compensated) give the general forms' bits, and stay within the
     oracle's bars.  Then jittered weights after a state of equal weights: the
     hooks judge the weights of their own call, not the state's."""
     v, _, bp, m, _ = chain("gaussian", N, 16, seed=N)
     rng = np.random.default_rng(N)
     out = rng.choice(N, N // 20, replace=False)
     v[out] *= 6.5 / np.max(np.abs(v[out]), axis=1, keepdims=True)
+    for i, j in ((5, 600), (7, 8), (40, N - 3)):
+        v[i] = v[j]
     w = np.full(N, 2.0 ** -12)
     G = m.compute_G(v, m.mass_solve(m.compute_r(m.l2_project(v, w))))
     collop.set_pair_mode(sym_ws, True, tiles)
