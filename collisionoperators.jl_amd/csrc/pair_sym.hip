@@ -34,16 +34,22 @@
 // against 23 per ordered pair in k_pair_f64.
 #include "collop_launch.hpp"
 
+#include <type_traits>
+
 namespace collop {
 namespace {
 
-// Block t -> super-rows (a, b): off-diagonal blocks (a < b) first, ordered by
-// b then a; the diagonal blocks last.
+// Block t -> super-rows (a, b): the S diagonal blocks first (their ordered sums
+// cost more per pair, and the grid's tail then holds only off-diagonal blocks),
+// then the off-diagonal blocks (a < b), ordered by b then a.  The grid has
+// n_off + S blocks.
 __device__ __forceinline__ void block_rows(long long t, long long n_off, int& a, int& b) {
-  if (t >= n_off) {
-    a = b = (int)(t - n_off);
+  const long long S = (long long)gridDim.x - n_off;
+  if (t < S) {
+    a = b = (int)t;
     return;
   }
+  t -= S;
   long long bb = (long long)((1.0 + sqrt(1.0 + 8.0 * (double)t)) * 0.5);
   while (bb * (bb - 1) / 2 > t) --bb;
   while ((bb + 1) * bb / 2 <= t) ++bb;
@@ -94,17 +100,29 @@ __device__ __forceinline__ void near_push(const NearOut& near, int g, int a) {
 // source (i + k) mod 64 sits at offset i + k and the fully unrolled step loop
 // reads it with two 16-byte and one 8-byte LDS load at immediate offsets from
 // one address -- and their accumulators.
-template <int R, bool METRIC, bool NEAR>
-struct SymLds64 {
-  struct Tile {
+// With equal weights (POW2W) the tiles carry no weights: 40 KB at R = 4, four
+// blocks per CU.
+template <int R, bool METRIC, bool POW2W>
+struct SymLds64Core {
+  struct TileW {
     double2 v[128];
     double2 g[128];
     double w[128];
   };
+  struct TileP {
+    double2 v[128];
+    double2 g[128];
+  };
+  using Tile = std::conditional_t<POW2W, TileP, TileW>;
   Tile L[2 * R];
   double Lacc[METRIC ? 5 : 2][128 * R];
-  int Lin[NEAR ? 128 * R : 1];
 };
+template <int R, bool METRIC, bool NEAR, bool POW2W = false>
+struct SymLds64 : SymLds64Core<R, METRIC, POW2W> {
+  int Lin[128 * R];  // sources in the box (near-pair search)
+};
+template <int R, bool METRIC, bool POW2W>
+struct SymLds64<R, METRIC, false, POW2W> : SymLds64Core<R, METRIC, POW2W> {};
 
 // Equal power-of-two weights (POW2W): every marker carries the same weight w =
 // 2^k, so the pair's two scale factors w_a c / |d|^3 and w_g c / |d|^3 are one,
@@ -117,26 +135,28 @@ struct SymLds64 {
 // side, as the mask does, and no two far markers coincide.
 constexpr double kFar = 1e150;
 
-// Block (a, b) of the symmetric FP64 sum over src[0, N) into part[S][C][N].
+// Block (a, b) of the symmetric FP64 sum over src[0, N) into part[S][C][N]:
+// super-rows of E = 128 R markers, R waves, Q = 2 targets per lane.  (Four
+// targets per lane on R/2 waves halve the rotations per pair and measured the
+// same time.)
 template <int R, bool METRIC, bool NEAR, bool POW2W = false>
 __device__ __forceinline__ void sym_block_f64(const Src64* __restrict__ src, int N, int a, int b,
-                                              double* __restrict__ part, SymLds64<R, METRIC, NEAR>& S,
+                                              double* __restrict__ part, SymLds64<R, METRIC, NEAR, POW2W>& S,
                                               const NearOut& near) {
   static_assert(!(POW2W && (METRIC || NEAR)), "the equal-weight form is the plain pair sum only");
-  constexpr int E = 128 * R, NTL = 2 * R, C = METRIC ? 5 : 2;
-  using Tile = typename SymLds64<R, METRIC, NEAR>::Tile;
+  constexpr int Q = 2, E = 128 * R, NTL = 2 * R, NW = R, C = METRIC ? 5 : 2;
+  using Tile = typename SymLds64<R, METRIC, NEAR, POW2W>::Tile;
   auto& L = S.L;
   auto& Lacc = S.Lacc;
-  auto& Lin = S.Lin;
   const int lane = threadIdx.x & 63, r = threadIdx.x >> 6;
   const size_t cs = (size_t)C * N;
-  double gv1[2], gv2[2], gg1[2], gg2[2], gw[2];
-  double f1[2], f2[2], m11[2], m12[2], m22[2];
-  bool gin[2];
-  int gm[2];
+  double gv1[Q], gv2[Q], gg1[Q], gg2[Q], gw[Q];
+  double f1[Q], f2[Q], m11[Q], m12[Q], m22[Q];
+  bool gin[Q];
+  int gm[Q];
 #pragma unroll
-  for (int q = 0; q < 2; ++q) {
-    gm[q] = a * E + r * 128 + q * 64 + lane;
+  for (int q = 0; q < Q; ++q) {
+    gm[q] = a * E + r * 64 * Q + q * 64 + lane;
     gv1[q] = gv2[q] = gg1[q] = gg2[q] = gw[q] = 0.0;
     gin[q] = false;
     if (gm[q] < N) {
@@ -156,7 +176,7 @@ __device__ __forceinline__ void sym_block_f64(const Src64* __restrict__ src, int
     for (int s = s0; s < s1; ++s) {
       const double av1 = src[s].v1, av2 = src[s].v2, ag1 = src[s].g1, ag2 = src[s].g2, aw = src[s].w;
 #pragma unroll
-      for (int q = 0; q < 2; ++q) {
+      for (int q = 0; q < Q; ++q) {
         double d1, d2, t;
         pair_term_f64<1>(gv1[q], gv2[q], gg1[q], gg2[q], av1, av2, ag1, ag2, aw, f1[q], f2[q], d1, d2, t);
         if constexpr (METRIC) {
@@ -172,7 +192,7 @@ __device__ __forceinline__ void sym_block_f64(const Src64* __restrict__ src, int
       }
     }
 #pragma unroll
-    for (int q = 0; q < 2; ++q) {
+    for (int q = 0; q < Q; ++q) {
       if (gm[q] >= N) continue;
       double* p = part + (size_t)a * cs + gm[q];
       p[0] = gin[q] ? f1[q] : 0.0;
@@ -187,41 +207,46 @@ __device__ __forceinline__ void sym_block_f64(const Src64* __restrict__ src, int
   }
   if constexpr (POW2W) {
 #pragma unroll
-    for (int q = 0; q < 2; ++q) {
+    for (int q = 0; q < Q; ++q) {
       gv1[q] = gin[q] ? gv1[q] : kFar;
       gv2[q] = gin[q] ? gv2[q] : kFar;
       gg1[q] = gin[q] ? gw[q] * gg1[q] : 0.0;
       gg2[q] = gin[q] ? gw[q] * gg2[q] : 0.0;
     }
   }
-  for (int e = threadIdx.x; e < E; e += 64 * R) {
-    const int m = b * E + e;
-    double x1 = 0.0, x2 = 0.0, y1 = 0.0, y2 = 0.0, w = 0.0;
-    int in = 0;
-    if (m < N) {
-      const Src64 s = src[m];
-      x1 = s.v1;
-      x2 = s.v2;
-      y1 = s.g1;
-      y2 = s.g2;
-      w = s.w;
-      in = s.in_domain != 0.0;
-    }
-    if constexpr (POW2W) {  // padding past N too: the weight no longer masks it
-      x1 = in ? x1 : -kFar;
-      x2 = in ? x2 : -kFar;
-      y1 = in ? w * y1 : 0.0;
-      y2 = in ? w * y2 : 0.0;
-    }
-    Tile& t = L[e >> 6];
-    const int j = e & 63;
-    t.v[j] = t.v[j + 64] = make_double2(x1, x2);
-    t.g[j] = t.g[j + 64] = make_double2(y1, y2);
-    t.w[j] = t.w[j + 64] = w;
-    if constexpr (NEAR) Lin[e] = in;
+  // sources of super-row b into the tiles, their accumulators cleared
+  auto stage = [&]() {
+    for (int e = threadIdx.x; e < E; e += 64 * NW) {
+      const int m = b * E + e;
+      double x1 = 0.0, x2 = 0.0, y1 = 0.0, y2 = 0.0, w = 0.0;
+      int in = 0;
+      if (m < N) {
+        const Src64 s = src[m];
+        x1 = s.v1;
+        x2 = s.v2;
+        y1 = s.g1;
+        y2 = s.g2;
+        w = s.w;
+        in = s.in_domain != 0.0;
+      }
+      Tile& t = L[e >> 6];
+      const int j = e & 63;
+      if constexpr (POW2W) {  // padding past N too: the weight no longer masks it
+        x1 = in ? x1 : -kFar;
+        x2 = in ? x2 : -kFar;
+        y1 = in ? w * y1 : 0.0;
+        y2 = in ? w * y2 : 0.0;
+      } else {
+        t.w[j] = t.w[j + 64] = w;
+      }
+      t.v[j] = t.v[j + 64] = make_double2(x1, x2);
+      t.g[j] = t.g[j + 64] = make_double2(y1, y2);
+      if constexpr (NEAR) S.Lin[e] = in;
 #pragma unroll
-    for (int c = 0; c < C; ++c) Lacc[c][e] = 0.0;
-  }
+      for (int c = 0; c < C; ++c) Lacc[c][e] = 0.0;
+    }
+  };
+  stage();
   __syncthreads();
   // GUARD = false: the pairs run without the coincidence guard while each lane
   // keeps the smallest high word of |d|^2; the caller reruns the block guarded
@@ -231,11 +256,12 @@ __device__ __forceinline__ void sym_block_f64(const Src64* __restrict__ src, int
   auto tiles = [&](auto guard_tag) {
   constexpr bool GUARD = decltype(guard_tag)::value;
   for (int p = 0; p < NTL; ++p) {
-    const int tile = (2 * r + p) % NTL;
+    const int tile = (Q * r + p) % NTL;  // the waves hold distinct tiles at every phase
     const int base = tile * 64;
     const double2* pv = &L[tile].v[lane];
     const double2* pg = &L[tile].g[lane];
-    const double* pw = &L[tile].w[lane];
+    const double* pw = nullptr;
+    if constexpr (!POW2W) pw = &L[tile].w[lane];
     // step k: lane i meets source base + (i + k) mod 64 and holds its accumulator
     double b1 = Lacc[0][base + lane], b2 = Lacc[1][base + lane];
     double n11 = 0.0, n12 = 0.0, n22 = 0.0;
@@ -247,9 +273,11 @@ __device__ __forceinline__ void sym_block_f64(const Src64* __restrict__ src, int
     auto step = [&](int k) {
       const int ia = base + ((lane + k) & 63);  // source index (near pairs only)
       const double2 av = pv[k], ag = pg[k];
-      const double av1 = av.x, av2 = av.y, ag1 = ag.x, ag2 = ag.y, aw = pw[k];
+      const double av1 = av.x, av2 = av.y, ag1 = ag.x, ag2 = ag.y;
+      double aw = 0.0;
+      if constexpr (!POW2W) aw = pw[k];
 #pragma unroll
-      for (int q = 0; q < 2; ++q) {
+      for (int q = 0; q < Q; ++q) {
         const double d1 = gv1[q] - av1;
         const double d2 = gv2[q] - av2;
         const double u22 = d1 * d1;
@@ -295,7 +323,7 @@ __device__ __forceinline__ void sym_block_f64(const Src64* __restrict__ src, int
           if (qd < near.qmax && qd >= 1e-24) {
             const int ma = b * E + ia;
             if (gin[q] && near.th2 * qd < aw * aw) near_push(near, gm[q], ma);
-            if (Lin[ia] && near.th2 * qd < gw[q] * gw[q]) near_push(near, ma, gm[q]);
+            if (S.Lin[ia] && near.th2 * qd < gw[q] * gw[q]) near_push(near, ma, gm[q]);
           }
         }
       }
@@ -329,20 +357,28 @@ __device__ __forceinline__ void sym_block_f64(const Src64* __restrict__ src, int
     tiles(std::true_type{});
   } else {
     tiles(std::false_type{});
-    // high word of 1e-24 (0x3AF357C2 299A88EA): every |d|^2 < 1e-24 has a high word <= it
-    if (__syncthreads_or(qhi_min <= 0x3AF357C2)) {
+    // The vote: one word per wave in the tiles, which no wave reads after the
+    // last phase's barrier.  High word of 1e-24 (0x3AF357C2 299A88EA): every
+    // |d|^2 < 1e-24 has a high word <= it.
+    int* vote = reinterpret_cast<int*>(&L[0]);
+    const bool near_pair = __builtin_amdgcn_ballot_w64(qhi_min <= 0x3AF357C2) != 0;
+    if (lane == 0) vote[r] = near_pair;
+    __syncthreads();
+    bool redo = false;
 #pragma unroll
-      for (int q = 0; q < 2; ++q) f1[q] = f2[q] = 0.0;
-      for (int e = threadIdx.x; e < E; e += 64 * R)
+    for (int i = 0; i < NW; ++i) redo = redo || vote[i] != 0;
+    if (redo) {  // block-uniform: every thread read the same words
+      __syncthreads();
 #pragma unroll
-        for (int c = 0; c < C; ++c) Lacc[c][e] = 0.0;
+      for (int q = 0; q < Q; ++q) f1[q] = f2[q] = 0.0;
+      stage();
       __syncthreads();
       tiles(std::true_type{});
     }
   }
   // targets of super-row a -> slot b; sources of super-row b -> slot a
 #pragma unroll
-  for (int q = 0; q < 2; ++q) {
+  for (int q = 0; q < Q; ++q) {
     if (gm[q] >= N) continue;
     double* p = part + (size_t)b * cs + gm[q];
     p[0] = gin[q] ? f1[q] : 0.0;
@@ -353,7 +389,7 @@ __device__ __forceinline__ void sym_block_f64(const Src64* __restrict__ src, int
       p[4 * (size_t)N] = gin[q] ? m22[q] : 0.0;
     }
   }
-  for (int e = threadIdx.x; e < E; e += 64 * R) {
+  for (int e = threadIdx.x; e < E; e += 64 * NW) {
     const int m = b * E + e;
     if (m >= N) continue;
     const bool in = src[m].in_domain != 0.0;
@@ -366,7 +402,7 @@ __device__ __forceinline__ void sym_block_f64(const Src64* __restrict__ src, int
 template <int R, bool METRIC, bool NEAR, bool POW2W = false>
 __global__ __launch_bounds__(64 * R) void k_pair_sym_f64(const Src64* __restrict__ src, int N, long long n_off,
                                                          double* __restrict__ part, NearOut near) {
-  __shared__ SymLds64<R, METRIC, NEAR> S;
+  __shared__ SymLds64<R, METRIC, NEAR, POW2W> S;
   if (near.live && *near.live == 0) return;
   int a, b;
   block_rows(blockIdx.x, n_off, a, b);
