@@ -667,7 +667,9 @@ def main():
     t_both_64 = sum(r64["stats"][i]["seconds"] for i in both)
     per_step = {k: {"ms": [round(st["seconds"] * 1e3, 1) for st in r["stats"]],
                     "maps": [st["iters"] for st in r["stats"]],
-                    "exit": [st["exit_reason"] for st in r["stats"]]} for k, r in (("chosen", rch), ("plain_fp64", r64))}
+                    "exit": [st["exit_reason"] for st in r["stats"]],
+                    "handover": [st["handover"] for st in r["stats"]]}
+                for k, r in [("chosen", rch), ("plain_fp64", r64)] + [(k, r) for k, (_, r) in comps.items()]}
     converged_both = {"steps": [W + i for i in both],
                       "chosen_steps_per_s": round(len(both) / t_both_ch, 4) if both else None,
                       "plain_fp64_steps_per_s": round(len(both) / t_both_64, 4) if both else None,

@@ -87,6 +87,7 @@ class StepParams(C.Structure):
         ("cl_theta", C.c_double),
         ("cl_kmax", C.c_int32),
         ("dc_fallback", C.c_int32),
+        ("dc_sticky", C.c_int32),
     ]
 
 
@@ -109,6 +110,7 @@ class StepStats(C.Structure):
         ("cl_fail", C.c_int32),
         ("stat_syncs", C.c_int32),
         ("handover", C.c_int32),
+        ("dc_skipped", C.c_int32),
     ]
 
     def as_dict(self) -> dict:
@@ -118,7 +120,7 @@ class StepStats(C.Structure):
             "entropy": self.entropy, "energy": self.energy,
             "momentum": (self.momentum[0], self.momentum[1]), "seconds": self.seconds,
             "inner_maps": self.inner_maps, "n_clusters": self.n_clusters, "cl_fail": self.cl_fail,
-            "stat_syncs": self.stat_syncs, "handover": self.handover,
+            "stat_syncs": self.stat_syncs, "handover": self.handover, "dc_skipped": self.dc_skipped,
         }
 
 

@@ -69,6 +69,7 @@ class SimParameters:
     cl_theta: float = 0.0            # extension: stiff clusters max(w_i, w_j) / |d| > cl_theta solved exactly (0 = off)
     cl_kmax: int = 32                # largest stiff cluster (markers)
     dc_fallback: bool = False        # extension: defect correction hands a stiff step to the Anderson solve
+    dc_sticky: bool = False          # extension: after an early hand-over, the next steps start with Anderson
 
     def step_params(self, precision: int | None = None, warmstart: int | None = None,
                     solver: int | None = None) -> _lib.StepParams:
@@ -95,7 +96,7 @@ class SimParameters:
             collision_model=_lib.MODEL_LB if self.collision_model == "lb" else _lib.MODEL_LANDAU, nu=self.nu,
             dc_eta_f64=self.dc_eta_f64, nk_krylov_max=self.nk_krylov_max, nk_eta_max=self.nk_eta_max, nk_fd_h=self.nk_fd_h,
             nk_fd_rel=self.nk_fd_rel, cl_theta=self.cl_theta, cl_kmax=self.cl_kmax,
-            dc_fallback=int(self.dc_fallback))
+            dc_fallback=int(self.dc_fallback), dc_sticky=int(self.dc_sticky))
 
 
 def parse_overrides(p: SimParameters, args: list[str]) -> SimParameters:
@@ -164,8 +165,10 @@ ANDERSON_F32C = dict(gpu_fp32=True, solver="anderson")
 # The chosen path: the defect-correction path (DEFECT) with the hand-over: when
 # a frozen-metric inner solve ends unconverged, an outer iteration does not
 # reduce the residual by 0.4, or the outer iteration stagnates (stiff near
-# pairs), the step continues with the Anderson solve in FP64 maps.
-CHOSEN = dict(DEFECT, dc_fallback=True)
+# pairs), the step continues with the Anderson solve in FP64 maps.  After a
+# hand-over within the first 2 outer iterations, the next steps start with the
+# Anderson solve until one converges (dc_sticky).
+CHOSEN = dict(DEFECT, dc_fallback=True, dc_sticky=True)
 # Stiff-cluster elimination (an opt-in extension): pairs with
 # max(w_i, w_j) / |d| > CLUSTER_TAU / dt at the start of a step form clusters
 # whose rows every FP64 map solves exactly.  The threshold keeps dt w / |d|
@@ -208,7 +211,8 @@ def chosen_description(name: str, mlp: bool) -> str:
             "rounding floor, then FP64 outer maps, an outer Anderson window of 3 and exact near pairs; an inner solve "
             "that misses its tolerance in 60 maps, an outer iteration that does not reduce the residual by 0.4, or "
             "outer stagnation hands the step to the Anderson solve (step_anderson!, FP64 maps, control on the "
-            "device); "
+            "device); after a hand-over within 2 outer iterations the next steps run that Anderson solve from the "
+            "start until one converges; "
             "FP32c Euler predictor" + (" + learned MLP warm start" if mlp else "")
             + "; symmetric pair sum (each unordered pair once)")
 

@@ -1522,6 +1522,9 @@ int validate_params(collop_ctx* ctx, const collop_step_params* p) {
     return fail(ctx, COLLOP_ERR_INVALID,
                 "defect correction needs 0 <= dc_outer_m <= 8, dc_near_theta >= 0 and dc_metric_reuse in {0, 1, 2}");
   if (p->dc_fallback != 0 && p->dc_fallback != 1) return fail(ctx, COLLOP_ERR_INVALID, "dc_fallback must be 0 or 1");
+  if (p->dc_sticky != 0 && p->dc_sticky != 1) return fail(ctx, COLLOP_ERR_INVALID, "dc_sticky must be 0 or 1");
+  if (p->dc_sticky && !(p->solver == COLLOP_SOLVER_DEFECT && p->dc_fallback))
+    return fail(ctx, COLLOP_ERR_INVALID, "dc_sticky needs the defect-correction solver with dc_fallback = 1");
   if (p->collision_model != COLLOP_MODEL_LANDAU && p->collision_model != COLLOP_MODEL_LB)
     return fail(ctx, COLLOP_ERR_INVALID, "unknown collision_model");
   if (p->collision_model == COLLOP_MODEL_LB &&
@@ -1899,7 +1902,18 @@ int anderson_device(collop_ctx* ctx, const collop_step_params& p, AndersonRun& a
 
 // One implicit step on the device-resident state (main.jl:226-307 with
 // step_anderson!, solver.jl:300-423).
-int step_once(collop_ctx* ctx, const collop_step_params& p, collop_step_stats* out) {
+// A step of the hand-over memory (dc_sticky) runs step_anderson! from the
+// start with FP64 maps and predictor, as the hand-over continues a step.
+constexpr int kStickyOuter = 2;  // a hand-over within this many outer iterations arms the memory
+
+int step_once(collop_ctx* ctx, const collop_step_params& p_in, collop_step_stats* out) {
+  const bool skip_dc = p_in.solver == COLLOP_SOLVER_DEFECT && p_in.dc_sticky && ctx->dc_skip;
+  collop_step_params p = p_in;
+  if (skip_dc) {
+    p.solver = COLLOP_SOLVER_ANDERSON;
+    p.precision = COLLOP_PREC_F64;
+    p.cl_theta = 0.0;
+  }
   const auto t0 = std::chrono::steady_clock::now();
   const long syncs0 = ctx->stat_syncs;
   const int n = (int)ctx->N;
@@ -2069,8 +2083,11 @@ int step_once(collop_ctx* ctx, const collop_step_params& p, collop_step_stats* o
   if (ctx->ncl > 0) HIPC(hipMemcpyAsync(&cl_fail, ctx->b_cl_fail.p, sizeof(int), hipMemcpyDeviceToHost, ctx->stream));
   HIPC(hipStreamSynchronize(ctx->stream));
   prof_collect(ctx);
+  if (p_in.dc_sticky)  // the memory: armed by an early hand-over, cleared by a converged Anderson step
+    ctx->dc_skip = skip_dc ? an.exit_reason != 0 : handover && dc.iters <= kStickyOuter;
   if (out) {
     out->handover = handover ? dc.iters : 0;
+    out->dc_skipped = skip_dc ? 1 : 0;
     if (handover) {
       out->iters = dc.iters + an.k_done;
       out->iters_f32 = dc.iters_f32 + an.iters32;
@@ -2122,6 +2139,7 @@ int collop_state_set(collop_ctx* ctx, const double* v, const double* w, int64_t 
     same_w = same_w && w[i] == w[0];
   }
   ctx->w_pow2 = same_w && equal_pow2_weights(w, 1);
+  ctx->dc_skip = false;
   HIPC(hipSetDevice(ctx->device));
   if ((r = ensure_scratch(ctx, N))) return r;
   const size_t n2b = (size_t)2 * N * 8;

@@ -82,6 +82,7 @@ struct collop_ctx {
   int sym_tiles = 0;     // its waves per block (0: automatic)
   bool fused_maps = true;  // fused small-kernel maps where the split mass solves apply
   bool w_pow2 = false;     // the state's markers all carry one weight, a power of two
+  bool dc_skip = false;    // hand-over memory (dc_sticky): the next defect-correction step runs Anderson
   bool pow2w_path = true;  // the symmetric sums take their equal-weight form where the weights allow
   bool profile = false;
   std::vector<std::pair<hipEvent_t, hipEvent_t>> ev_pool;

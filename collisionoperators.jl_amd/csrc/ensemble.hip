@@ -415,6 +415,7 @@ int ens_step(collop_ctx* ctx, EnsembleState* E, const collop_step_params& p, col
       s.momentum[1] = sc[4];
       s.seconds = sec;
       s.handover = 0;
+      s.dc_skipped = 0;
     }
   }
   return COLLOP_OK;

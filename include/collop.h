@@ -265,6 +265,13 @@ typedef struct {
                                not reduce it by 0.4 or (d) the outer stagnation
                                test fires (both from the best outer iterate's
                                map); 0 = off (the reference's step_defect!)    */
+  int32_t dc_sticky;        /* DEFECT with dc_fallback, across the steps of a
+                               state: 1 = after a step that handed over within
+                               its first 2 outer iterations, the next steps run
+                               the Anderson solve from the start (FP64 maps and
+                               predictor, no stiff clusters) until one of them
+                               converges; 0 = every step starts with the defect
+                               correction.  No counterpart in the reference.   */
 } collop_step_params;
 
 typedef struct {
@@ -287,6 +294,8 @@ typedef struct {
   int32_t handover;         /* DEFECT with dc_fallback: outer maps before the
                                step continued with the Anderson solve (0 = the
                                defect correction finished the step)           */
+  int32_t dc_skipped;       /* DEFECT with dc_sticky: 1 = the step ran the
+                               Anderson solve from the start                   */
 } collop_step_stats;
 
 /* Upload markers (v: N x 2 column-major, w: N) and project them; keeps S0. */

@@ -267,6 +267,50 @@ def test_chosen_path_steps_bimodal_n65536(gpu_ctx):
     assert handed >= 1
 
 
+def test_hand_over_memory_runs_the_anderson_step(gpu_ctx):
+    """The hand-over memory (dc_sticky) on the headline workload, N = 65536,
+    dt 0.02, Euler predictor: every step follows the rule (a hand-over within 2
+    outer iterations arms the memory, a converged Anderson step clears it), and
+    the first step it skips the defect correction is bitwise the Anderson step
+    (FP64 maps and predictor) from the same resident state, reached without the
+    memory."""
+    p = collop.chosen_config("bimodal_n65536")
+    assert p.dc_sticky and p.dc_fallback
+    v, w = collop.initial_markers(p)
+    ws = collop.build_workspace(p, ctx=gpu_ctx)
+    prm = p.step_params(warmstart=collop.WARM_EULER)
+    sim = collop.Simulation(ws, v, w)
+    armed, first = False, None
+    for k in range(21):
+        s = sim.run(prm, 1)[0]
+        print(f"step {k}: maps {s['iters']} handover {s['handover']} skipped {s['dc_skipped']} "
+              f"exit {s['exit_reason']}", flush=True)
+        assert s["dc_skipped"] == int(armed)
+        if armed:
+            assert s["handover"] == 0 and s["inner_maps"] == 0 and s["iters_f32"] == 0
+            first = k if first is None else first
+        armed = s["exit_reason"] != 0 if armed else 0 < s["handover"] <= 2
+        if first is not None and k >= first + 1:
+            break
+    assert first is not None, "no early hand-over in 21 steps"
+    # the same steps without the memory, then the Anderson step
+    q = collop.chosen_config("bimodal_n65536", dc_sticky=False)
+    sim_b = collop.Simulation(ws, v, w)
+    sim_b.run(q.step_params(warmstart=collop.WARM_EULER), first)
+    pa = collop.chosen_config("bimodal_n65536", dc_sticky=False, dc_fallback=False, solver="anderson")
+    sb = sim_b.run(pa.step_params(precision=collop.PREC_F64, warmstart=collop.WARM_EULER), 1)[0]
+    xb = sim_b.markers()
+    sim = collop.Simulation(ws, v, w)
+    sa = sim.run(prm, first + 1)[-1]
+    print(f"step {first}: memory {sa['iters']} maps exit {sa['exit_reason']}, Anderson {sb['iters']} maps "
+          f"exit {sb['exit_reason']}")
+    assert sa["dc_skipped"] == 1
+    assert (sa["iters"], sa["exit_reason"], sa["residual"]) == (sb["iters"], sb["exit_reason"], sb["residual"])
+    np.testing.assert_array_equal(sim.markers(), xb)
+    with pytest.raises(collop.CollopError):
+        sim.run(collop.chosen_config("bimodal_n65536", dc_fallback=False).step_params(), 1)
+
+
 def test_defect_handover_by_the_contraction_test_matches_oracle(gpu_ctx):
     """The hand-over's outer contraction test against the oracle's: with a loose
     inner tolerance (dc_eta = 0.8) the second FP64 outer iteration of a
