@@ -65,27 +65,38 @@ FP32_VECTOR_PEAK_TFLOPS = 157.3    # MI355X published FP32 vector peak (packed)
 FLOPS_PER_PAIR = 23
 
 
-def pair_kernels(tiles: int):
+def equal_pow2_weights(w) -> bool:
+    """Every weight the same power of two in [2^-60, 2^60]: the symmetric sums
+    then run their equal-weight form (collop_set_weight_path)."""
+    m, e = np.frexp(w[0])
+    return bool(np.all(w == w[0]) and m == 0.5 and -59 <= e <= 61)
+
+
+def pair_kernels(tiles: int, pow2w: bool = False):
     """Pair kernels by profiling kind (collop_profile): name, peak TFLOP/s, flops
     per pair unit, unit.  tiles = 0: the ordered sum (flops per ordered pair: 23
     for the pair sum, functions.jl:221-231, 38 with the metric,
     functions.jl:276-281); tiles = R: the symmetric sum with R waves per block
-    (flops per unordered pair: 28, 50 with the metric)."""
+    (flops per unordered pair: 28, 50 with the metric); pow2w: its equal-weight
+    form (the FP64 pair sum, the FP32c pair sum and metric kernels)."""
     if not tiles:
         return [("k_pair_f64<1,1,false>", FP64_VECTOR_PEAK_TFLOPS, 23, "ordered"),
                 ("k_pair_f32c<1,32,false>", FP32_VECTOR_PEAK_TFLOPS, 23, "ordered"),
                 ("k_pair_f64<1,1,true> (pair sum + metric)", FP64_VECTOR_PEAK_TFLOPS, 38, "ordered"),
                 ("k_pair_f32c<1,32,true> (pair sum + metric)", FP32_VECTOR_PEAK_TFLOPS, 38, "ordered")]
-    R = tiles
-    return [(f"k_pair_sym_f64<{R},false,false>", FP64_VECTOR_PEAK_TFLOPS, 28, "unordered"),
-            (f"k_pair_sym_f32c<{R},false,false>", FP32_VECTOR_PEAK_TFLOPS, 28, "unordered"),
+    R, e = tiles, (",true" if pow2w else "")
+    return [(f"k_pair_sym_f64<{R},false,false{e}>", FP64_VECTOR_PEAK_TFLOPS, 28, "unordered"),
+            (f"k_pair_sym_f32c<{R},false,false{e}>", FP32_VECTOR_PEAK_TFLOPS, 28, "unordered"),
             (f"k_pair_sym_f64<{R},true,false> (pair sum + metric)", FP64_VECTOR_PEAK_TFLOPS, 50, "unordered"),
-            (f"k_pair_sym_f32c<{R},true,false> (pair sum + metric)", FP32_VECTOR_PEAK_TFLOPS, 50, "unordered")]
+            (f"k_pair_sym_f32c<{R},true,false{e}> (pair sum + metric)", FP32_VECTOR_PEAK_TFLOPS, 50, "unordered")]
 # HBM bytes per pair-sum launch at bimodal_n65536 from rocprofv3 FETCH_SIZE (x2.0)
 # and WRITE_SIZE (x1.0), both scaled by the counters of a 256 MiB copy; see
-# profiles/pair_sum_bimodal_n65536.md (the metric kernels from an earlier run of
-# the same counters).
-PAIR_TRAFFIC_BYTES = {"k_pair_f64<1,1,false>": (2 * 17939 + 32768) * 1024,
+# profiles/pair_sum_bimodal_n65536.md (the general-form metric kernels from an
+# earlier run of the same counters).
+PAIR_TRAFFIC_BYTES = {"k_pair_sym_f64<4,false,false,true>": (2 * 48711 + 131072) * 1024,
+                      "k_pair_sym_f32c<4,false,false,true>": (2 * 19347 + 131072) * 1024,
+                      "k_pair_sym_f32c<4,true,false,true> (pair sum + metric)": (2 * 33961 + 327680) * 1024,
+                      "k_pair_f64<1,1,false>": (2 * 17939 + 32768) * 1024,
                       "k_pair_f32c<1,32,false>": 50730 * 1024,
                       "k_pair_f64<1,1,true> (pair sum + metric)": 117802 * 1024,
                       "k_pair_f32c<1,32,true> (pair sum + metric)": 99890 * 1024,
@@ -628,9 +639,10 @@ def main():
     # N_in (N_in - 1) ordered pairs per launch for either form; the metric kernels
     # add 15 per ordered pair (functions.jl:276-281).
     pairs = n_in * (n_in - 1)
+    pow2w = equal_pow2_weights(w)
     kinds = []
     for path, res in [("chosen", rch), ("plain_fp64", r64)] + [(k, r) for k, (_, r) in comps.items()]:
-        for k, (kname, peak, fpp_eval, unit) in enumerate(pair_kernels(res["tiles"])):
+        for k, (kname, peak, fpp_eval, unit) in enumerate(pair_kernels(res["tiles"], pow2w)):
             if res["pair_launches"][k]:
                 avg = res["pair_ms"][k] / res["pair_launches"][k]
                 fpp = 38 if "metric" in kname else 23
@@ -640,7 +652,7 @@ def main():
                               "frac": round(fpp * pairs / (avg * 1e-3) / 1e12 / peak, 4),
                               "ordered_pairs_per_s": round(pairs / (avg * 1e-3), 1)})
     dom = int(np.argmax(rch["pair_ms"]))
-    kname, peak, fpp_eval, unit = pair_kernels(rch["tiles"])[dom]
+    kname, peak, fpp_eval, unit = pair_kernels(rch["tiles"], pow2w)[dom]
     avg_ms = rch["pair_ms"][dom] / max(rch["pair_launches"][dom], 1)
     fpp = 38 if "metric" in kname else 23
     flops = fpp * pairs
