@@ -423,8 +423,8 @@ struct SymLds32 {
 };
 
 // POW2W: the equal power-of-two weight form of sym_block_f64, in FP32 (2^k is
-// exact there too; far points at +-kFar32); with the metric it keeps one weight
-// product, w y^3 for A.
+// exact there too; far points at +-kFar32); the metric sums take the weight
+// once, at the end.
 constexpr float kFar32 = 1e18f;
 
 template <int R, bool METRIC, bool NEAR, bool POW2W = false>
@@ -585,8 +585,9 @@ __device__ __forceinline__ void sym_block_f32c(const Src32* __restrict__ src, in
             ab[0] = __builtin_fmaf(-ka, d2, ab[0]);
             ab[1] = __builtin_fmaf(ka, d1, ab[1]);
           } else {
-            // POW2W: sa = sg = w y^3, and kg = ka = y^3 (w c), the same bits as (w y^3) c
-            const float sg = aw * y3, sa = POW2W ? sg : gw[q] * y3;
+            // POW2W: kg = ka = y^3 (w c), the same bits as (w y^3) c; the metric
+            // sums run without the weight (sg = sa = y^3) and take it at the end
+            const float sg = POW2W ? y3 : aw * y3, sa = POW2W ? y3 : gw[q] * y3;
             const float kg = POW2W ? y3 * c : sg * c, ka = POW2W ? kg : sa * c;
             bs[q][0] = __builtin_fmaf(kg, d2, bs[q][0]);
             bs[q][1] = __builtin_fmaf(-kg, d1, bs[q][1]);
@@ -648,20 +649,31 @@ __device__ __forceinline__ void sym_block_f32c(const Src32* __restrict__ src, in
       tiles(std::true_type{});
     }
   }
+  // with equal weights the metric sums take their weight w = 2^k here (exact:
+  // every product and TwoSum of the loop scaled by w gives the general form's bits)
+  constexpr int CW = POW2W && METRIC ? 2 : C;
 #pragma unroll
   for (int q = 0; q < 2; ++q) {
     if (gm[q] >= N) continue;
     double* p = part + (size_t)b * cs + gm[q];
 #pragma unroll
-    for (int c = 0; c < C; ++c) __builtin_nontemporal_store(gin[q] ? (double)hs[q][c] + (double)ls[q][c] : 0.0, &p[(size_t)c * N]);
+    for (int c = 0; c < C; ++c) {
+      double x = (double)hs[q][c] + (double)ls[q][c];
+      if (c >= CW) x *= (double)gw[q];
+      __builtin_nontemporal_store(gin[q] ? x : 0.0, &p[(size_t)c * N]);
+    }
   }
   for (int e = threadIdx.x; e < E; e += 64 * R) {
     const int m = b * E + e;
     if (m >= N) continue;
-    const bool in = src[m].in_domain != 0.0f;
+    const Src32 s = src[m];
     double* p = part + (size_t)a * cs + m;
 #pragma unroll
-    for (int c = 0; c < C; ++c) __builtin_nontemporal_store(in ? (double)Lh[c][e] + (double)Ll[c][e] : 0.0, &p[(size_t)c * N]);
+    for (int c = 0; c < C; ++c) {
+      double x = (double)Lh[c][e] + (double)Ll[c][e];
+      if (c >= CW) x *= (double)s.w;
+      __builtin_nontemporal_store(s.in_domain != 0.0f ? x : 0.0, &p[(size_t)c * N]);
+    }
   }
 }
 
