@@ -93,9 +93,9 @@ def pair_kernels(tiles: int, pow2w: bool = False):
 # and WRITE_SIZE (x1.0), both scaled by the counters of a 256 MiB copy; see
 # profiles/pair_sum_bimodal_n65536.md (the general-form metric kernels from an
 # earlier run of the same counters).
-PAIR_TRAFFIC_BYTES = {"k_pair_sym_f64<4,false,false,true>": (2 * 18673 + 131072) * 1024,
-                      "k_pair_sym_f32c<4,false,false,true>": (2 * 15314 + 131072) * 1024,
-                      "k_pair_sym_f32c<4,true,false,true> (pair sum + metric)": (2 * 8286 + 327680) * 1024,
+PAIR_TRAFFIC_BYTES = {"k_pair_sym_f64<4,false,false,true>": (2 * 48711 + 131072) * 1024,
+                      "k_pair_sym_f32c<4,false,false,true>": (2 * 19347 + 131072) * 1024,
+                      "k_pair_sym_f32c<4,true,false,true> (pair sum + metric)": (2 * 33961 + 327680) * 1024,
                       "k_pair_f64<1,1,false>": (2 * 17939 + 32768) * 1024,
                       "k_pair_f32c<1,32,false>": 50730 * 1024,
                       "k_pair_f64<1,1,true> (pair sum + metric)": 117802 * 1024,

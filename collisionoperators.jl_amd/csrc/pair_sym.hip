@@ -195,12 +195,12 @@ __device__ __forceinline__ void sym_block_f64(const Src64* __restrict__ src, int
     for (int q = 0; q < Q; ++q) {
       if (gm[q] >= N) continue;
       double* p = part + (size_t)a * cs + gm[q];
-      __builtin_nontemporal_store(gin[q] ? f1[q] : 0.0, &p[0]);
-      __builtin_nontemporal_store(gin[q] ? f2[q] : 0.0, &p[N]);
+      p[0] = gin[q] ? f1[q] : 0.0;
+      p[N] = gin[q] ? f2[q] : 0.0;
       if constexpr (METRIC) {
-        __builtin_nontemporal_store(gin[q] ? m11[q] : 0.0, &p[2 * (size_t)N]);
-        __builtin_nontemporal_store(gin[q] ? m12[q] : 0.0, &p[3 * (size_t)N]);
-        __builtin_nontemporal_store(gin[q] ? m22[q] : 0.0, &p[4 * (size_t)N]);
+        p[2 * (size_t)N] = gin[q] ? m11[q] : 0.0;
+        p[3 * (size_t)N] = gin[q] ? m12[q] : 0.0;
+        p[4 * (size_t)N] = gin[q] ? m22[q] : 0.0;
       }
     }
     return;
@@ -376,20 +376,17 @@ __device__ __forceinline__ void sym_block_f64(const Src64* __restrict__ src, int
       tiles(std::true_type{});
     }
   }
-  // targets of super-row a -> slot b; sources of super-row b -> slot a.  The
-  // slots go out with non-temporal stores (here and in every block of this
-  // file): streamed through L2, they would evict the source records the other
-  // blocks re-read (at N = 65536 the launch's HBM reads fall from 100 to 38 MB).
+  // targets of super-row a -> slot b; sources of super-row b -> slot a
 #pragma unroll
   for (int q = 0; q < Q; ++q) {
     if (gm[q] >= N) continue;
     double* p = part + (size_t)b * cs + gm[q];
-    __builtin_nontemporal_store(gin[q] ? f1[q] : 0.0, &p[0]);
-    __builtin_nontemporal_store(gin[q] ? f2[q] : 0.0, &p[N]);
+    p[0] = gin[q] ? f1[q] : 0.0;
+    p[N] = gin[q] ? f2[q] : 0.0;
     if constexpr (METRIC) {
-      __builtin_nontemporal_store(gin[q] ? m11[q] : 0.0, &p[2 * (size_t)N]);
-      __builtin_nontemporal_store(gin[q] ? m12[q] : 0.0, &p[3 * (size_t)N]);
-      __builtin_nontemporal_store(gin[q] ? m22[q] : 0.0, &p[4 * (size_t)N]);
+      p[2 * (size_t)N] = gin[q] ? m11[q] : 0.0;
+      p[3 * (size_t)N] = gin[q] ? m12[q] : 0.0;
+      p[4 * (size_t)N] = gin[q] ? m22[q] : 0.0;
     }
   }
   for (int e = threadIdx.x; e < E; e += 64 * NW) {
@@ -398,7 +395,7 @@ __device__ __forceinline__ void sym_block_f64(const Src64* __restrict__ src, int
     const bool in = src[m].in_domain != 0.0;
     double* p = part + (size_t)a * cs + m;
 #pragma unroll
-    for (int c = 0; c < C; ++c) __builtin_nontemporal_store(in ? Lacc[c][e] : 0.0, &p[(size_t)c * N]);
+    for (int c = 0; c < C; ++c) p[(size_t)c * N] = in ? Lacc[c][e] : 0.0;
   }
 }
 
@@ -499,7 +496,7 @@ __device__ __forceinline__ void sym_block_f32c(const Src32* __restrict__ src, in
       if (gm[q] >= N) continue;
       double* p = part + (size_t)a * cs + gm[q];
 #pragma unroll
-      for (int c = 0; c < C; ++c) __builtin_nontemporal_store(gin[q] ? (double)hs[q][c] + (double)ls[q][c] : 0.0, &p[(size_t)c * N]);
+      for (int c = 0; c < C; ++c) p[(size_t)c * N] = gin[q] ? (double)hs[q][c] + (double)ls[q][c] : 0.0;
     }
     return;
   }
@@ -660,7 +657,7 @@ __device__ __forceinline__ void sym_block_f32c(const Src32* __restrict__ src, in
     for (int c = 0; c < C; ++c) {
       double x = (double)hs[q][c] + (double)ls[q][c];
       if (c >= CW) x *= (double)gw[q];
-      __builtin_nontemporal_store(gin[q] ? x : 0.0, &p[(size_t)c * N]);
+      p[(size_t)c * N] = gin[q] ? x : 0.0;
     }
   }
   for (int e = threadIdx.x; e < E; e += 64 * R) {
@@ -672,7 +669,7 @@ __device__ __forceinline__ void sym_block_f32c(const Src32* __restrict__ src, in
     for (int c = 0; c < C; ++c) {
       double x = (double)Lh[c][e] + (double)Ll[c][e];
       if (c >= CW) x *= (double)s.w;
-      __builtin_nontemporal_store(s.in_domain != 0.0f ? x : 0.0, &p[(size_t)c * N]);
+      p[(size_t)c * N] = s.in_domain != 0.0f ? x : 0.0;
     }
   }
 }
