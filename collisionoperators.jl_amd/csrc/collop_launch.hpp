@@ -130,6 +130,22 @@ double scatter_scale(double w_abs_sum);  // 2^62 / W (fixed-point scale of the s
 // An Anderson update v1 <- beta (Gv - sum_j dG_j g_j) + (1 - beta) v1
 // (solver.jl:383-412) that a scatter applies to its input markers as it reads
 // them (single run: components N apart, dG columns 2N apart).
+// The sum over S partial slots part[k * stride + i], k = 0..S-1, in that order
+// (the bits of the plain loop), with eight loads in flight instead of one.
+__device__ __forceinline__ double slot_sum(const double* __restrict__ part, int S, size_t stride, size_t i) {
+  double s = 0.0;
+  int k = 0;
+  for (; k + 8 <= S; k += 8) {
+    double t[8];
+#pragma unroll
+    for (int u = 0; u < 8; ++u) t[u] = part[(size_t)(k + u) * stride + i];
+#pragma unroll
+    for (int u = 0; u < 8; ++u) s += t[u];
+  }
+  for (; k < S; ++k) s += part[(size_t)k * stride + i];
+  return s;
+}
+
 struct UpdateArgs {
   const double* Gv;
   const double* dG;

@@ -304,9 +304,7 @@ __global__ __launch_bounds__(BT) void k_pair_b(EnsView E, const int4* __restrict
 __global__ void k_slice_sum(const double* __restrict__ part, int S, int64_t len, double* __restrict__ out) {
   const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= len) return;
-  double s = 0.0;
-  for (int k = 0; k < S; ++k) s += part[(size_t)k * len + i];
-  out[i] = s;
+  out[i] = slot_sum(part, S, (size_t)len, (size_t)i);
 }
 
 int grid_for(int N, const PairTile& t) {

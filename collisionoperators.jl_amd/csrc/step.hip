@@ -104,7 +104,7 @@ __device__ __forceinline__ void epilogue_body(const EpiArgs& A, int N, int ld, s
     if (s_given)
       s = *s_given;
     else
-      for (int k = 0; k < A.S; ++k) s += A.part[(size_t)k * n2 + i];
+      s = slot_sum(A.part, A.S, (size_t)n2, (size_t)i);
     double gv = A.v0[e] + A.dt * s;
     const double x1 = A.v1[e];
     if (A.cl_row) {  // a stiff cluster's row: the cluster solve's value replaces the map's
@@ -168,8 +168,7 @@ __device__ __forceinline__ void euler_body(const double* __restrict__ part, int 
   const int n2 = 2 * N;
   if (i >= n2) return;
   const size_t e = el(i, N, ld);
-  double s = 0.0;
-  for (int k = 0; k < S; ++k) s += part[(size_t)k * n2 + i];
+  const double s = slot_sum(part, S, (size_t)n2, (size_t)i);
   if (dotv) dotv[e] = s;
   out[e] = v[e] + dt * s;
 }
